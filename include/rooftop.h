@@ -1,0 +1,168 @@
+/*
+ * rooftop.h -- C ABI of librooftop, the MI355X (gfx950) iterative-solve path of the rooftop-basis method of moments.
+ *
+ * The entry points replace the reference MATLAB package rooftopMoM 3.5.0 on its hot path; each cites the reference
+ * interface it stands in for. A MATLAB MEX gateway, a Python ctypes binding or any other FFI binds these symbols.
+ *
+ * Conventions
+ *   - Host arrays are interleaved complex (double re, double im) unless a function says otherwise, column-major like
+ *     MATLAB (mxGetComplexDoubles). A rooftop vector holds the x-directed rooftops first, then the y-directed ones,
+ *     each on its own grid with the x index fastest (src/+rooftopMoM/+basisFunctions/UniformRooftops.m:1-9,44-55).
+ *   - "_device" entry points take device pointers (complex128) and run on the context's HIP stream; they return when
+ *     the work is enqueued unless stated.
+ *   - Every function returns RT_OK (0) or a negative status. rt_last_error_id()/rt_last_error_message() (thread-local)
+ *     give the reference's error identifier and a message for the last failure on the calling thread.
+ *   - Handles are opaque; the library owns every device buffer behind them. Calls on one handle must not overlap.
+ */
+#ifndef ROOFTOP_H
+#define ROOFTOP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes and the reference error identifiers they carry. */
+#define RT_OK 0
+#define RT_EINVAL (-1)     /* validator errors, e.g. rooftopMoM:solver:invalidRestart                   */
+#define RT_ESIZE (-2)      /* rooftopMoM:product:sizeMismatch, rooftopMoM:solver:sizeMismatch          */
+#define RT_ENOMEM (-3)     /* rooftopMoM:resources:insufficientMemory                                   */
+#define RT_ESINGULAR (-4)  /* rooftopMoM:preconditioner:singular                                        */
+#define RT_EHIP (-5)       /* a HIP runtime error; the message holds the HIP error string               */
+
+/* Product kinds (rooftopMoM.Operator 'product', src/+rooftopMoM/Operator.m:21-26). All three compute the same
+ * Toeplitz product; on the GPU they share one circulant embedding of period >= 2n-1 per split direction. */
+#define RT_PRODUCT_FFT 0
+#define RT_PRODUCT_SPLIT_FFT 1
+#define RT_PRODUCT_LEAN_SPLIT_FFT 2
+
+/* Arithmetic of a product or preconditioner: complex128, complex64, or both stored (mixed solves). */
+#define RT_C128 0
+#define RT_C64 1
+#define RT_MIXED 2
+
+/* Generator modifiers (rooftopMoM.Operator 'modifier', Generators.m:25-37). */
+#define RT_MODIFIER_NONE 0
+#define RT_MODIFIER_REAL 1
+#define RT_MODIFIER_IMAG 2
+
+/* Preconditioner kinds. The reference's preconditioners are circulant (Preconditioner.m:41-51); these two are the
+ * north-star ones: Jacobi = diag(Z) (tests/+unit/+solver/ConjugateGradientTest.m:55-59) and the near-field
+ * preconditioner on 3x3-cell neighbourhoods (symmetrized sparse approximate inverse). */
+#define RT_PRECOND_JACOBI 1
+#define RT_PRECOND_NEARFIELD 2
+
+/* Methods (rooftopMoM.Solver 'method', Solver.m:27-32). */
+#define RT_METHOD_CG 0
+#define RT_METHOD_GMRES 1
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_problem rt_problem;
+typedef struct rt_product rt_product;
+
+/* Solver options: rooftopMoM.Solver name-value inputs (Solver.m:389-410) plus the precision of the inner loop.
+ *   precision       RT_C128 or RT_C64: arithmetic of the products inside the Krylov loop.
+ *   refine          0: the reference's loop as is (the true residual is reported, not enforced);
+ *                   1: an FP64 outer loop on the true residual norm(b - A x)/norm(b) until it is below tolerance
+ *                      (iterative refinement; required with precision RT_C64).
+ *   inner_tolerance relative tolerance of each inner solve when refine = 1 (0 picks 1e-5 for RT_C64, the tolerance
+ *                   itself for RT_C128).
+ *   max_outer       limit of outer refinement steps (0 picks 20). */
+typedef struct rt_solve_opts {
+    int32_t method;
+    double tolerance;          /* default 1e-6 */
+    int32_t max_iterations;    /* default 1000 */
+    int32_t restart;           /* GMRES restart, <= 0 means none (Inf) */
+    int32_t precision;
+    int32_t refine;
+    double inner_tolerance;
+    int32_t max_outer;
+} rt_solve_opts;
+
+/* rooftopMoM.solver.Output (src/+rooftopMoM/+solver/Output.m:14-21) plus the refinement count. residual_norms, if
+ * not NULL, receives up to residual_norms_capacity monitored norms (iteration_count + 1 of them per RHS). */
+typedef struct rt_output {
+    int32_t converged;
+    int32_t iteration_count;
+    double residual_norm;      /* true relative residual norm(b - A x)/norm(b), FP64 */
+    double time_s;
+    int32_t outer_count;
+    int32_t residual_norms_count;
+    int32_t residual_norms_capacity;
+    double *residual_norms;
+} rt_output;
+
+/* Context: one device, one HIP stream, and the byte ceiling of the reference's Budget
+ * (src/+rooftopMoM/+resources/Budget.m:82-106; 0 = the device's free memory). */
+int rt_ctx_create(int32_t device, uint64_t hbm_budget_bytes, rt_ctx **out);
+void rt_ctx_destroy(rt_ctx *ctx);
+/* Resources.held()/ceiling() (src/+rooftopMoM/Resources.m:55-71), in device bytes. */
+int rt_ctx_memory(rt_ctx *ctx, uint64_t *held, uint64_t *ceiling);
+/* Blocks until the context's stream is idle. */
+int rt_ctx_synchronize(rt_ctx *ctx);
+/* Device address of the context's HIP stream (hipStream_t), for callers that time or order work on it. */
+int rt_ctx_stream(rt_ctx *ctx, void **stream);
+
+/* rooftopMoM.Problem(mesh, k, 'quadratureOrder', [near far]) (src/+rooftopMoM/Problem.m:25-92) on a uniform
+ * countX x countY grid of sizeX x sizeY faces (rooftopMoM.Mesh, Mesh.m:163-185). The generators are assembled on the
+ * GPU on the first call that needs them (Problem.generators, Problem.m:135-145). mask (countX*countY bytes,
+ * column-major, NULL = full plate) keeps the rooftops whose two faces are both metal. */
+int rt_problem_create(rt_ctx *ctx, int32_t count_x, int32_t count_y, double size_x, double size_y,
+                      double wavenumber, int32_t near_order, int32_t far_order, const uint8_t *mask,
+                      rt_problem **out);
+void rt_problem_destroy(rt_problem *problem);
+/* Problem.rooftopCount (Problem.m:123-125): the unknowns of the full grid (a mask only zeroes rooftops). */
+int rt_problem_count(rt_problem *problem, int64_t *count);
+/* Problem.generators (Problem.m:135-145) copied to the host: magneticXX (countX-1)xcountY, magneticYY
+ * countX x(countY-1), electric countX x countY, complex128 column-major; a NULL pointer skips that generator. */
+int rt_problem_generators(rt_problem *problem, void *magnetic_xx, void *magnetic_yy, void *electric);
+/* Replaces the generators by given ones (as Generators(...) is built from stored data in
+ * tests/+reference/+assembler/+greensMatrix/ApplicatorTest.m:9-14); a missing direction takes NULL. */
+int rt_problem_set_generators(rt_problem *problem, const void *magnetic_xx, const void *magnetic_yy,
+                              const void *electric);
+
+/* rooftopMoM.Operator(problem, 'product', kind, 'modifier', m) (src/+rooftopMoM/Operator.m:7-54). */
+int rt_product_create(rt_problem *problem, int32_t kind, int32_t precision, int32_t modifier, rt_product **out);
+/* A preconditioner product (the role of rooftopMoM.Preconditioner, src/+rooftopMoM/Preconditioner.m:7-70);
+ * RT_ESINGULAR mirrors rooftopMoM:preconditioner:singular. */
+int rt_precond_create(rt_problem *problem, int32_t kind, int32_t precision, rt_product **out);
+void rt_product_destroy(rt_product *product);
+/* Product.apply (src/+rooftopMoM/Product.m:25-35) on nrhs column-major host vectors of n rooftops each; n must be
+ * the rooftop count (RT_ESIZE, rooftopMoM:product:sizeMismatch). Synchronous. precision selects the stored
+ * arithmetic for an RT_MIXED product (RT_C128 or RT_C64); otherwise pass the product's own. */
+int rt_product_apply(rt_product *product, const void *input, void *output, int64_t n, int64_t nrhs,
+                     int32_t precision);
+/* The same on device buffers (complex128), enqueued on the context's stream. */
+int rt_product_apply_device(rt_product *product, const void *input, void *output, int64_t n, int64_t nrhs,
+                            int32_t precision);
+/* Product.memory / workingMemory (Product.m:47-57): device bytes of the stored spectra or coefficients, and of the
+ * buffers one application uses. */
+int rt_product_memory(rt_product *product, uint64_t *stored, uint64_t *working);
+
+/* rooftopMoM.Solver(...).solve(operator, rhs, 'preconditioner', P, 'initialGuess', x0) (Solver.m:42-92,
+ * IterativeMethod.m:27-70). precond may be NULL; x0 may be NULL (zeros). nrhs right-hand sides are solved one
+ * after another into x (n x nrhs). out (may be NULL) reports the last right-hand side, or the worst for nrhs > 1. */
+int rt_solve(rt_ctx *ctx, rt_product *op, rt_product *precond, const rt_solve_opts *opts, const void *rhs,
+             int64_t n, int64_t nrhs, const void *x0, void *x, rt_output *out);
+/* The same on device buffers; synchronous (the loop polls its scalars). */
+int rt_solve_device(rt_ctx *ctx, rt_product *op, rt_product *precond, const rt_solve_opts *opts, const void *rhs,
+                    int64_t n, int64_t nrhs, const void *x0, void *x, rt_output *out);
+
+/* Device buffers owned by the caller through the library (counted in the context's budget). */
+int rt_device_alloc(rt_ctx *ctx, uint64_t bytes, void **ptr);
+int rt_device_free(rt_ctx *ctx, void *ptr);
+int rt_copy_to_device(rt_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int rt_copy_to_host(rt_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* Thread-local identifier (e.g. "rooftopMoM:product:sizeMismatch") and message of the last error. */
+const char *rt_last_error_id(void);
+const char *rt_last_error_message(void);
+/* Library version string. */
+const char *rt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROOFTOP_H */

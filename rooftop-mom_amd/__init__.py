@@ -1,0 +1,486 @@
+"""rooftop_mom_amd: the +rooftopMoM object interface (Mesh, Problem, Operator, Preconditioner, Excitation, Solver,
+Result) over librooftop.so, the gfx950 HIP library of the iterative-solve path.
+
+This package directory is named `rooftop-mom_amd`; it is imported under the module name `rooftop_mom_amd` (see
+load() in this repository's __graft_entry__.py). Every numerical step of the path -- generators, spectra, products,
+preconditioners, Krylov loops -- runs in librooftop.so on the GPU; without the library every class raises.
+
+Names, argument meanings, defaults and error identifiers follow the reference MATLAB package rooftopMoM 3.5.0:
+  Mesh           src/+rooftopMoM/Mesh.m:163-214
+  Problem        src/+rooftopMoM/Problem.m:25-206
+  Operator       src/+rooftopMoM/Operator.m:7-54 and Product.m:25-57
+  Preconditioner src/+rooftopMoM/Preconditioner.m:7-70 (kinds here: 'nearField', 'jacobi')
+  Excitation     src/+rooftopMoM/Excitation.m:29-117
+  Solver         src/+rooftopMoM/Solver.m:17-169
+  Result         src/+rooftopMoM/Result.m:14-94
+Additions: 'precision' ('c128', 'c64', 'mixed') on products, and 'precision'/'refine' on the solver (complex64
+inner loop inside an FP64 refinement loop on the true residual).
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+from ._abi import RooftopError, check, library
+
+Z0 = 376.730313668          # src/+rooftopMoM/+assembler/Constants.m:11
+SPEED_OF_LIGHT = 299792458.0
+
+_PRECISIONS = {'c128': _abi.RT_C128, 'c64': _abi.RT_C64, 'mixed': _abi.RT_MIXED}
+_PRODUCTS = {'fft': _abi.RT_PRODUCT_FFT, 'splitFFT': _abi.RT_PRODUCT_SPLIT_FFT,
+             'leanSplitFFT': _abi.RT_PRODUCT_LEAN_SPLIT_FFT}
+_MODIFIERS = {None: _abi.RT_MODIFIER_NONE, 'real': _abi.RT_MODIFIER_REAL, 'imag': _abi.RT_MODIFIER_IMAG}
+_PRECONDITIONERS = {'jacobi': _abi.RT_PRECOND_JACOBI, 'nearField': _abi.RT_PRECOND_NEARFIELD}
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """One GPU, one HIP stream and the memory ceiling of the reference's Resources/Budget (Resources.m:9-47)."""
+
+    _default = None
+
+    def __init__(self, device=0, memory=0):
+        lib = library()
+        h = C.c_void_p()
+        check(lib.rt_ctx_create(int(device), int(memory), C.byref(h)))
+        self.handle = h
+        self._objects = []
+
+    @classmethod
+    def default(cls):
+        if cls._default is None:
+            cls._default = Context()
+        return cls._default
+
+    def memory(self):
+        """(held, ceiling) device bytes (Resources.held/ceiling, Resources.m:55-71)."""
+        held, ceiling = C.c_uint64(), C.c_uint64()
+        check(library().rt_ctx_memory(self.handle, C.byref(held), C.byref(ceiling)))
+        return held.value, ceiling.value
+
+    def synchronize(self):
+        check(library().rt_ctx_synchronize(self.handle))
+
+    def stream(self):
+        s = C.c_void_p()
+        check(library().rt_ctx_stream(self.handle, C.byref(s)))
+        return s.value
+
+
+class Mesh:
+    """Uniform grid of countX x countY faces of sizeX x sizeY m (Mesh.m:163-185)."""
+
+    def __init__(self, countX, countY, sizeX, sizeY):
+        if int(countX) != countX or int(countY) != countY or countX < 1 or countY < 1:
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBePositive', 'face counts are positive')
+        if not (sizeX > 0 and sizeY > 0 and math.isfinite(sizeX) and math.isfinite(sizeY)):
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBePositive', 'face sizes are positive')
+        if countX == 1 and countY == 1:
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:mesh:noRooftops',
+                               'A single face has no interior edge, so it carries no rooftop.')
+        self.countX, self.countY = int(countX), int(countY)
+        self.sizeX, self.sizeY = float(sizeX), float(sizeY)
+
+    def count(self):
+        return self.countX, self.countY
+
+    def faceSize(self):
+        return self.sizeX, self.sizeY
+
+    def radius(self):
+        return math.hypot(self.countX * self.sizeX, self.countY * self.sizeY) / 2
+
+
+class Problem:
+    """The EFIE of a Mesh at one wavenumber (Problem.m:25-92); generators assembled on the GPU on first use.
+
+    mask (countX x countY booleans, optional) keeps the rooftops whose two faces are metal.
+    """
+
+    def __init__(self, mesh, wavenumber=None, electricalSize=None, quadratureOrder=(16, 8), mask=None,
+                 context=None):
+        if (wavenumber is None) == (electricalSize is None):
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:problem:invalidWavenumber',
+                               "A problem takes either the wavenumber or the 'electricalSize', not both or neither.")
+        if wavenumber is None:
+            wavenumber = electricalSize / mesh.radius()
+        orders = np.atleast_1d(quadratureOrder)
+        if orders.size not in (1, 2):
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:problem:invalidQuadratureOrder',
+                               'The quadrature order is one order or a pair [near, far].')
+        self.mesh = mesh
+        self.context = context or Context.default()
+        self.orders = (int(orders[0]), int(orders[-1]))
+        self._wavenumber = float(wavenumber)
+        self.mask = None
+        mptr = None
+        if mask is not None:
+            self.mask = np.asfortranarray(np.asarray(mask, dtype=np.uint8).reshape(mesh.countX, mesh.countY))
+            mptr = C.c_void_p(self.mask.ctypes.data)
+        h = C.c_void_p()
+        check(library().rt_problem_create(self.context.handle, mesh.countX, mesh.countY, mesh.sizeX, mesh.sizeY,
+                                          self._wavenumber, self.orders[0], self.orders[1], mptr, C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, 'handle', None) is not None and _abi._lib is not None:
+            _abi._lib.rt_problem_destroy(self.handle)
+            self.handle = None
+
+    def wavenumber(self):
+        return self._wavenumber
+
+    def electricalSize(self):
+        return self._wavenumber * self.mesh.radius()
+
+    def rooftopCount(self):
+        n = C.c_int64()
+        check(library().rt_problem_count(self.handle, C.byref(n)))
+        return n.value
+
+    def countDir(self, direction):
+        cx, cy = self.mesh.count()
+        return (cx - 1) * cy if direction == 'x' else cx * (cy - 1)
+
+    def generators(self):
+        """{'magneticXX', 'magneticYY', 'electric'} as complex128 arrays (None for a missing direction)."""
+        cx, cy = self.mesh.count()
+        out = {'magneticXX': np.zeros((cx - 1, cy), np.complex128, order='F') if cx > 1 else None,
+               'magneticYY': np.zeros((cx, cy - 1), np.complex128, order='F') if cy > 1 else None,
+               'electric': np.zeros((cx, cy), np.complex128, order='F')}
+        check(library().rt_problem_generators(self.handle, _ptr(out['magneticXX']), _ptr(out['magneticYY']),
+                                              _ptr(out['electric'])))
+        return out
+
+    def setGenerators(self, magneticXX, magneticYY, electric):
+        """Uses given generators (as the reference tests build Generators from stored data)."""
+        arrs = [None if g is None else np.asfortranarray(g, dtype=np.complex128)
+                for g in (magneticXX, magneticYY, electric)]
+        check(library().rt_problem_set_generators(self.handle, *[_ptr(a) for a in arrs]))
+
+
+class Product:
+    """Base of the products (Product.m): apply(x) for N x 1 (or N x R) vectors, memory(), workingMemory()."""
+
+    handle = None
+
+    def __init__(self, problem, handle, precision):
+        self._problem = problem
+        self.handle = handle
+        self.precision = precision
+
+    def __del__(self):
+        if getattr(self, 'handle', None) is not None and _abi._lib is not None:
+            _abi._lib.rt_product_destroy(self.handle)
+            self.handle = None
+
+    def problem(self):
+        return self._problem
+
+    def apply(self, x, precision=None):
+        """Product.apply (Product.m:25-35); raises rooftopMoM:product:sizeMismatch on a wrong size."""
+        n = self._problem.rooftopCount()
+        a = np.asarray(x)
+        if not np.issubdtype(a.dtype, np.number) or a.ndim not in (1, 2) or a.shape[0] != n:
+            raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:product:sizeMismatch',
+                               'The input must be a numeric %dx1 vector.' % n)
+        a = np.asfortranarray(a, dtype=np.complex128)
+        nrhs = 1 if a.ndim == 1 else a.shape[1]
+        out = np.empty_like(a)
+        prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
+                                                         else self.precision)
+        check(library().rt_product_apply(self.handle, _ptr(a), _ptr(out), n, nrhs, prec))
+        return out
+
+    def apply_device(self, x_ptr, y_ptr, nrhs=1, precision=None):
+        """Enqueues the product on device buffers (complex128) on the context's stream."""
+        prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
+                                                         else self.precision)
+        check(library().rt_product_apply_device(self.handle, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                                self._problem.rooftopCount(), nrhs, prec))
+
+    def memory(self):
+        s = C.c_uint64()
+        check(library().rt_product_memory(self.handle, C.byref(s), None))
+        return s.value
+
+    def workingMemory(self):
+        w = C.c_uint64()
+        check(library().rt_product_memory(self.handle, None, C.byref(w)))
+        return w.value
+
+
+class Operator(Product):
+    """rooftopMoM.Operator(problem, 'product', ..., 'modifier', ...) (Operator.m:7-54)."""
+
+    def __init__(self, problem, product='splitFFT', modifier=None, precision='c128'):
+        if product not in _PRODUCTS:
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember',
+                               "'product' must be 'fft', 'splitFFT' or 'leanSplitFFT'.")
+        h = C.c_void_p()
+        check(library().rt_product_create(problem.handle, _PRODUCTS[product], _PRECISIONS[precision],
+                                          _MODIFIERS[modifier], C.byref(h)))
+        super().__init__(problem, h, _PRECISIONS[precision])
+
+
+class Preconditioner(Product):
+    """A preconditioner product: 'nearField' (3x3-cell neighbourhoods) or 'jacobi' (diag(Z)).
+
+    Errors rooftopMoM:preconditioner:singular like Preconditioner.m:60-67.
+    """
+
+    def __init__(self, problem, kind='nearField', precision='c128'):
+        if kind not in _PRECONDITIONERS:
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember',
+                               "'kind' must be 'nearField' or 'jacobi'.")
+        h = C.c_void_p()
+        check(library().rt_precond_create(problem.handle, _PRECONDITIONERS[kind], _PRECISIONS[precision],
+                                          C.byref(h)))
+        super().__init__(problem, h, _PRECISIONS[precision])
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# Excitations (O(N) host formulas of the boundary inputs; the solve itself runs on the GPU).
+# ---------------------------------------------------------------------------------------------------------------
+def _gauss_legendre01(order):
+    x, w = np.polynomial.legendre.leggauss(order)
+    return (x + 1) / 2, w / 2
+
+
+class Excitation:
+    """Right-hand side of a Problem (Excitation.m): planeWave or deltaGap."""
+
+    def __init__(self, problem, kind, vector, parameters):
+        self._problem, self._type, self._vector, self._parameters = problem, kind, vector, parameters
+
+    def problem(self):
+        return self._problem
+
+    def type(self):
+        return self._type
+
+    def vector(self):
+        return self._vector
+
+    def parameters(self):
+        return self._parameters
+
+    @staticmethod
+    def planeWave(problem, direction=(0, 0, -1), polarization=(1, 0, 0), intensity=1.0):
+        """E = intensity * polarization * exp(-1i k direction . r) tested with every rooftop, computed in O(N) by
+        translation phases (generatePlaneWave.m:18-42) with the problem's far quadrature order."""
+        d = np.asarray(direction, float)
+        p = np.asarray(polarization, float)
+        for v, name in ((d, 'direction'), (p, 'polarization')):
+            if not np.any(v):
+                raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:excitation:zeroVector', 'The %s must not be zero.' % name)
+        d, p = d / np.linalg.norm(d), p / np.linalg.norm(p)
+        if abs(d @ p) > 10 * np.finfo(float).eps:
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:excitation:notPerpendicular',
+                               'The polarization must be perpendicular to the direction of propagation.')
+        vector = plane_wave_vector(problem, d, p, float(intensity))
+        return Excitation(problem, 'planeWave', vector,
+                          {'direction': d, 'polarization': p, 'intensity': float(intensity)})
+
+    @staticmethod
+    def deltaGap(problem, port=None, direction='x', voltage=1.0):
+        """voltage x edge length on the port rooftops (generateDeltaGap.m:13-14); the default port is the middle
+        line of edges (generateMiddlePort.m:14-28). port holds 0-based rooftop indices."""
+        cx, cy = problem.mesh.count()
+        n = problem.rooftopCount()
+        if port is None:
+            if problem.countDir(direction) == 0:
+                raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:excitation:noRooftops',
+                                   'The mesh has no %s-directed rooftops.' % direction)
+            port = middle_port(problem, direction)
+        port = np.asarray(port, dtype=np.int64)
+        if np.any(port >= n) or np.any(port < 0) or np.unique(port).size < port.size:
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:excitation:invalidPort',
+                               'The port must hold distinct rooftop indices up to %d.' % n)
+        vector = np.zeros(n, np.complex128)
+        vector[port] = voltage * edge_lengths(problem, port)
+        return Excitation(problem, 'deltaGap', vector, {'port': port, 'voltage': voltage})
+
+
+def edge_lengths(problem, idx):
+    idx = np.asarray(idx)
+    out = np.full(idx.shape, problem.mesh.sizeX)
+    out[idx < problem.countDir('x')] = problem.mesh.sizeY
+    return out
+
+
+def middle_port(problem, direction='x'):
+    cx, cy = problem.mesh.count()
+    if direction == 'x':
+        rows = cx - 1
+        return (cx // 2 - 1) + rows * np.arange(cy)
+    rows = cx
+    return problem.countDir('x') + np.arange(rows) + rows * (cy // 2 - 1)
+
+
+def plane_wave_vector(problem, d, p, intensity):
+    a, b = problem.mesh.faceSize()
+    cx, cy = problem.mesh.count()
+    k = problem.wavenumber()
+    t, w = _gauss_legendre01(problem.orders[1])
+    pu, pv = np.meshgrid(t, t, indexing='ij')   # u fastest after the column-major ravel
+    wu, wv = np.meshgrid(w, w, indexing='ij')
+    pu, pv, w2 = pu.ravel(order='F'), pv.ravel(order='F'), (wu * wv).ravel(order='F')
+    steps = -1j * k * d[:2] * np.array([a, b])
+    phases = np.exp(steps[0] * pu + steps[1] * pv)
+    parts = []
+    for n, (rows, cols, coord, second) in enumerate(((cx - 1, cy, pu, (1, 0)), (cx, cy - 1, pv, (0, 1)))):
+        if rows * cols == 0:
+            continue
+        amp = coord + np.exp(steps[0] * second[0] + steps[1] * second[1]) * (1 - coord)
+        ref = intensity * p[n] * a * b * (phases @ (w2 * amp))
+        trans = np.exp(steps[0] * np.arange(rows))[:, None] * np.exp(steps[1] * np.arange(cols))[None, :]
+        parts.append(ref * trans.ravel(order='F'))
+    vec = np.concatenate(parts)
+    if problem.mask is not None:
+        vec = vec * rooftop_existence(problem)
+    return vec
+
+
+def rooftop_existence(problem):
+    """Per rooftop: 1 if both its faces are metal (all ones without a mask)."""
+    cx, cy = problem.mesh.count()
+    if problem.mask is None:
+        return np.ones(problem.rooftopCount())
+    m = problem.mask.astype(bool)
+    ex = (m[:-1, :] & m[1:, :]).ravel(order='F') if cx > 1 else np.zeros(0, bool)
+    ey = (m[:, :-1] & m[:, 1:]).ravel(order='F') if cy > 1 else np.zeros(0, bool)
+    return np.concatenate([ex, ey]).astype(float)
+
+
+class Result:
+    """Outcome of Solver.solve (Result.m:14-94)."""
+
+    def __init__(self, solution, converged, iterationCount, residualNorms, residualNorm, time, excitation,
+                 outerCount=0):
+        self._solution, self._converged, self._iterations = solution, converged, iterationCount
+        self._norms, self._residual, self._time = residualNorms, residualNorm, time
+        self._excitation, self.outerCount = excitation, outerCount
+
+    def solution(self):
+        return self._solution
+
+    def converged(self):
+        return self._converged
+
+    def iterationCount(self):
+        return self._iterations
+
+    def residualNorms(self):
+        return self._norms
+
+    def residualNorm(self):
+        return self._residual
+
+    def time(self):
+        return self._time
+
+    def excitation(self):
+        return self._excitation
+
+    def _params(self, kind):
+        if self._excitation is None or self._excitation.type() != kind:
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:result:wrongExcitation',
+                               'This quantity needs a %s excitation.' % kind)
+        return self._excitation.parameters()
+
+    def inputImpedance(self):
+        """V / (edge lengths . I(port)) (computeInputImpedance.m:17)."""
+        p = self._params('deltaGap')
+        prob = self._excitation.problem()
+        return p['voltage'] / (edge_lengths(prob, p['port']) @ self._solution[p['port']])
+
+    def radarCrossSection(self):
+        """(k Z0)^2 |b.' I|^2 / (4 pi E^4) (computeRadarCrossSection.m:18-19)."""
+        p = self._params('planeWave')
+        k = self._excitation.problem().wavenumber()
+        return (k * Z0) ** 2 * abs(self._excitation.vector() @ self._solution) ** 2 / (4 * np.pi * p['intensity'] ** 4)
+
+    def normalizedRadarCrossSection(self):
+        return self.radarCrossSection() / (2 * np.pi / self._excitation.problem().wavenumber()) ** 2
+
+
+class Solver:
+    """rooftopMoM.Solver('method', ..., 'tolerance', ..., 'maxIterationCount', ..., 'restart', ...) (Solver.m:389-410).
+
+    precision 'c64' runs the products of the Krylov loop in complex64 inside an FP64 refinement loop on the true
+    residual (refine is then implied); refine=True alone wraps the FP64 loop the same way.
+    """
+
+    def __init__(self, method='cg', tolerance=1e-6, maxIterationCount=1000, restart=50, precision='c128',
+                 refine=False, innerTolerance=0.0, maxOuter=20):
+        if method not in ('cg', 'gmres'):
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember', "'method' must be 'cg' or 'gmres'.")
+        if not (0 < tolerance < 1):
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeLessThan', "'tolerance' must be below 1.")
+        if not (restart == math.inf or (restart > 0 and restart == int(restart))):
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidRestart',
+                               'The restart is a positive integer or Inf.')
+        if precision not in ('c128', 'c64'):
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember', "'precision' is 'c128' or 'c64'.")
+        self.method, self.tolerance, self.maxIterationCount = method, float(tolerance), int(maxIterationCount)
+        self.restart = restart
+        self.precision = precision
+        self.refine = bool(refine) or precision == 'c64'
+        self.innerTolerance, self.maxOuter = float(innerTolerance), int(maxOuter)
+
+    def options(self):
+        o = _abi.rt_solve_opts()
+        o.method = _abi.RT_METHOD_CG if self.method == 'cg' else _abi.RT_METHOD_GMRES
+        o.tolerance = self.tolerance
+        o.max_iterations = self.maxIterationCount
+        o.restart = 0 if self.restart == math.inf else int(self.restart)
+        o.precision = _PRECISIONS[self.precision]
+        o.refine = int(self.refine)
+        o.inner_tolerance = self.innerTolerance
+        o.max_outer = self.maxOuter
+        return o
+
+    def solve(self, operator, rightHandSide, preconditioner=None, initialGuess=None):
+        """Solves operator * x = rightHandSide on the GPU (Solver.m:42-92)."""
+        if not isinstance(operator, Product):
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidOperator',
+                               'An operator is a rooftop_mom_amd product on the GPU.')
+        excitation = rightHandSide if isinstance(rightHandSide, Excitation) else None
+        b = excitation.vector() if excitation else rightHandSide
+        b = np.asarray(b)
+        if not np.issubdtype(b.dtype, np.number) or not np.all(np.isfinite(b)):
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidRightHandSide',
+                               'The right-hand side is a rooftopMoM.Excitation or a numeric vector of finite values.')
+        b = np.asfortranarray(b, dtype=np.complex128)
+        n = operator.problem().rooftopCount()
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        if b.shape[0] != n:
+            raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
+                               'The operator does not match the %d unknowns of the right-hand side.' % b.shape[0])
+        if preconditioner is not None and preconditioner.problem().rooftopCount() != n:
+            raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
+                               'The preconditioner does not match the %d unknowns of the right-hand side.' % n)
+        x0 = None
+        if initialGuess is not None:
+            x0 = np.asfortranarray(initialGuess, dtype=np.complex128)
+            if x0.size != b.size:
+                raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
+                                   'The initial guess must have %d elements.' % n)
+        x = np.empty_like(b)
+        cap = self.maxIterationCount * 2 + 64
+        norms = np.zeros(cap)
+        out = _abi.rt_output()
+        out.residual_norms_capacity = cap
+        out.residual_norms = norms.ctypes.data_as(C.POINTER(C.c_double))
+        o = self.options()
+        check(library().rt_solve(operator.problem().context.handle, operator.handle,
+                                 preconditioner.handle if preconditioner is not None else None, C.byref(o),
+                                 _ptr(b), n, nrhs, _ptr(x0), _ptr(x), C.byref(out)))
+        return Result(x, bool(out.converged), out.iteration_count, norms[:min(cap, out.residual_norms_count)].copy(),
+                      out.residual_norm, out.time_s, excitation, out.outer_count)

@@ -1,0 +1,438 @@
+// The C ABI of librooftop (include/rooftop.h): argument checks with the reference's error identifiers, handle
+// ownership, host<->device staging. Every entry point catches the library's errors and returns their status.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "krylov.hpp"
+#include "rooftop.hpp"
+
+namespace rt {
+const char *last_error_id();
+const char *last_error_message();
+
+GridInfo GridInfo::make(int cx, int cy, double a, double b) {
+    GridInfo g;
+    g.cx = cx;
+    g.cy = cy;
+    g.a = a;
+    g.b = b;
+    g.rows_x = cx - 1;
+    g.cols_x = cy;
+    g.rows_y = cx;
+    g.cols_y = cy - 1;
+    g.nx = static_cast<int64_t>(g.rows_x) * g.cols_x;
+    g.ny = static_cast<int64_t>(g.rows_y) * g.cols_y;
+    g.n = g.nx + g.ny;
+    g.has_x = g.nx > 0;
+    g.has_y = g.ny > 0;
+    return g;
+}
+}  // namespace rt
+
+using namespace rt;
+
+struct rt_ctx {
+    Context c;
+};
+struct rt_problem {
+    std::unique_ptr<Problem> p;
+};
+struct rt_product {
+    std::unique_ptr<Product> p;
+};
+
+namespace {
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        set_last_error(RT_OK, "", "");
+        return RT_OK;
+    } catch (const Error &e) {
+        set_last_error(e.status, e.id, e.what());
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        set_last_error(RT_ENOMEM, "rooftopMoM:resources:insufficientMemory", "host allocation failed");
+        return RT_ENOMEM;
+    } catch (const std::exception &e) {
+        set_last_error(RT_EINVAL, "rooftop:internal", e.what());
+        return RT_EINVAL;
+    }
+}
+
+void need(bool ok, int status, const char *id, const char *msg) {
+    if (!ok) fail(status, id, "%s", msg);
+}
+
+SolveOptions options_of(const rt_solve_opts *o) {
+    SolveOptions s;
+    if (!o) return s;
+    s.method = o->method;
+    s.tolerance = o->tolerance;
+    s.max_iterations = o->max_iterations;
+    s.restart = o->restart;
+    s.precision = o->precision;
+    s.refine = o->refine;
+    s.inner_tolerance = o->inner_tolerance;
+    s.max_outer = o->max_outer > 0 ? o->max_outer : 20;
+    if (s.method != RT_METHOD_CG && s.method != RT_METHOD_GMRES)
+        fail(RT_EINVAL, "MATLAB:validators:mustBeMember", "The method is 'cg' or 'gmres'.");
+    if (!(s.tolerance > 0 && s.tolerance < 1))
+        fail(RT_EINVAL, "MATLAB:validators:mustBeLessThan", "The tolerance is a real number in (0, 1).");
+    if (s.max_iterations <= 0)
+        fail(RT_EINVAL, "MATLAB:validators:mustBePositive", "The maximum iteration count is a positive integer.");
+    if (s.precision != RT_C128 && s.precision != RT_C64)
+        fail(RT_EINVAL, "rooftop:solver:invalidPrecision", "The inner precision is RT_C128 or RT_C64.");
+    if (s.precision == RT_C64 && !s.refine)
+        fail(RT_EINVAL, "rooftop:solver:invalidPrecision",
+             "A complex64 inner loop needs the FP64 refinement loop (refine = 1).");
+    return s;
+}
+
+void fill_output(const SolveResult &r, rt_output *out) {
+    if (!out) return;
+    out->converged = r.converged;
+    out->iteration_count = r.iterations;
+    out->residual_norm = r.residual;
+    out->time_s = r.time;
+    out->outer_count = r.outer;
+    out->residual_norms_count = static_cast<int32_t>(r.norms.size());
+    if (out->residual_norms && out->residual_norms_capacity > 0) {
+        const size_t k = std::min<size_t>(r.norms.size(), static_cast<size_t>(out->residual_norms_capacity));
+        std::memcpy(out->residual_norms, r.norms.data(), k * sizeof(double));
+    }
+}
+
+// Solves nrhs systems one after the other on device buffers.
+SolveResult solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_opts *opts, const c128 *rhs,
+                         int64_t n, int64_t nrhs, const c128 *x0, c128 *x) {
+    need(ctx && op && op->p, RT_EINVAL, "rooftopMoM:solver:invalidOperator",
+         "An operator is a product created by rt_product_create.");
+    need(op->p->type == kOperator || op->p->type == kJacobi || op->p->type == kNearField, RT_EINVAL,
+         "rooftopMoM:solver:invalidOperator", "An operator is a product.");
+    need(n == op->p->dims.n, RT_ESIZE, "rooftopMoM:solver:sizeMismatch",
+         "The operator does not match the unknowns of the right-hand side.");
+    need(!pc || (pc->p && pc->p->dims.n == n), RT_ESIZE, "rooftopMoM:solver:sizeMismatch",
+         "The preconditioner does not match the unknowns of the right-hand side.");
+    need(nrhs >= 1, RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide", "At least one right-hand side is needed.");
+    SolveOptions o = options_of(opts);
+    need(op->p->has(o.precision), RT_EINVAL, "rooftop:solver:invalidPrecision",
+         "The operator does not hold the inner precision (create it with RT_MIXED).");
+    need(!pc || pc->p->has(o.precision), RT_EINVAL, "rooftop:solver:invalidPrecision",
+         "The preconditioner does not hold the inner precision (create it with RT_MIXED).");
+    ctx->c.activate();
+    SolveResult last;
+    for (int64_t r = 0; r < nrhs; ++r) {
+        SolveResult res = solve_one(&ctx->c, *op->p, pc ? pc->p.get() : nullptr, o, rhs + r * n,
+                                    x0 ? x0 + r * n : nullptr, x + r * n);
+        if (r == 0 || res.residual > last.residual || !res.converged) {
+            double t = last.time + res.time;
+            last = res;
+            last.time = t;
+        } else {
+            last.time += res.time;
+        }
+    }
+    return last;
+}
+
+void check_finite(const c128 *v, int64_t count) {
+    for (int64_t i = 0; i < count; ++i)
+        if (!std::isfinite(v[i].x) || !std::isfinite(v[i].y))
+            fail(RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide",
+                 "The right-hand side is a numeric vector of finite values.");
+}
+
+}  // namespace
+
+extern "C" {
+
+int rt_ctx_create(int32_t device, uint64_t budget, rt_ctx **out) {
+    return guarded([&] {
+        need(out != nullptr, RT_EINVAL, "rooftop:ctx", "out is NULL");
+        int count = 0;
+        RT_HIP(hipGetDeviceCount(&count));
+        need(device >= 0 && device < count, RT_EINVAL, "rooftop:ctx", "no such device");
+        auto c = std::make_unique<rt_ctx>();
+        c->c.device = device;
+        c->c.activate();
+        size_t free_b = 0, total_b = 0;
+        RT_HIP(hipMemGetInfo(&free_b, &total_b));
+        c->c.ceiling = budget ? budget : static_cast<uint64_t>(free_b);
+        RT_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+void rt_ctx_destroy(rt_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    if (ctx->c.stream) {
+        (void)hipStreamSynchronize(ctx->c.stream);
+        (void)hipStreamDestroy(ctx->c.stream);
+    }
+    for (auto &kv : ctx->c.sizes) (void)hipFree(kv.first);
+    delete ctx;
+}
+
+int rt_ctx_memory(rt_ctx *ctx, uint64_t *held, uint64_t *ceiling) {
+    return guarded([&] {
+        need(ctx, RT_EINVAL, "rooftop:ctx", "ctx is NULL");
+        if (held) *held = ctx->c.held;
+        if (ceiling) *ceiling = ctx->c.ceiling;
+    });
+}
+
+int rt_ctx_synchronize(rt_ctx *ctx) {
+    return guarded([&] {
+        need(ctx, RT_EINVAL, "rooftop:ctx", "ctx is NULL");
+        ctx->c.activate();
+        RT_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int rt_ctx_stream(rt_ctx *ctx, void **stream) {
+    return guarded([&] {
+        need(ctx && stream, RT_EINVAL, "rooftop:ctx", "ctx or stream is NULL");
+        *stream = reinterpret_cast<void *>(ctx->c.stream);
+    });
+}
+
+int rt_problem_create(rt_ctx *ctx, int32_t cx, int32_t cy, double a, double b, double k, int32_t near_order,
+                      int32_t far_order, const uint8_t *mask, rt_problem **out) {
+    return guarded([&] {
+        need(ctx && out, RT_EINVAL, "rooftop:problem", "ctx or out is NULL");
+        need(cx >= 1 && cy >= 1, RT_EINVAL, "MATLAB:validators:mustBePositive", "The face counts are positive.");
+        need(!(cx == 1 && cy == 1), RT_EINVAL, "rooftopMoM:mesh:noRooftops",
+             "A single face has no interior edge, so it carries no rooftop.");
+        need(a > 0 && b > 0 && std::isfinite(a) && std::isfinite(b), RT_EINVAL, "MATLAB:validators:mustBePositive",
+             "The face sizes are positive and finite.");
+        need(k > 0 && std::isfinite(k), RT_EINVAL, "rooftopMoM:problem:invalidWavenumber",
+             "The wavenumber is positive and finite.");
+        need(near_order >= 1 && far_order >= 1 && far_order <= 16 && near_order <= 64, RT_EINVAL,
+             "rooftopMoM:problem:invalidQuadratureOrder", "The quadrature orders are 1..64 (near) and 1..16 (far).");
+        ctx->c.activate();
+        auto p = std::make_unique<Problem>();
+        p->ctx = &ctx->c;
+        p->g = GridInfo::make(cx, cy, a, b);
+        p->k = k;
+        p->near_order = near_order;
+        p->far_order = far_order;
+        // Budget of the generators (estimateGenerators.m:11) checked up front, as Problem.m:210-219 does.
+        ctx->c.require(16ull * (p->g.n + static_cast<uint64_t>(cx) * cy), "The assembly of the generators");
+        if (mask) {
+            p->face_mask.assign(mask, mask + static_cast<size_t>(cx) * cy);
+            std::vector<uint8_t> ex(static_cast<size_t>(p->g.n));
+            auto face = [&](int x, int y) { return mask[x + static_cast<size_t>(cx) * y] != 0; };
+            for (int j = 0; j < p->g.cols_x; ++j)
+                for (int i = 0; i < p->g.rows_x; ++i)
+                    ex[i + static_cast<size_t>(p->g.rows_x) * j] = face(i, j) && face(i + 1, j);
+            for (int j = 0; j < p->g.cols_y; ++j)
+                for (int i = 0; i < p->g.rows_y; ++i)
+                    ex[p->g.nx + i + static_cast<size_t>(p->g.rows_y) * j] = face(i, j) && face(i, j + 1);
+            p->exists.allocate(&ctx->c, ex.size(), "The rooftop mask");
+            RT_HIP(hipMemcpy(p->exists.ptr, ex.data(), ex.size(), hipMemcpyHostToDevice));
+        }
+        auto h = std::make_unique<rt_problem>();
+        h->p = std::move(p);
+        *out = h.release();
+    });
+}
+
+void rt_problem_destroy(rt_problem *p) {
+    if (!p) return;
+    if (p->p && p->p->ctx) (void)hipStreamSynchronize(p->p->ctx->stream);
+    delete p;
+}
+
+int rt_problem_count(rt_problem *p, int64_t *count) {
+    return guarded([&] {
+        need(p && count, RT_EINVAL, "rooftop:problem", "NULL argument");
+        *count = p->p->g.n;
+    });
+}
+
+int rt_problem_generators(rt_problem *p, void *mxx, void *myy, void *e) {
+    return guarded([&] {
+        need(p != nullptr, RT_EINVAL, "rooftop:problem", "NULL problem");
+        Problem &q = *p->p;
+        q.ctx->activate();
+        q.ensure_generators();
+        if (mxx && q.g.has_x) RT_HIP(hipMemcpy(mxx, q.mxx.ptr, q.mxx.bytes(), hipMemcpyDeviceToHost));
+        if (myy && q.g.has_y) RT_HIP(hipMemcpy(myy, q.myy.ptr, q.myy.bytes(), hipMemcpyDeviceToHost));
+        if (e) RT_HIP(hipMemcpy(e, q.e.ptr, q.e.bytes(), hipMemcpyDeviceToHost));
+    });
+}
+
+int rt_problem_set_generators(rt_problem *p, const void *mxx, const void *myy, const void *e) {
+    return guarded([&] {
+        need(p != nullptr && e != nullptr, RT_EINVAL, "rooftop:problem", "NULL problem or electric generator");
+        Problem &q = *p->p;
+        q.ctx->activate();
+        need(!q.g.has_x || mxx, RT_EINVAL, "rooftop:problem", "the grid has x-directed rooftops: magneticXX needed");
+        need(!q.g.has_y || myy, RT_EINVAL, "rooftop:problem", "the grid has y-directed rooftops: magneticYY needed");
+        const int cx = q.g.cx, cy = q.g.cy;
+        if (q.g.has_x) {
+            q.mxx.allocate(q.ctx, static_cast<size_t>(cx - 1) * cy, "The generators");
+            RT_HIP(hipMemcpy(q.mxx.ptr, mxx, q.mxx.bytes(), hipMemcpyHostToDevice));
+        }
+        if (q.g.has_y) {
+            q.myy.allocate(q.ctx, static_cast<size_t>(cx) * (cy - 1), "The generators");
+            RT_HIP(hipMemcpy(q.myy.ptr, myy, q.myy.bytes(), hipMemcpyHostToDevice));
+        }
+        q.e.allocate(q.ctx, static_cast<size_t>(cx) * cy, "The generators");
+        RT_HIP(hipMemcpy(q.e.ptr, e, q.e.bytes(), hipMemcpyHostToDevice));
+        q.assembled = true;
+    });
+}
+
+int rt_product_create(rt_problem *p, int32_t kind, int32_t precision, int32_t modifier, rt_product **out) {
+    return guarded([&] {
+        need(p && out, RT_EINVAL, "rooftop:product", "NULL argument");
+        need(kind == RT_PRODUCT_FFT || kind == RT_PRODUCT_SPLIT_FFT || kind == RT_PRODUCT_LEAN_SPLIT_FFT, RT_EINVAL,
+             "MATLAB:validators:mustBeMember", "The product is 'fft', 'splitFFT' or 'leanSplitFFT'.");
+        need(precision == RT_C128 || precision == RT_C64 || precision == RT_MIXED, RT_EINVAL,
+             "rooftop:product:invalidPrecision", "The precision is RT_C128, RT_C64 or RT_MIXED.");
+        need(modifier >= RT_MODIFIER_NONE && modifier <= RT_MODIFIER_IMAG, RT_EINVAL, "rooftop:product",
+             "The modifier is none, real or imag.");
+        p->p->ctx->activate();
+        auto h = std::make_unique<rt_product>();
+        h->p = create_operator(p->p.get(), precision, modifier);
+        *out = h.release();
+    });
+}
+
+int rt_precond_create(rt_problem *p, int32_t kind, int32_t precision, rt_product **out) {
+    return guarded([&] {
+        need(p && out, RT_EINVAL, "rooftop:preconditioner", "NULL argument");
+        need(kind == RT_PRECOND_JACOBI || kind == RT_PRECOND_NEARFIELD, RT_EINVAL, "MATLAB:validators:mustBeMember",
+             "The preconditioner is 'jacobi' or 'nearField'.");
+        need(precision == RT_C128 || precision == RT_C64 || precision == RT_MIXED, RT_EINVAL,
+             "rooftop:product:invalidPrecision", "The precision is RT_C128, RT_C64 or RT_MIXED.");
+        p->p->ctx->activate();
+        auto h = std::make_unique<rt_product>();
+        h->p = create_preconditioner(p->p.get(), kind, precision);
+        *out = h.release();
+    });
+}
+
+void rt_product_destroy(rt_product *p) {
+    if (!p) return;
+    if (p->p && p->p->problem) (void)hipStreamSynchronize(p->p->problem->ctx->stream);
+    delete p;
+}
+
+int rt_product_apply_device(rt_product *p, const void *in, void *out, int64_t n, int64_t nrhs, int32_t precision) {
+    return guarded([&] {
+        need(p && in && out, RT_EINVAL, "rooftop:product", "NULL argument");
+        need(n == p->p->dims.n && nrhs >= 1 && nrhs <= 65535, RT_ESIZE, "rooftopMoM:product:sizeMismatch",
+             "The input must be a numeric Nx1 vector (N rooftops) per right-hand side.");
+        Context *c = p->p->problem->ctx;
+        c->activate();
+        p->p->apply(static_cast<const c128 *>(in), static_cast<c128 *>(out), static_cast<int>(nrhs), precision,
+                    c->stream);
+    });
+}
+
+int rt_product_apply(rt_product *p, const void *in, void *out, int64_t n, int64_t nrhs, int32_t precision) {
+    return guarded([&] {
+        need(p && in && out, RT_EINVAL, "rooftop:product", "NULL argument");
+        need(n == p->p->dims.n && nrhs >= 1 && nrhs <= 65535, RT_ESIZE, "rooftopMoM:product:sizeMismatch",
+             "The input must be a numeric Nx1 vector (N rooftops) per right-hand side.");
+        Context *c = p->p->problem->ctx;
+        c->activate();
+        DeviceArray<c128> din, dout;
+        din.allocate(c, static_cast<size_t>(n * nrhs), "the product's input");
+        dout.allocate(c, static_cast<size_t>(n * nrhs), "the product's output");
+        RT_HIP(hipMemcpyAsync(din.ptr, in, din.bytes(), hipMemcpyHostToDevice, c->stream));
+        p->p->apply(din.ptr, dout.ptr, static_cast<int>(nrhs), precision, c->stream);
+        RT_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int rt_product_memory(rt_product *p, uint64_t *stored, uint64_t *working) {
+    return guarded([&] {
+        need(p != nullptr, RT_EINVAL, "rooftop:product", "NULL product");
+        if (stored) *stored = p->p->stored_bytes();
+        if (working) *working = p->p->working_bytes();
+    });
+}
+
+int rt_solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_opts *opts, const void *rhs,
+                    int64_t n, int64_t nrhs, const void *x0, void *x, rt_output *out) {
+    return guarded([&] {
+        need(rhs && x, RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide", "NULL right-hand side or solution");
+        SolveResult r = solve_device(ctx, op, pc, opts, static_cast<const c128 *>(rhs), n, nrhs,
+                                     static_cast<const c128 *>(x0), static_cast<c128 *>(x));
+        fill_output(r, out);
+    });
+}
+
+int rt_solve(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_opts *opts, const void *rhs, int64_t n,
+             int64_t nrhs, const void *x0, void *x, rt_output *out) {
+    return guarded([&] {
+        need(ctx && rhs && x, RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide", "NULL argument");
+        need(n >= 1 && nrhs >= 1, RT_ESIZE, "rooftopMoM:solver:sizeMismatch", "empty right-hand side");
+        check_finite(static_cast<const c128 *>(rhs), n * nrhs);
+        Context *c = &ctx->c;
+        c->activate();
+        const size_t count = static_cast<size_t>(n * nrhs);
+        DeviceArray<c128> db, dx, dx0;
+        db.allocate(c, count, "the right-hand sides");
+        dx.allocate(c, count, "the solutions");
+        RT_HIP(hipMemcpyAsync(db.ptr, rhs, db.bytes(), hipMemcpyHostToDevice, c->stream));
+        if (x0) {
+            dx0.allocate(c, count, "the initial guesses");
+            RT_HIP(hipMemcpyAsync(dx0.ptr, x0, dx0.bytes(), hipMemcpyHostToDevice, c->stream));
+        }
+        SolveResult r = solve_device(ctx, op, pc, opts, db.ptr, n, nrhs, x0 ? dx0.ptr : nullptr, dx.ptr);
+        RT_HIP(hipMemcpyAsync(x, dx.ptr, dx.bytes(), hipMemcpyDeviceToHost, c->stream));
+        RT_HIP(hipStreamSynchronize(c->stream));
+        fill_output(r, out);
+    });
+}
+
+int rt_device_alloc(rt_ctx *ctx, uint64_t bytes, void **ptr) {
+    return guarded([&] {
+        need(ctx && ptr, RT_EINVAL, "rooftop:ctx", "NULL argument");
+        ctx->c.activate();
+        *ptr = ctx->c.alloc(bytes, "A caller buffer");
+    });
+}
+
+int rt_device_free(rt_ctx *ctx, void *ptr) {
+    return guarded([&] {
+        need(ctx != nullptr, RT_EINVAL, "rooftop:ctx", "NULL ctx");
+        ctx->c.activate();
+        RT_HIP(hipStreamSynchronize(ctx->c.stream));
+        ctx->c.release(ptr);
+    });
+}
+
+int rt_copy_to_device(rt_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    return guarded([&] {
+        need(ctx && dst && src, RT_EINVAL, "rooftop:ctx", "NULL argument");
+        ctx->c.activate();
+        RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+        RT_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int rt_copy_to_host(rt_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    return guarded([&] {
+        need(ctx && dst && src, RT_EINVAL, "rooftop:ctx", "NULL argument");
+        ctx->c.activate();
+        RT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+        RT_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+const char *rt_last_error_id(void) { return rt::last_error_id(); }
+const char *rt_last_error_message(void) { return rt::last_error_message(); }
+const char *rt_version(void) { return "rooftop-hip 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// Error state and device-memory accounting of librooftop.
+#include <cstdarg>
+#include <cstdio>
+
+#include "common.hpp"
+
+namespace rt {
+
+namespace {
+thread_local int t_status = RT_OK;
+thread_local std::string t_id;
+thread_local std::string t_message;
+
+std::string format_bytes(uint64_t b) {
+    char buf[64];
+    const char *units[] = {"B", "KiB", "MiB", "GiB", "TiB"};
+    double v = static_cast<double>(b);
+    int u = 0;
+    while (v >= 1024.0 && u < 4) { v /= 1024.0; ++u; }
+    std::snprintf(buf, sizeof buf, "%.3g %s", v, units[u]);
+    return buf;
+}
+}  // namespace
+
+void fail(int status, const char *id, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Error(status, id, buf);
+}
+
+void set_last_error(int status, const std::string &id, const std::string &message) {
+    t_status = status;
+    t_id = id;
+    t_message = message;
+}
+
+const char *last_error_id() { return t_id.c_str(); }
+const char *last_error_message() { return t_message.c_str(); }
+
+void Context::require(uint64_t bytes, const char *what) {
+    std::lock_guard<std::mutex> g(lock);
+    if (held + bytes > ceiling) {
+        uint64_t left = ceiling > held ? ceiling - held : 0;
+        fail(RT_ENOMEM, "rooftopMoM:resources:insufficientMemory",
+             "%s needs %s, but only %s of the memory ceiling of %s are left.", what, format_bytes(bytes).c_str(),
+             format_bytes(left).c_str(), format_bytes(ceiling).c_str());
+    }
+}
+
+void *Context::alloc(uint64_t bytes, const char *what) {
+    require(bytes, what);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(RT_ENOMEM, "rooftopMoM:resources:insufficientMemory", "%s needs %s, but the device refused it (%s).",
+             what, format_bytes(bytes).c_str(), hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> g(lock);
+    held += bytes;
+    sizes[p] = bytes;
+    return p;
+}
+
+void Context::release(void *ptr) {
+    if (!ptr) return;
+    uint64_t bytes = 0;
+    {
+        std::lock_guard<std::mutex> g(lock);
+        auto it = sizes.find(ptr);
+        if (it == sizes.end()) return;
+        bytes = it->second;
+        sizes.erase(it);
+        held -= bytes;
+    }
+    (void)hipFree(ptr);
+}
+
+}  // namespace rt

@@ -1,0 +1,131 @@
+// Shared types of librooftop: complex arithmetic, error state, device memory accounting and the context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+#include "rooftop.h"
+
+namespace rt {
+
+// ---------------------------------------------------------------------------------------------------------------
+// Complex numbers: plain POD pairs so buffers are interleaved (re, im) like MATLAB's interleaved complex API.
+// ---------------------------------------------------------------------------------------------------------------
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+    T x, y;
+};
+using c128 = cplx<double>;
+using c64 = cplx<float>;
+
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mk(T x, T y) { return cplx<T>{x, y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator*(cplx<T> a, cplx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator*(T s, cplx<T> a) { return {s * a.x, s * a.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> conjg(cplx<T> a) { return {a.x, -a.y}; }
+// conj(a) * b
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+    return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+template <typename T>
+__host__ __device__ __forceinline__ T abs2(cplx<T> a) { return a.x * a.x + a.y * a.y; }
+
+template <typename To, typename From>
+__host__ __device__ __forceinline__ cplx<To> cvt(cplx<From> a) { return {static_cast<To>(a.x), static_cast<To>(a.y)}; }
+
+// ---------------------------------------------------------------------------------------------------------------
+// Errors: every failure becomes an Error carrying a status, the reference's identifier and a message; the C ABI
+// catches it, stores it thread-locally and returns the status.
+// ---------------------------------------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int status;
+    std::string id;
+    Error(int s, std::string i, const std::string &m) : std::runtime_error(m), status(s), id(std::move(i)) {}
+};
+
+[[noreturn]] void fail(int status, const char *id, const char *fmt, ...);
+void set_last_error(int status, const std::string &id, const std::string &message);
+
+#define RT_HIP(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            ::rt::fail(RT_EHIP, "rooftop:hip", "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                                           \
+    } while (0)
+
+#define RT_LAUNCH_CHECK() RT_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------------------------------------------
+// Context: one device, one stream, a byte ceiling (the Budget of the reference) and the allocations it holds.
+// ---------------------------------------------------------------------------------------------------------------
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t ceiling = 0;
+    uint64_t held = 0;
+    std::mutex lock;
+    std::unordered_map<void *, uint64_t> sizes;
+
+    // Allocates device memory after checking the ceiling like Budget.require (Budget.m:82-106).
+    void *alloc(uint64_t bytes, const char *what);
+    void release(void *ptr);
+    // Raises rooftopMoM:resources:insufficientMemory if bytes more do not fit under the ceiling.
+    void require(uint64_t bytes, const char *what);
+    void activate() const { RT_HIP(hipSetDevice(device)); }
+};
+
+// A device buffer owned through a context.
+template <typename T>
+struct DeviceArray {
+    Context *ctx = nullptr;
+    T *ptr = nullptr;
+    size_t count = 0;
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray &) = delete;
+    DeviceArray &operator=(const DeviceArray &) = delete;
+    DeviceArray(DeviceArray &&o) noexcept : ctx(o.ctx), ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    DeviceArray &operator=(DeviceArray &&o) noexcept {
+        if (this != &o) { reset(); ctx = o.ctx; ptr = o.ptr; count = o.count; o.ptr = nullptr; o.count = 0; }
+        return *this;
+    }
+    ~DeviceArray() { reset(); }
+    void allocate(Context *c, size_t n, const char *what) {
+        reset();
+        ctx = c;
+        count = n;
+        ptr = n ? static_cast<T *>(c->alloc(n * sizeof(T), what)) : nullptr;
+    }
+    void reset() {
+        if (ptr && ctx) ctx->release(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    uint64_t bytes() const { return count * sizeof(T); }
+    void zero(hipStream_t s) { if (count) RT_HIP(hipMemsetAsync(ptr, 0, bytes(), s)); }
+};
+
+inline unsigned blocks_for(size_t n, unsigned threads, unsigned cap = 65535u * 16u) {
+    size_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return static_cast<unsigned>(b);
+}
+
+}  // namespace rt

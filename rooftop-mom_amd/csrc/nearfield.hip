@@ -1,0 +1,367 @@
+// Preconditioners on gfx950: the Jacobi diagonal and the near-field preconditioner on 3x3-cell neighbourhoods.
+//
+// The reference's preconditioners are circulant ones (src/+rooftopMoM/Preconditioner.m:41-51) and it passes Jacobi as
+// a handle (tests/+unit/+solver/ConjugateGradientTest.m:55-59). The near-field preconditioner is this project's:
+// a symmetrized sparse approximate inverse P = (M + M^T)/2 whose row i is the least-squares fit
+//     min || Z[F_i, S_i] m - e_i ||,
+// S_i the rooftops with their first face within one cell of that of i (the 3x3-cell neighbourhood, <= 18), F_i those
+// within two cells (<= 50). Entries of Z come from the generators through the spatial-block formulas of
+// +preconditioner/+utils/generateSpatialBlocks.m:19-61. Z depends on offsets only, so every rooftop whose
+// neighbourhood has the same existing rooftops shares its row: rows are fitted once per class, and P is applied as
+// an 18-point stencil over the two rooftop grids with the class coefficients.
+#include <unordered_map>
+#include <vector>
+
+#include "rooftop.hpp"
+
+namespace rt {
+
+constexpr int kSlots = 18;    // 2 directions x 3 x 3 first-face offsets
+constexpr int kFit = 50;      // 2 directions x 5 x 5
+
+// Generator value at absolute offsets (the generators are even in each offset); zero outside.
+__device__ __forceinline__ c128 gen_at(const c128 *g, int rows, int cols, int p, int q) {
+    p = p < 0 ? -p : p;
+    q = q < 0 ? -q : q;
+    if (!g || p >= rows || q >= cols) return mk(0.0, 0.0);
+    return g[p + static_cast<int64_t>(rows) * q];
+}
+
+struct GenView {
+    const c128 *mxx, *myy, *e;
+    int cx, cy;
+    double a, b;
+};
+
+// Entry of Z between a rooftop of direction dr and one of direction dc whose first faces differ by (p, q)
+// (generateSpatialBlocks.m:35-48).
+__device__ c128 z_entry(const GenView &v, int dr, int dc, int p, int q) {
+    auto E = [&](int x, int y) { return gen_at(v.e, v.cx, v.cy, x, y); };
+    if (dr == 0 && dc == 0)
+        return gen_at(v.mxx, v.cx - 1, v.cy, p, q) - (1.0 / (v.a * v.a)) * (2.0 * E(p, q) - E(p - 1, q) - E(p + 1, q));
+    if (dr == 1 && dc == 1)
+        return gen_at(v.myy, v.cx, v.cy - 1, p, q) - (1.0 / (v.b * v.b)) * (2.0 * E(p, q) - E(p, q - 1) - E(p, q + 1));
+    if (dr == 1) { p = -p; q = -q; }   // YX(p, q) = XY(-p, -q)
+    return (-1.0 / (v.a * v.b)) * (E(p, q) - E(p, q - 1) - E(p + 1, q) + E(p + 1, q - 1));
+}
+
+// One thread per class: build the fit matrix of the class's representative neighbourhood, complex Householder QR,
+// back substitution; coefficients by slot (0 where no rooftop). scratch holds kFit x (kSlots + 1) per class.
+__global__ void sai_fit(GenView v, const int *__restrict__ dir, const unsigned long long *__restrict__ bits,
+                        int classes, c128 *__restrict__ scratch, c128 *__restrict__ coef, int *__restrict__ bad) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= classes) return;
+    const unsigned long long mask = bits[c];
+    const int d0 = dir[c];
+    int slot_of_col[kSlots];
+    int col_dir[kSlots], col_x[kSlots], col_y[kSlots];
+    int row_dir[kFit], row_x[kFit], row_y[kFit];
+    int n = 0, m = 0, self = -1;
+    for (int d = 0; d < 2; ++d)
+        for (int dy = -2; dy <= 2; ++dy)
+            for (int dx = -2; dx <= 2; ++dx) {
+                const int bit = d * 25 + (dx + 2) + 5 * (dy + 2);
+                if (!((mask >> bit) & 1ull)) continue;
+                row_dir[m] = d;
+                row_x[m] = dx;
+                row_y[m] = dy;
+                if (d == d0 && dx == 0 && dy == 0) self = m;
+                ++m;
+                if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) {
+                    slot_of_col[n] = d * 9 + (dx + 1) + 3 * (dy + 1);
+                    col_dir[n] = d;
+                    col_x[n] = dx;
+                    col_y[n] = dy;
+                    ++n;
+                }
+            }
+    c128 *A = scratch + static_cast<int64_t>(c) * kFit * (kSlots + 1);   // column-major m x (n + 1), last = rhs
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i)
+            A[i + kFit * j] = z_entry(v, row_dir[i], col_dir[j], row_x[i] - col_x[j], row_y[i] - col_y[j]);
+    for (int i = 0; i < m; ++i) A[i + kFit * n] = mk(i == self ? 1.0 : 0.0, 0.0);
+    bool ok = self >= 0 && n > 0;
+    for (int k = 0; k < n && ok; ++k) {
+        double nrm2 = 0;
+        for (int i = k; i < m; ++i) nrm2 += abs2(A[i + kFit * k]);
+        const double nrm = sqrt(nrm2);
+        if (!(nrm > 0)) { ok = false; break; }
+        const c128 x0 = A[k + kFit * k];
+        const double ax0 = sqrt(abs2(x0));
+        const c128 phase = ax0 > 0 ? mk(x0.x / ax0, x0.y / ax0) : mk(1.0, 0.0);
+        // v = x + phase * nrm e_0, stored in place of column k; tau = 1 / (nrm (nrm + |x0|)).
+        A[k + kFit * k] = x0 + nrm * phase;
+        const double tau = 1.0 / (nrm * (nrm + ax0));
+        for (int j = k + 1; j <= n; ++j) {
+            c128 w = mk(0.0, 0.0);
+            for (int i = k; i < m; ++i) w = w + cmulc(A[i + kFit * k], A[i + kFit * j]);
+            w = tau * w;
+            for (int i = k; i < m; ++i) A[i + kFit * j] = A[i + kFit * j] - A[i + kFit * k] * w;
+        }
+        A[k + kFit * k] = -nrm * phase;   // R(k, k)
+    }
+    c128 x[kSlots];
+    if (ok) {
+        for (int k = n - 1; k >= 0; --k) {
+            c128 s = A[k + kFit * n];
+            for (int j = k + 1; j < n; ++j) s = s - A[k + kFit * j] * x[j];
+            const c128 r = A[k + kFit * k];
+            const double den = abs2(r);
+            x[k] = (1.0 / den) * cmulc(r, s);
+            if (!(den > 0) || !isfinite(x[k].x) || !isfinite(x[k].y)) ok = false;
+        }
+    }
+    for (int s = 0; s < kSlots; ++s) coef[static_cast<int64_t>(c) * kSlots + s] = mk(0.0, 0.0);
+    if (!ok) {
+        atomicAdd(bad, 1);
+        return;
+    }
+    for (int j = 0; j < n; ++j) coef[static_cast<int64_t>(c) * kSlots + slot_of_col[j]] = x[j];
+}
+
+// z = P r with P = (M + M^T)/2 as an 18-point stencil; one thread per rooftop and right-hand side.
+template <typename TC>
+__global__ void __launch_bounds__(256) sai_apply(const c128 *__restrict__ in, c128 *__restrict__ out,
+                                                 const int *__restrict__ cls, const cplx<TC> *__restrict__ coef,
+                                                 int rows_x, int cols_x, int rows_y, int cols_y, int64_t nx,
+                                                 int64_t n, int nrhs) {
+    const int64_t id = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (id >= n * nrhs) return;
+    const int64_t r = id / n;
+    const int64_t i = id - r * n;
+    const int ci = cls[i];
+    if (ci < 0) {
+        out[id] = mk(0.0, 0.0);
+        return;
+    }
+    int di, fx, fy;
+    if (i < nx) { di = 0; fx = static_cast<int>(i % rows_x); fy = static_cast<int>(i / rows_x); }
+    else { di = 1; const int64_t k = i - nx; fx = static_cast<int>(k % rows_y); fy = static_cast<int>(k / rows_y); }
+    const c128 *x = in + r * n;
+    cplx<TC> acc = mk<TC>(0, 0);
+    for (int d = 0; d < 2; ++d) {
+        const int rows = d ? rows_y : rows_x, cols = d ? cols_y : cols_x;
+        const int64_t base = d ? nx : 0;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int gx = fx + dx, gy = fy + dy;
+                if (gx < 0 || gy < 0 || gx >= rows || gy >= cols) continue;
+                const int64_t j = base + gx + static_cast<int64_t>(rows) * gy;
+                const int cj = cls[j];
+                if (cj < 0) continue;
+                const int s = d * 9 + (dx + 1) + 3 * (dy + 1);
+                const int sm = di * 9 + (1 - dx) + 3 * (1 - dy);
+                const cplx<TC> cf = coef[static_cast<int64_t>(ci) * kSlots + s] + coef[static_cast<int64_t>(cj) * kSlots + sm];
+                acc = acc + cf * cvt<TC>(x[j]);
+            }
+    }
+    out[id] = cvt<double>(static_cast<TC>(0.5) * acc);
+}
+
+// Jacobi: z = r / diag(Z); diag is XX(0,0) on x rooftops and YY(0,0) on y rooftops (generateSpatialBlocks.m:35-45).
+__global__ void jacobi_diag(GenView v, c128 *__restrict__ inv) {
+    if (threadIdx.x != 0) return;
+    for (int d = 0; d < 2; ++d) {
+        if (d == 0 && !v.mxx) { inv[0] = mk(0.0, 0.0); continue; }
+        if (d == 1 && !v.myy) { inv[1] = mk(0.0, 0.0); continue; }
+        const c128 z = z_entry(v, d, d, 0, 0);
+        const double den = abs2(z);
+        inv[d] = (1.0 / den) * conjg(z);
+    }
+}
+
+template <typename TC>
+__global__ void jacobi_apply(const c128 *__restrict__ in, c128 *__restrict__ out, const cplx<TC> *__restrict__ inv,
+                             const uint8_t *__restrict__ exists, int64_t nx, int64_t n, int nrhs) {
+    const int64_t total = n * nrhs;
+    for (int64_t id = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; id < total;
+         id += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t i = id % n;
+        cplx<TC> v = inv[i < nx ? 0 : 1] * cvt<TC>(in[id]);
+        if (exists && !exists[i]) v = mk<TC>(0, 0);
+        out[id] = cvt<double>(v);
+    }
+}
+
+__global__ void to_c64_pc(const c128 *__restrict__ a, c64 *__restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        b[i] = cvt<float>(a[i]);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// Host side.
+// ------------------------------------------------------------------------------------------------------------------
+// Existence of every rooftop of the grid (a rooftop exists if both its faces are metal; all of them on a full plate).
+static std::vector<uint8_t> rooftop_existence(const Problem &p) {
+    const GridInfo &g = p.g;
+    std::vector<uint8_t> ex(static_cast<size_t>(g.n), 1);
+    if (p.face_mask.empty()) return ex;
+    auto face = [&](int x, int y) { return p.face_mask[x + static_cast<size_t>(g.cx) * y] != 0; };
+    for (int j = 0; j < g.cols_x; ++j)
+        for (int i = 0; i < g.rows_x; ++i) ex[i + static_cast<size_t>(g.rows_x) * j] = face(i, j) && face(i + 1, j);
+    for (int j = 0; j < g.cols_y; ++j)
+        for (int i = 0; i < g.rows_y; ++i)
+            ex[g.nx + i + static_cast<size_t>(g.rows_y) * j] = face(i, j) && face(i, j + 1);
+    return ex;
+}
+
+static GenView view_of(Problem *p) {
+    GenView v;
+    v.mxx = p->g.has_x ? p->mxx.ptr : nullptr;
+    v.myy = p->g.has_y ? p->myy.ptr : nullptr;
+    v.e = p->e.ptr;
+    v.cx = p->g.cx;
+    v.cy = p->g.cy;
+    v.a = p->g.a;
+    v.b = p->g.b;
+    return v;
+}
+
+std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precision) {
+    Context *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const GridInfo &g = p->g;
+    p->ensure_generators();
+    auto pc = std::make_unique<Product>();
+    pc->problem = p;
+    pc->precision = precision;
+    pc->dims.n = g.n;
+    pc->dims.nx_count = g.nx;
+    pc->dims.rows_x = g.rows_x;
+    pc->dims.rows_y = g.rows_y;
+    pc->dims.cols_x = g.has_x ? g.cols_x : 0;
+    pc->dims.cols_y = g.has_y ? g.cols_y : 0;
+    const GenView v = view_of(p);
+    DeviceArray<int> bad;
+    bad.allocate(ctx, 1, "preconditioner status");
+    bad.zero(s);
+    if (kind == RT_PRECOND_JACOBI) {
+        pc->type = kJacobi;
+        pc->diag64.allocate(ctx, 2, "The Jacobi diagonal");
+        hipLaunchKernelGGL(jacobi_diag, dim3(1), dim3(64), 0, s, v, pc->diag64.ptr);
+        RT_LAUNCH_CHECK();
+        c128 h[2];
+        RT_HIP(hipMemcpyAsync(h, pc->diag64.ptr, sizeof h, hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        for (int d = 0; d < 2; ++d) {
+            const bool present = d ? g.has_y : g.has_x;
+            if (present && !(std::isfinite(h[d].x) && std::isfinite(h[d].y) && (h[d].x != 0 || h[d].y != 0)))
+                fail(RT_ESINGULAR, "rooftopMoM:preconditioner:singular",
+                     "The preconditioner is singular: the diagonal of the matrix has a zero or non-finite entry.");
+        }
+        if (precision != RT_C128) {
+            pc->diag32.allocate(ctx, 2, "The Jacobi diagonal");
+            hipLaunchKernelGGL(to_c64_pc, dim3(1), dim3(64), 0, s, pc->diag64.ptr, pc->diag32.ptr, int64_t(2));
+            RT_LAUNCH_CHECK();
+        }
+    } else if (kind == RT_PRECOND_NEARFIELD) {
+        pc->type = kNearField;
+        const std::vector<uint8_t> ex = rooftop_existence(*p);
+        auto exists_at = [&](int d, int x, int y) -> bool {
+            const int rows = d ? g.rows_y : g.rows_x, cols = d ? g.cols_y : g.cols_x;
+            if (x < 0 || y < 0 || x >= rows || y >= cols) return false;
+            return ex[(d ? g.nx : 0) + x + static_cast<size_t>(rows) * y] != 0;
+        };
+        std::vector<int> cls(static_cast<size_t>(g.n), -1);
+        std::vector<int> rep_dir;
+        std::vector<unsigned long long> rep_bits;
+        std::unordered_map<unsigned long long, int> index;
+        for (int d = 0; d < 2; ++d) {
+            const int rows = d ? g.rows_y : g.rows_x, cols = d ? g.cols_y : g.cols_x;
+            if (!(d ? g.has_y : g.has_x)) continue;
+            for (int y = 0; y < cols; ++y)
+                for (int x = 0; x < rows; ++x) {
+                    const size_t i = (d ? g.nx : 0) + x + static_cast<size_t>(rows) * y;
+                    if (!ex[i]) continue;
+                    unsigned long long bits = 0;
+                    for (int dd = 0; dd < 2; ++dd)
+                        for (int dy = -2; dy <= 2; ++dy)
+                            for (int dx = -2; dx <= 2; ++dx)
+                                if (exists_at(dd, x + dx, y + dy))
+                                    bits |= 1ull << (dd * 25 + (dx + 2) + 5 * (dy + 2));
+                    const unsigned long long key = bits | (static_cast<unsigned long long>(d) << 60);
+                    auto it = index.find(key);
+                    if (it == index.end()) {
+                        it = index.emplace(key, static_cast<int>(rep_dir.size())).first;
+                        rep_dir.push_back(d);
+                        rep_bits.push_back(bits);
+                    }
+                    cls[i] = it->second;
+                }
+        }
+        const int classes = static_cast<int>(rep_dir.size());
+        pc->class_count = classes;
+        pc->cls.allocate(ctx, cls.size(), "The near-field classes");
+        RT_HIP(hipMemcpyAsync(pc->cls.ptr, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        pc->coef64.allocate(ctx, static_cast<size_t>(classes) * kSlots, "The near-field coefficients");
+        if (classes > 0) {
+            DeviceArray<int> d_dir;
+            DeviceArray<unsigned long long> d_bits;
+            DeviceArray<c128> scratch;
+            d_dir.allocate(ctx, classes, "near-field fit");
+            d_bits.allocate(ctx, classes, "near-field fit");
+            scratch.allocate(ctx, static_cast<size_t>(classes) * kFit * (kSlots + 1), "near-field fit");
+            RT_HIP(hipMemcpyAsync(d_dir.ptr, rep_dir.data(), classes * sizeof(int), hipMemcpyHostToDevice, s));
+            RT_HIP(hipMemcpyAsync(d_bits.ptr, rep_bits.data(), classes * sizeof(unsigned long long),
+                                  hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(sai_fit, dim3((classes + 63) / 64), dim3(64), 0, s, v, d_dir.ptr, d_bits.ptr, classes,
+                               scratch.ptr, pc->coef64.ptr, bad.ptr);
+            RT_LAUNCH_CHECK();
+            int h_bad = 0;
+            RT_HIP(hipMemcpyAsync(&h_bad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+            RT_HIP(hipStreamSynchronize(s));
+            if (h_bad)
+                fail(RT_ESINGULAR, "rooftopMoM:preconditioner:singular",
+                     "The preconditioner is singular: %d of %d near-field fits are rank deficient or not finite.",
+                     h_bad, classes);
+        }
+        if (precision != RT_C128 && classes > 0) {
+            pc->coef32.allocate(ctx, static_cast<size_t>(classes) * kSlots, "The near-field coefficients");
+            hipLaunchKernelGGL(to_c64_pc, dim3(blocks_for(static_cast<size_t>(classes) * kSlots, 256, 1024)), dim3(256),
+                               0, s, pc->coef64.ptr, pc->coef32.ptr, static_cast<int64_t>(classes) * kSlots);
+            RT_LAUNCH_CHECK();
+        }
+    } else {
+        fail(RT_EINVAL, "rooftop:preconditioner", "unknown preconditioner kind %d", kind);
+    }
+    if (precision == RT_C64) {
+        RT_HIP(hipStreamSynchronize(s));
+        pc->diag64.reset();
+        pc->coef64.reset();
+    }
+    RT_HIP(hipStreamSynchronize(s));
+    return pc;
+}
+
+void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s) {
+    if (prec == RT_MIXED) prec = RT_C128;
+    if (!pc.has(prec)) fail(RT_EINVAL, "rooftop:product", "the preconditioner does not hold the requested precision");
+    const ProdDims &d = pc.dims;
+    const int64_t total = d.n * nrhs;
+    if (pc.type == kJacobi) {
+        const unsigned blocks = blocks_for(total, 256, 8192);
+        if (prec == RT_C128)
+            hipLaunchKernelGGL(jacobi_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag64.ptr,
+                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs);
+        else
+            hipLaunchKernelGGL(jacobi_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag32.ptr,
+                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs);
+        RT_LAUNCH_CHECK();
+        return;
+    }
+    const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
+    if (pc.class_count == 0) {
+        RT_HIP(hipMemsetAsync(out, 0, total * sizeof(c128), s));
+        return;
+    }
+    if (prec == RT_C128)
+        hipLaunchKernelGGL(sai_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef64.ptr,
+                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs);
+    else
+        hipLaunchKernelGGL(sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef32.ptr,
+                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs);
+    RT_LAUNCH_CHECK();
+}
+
+}  // namespace rt

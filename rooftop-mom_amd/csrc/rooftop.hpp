@@ -1,0 +1,105 @@
+// Internal objects of librooftop: the problem (grid, wavenumber, generators on the device), the Toeplitz product
+// (circulant embedding spectra and the three-pass FFT matvec) and the preconditioners.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "fft.hpp"
+
+namespace rt {
+
+// Grid of countX x countY faces of sizeX x sizeY and its rooftop layout (UniformRooftops.m:31-101): x-directed
+// rooftops on a (countX-1) x countY grid first, then y-directed ones on countX x (countY-1), x index fastest.
+struct GridInfo {
+    int cx = 0, cy = 0;
+    double a = 0, b = 0;
+    int rows_x = 0, cols_x = 0, rows_y = 0, cols_y = 0;
+    int64_t nx = 0, ny = 0, n = 0;
+    bool has_x = false, has_y = false;
+    static GridInfo make(int cx, int cy, double a, double b);
+};
+
+// Dimensions of one product, passed by value to the kernels.
+struct ProdDims {
+    int rows_x, rows_y;       // rooftop rows (x extent) of each component
+    int cols_x, cols_y;       // rooftop columns (y extent), 0 for a missing component
+    int ny;                   // columns of the work array per component (max of cols_x, cols_y)
+    int64_t nx_count;         // number of x-directed rooftops (offset of the y part of a vector)
+    int64_t n;                // rooftops per right-hand side
+    int Mx, My;               // embedding periods
+    int Hx, Hy;               // folded spectrum extents M/2 + 1
+};
+
+struct Problem {
+    Context *ctx = nullptr;
+    GridInfo g;
+    double k = 0;
+    int near_order = 16, far_order = 8;
+    std::vector<uint8_t> face_mask;          // countX*countY, empty for a full plate
+    DeviceArray<uint8_t> exists;             // per rooftop (n), empty for a full plate
+    DeviceArray<c128> mxx, myy, e;           // generators, column-major
+    bool assembled = false;
+    void ensure_generators();                // assembles on the GPU on first use (Problem.generators)
+    void assemble();                         // generator.hip
+};
+
+// Per-dimension tables of a product: twiddles, fold index and divergence factor D(k) = (1 - exp(2 pi i k/M))/size
+// at each digit-reversed output position.
+struct DimTables {
+    FftPlan plan;
+    DeviceArray<c128> tw;
+    DeviceArray<c64> tw32;
+    DeviceArray<int> fold;
+    DeviceArray<c128> d;
+    DeviceArray<c64> d32;
+    void build(Context *ctx, int M, double size, hipStream_t s);
+};
+
+// Launch shape of the three matvec passes (picked by the autotuner or the default heuristic).
+struct PassShape {
+    int cols_per_block = 1;   // pass A / C
+    int threads_ac = 256;
+    int tile = 4;             // pass B: positions along x per block
+    int threads_b = 256;
+};
+
+enum ProductType { kOperator = 0, kJacobi = 1, kNearField = 2 };
+
+struct Product {
+    int type = kOperator;
+    Problem *problem = nullptr;
+    int precision = RT_C128;
+    int modifier = RT_MODIFIER_NONE;
+    ProdDims dims{};
+    // Operator.
+    DimTables tx, ty;
+    DeviceArray<c128> planes64;      // [3][Hx][Hy]: magneticXX, magneticYY, electric, scaled by 1/(Mx My)
+    DeviceArray<c64> planes32;
+    DeviceArray<c128> work64;        // [nrhs][2][ny][Mx]
+    DeviceArray<c64> work32;
+    int work_rhs64 = 0, work_rhs32 = 0;
+    PassShape shape64, shape32;
+    // Preconditioners.
+    DeviceArray<c128> diag64;                 // Jacobi: 1/diag(Z) per rooftop
+    DeviceArray<c64> diag32;
+    DeviceArray<c128> coef64;                 // near field: [classes][18] symmetrized-SAI row coefficients
+    DeviceArray<c64> coef32;
+    DeviceArray<int> cls;                     // near field: class of every rooftop (-1: no rooftop)
+    int class_count = 0;
+
+    bool has(int prec) const { return precision == RT_MIXED || precision == prec; }
+    void apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s);
+    uint64_t stored_bytes() const;
+    uint64_t working_bytes() const;
+};
+
+// product.hip
+std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier);
+void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s);
+// nearfield.hip
+std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precision);
+void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s);
+
+}  // namespace rt

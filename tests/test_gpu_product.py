@@ -1,0 +1,132 @@
+"""GPU parity of the generators and the Toeplitz product (librooftop.so through the C ABI) against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import rel
+from oracle import assembly as A
+from oracle import spectra as S
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = {'plate': (5, 4, 0.1, 0.15), 'oddPlate': (11, 7, 0.1, 0.1), 'twoFacesWide': (2, 5, 0.1, 0.1),
+         'stripX': (9, 1, 0.1, 0.02), 'stripY': (1, 6, 0.03, 0.1)}
+
+
+@pytest.fixture(scope='module')
+def rt():
+    import __graft_entry__ as ge
+    return ge.load()
+
+
+def _problem(rt, cx, cy, a, b, orders, k=2 * np.pi, mask=None):
+    return rt.Problem(rt.Mesh(cx, cy, a, b), k, quadratureOrder=orders, mask=mask)
+
+
+def _random(n, seed=1, nrhs=None):
+    rng = np.random.default_rng(seed)
+    shape = (n,) if nrhs is None else (n, nrhs)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def test_products_match_reference_mat(rt, reference_data):
+    """tests/+reference/+assembler/+greensMatrix/ApplicatorTest.m:5-21 (1e-13) with the stored v2 generators."""
+    for case in reference_data:
+        print('reference case', case['countX'], case['countY'], flush=True)
+        p = _problem(rt, int(case['countX']), int(case['countY']), case['sizeX'], case['sizeY'],
+                     (int(case['order']),) * 2, case['wavenumber'])
+        p.setGenerators(case['magneticXX'], case['magneticYY'], case['electric'])
+        for product in ('fft', 'splitFFT', 'leanSplitFFT'):
+            y = rt.Operator(p, product=product).apply(case['vector'])
+            assert rel(y, case['productFFT']) < 1e-13
+            assert rel(y, case['productSplit']) < 1e-13
+
+
+@pytest.mark.parametrize('name', sorted(GRIDS))
+def test_generators_match_oracle(rt, name):
+    cx, cy, a, b = GRIDS[name]
+    for orders in ((3, 3), (16, 8)):
+        print('generators', name, orders, flush=True)
+        gpu = _problem(rt, cx, cy, a, b, orders).generators()
+        ref = A.generate(A.Grid(cx, cy, a, b), 2 * np.pi, orders)
+        for key in ('magneticXX', 'magneticYY', 'electric'):
+            if ref[key] is None:
+                assert gpu[key] is None
+                continue
+            np.testing.assert_allclose(gpu[key], ref[key], rtol=1e-13, atol=1e-14 * np.abs(ref[key]).max())
+
+
+def test_generators_elongated_cells_and_larger_grid(rt):
+    for case in ((6, 40, 0.1, 0.01), (30, 1, 0.025, 0.004), (40, 33, 0.05, 0.05)):
+        print('generators', case, flush=True)
+        gpu = _problem(rt, *case, (16, 8)).generators()
+        ref = A.generate(A.Grid(*case), 2 * np.pi, (16, 8))
+        for key in ('magneticXX', 'magneticYY', 'electric'):
+            if ref[key] is not None:
+                assert rel(gpu[key], ref[key]) < 1e-13
+
+
+@pytest.mark.parametrize('name', sorted(GRIDS))
+def test_products_match_dense_matrix(rt, name):
+    """tests/+unit/+assembler/+greensMatrix/ApplicatorTest.m:7-29 (1e-12)."""
+    cx, cy, a, b = GRIDS[name]
+    print('product', name, flush=True)
+    p = _problem(rt, cx, cy, a, b, (3, 3))
+    grid = A.Grid(cx, cy, a, b)
+    Z = S.dense_matrix(A.generate(grid, 2 * np.pi, (3, 3)), grid)
+    x = _random(grid.count)
+    op = rt.Operator(p, precision='mixed')
+    assert rel(op.apply(x), Z @ x) < 1e-12
+    assert rel(op.apply(x, precision='c64'), Z @ x) < 1e-5
+
+
+def test_multiple_rhs_and_modifier(rt):
+    p = _problem(rt, 9, 7, 0.1, 0.12, (4, 4), k=4.0)
+    grid = A.Grid(9, 7, 0.1, 0.12)
+    gen = A.generate(grid, 4.0, (4, 4))
+    Z = S.dense_matrix(gen, grid)
+    X = _random(grid.count, 3, nrhs=5)
+    Y = rt.Operator(p).apply(X)
+    for r in range(5):
+        assert rel(Y[:, r], Z @ X[:, r]) < 1e-12
+    Yi = rt.Operator(p, modifier='imag').apply(X[:, 0])
+    assert rel(Yi, np.imag(Z) @ X[:, 0]) < 1e-12
+
+
+def test_plate_256_matches_oracle_split_fft(rt):
+    """The 256 x 256 plate (lambda/10) at production orders against the oracle's split-FFT product."""
+    n = 256
+    print('plate 256', flush=True)
+    p = _problem(rt, n, n, 0.1, 0.1, (16, 8))
+    grid = A.Grid(n, n, 0.1, 0.1)
+    gen = A.generate(grid, 2 * np.pi, (16, 8))
+    gpu = p.generators()
+    for key in ('magneticXX', 'magneticYY', 'electric'):
+        assert rel(gpu[key], gen[key]) < 1e-13
+    x = _random(grid.count, 5)
+    ref = S.split_fft_product(S.subband_spectra(gen, grid), grid, x)
+    op = rt.Operator(p, precision='mixed')
+    assert rel(op.apply(x), ref) < 1e-12
+    assert rel(op.apply(x, precision='c64'), ref) < 2e-6
+
+
+def test_masked_product_is_restriction(rt):
+    cx, cy = 10, 8
+    mask = np.ones((cx, cy), bool)
+    mask[3:6, 2:5] = False
+    mask[0, :] = False
+    p = _problem(rt, cx, cy, 0.1, 0.1, (4, 4), mask=mask)
+    grid = A.Grid(cx, cy, 0.1, 0.1)
+    Z = S.dense_matrix(A.generate(grid, 2 * np.pi, (4, 4)), grid)
+    ex = np.concatenate([(mask[:-1] & mask[1:]).ravel(order='F'), (mask[:, :-1] & mask[:, 1:]).ravel(order='F')])
+    x = _random(grid.count, 7)
+    expected = ex * (Z @ (ex * x))
+    assert rel(rt.Operator(p).apply(x), expected) < 1e-12
+
+
+def test_apply_checks_the_input(rt):
+    op = rt.Operator(_problem(rt, 5, 4, 0.1, 0.15, (3, 3)))
+    for bad in (np.ones(3), np.ones((1, 31)), np.array(['a'] * 31)):
+        with pytest.raises(rt.RooftopError) as e:
+            op.apply(bad)
+        assert e.value.identifier == 'rooftopMoM:product:sizeMismatch'
+    assert op.memory() > 0 and op.workingMemory() > 16 * 31

@@ -1,0 +1,135 @@
+"""GPU parity of the preconditioners and the Krylov loops (librooftop.so through the C ABI) against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import rel
+from oracle import assembly as A
+from oracle import excitation as X
+from oracle import nearfield as NF
+from oracle import solvers as SV
+from oracle import spectra as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def rt():
+    import __graft_entry__ as ge
+    return ge.load()
+
+
+def _setup(rt, cx, cy, a, b, orders, k=2 * np.pi, mask=None):
+    p = rt.Problem(rt.Mesh(cx, cy, a, b), k, quadratureOrder=orders, mask=mask)
+    grid = A.Grid(cx, cy, a, b)
+    gen = A.generate(grid, k, orders)
+    return p, grid, gen
+
+
+def test_jacobi_matches_diagonal(rt):
+    p, grid, gen = _setup(rt, 7, 5, 0.1, 0.15, (4, 4))
+    Z = S.dense_matrix(gen, grid)
+    x = np.random.default_rng(1).standard_normal(grid.count) + 1j
+    assert rel(rt.Preconditioner(p, 'jacobi').apply(x), x / np.diag(Z)) < 1e-13
+
+
+@pytest.mark.parametrize('size', [(7, 5, 0.1, 0.15), (12, 12, 0.1, 0.1), (2, 6, 0.1, 0.1), (9, 1, 0.1, 0.02)])
+def test_near_field_matches_oracle(rt, size):
+    print('near field', size, flush=True)
+    p, grid, gen = _setup(rt, *size, (4, 4))
+    P = NF.NearField(gen, grid)
+    x = np.random.default_rng(2).standard_normal(grid.count) + 1j * np.random.default_rng(3).standard_normal(grid.count)
+    pc = rt.Preconditioner(p, 'nearField', precision='mixed')
+    assert rel(pc.apply(x), P.apply(x)) < 1e-11
+    assert rel(pc.apply(x, precision='c64'), P.apply(x)) < 1e-5
+
+
+def test_near_field_masked(rt):
+    cx, cy = 10, 9
+    mask = np.ones((cx, cy), bool)
+    mask[4:7, 3:6] = False
+    p, grid, gen = _setup(rt, cx, cy, 0.1, 0.1, (4, 4), mask=mask)
+    mx = mask[:-1] & mask[1:]
+    my = mask[:, :-1] & mask[:, 1:]
+    P = NF.NearField(gen, grid, mx, my)
+    x = np.random.default_rng(4).standard_normal(grid.count) + 0.5j
+    assert rel(rt.Preconditioner(p, 'nearField').apply(x), P.apply(x)) < 1e-11
+
+
+@pytest.mark.parametrize('method', ['cg', 'gmres'])
+def test_plate_solve_matches_oracle(rt, method):
+    """SolverTest.m:9-21 (8x8 plate, plane wave, 1e-10) plus the iteration record against the oracle's loop."""
+    p, grid, gen = _setup(rt, 8, 8, 0.05, 0.05, (3, 3))
+    exc = rt.Excitation.planeWave(p)
+    b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [1, 0, 0], 1.0, order=3)
+    assert rel(exc.vector(), b) < 1e-13
+    res = rt.Solver(method, tolerance=1e-10).solve(rt.Operator(p), exc)
+    Z = S.dense_matrix(gen, grid)
+    assert res.converged() and rel(Z @ res.solution(), b) < 1e-10
+    ref = SV.solve(method, Z, b, tol=1e-10, max_iter=1000, restart=50)
+    assert res.iterationCount() == ref['iterationCount']
+    np.testing.assert_allclose(res.residualNorms(), ref['residualNorms'], rtol=1e-6, atol=1e-13)
+    assert abs(res.residualNorm() - ref['residualNorm']) < 1e-12
+
+
+@pytest.mark.parametrize('method', ['cg', 'gmres'])
+def test_preconditioned_solve_matches_oracle(rt, method):
+    p, grid, gen = _setup(rt, 16, 14, 0.1, 0.1, (6, 4))
+    Z = S.dense_matrix(gen, grid)
+    P = NF.NearField(gen, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0.3, 0.4, -np.sqrt(0.75)], [0.8, -0.6, 0], 1.0, order=4)
+    res = rt.Solver(method, tolerance=1e-8, restart=20).solve(rt.Operator(p), b,
+                                                            preconditioner=rt.Preconditioner(p, 'nearField'))
+    ref = SV.solve(method, Z, b, tol=1e-8, max_iter=1000, restart=20, preconditioner=P.apply)
+    assert res.converged() and res.iterationCount() == ref['iterationCount']
+    assert rel(res.solution(), ref['solution']) < 1e-6
+    plain = rt.Solver(method, tolerance=1e-8, restart=20).solve(rt.Operator(p), b)
+    assert res.iterationCount() < plain.iterationCount()
+
+
+@pytest.mark.parametrize('method', ['cg', 'gmres'])
+def test_mixed_precision_refinement_reaches_fp64_accuracy(rt, method):
+    p, grid, gen = _setup(rt, 24, 20, 0.1, 0.1, (8, 4))
+    Z = S.dense_matrix(gen, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [1, 0, 0], 1.0, order=4)
+    op = rt.Operator(p, precision='mixed')
+    pc = rt.Preconditioner(p, 'nearField', precision='mixed')
+    mixed = rt.Solver(method, tolerance=1e-10, precision='c64').solve(op, b, preconditioner=pc)
+    assert mixed.converged() and mixed.outerCount >= 1
+    assert mixed.residualNorm() < 1e-10
+    assert rel(Z @ mixed.solution(), b) < 1e-10
+    fp64 = rt.Solver(method, tolerance=1e-10, refine=True).solve(op, b, preconditioner=pc)
+    assert rel(mixed.solution(), fp64.solution()) < 1e-8
+
+
+def test_dipole_input_impedance(rt):
+    """Config 1: 64 x 4 strip dipole, lambda/64 at 1 GHz, middle port, COCG + Jacobi to 1e-10 vs dense solve."""
+    k = 2 * np.pi * 1e9 / 299792458.0
+    a = 2 * np.pi / k / 64
+    p, grid, gen = _setup(rt, 64, 4, a, a, (16, 8), k=k)
+    exc = rt.Excitation.deltaGap(p)
+    res = rt.Solver('cg', tolerance=1e-10, maxIterationCount=5000).solve(
+        rt.Operator(p), exc, preconditioner=rt.Preconditioner(p, 'jacobi'))
+    Z = S.dense_matrix(gen, grid)
+    port = X.middle_port(grid, 'x')
+    b = X.delta_gap(grid, port, 1.0)
+    assert rel(exc.vector(), b) < 1e-15
+    current = np.linalg.solve(Z, b)
+    assert res.converged() and rel(res.solution(), current) < 1e-8
+    zin = res.inputImpedance()
+    ref = X.input_impedance(grid, current, port, 1.0)
+    assert abs(zin - ref) < 1e-8 * abs(ref)
+
+
+def test_solver_errors(rt):
+    p, grid, _ = _setup(rt, 5, 4, 0.1, 0.15, (3, 3))
+    op = rt.Operator(p)
+    with pytest.raises(rt.RooftopError) as e:
+        rt.Solver().solve(op, np.ones(4))
+    assert e.value.identifier == 'rooftopMoM:solver:sizeMismatch'
+    with pytest.raises(rt.RooftopError) as e:
+        rt.Solver().solve(op, np.r_[np.ones(30), np.nan])
+    assert e.value.identifier == 'rooftopMoM:solver:invalidRightHandSide'
+    zero = rt.Solver().solve(op, np.zeros(grid.count))
+    assert zero.converged() and zero.iterationCount() == 0 and np.all(zero.solution() == 0)
+    limited = rt.Solver('gmres', tolerance=1e-14, maxIterationCount=7, restart=5).solve(op, np.ones(grid.count))
+    assert not limited.converged() and limited.iterationCount() == 7
