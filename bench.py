@@ -1,0 +1,195 @@
+"""Benchmark of the iterative-solve path on one MI355X.
+
+Workload (one step = one solve): a square PEC plate of 256 x 256 cells of lambda/10 (k = 2 pi, a = b = 0.1 m,
+130,560 rooftops, quadrature orders [16, 8]), a normally incident x-polarized plane wave, GMRES(50) with the
+near-field preconditioner on 3x3-cell neighbourhoods, to a true relative residual of 1e-6. Two forms:
+  fp64   complex128 products inside an FP64 refinement loop on the true residual;
+  mixed  complex64 products (operator and preconditioner) inside the same FP64 refinement loop.
+The right-hand side and the solution stay in HBM (rt_solve_device); one solve is timed from enqueue to its
+convergence. value = mixed solves/s over the timed steps.
+
+The JSON line also carries p50/p99 latency per form, the error of the mixed solution against the FP64 one, a live
+roofline of the dominant matvec kernel (pass B: y transforms + fused spectral multiply), measured with HIP events
+on the library's stream over the timed solves, and a CPU baseline: the oracle's FP64 split-FFT GMRES on a bounded
+sample of the same solve, timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+"""
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'solves/s + p50/p99 latency at rel.res 1e-6, mixed vs FP64; matvec HBM GB/s'
+PEAK_HBM_GBS = 8000.0
+
+
+def percentile(values, q):
+    return float(np.percentile(np.asarray(values), q)) if values else None
+
+
+def run_form(rt, ctx, op, pc, solver, d_b, d_x, n, steps, warmup):
+    lib = rt.library()
+    o = solver.options()
+    out = rt._abi.rt_output()
+    times, iters, outers, residuals = [], [], [], []
+    for s in range(warmup + steps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        rt.check(lib.rt_solve_device(ctx.handle, op.handle, pc.handle, C.byref(o), C.c_void_p(d_b), n, 1, None,
+                                     C.c_void_p(d_x), C.byref(out)))
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        if s >= warmup:
+            times.append(dt)
+            iters.append(out.iteration_count)
+            outers.append(out.outer_count)
+            residuals.append(out.residual_norm)
+        if not out.converged:
+            raise RuntimeError('solve did not converge: residual %.3e' % out.residual_norm)
+    return times, iters, outers, residuals
+
+
+def cpu_baseline(n, budget_s=10.0):
+    """The oracle's FP64 path (scipy FFTs on the host cores of this GPU's share, split-FFT product, sparse near-field
+    apply, GMRES(50) with CGS2 and the same tolerance) on the same plate: whole solves repeated until about budget_s
+    seconds of CPU work (at least one), setup excluded."""
+    from oracle import assembly as A
+    from oracle import excitation as X
+    from oracle import nearfield as NF
+    from oracle import solvers as SV
+    from oracle import spectra as S
+    cores = max(1, min(16, os.cpu_count() or 1))
+    S.set_workers(cores)
+    grid = A.Grid(n, n, 0.1, 0.1)
+    gen = A.generate(grid, 2 * np.pi, (16, 8))
+    spec = S.subband_spectra(gen, grid)
+    pre = NF.NearField(gen, grid, by_class=True)
+    b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [1, 0, 0], 1.0, order=8)
+    op = lambda x: S.split_fft_product(spec, grid, x)
+    SV.solve('gmres', op, b, tol=1e-6, max_iter=2, restart=50, preconditioner=pre.apply)   # warm-up
+    count, iters, res = 0, 0, 0.0
+    t0 = time.perf_counter()
+    while count == 0 or time.perf_counter() - t0 < budget_s:
+        out = SV.solve('gmres', op, b, tol=1e-6, max_iter=1000, restart=50, preconditioner=pre.apply)
+        count += 1
+        iters = out['iterationCount']
+        res = out['residualNorm']
+    dt = time.perf_counter() - t0
+    S.set_workers(1)
+    return {'value': count / dt, 'unit': 'solves/s', 'cores': cores, 'kind': 'port',
+            'sample': '%d whole solves of the same %dx%d plate in %.1f s: oracle FP64 split-FFT GMRES(50) + '
+                      'near-field, %d iterations each, true residual %.2e (scipy FFT on %d threads)'
+                      % (count, n, n, dt, iters, res, cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=30)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--n', type=int, default=256)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: this build runs the path on a single MI355X and does not shard a solve '
+              'across GPUs (independent solves would shard with no exchange); run it with --gpus 1.',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    rt = ge.load()
+    lib = rt.library()
+    ctx = rt.Context.default()
+    n = args.n
+    print('bench: plate %dx%d, assembling generators and spectra' % (n, n), flush=True)
+    t0 = time.perf_counter()
+    problem = rt.Problem(rt.Mesh(n, n, 0.1, 0.1), 2 * np.pi, quadratureOrder=(16, 8))
+    op = rt.Operator(problem, precision='mixed')
+    pc = rt.Preconditioner(problem, 'nearField', precision='mixed')
+    setup_s = time.perf_counter() - t0
+    exc = rt.Excitation.planeWave(problem)
+    N = problem.rooftopCount()
+    b = np.ascontiguousarray(exc.vector(), dtype=np.complex128)
+    d_b, d_x, d_x64 = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    for ptr in (d_b, d_x, d_x64):
+        rt.check(lib.rt_device_alloc(ctx.handle, 16 * N, C.byref(ptr)))
+    rt.check(lib.rt_copy_to_device(ctx.handle, d_b, b.ctypes.data_as(C.c_void_p), 16 * N))
+
+    fp64 = rt.Solver('gmres', tolerance=1e-6, restart=50, refine=True)
+    mixed = rt.Solver('gmres', tolerance=1e-6, restart=50, precision='c64')
+    print('bench: fp64 form', flush=True)
+    t64, i64, o64, r64 = run_form(rt, ctx, op, pc, fp64, d_b.value, d_x64.value, N, args.steps, args.warmup)
+    print('bench: mixed form', flush=True)
+    rt.check(lib.rt_ctx_set_timing(ctx.handle, 0))
+    launches = (C.c_int64 * 2)()
+    ms = (C.c_double * 6)()
+    by = (C.c_double * 6)()
+    rt.check(lib.rt_ctx_kernel_timing(ctx.handle, launches, ms, by))        # reset the counters
+    rt.check(lib.rt_ctx_set_timing(ctx.handle, 1))
+    tmx, imx, omx, rmx = run_form(rt, ctx, op, pc, mixed, d_b.value, d_x.value, N, args.steps, args.warmup)
+    rt.check(lib.rt_ctx_set_timing(ctx.handle, 0))
+    rt.check(lib.rt_ctx_kernel_timing(ctx.handle, launches, ms, by))
+    x64 = np.empty(N, np.complex128)
+    xmx = np.empty(N, np.complex128)
+    rt.check(lib.rt_copy_to_host(ctx.handle, x64.ctypes.data_as(C.c_void_p), d_x64, 16 * N))
+    rt.check(lib.rt_copy_to_host(ctx.handle, xmx.ctypes.data_as(C.c_void_p), d_x, 16 * N))
+    err = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+
+    # Live roofline: complex64 pass B launches of the timed mixed solves (inner products); FP64 ones are the outer
+    # residuals. ms/bytes are per launch averages.
+    b_ms, b_bytes = ms[4], by[4]
+    achieved = b_bytes / (b_ms * 1e-3) / 1e9 if b_ms > 0 else 0.0
+    prof_ms = (C.c_double * 3)()
+    prof_by = (C.c_double * 3)()
+    rt.check(lib.rt_product_profile(op.handle, 1, 50, rt._abi.RT_C64, prof_ms, prof_by))
+    matvec = {'c64': {}, 'c128': {}}
+    for name, prec in (('c64', rt._abi.RT_C64), ('c128', rt._abi.RT_C128)):
+        rt.check(lib.rt_product_profile(op.handle, 1, 50, prec, prof_ms, prof_by))
+        total_ms = sum(prof_ms)
+        total_bytes = sum(prof_by)
+        matvec[name] = {'ms': round(total_ms, 5), 'bytes': total_bytes,
+                        'GBps': round(total_bytes / (total_ms * 1e-3) / 1e9, 1),
+                        'pass_ms': [round(v, 5) for v in prof_ms]}
+
+    steps = args.steps
+    value = steps / sum(tmx)
+    line = {
+        'metric': METRIC, 'value': round(value, 3), 'unit': 'solves/s', 'n_gpus': 1, 'steps': steps,
+        'warmup': args.warmup, 'ms_per_step': round(1e3 * sum(tmx) / steps, 3), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'complex64 products, FP64 refinement',
+        'data': 'synthetic',
+        'config': {'workload': 'plate %dx%d cells lambda/10, normal-incidence x-polarized plane wave, GMRES(50) + '
+                               'near-field 3x3, true rel.res 1e-6' % (n, n), 'unknowns': N, 'cells': [n, n],
+                   'embedding': [2 * n, 2 * n], 'quadrature_orders': [16, 8]},
+        'mixed': {'solves_per_s': round(value, 3), 'p50_ms': round(1e3 * percentile(tmx, 50), 3),
+                  'p99_ms': round(1e3 * percentile(tmx, 99), 3), 'iterations': int(np.median(imx)),
+                  'outer': int(np.median(omx)), 'residual': max(rmx)},
+        'fp64': {'solves_per_s': round(steps / sum(t64), 3), 'p50_ms': round(1e3 * percentile(t64, 50), 3),
+                 'p99_ms': round(1e3 * percentile(t64, 99), 3), 'iterations': int(np.median(i64)),
+                 'outer': int(np.median(o64)), 'residual': max(r64)},
+        'mixed_vs_fp64_speedup': round(sum(t64) / sum(tmx), 3),
+        'error_mixed_vs_fp64': err,
+        'roofline': {'bound': 'hbm', 'kernel': 'pass_b<float> (y FFT + spectral multiply + inverse y FFT)',
+                     'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
+                     'launches': int(launches[1]), 'ms_per_launch': round(b_ms, 5), 'bytes_per_launch': b_bytes},
+        'matvec': matvec,
+        'setup_s': round(setup_s, 3),
+    }
+    if not args.no_cpu:
+        print('bench: cpu baseline', flush=True)
+        line['cpu_baseline'] = cpu_baseline(n)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

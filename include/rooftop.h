@@ -140,6 +140,21 @@ int rt_product_apply_device(rt_product *product, const void *input, void *output
  * buffers one application uses. */
 int rt_product_memory(rt_product *product, uint64_t *stored, uint64_t *working);
 
+/* Measurement (no counterpart in the reference): runs `reps` applications of the operator on nrhs device vectors
+ * and times each of its three kernels (pass A: x transforms, pass B: y transforms + spectral multiply, pass C:
+ * inverse x transforms) with HIP events on the context's stream. ms[k] is the average duration of one launch of
+ * pass k; bytes[k] its algorithmic HBM bytes per launch (vector and spectra traffic, each element once). */
+int rt_product_profile(rt_product *product, int64_t nrhs, int32_t reps, int32_t precision, double *ms,
+                       double *bytes);
+
+/* Measurement (no counterpart in the reference): while timing is on, every operator application records HIP events
+ * around its three kernels on the context's stream. rt_ctx_kernel_timing drains them (synchronizing the stream) and
+ * reports, per precision p (0: RT_C128, 1: RT_C64) and pass k, the launches counted, the average duration of one
+ * launch in ms (ms[3p + k]) and the average algorithmic bytes of one launch (bytes[3p + k]); then it resets the
+ * counters. */
+int rt_ctx_set_timing(rt_ctx *ctx, int32_t on);
+int rt_ctx_kernel_timing(rt_ctx *ctx, int64_t *launches, double *ms, double *bytes);
+
 /* rooftopMoM.Solver(...).solve(operator, rhs, 'preconditioner', P, 'initialGuess', x0) (Solver.m:42-92,
  * IterativeMethod.m:27-70). precond may be NULL; x0 may be NULL (zeros). nrhs right-hand sides are solved one
  * after another into x (n x nrhs). out (may be NULL) reports the last right-hand side, or the worst for nrhs > 1. */

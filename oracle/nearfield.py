@@ -81,11 +81,51 @@ def sai_matrix(generators, grid, mask_x=None, mask_y=None, radius=1, fit_radius=
     return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
 
 
+def sai_matrix_by_class(generators, grid, mask_x=None, mask_y=None):
+    """The same M as sai_matrix, fitting one row per class of rooftops whose 5x5 windows hold the same existing
+    rooftops (Z depends on offsets only, so such rooftops share their row). Vectorized for large grids."""
+    idx, d, fx, fy = rooftop_offsets(grid, mask_x, mask_y)
+    n = grid.count
+    cx, cy = grid.count_x, grid.count_y
+    present = np.zeros((2, cx + 4, cy + 4), dtype=bool)     # existence padded by 2 on every side
+    present[d, fx + 2, fy + 2] = True
+    sig = np.zeros(idx.size, dtype=np.uint64)
+    for dd in range(2):
+        for dy in range(-2, 3):
+            for dx in range(-2, 3):
+                bit = np.uint64(dd * 25 + (dx + 2) + 5 * (dy + 2))
+                sig |= present[dd, fx + 2 + dx, fy + 2 + dy].astype(np.uint64) << bit
+    key = sig | (d.astype(np.uint64) << np.uint64(60))
+    _, first, cls = np.unique(key, return_index=True, return_inverse=True)
+    slot_coef = np.zeros((first.size, 18), dtype=np.complex128)
+    for c, k in enumerate(first):
+        near = np.nonzero((np.abs(fx - fx[k]) <= 1) & (np.abs(fy - fy[k]) <= 1))[0]
+        fit = np.nonzero((np.abs(fx - fx[k]) <= 2) & (np.abs(fy - fy[k]) <= 2))[0]
+        a = entries(generators, grid, idx[fit], idx[near])
+        m = np.linalg.lstsq(a, (fit == k).astype(np.complex128), rcond=None)[0]
+        slot_coef[c, d[near] * 9 + (fx[near] - fx[k] + 1) + 3 * (fy[near] - fy[k] + 1)] = m
+    # Global index of every (direction, first-face offset), -1 where no rooftop exists.
+    where = np.full((2, cx + 2, cy + 2), -1, dtype=np.int64)
+    where[d, fx + 1, fy + 1] = idx
+    rows, cols, vals = [], [], []
+    for dd in range(2):
+        for dy in range(-1, 2):
+            for dx in range(-1, 2):
+                s = dd * 9 + (dx + 1) + 3 * (dy + 1)
+                j = where[dd, fx + 1 + dx, fy + 1 + dy]
+                ok = j >= 0
+                rows.append(idx[ok])
+                cols.append(j[ok])
+                vals.append(slot_coef[cls[ok], s])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
 class NearField:
     """P = (M + M^T) / 2, the symmetrized near-field SAI (see the module docstring)."""
 
-    def __init__(self, generators, grid, mask_x=None, mask_y=None):
-        m = sai_matrix(generators, grid, mask_x, mask_y)
+    def __init__(self, generators, grid, mask_x=None, mask_y=None, by_class=False):
+        build = sai_matrix_by_class if by_class else sai_matrix
+        m = build(generators, grid, mask_x, mask_y)
         self.matrix = ((m + m.T) * 0.5).tocsr()
 
     def apply(self, r):

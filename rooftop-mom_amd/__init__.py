@@ -173,6 +173,7 @@ class Product:
         self._problem = problem
         self.handle = handle
         self.precision = precision
+        self._count = problem.rooftopCount()
 
     def __del__(self):
         if getattr(self, 'handle', None) is not None and _abi._lib is not None:
@@ -184,7 +185,7 @@ class Product:
 
     def apply(self, x, precision=None):
         """Product.apply (Product.m:25-35); raises rooftopMoM:product:sizeMismatch on a wrong size."""
-        n = self._problem.rooftopCount()
+        n = self._count
         a = np.asarray(x)
         if not np.issubdtype(a.dtype, np.number) or a.ndim not in (1, 2) or a.shape[0] != n:
             raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:product:sizeMismatch',
@@ -202,7 +203,7 @@ class Product:
         prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
                                                          else self.precision)
         check(library().rt_product_apply_device(self.handle, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
-                                                self._problem.rooftopCount(), nrhs, prec))
+                                                self._count, nrhs, prec))
 
     def memory(self):
         s = C.c_uint64()

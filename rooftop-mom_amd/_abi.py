@@ -34,6 +34,8 @@ SIGNATURES = [
     ('rt_ctx_memory', C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ('rt_ctx_synchronize', C.c_int, [P]),
     ('rt_ctx_stream', C.c_int, [P, C.POINTER(P)]),
+    ('rt_ctx_set_timing', C.c_int, [P, C.c_int32]),
+    ('rt_ctx_kernel_timing', C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ('rt_problem_create', C.c_int, [P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32,
                                     C.c_int32, P, C.POINTER(P)]),
     ('rt_problem_destroy', None, [P]),
@@ -46,6 +48,8 @@ SIGNATURES = [
     ('rt_product_apply', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
     ('rt_product_apply_device', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
     ('rt_product_memory', C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ('rt_product_profile', C.c_int, [P, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     ('rt_solve', C.c_int, [P, P, P, C.POINTER(rt_solve_opts), P, C.c_int64, C.c_int64, P, P,
                            C.POINTER(rt_output)]),
     ('rt_solve_device', C.c_int, [P, P, P, C.POINTER(rt_solve_opts), P, C.c_int64, C.c_int64, P, P,

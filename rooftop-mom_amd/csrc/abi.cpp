@@ -36,10 +36,12 @@ using namespace rt;
 struct rt_ctx {
     Context c;
 };
+// A product keeps its problem alive (shared ownership), so handles may be destroyed in any order.
 struct rt_problem {
-    std::unique_ptr<Problem> p;
+    std::shared_ptr<Problem> p;
 };
 struct rt_product {
+    std::shared_ptr<Problem> owner;   // declared first: destroyed after the product
     std::unique_ptr<Product> p;
 };
 
@@ -175,6 +177,9 @@ void rt_ctx_destroy(rt_ctx *ctx) {
         (void)hipStreamDestroy(ctx->c.stream);
     }
     for (auto &kv : ctx->c.sizes) (void)hipFree(kv.first);
+    for (auto &r : ctx->c.pending)
+        for (hipEvent_t e : r.ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->c.event_pool) (void)hipEventDestroy(e);
     delete ctx;
 }
 
@@ -198,6 +203,33 @@ int rt_ctx_stream(rt_ctx *ctx, void **stream) {
     return guarded([&] {
         need(ctx && stream, RT_EINVAL, "rooftop:ctx", "ctx or stream is NULL");
         *stream = reinterpret_cast<void *>(ctx->c.stream);
+    });
+}
+
+int rt_ctx_set_timing(rt_ctx *ctx, int32_t on) {
+    return guarded([&] {
+        need(ctx != nullptr, RT_EINVAL, "rooftop:ctx", "ctx is NULL");
+        ctx->c.timing = on != 0;
+    });
+}
+
+int rt_ctx_kernel_timing(rt_ctx *ctx, int64_t *launches, double *ms, double *bytes) {
+    return guarded([&] {
+        need(ctx && launches && ms && bytes, RT_EINVAL, "rooftop:ctx", "NULL argument");
+        ctx->c.activate();
+        RT_HIP(hipStreamSynchronize(ctx->c.stream));
+        ctx->c.drain_timing();
+        for (int p = 0; p < 2; ++p) {
+            launches[p] = ctx->c.launches[p];
+            for (int k = 0; k < 3; ++k) {
+                const double n = ctx->c.launches[p] ? static_cast<double>(ctx->c.launches[p]) : 1.0;
+                ms[3 * p + k] = ctx->c.pass_ms[p][k] / n;
+                bytes[3 * p + k] = ctx->c.pass_bytes[p][k] / n;
+                ctx->c.pass_ms[p][k] = 0;
+                ctx->c.pass_bytes[p][k] = 0;
+            }
+            ctx->c.launches[p] = 0;
+        }
     });
 }
 
@@ -237,7 +269,7 @@ int rt_problem_create(rt_ctx *ctx, int32_t cx, int32_t cy, double a, double b, d
             RT_HIP(hipMemcpy(p->exists.ptr, ex.data(), ex.size(), hipMemcpyHostToDevice));
         }
         auto h = std::make_unique<rt_problem>();
-        h->p = std::move(p);
+        h->p = std::shared_ptr<Problem>(std::move(p));
         *out = h.release();
     });
 }
@@ -300,6 +332,7 @@ int rt_product_create(rt_problem *p, int32_t kind, int32_t precision, int32_t mo
              "The modifier is none, real or imag.");
         p->p->ctx->activate();
         auto h = std::make_unique<rt_product>();
+        h->owner = p->p;
         h->p = create_operator(p->p.get(), precision, modifier);
         *out = h.release();
     });
@@ -314,6 +347,7 @@ int rt_precond_create(rt_problem *p, int32_t kind, int32_t precision, rt_product
              "rooftop:product:invalidPrecision", "The precision is RT_C128, RT_C64 or RT_MIXED.");
         p->p->ctx->activate();
         auto h = std::make_unique<rt_product>();
+        h->owner = p->p;
         h->p = create_preconditioner(p->p.get(), kind, precision);
         *out = h.release();
     });
@@ -359,6 +393,16 @@ int rt_product_memory(rt_product *p, uint64_t *stored, uint64_t *working) {
         need(p != nullptr, RT_EINVAL, "rooftop:product", "NULL product");
         if (stored) *stored = p->p->stored_bytes();
         if (working) *working = p->p->working_bytes();
+    });
+}
+
+int rt_product_profile(rt_product *p, int64_t nrhs, int32_t reps, int32_t precision, double *ms, double *bytes) {
+    return guarded([&] {
+        need(p && ms && bytes && reps >= 1 && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product",
+             "invalid profile arguments");
+        Context *c = p->p->problem->ctx;
+        c->activate();
+        operator_profile(*p->p, static_cast<int>(nrhs), reps, precision, ms, bytes, c->stream);
     });
 }
 

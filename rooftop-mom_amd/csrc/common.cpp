@@ -79,4 +79,30 @@ void Context::release(void *ptr) {
     (void)hipFree(ptr);
 }
 
+hipEvent_t Context::take_event() {
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    RT_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void Context::drain_timing() {
+    for (PassRecord &r : pending) {
+        RT_HIP(hipEventSynchronize(r.ev[3]));
+        for (int k = 0; k < 3; ++k) {
+            float t = 0;
+            RT_HIP(hipEventElapsedTime(&t, r.ev[k], r.ev[k + 1]));
+            pass_ms[r.precision][k] += t;
+            pass_bytes[r.precision][k] += r.bytes[k];
+        }
+        launches[r.precision] += 1;
+        for (hipEvent_t e : r.ev) event_pool.push_back(e);
+    }
+    pending.clear();
+}
+
 }  // namespace rt

@@ -9,6 +9,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "rooftop.h"
 
@@ -82,6 +83,21 @@ struct Context {
     uint64_t held = 0;
     std::mutex lock;
     std::unordered_map<void *, uint64_t> sizes;
+
+    // Kernel timing of the operator passes (rt_ctx_set_timing): events recorded around each pass, drained later.
+    struct PassRecord {
+        hipEvent_t ev[4];
+        int precision;        // 0 FP64, 1 complex64
+        double bytes[3];
+    };
+    bool timing = false;
+    std::vector<PassRecord> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t launches[2] = {0, 0};
+    double pass_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double pass_bytes[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    hipEvent_t take_event();
+    void drain_timing();
 
     // Allocates device memory after checking the ceiling like Budget.require (Budget.m:82-106).
     void *alloc(uint64_t bytes, const char *what);

@@ -299,30 +299,57 @@ static void set_lds(const void *kernel, size_t bytes) {
     }
 }
 
+// Algorithmic HBM bytes of one launch of each pass: every vector element, work element and spectrum value once.
+template <typename TC>
+static void pass_bytes(const ProdDims &d, int nrhs, double *bytes) {
+    const double sc = sizeof(cplx<TC>);
+    const double vec = 16.0 * d.n * nrhs;
+    const double work_bytes = sc * (static_cast<double>(d.cols_x) + d.cols_y) * d.Mx * nrhs;
+    bytes[0] = vec + work_bytes;
+    bytes[1] = 2 * work_bytes + 3.0 * sc * d.Hx * d.Hy;
+    bytes[2] = work_bytes + vec;
+}
+
 template <typename TC>
 static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
                        const cplx<TC> *planes, hipStream_t s) {
     const ProdDims &d = op.dims;
+    Context *ctx = op.problem->ctx;
     const uint8_t *ex = op.problem->exists.ptr;
     const int total_cols = nrhs * (d.cols_x + d.cols_y);
     const int cpb = sh.cols_per_block;
     const size_t lds_ac = static_cast<size_t>(cpb) * d.Mx * sizeof(cplx<TC>);
     const unsigned blocks_ac = (total_cols + cpb - 1) / cpb;
+    const size_t lds_b = static_cast<size_t>(2) * sh.tile * (d.My + 1) * sizeof(cplx<TC>);
     set_lds<TC>(reinterpret_cast<const void *>(&pass_a<TC, double>), lds_ac);
     set_lds<TC>(reinterpret_cast<const void *>(&pass_c<TC, double>), lds_ac);
+    set_lds<TC>(reinterpret_cast<const void *>(&pass_b<TC>), lds_b);
+    Context::PassRecord rec{};
+    const bool timed = ctx->timing;
+    if (timed) {
+        for (auto &e : rec.ev) e = ctx->take_event();
+        rec.precision = sizeof(TC) == sizeof(float) ? 1 : 0;
+        pass_bytes<TC>(d, nrhs, rec.bytes);
+        RT_HIP(hipEventRecord(rec.ev[0], s));
+    }
     hipLaunchKernelGGL((pass_a<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s,
                        reinterpret_cast<const cplx<double> *>(in), work, ex, d, op.tx.plan, tw_of<TC>(op.tx), cpb,
                        total_cols);
     RT_LAUNCH_CHECK();
-    const size_t lds_b = static_cast<size_t>(2) * sh.tile * (d.My + 1) * sizeof(cplx<TC>);
-    set_lds<TC>(reinterpret_cast<const void *>(&pass_b<TC>), lds_b);
+    if (timed) RT_HIP(hipEventRecord(rec.ev[1], s));
     hipLaunchKernelGGL((pass_b<TC>), dim3(d.Mx / sh.tile, nrhs), dim3(sh.threads_b), lds_b, s, work, planes,
                        op.tx.fold.ptr, op.ty.fold.ptr, d_of<TC>(op.tx), d_of<TC>(op.ty), d, op.ty.plan,
                        tw_of<TC>(op.ty), sh.tile);
     RT_LAUNCH_CHECK();
+    if (timed) RT_HIP(hipEventRecord(rec.ev[2], s));
     hipLaunchKernelGGL((pass_c<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s, work,
                        reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), cpb, total_cols);
     RT_LAUNCH_CHECK();
+    if (timed) {
+        RT_HIP(hipEventRecord(rec.ev[3], s));
+        ctx->pending.push_back(rec);
+        if (ctx->pending.size() > 4096) ctx->drain_timing();
+    }
 }
 
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s) {
@@ -418,6 +445,59 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
     }
     RT_HIP(hipStreamSynchronize(s));
     return op;
+}
+
+// Times `reps` applications with the context's kernel timing (the first application warms up and is not counted).
+template <typename TC>
+static void profile_passes(Product &op, int nrhs, int reps, cplx<TC> *work, const cplx<TC> *planes, double *ms,
+                           double *bytes, hipStream_t s) {
+    const ProdDims &d = op.dims;
+    Context *ctx = op.problem->ctx;
+    DeviceArray<c128> in, out;
+    in.allocate(ctx, static_cast<size_t>(d.n) * nrhs, "profile input");
+    out.allocate(ctx, static_cast<size_t>(d.n) * nrhs, "profile output");
+    in.zero(s);
+    const PassShape sh = default_shape<TC>(d, nrhs);
+    run_passes<TC>(op, in.ptr, out.ptr, work, nrhs, sh, planes, s);
+    RT_HIP(hipStreamSynchronize(s));
+    const bool was = ctx->timing;
+    ctx->drain_timing();
+    const int p = sizeof(TC) == sizeof(float) ? 1 : 0;
+    const int64_t n0 = ctx->launches[p];
+    double ms0[3], by0[3];
+    for (int k = 0; k < 3; ++k) { ms0[k] = ctx->pass_ms[p][k]; by0[k] = ctx->pass_bytes[p][k]; }
+    ctx->timing = true;
+    for (int it = 0; it < reps; ++it) run_passes<TC>(op, in.ptr, out.ptr, work, nrhs, sh, planes, s);
+    ctx->drain_timing();
+    ctx->timing = was;
+    const double cnt = static_cast<double>(ctx->launches[p] - n0);
+    for (int k = 0; k < 3; ++k) {
+        ms[k] = (ctx->pass_ms[p][k] - ms0[k]) / cnt;
+        bytes[k] = (ctx->pass_bytes[p][k] - by0[k]) / cnt;
+        ctx->pass_ms[p][k] = ms0[k];
+        ctx->pass_bytes[p][k] = by0[k];
+    }
+    ctx->launches[p] = n0;
+}
+
+void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s) {
+    if (op.type != kOperator) fail(RT_EINVAL, "rooftop:product", "only an operator has matvec passes to profile");
+    if (prec == RT_MIXED) prec = RT_C128;
+    if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
+    const size_t per_rhs = static_cast<size_t>(2) * op.dims.ny * op.dims.Mx;
+    if (prec == RT_C128) {
+        if (op.work_rhs64 < nrhs) {
+            op.work64.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
+            op.work_rhs64 = nrhs;
+        }
+        profile_passes<double>(op, nrhs, reps, op.work64.ptr, op.planes64.ptr, ms, bytes, s);
+    } else {
+        if (op.work_rhs32 < nrhs) {
+            op.work32.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
+            op.work_rhs32 = nrhs;
+        }
+        profile_passes<float>(op, nrhs, reps, op.work32.ptr, op.planes32.ptr, ms, bytes, s);
+    }
 }
 
 uint64_t Product::stored_bytes() const {

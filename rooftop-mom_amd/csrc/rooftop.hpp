@@ -98,6 +98,7 @@ struct Product {
 // product.hip
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier);
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s);
+void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s);
 // nearfield.hip
 std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precision);
 void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s);

@@ -130,3 +130,16 @@ def test_apply_checks_the_input(rt):
             op.apply(bad)
         assert e.value.identifier == 'rooftopMoM:product:sizeMismatch'
     assert op.memory() > 0 and op.workingMemory() > 16 * 31
+
+
+def test_handles_may_be_destroyed_in_any_order(rt):
+    """A product keeps its problem alive: destroying the problem handle first leaves the product usable."""
+    p = _problem(rt, 6, 5, 0.1, 0.1, (3, 3))
+    op = rt.Operator(p)
+    x = _random(p.rooftopCount(), 9)
+    before = op.apply(x)
+    rt.library().rt_problem_destroy(p.handle)
+    p.handle = None
+    assert rel(op.apply(x), before) == 0
+    rt.library().rt_product_destroy(op.handle)
+    op.handle = None

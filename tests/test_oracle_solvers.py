@@ -152,3 +152,16 @@ def test_near_field_preconditioner_cuts_iterations(method):
     plain = SV.solve(method, op, b, tol=1e-6, max_iter=500, restart=50)
     pre = SV.solve(method, op, b, tol=1e-6, max_iter=500, restart=50, preconditioner=NF.NearField(g, grid).apply)
     assert pre['converged'] and pre['iterationCount'] < plain['iterationCount'] / 2
+
+
+def test_near_field_by_class_equals_per_rooftop_fit():
+    """Rooftops with the same existing neighbourhood share their SAI row (Z depends on offsets only)."""
+    grid = A.Grid(11, 9, 0.1, 0.12)
+    g = A.generate(grid, 2 * np.pi, (4, 4))
+    mask = np.ones((11, 9), bool)
+    mask[4:7, 3:5] = False
+    mx, my = mask[:-1] & mask[1:], mask[:, :-1] & mask[:, 1:]
+    for args in ((), (mx, my)):
+        a = NF.sai_matrix(g, grid, *args).toarray()
+        b = NF.sai_matrix_by_class(g, grid, *args).toarray()
+        assert np.abs(a - b).max() < 1e-11 * np.abs(a).max()
