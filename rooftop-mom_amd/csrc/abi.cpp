@@ -120,6 +120,8 @@ SolveResult solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_s
     need(!pc || (pc->p && pc->p->dims.n == n), RT_ESIZE, "rooftopMoM:solver:sizeMismatch",
          "The preconditioner does not match the unknowns of the right-hand side.");
     need(nrhs >= 1, RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide", "At least one right-hand side is needed.");
+    need(op->p->problem->ctx == &ctx->c && (!pc || pc->p->problem->ctx == &ctx->c), RT_EINVAL,
+         "rooftop:solver:contextMismatch", "The operator and the preconditioner must belong to the solving context.");
     SolveOptions o = options_of(opts);
     need(op->p->has(o.precision), RT_EINVAL, "rooftop:solver:invalidPrecision",
          "The operator does not hold the inner precision (create it with RT_MIXED).");
@@ -180,6 +182,7 @@ void rt_ctx_destroy(rt_ctx *ctx) {
     for (auto &r : ctx->c.pending)
         for (hipEvent_t e : r.ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->c.event_pool) (void)hipEventDestroy(e);
+    if (ctx->c.host_pinned) (void)hipHostFree(ctx->c.host_pinned);
     delete ctx;
 }
 

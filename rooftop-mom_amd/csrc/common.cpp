@@ -79,6 +79,22 @@ void Context::release(void *ptr) {
     (void)hipFree(ptr);
 }
 
+void *Context::scratch(int slot, uint64_t bytes, const char *what) {
+    if (static_cast<int>(pool.size()) <= slot) pool.resize(slot + 1, {nullptr, 0});
+    auto &e = pool[slot];
+    if (e.second < bytes) {
+        if (e.first) {
+            RT_HIP(hipStreamSynchronize(stream));
+            release(e.first);
+        }
+        e.first = nullptr;
+        e.second = 0;
+        e.first = alloc(bytes, what);
+        e.second = bytes;
+    }
+    return e.first;
+}
+
 hipEvent_t Context::take_event() {
     if (!event_pool.empty()) {
         hipEvent_t e = event_pool.back();

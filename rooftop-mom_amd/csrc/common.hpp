@@ -90,6 +90,9 @@ struct Context {
         int precision;        // 0 FP64, 1 complex64
         double bytes[3];
     };
+    // Set by a Krylov loop while it runs: the kernels of the products return at once when *skip != 0 (the loop has
+    // converged on the device and the host has not polled yet).
+    const int *skip = nullptr;
     bool timing = false;
     std::vector<PassRecord> pending;
     std::vector<hipEvent_t> event_pool;
@@ -98,6 +101,11 @@ struct Context {
     double pass_bytes[2][3] = {{0, 0, 0}, {0, 0, 0}};
     hipEvent_t take_event();
     void drain_timing();
+
+    // Solver buffers kept between solves (grown on demand, freed with the context): slot -> (pointer, bytes).
+    std::vector<std::pair<void *, uint64_t>> pool;
+    void *host_pinned = nullptr;   // 4 KiB pinned mirror of the solver's device scalars
+    void *scratch(int slot, uint64_t bytes, const char *what);
 
     // Allocates device memory after checking the ceiling like Budget.require (Budget.m:82-106).
     void *alloc(uint64_t bytes, const char *what);

@@ -64,7 +64,9 @@ template <> const c64 *d_of<float>(const DimTables &t) { return t.d32.ptr; }
 template <typename TC, typename TIO>
 __global__ void __launch_bounds__(256) pass_a(const cplx<TIO> *__restrict__ in, cplx<TC> *__restrict__ work,
                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
-                                              const cplx<TC> *__restrict__ tw, int cols_per_block, int total_cols) {
+                                              const cplx<TC> *__restrict__ tw, int cols_per_block, int total_cols,
+                                              const int *__restrict__ skip) {
+    if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int M = d.Mx;
@@ -106,7 +108,9 @@ __global__ void __launch_bounds__(256) pass_a(const cplx<TIO> *__restrict__ in, 
 template <typename TC, typename TIO>
 __global__ void __launch_bounds__(256) pass_c(const cplx<TC> *__restrict__ work, cplx<TIO> *__restrict__ out,
                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
-                                              const cplx<TC> *__restrict__ tw, int cols_per_block, int total_cols) {
+                                              const cplx<TC> *__restrict__ tw, int cols_per_block, int total_cols,
+                                              const int *__restrict__ skip) {
+    if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int M = d.Mx;
@@ -151,7 +155,9 @@ template <typename TC>
 __global__ void __launch_bounds__(256) pass_b(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ planes,
                                               const int *__restrict__ fold_x, const int *__restrict__ fold_y,
                                               const cplx<TC> *__restrict__ dx, const cplx<TC> *__restrict__ dy,
-                                              ProdDims d, FftPlan plan, const cplx<TC> *__restrict__ tw, int tile) {
+                                              ProdDims d, FftPlan plan, const cplx<TC> *__restrict__ tw, int tile,
+                                              const int *__restrict__ skip) {
+    if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int My = d.My;
@@ -269,12 +275,13 @@ static PassShape default_shape(const ProdDims &d, int nrhs) {
     s.cols_per_block = cpb;
     int thr = cpb * d.Mx / 4;
     s.threads_ac = thr < 64 ? 64 : thr > 256 ? 256 : (thr + 63) / 64 * 64;
-    int tile = lds_limit_bytes() / (2 * (d.My + 1) * es);
-    if (tile < 1) tile = 1;
-    if (tile > 16) tile = 16;
-    while (d.Mx % tile) tile >>= 1;
-    // Keep at least ~512 blocks in flight when the grid allows it.
-    while (tile > 1 && (d.Mx / tile) * nrhs < 512) tile >>= 1;
+    // Pass B tile: a power of two (it divides Mx), at most 16 positions, within the LDS budget; at least 8 positions
+    // (64-byte row segments) when the LDS allows, then halved while fewer than ~256 blocks would run.
+    int lim_b = (80 * 1024) / (2 * (d.My + 1) * es);
+    int tile = 1;
+    while (tile * 2 <= lim_b && tile * 2 <= 16 && tile * 2 <= d.Mx) tile *= 2;
+    const int floor_tile = 64 / es > 1 ? 64 / es : 1;
+    while (tile > floor_tile && (d.Mx / tile) * nrhs < 256) tile >>= 1;
     s.tile = tile;
     int tb = 2 * tile * d.My / 4;
     s.threads_b = tb < 64 ? 64 : tb > 256 ? 256 : (tb + 63) / 64 * 64;
@@ -312,7 +319,7 @@ static void pass_bytes(const ProdDims &d, int nrhs, double *bytes) {
 
 template <typename TC>
 static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
-                       const cplx<TC> *planes, hipStream_t s) {
+                       const cplx<TC> *planes, hipStream_t s, const int *skip = nullptr) {
     const ProdDims &d = op.dims;
     Context *ctx = op.problem->ctx;
     const uint8_t *ex = op.problem->exists.ptr;
@@ -334,16 +341,17 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     }
     hipLaunchKernelGGL((pass_a<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s,
                        reinterpret_cast<const cplx<double> *>(in), work, ex, d, op.tx.plan, tw_of<TC>(op.tx), cpb,
-                       total_cols);
+                       total_cols, skip);
     RT_LAUNCH_CHECK();
     if (timed) RT_HIP(hipEventRecord(rec.ev[1], s));
     hipLaunchKernelGGL((pass_b<TC>), dim3(d.Mx / sh.tile, nrhs), dim3(sh.threads_b), lds_b, s, work, planes,
                        op.tx.fold.ptr, op.ty.fold.ptr, d_of<TC>(op.tx), d_of<TC>(op.ty), d, op.ty.plan,
-                       tw_of<TC>(op.ty), sh.tile);
+                       tw_of<TC>(op.ty), sh.tile, skip);
     RT_LAUNCH_CHECK();
     if (timed) RT_HIP(hipEventRecord(rec.ev[2], s));
     hipLaunchKernelGGL((pass_c<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s, work,
-                       reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), cpb, total_cols);
+                       reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), cpb, total_cols,
+                       skip);
     RT_LAUNCH_CHECK();
     if (timed) {
         RT_HIP(hipEventRecord(rec.ev[3], s));
@@ -352,7 +360,7 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     }
 }
 
-void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s) {
+void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip) {
     if (prec == RT_MIXED) prec = RT_C128;
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
     const ProdDims &d = op.dims;
@@ -364,14 +372,14 @@ void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, 
             op.shape64 = default_shape<double>(d, nrhs);
         }
         PassShape sh = default_shape<double>(d, nrhs);
-        run_passes<double>(op, in, out, op.work64.ptr, nrhs, sh, op.planes64.ptr, s);
+        run_passes<double>(op, in, out, op.work64.ptr, nrhs, sh, op.planes64.ptr, s, skip);
     } else {
         if (op.work_rhs32 < nrhs) {
             op.work32.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
             op.work_rhs32 = nrhs;
         }
         PassShape sh = default_shape<float>(d, nrhs);
-        run_passes<float>(op, in, out, op.work32.ptr, nrhs, sh, op.planes32.ptr, s);
+        run_passes<float>(op, in, out, op.work32.ptr, nrhs, sh, op.planes32.ptr, s, skip);
     }
 }
 
@@ -511,11 +519,11 @@ uint64_t Product::working_bytes() const {
     return per * (has(RT_C128) ? sizeof(c128) : sizeof(c64));
 }
 
-void Product::apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s) {
+void Product::apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip) {
     if (type == kOperator)
-        operator_apply(*this, in, out, nrhs, prec, s);
+        operator_apply(*this, in, out, nrhs, prec, s, skip);
     else
-        preconditioner_apply(*this, in, out, nrhs, prec, s);
+        preconditioner_apply(*this, in, out, nrhs, prec, s, skip);
 }
 
 }  // namespace rt

@@ -138,3 +138,21 @@ def test_solver_errors(rt):
     assert zero.converged() and zero.iterationCount() == 0 and np.all(zero.solution() == 0)
     limited = rt.Solver('gmres', tolerance=1e-14, maxIterationCount=7, restart=5).solve(op, np.ones(grid.count))
     assert not limited.converged() and limited.iterationCount() == 7
+
+
+@pytest.mark.parametrize('method', ['cg', 'gmres'])
+def test_initial_guess(rt, method):
+    """IterativeMethodTest.m:216-222: an exact initial guess needs no iteration; a rough one still converges."""
+    p, grid, gen = _setup(rt, 10, 9, 0.1, 0.1, (4, 4))
+    Z = S.dense_matrix(gen, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [0, 1, 0], 1.0, order=4)
+    exact = np.linalg.solve(Z, b)
+    op = rt.Operator(p)
+    pc = rt.Preconditioner(p, 'nearField')
+    res = rt.Solver(method, tolerance=1e-10).solve(op, b, preconditioner=pc, initialGuess=exact)
+    assert res.converged() and res.iterationCount() == 0 and rel(res.solution(), exact) < 1e-12
+    rough = exact * (1 + 0.01 * np.random.default_rng(5).standard_normal(grid.count))
+    res = rt.Solver(method, tolerance=1e-10).solve(op, b, preconditioner=pc, initialGuess=rough)
+    cold = rt.Solver(method, tolerance=1e-10).solve(op, b, preconditioner=pc)
+    assert res.converged() and rel(res.solution(), exact) < 1e-8
+    assert res.iterationCount() <= cold.iterationCount()
