@@ -66,8 +66,8 @@ typedef struct rt_product rt_product;
  *   refine          0: the reference's loop as is (the true residual is reported, not enforced);
  *                   1: an FP64 outer loop on the true residual norm(b - A x)/norm(b) until it is below tolerance
  *                      (iterative refinement; required with precision RT_C64).
- *   inner_tolerance relative tolerance of each inner solve when refine = 1 (0 picks 1e-5 for RT_C64, the tolerance
- *                   itself for RT_C128).
+ *   inner_tolerance relative tolerance of each inner solve when refine = 1 (0 picks the tolerance itself, at least
+ *                   1e-6 for complex64 GMRES and 1e-4 for complex64 COCG).
  *   max_outer       limit of outer refinement steps (0 picks 20). */
 typedef struct rt_solve_opts {
     int32_t method;

@@ -57,98 +57,98 @@ template <typename T> const cplx<T> *d_of(const DimTables &t);
 template <> const c128 *d_of<double>(const DimTables &t) { return t.d.ptr; }
 template <> const c64 *d_of<float>(const DimTables &t) { return t.d32.ptr; }
 
+// Work array of the passes, per right-hand side: [component][x position (digit-reversed)][y column], the y columns
+// of one x position contiguous (row length ldj >= the columns of either component). Pass A writes it transposed in
+// 64-byte row segments, pass B reads and writes whole contiguous columns, pass C reads it back in row segments.
+__host__ __device__ inline int64_t work_index(const ProdDims &d, int r, int comp, int kx, int j) {
+    return ((static_cast<int64_t>(r) * 2 + comp) * d.Mx + kx) * d.ldj + j;
+}
+
+// Decodes a pass-A/C block into (right-hand side, component, first y column).
+__device__ inline void group_of(const ProdDims &d, int group, int blk, int &r, int &comp, int &j0) {
+    const int gx = (d.cols_x + group - 1) / group, gy = (d.cols_y + group - 1) / group;
+    const int per = gx + gy;
+    r = blk / per;
+    const int rem = blk - r * per;
+    comp = rem < gx ? 0 : 1;
+    j0 = (comp ? rem - gx : rem) * group;
+}
+
 // ------------------------------------------------------------------------------------------------------------------
-// Pass A: forward FFT along x of every rooftop column, zero-padded from its rows to Mx (one LDS column per column).
-// Columns are enumerated per right-hand side as [x columns 0..cols_x) then [y columns 0..cols_y).
+// Pass A: forward FFT along x of `group` consecutive rooftop columns of one component (zero-padded from their rows
+// to Mx), written transposed into the work array.
 // ------------------------------------------------------------------------------------------------------------------
 template <typename TC, typename TIO>
 __global__ void __launch_bounds__(256) pass_a(const cplx<TIO> *__restrict__ in, cplx<TC> *__restrict__ work,
                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
-                                              const cplx<TC> *__restrict__ tw, int cols_per_block, int total_cols,
+                                              const cplx<TC> *__restrict__ tw, int group,
                                               const int *__restrict__ skip) {
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int M = d.Mx;
-    const int stride = M;
-    const int first = blockIdx.x * cols_per_block;
-    const int ncols = min(cols_per_block, total_cols - first);
-    const int per_rhs = d.cols_x + d.cols_y;
+    const int stride = M + 1;
+    int r, comp, j0;
+    group_of(d, group, blockIdx.x, r, comp, j0);
+    const int cols = comp ? d.cols_y : d.cols_x;
+    const int rows = comp ? d.rows_y : d.rows_x;
+    const int ncols = min(group, cols - j0);
+    const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
     for (int e = threadIdx.x; e < ncols * M; e += blockDim.x) {
         const int c = e / M, i = e - c * M;
-        const int col = first + c;
-        const int r = col / per_rhs;
-        const int rem = col - r * per_rhs;
-        const int comp = rem < d.cols_x ? 0 : 1;
-        const int j = comp ? rem - d.cols_x : rem;
-        const int rows = comp ? d.rows_y : d.rows_x;
         cplx<TC> v = mk<TC>(0, 0);
         if (i < rows) {
-            const int64_t idx = (comp ? d.nx_count : 0) + static_cast<int64_t>(j) * rows + i;
-            if (!exists || exists[idx]) v = cvt<TC>(in[static_cast<int64_t>(r) * d.n + idx]);
+            const int64_t idx = static_cast<int64_t>(j0 + c) * rows + i;
+            if (!exists || exists[(comp ? d.nx_count : 0) + idx]) v = cvt<TC>(in[off + idx]);
         }
         buf[c * stride + i] = v;
     }
     __syncthreads();
     lds_fft<TC, false>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
     for (int e = threadIdx.x; e < ncols * M; e += blockDim.x) {
-        const int c = e / M, i = e - c * M;
-        const int col = first + c;
-        const int r = col / per_rhs;
-        const int rem = col - r * per_rhs;
-        const int comp = rem < d.cols_x ? 0 : 1;
-        const int j = comp ? rem - d.cols_x : rem;
-        work[((static_cast<int64_t>(r) * 2 + comp) * d.ny + j) * M + i] = buf[c * stride + i];
+        const int kx = e / ncols, c = e - kx * ncols;
+        work[work_index(d, r, comp, kx, j0 + c)] = buf[c * stride + kx];
     }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Pass C: inverse FFT along x, keep the first rows of each column (the cut of FFTApplicator.m:80-84).
+// Pass C: inverse FFT along x of `group` consecutive columns read back from the work array; the first rows of each
+// column are the product (the cut of FFTApplicator.m:80-84).
 // ------------------------------------------------------------------------------------------------------------------
 template <typename TC, typename TIO>
 __global__ void __launch_bounds__(256) pass_c(const cplx<TC> *__restrict__ work, cplx<TIO> *__restrict__ out,
                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
-                                              const cplx<TC> *__restrict__ tw, int cols_per_block, int total_cols,
+                                              const cplx<TC> *__restrict__ tw, int group,
                                               const int *__restrict__ skip) {
     if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int M = d.Mx;
-    const int stride = M;
-    const int first = blockIdx.x * cols_per_block;
-    const int ncols = min(cols_per_block, total_cols - first);
-    const int per_rhs = d.cols_x + d.cols_y;
+    const int stride = M + 1;
+    int r, comp, j0;
+    group_of(d, group, blockIdx.x, r, comp, j0);
+    const int cols = comp ? d.cols_y : d.cols_x;
+    const int rows = comp ? d.rows_y : d.rows_x;
+    const int ncols = min(group, cols - j0);
     for (int e = threadIdx.x; e < ncols * M; e += blockDim.x) {
-        const int c = e / M, i = e - c * M;
-        const int col = first + c;
-        const int r = col / per_rhs;
-        const int rem = col - r * per_rhs;
-        const int comp = rem < d.cols_x ? 0 : 1;
-        const int j = comp ? rem - d.cols_x : rem;
-        buf[c * stride + i] = work[((static_cast<int64_t>(r) * 2 + comp) * d.ny + j) * M + i];
+        const int kx = e / ncols, c = e - kx * ncols;
+        buf[c * stride + kx] = work[work_index(d, r, comp, kx, j0 + c)];
     }
     __syncthreads();
     lds_fft<TC, true>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
-    for (int e = threadIdx.x; e < ncols * M; e += blockDim.x) {
-        const int c = e / M, i = e - c * M;
-        const int col = first + c;
-        const int r = col / per_rhs;
-        const int rem = col - r * per_rhs;
-        const int comp = rem < d.cols_x ? 0 : 1;
-        const int j = comp ? rem - d.cols_x : rem;
-        const int rows = comp ? d.rows_y : d.rows_x;
-        if (i < rows) {
-            const int64_t idx = (comp ? d.nx_count : 0) + static_cast<int64_t>(j) * rows + i;
-            cplx<TC> v = buf[c * stride + i];
-            if (exists && !exists[idx]) v = mk<TC>(0, 0);
-            out[static_cast<int64_t>(r) * d.n + idx] = cvt<TIO>(v);
-        }
+    const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
+    for (int e = threadIdx.x; e < ncols * rows; e += blockDim.x) {
+        const int c = e / rows, i = e - c * rows;
+        const int64_t idx = static_cast<int64_t>(j0 + c) * rows + i;
+        cplx<TC> v = buf[c * stride + i];
+        if (exists && !exists[(comp ? d.nx_count : 0) + idx]) v = mk<TC>(0, 0);
+        out[off + idx] = cvt<TIO>(v);
     }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Pass B: for a tile of `tile` consecutive x positions, forward FFT along y of both components (zero-padded from
-// their columns to My), the three-plane spectral multiply, inverse FFT along y, and the cut back to the columns.
+// Pass B: for `tile` consecutive x positions, forward FFT along y of both components (contiguous columns,
+// zero-padded to My), the three-plane spectral multiply, inverse FFT along y, and the cut back to the columns.
 // LDS holds [component][tile][My (+1 pad)].
 // ------------------------------------------------------------------------------------------------------------------
 template <typename TC>
@@ -164,19 +164,14 @@ __global__ void __launch_bounds__(256) pass_b(cplx<TC> *__restrict__ work, const
     const int stride = My + 1;
     const int px0 = blockIdx.x * tile;
     const int r = blockIdx.y;
-    const int ncomp = 2;
-    const int ncols = ncomp * tile;
-    cplx<TC> *base = work + static_cast<int64_t>(r) * 2 * d.ny * d.Mx;
-    // Load (tile positions contiguous in memory per row) and zero the padding rows.
+    const int ncols = 2 * tile;
     for (int e = threadIdx.x; e < ncols * My; e += blockDim.x) {
-        const int t = e % tile;
-        const int rest = e / tile;
-        const int j = rest % My;
-        const int comp = rest / My;
+        const int col = e / My, j = e - col * My;
+        const int comp = col / tile, t = col - comp * tile;
         const int cols = comp ? d.cols_y : d.cols_x;
         cplx<TC> v = mk<TC>(0, 0);
-        if (j < cols) v = base[(static_cast<int64_t>(comp) * d.ny + j) * d.Mx + px0 + t];
-        buf[(comp * tile + t) * stride + j] = v;
+        if (j < cols) v = work[work_index(d, r, comp, px0 + t, j)];
+        buf[col * stride + j] = v;
     }
     __syncthreads();
     lds_fft<TC, false>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
@@ -196,12 +191,10 @@ __global__ void __launch_bounds__(256) pass_b(cplx<TC> *__restrict__ work, const
     __syncthreads();
     lds_fft<TC, true>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
     for (int e = threadIdx.x; e < ncols * My; e += blockDim.x) {
-        const int t = e % tile;
-        const int rest = e / tile;
-        const int j = rest % My;
-        const int comp = rest / My;
+        const int col = e / My, j = e - col * My;
+        const int comp = col / tile, t = col - comp * tile;
         const int cols = comp ? d.cols_y : d.cols_x;
-        if (j < cols) base[(static_cast<int64_t>(comp) * d.ny + j) * d.Mx + px0 + t] = buf[(comp * tile + t) * stride + j];
+        if (j < cols) work[work_index(d, r, comp, px0 + t, j)] = buf[col * stride + j];
     }
 }
 
@@ -259,31 +252,29 @@ __global__ void to_c64(const c128 *__restrict__ a, c64 *__restrict__ b, int64_t 
 // ------------------------------------------------------------------------------------------------------------------
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
-static int lds_limit_bytes() { return 64 * 1024; }
-
+// Default launch shape (the autotuner may replace it). Pass A/C: `group` y columns per block, 64-byte row segments
+// when enough blocks remain (>= 256), fewer columns otherwise. Pass B: one x position per block unless the grid
+// would be very large.
 template <typename TC>
 static PassShape default_shape(const ProdDims &d, int nrhs) {
     PassShape s;
     const int es = sizeof(cplx<TC>);
-    const int total_cols = nrhs * (d.cols_x + d.cols_y);
-    int lim = lds_limit_bytes() / (d.Mx * es);
-    if (lim < 1) lim = 1;
-    int cpb = total_cols / 1024;
-    if (cpb < 1) cpb = 1;
-    if (cpb > lim) cpb = lim;
-    if (cpb > 16) cpb = 16;
-    s.cols_per_block = cpb;
-    int thr = cpb * d.Mx / 4;
+    int group = 64 / es;
+    const int lds_cols = static_cast<int>((64 * 1024) / ((d.Mx + 1) * es));
+    auto blocks_ac = [&](int g) {
+        return static_cast<int64_t>(nrhs) * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g);
+    };
+    while (group > 1 && (group > lds_cols || blocks_ac(group) < 256)) group >>= 1;
+    s.cols_per_block = group;
+    const int thr = group * d.Mx / 4;
     s.threads_ac = thr < 64 ? 64 : thr > 256 ? 256 : (thr + 63) / 64 * 64;
-    // Pass B tile: a power of two (it divides Mx), at most 16 positions, within the LDS budget; at least 8 positions
-    // (64-byte row segments) when the LDS allows, then halved while fewer than ~256 blocks would run.
-    int lim_b = (80 * 1024) / (2 * (d.My + 1) * es);
     int tile = 1;
-    while (tile * 2 <= lim_b && tile * 2 <= 16 && tile * 2 <= d.Mx) tile *= 2;
-    const int floor_tile = 64 / es > 1 ? 64 / es : 1;
-    while (tile > floor_tile && (d.Mx / tile) * nrhs < 256) tile >>= 1;
+    const int lds_tile = static_cast<int>((80 * 1024) / (2 * (d.My + 1) * es));
+    while (tile * 2 <= 16 && tile * 2 <= lds_tile && tile * 2 <= d.Mx &&
+           static_cast<int64_t>(d.Mx / (tile * 2)) * nrhs >= 8192)
+        tile *= 2;
     s.tile = tile;
-    int tb = 2 * tile * d.My / 4;
+    const int tb = 2 * tile * d.My / 4;
     s.threads_b = tb < 64 ? 64 : tb > 256 ? 256 : (tb + 63) / 64 * 64;
     return s;
 }
@@ -323,10 +314,9 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     const ProdDims &d = op.dims;
     Context *ctx = op.problem->ctx;
     const uint8_t *ex = op.problem->exists.ptr;
-    const int total_cols = nrhs * (d.cols_x + d.cols_y);
-    const int cpb = sh.cols_per_block;
-    const size_t lds_ac = static_cast<size_t>(cpb) * d.Mx * sizeof(cplx<TC>);
-    const unsigned blocks_ac = (total_cols + cpb - 1) / cpb;
+    const int g = sh.cols_per_block;
+    const unsigned blocks_ac = static_cast<unsigned>(nrhs * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g));
+    const size_t lds_ac = static_cast<size_t>(g) * (d.Mx + 1) * sizeof(cplx<TC>);
     const size_t lds_b = static_cast<size_t>(2) * sh.tile * (d.My + 1) * sizeof(cplx<TC>);
     set_lds<TC>(reinterpret_cast<const void *>(&pass_a<TC, double>), lds_ac);
     set_lds<TC>(reinterpret_cast<const void *>(&pass_c<TC, double>), lds_ac);
@@ -340,8 +330,7 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
         RT_HIP(hipEventRecord(rec.ev[0], s));
     }
     hipLaunchKernelGGL((pass_a<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s,
-                       reinterpret_cast<const cplx<double> *>(in), work, ex, d, op.tx.plan, tw_of<TC>(op.tx), cpb,
-                       total_cols, skip);
+                       reinterpret_cast<const cplx<double> *>(in), work, ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip);
     RT_LAUNCH_CHECK();
     if (timed) RT_HIP(hipEventRecord(rec.ev[1], s));
     hipLaunchKernelGGL((pass_b<TC>), dim3(d.Mx / sh.tile, nrhs), dim3(sh.threads_b), lds_b, s, work, planes,
@@ -350,8 +339,7 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     RT_LAUNCH_CHECK();
     if (timed) RT_HIP(hipEventRecord(rec.ev[2], s));
     hipLaunchKernelGGL((pass_c<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s, work,
-                       reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), cpb, total_cols,
-                       skip);
+                       reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip);
     RT_LAUNCH_CHECK();
     if (timed) {
         RT_HIP(hipEventRecord(rec.ev[3], s));
@@ -364,7 +352,7 @@ void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, 
     if (prec == RT_MIXED) prec = RT_C128;
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
     const ProdDims &d = op.dims;
-    const size_t per_rhs = static_cast<size_t>(2) * d.ny * d.Mx;
+    const size_t per_rhs = static_cast<size_t>(2) * d.ldj * d.Mx;
     if (prec == RT_C128) {
         if (op.work_rhs64 < nrhs) {
             op.work64.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
@@ -398,6 +386,7 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
     d.cols_x = g.has_x ? g.cols_x : 0;
     d.cols_y = g.has_y ? g.cols_y : 0;
     d.ny = d.cols_x > d.cols_y ? d.cols_x : d.cols_y;
+    d.ldj = (d.ny + 7) / 8 * 8;
     d.nx_count = g.nx;
     d.n = g.n;
     d.Mx = g.cx > 1 ? next_pow2(2L * g.cx - 1) : 1;
@@ -492,7 +481,7 @@ void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, dou
     if (op.type != kOperator) fail(RT_EINVAL, "rooftop:product", "only an operator has matvec passes to profile");
     if (prec == RT_MIXED) prec = RT_C128;
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
-    const size_t per_rhs = static_cast<size_t>(2) * op.dims.ny * op.dims.Mx;
+    const size_t per_rhs = static_cast<size_t>(2) * op.dims.ldj * op.dims.Mx;
     if (prec == RT_C128) {
         if (op.work_rhs64 < nrhs) {
             op.work64.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
@@ -515,7 +504,7 @@ uint64_t Product::stored_bytes() const {
 
 uint64_t Product::working_bytes() const {
     if (type != kOperator) return 0;
-    const uint64_t per = static_cast<uint64_t>(2) * dims.ny * dims.Mx;
+    const uint64_t per = static_cast<uint64_t>(2) * dims.ldj * dims.Mx;
     return per * (has(RT_C128) ? sizeof(c128) : sizeof(c64));
 }
 

@@ -26,6 +26,7 @@ struct ProdDims {
     int rows_x, rows_y;       // rooftop rows (x extent) of each component
     int cols_x, cols_y;       // rooftop columns (y extent), 0 for a missing component
     int ny;                   // columns of the work array per component (max of cols_x, cols_y)
+    int ldj;                  // row length of the work array (ny rounded up to 8)
     int64_t nx_count;         // number of x-directed rooftops (offset of the y part of a vector)
     int64_t n;                // rooftops per right-hand side
     int Mx, My;               // embedding periods
@@ -59,9 +60,9 @@ struct DimTables {
 
 // Launch shape of the three matvec passes (picked by the autotuner or the default heuristic).
 struct PassShape {
-    int cols_per_block = 1;   // pass A / C
+    int cols_per_block = 1;   // pass A / C: y columns per block
     int threads_ac = 256;
-    int tile = 4;             // pass B: positions along x per block
+    int tile = 1;             // pass B: x positions per block
     int threads_b = 256;
 };
 
@@ -77,7 +78,7 @@ struct Product {
     DimTables tx, ty;
     DeviceArray<c128> planes64;      // [3][Hx][Hy]: magneticXX, magneticYY, electric, scaled by 1/(Mx My)
     DeviceArray<c64> planes32;
-    DeviceArray<c128> work64;        // [nrhs][2][ny][Mx]
+    DeviceArray<c128> work64;        // [nrhs][2][Mx][ldj]
     DeviceArray<c64> work32;
     int work_rhs64 = 0, work_rhs32 = 0;
     PassShape shape64, shape32;
