@@ -13,7 +13,9 @@ roofline of the dominant matvec kernel (pass B: y transforms + fused spectral mu
 on the library's stream over the timed solves, and a CPU baseline: the oracle's FP64 split-FFT GMRES on a bounded
 sample of the same solve, timed on this host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+A third form solves the same plate under 64 plane waves together (batched mixed and batched FP64 solves/s).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--batch 64] [--no-cpu]
 """
 
 import argparse
@@ -58,6 +60,41 @@ def run_form(rt, ctx, op, pc, solver, d_b, d_x, n, steps, warmup):
     return times, iters, outers, residuals
 
 
+def plane_wave_batch(rt, problem, count, seed=0):
+    """count plane waves with (theta, phi) uniform in [0, 80] x [0, 360) degrees and TE or TM polarization drawn from
+    the seed; incidence from above the plate (z > 0)."""
+    rng = np.random.default_rng(seed)
+    cols = []
+    for _ in range(count):
+        th, ph = np.radians(rng.uniform(0, 80)), np.radians(rng.uniform(0, 360))
+        d = -np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+        te = np.array([-np.sin(ph), np.cos(ph), 0.0])
+        tm = np.array([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)])
+        cols.append(rt.Excitation.planeWave(problem, d, te if rng.random() < 0.5 else tm).vector())
+    return np.asfortranarray(np.stack(cols, axis=1), dtype=np.complex128)
+
+
+def run_batch(rt, ctx, op, pc, solver, d_b, d_x, n, count, steps, warmup):
+    """Solves the count right-hand sides in d_b together, steps + warmup times; returns per-batch seconds and the
+    per-system outputs of the last batch."""
+    lib = rt.library()
+    o = solver.options()
+    outs = (rt._abi.rt_output * count)()
+    times = []
+    for s in range(warmup + steps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        rt.check(lib.rt_solve_device(ctx.handle, op.handle, pc.handle, C.byref(o), C.c_void_p(d_b), n, count, None,
+                                     C.c_void_p(d_x), outs))
+        ctx.synchronize()
+        if s >= warmup:
+            times.append(time.perf_counter() - t0)
+        bad = [r for r in range(count) if not outs[r].converged]
+        if bad:
+            raise RuntimeError('batched solve: %d systems did not converge' % len(bad))
+    return times, [(o_.iteration_count, o_.outer_count, o_.residual_norm) for o_ in outs]
+
+
 def cpu_baseline(n, budget_s=10.0):
     """The oracle's FP64 path (scipy FFTs on the host cores of this GPU's share, split-FFT product, sparse near-field
     apply, GMRES(50) with CGS2 and the same tolerance) on the same plate: whole solves repeated until about budget_s
@@ -97,6 +134,7 @@ def main():
     ap.add_argument('--steps', type=int, default=30)
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--n', type=int, default=256)
+    ap.add_argument('--batch', type=int, default=64, help='right-hand sides of the batched form (0: skip it)')
     ap.add_argument('--no-cpu', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -160,6 +198,32 @@ def main():
                         'GBps': round(total_bytes / (total_ms * 1e-3) / 1e9, 1),
                         'pass_ms': [round(v, 5) for v in prof_ms]}
 
+    # Batched form: the same plate under `batch` plane waves, solved together (one product launch per pass for all).
+    batch = {}
+    if args.batch > 0:
+        R = args.batch
+        B = plane_wave_batch(rt, problem, R, seed=1)
+        d_B, d_X = C.c_void_p(), C.c_void_p()
+        for ptr in (d_B, d_X):
+            rt.check(lib.rt_device_alloc(ctx.handle, 16 * N * R, C.byref(ptr)))
+        rt.check(lib.rt_copy_to_device(ctx.handle, d_B, B.ctypes.data_as(C.c_void_p), 16 * N * R))
+        bsteps, bwarm = max(1, args.steps // 10), 1
+        for name, solver in (('mixed', mixed), ('fp64', fp64)):
+            print('bench: batched %s form, %d right-hand sides' % (name, R), flush=True)
+            bt, outs = run_batch(rt, ctx, op, pc, solver, d_B.value, d_X.value, N, R, bsteps, bwarm)
+            its = [q[0] for q in outs]
+            batch[name] = {'solves_per_s': round(R * bsteps / sum(bt), 3),
+                           'batch_p50_ms': round(1e3 * percentile(bt, 50), 3),
+                           'batch_p99_ms': round(1e3 * percentile(bt, 99), 3),
+                           'iterations_median': int(np.median(its)), 'iterations_max': int(max(its)),
+                           'residual_max': max(q[2] for q in outs), 'steps': bsteps}
+        batch['rhs'] = R
+        batch['excitations'] = ('plane waves, theta U[0,80] deg, phi U[0,360) deg, TE/TM at random (seed 1), '
+                                'solved together')
+        batch['mixed_vs_fp64_speedup'] = round(batch['mixed']['solves_per_s'] / batch['fp64']['solves_per_s'], 3)
+        for ptr in (d_B, d_X):
+            rt.check(lib.rt_device_free(ctx.handle, ptr))
+
     steps = args.steps
     value = steps / sum(tmx)
     line = {
@@ -183,6 +247,7 @@ def main():
                      'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
                      'launches': int(launches[1]), 'ms_per_launch': round(b_ms, 5), 'bytes_per_launch': b_bytes},
         'matvec': matvec,
+        'batched': batch,
         'setup_s': round(setup_s, 3),
     }
     if not args.no_cpu:

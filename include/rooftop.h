@@ -156,8 +156,10 @@ int rt_ctx_set_timing(rt_ctx *ctx, int32_t on);
 int rt_ctx_kernel_timing(rt_ctx *ctx, int64_t *launches, double *ms, double *bytes);
 
 /* rooftopMoM.Solver(...).solve(operator, rhs, 'preconditioner', P, 'initialGuess', x0) (Solver.m:42-92,
- * IterativeMethod.m:27-70). precond may be NULL; x0 may be NULL (zeros). nrhs right-hand sides are solved one
- * after another into x (n x nrhs). out (may be NULL) reports the last right-hand side, or the worst for nrhs > 1. */
+ * IterativeMethod.m:27-70). precond may be NULL; x0 may be NULL (zeros). The nrhs (1..1024) right-hand sides are
+ * solved together into x (n x nrhs): one Krylov loop per right-hand side, all advanced in lockstep so that every
+ * product and preconditioner launch serves all of them; each stops at its own iteration with the result it would
+ * have alone. out (may be NULL) points to nrhs records, one per right-hand side. */
 int rt_solve(rt_ctx *ctx, rt_product *op, rt_product *precond, const rt_solve_opts *opts, const void *rhs,
              int64_t n, int64_t nrhs, const void *x0, void *x, rt_output *out);
 /* The same on device buffers; synchronous (the loop polls its scalars). */

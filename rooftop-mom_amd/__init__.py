@@ -448,7 +448,11 @@ class Solver:
         return o
 
     def solve(self, operator, rightHandSide, preconditioner=None, initialGuess=None):
-        """Solves operator * x = rightHandSide on the GPU (Solver.m:42-92)."""
+        """Solves operator * x = rightHandSide on the GPU (Solver.m:42-92).
+
+        A matrix right-hand side (N x R) solves its R columns together, in lockstep, and returns one Result per column,
+        each equal to the Result of solving that column alone.
+        """
         if not isinstance(operator, Product):
             raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidOperator',
                                'An operator is a rooftop_mom_amd product on the GPU.')
@@ -475,13 +479,16 @@ class Solver:
                                    'The initial guess must have %d elements.' % n)
         x = np.empty_like(b)
         cap = self.maxIterationCount * 2 + 64
-        norms = np.zeros(cap)
-        out = _abi.rt_output()
-        out.residual_norms_capacity = cap
-        out.residual_norms = norms.ctypes.data_as(C.POINTER(C.c_double))
+        norms = np.zeros((nrhs, cap))
+        outs = (_abi.rt_output * nrhs)()
+        for r in range(nrhs):
+            outs[r].residual_norms_capacity = cap
+            outs[r].residual_norms = norms[r].ctypes.data_as(C.POINTER(C.c_double))
         o = self.options()
         check(library().rt_solve(operator.problem().context.handle, operator.handle,
                                  preconditioner.handle if preconditioner is not None else None, C.byref(o),
-                                 _ptr(b), n, nrhs, _ptr(x0), _ptr(x), C.byref(out)))
-        return Result(x, bool(out.converged), out.iteration_count, norms[:min(cap, out.residual_norms_count)].copy(),
-                      out.residual_norm, out.time_s, excitation, out.outer_count)
+                                 _ptr(b), n, nrhs, _ptr(x0), _ptr(x), outs))
+        results = [Result(x if b.ndim == 1 else x[:, r].copy(), bool(out.converged), out.iteration_count,
+                          norms[r, :min(cap, out.residual_norms_count)].copy(), out.residual_norm, out.time_s,
+                          excitation, out.outer_count) for r, out in enumerate(outs)]
+        return results[0] if b.ndim == 1 else results

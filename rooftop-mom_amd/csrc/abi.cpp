@@ -108,9 +108,9 @@ void fill_output(const SolveResult &r, rt_output *out) {
     }
 }
 
-// Solves nrhs systems one after the other on device buffers.
-SolveResult solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_opts *opts, const c128 *rhs,
-                         int64_t n, int64_t nrhs, const c128 *x0, c128 *x) {
+// Solves nrhs systems in lockstep on device buffers (one batched Krylov loop); one result per system.
+std::vector<SolveResult> solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_opts *opts,
+                                      const c128 *rhs, int64_t n, int64_t nrhs, const c128 *x0, c128 *x) {
     need(ctx && op && op->p, RT_EINVAL, "rooftopMoM:solver:invalidOperator",
          "An operator is a product created by rt_product_create.");
     need(op->p->type == kOperator || op->p->type == kJacobi || op->p->type == kNearField, RT_EINVAL,
@@ -119,7 +119,8 @@ SolveResult solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_s
          "The operator does not match the unknowns of the right-hand side.");
     need(!pc || (pc->p && pc->p->dims.n == n), RT_ESIZE, "rooftopMoM:solver:sizeMismatch",
          "The preconditioner does not match the unknowns of the right-hand side.");
-    need(nrhs >= 1, RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide", "At least one right-hand side is needed.");
+    need(nrhs >= 1 && nrhs <= kMaxBatch, RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide",
+         "Between 1 and 1024 right-hand sides are solved at once.");
     need(op->p->problem->ctx == &ctx->c && (!pc || pc->p->problem->ctx == &ctx->c), RT_EINVAL,
          "rooftop:solver:contextMismatch", "The operator and the preconditioner must belong to the solving context.");
     SolveOptions o = options_of(opts);
@@ -128,19 +129,12 @@ SolveResult solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_s
     need(!pc || pc->p->has(o.precision), RT_EINVAL, "rooftop:solver:invalidPrecision",
          "The preconditioner does not hold the inner precision (create it with RT_MIXED).");
     ctx->c.activate();
-    SolveResult last;
-    for (int64_t r = 0; r < nrhs; ++r) {
-        SolveResult res = solve_one(&ctx->c, *op->p, pc ? pc->p.get() : nullptr, o, rhs + r * n,
-                                    x0 ? x0 + r * n : nullptr, x + r * n);
-        if (r == 0 || res.residual > last.residual || !res.converged) {
-            double t = last.time + res.time;
-            last = res;
-            last.time = t;
-        } else {
-            last.time += res.time;
-        }
-    }
-    return last;
+    return solve_batch(&ctx->c, *op->p, pc ? pc->p.get() : nullptr, o, rhs, x0, x, static_cast<int>(nrhs));
+}
+
+void fill_outputs(const std::vector<SolveResult> &r, rt_output *out) {
+    if (!out) return;
+    for (size_t k = 0; k < r.size(); ++k) fill_output(r[k], out + k);
 }
 
 void check_finite(const c128 *v, int64_t count) {
@@ -413,9 +407,9 @@ int rt_solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_
                     int64_t n, int64_t nrhs, const void *x0, void *x, rt_output *out) {
     return guarded([&] {
         need(rhs && x, RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide", "NULL right-hand side or solution");
-        SolveResult r = solve_device(ctx, op, pc, opts, static_cast<const c128 *>(rhs), n, nrhs,
-                                     static_cast<const c128 *>(x0), static_cast<c128 *>(x));
-        fill_output(r, out);
+        const auto r = solve_device(ctx, op, pc, opts, static_cast<const c128 *>(rhs), n, nrhs,
+                                    static_cast<const c128 *>(x0), static_cast<c128 *>(x));
+        fill_outputs(r, out);
     });
 }
 
@@ -436,10 +430,10 @@ int rt_solve(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_opts *o
             dx0.allocate(c, count, "the initial guesses");
             RT_HIP(hipMemcpyAsync(dx0.ptr, x0, dx0.bytes(), hipMemcpyHostToDevice, c->stream));
         }
-        SolveResult r = solve_device(ctx, op, pc, opts, db.ptr, n, nrhs, x0 ? dx0.ptr : nullptr, dx.ptr);
+        const auto r = solve_device(ctx, op, pc, opts, db.ptr, n, nrhs, x0 ? dx0.ptr : nullptr, dx.ptr);
         RT_HIP(hipMemcpyAsync(x, dx.ptr, dx.bytes(), hipMemcpyDeviceToHost, c->stream));
         RT_HIP(hipStreamSynchronize(c->stream));
-        fill_output(r, out);
+        fill_outputs(r, out);
     });
 }
 

@@ -104,7 +104,8 @@ struct Context {
 
     // Solver buffers kept between solves (grown on demand, freed with the context): slot -> (pointer, bytes).
     std::vector<std::pair<void *, uint64_t>> pool;
-    void *host_pinned = nullptr;   // 4 KiB pinned mirror of the solver's device scalars
+    void *host_pinned = nullptr;   // pinned staging of the solver's device scalars and loop states
+    size_t host_pinned_bytes = 0;
     void *scratch(int slot, uint64_t bytes, const char *what);
 
     // Allocates device memory after checking the ceiling like Budget.require (Budget.m:82-106).

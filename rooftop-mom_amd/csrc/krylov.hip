@@ -1,22 +1,27 @@
-// The preconditioned Krylov loops on gfx950: COCG and restarted GMRES(m) with CGS2, FP64 vectors on the device,
-// and the FP64 iterative-refinement loop around an inner solve whose products run in complex64.
+// The preconditioned Krylov loops on gfx950: COCG and restarted GMRES(m) with CGS2 on R right-hand sides at once, FP64
+// vectors on the device, and the FP64 iterative-refinement loop around an inner solve whose products run in
+// complex64.
 //
 // Reference: src/+rooftopMoM/+solver/IterativeMethod.m:27-70, ConjugateGradient.m:32-59,
 // PreconditionedConjugateGradient.m:11-23, @GeneralizedMinimumResidual/iterate_.m:1-85, orthogonalize_.m:15-23,
-// generateRotation_.m:11-18.
+// generateRotation_.m:11-18. The reference solves one right-hand side per call; here R systems with the same operator
+// advance in lockstep, every kernel covering all of them (blockIdx.y = system), and every product applied to the R
+// vectors in one launch per pass, so the spectra and the near-field coefficients are read once per step for all R.
 //
 // Everything a step needs stays on the device: the scalars (rho, d^T A d, the Hessenberg column), the Givens rotations,
-// the residual history and the stopping test, written by one-block "finalize" kernels. The host enqueues a window of
-// steps and polls the loop state once per window; when a finalize kernel stops the loop (convergence, breakdown, the
-// iteration limit) it raises a flag that makes every later kernel of the window -- products, preconditioner, vector
-// kernels -- return at once, so the result is exactly that of stopping at that step. Reductions are deterministic:
-// fixed row blocks, wave64 shuffles, and a fixed-order second stage.
+// the residual histories and the stopping tests, written by one-block-per-system "finalize" kernels. The host enqueues
+// a window of steps and polls the R loop states once per window. A system that stops (convergence, breakdown, the
+// iteration limit) raises its flag, and every later kernel of that system -- products, preconditioner, vector kernels
+// -- returns at once, so each system's result is exactly that of stopping at its own step. Reductions are
+// deterministic: fixed row blocks, wave64 shuffles and a fixed-order second stage.
 //
 // CGS2 (orthogonalize_.m:15-23) runs as three row-partitioned passes over the live columns only (the reference takes
 // the whole basis with zeroed coefficients): h1 = V^H w; w -= V h1 fused with h2 = V^H w; w -= V h2 fused with |w|^2.
+// The bases are stored column by column across the systems, V[k][r][n], so column k of every system is one batch of
+// R vectors for the product.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <complex>
 #include <vector>
 
 #include "krylov.hpp"
@@ -24,14 +29,22 @@
 namespace rt {
 
 constexpr int kThreads = 256;
-constexpr int kMaxBlocks = 2048;
 constexpr int kLdp = 64;        // partial columns per block
-constexpr int kPoll = 8;        // steps enqueued between two polls of the loop state
+constexpr int kPoll = 8;        // steps enqueued between two polls of the loop states
+
+// Scalar slots of a system.
+enum ScalSlot {
+    kRho0 = 0, kRho1 = 1, kDad = 2, kRnormCg = 3,   // COCG
+    kH2 = 0, kH1 = 64,                              // GMRES: second- and first-pass coefficients
+    kRnorm = 128, kInvSub = 130, kReference = 132, kTol = 134, kBnorm = 136,
+    kY = 256,                                       // GMRES: solution coefficients of a cycle
+    kOuterB = 400, kOuterRes = 401
+};
 
 // Pool slots of the context's solver buffers.
-enum Slot {
+enum PoolSlot {
     kSlotPartial, kSlotScal, kSlotState, kSlotHistory, kSlotV, kSlotW, kSlotT, kSlotR, kSlotZ, kSlotD, kSlotAd,
-    kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD
+    kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags
 };
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -55,282 +68,301 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     return r;
 }
 
-#define SKIP_IF_DONE() \
-    if (skip && *skip) return
+// The system of this block; returns from the kernel when that system is stopped (uniform over the block).
+#define SYSTEM(r)             \
+    const int r = blockIdx.y; \
+    if (st && st[r].done) return
 
-// partial[b * kLdp] = sum over block b's rows of x^T y (or x^H y).
+// partial[(r * blocks + b) * kLdp] = sum over block b's rows of system r of x^T y (or x^H y).
 template <bool CONJ>
 __global__ void __launch_bounds__(kThreads) k_dot(const c128 *__restrict__ x, const c128 *__restrict__ y, int64_t n,
                                                   int64_t chunk, c128 *__restrict__ partial,
-                                                  const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                  const LoopState *__restrict__ st) {
+    SYSTEM(r);
     __shared__ double sh[8];
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    const c128 *xr = x + r * n, *yr = y + r * n;
     double re = 0, im = 0;
     for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
-        const c128 p = CONJ ? cmulc(x[i], y[i]) : x[i] * y[i];
+        const c128 p = CONJ ? cmulc(xr[i], yr[i]) : xr[i] * yr[i];
         re += p.x;
         im += p.y;
     }
     re = block_sum(re, sh);
     im = block_sum(im, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x * kLdp] = mk(re, im);
+    if (threadIdx.x == 0) partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp] = mk(re, im);
 }
 
 __global__ void __launch_bounds__(kThreads) k_nrm2(const c128 *__restrict__ x, int64_t n, int64_t chunk,
-                                                   c128 *__restrict__ partial, const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                   c128 *__restrict__ partial, const LoopState *__restrict__ st) {
+    SYSTEM(r);
     __shared__ double sh[8];
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    const c128 *xr = x + r * n;
     double s = 0;
-    for (int64_t i = base + threadIdx.x; i < end; i += kThreads) s += abs2(x[i]);
+    for (int64_t i = base + threadIdx.x; i < end; i += kThreads) s += abs2(xr[i]);
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x * kLdp] = mk(s, 0.0);
+    if (threadIdx.x == 0) partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp] = mk(s, 0.0);
 }
 
-// out[k] = sum_b partial[b * kLdp + k], one block per column k; sqrt_re: the square root of the real part (a norm).
+// scal[r][slot + k] = sum_b partial[(r * blocks + b) * kLdp + k], one block per (k, r); sqrt_re: a norm.
 __global__ void __launch_bounds__(kThreads) k_reduce(const c128 *__restrict__ partial, int blocks,
-                                                     c128 *__restrict__ out, int sqrt_re,
-                                                     const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                     c128 *__restrict__ scal, int slot, int sqrt_re,
+                                                     const LoopState *__restrict__ st) {
+    SYSTEM(r);
     __shared__ double sh[8];
     const int k = blockIdx.x;
     double re = 0, im = 0;
     for (int b = threadIdx.x; b < blocks; b += kThreads) {
-        const c128 v = partial[b * kLdp + k];
+        const c128 v = partial[(static_cast<int64_t>(r) * blocks + b) * kLdp + k];
         re += v.x;
         im += v.y;
     }
     re = block_sum(re, sh);
     im = block_sum(im, sh);
-    if (threadIdx.x == 0) out[k] = sqrt_re ? mk(sqrt(re), 0.0) : mk(re, im);
+    if (threadIdx.x == 0) scal[r * kScal + slot + k] = sqrt_re ? mk(sqrt(re), 0.0) : mk(re, im);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Vector kernels.
+// Vector kernels (grid.y = system).
 // ------------------------------------------------------------------------------------------------------------------
-#define GRID_STRIDE(i, n)                                                                  \
+#define ROWS(i, n)                                                                         \
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < (n); \
          i += static_cast<int64_t>(gridDim.x) * blockDim.x)
 
-__global__ void k_copy(const c128 *__restrict__ x, c128 *__restrict__ y, int64_t n, const int *__restrict__ skip) {
-    SKIP_IF_DONE();
-    GRID_STRIDE(i, n) y[i] = x[i];
+__global__ void k_copy(const c128 *__restrict__ x, c128 *__restrict__ y, int64_t n, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int64_t o = r * n;
+    ROWS(i, n) y[o + i] = x[o + i];
 }
-__global__ void k_scale(const c128 *__restrict__ x, c128 a, c128 *__restrict__ y, int64_t n,
-                        const int *__restrict__ skip) {
-    SKIP_IF_DONE();
-    GRID_STRIDE(i, n) y[i] = a * x[i];
+__global__ void k_sub(const c128 *__restrict__ b, const c128 *__restrict__ ax, c128 *__restrict__ res, int64_t n,
+                      const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int64_t o = r * n;
+    ROWS(i, n) res[o + i] = b[o + i] - ax[o + i];
 }
-// y = a * x with a read from the device.
-__global__ void k_scale_dev(const c128 *__restrict__ x, const c128 *__restrict__ a, c128 *__restrict__ y, int64_t n,
-                            const int *__restrict__ skip) {
-    SKIP_IF_DONE();
-    const c128 s = *a;
-    GRID_STRIDE(i, n) y[i] = s * x[i];
+__global__ void k_add(const c128 *__restrict__ x, c128 *__restrict__ y, int64_t n, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int64_t o = r * n;
+    ROWS(i, n) y[o + i] = y[o + i] + x[o + i];
 }
-__global__ void k_sub(const c128 *__restrict__ b, const c128 *__restrict__ ax, c128 *__restrict__ r, int64_t n,
-                      const int *__restrict__ skip) {
-    SKIP_IF_DONE();
-    GRID_STRIDE(i, n) r[i] = b[i] - ax[i];
+// Per-system state reset: stopped = finished[r] (systems left out of this loop), everything else zero.
+__global__ void k_reset_state(LoopState *st, const int *__restrict__ finished, int R) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int f = finished[r];
+    st[r] = LoopState{f, 0, f, 0, 0, {0, 0, 0}};
 }
-__global__ void k_axpy(c128 a, const c128 *__restrict__ x, c128 *__restrict__ y, int64_t n,
-                       const int *__restrict__ skip) {
-    SKIP_IF_DONE();
-    GRID_STRIDE(i, n) y[i] = y[i] + a * x[i];
+// Marks the systems whose starting residual is already below their tolerance as converged at iteration 0.
+__global__ void k_initial_check(LoopState *st, const c128 *__restrict__ scal, int num_slot, int den_slot, int R) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R || st[r].done) return;
+    const double res = scal[r * kScal + num_slot].x / scal[r * kScal + den_slot].x;
+    if (res < scal[r * kScal + kTol].x) st[r].done = st[r].finished = st[r].converged = 1;
 }
-__global__ void k_zero_state(LoopState *st) {
-    if (threadIdx.x == 0) *st = LoopState{0, 0, 0, 0, {0, 0, 0, 0}};
+__global__ void k_copy_slot(c128 *scal, int from, int to, int R) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) scal[r * kScal + to] = scal[r * kScal + from];
 }
 
 // ------------------------------------------------------------------------------------------------------------------
 // COCG kernels.
 // ------------------------------------------------------------------------------------------------------------------
 // d = z + (rho_new / rho_old) d.
-__global__ void k_cocg_direction(const c128 *__restrict__ z, c128 *__restrict__ d, const c128 *__restrict__ sc,
-                                 int i_new, int i_old, int64_t n, const int *__restrict__ skip) {
-    SKIP_IF_DONE();
-    const c128 rn = sc[i_new], ro = sc[i_old];
+__global__ void k_cocg_direction(const c128 *__restrict__ z, c128 *__restrict__ d, const c128 *__restrict__ scal,
+                                 int i_new, int i_old, int64_t n, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const c128 rn = scal[r * kScal + i_new], ro = scal[r * kScal + i_old];
     const c128 beta = (1.0 / abs2(ro)) * cmulc(ro, rn);
-    GRID_STRIDE(i, n) d[i] = z[i] + beta * d[i];
+    const int64_t o = r * n;
+    ROWS(i, n) d[o + i] = z[o + i] + beta * d[o + i];
 }
 
 // alpha = rho / (d^T A d); x += alpha d; r -= alpha Ad; per-block partial of |r|^2.
-__global__ void __launch_bounds__(kThreads) k_cocg_update(c128 *__restrict__ x, c128 *__restrict__ r,
+__global__ void __launch_bounds__(kThreads) k_cocg_update(c128 *__restrict__ x, c128 *__restrict__ res,
                                                           const c128 *__restrict__ d, const c128 *__restrict__ ad,
-                                                          const c128 *__restrict__ sc, int i_rho, int i_dad,
-                                                          int64_t n, int64_t chunk, c128 *__restrict__ partial,
-                                                          const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                          const c128 *__restrict__ scal, int i_rho, int64_t n,
+                                                          int64_t chunk, c128 *__restrict__ partial,
+                                                          const LoopState *__restrict__ st) {
+    SYSTEM(r);
     __shared__ double sh[8];
-    const c128 rho = sc[i_rho], dad = sc[i_dad];
+    const c128 rho = scal[r * kScal + i_rho], dad = scal[r * kScal + kDad];
     const c128 alpha = (1.0 / abs2(dad)) * cmulc(dad, rho);
-    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk), o = r * n;
     double s = 0;
     for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
-        x[i] = x[i] + alpha * d[i];
-        const c128 v = r[i] - alpha * ad[i];
-        r[i] = v;
+        x[o + i] = x[o + i] + alpha * d[o + i];
+        const c128 v = res[o + i] - alpha * ad[o + i];
+        res[o + i] = v;
         s += abs2(v);
     }
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x * kLdp] = mk(s, 0.0);
+    if (threadIdx.x == 0) partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp] = mk(s, 0.0);
 }
 
-// The residual norm of iteration `it`, the stopping test (ConjugateGradient.m:126-136).
+// The residual norm of iteration `it`, the stopping test (ConjugateGradient.m:126-136). Grid (1, R).
 __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restrict__ partial, int blocks,
-                                                            double bnorm, double tol, int it, int max_iter,
-                                                            LoopState *st, double *history,
-                                                            const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                            const c128 *__restrict__ scal, int it, int max_iter,
+                                                            LoopState *st, double *history, int cap) {
+    SYSTEM(r);
     __shared__ double sh[8];
     double s = 0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[b * kLdp].x;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[(static_cast<int64_t>(r) * blocks + b) * kLdp].x;
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) {
-        const double res = sqrt(s) / bnorm;
-        history[it] = res;
-        st->iterations = it;
-        if (res < tol) {
-            st->converged = 1;
-            st->done = 1;
-        } else if (!isfinite(res) || it >= max_iter) {
-            st->done = 1;
-        }
+    if (threadIdx.x != 0) return;
+    const double res = sqrt(s) / scal[r * kScal + kBnorm].x;
+    history[static_cast<int64_t>(r) * cap + it] = res;
+    st[r].iterations = it;
+    if (res < scal[r * kScal + kTol].x) {
+        st[r].converged = st[r].finished = st[r].done = 1;
+    } else if (!isfinite(res)) {
+        st[r].finished = st[r].done = 1;
+    } else if (it >= max_iter) {
+        st[r].done = 1;
     }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// GMRES kernels.
+// GMRES kernels. Column k of the basis of system r is V + (k * R + r) * n (R = gridDim.y).
 // ------------------------------------------------------------------------------------------------------------------
-// partial[b * kLdp + k] = sum over block b's rows of conj(V[:, k]) w, k < cnt <= 64. Wave q takes the columns
-// q, q + 4, ...; its lanes run over the block's rows (coalesced column reads), w is read once per wave.
-__global__ void __launch_bounds__(kThreads) k_cgs_dot(const c128 *__restrict__ V, int64_t ldv, int cnt,
+__device__ __forceinline__ const c128 *basis_col(const c128 *V, int k, int r, int R, int64_t n) {
+    return V + (static_cast<int64_t>(k) * R + r) * n;
+}
+
+// partial[(r * blocks + b) * kLdp + k] = sum over block b's rows of conj(V_r[:, k]) w_r, k < cnt <= 64. Wave q takes
+// the columns q, q + 4, ...; its lanes run over the block's rows (coalesced column reads).
+__global__ void __launch_bounds__(kThreads) k_cgs_dot(const c128 *__restrict__ V, int cnt,
                                                       const c128 *__restrict__ w, int64_t n, int64_t chunk,
-                                                      c128 *__restrict__ partial, const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                      c128 *__restrict__ partial, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int R = gridDim.y;
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const c128 *wr = w + r * n;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
     double re[16], im[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) re[c] = im[c] = 0;
-    for (int64_t r = base + lane; r < end; r += 64) {
-        const c128 wr = w[r];
+    for (int64_t i = base + lane; i < end; i += 64) {
+        const c128 wi = wr[i];
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
             const int k = q + 4 * c;
             if (k < cnt) {
-                const c128 v = V[k * ldv + r];
-                re[c] += v.x * wr.x + v.y * wr.y;
-                im[c] += v.x * wr.y - v.y * wr.x;
+                const c128 v = basis_col(V, k, r, R, n)[i];
+                re[c] += v.x * wi.x + v.y * wi.y;
+                im[c] += v.x * wi.y - v.y * wi.x;
             }
         }
     }
+    c128 *out = partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp;
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         const int k = q + 4 * c;
         if (k < cnt) {
             const double a = wave_sum(re[c]), b = wave_sum(im[c]);
-            if (lane == 0) partial[blockIdx.x * kLdp + k] = mk(a, b);
+            if (lane == 0) out[k] = mk(a, b);
         }
     }
 }
 
-// w -= V[:, 0:cnt] h, then partial[b * kLdp + k] = conj(V[:, k]) w over the block's rows (w staged in LDS per tile of
-// 256 rows).
-__global__ void __launch_bounds__(kThreads) k_cgs_update_dot(const c128 *__restrict__ V, int64_t ldv, int cnt,
-                                                             const c128 *__restrict__ h, c128 *__restrict__ w,
+// w_r -= V_r h1, then partial = conj(V_r[:, k]) w_r over the block's rows (w staged in LDS per tile of 256 rows).
+__global__ void __launch_bounds__(kThreads) k_cgs_update_dot(const c128 *__restrict__ V, int cnt,
+                                                             const c128 *__restrict__ scal, c128 *__restrict__ w,
                                                              int64_t n, int64_t chunk, c128 *__restrict__ partial,
-                                                             const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                             const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int R = gridDim.y;
     __shared__ c128 hs[kLdp];
     __shared__ c128 wt[kThreads];
-    for (int k = threadIdx.x; k < cnt; k += kThreads) hs[k] = h[k];
+    for (int k = threadIdx.x; k < cnt; k += kThreads) hs[k] = scal[r * kScal + kH1 + k];
     __syncthreads();
+    c128 *wr = w + r * n;
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
     double re[16], im[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) re[c] = im[c] = 0;
     for (int64_t t0 = base; t0 < end; t0 += kThreads) {
-        const int64_t r = t0 + threadIdx.x;
+        const int64_t i = t0 + threadIdx.x;
         c128 v = mk(0.0, 0.0);
-        if (r < end) {
-            v = w[r];
-            for (int k = 0; k < cnt; ++k) v = v - hs[k] * V[k * ldv + r];
-            w[r] = v;
+        if (i < end) {
+            v = wr[i];
+            for (int k = 0; k < cnt; ++k) v = v - hs[k] * basis_col(V, k, r, R, n)[i];
+            wr[i] = v;
         }
         wt[threadIdx.x] = v;
         __syncthreads();
         for (int u = 0; u < 4; ++u) {
             const int row = lane + 64 * u;
-            const int64_t rr = t0 + row;
-            if (rr < end) {
-                const c128 wr = wt[row];
+            const int64_t ii = t0 + row;
+            if (ii < end) {
+                const c128 wi = wt[row];
 #pragma unroll
                 for (int c = 0; c < 16; ++c) {
                     const int k = q + 4 * c;
                     if (k < cnt) {
-                        const c128 vv = V[k * ldv + rr];
-                        re[c] += vv.x * wr.x + vv.y * wr.y;
-                        im[c] += vv.x * wr.y - vv.y * wr.x;
+                        const c128 vv = basis_col(V, k, r, R, n)[ii];
+                        re[c] += vv.x * wi.x + vv.y * wi.y;
+                        im[c] += vv.x * wi.y - vv.y * wi.x;
                     }
                 }
             }
         }
         __syncthreads();
     }
+    c128 *out = partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp;
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         const int k = q + 4 * c;
         if (k < cnt) {
             const double a = wave_sum(re[c]), b = wave_sum(im[c]);
-            if (lane == 0) partial[blockIdx.x * kLdp + k] = mk(a, b);
+            if (lane == 0) out[k] = mk(a, b);
         }
     }
 }
 
-// w -= V[:, 0:cnt] h; partial[b * kLdp] = |w|^2 over the block's rows.
-__global__ void __launch_bounds__(kThreads) k_cgs_update_norm(const c128 *__restrict__ V, int64_t ldv, int cnt,
-                                                              const c128 *__restrict__ h, c128 *__restrict__ w,
+// w_r -= V_r h2; partial = |w_r|^2 over the block's rows.
+__global__ void __launch_bounds__(kThreads) k_cgs_update_norm(const c128 *__restrict__ V, int cnt,
+                                                              const c128 *__restrict__ scal, c128 *__restrict__ w,
                                                               int64_t n, int64_t chunk, c128 *__restrict__ partial,
-                                                              const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                              const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int R = gridDim.y;
     __shared__ c128 hs[kLdp];
     __shared__ double sh[8];
-    for (int k = threadIdx.x; k < cnt; k += kThreads) hs[k] = h[k];
+    for (int k = threadIdx.x; k < cnt; k += kThreads) hs[k] = scal[r * kScal + kH2 + k];
     __syncthreads();
+    c128 *wr = w + r * n;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
     double s = 0;
-    for (int64_t r = base + threadIdx.x; r < end; r += kThreads) {
-        c128 v = w[r];
-        for (int k = 0; k < cnt; ++k) v = v - hs[k] * V[k * ldv + r];
-        w[r] = v;
+    for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
+        c128 v = wr[i];
+        for (int k = 0; k < cnt; ++k) v = v - hs[k] * basis_col(V, k, r, R, n)[i];
+        wr[i] = v;
         s += abs2(v);
     }
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x * kLdp] = mk(s, 0.0);
+    if (threadIdx.x == 0) partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp] = mk(s, 0.0);
 }
 
 // Step j (0-based) of the cycle, iteration `it` overall: the new Hessenberg column h1 + h2 and |w|, the previous
 // rotations, the new rotation, the rotated right-hand side g, the monitored residual |g(j+1)| / reference and the
-// stopping test (iterate_.m:252-278); inv_sub = 1 / H(j+1, j) for the next basis vector.
+// stopping test (iterate_.m:252-278); inv_sub = 1 / H(j+1, j) for the next basis vector. Grid (1, R).
 __global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restrict__ partial, int blocks,
-                                                             const c128 *__restrict__ h1,
-                                                             const c128 *__restrict__ h2, c128 *__restrict__ H,
-                                                             int ldh, c128 *__restrict__ g, c128 *__restrict__ cs,
-                                                             c128 *__restrict__ sn, int j, int it, double reference,
-                                                             double tol, int max_iter, LoopState *st,
-                                                             double *history, c128 *inv_sub,
-                                                             const int *__restrict__ skip) {
-    SKIP_IF_DONE();
+                                                             c128 *__restrict__ scal, c128 *__restrict__ Hall,
+                                                             c128 *__restrict__ gall, int m, int j, int it,
+                                                             int max_iter, LoopState *st, double *history, int cap) {
+    SYSTEM(r);
     __shared__ double sh[8];
     double s = 0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[b * kLdp].x;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[(static_cast<int64_t>(r) * blocks + b) * kLdp].x;
     s = block_sum(s, sh);
     if (threadIdx.x != 0) return;
-    c128 *col = H + static_cast<int64_t>(ldh) * j;
-    for (int k = 0; k <= j; ++k) col[k] = h1[k] + h2[k];
+    c128 *sc = scal + r * kScal;
+    const int ldh = m + 1;
+    c128 *col = Hall + static_cast<int64_t>(r) * ldh * m + static_cast<int64_t>(ldh) * j;
+    c128 *g = gall + static_cast<int64_t>(r) * 3 * ldh, *cs = g + ldh, *sn = g + 2 * ldh;
+    for (int k = 0; k <= j; ++k) col[k] = sc[kH1 + k] + sc[kH2 + k];
     const double sub = sqrt(s);
     col[j + 1] = mk(sub, 0.0);
     for (int k = 0; k < j; ++k) {
@@ -351,30 +383,63 @@ __global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restr
     col[j + 1] = mk(0.0, 0.0);
     g[j + 1] = mk(0.0, 0.0) - sv * g[j];
     g[j] = cmulc(c, g[j]);
-    const double res = sqrt(abs2(g[j + 1])) / reference;
-    history[it] = res;
-    st->steps = j + 1;
-    st->iterations = it;
-    const bool conv = res < tol;
+    const double res = sqrt(abs2(g[j + 1])) / sc[kReference].x;
+    history[static_cast<int64_t>(r) * cap + it] = res;
+    st[r].steps = j + 1;
+    st[r].iterations = it;
+    const bool conv = res < sc[kTol].x;
     if (conv) {
-        st->converged = 1;
-        st->done = 1;
-    } else if (it >= max_iter || !isfinite(res)) {
-        st->done = 1;
+        st[r].converged = st[r].finished = st[r].done = 1;
+    } else if (!isfinite(res)) {
+        st[r].finished = st[r].done = 1;
+    } else if (it >= max_iter) {
+        st[r].done = 1;
     }
-    *inv_sub = (!conv && sub != 0) ? mk(1.0 / sub, 0.0) : mk(0.0, 0.0);
+    sc[kInvSub] = (!conv && sub != 0) ? mk(1.0 / sub, 0.0) : mk(0.0, 0.0);
 }
 
-// g = (rnorm, 0, ..., 0) at the start of a cycle.
-__global__ void k_gmres_init(c128 *__restrict__ g, int m, double rnorm) {
-    for (int k = threadIdx.x; k <= m; k += blockDim.x) g[k] = mk(k == 0 ? rnorm : 0.0, 0.0);
+// v_{col} = w * inv_sub.
+__global__ void k_next_basis(const c128 *__restrict__ w, c128 *__restrict__ V, int col, int64_t n,
+                             const c128 *__restrict__ scal, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const c128 f = scal[r * kScal + kInvSub];
+    const c128 *wr = w + r * n;
+    c128 *v = V + (static_cast<int64_t>(col) * gridDim.y + r) * n;
+    ROWS(i, n) v[i] = f * wr[i];
 }
 
-// y = H(0:steps, 0:steps) \ g(0:steps) (upper triangular after the rotations), steps read from the loop state.
-__global__ void k_gmres_solve(const c128 *__restrict__ H, int ldh, const c128 *__restrict__ g,
-                              const LoopState *st, c128 *__restrict__ y) {
+// Start of a cycle: v_0 = w / |w|, g = (|w|, 0, ..., 0).
+__global__ void k_gmres_start(const c128 *__restrict__ w, c128 *__restrict__ V, int m1, int64_t n,
+                              const c128 *__restrict__ scal, c128 *__restrict__ gall, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const double rn = scal[r * kScal + kRnorm].x;
+    const c128 *wr = w + r * n;
+    c128 *v = V + static_cast<int64_t>(r) * n;
+    ROWS(i, n) v[i] = (1.0 / rn) * wr[i];
+    if (blockIdx.x == 0) {
+        c128 *g = gall + static_cast<int64_t>(r) * 3 * m1;
+        for (int k = threadIdx.x; k < m1; k += blockDim.x) g[k] = mk(k == 0 ? rn : 0.0, 0.0);
+    }
+}
+
+// Start of a cycle: the stop flag is the permanent one, the step count is zero.
+__global__ void k_cycle_reset(LoopState *st, int R) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    st[r].done = st[r].finished;
+    st[r].steps = 0;
+}
+
+// y = H(0:steps, 0:steps) \ g(0:steps) (upper triangular after the rotations). Grid (1, R); no-op at steps = 0.
+__global__ void k_gmres_solve(const c128 *__restrict__ Hall, const c128 *__restrict__ gall, int m,
+                              const LoopState *st, c128 *__restrict__ scal) {
+    const int r = blockIdx.y;
     if (threadIdx.x != 0) return;
-    const int steps = st->steps;
+    const int steps = st[r].steps;
+    const int ldh = m + 1;
+    const c128 *H = Hall + static_cast<int64_t>(r) * ldh * m;
+    const c128 *g = gall + static_cast<int64_t>(r) * 3 * ldh;
+    c128 *y = scal + r * kScal + kY;
     for (int k = steps - 1; k >= 0; --k) {
         c128 acc = g[k];
         for (int l = k + 1; l < steps; ++l) acc = acc - H[k + static_cast<int64_t>(ldh) * l] * y[l];
@@ -383,373 +448,444 @@ __global__ void k_gmres_solve(const c128 *__restrict__ H, int ldh, const c128 *_
     }
 }
 
-// x += V[:, 0:steps] y, steps read from the loop state.
-__global__ void k_gemv_add(const c128 *__restrict__ V, int64_t ldv, const LoopState *st,
-                           const c128 *__restrict__ y, c128 *__restrict__ x, int64_t n) {
+// x_r += V_r[:, 0:steps] y_r. Grid (blocks, R); no-op at steps = 0.
+__global__ void k_gemv_add(const c128 *__restrict__ V, const LoopState *st, const c128 *__restrict__ scal,
+                           c128 *__restrict__ x, int64_t n) {
     __shared__ c128 ys[kLdp];
-    const int cnt = st->steps;
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) ys[k] = y[k];
+    const int r = blockIdx.y, R = gridDim.y;
+    const int cnt = st[r].steps;
+    if (cnt == 0) return;
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) ys[k] = scal[r * kScal + kY + k];
     __syncthreads();
-    GRID_STRIDE(i, n) {
-        c128 v = x[i];
-        for (int k = 0; k < cnt; ++k) v = v + ys[k] * V[k * ldv + i];
-        x[i] = v;
+    c128 *xr = x + r * n;
+    ROWS(i, n) {
+        c128 v = xr[i];
+        for (int k = 0; k < cnt; ++k) v = v + ys[k] * basis_col(V, k, r, R, n)[i];
+        xr[i] = v;
     }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
-static unsigned vec_blocks(int64_t n) { return blocks_for(n, 256, 4096); }
+static unsigned vec_blocks(int64_t n) { return blocks_for(n, 256, 1024); }
 
-Workspace::Workspace(Context *c, int64_t n_, int max_iter) : ctx(c), s(c->stream), n(n_) {
+static c128 *vec(Context *ctx, int slot, int64_t count) {
+    return static_cast<c128 *>(ctx->scratch(slot, sizeof(c128) * count, "the solver's vectors"));
+}
+
+Workspace::Workspace(Context *c, int64_t n_, int R_, int max_iter) : ctx(c), s(c->stream), n(n_), R(R_) {
+    int cap_blocks = 16384 / R;
+    cap_blocks = std::max(16, std::min(2048, cap_blocks));
     int64_t b = (n + kThreads - 1) / kThreads;
-    if (b > kMaxBlocks) b = kMaxBlocks;
-    if (b < 1) b = 1;
-    int64_t per = (n + b - 1) / b;
+    b = std::max<int64_t>(1, std::min<int64_t>(b, cap_blocks));
+    const int64_t per = (n + b - 1) / b;
     chunk = (per + kThreads - 1) / kThreads * kThreads;
     blocks = static_cast<int>((n + chunk - 1) / chunk);
-    if (blocks < 1) blocks = 1;
-    partial = static_cast<c128 *>(c->scratch(kSlotPartial, sizeof(c128) * kLdp * kMaxBlocks, "the solver's scratch"));
-    scal = static_cast<c128 *>(c->scratch(kSlotScal, sizeof(c128) * 256, "the solver's scalars"));
-    state = static_cast<LoopState *>(c->scratch(kSlotState, sizeof(LoopState), "the solver's scalars"));
-    history_cap = max_iter + 2;
-    history = static_cast<double *>(c->scratch(kSlotHistory, sizeof(double) * history_cap, "the solver's history"));
-    if (!c->host_pinned) RT_HIP(hipHostMalloc(&c->host_pinned, 4096, hipHostMallocDefault));
-    host = static_cast<c128 *>(c->host_pinned);
+    partial = vec(c, kSlotPartial, static_cast<int64_t>(R) * blocks * kLdp);
+    scal = vec(c, kSlotScal, static_cast<int64_t>(R) * kScal);
+    state = static_cast<LoopState *>(c->scratch(kSlotState, sizeof(LoopState) * R, "the solver's state"));
+    cap = max_iter + 2;
+    history = static_cast<double *>(c->scratch(kSlotHistory, sizeof(double) * cap * R, "the solver's history"));
+    host_bytes = std::max<size_t>(4096, static_cast<size_t>(R) * 64);
+    if (!c->host_pinned || c->host_pinned_bytes < host_bytes) {
+        if (c->host_pinned) RT_HIP(hipHostFree(c->host_pinned));
+        c->host_pinned = nullptr;
+        RT_HIP(hipHostMalloc(&c->host_pinned, host_bytes, hipHostMallocDefault));
+        c->host_pinned_bytes = host_bytes;
+    }
+    host = static_cast<unsigned char *>(c->host_pinned);
 }
 
-void Workspace::dot(const c128 *x, const c128 *y, bool conj, int slot) {
+void Workspace::dot(const c128 *x, const c128 *y, bool conj, int slot, const LoopState *st) {
     if (conj)
-        hipLaunchKernelGGL(k_dot<true>, dim3(blocks), dim3(kThreads), 0, s, x, y, n, chunk, partial, ctx->skip);
+        hipLaunchKernelGGL(k_dot<true>, dim3(blocks, R), dim3(kThreads), 0, s, x, y, n, chunk, partial, st);
     else
-        hipLaunchKernelGGL(k_dot<false>, dim3(blocks), dim3(kThreads), 0, s, x, y, n, chunk, partial, ctx->skip);
+        hipLaunchKernelGGL(k_dot<false>, dim3(blocks, R), dim3(kThreads), 0, s, x, y, n, chunk, partial, st);
     RT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, s, partial, blocks, scal + slot, 0, ctx->skip);
-    RT_LAUNCH_CHECK();
-}
-
-void Workspace::nrm2(const c128 *x, int slot) {
-    hipLaunchKernelGGL(k_nrm2, dim3(blocks), dim3(kThreads), 0, s, x, n, chunk, partial, ctx->skip);
-    RT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, s, partial, blocks, scal + slot, 1, ctx->skip);
+    hipLaunchKernelGGL(k_reduce, dim3(1, R), dim3(kThreads), 0, s, partial, blocks, scal, slot, 0, st);
     RT_LAUNCH_CHECK();
 }
 
-void Workspace::fetch(int first, int count) {
-    RT_HIP(hipMemcpyAsync(host + first, scal + first, count * sizeof(c128), hipMemcpyDeviceToHost, s));
+void Workspace::nrm2(const c128 *x, int slot, const LoopState *st) {
+    hipLaunchKernelGGL(k_nrm2, dim3(blocks, R), dim3(kThreads), 0, s, x, n, chunk, partial, st);
+    RT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce, dim3(1, R), dim3(kThreads), 0, s, partial, blocks, scal, slot, 1, st);
+    RT_LAUNCH_CHECK();
+}
+
+std::vector<double> Workspace::fetch_re(int slot) {
+    RT_HIP(hipMemcpy2DAsync(host, sizeof(c128), scal + slot, sizeof(c128) * kScal, sizeof(c128), R,
+                            hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    std::vector<double> out(R);
+    const c128 *h = reinterpret_cast<const c128 *>(host);
+    for (int r = 0; r < R; ++r) out[r] = h[r].x;
+    return out;
+}
+
+void Workspace::put_re(int slot, const std::vector<double> &v) {
+    c128 *h = reinterpret_cast<c128 *>(host);
+    for (int r = 0; r < R; ++r) h[r] = mk(v[r], 0.0);
+    RT_HIP(hipMemcpy2DAsync(scal + slot, sizeof(c128) * kScal, host, sizeof(c128), sizeof(c128), R,
+                            hipMemcpyHostToDevice, s));
+    RT_HIP(hipStreamSynchronize(s));   // the staging buffer is reused by the next call
+}
+
+void Workspace::reset_state(const std::vector<char> &finished) {
+    int *flags = static_cast<int *>(ctx->scratch(kSlotFlags, sizeof(int) * R, "the solver's state"));
+    int *h = reinterpret_cast<int *>(host);
+    for (int r = 0; r < R; ++r) h[r] = finished[r] ? 1 : 0;
+    RT_HIP(hipMemcpyAsync(flags, host, sizeof(int) * R, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_reset_state, dim3((R + 63) / 64), dim3(64), 0, s, state, flags, R);
+    RT_LAUNCH_CHECK();
     RT_HIP(hipStreamSynchronize(s));
 }
 
-LoopState Workspace::read_state() {
-    LoopState *h = reinterpret_cast<LoopState *>(reinterpret_cast<char *>(host) + 256 * sizeof(c128) - 64);
-    RT_HIP(hipMemcpyAsync(h, state, sizeof(LoopState), hipMemcpyDeviceToHost, s));
+std::vector<LoopState> Workspace::read_state() {
+    RT_HIP(hipMemcpyAsync(host, state, sizeof(LoopState) * R, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    return *h;
+    const LoopState *h = reinterpret_cast<const LoopState *>(host);
+    return std::vector<LoopState>(h, h + R);
 }
 
-void Workspace::read_history(int first, int count, std::vector<double> &out) {
-    if (count <= 0) return;
-    std::vector<double> tmp(count);
-    RT_HIP(hipMemcpy(tmp.data(), history + first, count * sizeof(double), hipMemcpyDeviceToHost));
-    out.insert(out.end(), tmp.begin(), tmp.end());
+std::vector<double> Workspace::read_history(int r, int first, int count) {
+    std::vector<double> out(count > 0 ? count : 0);
+    if (count > 0)
+        RT_HIP(hipMemcpy(out.data(), history + static_cast<int64_t>(r) * cap + first, count * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return out;
 }
 
-void Workspace::copy(const c128 *x, c128 *y) {
-    hipLaunchKernelGGL(k_copy, dim3(vec_blocks(n)), dim3(256), 0, s, x, y, n, ctx->skip);
+void Workspace::copy(const c128 *x, c128 *y, const LoopState *st) {
+    hipLaunchKernelGGL(k_copy, dim3(vec_blocks(n), R), dim3(256), 0, s, x, y, n, st);
     RT_LAUNCH_CHECK();
 }
-void Workspace::scale(const c128 *x, c128 a, c128 *y) {
-    hipLaunchKernelGGL(k_scale, dim3(vec_blocks(n)), dim3(256), 0, s, x, a, y, n, ctx->skip);
+void Workspace::sub(const c128 *b, const c128 *ax, c128 *r, const LoopState *st) {
+    hipLaunchKernelGGL(k_sub, dim3(vec_blocks(n), R), dim3(256), 0, s, b, ax, r, n, st);
     RT_LAUNCH_CHECK();
 }
-void Workspace::sub(const c128 *b, const c128 *ax, c128 *r) {
-    hipLaunchKernelGGL(k_sub, dim3(vec_blocks(n)), dim3(256), 0, s, b, ax, r, n, ctx->skip);
+void Workspace::add(const c128 *x, c128 *y, const LoopState *st) {
+    hipLaunchKernelGGL(k_add, dim3(vec_blocks(n), R), dim3(256), 0, s, x, y, n, st);
     RT_LAUNCH_CHECK();
 }
-void Workspace::axpy(c128 a, const c128 *x, c128 *y) {
-    hipLaunchKernelGGL(k_axpy, dim3(vec_blocks(n)), dim3(256), 0, s, a, x, y, n, ctx->skip);
-    RT_LAUNCH_CHECK();
-}
-void Workspace::zero(c128 *x) { RT_HIP(hipMemsetAsync(x, 0, n * sizeof(c128), s)); }
+void Workspace::zero(c128 *x) { RT_HIP(hipMemsetAsync(x, 0, static_cast<size_t>(R) * n * sizeof(c128), s)); }
 
-// Points the products' kernels at the loop's stop flag while a loop runs.
-struct SkipGuard {
-    Context *ctx;
-    SkipGuard(Context *c, const LoopState *st) : ctx(c) { ctx->skip = &st->done; }
-    ~SkipGuard() { ctx->skip = nullptr; }
+struct InnerResult {
+    std::vector<char> converged;
+    std::vector<int> iterations;
+    std::vector<std::vector<double>> norms;   // monitored norms from iteration 0
 };
 
-static c128 *vec(Context *ctx, int slot, int64_t n, int64_t count = 1) {
-    return static_cast<c128 *>(ctx->scratch(slot, sizeof(c128) * n * count, "the solver's vectors"));
+static bool all_done(const std::vector<LoopState> &st) {
+    for (const auto &x : st)
+        if (!x.done) return false;
+    return true;
+}
+
+static InnerResult collect(Workspace &ws, const std::vector<LoopState> &state, const std::vector<char> &skip_sys,
+                           const std::vector<double> &num, const std::vector<double> &den) {
+    const int R = ws.R;
+    InnerResult res;
+    res.converged.assign(R, 0);
+    res.iterations.assign(R, 0);
+    res.norms.resize(R);
+    for (int q = 0; q < R; ++q) {
+        if (skip_sys[q]) continue;
+        res.converged[q] = state[q].converged;
+        res.iterations[q] = state[q].iterations;
+        res.norms[q].push_back(num[q] / den[q]);
+        std::vector<double> h = ws.read_history(q, 1, state[q].iterations);
+        res.norms[q].insert(res.norms[q].end(), h.begin(), h.end());
+    }
+    return res;
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// COCG (ConjugateGradient.m:32-59). Scalar slots: 0/1 rho (alternating), 2 d^T A d, 3 |r|.
+// COCG (ConjugateGradient.m:32-59) on the systems not marked `skip_sys`. Tolerances per system.
 // ------------------------------------------------------------------------------------------------------------------
 static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c128 *b, c128 *x, bool has_x0,
-                        double tol, int max_iter, double bnorm) {
-    InnerResult res;
+                        const std::vector<double> &tol, const std::vector<char> &skip_sys, int max_iter) {
     Context *ctx = ws.ctx;
     const int64_t n = ws.n;
-    c128 *r = vec(ctx, kSlotR, n), *d = vec(ctx, kSlotD, n), *ad = vec(ctx, kSlotAd, n);
-    c128 *z = P ? vec(ctx, kSlotZ, n) : nullptr;
+    const int R = ws.R;
+    const LoopState *st = ws.state;
+    c128 *r = vec(ctx, kSlotR, R * n), *d = vec(ctx, kSlotD, R * n), *ad = vec(ctx, kSlotAd, R * n);
+    c128 *z = P ? vec(ctx, kSlotZ, R * n) : nullptr;
+    ws.reset_state(skip_sys);
+    ws.put_re(kTol, tol);
     if (has_x0) {
-        A(x, ad);
-        ws.sub(b, ad, r);
+        A(x, ad, ws.skip());
+        ws.sub(b, ad, r, st);
     } else {
         ws.zero(x);
-        ws.copy(b, r);
+        ws.copy(b, r, st);
     }
-    ws.nrm2(r, 3);
-    ws.fetch(3, 1);
-    res.norms.push_back(ws.host[3].x / bnorm);
-    if (res.norms.back() < tol) {
-        res.converged = true;
-        return res;
-    }
-    hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, ws.s, ws.state);
+    ws.nrm2(r, kRnormCg, st);
+    ws.nrm2(b, kBnorm, st);
+    hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnormCg, kBnorm, R);
     RT_LAUNCH_CHECK();
-    SkipGuard guard(ctx, ws.state);
+    const std::vector<double> r0 = ws.fetch_re(kRnormCg), b0 = ws.fetch_re(kBnorm);
+    std::vector<LoopState> state = ws.read_state();
     int it = 0;
-    LoopState st{};
-    while (it < max_iter) {
+    while (it < max_iter && !all_done(state)) {
         const int window = std::min(kPoll, max_iter - it);
         for (int w = 0; w < window; ++w) {
             ++it;
             const c128 *zz = r;
             if (P) {
-                (*P)(r, z);
+                (*P)(r, z, ws.skip());
                 zz = z;
             }
-            const int i_new = it & 1, i_old = 1 - i_new;
-            ws.dot(r, zz, false, i_new);
+            const int i_new = (it & 1) ? kRho1 : kRho0, i_old = (it & 1) ? kRho0 : kRho1;
+            ws.dot(r, zz, false, i_new, st);
             if (it == 1) {
-                ws.copy(zz, d);
+                ws.copy(zz, d, st);
             } else {
-                hipLaunchKernelGGL(k_cocg_direction, dim3(vec_blocks(n)), dim3(256), 0, ws.s, zz, d, ws.scal, i_new,
-                                   i_old, n, ctx->skip);
+                hipLaunchKernelGGL(k_cocg_direction, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, zz, d, ws.scal, i_new,
+                                   i_old, n, st);
                 RT_LAUNCH_CHECK();
             }
-            A(d, ad);
-            ws.dot(d, ad, false, 2);
-            hipLaunchKernelGGL(k_cocg_update, dim3(ws.blocks), dim3(kThreads), 0, ws.s, x, r, d, ad, ws.scal, i_new, 2,
-                               n, ws.chunk, ws.partial, ctx->skip);
+            A(d, ad, ws.skip());
+            ws.dot(d, ad, false, kDad, st);
+            hipLaunchKernelGGL(k_cocg_update, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, x, r, d, ad, ws.scal,
+                               i_new, n, ws.chunk, ws.partial, st);
             RT_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_cocg_finalize, dim3(1), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, bnorm, tol,
-                               it, max_iter, ws.state, ws.history, ctx->skip);
+            hipLaunchKernelGGL(k_cocg_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
+                               it, max_iter, ws.state, ws.history, ws.cap);
             RT_LAUNCH_CHECK();
         }
-        st = ws.read_state();
-        if (st.done) break;
+        state = ws.read_state();
     }
-    ws.read_history(1, st.iterations, res.norms);
-    res.converged = st.converged;
-    res.iterations = st.iterations;
-    return res;
+    return collect(ws, state, skip_sys, r0, b0);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// GMRES(m), left preconditioned, CGS2 (iterate_.m:1-85). Scalar slots: [0, 64) second-pass coefficients, [64, 128)
-// first-pass coefficients, 128 |r|, 130 1 / H(j+1, j), [160, 224) the solution coefficients y.
+// GMRES(m), left preconditioned, CGS2 (iterate_.m:1-85), on the systems not marked `skip_sys`.
 // ------------------------------------------------------------------------------------------------------------------
 static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c128 *b, c128 *x, bool has_x0,
-                         double tol, int max_iter, int restart) {
-    InnerResult res;
+                         const std::vector<double> &tol, const std::vector<char> &skip_sys, int max_iter,
+                         int restart) {
     Context *ctx = ws.ctx;
     const int64_t n = ws.n;
+    const int R = ws.R;
+    const LoopState *st = ws.state;
     int64_t m64 = restart > 0 ? restart : n;
-    if (m64 > n) m64 = n;
-    if (m64 > max_iter) m64 = max_iter;
+    m64 = std::min<int64_t>({m64, n, static_cast<int64_t>(max_iter)});
     const int m = static_cast<int>(m64);
     if (m > 62) fail(RT_EINVAL, "rooftopMoM:solver:invalidRestart", "restarts above 62 are not supported on the GPU");
-    c128 *V = vec(ctx, kSlotV, n, m + 1), *w = vec(ctx, kSlotW, n), *t = vec(ctx, kSlotT, n);
-    c128 *H = vec(ctx, kSlotH, m + 1, m);
-    c128 *gcs = vec(ctx, kSlotG, m + 1, 3);
-    c128 *g = gcs, *cs = gcs + (m + 1), *sn = gcs + 2 * (m + 1);
+    const int m1 = m + 1;
+    c128 *V = vec(ctx, kSlotV, static_cast<int64_t>(R) * m1 * n);
+    c128 *w = vec(ctx, kSlotW, R * n), *t = vec(ctx, kSlotT, R * n);
+    c128 *H = vec(ctx, kSlotH, static_cast<int64_t>(R) * m1 * m);
+    c128 *g = vec(ctx, kSlotG, static_cast<int64_t>(R) * 3 * m1);
     auto precondition = [&](const c128 *in, c128 *out) {
-        if (P) (*P)(in, out);
-        else if (in != out) ws.copy(in, out);
+        if (P) (*P)(in, out, ws.skip());
+        else ws.copy(in, out, st);
     };
+    ws.reset_state(skip_sys);
+    ws.put_re(kTol, tol);
     precondition(b, w);   // reference = norm(P b)
-    ws.nrm2(w, 128);
-    ws.fetch(128, 1);
-    const double reference = ws.host[128].x;
+    ws.nrm2(w, kReference, st);
     if (has_x0) {
-        A(x, t);
-        ws.sub(b, t, t);
+        A(x, t, ws.skip());
+        ws.sub(b, t, t, st);
         precondition(t, w);
-        ws.nrm2(w, 128);
-        ws.fetch(128, 1);
+        ws.nrm2(w, kRnorm, st);
     } else {
         ws.zero(x);
+        hipLaunchKernelGGL(k_copy_slot, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.scal, kReference, kRnorm, R);
+        RT_LAUNCH_CHECK();
     }
-    double rnorm = ws.host[128].x;
-    res.norms.push_back(rnorm / reference);
-    bool conv = res.norms.back() < tol;
-    int it = 0;
-    hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, ws.s, ws.state);
+    hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnorm, kReference,
+                       R);
     RT_LAUNCH_CHECK();
-    while (!conv && it < max_iter) {
-        ws.scale(w, mk(1.0 / rnorm, 0.0), V);
-        hipLaunchKernelGGL(k_gmres_init, dim3(1), dim3(64), 0, ws.s, g, m, rnorm);
+    const std::vector<double> r0 = ws.fetch_re(kRnorm), ref = ws.fetch_re(kReference);
+    std::vector<LoopState> state = ws.read_state();
+    auto any_active = [&]() {
+        for (const auto &q : state)
+            if (!q.finished) return true;
+        return false;
+    };
+    int it = 0;
+    while (it < max_iter && any_active()) {
+        hipLaunchKernelGGL(k_cycle_reset, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, R);
         RT_LAUNCH_CHECK();
-        const int it0 = it;
-        LoopState st{};
-        {
-            SkipGuard guard(ctx, ws.state);
-            int step = 0;
-            while (step < m && it < max_iter) {
-                const int window = std::min({kPoll, m - step, max_iter - it});
-                for (int k = 0; k < window; ++k) {
-                    const int j = step;
-                    ++step;
-                    ++it;
-                    const c128 *vj = V + static_cast<int64_t>(j) * n;
-                    A(vj, t);
-                    precondition(t, w);
-                    hipLaunchKernelGGL(k_cgs_dot, dim3(ws.blocks), dim3(kThreads), 0, ws.s, V, n, step, w, n,
-                                       ws.chunk, ws.partial, ctx->skip);
-                    RT_LAUNCH_CHECK();
-                    hipLaunchKernelGGL(k_reduce, dim3(step), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
-                                       ws.scal + 64, 0, ctx->skip);
-                    RT_LAUNCH_CHECK();
-                    hipLaunchKernelGGL(k_cgs_update_dot, dim3(ws.blocks), dim3(kThreads), 0, ws.s, V, n, step,
-                                       ws.scal + 64, w, n, ws.chunk, ws.partial, ctx->skip);
-                    RT_LAUNCH_CHECK();
-                    hipLaunchKernelGGL(k_reduce, dim3(step), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
-                                       0, ctx->skip);
-                    RT_LAUNCH_CHECK();
-                    hipLaunchKernelGGL(k_cgs_update_norm, dim3(ws.blocks), dim3(kThreads), 0, ws.s, V, n, step,
-                                       ws.scal, w, n, ws.chunk, ws.partial, ctx->skip);
-                    RT_LAUNCH_CHECK();
-                    hipLaunchKernelGGL(k_gmres_finalize, dim3(1), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
-                                       ws.scal + 64, ws.scal, H, m + 1, g, cs, sn, j, it, reference, tol, max_iter,
-                                       ws.state, ws.history, ws.scal + 130, ctx->skip);
-                    RT_LAUNCH_CHECK();
-                    if (step < m + 1) {
-                        hipLaunchKernelGGL(k_scale_dev, dim3(vec_blocks(n)), dim3(256), 0, ws.s, w, ws.scal + 130,
-                                           V + static_cast<int64_t>(step) * n, n, ctx->skip);
-                        RT_LAUNCH_CHECK();
-                    }
-                }
-                st = ws.read_state();
-                if (st.done) break;
+        hipLaunchKernelGGL(k_gmres_start, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, V, m1, n, ws.scal, g, st);
+        RT_LAUNCH_CHECK();
+        int step = 0;
+        while (step < m && it < max_iter) {
+            const int window = std::min({kPoll, m - step, max_iter - it});
+            for (int k = 0; k < window; ++k) {
+                const int j = step;
+                ++step;
+                ++it;
+                A(V + static_cast<int64_t>(j) * R * n, t, ws.skip());   // column j of every system
+                precondition(t, w);
+                hipLaunchKernelGGL(k_cgs_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, w, n, ws.chunk,
+                                   ws.partial, st);
+                RT_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
+                                   kH1, 0, st);
+                RT_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_cgs_update_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
+                                   w, n, ws.chunk, ws.partial, st);
+                RT_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
+                                   kH2, 0, st);
+                RT_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_cgs_update_norm, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
+                                   w, n, ws.chunk, ws.partial, st);
+                RT_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_gmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
+                                   ws.scal, H, g, m, j, it, max_iter, ws.state, ws.history, ws.cap);
+                RT_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_next_basis, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, V, step, n, ws.scal,
+                                   st);
+                RT_LAUNCH_CHECK();
             }
+            state = ws.read_state();
+            if (all_done(state)) break;
         }
-        it = st.iterations;
-        ws.read_history(it0 + 1, it - it0, res.norms);
-        conv = st.converged;
-        // x += V y with y = H \ g over the steps of this cycle.
-        hipLaunchKernelGGL(k_gmres_solve, dim3(1), dim3(64), 0, ws.s, H, m + 1, g, ws.state, ws.scal + 160);
+        // x += V y with y = H \ g over the steps of this cycle (systems idle this cycle have 0 steps).
+        hipLaunchKernelGGL(k_gmres_solve, dim3(1, R), dim3(64), 0, ws.s, H, g, m, ws.state, ws.scal);
         RT_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gemv_add, dim3(vec_blocks(n)), dim3(256), 0, ws.s, V, n, ws.state, ws.scal + 160, x, n);
+        hipLaunchKernelGGL(k_gemv_add, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, V, ws.state, ws.scal, x, n);
         RT_LAUNCH_CHECK();
-        if (!conv && it < max_iter && !(st.done && !std::isfinite(res.norms.back()))) {
-            // The next cycle starts from the true preconditioned residual.
-            A(x, t);
-            ws.sub(b, t, t);
-            precondition(t, w);
-            ws.nrm2(w, 128);
-            ws.fetch(128, 1);
-            rnorm = ws.host[128].x;
-            hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, ws.s, ws.state);
-            RT_LAUNCH_CHECK();
-            // Keep the global iteration count in the state for the next cycle's finalize kernels.
-        } else {
-            break;
-        }
+        state = ws.read_state();
+        if (it >= max_iter || !any_active()) break;
+        // The next cycle starts from the true preconditioned residual (finished systems are skipped).
+        hipLaunchKernelGGL(k_cycle_reset, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, R);
+        RT_LAUNCH_CHECK();
+        A(x, t, ws.skip());
+        ws.sub(b, t, t, st);
+        precondition(t, w);
+        ws.nrm2(w, kRnorm, st);
     }
     RT_HIP(hipStreamSynchronize(ws.s));
-    res.converged = conv;
-    res.iterations = it;
-    return res;
+    return collect(ws, state, skip_sys, r0, ref);
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// IterativeMethod.solve (IterativeMethod.m:27-70) with the optional FP64 refinement loop on the true residual.
+// IterativeMethod.solve (IterativeMethod.m:27-70) on R systems, with the optional FP64 refinement loop on the true
+// residual.
 // ------------------------------------------------------------------------------------------------------------------
-SolveResult solve_one(Context *ctx, Product &op, Product *pc, const SolveOptions &o, const c128 *b, const c128 *x0,
-                      c128 *x) {
+std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, const SolveOptions &o, const c128 *b,
+                                     const c128 *x0, c128 *x, int R) {
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t n = op.dims.n;
-    Workspace ws(ctx, n, o.max_iterations);
+    Workspace ws(ctx, n, R, o.max_iterations);
     hipStream_t s = ctx->stream;
-    SolveResult out;
-    ws.nrm2(b, 200);
-    ws.fetch(200, 1);
-    const double bnorm = ws.host[200].x;
-    c128 *t = vec(ctx, kSlotOuterT, n);
-    // The outer residual runs in FP64 whenever the operator holds FP64 spectra.
-    const int outer_prec = op.has(RT_C128) ? RT_C128 : RT_C64;
+    std::vector<SolveResult> out(R);
+    ws.nrm2(b, kOuterB, nullptr);
+    const std::vector<double> bnorm = ws.fetch_re(kOuterB);
+    std::vector<char> zero_rhs(R, 0);
+    bool any = false;
+    for (int r = 0; r < R; ++r) {
+        zero_rhs[r] = !(bnorm[r] > 0);
+        any = any || !zero_rhs[r];
+    }
+    c128 *t = vec(ctx, kSlotOuterT, R * n);
+    const int outer_prec = op.has(RT_C128) ? RT_C128 : RT_C64;   // the outer residual in FP64 when held
+    // norm(b - A x) / norm(b) of every system; t keeps b - A x.
     auto true_residual = [&](const c128 *xx) {
-        op.apply(xx, t, 1, outer_prec, s);
-        ws.sub(b, t, t);
-        ws.nrm2(t, 201);
-        ws.fetch(201, 1);
-        return ws.host[201].x / bnorm;
+        op.apply(xx, t, R, outer_prec, s);
+        ws.sub(b, t, t, nullptr);
+        ws.nrm2(t, kOuterRes, nullptr);
+        std::vector<double> rel = ws.fetch_re(kOuterRes);
+        for (int r = 0; r < R; ++r) rel[r] = zero_rhs[r] ? 0.0 : rel[r] / bnorm[r];
+        return rel;
     };
-    if (!(bnorm > 0)) {
+    const int inner_prec = o.precision;
+    LinOp A = [&](const c128 *in, c128 *res, const int *skip) {
+        op.apply(in, res, R, inner_prec, s, skip, kStateInts);
+    };
+    LinOp Pop = [&](const c128 *in, c128 *res, const int *skip) {
+        pc->apply(in, res, R, inner_prec, s, skip, kStateInts);
+    };
+    const LinOp *P = pc ? &Pop : nullptr;
+    auto run = [&](const c128 *rhs, c128 *sol, bool has_x0, const std::vector<double> &tol,
+                   const std::vector<char> &skip_sys, int max_it) {
+        if (o.method == RT_METHOD_CG) return cocg(ws, A, P, rhs, sol, has_x0, tol, skip_sys, max_it);
+        return gmres(ws, A, P, rhs, sol, has_x0, tol, skip_sys, max_it, o.restart);
+    };
+    if (!any) {
         ws.zero(x);
-        RT_HIP(hipStreamSynchronize(s));
-        out.converged = true;
-        out.norms.push_back(0.0);
-        out.residual = 0;
+        for (auto &r : out) {
+            r.converged = true;
+            r.norms.push_back(0.0);
+        }
+    } else if (!o.refine) {
+        if (x0 && x0 != x) ws.copy(x0, x, nullptr);
+        std::vector<double> tol(R, o.tolerance);
+        InnerResult ir = run(b, x, x0 != nullptr, tol, zero_rhs, o.max_iterations);
+        const std::vector<double> rel = true_residual(x);
+        for (int r = 0; r < R; ++r) {
+            if (zero_rhs[r]) {
+                out[r].converged = true;
+                out[r].norms.push_back(0.0);
+                continue;
+            }
+            out[r].converged = ir.converged[r];
+            out[r].iterations = ir.iterations[r];
+            out[r].norms = ir.norms[r];
+            out[r].residual = rel[r];
+        }
     } else {
-        const int inner_prec = o.precision;
-        LinOp A = [&](const c128 *in, c128 *res) { op.apply(in, res, 1, inner_prec, s, ctx->skip); };
-        LinOp Pop = [&](const c128 *in, c128 *res) { pc->apply(in, res, 1, inner_prec, s, ctx->skip); };
-        const LinOp *P = pc ? &Pop : nullptr;
-        auto run = [&](const c128 *rhs, c128 *sol, bool has_x0, double tol, int max_it) {
-            if (o.method == RT_METHOD_CG) {
-                ws.nrm2(rhs, 202);
-                ws.fetch(202, 1);
-                return cocg(ws, A, P, rhs, sol, has_x0, tol, max_it, ws.host[202].x);
+        // x_{k+1} = x_k + d_k, A d_k = b - A x_k solved by the inner method in its precision.
+        if (x0) ws.copy(x0, x, nullptr);
+        else ws.zero(x);
+        c128 *rv = vec(ctx, kSlotOuterR, R * n), *d = vec(ctx, kSlotOuterD, R * n);
+        std::vector<double> rel = true_residual(x);
+        // Inner tolerance: complex64 GMRES reaches about 1e-6 before the rounding of its products shows; COCG's
+        // short recurrence loses the complex symmetry of complex64 products near 1e-4.
+        double inner_tol = o.tolerance;
+        if (o.inner_tolerance > 0) inner_tol = o.inner_tolerance;
+        else if (o.precision == RT_C64) inner_tol = std::max(o.tolerance, o.method == RT_METHOD_CG ? 1e-4 : 1e-6);
+        std::vector<char> done(R);
+        std::vector<int> total(R, 0);
+        for (int r = 0; r < R; ++r) {
+            done[r] = zero_rhs[r] || rel[r] < o.tolerance;
+            out[r].norms.push_back(rel[r]);
+        }
+        int outer = 0, used = 0;
+        while (outer < o.max_outer && used < o.max_iterations) {
+            bool active = false;
+            for (int r = 0; r < R; ++r) active = active || !done[r];
+            if (!active) break;
+            ++outer;
+            ws.copy(t, rv, nullptr);   // t holds b - A x from true_residual
+            // Each system asks the inner solve for what is left to reach the tolerance, no less than the floor.
+            std::vector<double> want(R, 0.5);
+            for (int r = 0; r < R; ++r)
+                if (!done[r]) want[r] = std::min(0.5, std::max(inner_tol, 0.5 * o.tolerance / rel[r]));
+            InnerResult ir = run(rv, d, false, want, done, o.max_iterations - used);
+            int most = 0;
+            for (int r = 0; r < R; ++r) {
+                if (done[r]) continue;
+                total[r] += ir.iterations[r];
+                most = std::max(most, ir.iterations[r]);
+                out[r].outer = outer;
+                for (size_t k = 1; k < ir.norms[r].size(); ++k) out[r].norms.push_back(ir.norms[r][k] * rel[r]);
             }
-            return gmres(ws, A, P, rhs, sol, has_x0, tol, max_it, o.restart);
-        };
-        if (!o.refine) {
-            if (x0 && x0 != x) ws.copy(x0, x);
-            InnerResult r = run(b, x, x0 != nullptr, o.tolerance, o.max_iterations);
-            out.converged = r.converged;
-            out.iterations = r.iterations;
-            out.norms = r.norms;
-            out.residual = true_residual(x);
-        } else {
-            // x_{k+1} = x_k + d_k, A d_k = b - A x_k solved by the inner method in its precision.
-            if (x0) ws.copy(x0, x);
-            else ws.zero(x);
-            c128 *r = vec(ctx, kSlotOuterR, n), *d = vec(ctx, kSlotOuterD, n);
-            double rel = true_residual(x);
-            out.norms.push_back(rel);
-            // Inner tolerance: complex64 GMRES reaches about 1e-6 before the rounding of its products shows; COCG's
-            // short recurrence loses the complex symmetry of complex64 products near 1e-4.
-            double inner_tol = o.tolerance;
-            if (o.inner_tolerance > 0) inner_tol = o.inner_tolerance;
-            else if (o.precision == RT_C64) inner_tol = std::max(o.tolerance, o.method == RT_METHOD_CG ? 1e-4 : 1e-6);
-            int total = 0;
-            while (!(rel < o.tolerance) && out.outer < o.max_outer && total < o.max_iterations) {
-                ++out.outer;
-                ws.copy(t, r);   // t holds b - A x from true_residual
-                // Ask the inner solve for what is left to reach the tolerance, but no less than its floor.
-                double want = 0.5 * o.tolerance / rel;
-                if (want < inner_tol) want = inner_tol;
-                if (want > 0.5) want = 0.5;
-                InnerResult ir = run(r, d, false, want, o.max_iterations - total);
-                total += ir.iterations;
-                for (size_t k = 1; k < ir.norms.size(); ++k) out.norms.push_back(ir.norms[k] * rel);
-                ws.axpy(mk(1.0, 0.0), d, x);
-                const double next = true_residual(x);
-                rel = next;
-                if (!std::isfinite(next)) break;
-            }
-            out.iterations = total;
-            out.residual = rel;
-            out.converged = rel < o.tolerance;
+            used += most;
+            ws.add(d, x, nullptr);   // d is zero for the systems the inner solve left out
+            rel = true_residual(x);
+            for (int r = 0; r < R; ++r)
+                if (!done[r] && (rel[r] < o.tolerance || !std::isfinite(rel[r]))) done[r] = 1;
+        }
+        for (int r = 0; r < R; ++r) {
+            out[r].iterations = total[r];
+            out[r].residual = rel[r];
+            out[r].converged = zero_rhs[r] || rel[r] < o.tolerance;
         }
     }
+    // A zero right-hand side has the zero solution (IterativeMethod.m:50-57), whatever the initial guess.
+    for (int r = 0; r < R; ++r)
+        if (zero_rhs[r]) RT_HIP(hipMemsetAsync(x + static_cast<int64_t>(r) * n, 0, sizeof(c128) * n, s));
     RT_HIP(hipStreamSynchronize(s));
-    out.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (auto &r : out) r.time = time;
     return out;
 }
 

@@ -1,4 +1,4 @@
-// Krylov solvers of librooftop (krylov.hip).
+// Krylov solvers of librooftop (krylov.hip): COCG and GMRES(m) on R right-hand sides at once.
 #pragma once
 
 #include <functional>
@@ -8,7 +8,8 @@
 
 namespace rt {
 
-using LinOp = std::function<void(const c128 *, c128 *)>;
+// A batched linear map on R systems (vectors of n, system-major); skip: per-system stop flags (stride kStateInts).
+using LinOp = std::function<void(const c128 *, c128 *, const int *)>;
 
 struct SolveOptions {
     int method = RT_METHOD_CG;
@@ -21,12 +22,6 @@ struct SolveOptions {
     int max_outer = 20;
 };
 
-struct InnerResult {
-    bool converged = false;
-    int iterations = 0;
-    std::vector<double> norms;
-};
-
 struct SolveResult {
     bool converged = false;
     int iterations = 0;
@@ -36,43 +31,51 @@ struct SolveResult {
     double time = 0;
 };
 
-// Device state of one Krylov loop, written by its finalize kernels and polled by the host.
+// Device state of one system of a Krylov loop, written by its finalize kernels and polled by the host.
 struct LoopState {
-    int done;         // converged, broke down or reached the iteration limit: every later kernel returns at once
+    int done;         // every later kernel of this system returns at once (finished, or the iteration limit)
     int converged;
-    int iterations;   // iterations completed (global count)
+    int finished;     // converged or broke down: stays done across GMRES cycles
+    int iterations;   // iterations completed
     int steps;        // GMRES: steps of the current cycle completed
-    int pad[4];
+    int pad[3];
 };
+constexpr int kStateInts = sizeof(LoopState) / sizeof(int);
+constexpr int kScal = 512;   // device scalar slots per system
+constexpr int kMaxBatch = 1024;   // systems of one batched solve
 
-// Device scratch of one solve, taken from the context's buffer pool: reduction partials, device scalars, the loop
-// state, the residual history and their pinned host mirror.
+// Device scratch of one batched solve, from the context's buffer pool.
 struct Workspace {
     Context *ctx;
     hipStream_t s;
     int64_t n;
-    int blocks;          // blocks of every row-partitioned kernel
+    int R;
+    int blocks;          // row blocks per system of every reduction
     int64_t chunk;       // rows per block (a multiple of 256)
-    c128 *partial;       // [blocks][64]
-    c128 *scal;          // 256 device scalars
-    LoopState *state;
-    double *history;     // residual norms by iteration
-    int history_cap;
-    c128 *host;          // pinned mirror (256 scalars)
-    Workspace(Context *c, int64_t n_, int max_iter);
-    void nrm2(const c128 *x, int slot);
-    void dot(const c128 *x, const c128 *y, bool conj, int slot);
-    void fetch(int first, int count);
-    void copy(const c128 *x, c128 *y);
-    void scale(const c128 *x, c128 a, c128 *y);
-    void sub(const c128 *b, const c128 *ax, c128 *r);
-    void axpy(c128 a, const c128 *x, c128 *y);
+    c128 *partial;       // [R][blocks][64]
+    c128 *scal;          // [R][kScal]
+    LoopState *state;    // [R]
+    double *history;     // [R][cap]
+    int cap;
+    unsigned char *host; // pinned staging
+    size_t host_bytes;
+    Workspace(Context *c, int64_t n_, int R_, int max_iter);
+    const int *skip() const { return &state->done; }
+    void nrm2(const c128 *x, int slot, const LoopState *st);
+    void dot(const c128 *x, const c128 *y, bool conj, int slot, const LoopState *st);
+    std::vector<double> fetch_re(int slot);                  // real part of one slot of every system
+    void put_re(int slot, const std::vector<double> &v);     // sets one slot of every system (real)
+    void copy(const c128 *x, c128 *y, const LoopState *st);
+    void sub(const c128 *b, const c128 *ax, c128 *r, const LoopState *st);
+    void add(const c128 *x, c128 *y, const LoopState *st);   // y += x
     void zero(c128 *x);
-    LoopState read_state();
-    void read_history(int first, int count, std::vector<double> &out);
+    void reset_state(const std::vector<char> &finished);
+    std::vector<LoopState> read_state();
+    std::vector<double> read_history(int r, int first, int count);
 };
 
-SolveResult solve_one(Context *ctx, Product &op, Product *pc, const SolveOptions &o, const c128 *b, const c128 *x0,
-                      c128 *x);
+// Solves R systems A x_r = b_r (b, x0, x: R vectors of n, system-major). One result per system.
+std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, const SolveOptions &o, const c128 *b,
+                                     const c128 *x0, c128 *x, int R);
 
 }  // namespace rt

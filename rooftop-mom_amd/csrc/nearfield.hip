@@ -124,11 +124,12 @@ template <typename TC>
 __global__ void __launch_bounds__(256) sai_apply(const c128 *__restrict__ in, c128 *__restrict__ out,
                                                  const int *__restrict__ cls, const cplx<TC> *__restrict__ coef,
                                                  int rows_x, int cols_x, int rows_y, int cols_y, int64_t nx,
-                                                 int64_t n, int nrhs, const int *__restrict__ skip) {
-    if (skip && *skip) return;
+                                                 int64_t n, int nrhs, const int *__restrict__ skip,
+                                                 int skip_stride) {
     const int64_t id = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (id >= n * nrhs) return;
     const int64_t r = id / n;
+    if (skip && skip[r * skip_stride]) return;
     const int64_t i = id - r * n;
     const int ci = cls[i];
     if (ci < 0) {
@@ -174,11 +175,11 @@ __global__ void jacobi_diag(GenView v, c128 *__restrict__ inv) {
 template <typename TC>
 __global__ void jacobi_apply(const c128 *__restrict__ in, c128 *__restrict__ out, const cplx<TC> *__restrict__ inv,
                              const uint8_t *__restrict__ exists, int64_t nx, int64_t n, int nrhs,
-                             const int *__restrict__ skip) {
-    if (skip && *skip) return;
+                             const int *__restrict__ skip, int skip_stride) {
     const int64_t total = n * nrhs;
     for (int64_t id = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; id < total;
          id += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        if (skip && skip[(id / n) * skip_stride]) continue;
         const int64_t i = id % n;
         cplx<TC> v = inv[i < nx ? 0 : 1] * cvt<TC>(in[id]);
         if (exists && !exists[i]) v = mk<TC>(0, 0);
@@ -338,7 +339,7 @@ std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precisi
 }
 
 void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s,
-                          const int *skip) {
+                          const int *skip, int skip_stride) {
     if (prec == RT_MIXED) prec = RT_C128;
     if (!pc.has(prec)) fail(RT_EINVAL, "rooftop:product", "the preconditioner does not hold the requested precision");
     const ProdDims &d = pc.dims;
@@ -347,10 +348,10 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
         const unsigned blocks = blocks_for(total, 256, 8192);
         if (prec == RT_C128)
             hipLaunchKernelGGL(jacobi_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag64.ptr,
-                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip);
+                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip, skip_stride);
         else
             hipLaunchKernelGGL(jacobi_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag32.ptr,
-                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip);
+                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip, skip_stride);
         RT_LAUNCH_CHECK();
         return;
     }
@@ -358,10 +359,10 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     if (pc.class_count == 0) fail(RT_EINVAL, "rooftop:preconditioner", "the grid has no rooftop to precondition");
     if (prec == RT_C128)
         hipLaunchKernelGGL(sai_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef64.ptr,
-                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip);
+                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
     else
         hipLaunchKernelGGL(sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef32.ptr,
-                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip);
+                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
     RT_LAUNCH_CHECK();
 }
 

@@ -82,14 +82,14 @@ template <typename TC, typename TIO>
 __global__ void __launch_bounds__(256) pass_a(const cplx<TIO> *__restrict__ in, cplx<TC> *__restrict__ work,
                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
                                               const cplx<TC> *__restrict__ tw, int group,
-                                              const int *__restrict__ skip) {
-    if (skip && *skip) return;
+                                              const int *__restrict__ skip, int skip_stride) {
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int M = d.Mx;
     const int stride = M + 1;
     int r, comp, j0;
     group_of(d, group, blockIdx.x, r, comp, j0);
+    if (skip && skip[r * skip_stride]) return;
     const int cols = comp ? d.cols_y : d.cols_x;
     const int rows = comp ? d.rows_y : d.rows_x;
     const int ncols = min(group, cols - j0);
@@ -119,14 +119,14 @@ template <typename TC, typename TIO>
 __global__ void __launch_bounds__(256) pass_c(const cplx<TC> *__restrict__ work, cplx<TIO> *__restrict__ out,
                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
                                               const cplx<TC> *__restrict__ tw, int group,
-                                              const int *__restrict__ skip) {
-    if (skip && *skip) return;
+                                              const int *__restrict__ skip, int skip_stride) {
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int M = d.Mx;
     const int stride = M + 1;
     int r, comp, j0;
     group_of(d, group, blockIdx.x, r, comp, j0);
+    if (skip && skip[r * skip_stride]) return;
     const int cols = comp ? d.cols_y : d.cols_x;
     const int rows = comp ? d.rows_y : d.rows_x;
     const int ncols = min(group, cols - j0);
@@ -156,14 +156,14 @@ __global__ void __launch_bounds__(256) pass_b(cplx<TC> *__restrict__ work, const
                                               const int *__restrict__ fold_x, const int *__restrict__ fold_y,
                                               const cplx<TC> *__restrict__ dx, const cplx<TC> *__restrict__ dy,
                                               ProdDims d, FftPlan plan, const cplx<TC> *__restrict__ tw, int tile,
-                                              const int *__restrict__ skip) {
-    if (skip && *skip) return;
+                                              const int *__restrict__ skip, int skip_stride) {
+    const int r = blockIdx.y;
+    if (skip && skip[r * skip_stride]) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
     const int My = d.My;
     const int stride = My + 1;
     const int px0 = blockIdx.x * tile;
-    const int r = blockIdx.y;
     const int ncols = 2 * tile;
     for (int e = threadIdx.x; e < ncols * My; e += blockDim.x) {
         const int col = e / My, j = e - col * My;
@@ -310,7 +310,7 @@ static void pass_bytes(const ProdDims &d, int nrhs, double *bytes) {
 
 template <typename TC>
 static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
-                       const cplx<TC> *planes, hipStream_t s, const int *skip = nullptr) {
+                       const cplx<TC> *planes, hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
     const ProdDims &d = op.dims;
     Context *ctx = op.problem->ctx;
     const uint8_t *ex = op.problem->exists.ptr;
@@ -330,16 +330,16 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
         RT_HIP(hipEventRecord(rec.ev[0], s));
     }
     hipLaunchKernelGGL((pass_a<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s,
-                       reinterpret_cast<const cplx<double> *>(in), work, ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip);
+                       reinterpret_cast<const cplx<double> *>(in), work, ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip, skip_stride);
     RT_LAUNCH_CHECK();
     if (timed) RT_HIP(hipEventRecord(rec.ev[1], s));
     hipLaunchKernelGGL((pass_b<TC>), dim3(d.Mx / sh.tile, nrhs), dim3(sh.threads_b), lds_b, s, work, planes,
                        op.tx.fold.ptr, op.ty.fold.ptr, d_of<TC>(op.tx), d_of<TC>(op.ty), d, op.ty.plan,
-                       tw_of<TC>(op.ty), sh.tile, skip);
+                       tw_of<TC>(op.ty), sh.tile, skip, skip_stride);
     RT_LAUNCH_CHECK();
     if (timed) RT_HIP(hipEventRecord(rec.ev[2], s));
     hipLaunchKernelGGL((pass_c<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s, work,
-                       reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip);
+                       reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip, skip_stride);
     RT_LAUNCH_CHECK();
     if (timed) {
         RT_HIP(hipEventRecord(rec.ev[3], s));
@@ -348,7 +348,8 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     }
 }
 
-void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip) {
+void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
+                    int skip_stride) {
     if (prec == RT_MIXED) prec = RT_C128;
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
     const ProdDims &d = op.dims;
@@ -360,14 +361,14 @@ void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, 
             op.shape64 = default_shape<double>(d, nrhs);
         }
         PassShape sh = default_shape<double>(d, nrhs);
-        run_passes<double>(op, in, out, op.work64.ptr, nrhs, sh, op.planes64.ptr, s, skip);
+        run_passes<double>(op, in, out, op.work64.ptr, nrhs, sh, op.planes64.ptr, s, skip, skip_stride);
     } else {
         if (op.work_rhs32 < nrhs) {
             op.work32.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
             op.work_rhs32 = nrhs;
         }
         PassShape sh = default_shape<float>(d, nrhs);
-        run_passes<float>(op, in, out, op.work32.ptr, nrhs, sh, op.planes32.ptr, s, skip);
+        run_passes<float>(op, in, out, op.work32.ptr, nrhs, sh, op.planes32.ptr, s, skip, skip_stride);
     }
 }
 
@@ -508,11 +509,11 @@ uint64_t Product::working_bytes() const {
     return per * (has(RT_C128) ? sizeof(c128) : sizeof(c64));
 }
 
-void Product::apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip) {
+void Product::apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip, int skip_stride) {
     if (type == kOperator)
-        operator_apply(*this, in, out, nrhs, prec, s, skip);
+        operator_apply(*this, in, out, nrhs, prec, s, skip, skip_stride);
     else
-        preconditioner_apply(*this, in, out, nrhs, prec, s, skip);
+        preconditioner_apply(*this, in, out, nrhs, prec, s, skip, skip_stride);
 }
 
 }  // namespace rt

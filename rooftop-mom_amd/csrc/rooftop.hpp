@@ -91,19 +91,22 @@ struct Product {
     int class_count = 0;
 
     bool has(int prec) const { return precision == RT_MIXED || precision == prec; }
-    // skip: optional device flag; when it is non-zero at run time every kernel of the product returns at once.
-    void apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip = nullptr);
+    // skip: optional device flags, one per right-hand side r at skip[r * skip_stride] (stride 0: one flag for all);
+    // the kernels of a right-hand side whose flag is non-zero at run time return at once.
+    void apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip = nullptr,
+               int skip_stride = 0);
     uint64_t stored_bytes() const;
     uint64_t working_bytes() const;
 };
 
 // product.hip
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier);
-void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip);
+void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
+                    int skip_stride);
 void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s);
 // nearfield.hip
 std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precision);
 void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s,
-                          const int *skip);
+                          const int *skip, int skip_stride);
 
 }  // namespace rt

@@ -156,3 +156,43 @@ def test_initial_guess(rt, method):
     cold = rt.Solver(method, tolerance=1e-10).solve(op, b, preconditioner=pc)
     assert res.converged() and rel(res.solution(), exact) < 1e-8
     assert res.iterationCount() <= cold.iterationCount()
+
+
+@pytest.mark.parametrize('method,precision', [('cg', 'c128'), ('gmres', 'c128'), ('cg', 'c64'), ('gmres', 'c64')])
+def test_batched_solve_matches_single_solves(rt, method, precision):
+    """R right-hand sides solved together give, column by column, the results of solving each alone (iteration
+    counts, monitored norms, solutions), including a zero column and an exact initial guess."""
+    p, grid, gen = _setup(rt, 16, 14, 0.1, 0.1, (6, 4))
+    rng = np.random.default_rng(11)
+    cols = []
+    for q in range(5):
+        d = rng.standard_normal(3)
+        d[2] = -abs(d[2]) - 0.5
+        d /= np.linalg.norm(d)
+        e = np.cross(d, [0.0, 1.0, 0.0])
+        cols.append(X.plane_wave(grid, 2 * np.pi, d, e / np.linalg.norm(e), 1.0, order=4))
+    cols.insert(2, np.zeros(grid.count, complex))
+    B = np.stack(cols, axis=1)
+    op = rt.Operator(p, precision='mixed')
+    pc = rt.Preconditioner(p, 'nearField', precision='mixed')
+    solver = rt.Solver(method, tolerance=1e-9, restart=12, precision=precision)
+    Z = S.dense_matrix(gen, grid)
+    exact = np.linalg.solve(Z, B[:, 4])
+    X0 = np.zeros_like(B)
+    X0[:, 4] = exact
+    batch = solver.solve(op, B, preconditioner=pc, initialGuess=X0)
+    assert len(batch) == B.shape[1]
+    for q in range(B.shape[1]):
+        print('column', q, flush=True)
+        alone = solver.solve(op, B[:, q], preconditioner=pc, initialGuess=X0[:, q])
+        got = batch[q]
+        assert got.converged() == alone.converged()
+        assert got.iterationCount() == alone.iterationCount()
+        np.testing.assert_allclose(got.residualNorms(), alone.residualNorms(), rtol=1e-9, atol=1e-300)
+        assert rel(got.solution(), alone.solution()) < 1e-12 or np.all(alone.solution() == 0)
+        if q == 2:
+            assert got.iterationCount() == 0 and np.all(got.solution() == 0)
+        elif q == 4:
+            assert got.iterationCount() == 0
+        else:
+            assert got.converged() and rel(Z @ got.solution(), B[:, q]) < 1e-9
