@@ -147,6 +147,16 @@ int rt_product_memory(rt_product *product, uint64_t *stored, uint64_t *working);
 int rt_product_profile(rt_product *product, int64_t nrhs, int32_t reps, int32_t precision, double *ms,
                        double *bytes);
 
+/* Autotuner (no counterpart in the reference; the north star's FFT tile and radix schedule): times the launch shapes
+ * of the three passes for nrhs right-hand sides in the given precision -- plan radices up to 8 or 16, pass A/C
+ * rooftop columns per block, pass B x positions per block -- and keeps the fastest for every later application with
+ * that many right-hand sides. shape (may be NULL) receives {largest radix bits, columns per block, positions per
+ * block}; ms (may be NULL) the summed pass time of one application with it. */
+int rt_product_tune(rt_product *product, int64_t nrhs, int32_t precision, int32_t *shape, double *ms);
+/* Sets the launch shape used for nrhs right-hand sides in the given precision (a saved rt_product_tune result);
+ * RT_EINVAL when the shape is malformed or does not fit the device for this product. */
+int rt_product_set_shape(rt_product *product, int64_t nrhs, int32_t precision, const int32_t *shape);
+
 /* Measurement (no counterpart in the reference): while timing is on, every operator application records HIP events
  * around its three kernels on the context's stream. rt_ctx_kernel_timing drains them (synchronizing the stream) and
  * reports, per precision p (0: RT_C128, 1: RT_C64) and pass k, the launches counted, the average duration of one

@@ -228,6 +228,32 @@ class Operator(Product):
                                           _MODIFIERS[modifier], C.byref(h)))
         super().__init__(problem, h, _PRECISIONS[precision])
 
+    def tune(self, nrhs=1, precision=None):
+        """Times the launch shapes of the matvec passes for nrhs right-hand sides and keeps the fastest (the FFT
+        tile and radix schedule); returns ({'bits', 'group', 'tile'}, milliseconds of one application)."""
+        prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
+                                                         else self.precision)
+        shape = (C.c_int32 * 3)()
+        ms = C.c_double()
+        check(library().rt_product_tune(self.handle, nrhs, prec, shape, C.byref(ms)))
+        return {'bits': shape[0], 'group': shape[1], 'tile': shape[2]}, ms.value
+
+    def setShape(self, shape, nrhs=1, precision=None):
+        """Uses shape {'bits', 'group', 'tile'} for nrhs right-hand sides (a saved tune() result)."""
+        prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
+                                                         else self.precision)
+        arr = (C.c_int32 * 3)(shape['bits'], shape['group'], shape['tile'])
+        check(library().rt_product_set_shape(self.handle, nrhs, prec, arr))
+
+    def profile(self, nrhs=1, reps=20, precision=None):
+        """Average per-launch milliseconds and algorithmic bytes of the three passes (HIP events)."""
+        prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
+                                                         else self.precision)
+        ms = (C.c_double * 3)()
+        by = (C.c_double * 3)()
+        check(library().rt_product_profile(self.handle, nrhs, reps, prec, ms, by))
+        return list(ms), list(by)
+
 
 class Preconditioner(Product):
     """A preconditioner product: 'nearField' (3x3-cell neighbourhoods) or 'jacobi' (diag(Z)).

@@ -50,6 +50,8 @@ SIGNATURES = [
     ('rt_product_memory', C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ('rt_product_profile', C.c_int, [P, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
+    ('rt_product_tune', C.c_int, [P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    ('rt_product_set_shape', C.c_int, [P, C.c_int64, C.c_int32, C.POINTER(C.c_int32)]),
     ('rt_solve', C.c_int, [P, P, P, C.POINTER(rt_solve_opts), P, C.c_int64, C.c_int64, P, P,
                            C.POINTER(rt_output)]),
     ('rt_solve_device', C.c_int, [P, P, P, C.POINTER(rt_solve_opts), P, C.c_int64, C.c_int64, P, P,

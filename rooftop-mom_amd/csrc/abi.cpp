@@ -403,6 +403,30 @@ int rt_product_profile(rt_product *p, int64_t nrhs, int32_t reps, int32_t precis
     });
 }
 
+int rt_product_tune(rt_product *p, int64_t nrhs, int32_t precision, int32_t *shape, double *ms) {
+    return guarded([&] {
+        need(p && p->p && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product", "invalid tuning arguments");
+        Context *c = p->p->problem->ctx;
+        c->activate();
+        int sh[3] = {0, 0, 0};
+        double t = 0;
+        operator_tune(*p->p, static_cast<int>(nrhs), precision, sh, &t, c->stream);
+        if (shape)
+            for (int k = 0; k < 3; ++k) shape[k] = sh[k];
+        if (ms) *ms = t;
+    });
+}
+
+int rt_product_set_shape(rt_product *p, int64_t nrhs, int32_t precision, const int32_t *shape) {
+    return guarded([&] {
+        need(p && p->p && shape && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product", "invalid shape arguments");
+        Context *c = p->p->problem->ctx;
+        c->activate();
+        const int sh[3] = {shape[0], shape[1], shape[2]};
+        operator_set_shape(*p->p, static_cast<int>(nrhs), precision, sh);
+    });
+}
+
 int rt_solve_device(rt_ctx *ctx, rt_product *op, rt_product *pc, const rt_solve_opts *opts, const void *rhs,
                     int64_t n, int64_t nrhs, const void *x0, void *x, rt_output *out) {
     return guarded([&] {

@@ -1,12 +1,24 @@
 // The Toeplitz product of the rooftop Green's matrix on gfx950: the three generators embedded evenly into an
 // Mx x My circulant (period >= 2n-1 per split direction), their spectra folded by their even symmetry, and the
-// matvec as three LDS-staged FFT passes with the spectral multiply fused into the middle one.
+// matvec as three FFT passes with the spectral multiply fused into the middle one.
 //
 // Reference: the products of src/+rooftopMoM/+assembler/+greensMatrix/FFTApplicator.m:14-87 and
 // @SplitFFTApplicator/applyBothDirections_.m:14-85 with the block spectra of +utils/generateBlockSpectra.m:19-50.
 // With Dx = (1 - sx)/a, Dy = (1 - sy)/b the four block spectra collapse into three planes:
 //     out_x = Mxx X - Dx E (conj(Dx) X + conj(Dy) Y),   out_y = Myy Y - Dy E (conj(Dx) X + conj(Dy) Y),
 // the x and y current spectra X, Y and the face-charge spectrum conj(Dx) X + conj(Dy) Y ("Jx, Jy and charge").
+//
+// Passes (every transform is a team FFT, fft.hpp):
+//   A  forward x transform of `group` rooftop columns of one component: the first round reads the rooftop vector
+//      (threads along x, coalesced), the last round writes the work array with the threads along the columns, so
+//      the transposition costs no pass of its own;
+//   B  per x position: forward y transforms of both components, the three-plane multiply in registers between the
+//      last forward round and the first inverse round, inverse y transforms; contiguous work columns;
+//   C  inverse x transform of `group` columns, the mirror of A, cut to the rooftop rows.
+// Blocks are numbered XCD-major (blockIdx b runs on XCD b mod 8): consecutive logical blocks -- adjacent column
+// groups in A/C, the right-hand sides of one x position in B -- share an L2, so their partial cache lines of the
+// work array and their spectrum rows meet there.
+#include <algorithm>
 #include <cmath>
 
 #include "rooftop.hpp"
@@ -17,54 +29,63 @@ namespace rt {
 // Tables.
 // ------------------------------------------------------------------------------------------------------------------
 void DimTables::build(Context *ctx, int M, double size, hipStream_t s) {
-    plan = make_plan(M);
-    std::vector<c128> h_tw(M), h_d(M);
-    std::vector<c64> h_tw32(M), h_d32(M);
-    std::vector<int> h_fold(M);
     const long double pi = 3.141592653589793238462643383279502884L;
-    for (int t = 0; t < M; ++t) {
-        long double ang = -2.0L * pi * static_cast<long double>(t) / static_cast<long double>(M);
-        h_tw[t] = {static_cast<double>(std::cos(ang)), static_cast<double>(std::sin(ang))};
-        h_tw32[t] = {static_cast<float>(h_tw[t].x), static_cast<float>(h_tw[t].y)};
+    for (int v = 0; v < 2; ++v) {
+        PlanTables &t = p[v];
+        t.plan = make_plan(M, v == 0 ? 3 : 4);
+        const std::vector<c128> h_tw = plan_twiddles(t.plan);
+        std::vector<c64> h_tw32(h_tw.size());
+        for (size_t i = 0; i < h_tw.size(); ++i) h_tw32[i] = cvt<float>(h_tw[i]);
+        std::vector<c128> h_d(M);
+        std::vector<c64> h_d32(M);
+        std::vector<int> h_fold(M);
+        const std::vector<int> freq = output_frequencies(t.plan);
+        for (int q = 0; q < M; ++q) {
+            const int k = freq[q];
+            h_fold[q] = k <= M - k ? k : M - k;
+            const long double ang = 2.0L * pi * static_cast<long double>(k) / static_cast<long double>(M);
+            // D = (1 - exp(i ang)) / size; 1 - cos = 2 sin^2(ang/2) keeps low frequencies accurate.
+            const long double sh = std::sin(ang / 2);
+            h_d[q] = {static_cast<double>(2.0L * sh * sh / size), static_cast<double>(-std::sin(ang) / size)};
+            h_d32[q] = cvt<float>(h_d[q]);
+        }
+        t.tw.allocate(ctx, h_tw.size(), "FFT twiddles");
+        t.tw32.allocate(ctx, h_tw.size(), "FFT twiddles");
+        t.fold.allocate(ctx, M, "FFT fold table");
+        t.d.allocate(ctx, M, "divergence table");
+        t.d32.allocate(ctx, M, "divergence table");
+        RT_HIP(hipMemcpyAsync(t.tw.ptr, h_tw.data(), h_tw.size() * sizeof(c128), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(t.tw32.ptr, h_tw32.data(), h_tw.size() * sizeof(c64), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(t.fold.ptr, h_fold.data(), M * sizeof(int), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(t.d.ptr, h_d.data(), M * sizeof(c128), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(t.d32.ptr, h_d32.data(), M * sizeof(c64), hipMemcpyHostToDevice, s));
+        RT_HIP(hipStreamSynchronize(s));
     }
-    std::vector<int> freq = output_frequencies(plan);
-    for (int p = 0; p < M; ++p) {
-        int k = freq[p];
-        h_fold[p] = k <= M - k ? k : M - k;
-        long double ang = 2.0L * pi * static_cast<long double>(k) / static_cast<long double>(M);
-        // D = (1 - exp(i ang)) / size; 1 - cos = 2 sin^2(ang/2) keeps low frequencies accurate.
-        long double sh = std::sin(ang / 2);
-        h_d[p] = {static_cast<double>(2.0L * sh * sh / size), static_cast<double>(-std::sin(ang) / size)};
-        h_d32[p] = {static_cast<float>(h_d[p].x), static_cast<float>(h_d[p].y)};
-    }
-    tw.allocate(ctx, M, "FFT twiddles");
-    tw32.allocate(ctx, M, "FFT twiddles");
-    fold.allocate(ctx, M, "FFT fold table");
-    d.allocate(ctx, M, "divergence table");
-    d32.allocate(ctx, M, "divergence table");
-    RT_HIP(hipMemcpyAsync(tw.ptr, h_tw.data(), M * sizeof(c128), hipMemcpyHostToDevice, s));
-    RT_HIP(hipMemcpyAsync(tw32.ptr, h_tw32.data(), M * sizeof(c64), hipMemcpyHostToDevice, s));
-    RT_HIP(hipMemcpyAsync(fold.ptr, h_fold.data(), M * sizeof(int), hipMemcpyHostToDevice, s));
-    RT_HIP(hipMemcpyAsync(d.ptr, h_d.data(), M * sizeof(c128), hipMemcpyHostToDevice, s));
-    RT_HIP(hipMemcpyAsync(d32.ptr, h_d32.data(), M * sizeof(c64), hipMemcpyHostToDevice, s));
-    RT_HIP(hipStreamSynchronize(s));
 }
 
-template <typename T> const cplx<T> *tw_of(const DimTables &t);
-template <> const c128 *tw_of<double>(const DimTables &t) { return t.tw.ptr; }
-template <> const c64 *tw_of<float>(const DimTables &t) { return t.tw32.ptr; }
-template <typename T> const cplx<T> *d_of(const DimTables &t);
-template <> const c128 *d_of<double>(const DimTables &t) { return t.d.ptr; }
-template <> const c64 *d_of<float>(const DimTables &t) { return t.d32.ptr; }
+template <typename T> const cplx<T> *tw_of(const PlanTables &t);
+template <> const c128 *tw_of<double>(const PlanTables &t) { return t.tw.ptr; }
+template <> const c64 *tw_of<float>(const PlanTables &t) { return t.tw32.ptr; }
+template <typename T> const cplx<T> *d_of(const PlanTables &t);
+template <> const c128 *d_of<double>(const PlanTables &t) { return t.d.ptr; }
+template <> const c64 *d_of<float>(const PlanTables &t) { return t.d32.ptr; }
 
-// Work array of the passes, per right-hand side: [component][x position (digit-reversed)][y column], the y columns
-// of one x position contiguous (row length ldj >= the columns of either component). Pass A writes it transposed in
-// 64-byte row segments, pass B reads and writes whole contiguous columns, pass C reads it back in row segments.
+// Work array of the passes, per right-hand side: [component][x position (plan order)][y column], the y columns of
+// one x position contiguous (row length ldj >= the columns of either component).
 __host__ __device__ inline int64_t work_index(const ProdDims &d, int r, int comp, int kx, int j) {
     return ((static_cast<int64_t>(r) * 2 + comp) * d.Mx + kx) * d.ldj + j;
 }
 
-// Decodes a pass-A/C block into (right-hand side, component, first y column).
+// Logical block of hardware block b in a grid of 8 * ceil(nblocks / 8): XCD b mod 8 runs the logical blocks
+// [(b mod 8) per, (b mod 8 + 1) per) in order. -1 for the padding blocks.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    const int per = (nblocks + 7) >> 3;
+    const int lb = (b & 7) * per + (b >> 3);
+    return lb < nblocks ? lb : -1;
+}
+static unsigned xcd_grid(int64_t nblocks) { return static_cast<unsigned>((nblocks + 7) / 8 * 8); }
+
+// Decodes a pass-A/C logical block into (right-hand side, component, first rooftop column).
 __device__ inline void group_of(const ProdDims &d, int group, int blk, int &r, int &comp, int &j0) {
     const int gx = (d.cols_x + group - 1) / group, gy = (d.cols_y + group - 1) / group;
     const int per = gx + gy;
@@ -75,136 +96,283 @@ __device__ inline void group_of(const ProdDims &d, int group, int blk, int &r, i
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Pass A: forward FFT along x of `group` consecutive rooftop columns of one component (zero-padded from their rows
-// to Mx), written transposed into the work array.
+// Pass A: forward x transforms of `group` consecutive rooftop columns of one component (zero-padded from their rows
+// to Mx), written into the work array. LDS holds [x position (padded)][column].
 // ------------------------------------------------------------------------------------------------------------------
-template <typename TC, typename TIO>
-__global__ void __launch_bounds__(256) pass_a(const cplx<TIO> *__restrict__ in, cplx<TC> *__restrict__ work,
-                                              const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
-                                              const cplx<TC> *__restrict__ tw, int group,
-                                              const int *__restrict__ skip, int skip_stride) {
+template <typename TC, int E, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_a(const c128 *__restrict__ in, cplx<TC> *__restrict__ work,
+                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan p,
+                                               const cplx<TC> *__restrict__ tw, int G, int nblocks,
+                                               const int *__restrict__ skip, int skip_stride) {
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    int r, comp, j0;
+    group_of(d, G, lb, r, comp, j0);
+    if (skip && skip[r * skip_stride]) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
-    const int M = d.Mx;
-    const int stride = M + 1;
-    int r, comp, j0;
-    group_of(d, group, blockIdx.x, r, comp, j0);
-    if (skip && skip[r * skip_stride]) return;
     const int cols = comp ? d.cols_y : d.cols_x;
     const int rows = comp ? d.rows_y : d.rows_x;
-    const int ncols = min(group, cols - j0);
+    const int ncols = min(G, cols - j0);
+    const int T = team_threads(p);
     const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
-    for (int e = threadIdx.x; e < ncols * M; e += blockDim.x) {
-        const int c = e / M, i = e - c * M;
-        cplx<TC> v = mk<TC>(0, 0);
-        if (i < rows) {
-            const int64_t idx = static_cast<int64_t>(j0 + c) * rows + i;
-            if (!exists || exists[(comp ? d.nx_count : 0) + idx]) v = cvt<TC>(in[off + idx]);
-        }
-        buf[c * stride + i] = v;
+    const uint8_t *ex = exists ? exists + (comp ? d.nx_count : 0) : nullptr;
+    const int K = p.K;
+    auto lidx = [&](int pos, int c) { return padded(p, pos) * G + c; };
+    {   // Round 0, threads along x: reads the rooftop rows of column c.
+        const int c = threadIdx.x / T, t = threadIdx.x - c * T;
+        const bool live = c < ncols;
+        const int64_t col = static_cast<int64_t>(j0 + c) * rows;
+        team_round<TC, E, false, true>(
+            p, 0, t, tw, K > 1,
+            [&](int pos) {
+                if (!live || pos >= rows) return mk<TC>(0, 0);
+                if (ex && !ex[col + pos]) return mk<TC>(0, 0);
+                return cvt<TC>(in[off + col + pos]);
+            },
+            [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
     }
     __syncthreads();
-    lds_fft<TC, false>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
-    for (int e = threadIdx.x; e < ncols * M; e += blockDim.x) {
-        const int kx = e / ncols, c = e - kx * ncols;
-        work[work_index(d, r, comp, kx, j0 + c)] = buf[c * stride + kx];
+    // Rounds 1 .. K-1, threads along the columns; the last one writes the work array.
+    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    for (int s = 1; s < K - 1; ++s) {
+        team_round<TC, E, false, false>(
+            p, s, t, tw, true, [&](int pos) { return buf[lidx(pos, c)]; },
+            [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
+        __syncthreads();
+    }
+    if (K > 1) {
+        team_round<TC, E, false, false>(
+            p, K - 1, t, tw, false, [&](int pos) { return buf[lidx(pos, c)]; },
+            [&](int pos, cplx<TC> v) {
+                if (c < ncols) work[work_index(d, r, comp, pos, j0 + c)] = v;
+            });
+    } else if (c < ncols) {
+        for (int pos = t; pos < p.M; pos += T) work[work_index(d, r, comp, pos, j0 + c)] = buf[lidx(pos, c)];
     }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Pass C: inverse FFT along x of `group` consecutive columns read back from the work array; the first rows of each
+// Pass C: inverse x transforms of `group` consecutive columns read back from the work array; the first rows of each
 // column are the product (the cut of FFTApplicator.m:80-84).
 // ------------------------------------------------------------------------------------------------------------------
-template <typename TC, typename TIO>
-__global__ void __launch_bounds__(256) pass_c(const cplx<TC> *__restrict__ work, cplx<TIO> *__restrict__ out,
-                                              const uint8_t *__restrict__ exists, ProdDims d, FftPlan plan,
-                                              const cplx<TC> *__restrict__ tw, int group,
-                                              const int *__restrict__ skip, int skip_stride) {
+template <typename TC, int E, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_c(const cplx<TC> *__restrict__ work, c128 *__restrict__ out,
+                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan p,
+                                               const cplx<TC> *__restrict__ tw, int G, int nblocks,
+                                               const int *__restrict__ skip, int skip_stride) {
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    int r, comp, j0;
+    group_of(d, G, lb, r, comp, j0);
+    if (skip && skip[r * skip_stride]) return;
     extern __shared__ unsigned char smem_raw[];
     cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
-    const int M = d.Mx;
-    const int stride = M + 1;
-    int r, comp, j0;
-    group_of(d, group, blockIdx.x, r, comp, j0);
-    if (skip && skip[r * skip_stride]) return;
     const int cols = comp ? d.cols_y : d.cols_x;
     const int rows = comp ? d.rows_y : d.rows_x;
-    const int ncols = min(group, cols - j0);
-    for (int e = threadIdx.x; e < ncols * M; e += blockDim.x) {
-        const int kx = e / ncols, c = e - kx * ncols;
-        buf[c * stride + kx] = work[work_index(d, r, comp, kx, j0 + c)];
+    const int ncols = min(G, cols - j0);
+    const int T = team_threads(p);
+    const int K = p.K;
+    auto lidx = [&](int pos, int c) { return padded(p, pos) * G + c; };
+    {
+        const int c = threadIdx.x % G, t = threadIdx.x / G;
+        if (K > 1) {
+            // Round K-1 (no twiddles), threads along the columns, read from the work array.
+            team_round<TC, E, true, false>(
+                p, K - 1, t, tw, false,
+                [&](int pos) { return c < ncols ? work[work_index(d, r, comp, pos, j0 + c)] : mk<TC>(0, 0); },
+                [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
+            __syncthreads();
+            for (int s = K - 2; s >= 1; --s) {
+                team_round<TC, E, true, false>(
+                    p, s, t, tw, true, [&](int pos) { return buf[lidx(pos, c)]; },
+                    [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
+                __syncthreads();
+            }
+        } else {
+            for (int pos = t; pos < p.M; pos += T)
+                buf[lidx(pos, c)] = c < ncols ? work[work_index(d, r, comp, pos, j0 + c)] : mk<TC>(0, 0);
+            __syncthreads();
+        }
     }
-    __syncthreads();
-    lds_fft<TC, true>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
+    // Round 0, threads along x: writes the rooftop rows of column c.
+    const int c = threadIdx.x / T, t = threadIdx.x - c * T;
+    const bool live = c < ncols;
     const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
-    for (int e = threadIdx.x; e < ncols * rows; e += blockDim.x) {
-        const int c = e / rows, i = e - c * rows;
-        const int64_t idx = static_cast<int64_t>(j0 + c) * rows + i;
-        cplx<TC> v = buf[c * stride + i];
-        if (exists && !exists[(comp ? d.nx_count : 0) + idx]) v = mk<TC>(0, 0);
-        out[off + idx] = cvt<TIO>(v);
-    }
+    const int64_t col = static_cast<int64_t>(j0 + c) * rows;
+    const uint8_t *ex = exists ? exists + (comp ? d.nx_count : 0) : nullptr;
+    team_round<TC, E, true, false>(
+        p, 0, t, tw, K > 1, [&](int pos) { return buf[lidx(pos, c)]; },
+        [&](int pos, cplx<TC> v) {
+            if (!live || pos >= rows) return;
+            if (ex && !ex[col + pos]) v = mk<TC>(0, 0);
+            out[off + col + pos] = cvt<double>(v);
+        });
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Pass B: for `tile` consecutive x positions, forward FFT along y of both components (contiguous columns,
-// zero-padded to My), the three-plane spectral multiply, inverse FFT along y, and the cut back to the columns.
-// LDS holds [component][tile][My (+1 pad)].
+// Pass B: for `tile` consecutive x positions, the forward y transforms of both components (contiguous columns,
+// zero-padded to My), the three-plane spectral multiply, the inverse y transforms and the cut back to the columns.
+// A team of T threads owns one x position and both of its columns; LDS holds [position][component][y (padded)].
 // ------------------------------------------------------------------------------------------------------------------
-template <typename TC>
-__global__ void __launch_bounds__(256) pass_b(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ planes,
-                                              const int *__restrict__ fold_x, const int *__restrict__ fold_y,
-                                              const cplx<TC> *__restrict__ dx, const cplx<TC> *__restrict__ dy,
-                                              ProdDims d, FftPlan plan, const cplx<TC> *__restrict__ tw, int tile,
-                                              const int *__restrict__ skip, int skip_stride) {
-    const int r = blockIdx.y;
+template <typename TC, int E, int R>
+__device__ __forceinline__ void multiply_round(const FftPlan &p, int t, int px, cplx<TC> *col0, cplx<TC> *col1,
+                                               cplx<TC> *__restrict__ w0, cplx<TC> *__restrict__ w1,
+                                               int n0, int n1, bool direct, const cplx<TC> *__restrict__ planes,
+                                               int64_t plane, int Hy, int fx, const int *__restrict__ fold_y,
+                                               cplx<TC> dxv, const cplx<TC> *__restrict__ dy) {
+    constexpr int U = E / R;
+    const int T = p.M >> p.log2E;
+    const cplx<TC> *rowp = planes + static_cast<int64_t>(fx) * Hy;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int base = (t + u * T) * R;   // the last round has stride 1: positions base + q
+        cplx<TC> X[R], Y[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int pos = base + q;
+            if (direct) {
+                X[q] = (R == 1 || q < R / 2) && pos < n0 ? w0[pos] : mk<TC>(0, 0);
+                Y[q] = (R == 1 || q < R / 2) && pos < n1 ? w1[pos] : mk<TC>(0, 0);
+            } else {
+                X[q] = col0[padded(p, pos)];
+                Y[q] = col1[padded(p, pos)];
+            }
+        }
+        dft<TC, R, false, false>(X);
+        dft<TC, R, false, false>(Y);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int py = base + q;
+            const int f = fold_y[py];
+            const cplx<TC> a = rowp[f], b = rowp[plane + f], el = rowp[2 * plane + f];
+            const cplx<TC> dyv = dy[py];
+            const cplx<TC> phi = el * (cmulc(dxv, X[q]) + cmulc(dyv, Y[q]));
+            X[q] = a * X[q] - dxv * phi;
+            Y[q] = b * Y[q] - dyv * phi;
+        }
+        dft<TC, R, true, false>(X);
+        dft<TC, R, true, false>(Y);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int pos = base + q;
+            if (direct) {
+                if (pos < n0) w0[pos] = X[q];
+                if (pos < n1) w1[pos] = Y[q];
+            } else {
+                col0[padded(p, pos)] = X[q];
+                col1[padded(p, pos)] = Y[q];
+            }
+        }
+    }
+}
+
+template <typename TC, int E, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_b(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ planes,
+                                               const int *__restrict__ fold_x, const int *__restrict__ fold_y,
+                                               const cplx<TC> *__restrict__ dx, const cplx<TC> *__restrict__ dy,
+                                               ProdDims d, FftPlan p, const cplx<TC> *__restrict__ tw, int tile,
+                                               int R, int nblocks, const int *__restrict__ skip, int skip_stride) {
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    const int r = lb % R, pt = lb / R;
     if (skip && skip[r * skip_stride]) return;
     extern __shared__ unsigned char smem_raw[];
-    cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
-    const int My = d.My;
-    const int stride = My + 1;
-    const int px0 = blockIdx.x * tile;
-    const int ncols = 2 * tile;
-    for (int e = threadIdx.x; e < ncols * My; e += blockDim.x) {
-        const int col = e / My, j = e - col * My;
-        const int comp = col / tile, t = col - comp * tile;
-        const int cols = comp ? d.cols_y : d.cols_x;
-        cplx<TC> v = mk<TC>(0, 0);
-        if (j < cols) v = work[work_index(d, r, comp, px0 + t, j)];
-        buf[col * stride + j] = v;
-    }
-    __syncthreads();
-    lds_fft<TC, false>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
+    const int T = team_threads(p);
+    const int tt = threadIdx.x / T, t = threadIdx.x - tt * T;
+    const int px = pt * tile + tt;
+    const int Mp = padded_length(p);
+    cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + static_cast<int64_t>(2 * tt) * Mp;
+    cplx<TC> *col1 = col0 + Mp;
+    cplx<TC> *w0 = work + work_index(d, r, 0, px, 0);
+    cplx<TC> *w1 = work + work_index(d, r, 1, px, 0);
+    const int n0 = d.cols_x, n1 = d.cols_y;
     const int64_t plane = static_cast<int64_t>(d.Hx) * d.Hy;
-    for (int e = threadIdx.x; e < tile * My; e += blockDim.x) {
-        const int t = e / My, py = e - t * My;
-        const int px = px0 + t;
-        const int64_t f = static_cast<int64_t>(fold_x[px]) * d.Hy + fold_y[py];
-        const cplx<TC> a = planes[f], b = planes[plane + f], el = planes[2 * plane + f];
-        const cplx<TC> dxv = dx[px], dyv = dy[py];
-        const cplx<TC> X = buf[t * stride + py];
-        const cplx<TC> Y = buf[(tile + t) * stride + py];
-        const cplx<TC> phi = el * (cmulc(dxv, X) + cmulc(dyv, Y));
-        buf[t * stride + py] = a * X - dxv * phi;
-        buf[(tile + t) * stride + py] = b * Y - dyv * phi;
+    const int fx = fold_x[px];
+    const cplx<TC> dxv = dx[px];
+    const int K = p.K;
+    auto middle = [&](bool direct) {
+        switch (p.lr[K - 1]) {
+        case 0:
+            multiply_round<TC, E, 1>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx, fold_y,
+                                     dxv, dy);
+            break;
+        case 1:
+            if constexpr (E >= 2)
+                multiply_round<TC, E, 2>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
+                                         fold_y, dxv, dy);
+            break;
+        case 2:
+            if constexpr (E >= 4)
+                multiply_round<TC, E, 4>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
+                                         fold_y, dxv, dy);
+            break;
+        case 3:
+            if constexpr (E >= 8)
+                multiply_round<TC, E, 8>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
+                                         fold_y, dxv, dy);
+            break;
+        default:
+            if constexpr (E >= 16)
+                multiply_round<TC, E, 16>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
+                                          fold_y, dxv, dy);
+            break;
+        }
+    };
+    if (K == 1) {   // one round: straight from the work array and back
+        middle(true);
+        return;
     }
+    // Forward round 0 from the work array (the upper half of the embedding is zero).
+    team_round<TC, E, false, true>(
+        p, 0, t, tw, true, [&](int pos) { return pos < n0 ? w0[pos] : mk<TC>(0, 0); },
+        [&](int pos, cplx<TC> v) { col0[padded(p, pos)] = v; });
+    team_round<TC, E, false, true>(
+        p, 0, t, tw, true, [&](int pos) { return pos < n1 ? w1[pos] : mk<TC>(0, 0); },
+        [&](int pos, cplx<TC> v) { col1[padded(p, pos)] = v; });
     __syncthreads();
-    lds_fft<TC, true>(buf, ncols, stride, plan, tw, threadIdx.x, blockDim.x);
-    for (int e = threadIdx.x; e < ncols * My; e += blockDim.x) {
-        const int col = e / My, j = e - col * My;
-        const int comp = col / tile, t = col - comp * tile;
-        const int cols = comp ? d.cols_y : d.cols_x;
-        if (j < cols) work[work_index(d, r, comp, px0 + t, j)] = buf[col * stride + j];
+    for (int s = 1; s < K - 1; ++s) {
+        team_round<TC, E, false, false>(
+            p, s, t, tw, true, [&](int pos) { return col0[padded(p, pos)]; },
+            [&](int pos, cplx<TC> v) { col0[padded(p, pos)] = v; });
+        team_round<TC, E, false, false>(
+            p, s, t, tw, true, [&](int pos) { return col1[padded(p, pos)]; },
+            [&](int pos, cplx<TC> v) { col1[padded(p, pos)] = v; });
+        __syncthreads();
     }
+    middle(false);
+    __syncthreads();
+    for (int s = K - 2; s >= 1; --s) {
+        team_round<TC, E, true, false>(
+            p, s, t, tw, true, [&](int pos) { return col0[padded(p, pos)]; },
+            [&](int pos, cplx<TC> v) { col0[padded(p, pos)] = v; });
+        team_round<TC, E, true, false>(
+            p, s, t, tw, true, [&](int pos) { return col1[padded(p, pos)]; },
+            [&](int pos, cplx<TC> v) { col1[padded(p, pos)] = v; });
+        __syncthreads();
+    }
+    // Inverse round 0 into the work array, cut to the columns.
+    team_round<TC, E, true, false>(
+        p, 0, t, tw, true, [&](int pos) { return col0[padded(p, pos)]; },
+        [&](int pos, cplx<TC> v) {
+            if (pos < n0) w0[pos] = v;
+        });
+    team_round<TC, E, true, false>(
+        p, 0, t, tw, true, [&](int pos) { return col1[padded(p, pos)]; },
+        [&](int pos, cplx<TC> v) {
+            if (pos < n1) w1[pos] = v;
+        });
 }
 
 // ------------------------------------------------------------------------------------------------------------------
 // Spectra: the even embedding of a generator (G(p mod Mx, q mod My) = g(|p|, |q|) for |p| < rows, |q| < cols), its
-// forward 2D FFT, and the fold to [0, Mx/2] x [0, My/2] (an even real-index circulant has an even spectrum).
+// forward 2D FFT, and the fold to [0, Mx/2] x [0, My/2] (an even real-index circulant has an even spectrum). One
+// block (one team) per column; plans with radices up to 8.
 // ------------------------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) spectra_a(const c128 *__restrict__ g, int rows, int cols, int modifier,
-                                                 c128 *__restrict__ out, int Mx, int My, FftPlan plan,
-                                                 const c128 *__restrict__ tw) {
+template <int E>
+__global__ void __launch_bounds__(1024) spectra_a(const c128 *__restrict__ g, int rows, int cols, int modifier,
+                                                  c128 *__restrict__ out, int Mx, int My, FftPlan plan,
+                                                  const c128 *__restrict__ tw) {
     extern __shared__ unsigned char smem_raw[];
     c128 *buf = reinterpret_cast<c128 *>(smem_raw);
     const int q = blockIdx.x;
@@ -217,29 +385,29 @@ __global__ void __launch_bounds__(256) spectra_a(const c128 *__restrict__ g, int
             if (modifier == RT_MODIFIER_REAL) v.y = 0;
             if (modifier == RT_MODIFIER_IMAG) v = mk(v.y, 0.0);
         }
-        buf[i] = v;
+        buf[padded(plan, i)] = v;
     }
     __syncthreads();
-    lds_fft<double, false>(buf, 1, Mx, plan, tw, threadIdx.x, blockDim.x);
-    for (int i = threadIdx.x; i < Mx; i += blockDim.x) out[static_cast<int64_t>(q) * Mx + i] = buf[i];
+    team_fft_lds<double, E, false>(buf, plan, tw, threadIdx.x);
+    for (int i = threadIdx.x; i < Mx; i += blockDim.x) out[static_cast<int64_t>(q) * Mx + i] = buf[padded(plan, i)];
 }
 
-__global__ void __launch_bounds__(256) spectra_b(const c128 *__restrict__ in, c128 *__restrict__ plane, int Mx,
-                                                 int My, int Hy, FftPlan plan, const c128 *__restrict__ tw,
-                                                 const int *__restrict__ fold_x, const int *__restrict__ fold_y,
-                                                 const int *__restrict__ freq_x, const int *__restrict__ freq_y,
-                                                 double scale) {
+template <int E>
+__global__ void __launch_bounds__(1024) spectra_b(const c128 *__restrict__ in, c128 *__restrict__ plane, int Mx,
+                                                  int My, int Hy, FftPlan plan, const c128 *__restrict__ tw,
+                                                  const int *__restrict__ freq_x, const int *__restrict__ freq_y,
+                                                  double scale) {
     extern __shared__ unsigned char smem_raw[];
     c128 *buf = reinterpret_cast<c128 *>(smem_raw);
     const int px = blockIdx.x;
-    for (int q = threadIdx.x; q < My; q += blockDim.x) buf[q] = in[static_cast<int64_t>(q) * Mx + px];
+    for (int q = threadIdx.x; q < My; q += blockDim.x) buf[padded(plan, q)] = in[static_cast<int64_t>(q) * Mx + px];
     __syncthreads();
-    lds_fft<double, false>(buf, 1, My, plan, tw, threadIdx.x, blockDim.x);
+    team_fft_lds<double, E, false>(buf, plan, tw, threadIdx.x);
     const int kx = freq_x[px];
     if (2 * kx > Mx) return;
     for (int py = threadIdx.x; py < My; py += blockDim.x) {
         const int ky = freq_y[py];
-        if (2 * ky <= My) plane[static_cast<int64_t>(fold_x[px]) * Hy + fold_y[py]] = scale * buf[py];
+        if (2 * ky <= My) plane[static_cast<int64_t>(kx) * Hy + ky] = scale * buf[padded(plan, py)];
     }
 }
 
@@ -252,49 +420,81 @@ __global__ void to_c64(const c128 *__restrict__ a, c64 *__restrict__ b, int64_t 
 // ------------------------------------------------------------------------------------------------------------------
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
-// Default launch shape (the autotuner may replace it). Pass A/C: `group` y columns per block, 64-byte row segments
-// when enough blocks remain (>= 256), fewer columns otherwise. Pass B: one x position per block unless the grid
-// would be very large.
+static size_t lds_bytes_max() {
+    static size_t cached = 0;
+    if (!cached) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        cached = static_cast<size_t>(v);
+    }
+    return cached;
+}
+
+// Raises a kernel's dynamic LDS limit above the 64 KiB default (once per kernel and size).
+static void set_lds(const void *kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    if (bytes > lds_bytes_max())
+        fail(RT_EINVAL, "rooftop:product", "a pass needs %zu bytes of LDS, more than this device has", bytes);
+    static std::mutex lock;
+    static std::unordered_map<const void *, size_t> raised;
+    std::lock_guard<std::mutex> g(lock);
+    size_t &have = raised[kernel];
+    if (have >= bytes) return;
+    RT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    have = bytes;
+}
+
 template <typename TC>
-static PassShape default_shape(const ProdDims &d, int nrhs) {
+static size_t lds_ac(const FftPlan &p, int group) {
+    return static_cast<size_t>(padded_length(p)) * group * sizeof(cplx<TC>);
+}
+template <typename TC>
+static size_t lds_b(const FftPlan &p, int tile) {
+    return static_cast<size_t>(2) * tile * padded_length(p) * sizeof(cplx<TC>);
+}
+static int64_t blocks_ac(const ProdDims &d, int nrhs, int g) {
+    return static_cast<int64_t>(nrhs) * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g);
+}
+
+// Whether a shape fits the device: threads per block and LDS.
+template <typename TC>
+static bool shape_fits(const Product &op, const PassShape &sh) {
+    const FftPlan &px = op.tx.of(sh.bits).plan, &py = op.ty.of(sh.bits).plan;
+    const size_t lmax = lds_bytes_max();
+    return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx && team_threads(px) * sh.group <= 1024 &&
+           team_threads(py) * sh.tile <= 1024 && lds_ac<TC>(px, sh.group) <= lmax && lds_b<TC>(py, sh.tile) <= lmax;
+}
+
+// Default launch shape (the autotuner may replace it): radices up to 16 in complex64 and up to 8 in complex128;
+// pass A/C groups of 64-byte column segments while the blocks stay numerous; pass B teams of up to 256 threads.
+template <typename TC>
+static PassShape default_shape(const Product &op, int nrhs, int bits = 0) {
+    const ProdDims &d = op.dims;
     PassShape s;
+    s.bits = bits ? bits : sizeof(TC) == sizeof(float) ? 4 : 3;
+    const FftPlan &px = op.tx.of(s.bits).plan, &py = op.ty.of(s.bits).plan;
     const int es = sizeof(cplx<TC>);
-    int group = 64 / es;
-    const int lds_cols = static_cast<int>((64 * 1024) / ((d.Mx + 1) * es));
-    auto blocks_ac = [&](int g) {
-        return static_cast<int64_t>(nrhs) * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g);
-    };
-    while (group > 1 && (group > lds_cols || blocks_ac(group) < 256)) group >>= 1;
-    s.cols_per_block = group;
-    const int thr = group * d.Mx / 4;
-    s.threads_ac = thr < 64 ? 64 : thr > 256 ? 256 : (thr + 63) / 64 * 64;
+    int g = std::max(1, 64 / es);
+    while (g > 1 && (team_threads(px) * g > 1024 || lds_ac<TC>(px, g) > 64 * 1024 || blocks_ac(d, nrhs, g) < 1024))
+        g >>= 1;
+    s.group = g;
     int tile = 1;
-    const int lds_tile = static_cast<int>((80 * 1024) / (2 * (d.My + 1) * es));
-    while (tile * 2 <= 16 && tile * 2 <= lds_tile && tile * 2 <= d.Mx &&
-           static_cast<int64_t>(d.Mx / (tile * 2)) * nrhs >= 8192)
+    while (tile * 2 <= d.Mx && team_threads(py) * tile * 2 <= 256 && lds_b<TC>(py, tile * 2) <= 48 * 1024 &&
+           static_cast<int64_t>(d.Mx / (tile * 2)) * nrhs >= 2048)
         tile *= 2;
     s.tile = tile;
-    const int tb = 2 * tile * d.My / 4;
-    s.threads_b = tb < 64 ? 64 : tb > 256 ? 256 : (tb + 63) / 64 * 64;
     return s;
 }
 
-static size_t lds_bytes_max() {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
-        v = 64 * 1024;
-    return static_cast<size_t>(v);
-}
-
 template <typename TC>
-static void set_lds(const void *kernel, size_t bytes) {
-    if (bytes > 64 * 1024) {
-        if (bytes > lds_bytes_max())
-            fail(RT_EINVAL, "rooftop:product", "an FFT column of %zu bytes exceeds the LDS of this device", bytes);
-        RT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-    }
+static PassShape shape_for(const Product &op, int nrhs) {
+    const auto &tuned = sizeof(TC) == sizeof(float) ? op.tuned32 : op.tuned64;
+    for (const auto &e : tuned)
+        if (e.first == nrhs) return e.second;
+    return default_shape<TC>(op, nrhs);
 }
 
 // Algorithmic HBM bytes of one launch of each pass: every vector element, work element and spectrum value once.
@@ -308,19 +508,65 @@ static void pass_bytes(const ProdDims &d, int nrhs, double *bytes) {
     bytes[2] = work_bytes + vec;
 }
 
+// Kernels are instantiated per largest radix E of the plan and per bound MAXT on the block size (the register
+// budget of a thread grows as MAXT shrinks).
+#define RT_BY_E(E_VALUE, CALL)                      \
+    switch (E_VALUE) {                              \
+    case 1: { constexpr int E = 1; CALL; } break;   \
+    case 2: { constexpr int E = 2; CALL; } break;   \
+    case 4: { constexpr int E = 4; CALL; } break;   \
+    case 8: { constexpr int E = 8; CALL; } break;   \
+    default: { constexpr int E = 16; CALL; } break; \
+    }
+#define RT_BY_MAXT(THREADS, CALL)                                   \
+    if ((THREADS) <= 256) { constexpr int MAXT = 256; CALL; }       \
+    else if ((THREADS) <= 512) { constexpr int MAXT = 512; CALL; }  \
+    else { constexpr int MAXT = 1024; CALL; }
+
+template <typename TC, int E, int MAXT>
+static void launch_a(const Product &op, const PlanTables &t, const c128 *in, cplx<TC> *work, int nrhs, int g,
+                     hipStream_t s, const int *skip, int skip_stride) {
+    const FftPlan &p = t.plan;
+    const int64_t nb = blocks_ac(op.dims, nrhs, g);
+    const size_t lds = lds_ac<TC>(p, g);
+    set_lds(reinterpret_cast<const void *>(&pass_a<TC, E, MAXT>), lds);
+    hipLaunchKernelGGL((pass_a<TC, E, MAXT>), dim3(xcd_grid(nb)), dim3(team_threads(p) * g), lds, s, in, work,
+                       op.problem->exists.ptr, op.dims, p, tw_of<TC>(t), g, static_cast<int>(nb), skip, skip_stride);
+    RT_LAUNCH_CHECK();
+}
+
+template <typename TC, int E, int MAXT>
+static void launch_c(const Product &op, const PlanTables &t, const cplx<TC> *work, c128 *out, int nrhs, int g,
+                     hipStream_t s, const int *skip, int skip_stride) {
+    const FftPlan &p = t.plan;
+    const int64_t nb = blocks_ac(op.dims, nrhs, g);
+    const size_t lds = lds_ac<TC>(p, g);
+    set_lds(reinterpret_cast<const void *>(&pass_c<TC, E, MAXT>), lds);
+    hipLaunchKernelGGL((pass_c<TC, E, MAXT>), dim3(xcd_grid(nb)), dim3(team_threads(p) * g), lds, s, work, out,
+                       op.problem->exists.ptr, op.dims, p, tw_of<TC>(t), g, static_cast<int>(nb), skip, skip_stride);
+    RT_LAUNCH_CHECK();
+}
+
+template <typename TC, int E, int MAXT>
+static void launch_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
+                     const cplx<TC> *planes, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride) {
+    const FftPlan &p = ty.plan;
+    const int64_t nb = static_cast<int64_t>(op.dims.Mx / tile) * nrhs;
+    const size_t lds = lds_b<TC>(p, tile);
+    set_lds(reinterpret_cast<const void *>(&pass_b<TC, E, MAXT>), lds);
+    hipLaunchKernelGGL((pass_b<TC, E, MAXT>), dim3(xcd_grid(nb)), dim3(team_threads(p) * tile), lds, s, work,
+                       planes, tx.fold.ptr, ty.fold.ptr, d_of<TC>(tx), d_of<TC>(ty), op.dims, p, tw_of<TC>(ty), tile,
+                       nrhs, static_cast<int>(nb), skip, skip_stride);
+    RT_LAUNCH_CHECK();
+}
+
 template <typename TC>
 static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
                        const cplx<TC> *planes, hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
     const ProdDims &d = op.dims;
     Context *ctx = op.problem->ctx;
-    const uint8_t *ex = op.problem->exists.ptr;
-    const int g = sh.cols_per_block;
-    const unsigned blocks_ac = static_cast<unsigned>(nrhs * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g));
-    const size_t lds_ac = static_cast<size_t>(g) * (d.Mx + 1) * sizeof(cplx<TC>);
-    const size_t lds_b = static_cast<size_t>(2) * sh.tile * (d.My + 1) * sizeof(cplx<TC>);
-    set_lds<TC>(reinterpret_cast<const void *>(&pass_a<TC, double>), lds_ac);
-    set_lds<TC>(reinterpret_cast<const void *>(&pass_c<TC, double>), lds_ac);
-    set_lds<TC>(reinterpret_cast<const void *>(&pass_b<TC>), lds_b);
+    const PlanTables &tx = op.tx.of(sh.bits), &ty = op.ty.of(sh.bits);
+    const int ex = 1 << tx.plan.log2E, ey = 1 << ty.plan.log2E;
     Context::PassRecord rec{};
     const bool timed = ctx->timing;
     if (timed) {
@@ -329,18 +575,12 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
         pass_bytes<TC>(d, nrhs, rec.bytes);
         RT_HIP(hipEventRecord(rec.ev[0], s));
     }
-    hipLaunchKernelGGL((pass_a<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s,
-                       reinterpret_cast<const cplx<double> *>(in), work, ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip, skip_stride);
-    RT_LAUNCH_CHECK();
+    const int ta = team_threads(tx.plan) * sh.group, tb = team_threads(ty.plan) * sh.tile;
+    RT_BY_E(ex, RT_BY_MAXT(ta, (launch_a<TC, E, MAXT>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride))));
     if (timed) RT_HIP(hipEventRecord(rec.ev[1], s));
-    hipLaunchKernelGGL((pass_b<TC>), dim3(d.Mx / sh.tile, nrhs), dim3(sh.threads_b), lds_b, s, work, planes,
-                       op.tx.fold.ptr, op.ty.fold.ptr, d_of<TC>(op.tx), d_of<TC>(op.ty), d, op.ty.plan,
-                       tw_of<TC>(op.ty), sh.tile, skip, skip_stride);
-    RT_LAUNCH_CHECK();
+    RT_BY_E(ey, RT_BY_MAXT(tb, (launch_b<TC, E, MAXT>(op, tx, ty, work, planes, nrhs, sh.tile, s, skip, skip_stride))));
     if (timed) RT_HIP(hipEventRecord(rec.ev[2], s));
-    hipLaunchKernelGGL((pass_c<TC, double>), dim3(blocks_ac), dim3(sh.threads_ac), lds_ac, s, work,
-                       reinterpret_cast<cplx<double> *>(out), ex, d, op.tx.plan, tw_of<TC>(op.tx), g, skip, skip_stride);
-    RT_LAUNCH_CHECK();
+    RT_BY_E(ex, RT_BY_MAXT(ta, (launch_c<TC, E, MAXT>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride))));
     if (timed) {
         RT_HIP(hipEventRecord(rec.ev[3], s));
         ctx->pending.push_back(rec);
@@ -348,28 +588,46 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     }
 }
 
+template <typename TC>
+static cplx<TC> *work_of(Product &op, int nrhs) {
+    const size_t per_rhs = static_cast<size_t>(2) * op.dims.ldj * op.dims.Mx;
+    if (sizeof(TC) == sizeof(double)) {
+        if (op.work_rhs64 < nrhs) {
+            op.work64.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
+            op.work_rhs64 = nrhs;
+        }
+        return reinterpret_cast<cplx<TC> *>(op.work64.ptr);
+    }
+    if (op.work_rhs32 < nrhs) {
+        op.work32.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
+        op.work_rhs32 = nrhs;
+    }
+    return reinterpret_cast<cplx<TC> *>(op.work32.ptr);
+}
+
+template <typename TC>
+static const cplx<TC> *planes_of(const Product &op) {
+    if (sizeof(TC) == sizeof(double)) return reinterpret_cast<const cplx<TC> *>(op.planes64.ptr);
+    return reinterpret_cast<const cplx<TC> *>(op.planes32.ptr);
+}
+
+template <typename TC>
+static void apply_as(Product &op, const c128 *in, c128 *out, int nrhs, hipStream_t s, const int *skip,
+                     int skip_stride) {
+    cplx<TC> *work = work_of<TC>(op, nrhs);
+    const PassShape sh = shape_for<TC>(op, nrhs);
+    if (!shape_fits<TC>(op, sh))
+        fail(RT_EINVAL, "rooftop:product", "the FFT columns of this product (%d x %d) exceed the LDS of this device",
+             op.dims.Mx, op.dims.My);
+    run_passes<TC>(op, in, out, work, nrhs, sh, planes_of<TC>(op), s, skip, skip_stride);
+}
+
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
                     int skip_stride) {
     if (prec == RT_MIXED) prec = RT_C128;
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
-    const ProdDims &d = op.dims;
-    const size_t per_rhs = static_cast<size_t>(2) * d.ldj * d.Mx;
-    if (prec == RT_C128) {
-        if (op.work_rhs64 < nrhs) {
-            op.work64.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
-            op.work_rhs64 = nrhs;
-            op.shape64 = default_shape<double>(d, nrhs);
-        }
-        PassShape sh = default_shape<double>(d, nrhs);
-        run_passes<double>(op, in, out, op.work64.ptr, nrhs, sh, op.planes64.ptr, s, skip, skip_stride);
-    } else {
-        if (op.work_rhs32 < nrhs) {
-            op.work32.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
-            op.work_rhs32 = nrhs;
-        }
-        PassShape sh = default_shape<float>(d, nrhs);
-        run_passes<float>(op, in, out, op.work32.ptr, nrhs, sh, op.planes32.ptr, s, skip, skip_stride);
-    }
+    if (prec == RT_C128) apply_as<double>(op, in, out, nrhs, s, skip, skip_stride);
+    else apply_as<float>(op, in, out, nrhs, s, skip, skip_stride);
 }
 
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier) {
@@ -403,9 +661,10 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
     op->planes64.allocate(ctx, 3 * plane_elems, "The spectra of the product");
     op->planes64.zero(s);
     {
+        const PlanTables &tx = op->tx.of(3), &ty = op->ty.of(3);
         DeviceArray<c128> tmp;
         tmp.allocate(ctx, static_cast<size_t>(d.Mx) * d.My, "The spectra of the product (transform buffer)");
-        std::vector<int> fx = output_frequencies(op->tx.plan), fy = output_frequencies(op->ty.plan);
+        std::vector<int> fx = output_frequencies(tx.plan), fy = output_frequencies(ty.plan);
         DeviceArray<int> dfx, dfy;
         dfx.allocate(ctx, d.Mx, "FFT frequency table");
         dfy.allocate(ctx, d.My, "FFT frequency table");
@@ -415,18 +674,24 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
         const c128 *gens[3] = {g.has_x ? p->mxx.ptr : nullptr, g.has_y ? p->myy.ptr : nullptr, p->e.ptr};
         const int rows[3] = {g.cx - 1, g.cx, g.cx};
         const int cols[3] = {g.cy, g.cy - 1, g.cy};
+        const size_t la = static_cast<size_t>(padded_length(tx.plan)) * sizeof(c128);
+        const size_t lb = static_cast<size_t>(padded_length(ty.plan)) * sizeof(c128);
+        const int ex = 1 << tx.plan.log2E, ey = 1 << ty.plan.log2E;
         for (int k = 0; k < 3; ++k) {
             if (!gens[k]) continue;
-            size_t lds_a = static_cast<size_t>(d.Mx) * sizeof(c128);
-            size_t lds_b = static_cast<size_t>(d.My) * sizeof(c128);
-            set_lds<double>(reinterpret_cast<const void *>(&spectra_a), lds_a);
-            set_lds<double>(reinterpret_cast<const void *>(&spectra_b), lds_b);
-            hipLaunchKernelGGL(spectra_a, dim3(d.My), dim3(256), lds_a, s, gens[k], rows[k], cols[k], modifier,
-                               tmp.ptr, d.Mx, d.My, op->tx.plan, op->tx.tw.ptr);
+            RT_BY_E(ex, ({
+                        set_lds(reinterpret_cast<const void *>(&spectra_a<E>), la);
+                        hipLaunchKernelGGL((spectra_a<E>), dim3(d.My), dim3(std::max(64, team_threads(tx.plan))), la,
+                                           s, gens[k], rows[k], cols[k], modifier, tmp.ptr, d.Mx, d.My, tx.plan,
+                                           tx.tw.ptr);
+                    }));
             RT_LAUNCH_CHECK();
-            hipLaunchKernelGGL(spectra_b, dim3(d.Mx), dim3(256), lds_b, s, tmp.ptr, op->planes64.ptr + k * plane_elems,
-                               d.Mx, d.My, d.Hy, op->ty.plan, op->ty.tw.ptr, op->tx.fold.ptr, op->ty.fold.ptr,
-                               dfx.ptr, dfy.ptr, scale);
+            RT_BY_E(ey, ({
+                        set_lds(reinterpret_cast<const void *>(&spectra_b<E>), lb);
+                        hipLaunchKernelGGL((spectra_b<E>), dim3(d.Mx), dim3(std::max(64, team_threads(ty.plan))), lb,
+                                           s, tmp.ptr, op->planes64.ptr + k * plane_elems, d.Mx, d.My, d.Hy, ty.plan,
+                                           ty.tw.ptr, dfx.ptr, dfy.ptr, scale);
+                    }));
             RT_LAUNCH_CHECK();
         }
         RT_HIP(hipStreamSynchronize(s));
@@ -445,27 +710,29 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
     return op;
 }
 
-// Times `reps` applications with the context's kernel timing (the first application warms up and is not counted).
+// ------------------------------------------------------------------------------------------------------------------
+// Measurement and tuning.
+// ------------------------------------------------------------------------------------------------------------------
+// Average per-launch milliseconds and algorithmic bytes of each pass over `reps` applications with shape sh (one
+// untimed application first), through the context's kernel timing.
 template <typename TC>
-static void profile_passes(Product &op, int nrhs, int reps, cplx<TC> *work, const cplx<TC> *planes, double *ms,
-                           double *bytes, hipStream_t s) {
-    const ProdDims &d = op.dims;
+static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, const c128 *in, c128 *out, double *ms,
+                       double *bytes, hipStream_t s) {
     Context *ctx = op.problem->ctx;
-    DeviceArray<c128> in, out;
-    in.allocate(ctx, static_cast<size_t>(d.n) * nrhs, "profile input");
-    out.allocate(ctx, static_cast<size_t>(d.n) * nrhs, "profile output");
-    in.zero(s);
-    const PassShape sh = default_shape<TC>(d, nrhs);
-    run_passes<TC>(op, in.ptr, out.ptr, work, nrhs, sh, planes, s);
+    cplx<TC> *work = work_of<TC>(op, nrhs);
+    run_passes<TC>(op, in, out, work, nrhs, sh, planes_of<TC>(op), s);
     RT_HIP(hipStreamSynchronize(s));
     const bool was = ctx->timing;
     ctx->drain_timing();
     const int p = sizeof(TC) == sizeof(float) ? 1 : 0;
     const int64_t n0 = ctx->launches[p];
     double ms0[3], by0[3];
-    for (int k = 0; k < 3; ++k) { ms0[k] = ctx->pass_ms[p][k]; by0[k] = ctx->pass_bytes[p][k]; }
+    for (int k = 0; k < 3; ++k) {
+        ms0[k] = ctx->pass_ms[p][k];
+        by0[k] = ctx->pass_bytes[p][k];
+    }
     ctx->timing = true;
-    for (int it = 0; it < reps; ++it) run_passes<TC>(op, in.ptr, out.ptr, work, nrhs, sh, planes, s);
+    for (int it = 0; it < reps; ++it) run_passes<TC>(op, in, out, work, nrhs, sh, planes_of<TC>(op), s);
     ctx->drain_timing();
     ctx->timing = was;
     const double cnt = static_cast<double>(ctx->launches[p] - n0);
@@ -478,24 +745,116 @@ static void profile_passes(Product &op, int nrhs, int reps, cplx<TC> *work, cons
     ctx->launches[p] = n0;
 }
 
+template <typename TC>
+static void profile_as(Product &op, int nrhs, int reps, double *ms, double *bytes, hipStream_t s) {
+    Context *ctx = op.problem->ctx;
+    DeviceArray<c128> in, out;
+    in.allocate(ctx, static_cast<size_t>(op.dims.n) * nrhs, "profile input");
+    out.allocate(ctx, static_cast<size_t>(op.dims.n) * nrhs, "profile output");
+    in.zero(s);
+    const PassShape sh = shape_for<TC>(op, nrhs);
+    if (!shape_fits<TC>(op, sh))
+        fail(RT_EINVAL, "rooftop:product", "the FFT columns of this product (%d x %d) exceed the LDS of this device",
+             op.dims.Mx, op.dims.My);
+    time_shape<TC>(op, nrhs, reps, sh, in.ptr, out.ptr, ms, bytes, s);
+}
+
 void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s) {
     if (op.type != kOperator) fail(RT_EINVAL, "rooftop:product", "only an operator has matvec passes to profile");
     if (prec == RT_MIXED) prec = RT_C128;
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
-    const size_t per_rhs = static_cast<size_t>(2) * op.dims.ldj * op.dims.Mx;
-    if (prec == RT_C128) {
-        if (op.work_rhs64 < nrhs) {
-            op.work64.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
-            op.work_rhs64 = nrhs;
+    if (prec == RT_C128) profile_as<double>(op, nrhs, reps, ms, bytes, s);
+    else profile_as<float>(op, nrhs, reps, ms, bytes, s);
+}
+
+// The autotuner: for each plan (radices up to 8 or up to 16) it times the pass A + C column groups with the default
+// tile, then the pass B tiles with the best group, and keeps the fastest plan; the shape is recorded for this batch
+// size. Pass A/C time depends on the group only and pass B time on the tile only. The inputs are zeros: the passes
+// do the same work on any data.
+template <typename TC>
+static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipStream_t s) {
+    Context *ctx = op.problem->ctx;
+    DeviceArray<c128> in, out;
+    in.allocate(ctx, static_cast<size_t>(op.dims.n) * nrhs, "tuning input");
+    out.allocate(ctx, static_cast<size_t>(op.dims.n) * nrhs, "tuning output");
+    in.zero(s);
+    PassShape best = default_shape<TC>(op, nrhs);
+    double best_total = 1e300;
+    for (int bits = 3; bits <= 4; ++bits) {
+        PassShape sh = default_shape<TC>(op, nrhs, bits);
+        double ac = 1e300, b = 1e300;
+        int g_best = 0, t_best = 0;
+        for (int g = 1; g <= 16; g *= 2) {
+            PassShape c = sh;
+            c.group = g;
+            if (!shape_fits<TC>(op, c)) continue;
+            double ms[3], by[3];
+            time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s);
+            if (ms[0] + ms[2] < ac) {
+                ac = ms[0] + ms[2];
+                g_best = g;
+            }
         }
-        profile_passes<double>(op, nrhs, reps, op.work64.ptr, op.planes64.ptr, ms, bytes, s);
-    } else {
-        if (op.work_rhs32 < nrhs) {
-            op.work32.allocate(op.problem->ctx, per_rhs * nrhs, "the product's working memory");
-            op.work_rhs32 = nrhs;
+        if (!g_best) continue;
+        sh.group = g_best;
+        for (int tile = 1; tile <= 16; tile *= 2) {
+            PassShape c = sh;
+            c.tile = tile;
+            if (!shape_fits<TC>(op, c)) continue;
+            double ms[3], by[3];
+            time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s);
+            if (ms[1] < b) {
+                b = ms[1];
+                t_best = tile;
+            }
         }
-        profile_passes<float>(op, nrhs, reps, op.work32.ptr, op.planes32.ptr, ms, bytes, s);
+        if (!t_best) continue;
+        if (ac + b < best_total) {
+            best_total = ac + b;
+            best = PassShape{bits, g_best, t_best};
+        }
     }
+    auto &tuned = sizeof(TC) == sizeof(float) ? op.tuned32 : op.tuned64;
+    bool found = false;
+    for (auto &e : tuned)
+        if (e.first == nrhs) {
+            e.second = best;
+            found = true;
+        }
+    if (!found) tuned.emplace_back(nrhs, best);
+    if (best_ms) *best_ms = best_total;
+    return best;
+}
+
+void operator_tune(Product &op, int nrhs, int prec, int *shape, double *ms, hipStream_t s) {
+    if (op.type != kOperator) fail(RT_EINVAL, "rooftop:product", "only an operator has matvec passes to tune");
+    if (prec == RT_MIXED) prec = RT_C128;
+    if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
+    const PassShape sh = prec == RT_C128 ? tune_as<double>(op, nrhs, 5, ms, s) : tune_as<float>(op, nrhs, 5, ms, s);
+    if (shape) {
+        shape[0] = sh.bits;
+        shape[1] = sh.group;
+        shape[2] = sh.tile;
+    }
+}
+
+void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
+    if (op.type != kOperator) fail(RT_EINVAL, "rooftop:product", "only an operator has matvec passes");
+    if (prec == RT_MIXED) prec = RT_C128;
+    if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
+    const PassShape sh{shape[0], shape[1], shape[2]};
+    auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
+    if ((sh.bits != 3 && sh.bits != 4) || !pow2(sh.group) || !pow2(sh.tile) || sh.group > 16 || sh.tile > 16)
+        fail(RT_EINVAL, "rooftop:product", "a shape is {3 or 4, a power of two <= 16, a power of two <= 16}");
+    const bool fits = prec == RT_C128 ? shape_fits<double>(op, sh) : shape_fits<float>(op, sh);
+    if (!fits) fail(RT_EINVAL, "rooftop:product", "the shape does not fit the device for this product");
+    auto &tuned = prec == RT_C128 ? op.tuned64 : op.tuned32;
+    for (auto &e : tuned)
+        if (e.first == nrhs) {
+            e.second = sh;
+            return;
+        }
+    tuned.emplace_back(nrhs, sh);
 }
 
 uint64_t Product::stored_bytes() const {

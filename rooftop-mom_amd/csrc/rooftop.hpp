@@ -48,22 +48,28 @@ struct Problem {
 
 // Per-dimension tables of a product: twiddles, fold index and divergence factor D(k) = (1 - exp(2 pi i k/M))/size
 // at each digit-reversed output position.
-struct DimTables {
+// Tables of one transform direction for one plan.
+struct PlanTables {
     FftPlan plan;
-    DeviceArray<c128> tw;
+    DeviceArray<c128> tw;     // twiddles w_L^(q n) of the rounds
     DeviceArray<c64> tw32;
-    DeviceArray<int> fold;
-    DeviceArray<c128> d;
+    DeviceArray<int> fold;    // fold index min(k, M - k) of the frequency k held at each position
+    DeviceArray<c128> d;      // divergence factor (1 - exp(2 pi i k / M)) / size at each position
     DeviceArray<c64> d32;
+};
+
+// The tables of one direction for the plans with radices up to 8 and up to 16.
+struct DimTables {
+    PlanTables p[2];
     void build(Context *ctx, int M, double size, hipStream_t s);
+    const PlanTables &of(int bits) const { return p[bits >= 4 ? 1 : 0]; }
 };
 
 // Launch shape of the three matvec passes (picked by the autotuner or the default heuristic).
 struct PassShape {
-    int cols_per_block = 1;   // pass A / C: y columns per block
-    int threads_ac = 256;
-    int tile = 1;             // pass B: x positions per block
-    int threads_b = 256;
+    int bits = 4;    // largest radix of the plans: 2^bits elements per thread
+    int group = 1;   // pass A / C: rooftop columns per block
+    int tile = 1;    // pass B: x positions per block
 };
 
 enum ProductType { kOperator = 0, kJacobi = 1, kNearField = 2 };
@@ -81,7 +87,7 @@ struct Product {
     DeviceArray<c128> work64;        // [nrhs][2][Mx][ldj]
     DeviceArray<c64> work32;
     int work_rhs64 = 0, work_rhs32 = 0;
-    PassShape shape64, shape32;
+    std::vector<std::pair<int, PassShape>> tuned64, tuned32;   // shapes picked by the autotuner, per batch size
     // Preconditioners.
     DeviceArray<c128> diag64;                 // Jacobi: 1/diag(Z) per rooftop
     DeviceArray<c64> diag32;
@@ -104,6 +110,11 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
                     int skip_stride);
 void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s);
+// Times the launch shapes of the passes for nrhs right-hand sides and keeps the fastest for that batch size;
+// shape (may be NULL) receives {bits, group, tile}, ms the summed pass time of one application.
+void operator_tune(Product &op, int nrhs, int prec, int *shape, double *ms, hipStream_t s);
+// Sets the shape {bits, group, tile} used for nrhs right-hand sides (a saved tuning result).
+void operator_set_shape(Product &op, int nrhs, int prec, const int *shape);
 // nearfield.hip
 std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precision);
 void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s,

@@ -143,3 +143,49 @@ def test_handles_may_be_destroyed_in_any_order(rt):
     assert rel(op.apply(x), before) == 0
     rt.library().rt_product_destroy(op.handle)
     op.handle = None
+
+
+def _oracle_product(p, cx, cy, a, b, x):
+    """The oracle's split-FFT product on the GPU's own generators (isolates the product from the assembly)."""
+    grid = A.Grid(cx, cy, a, b)
+    spectra = S.subband_spectra(p.generators(), grid)
+    return np.stack([S.split_fft_product(spectra, grid, x[:, r]) for r in range(x.shape[1])], axis=1)
+
+
+@pytest.mark.parametrize('size', [(17, 33), (64, 3), (3, 100), (130, 40), (1, 40), (40, 1)])
+def test_product_shapes_across_transform_lengths(rt, size):
+    """Every launch shape the autotuner may pick (both plans, all column groups and tiles that fit) gives the same
+    product, on grids whose embeddings cover transform lengths 1 .. 256 in both directions."""
+    cx, cy = size
+    print('shapes', size, flush=True)
+    p = _problem(rt, cx, cy, 0.1, 0.1, (3, 3))
+    x = _random(p.rooftopCount(), 9, nrhs=3)
+    ref = _oracle_product(p, cx, cy, 0.1, 0.1, x)
+    op = rt.Operator(p, precision='mixed')
+    for prec, tol in (('c128', 1e-12), ('c64', 5e-6)):
+        for bits in (3, 4):
+            for group in (1, 2, 4, 8, 16):
+                for tile in (1, 4, 16):
+                    try:
+                        op.setShape({'bits': bits, 'group': group, 'tile': tile}, nrhs=3, precision=prec)
+                    except rt.RooftopError:
+                        continue   # does not fit this product
+                    assert rel(op.apply(x, precision=prec), ref) < tol, (prec, bits, group, tile)
+
+
+def test_plate_512_product_and_tuning(rt):
+    """The 512 x 512 plate of the batched configuration (transforms of 1024): both precisions against the oracle,
+    before and after tuning for the batch."""
+    n = 512
+    print('plate 512', flush=True)
+    p = _problem(rt, n, n, 0.1, 0.1, (4, 4))
+    x = _random(p.rooftopCount(), 11, nrhs=2)
+    ref = _oracle_product(p, n, n, 0.1, 0.1, x)
+    op = rt.Operator(p, precision='mixed')
+    assert rel(op.apply(x), ref) < 1e-12
+    assert rel(op.apply(x, precision='c64'), ref) < 5e-6
+    for prec in ('c128', 'c64'):
+        shape, ms = op.tune(nrhs=2, precision=prec)
+        assert shape['bits'] in (3, 4) and ms > 0
+    assert rel(op.apply(x), ref) < 1e-12
+    assert rel(op.apply(x, precision='c64'), ref) < 5e-6
