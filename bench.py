@@ -154,6 +154,14 @@ def main():
     op = rt.Operator(problem, precision='mixed')
     pc = rt.Preconditioner(problem, 'nearField', precision='mixed')
     setup_s = time.perf_counter() - t0
+    # Launch shapes of the matvec passes for the batch sizes timed below (the product's autotuner; a one-off per
+    # product and batch size, like an FFT plan, not part of a solve).
+    t0 = time.perf_counter()
+    tuned = {}
+    for nrhs in sorted({1, args.batch} - {0}):
+        for prec in ('c64', 'c128'):
+            tuned['%s x%d' % (prec, nrhs)] = op.tune(nrhs=nrhs, precision=prec)[0]
+    tune_s = time.perf_counter() - t0
     exc = rt.Excitation.planeWave(problem)
     N = problem.rooftopCount()
     b = np.ascontiguousarray(exc.vector(), dtype=np.complex128)
@@ -249,6 +257,7 @@ def main():
         'matvec': matvec,
         'batched': batch,
         'setup_s': round(setup_s, 3),
+        'tuning': {'seconds': round(tune_s, 3), 'shapes': tuned},
     }
     if not args.no_cpu:
         print('bench: cpu baseline', flush=True)

@@ -22,6 +22,7 @@ constexpr int kMaxRounds = 8;
 struct FftPlan {
     int M = 1;
     int log2M = 0;
+    int bits = 0;                  // the radix limit 2^bits the plan was made with
     int K = 0;                     // rounds
     int log2E = 0;                 // log2 of the largest radix: elements per thread
     int lr[kMaxRounds] = {};       // log2 radix of each round
@@ -130,7 +131,9 @@ __device__ __forceinline__ void team_round_r(const FftPlan &p, int s, int t, con
     const int T_ = p.M >> p.log2E;
     const int ls = p.ll[s] - p.lr[s];
     const cplx<T> *tws = tw + p.twoff[s];
-#pragma unroll
+    // One butterfly at a time: its loads, DFT and stores are independent of the next one's, so the register budget
+    // of a thread is one butterfly's (the waves of a CU hide the latency instead).
+#pragma unroll 1
     for (int u = 0; u < U; ++u) {
         const int beta = t + u * T_;
         const int n = beta & ((1 << ls) - 1);

@@ -11,6 +11,7 @@ FftPlan make_plan(int M, int maxbits) {
         fail(RT_EINVAL, "rooftop:fft", "transform length %d is not a power of two", M);
     if (maxbits < 1 || maxbits > 4) fail(RT_EINVAL, "rooftop:fft", "radices up to 16 are supported");
     p.M = M;
+    p.bits = maxbits;
     while ((1 << p.log2M) < M) ++p.log2M;
     const int m = p.log2M;
     if (m == 0) {

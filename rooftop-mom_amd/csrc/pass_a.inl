@@ -1,0 +1,82 @@
+// Pass A of the matvec (passes.hpp): forward x transforms of `group` consecutive rooftop columns of one component,
+// zero-padded from their rows to Mx, written into the work array. LDS holds [column][x position (padded)].
+#include "passes.hpp"
+
+namespace rt {
+
+template <typename TC, int LOG2M, int BITS, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_a_k(const c128 *__restrict__ in, cplx<TC> *__restrict__ work,
+                                                 const uint8_t *__restrict__ exists, ProdDims d,
+                                                 const cplx<TC> *__restrict__ tw, int G, int CS, int nblocks,
+                                                 const int *__restrict__ skip, int skip_stride) {
+    using P = CPlan<LOG2M, BITS>;
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    int r, comp, j0;
+    group_of(d, G, lb, r, comp, j0);
+    if (skip && skip[r * skip_stride]) return;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
+    const int cols = comp ? d.cols_y : d.cols_x;
+    const int rows = comp ? d.rows_y : d.rows_x;
+    const int ncols = min(G, cols - j0);
+    const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
+    {   // Round 0, threads along x: reads the rooftop rows of column c.
+        const int c = threadIdx.x / P::T, t = threadIdx.x - c * P::T;
+        const bool live = c < ncols;
+        const int64_t col = static_cast<int64_t>(j0 + c) * rows;
+        const c128 *src = in + off + col;
+        const uint8_t *exc = exists ? exists + (comp ? d.nx_count : 0) + col : nullptr;
+        cplx<TC> *lds = buf + c * CS;
+        cround<TC, P, 0, false, true, (P::K > 1)>(
+            t, tw,
+            [&](int pos, int) {
+                if (!live || pos >= rows || (exc && !exc[pos])) return mk<TC>(0, 0);
+                return cvt<TC>(src[pos]);
+            },
+            [&](int, int pp, cplx<TC> v) { lds[pp] = v; });
+    }
+    __syncthreads();
+    // Rounds 1 .. K-1, threads along the columns; the last one writes the work array.
+    const int c = threadIdx.x % G, t = threadIdx.x / G;
+    cplx<TC> *lds = buf + c * CS;
+    static_for<1, P::K - 1>([&](auto S) {
+        cround<TC, P, decltype(S)::value, false, false, true>(
+            t, tw, [&](int, int pp) { return lds[pp]; }, [&](int, int pp, cplx<TC> v) { lds[pp] = v; });
+        __syncthreads();
+    });
+    if (c >= ncols) return;
+    cplx<TC> *dst = work + work_index(d, r, comp, 0, j0 + c);
+    const int64_t ldj = d.ldj;
+    if constexpr (P::K > 1) {
+        cround<TC, P, P::K - 1, false, false, false>(
+            t, tw, [&](int, int pp) { return lds[pp]; }, [&](int pos, int, cplx<TC> v) { dst[pos * ldj] = v; });
+    } else {
+        for (int pos = t; pos < P::M; pos += P::T) dst[pos * ldj] = lds[pos + (pos >> P::log2E)];
+    }
+}
+
+template <typename TC>
+void launch_pass_a(const Product &op, const PlanTables &t, const c128 *in, cplx<TC> *work, int nrhs, int g,
+                   hipStream_t s, const int *skip, int skip_stride) {
+    const FftPlan &p = t.plan;
+    constexpr int es = sizeof(cplx<TC>);
+    const int64_t nb = blocks_ac(op.dims, nrhs, g);
+    const int cs = column_stride(p, g, es);
+    const size_t lds = lds_ac_bytes(p, g, es);
+    const int threads = team_threads(p) * g;
+    const cplx<TC> *tw = sizeof(TC) == sizeof(double) ? reinterpret_cast<const cplx<TC> *>(t.tw.ptr)
+                                                      : reinterpret_cast<const cplx<TC> *>(t.tw32.ptr);
+#define RT_LAUNCH_A(L, B)                                                                                   \
+    {                                                                                                       \
+        constexpr int MT = (es == 16 && CPlan<L, B>::log2E >= 4) ? 512 : 1024;                              \
+        set_lds(reinterpret_cast<const void *>(&pass_a_k<TC, L, B, MT>), lds);                              \
+        hipLaunchKernelGGL((pass_a_k<TC, L, B, MT>), dim3(xcd_grid(nb)), dim3(threads), lds, s, in, work,   \
+                           op.problem->exists.ptr, op.dims, tw, g, cs, static_cast<int>(nb), skip, skip_stride); \
+    }
+    RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_A);
+#undef RT_LAUNCH_A
+    RT_LAUNCH_CHECK();
+}
+
+}  // namespace rt

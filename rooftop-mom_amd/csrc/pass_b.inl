@@ -1,0 +1,160 @@
+// Pass B of the matvec (passes.hpp): for `tile` consecutive x positions, the forward y transforms of both components
+// (contiguous work columns, zero-padded to My), the three-plane spectral multiply, the inverse y transforms and the
+// cut back to the columns. A team of T threads owns one x position and both of its columns; LDS holds
+// [position][component][y (padded)].
+#include "passes.hpp"
+
+namespace rt {
+
+// The last forward round, the multiply and the first inverse round on the registers of one butterfly at a time.
+// Spectra: row = bspec + fold(kx) My 3, position py holds (Mxx, Myy, E) at row[3 py ..]; dy in position order.
+// The multiply runs in chunks of CH positions behind scheduling barriers, which bounds the spectrum loads in flight
+// (and the registers they hold).
+template <typename TC, class P, bool DIRECT>
+__device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *col1, cplx<TC> *__restrict__ w0,
+                                               cplx<TC> *__restrict__ w1, int n0, int n1,
+                                               const cplx<TC> *__restrict__ row, cplx<TC> dxv,
+                                               const cplx<TC> *__restrict__ dy) {
+    constexpr int S = P::K - 1;
+    constexpr int R = 1 << P::lr(S), U = P::E / R;
+    constexpr int CH = sizeof(TC) == sizeof(float) ? 8 : 4;
+#pragma unroll 1
+    for (int u = 0; u < U; ++u) {
+        const int base = (t + u * P::T) * R;   // the last round has stride 1: positions base + q
+        const int pb = base + (base >> P::log2E);
+        cplx<TC> X[R], Y[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if constexpr (DIRECT) {
+                const bool in_half = R == 1 || q < R / 2;
+                X[q] = in_half && base + q < n0 ? w0[base + q] : mk<TC>(0, 0);
+                Y[q] = in_half && base + q < n1 ? w1[base + q] : mk<TC>(0, 0);
+            } else {
+                X[q] = col0[pb + q];
+                Y[q] = col1[pb + q];
+            }
+        }
+        dft<TC, R, false, false>(X);
+        dft<TC, R, false, false>(Y);
+        const cplx<TC> *sp = row + 3 * base;
+        const cplx<TC> *dyp = dy + base;
+#pragma unroll
+        for (int q0 = 0; q0 < R; q0 += CH) {
+#pragma unroll
+            for (int q = q0; q < q0 + CH && q < R; ++q) {
+                const cplx<TC> a = sp[3 * q], b = sp[3 * q + 1], el = sp[3 * q + 2];
+                const cplx<TC> dyv = dyp[q];
+                const cplx<TC> phi = el * (cmulc(dxv, X[q]) + cmulc(dyv, Y[q]));
+                X[q] = a * X[q] - dxv * phi;
+                Y[q] = b * Y[q] - dyv * phi;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        dft<TC, R, true, false>(X);
+        dft<TC, R, true, false>(Y);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if constexpr (DIRECT) {
+                if (base + q < n0) w0[base + q] = X[q];
+                if (base + q < n1) w1[base + q] = Y[q];
+            } else {
+                col0[pb + q] = X[q];
+                col1[pb + q] = Y[q];
+            }
+        }
+    }
+}
+
+template <typename TC, int LOG2M, int BITS, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ bspec,
+                                                 const int *__restrict__ fold_x, const cplx<TC> *__restrict__ dx,
+                                                 const cplx<TC> *__restrict__ dy, ProdDims d,
+                                                 const cplx<TC> *__restrict__ tw, int tile, int R, int nblocks,
+                                                 const int *__restrict__ skip, int skip_stride) {
+    using P = CPlan<LOG2M, BITS>;
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    const int r = lb % R, pt = lb / R;
+    if (skip && skip[r * skip_stride]) return;
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int Mp = P::M + (P::M >> P::log2E);
+    const int tt = threadIdx.x / P::T, t = threadIdx.x - tt * P::T;
+    const int px = pt * tile + tt;
+    cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + 2 * tt * Mp;
+    cplx<TC> *col1 = col0 + Mp;
+    cplx<TC> *w0 = work + work_index(d, r, 0, px, 0);
+    cplx<TC> *w1 = work + work_index(d, r, 1, px, 0);
+    const int n0 = d.cols_x, n1 = d.cols_y;
+    const cplx<TC> *row = bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+    const cplx<TC> dxv = dx[px];
+    if constexpr (P::K == 1) {   // one round: straight from the work array and back
+        multiply_round<TC, P, true>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
+    } else {
+        // Forward round 0 from the work array (the upper half of the embedding is zero).
+        cround<TC, P, 0, false, true, true>(
+            t, tw, [&](int pos, int) { return pos < n0 ? w0[pos] : mk<TC>(0, 0); },
+            [&](int, int pp, cplx<TC> v) { col0[pp] = v; });
+        cround<TC, P, 0, false, true, true>(
+            t, tw, [&](int pos, int) { return pos < n1 ? w1[pos] : mk<TC>(0, 0); },
+            [&](int, int pp, cplx<TC> v) { col1[pp] = v; });
+        __syncthreads();
+        static_for<1, P::K - 1>([&](auto S) {
+            constexpr int s = decltype(S)::value;
+            cround<TC, P, s, false, false, true>(
+                t, tw, [&](int, int pp) { return col0[pp]; }, [&](int, int pp, cplx<TC> v) { col0[pp] = v; });
+            cround<TC, P, s, false, false, true>(
+                t, tw, [&](int, int pp) { return col1[pp]; }, [&](int, int pp, cplx<TC> v) { col1[pp] = v; });
+            __syncthreads();
+        });
+        multiply_round<TC, P, false>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
+        __syncthreads();
+        static_for_down<1, P::K - 1>([&](auto S) {
+            constexpr int s = decltype(S)::value;
+            cround<TC, P, s, true, false, true>(
+                t, tw, [&](int, int pp) { return col0[pp]; }, [&](int, int pp, cplx<TC> v) { col0[pp] = v; });
+            cround<TC, P, s, true, false, true>(
+                t, tw, [&](int, int pp) { return col1[pp]; }, [&](int, int pp, cplx<TC> v) { col1[pp] = v; });
+            __syncthreads();
+        });
+        // Inverse round 0 into the work array, cut to the columns (the upper half of the period is never kept).
+        cround<TC, P, 0, true, true, true>(
+            t, tw, [&](int, int pp) { return col0[pp]; },
+            [&](int pos, int, cplx<TC> v) {
+                if (pos < n0) w0[pos] = v;
+            });
+        cround<TC, P, 0, true, true, true>(
+            t, tw, [&](int, int pp) { return col1[pp]; },
+            [&](int pos, int, cplx<TC> v) {
+                if (pos < n1) w1[pos] = v;
+            });
+    }
+}
+
+template <typename TC>
+void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
+                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride) {
+    const FftPlan &p = ty.plan;
+    constexpr int es = sizeof(cplx<TC>);
+    const int64_t nb = static_cast<int64_t>(op.dims.Mx / tile) * nrhs;
+    const size_t lds = lds_b_bytes(p, tile, es);
+    const int threads = team_threads(p) * tile;
+    const bool dbl = sizeof(TC) == sizeof(double);
+    const cplx<TC> *tw = dbl ? reinterpret_cast<const cplx<TC> *>(ty.tw.ptr)
+                             : reinterpret_cast<const cplx<TC> *>(ty.tw32.ptr);
+    const cplx<TC> *dx = dbl ? reinterpret_cast<const cplx<TC> *>(tx.d.ptr)
+                             : reinterpret_cast<const cplx<TC> *>(tx.d32.ptr);
+    const cplx<TC> *dy = dbl ? reinterpret_cast<const cplx<TC> *>(ty.d.ptr)
+                             : reinterpret_cast<const cplx<TC> *>(ty.d32.ptr);
+#define RT_LAUNCH_B(L, B)                                                                                      \
+    {                                                                                                          \
+        constexpr int MT = CPlan<L, B>::T > 256 ? CPlan<L, B>::T : 256;                                        \
+        set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, B, MT>), lds);                                 \
+        hipLaunchKernelGGL((pass_b_k<TC, L, B, MT>), dim3(xcd_grid(nb)), dim3(threads), lds, s, work, bspec,   \
+                           tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, static_cast<int>(nb), skip, skip_stride); \
+    }
+    RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_B);
+#undef RT_LAUNCH_B
+    RT_LAUNCH_CHECK();
+}
+
+}  // namespace rt

@@ -1,0 +1,86 @@
+// Pass C of the matvec (passes.hpp): inverse x transforms of `group` consecutive columns read back from the work
+// array; the first rows of each column are the product (the cut of FFTApplicator.m:80-84).
+#include "passes.hpp"
+
+namespace rt {
+
+template <typename TC, int LOG2M, int BITS, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ work, c128 *__restrict__ out,
+                                                 const uint8_t *__restrict__ exists, ProdDims d,
+                                                 const cplx<TC> *__restrict__ tw, int G, int CS, int nblocks,
+                                                 const int *__restrict__ skip, int skip_stride) {
+    using P = CPlan<LOG2M, BITS>;
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    int r, comp, j0;
+    group_of(d, G, lb, r, comp, j0);
+    if (skip && skip[r * skip_stride]) return;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
+    const int cols = comp ? d.cols_y : d.cols_x;
+    const int rows = comp ? d.rows_y : d.rows_x;
+    const int ncols = min(G, cols - j0);
+    {
+        const int c = threadIdx.x % G, t = threadIdx.x / G;
+        const bool live = c < ncols;
+        const cplx<TC> *src = work + work_index(d, r, comp, 0, j0 + (live ? c : 0));
+        const int64_t ldj = d.ldj;
+        cplx<TC> *lds = buf + c * CS;
+        if constexpr (P::K > 1) {
+            // Round K-1 (no twiddles), threads along the columns, read from the work array.
+            cround<TC, P, P::K - 1, true, false, false>(
+                t, tw, [&](int pos, int) { return live ? src[pos * ldj] : mk<TC>(0, 0); },
+                [&](int, int pp, cplx<TC> v) { lds[pp] = v; });
+            __syncthreads();
+            static_for_down<1, P::K - 1>([&](auto S) {
+                cround<TC, P, decltype(S)::value, true, false, true>(
+                    t, tw, [&](int, int pp) { return lds[pp]; }, [&](int, int pp, cplx<TC> v) { lds[pp] = v; });
+                __syncthreads();
+            });
+        } else {
+            for (int pos = t; pos < P::M; pos += P::T)
+                lds[pos + (pos >> P::log2E)] = live ? src[pos * ldj] : mk<TC>(0, 0);
+            __syncthreads();
+        }
+    }
+    // Round 0, threads along x: writes the rooftop rows of column c.
+    const int c = threadIdx.x / P::T, t = threadIdx.x - c * P::T;
+    if (c >= ncols) return;
+    const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
+    const int64_t col = static_cast<int64_t>(j0 + c) * rows;
+    c128 *dst = out + off + col;
+    const uint8_t *exc = exists ? exists + (comp ? d.nx_count : 0) + col : nullptr;
+    const cplx<TC> *lds = buf + c * CS;
+    cround<TC, P, 0, true, (P::M >= 2), (P::K > 1)>(
+        t, tw, [&](int, int pp) { return lds[pp]; },
+        [&](int pos, int, cplx<TC> v) {
+            if (pos >= rows) return;
+            if (exc && !exc[pos]) v = mk<TC>(0, 0);
+            dst[pos] = cvt<double>(v);
+        });
+}
+
+template <typename TC>
+void launch_pass_c(const Product &op, const PlanTables &t, const cplx<TC> *work, c128 *out, int nrhs, int g,
+                   hipStream_t s, const int *skip, int skip_stride) {
+    const FftPlan &p = t.plan;
+    constexpr int es = sizeof(cplx<TC>);
+    const int64_t nb = blocks_ac(op.dims, nrhs, g);
+    const int cs = column_stride(p, g, es);
+    const size_t lds = lds_ac_bytes(p, g, es);
+    const int threads = team_threads(p) * g;
+    const cplx<TC> *tw = sizeof(TC) == sizeof(double) ? reinterpret_cast<const cplx<TC> *>(t.tw.ptr)
+                                                      : reinterpret_cast<const cplx<TC> *>(t.tw32.ptr);
+#define RT_LAUNCH_C(L, B)                                                                                   \
+    {                                                                                                       \
+        constexpr int MT = (es == 16 && CPlan<L, B>::log2E >= 4) ? 512 : 1024;                              \
+        set_lds(reinterpret_cast<const void *>(&pass_c_k<TC, L, B, MT>), lds);                              \
+        hipLaunchKernelGGL((pass_c_k<TC, L, B, MT>), dim3(xcd_grid(nb)), dim3(threads), lds, s, work, out,  \
+                           op.problem->exists.ptr, op.dims, tw, g, cs, static_cast<int>(nb), skip, skip_stride); \
+    }
+    RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_C);
+#undef RT_LAUNCH_C
+    RT_LAUNCH_CHECK();
+}
+
+}  // namespace rt

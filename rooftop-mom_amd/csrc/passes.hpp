@@ -1,0 +1,198 @@
+// The three matvec passes of the Toeplitz product (product.hip), specialised per transform plan at compile time so
+// that every position offset of a round is an immediate and each kernel carries only its own radix path.
+//
+//   A  forward x transform of `group` rooftop columns of one component: the first round reads the rooftop vector
+//      (threads along x, coalesced), the last round writes the work array with the threads along the columns, so
+//      the transposition costs no pass of its own;
+//   B  per x position: forward y transforms of both components, the three-plane multiply in registers between the
+//      last forward round and the first inverse round, inverse y transforms; contiguous work columns;
+//   C  inverse x transform of `group` columns, the mirror of A, cut to the rooftop rows.
+// Blocks are numbered XCD-major (blockIdx b runs on XCD b mod 8): consecutive logical blocks -- adjacent column
+// groups in A/C, the right-hand sides of one x position in B -- share an L2, so their partial cache lines of the
+// work array and their spectrum rows meet there.
+#pragma once
+
+#include <type_traits>
+
+#include "rooftop.hpp"
+
+namespace rt {
+
+// ------------------------------------------------------------------------------------------------------------------
+// Compile-time plan: the same schedule as make_plan(1 << LOG2M, BITS) (fft_plan.cpp), whose tables it reads.
+// ------------------------------------------------------------------------------------------------------------------
+template <int LOG2M, int BITS>
+struct CPlan {
+    static constexpr int m = LOG2M;
+    static constexpr int M = 1 << m;
+    static constexpr int K = m == 0 ? 1 : (m + BITS - 1) / BITS;
+    static constexpr int base_bits = m == 0 ? 0 : m / K;
+    static constexpr int extra = m == 0 ? 0 : m % K;
+    static constexpr int lr(int s) { return m == 0 ? 0 : base_bits + (s < extra ? 1 : 0); }
+    static constexpr int ll(int s) {
+        int l = m;
+        for (int i = 0; i < s; ++i) l -= lr(i);
+        return l;
+    }
+    static constexpr int twoff(int s) {
+        int o = 0;
+        for (int i = 0; i < s; ++i) o += 1 << ll(i);
+        return o;
+    }
+    static constexpr int log2E = m == 0 ? 0 : base_bits + (extra > 0 ? 1 : 0);
+    static constexpr int E = 1 << log2E;
+    static constexpr int T = M >> log2E;   // threads of a team
+};
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(ic<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+// E-1 down to B.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for_down(F &&f) {
+    if constexpr (B < E) {
+        f(ic<E - 1>{});
+        static_for_down<B, E - 1>(f);
+    }
+}
+
+// One round S of a team with a compile-time plan. Thread t holds butterflies beta = t + u T (u < E / r); butterfly
+// beta covers positions base + q Str (Str = L / r). load(pos, ppos) / store(pos, ppos, v) receive the position and
+// its padded LDS offset, both a per-butterfly value plus an immediate.
+// HALF (forward): the upper half of the inputs is zero (the padding of an embedding); HALF (inverse): the upper half
+// of the outputs is cut away, so it is neither stored nor computed.
+template <typename T, class P, int S, bool INV, bool HALF, bool TWID, typename Load, typename Store>
+__device__ __forceinline__ void cround(int t, const cplx<T> *__restrict__ tw, Load &&load, Store &&store) {
+    constexpr int lr = P::lr(S), ll = P::ll(S), R = 1 << lr, U = P::E / R, ls = ll - lr;
+    const cplx<T> *tws = tw + P::twoff(S);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int beta = t + u * P::T;
+        const int n = beta & ((1 << ls) - 1);
+        const int base = ((beta >> ls) << ll) + n;
+        const int pb = base + (base >> P::log2E);
+        cplx<T> v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int o = q << ls;
+            const int po = o + (o >> P::log2E);
+            v[q] = (!INV && HALF && R > 1 && q >= R / 2) ? mk<T>(0, 0) : load(base + o, pb + po);
+        }
+        if constexpr (!INV) {
+            dft<T, R, false, HALF>(v);
+            if constexpr (TWID) {
+                const cplx<T> *w = tws + n * R;
+#pragma unroll
+                for (int q = 1; q < R; ++q) v[q] = v[q] * w[q];
+            }
+        } else {
+            if constexpr (TWID) {
+                const cplx<T> *w = tws + n * R;
+#pragma unroll
+                for (int q = 1; q < R; ++q) v[q] = cmulc(w[q], v[q]);
+            }
+            dft<T, R, true, false>(v);
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (INV && HALF && R > 1 && q >= R / 2) continue;
+            const int o = q << ls;
+            const int po = o + (o >> P::log2E);
+            store(base + o, pb + po, v[q]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// Shared helpers.
+// ------------------------------------------------------------------------------------------------------------------
+// Work array of the passes, per right-hand side: [component][x position (plan order)][y column], the y columns of
+// one x position contiguous (row length ldj >= the columns of either component).
+__host__ __device__ inline int64_t work_index(const ProdDims &d, int r, int comp, int kx, int j) {
+    return ((static_cast<int64_t>(r) * 2 + comp) * d.Mx + kx) * d.ldj + j;
+}
+
+// Logical block of hardware block b in a grid of 8 * ceil(nblocks / 8): XCD b mod 8 runs the logical blocks
+// [(b mod 8) per, (b mod 8 + 1) per) in order. -1 for the padding blocks.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    const int per = (nblocks + 7) >> 3;
+    const int lb = (b & 7) * per + (b >> 3);
+    return lb < nblocks ? lb : -1;
+}
+inline unsigned xcd_grid(int64_t nblocks) { return static_cast<unsigned>((nblocks + 7) / 8 * 8); }
+
+// Decodes a pass-A/C logical block into (right-hand side, component, first rooftop column).
+__device__ inline void group_of(const ProdDims &d, int group, int blk, int &r, int &comp, int &j0) {
+    const int gx = (d.cols_x + group - 1) / group, gy = (d.cols_y + group - 1) / group;
+    const int per = gx + gy;
+    r = blk / per;
+    const int rem = blk - r * per;
+    comp = rem < gx ? 0 : 1;
+    j0 = (comp ? rem - gx : rem) * group;
+}
+
+inline int64_t blocks_ac(const ProdDims &d, int nrhs, int g) {
+    return static_cast<int64_t>(nrhs) * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g);
+}
+
+// Block-size bound of each pass (threads = team size x columns or x positions per block): pass A/C at most 1024
+// (512 for complex128 with radix 16, whose butterflies need more registers), pass B one team of at least 256.
+inline int max_threads_ac(int es, const FftPlan &p) { return es == 16 && p.log2E >= 4 ? 512 : 1024; }
+inline int max_threads_b(const FftPlan &p) { return team_threads(p) > 256 ? team_threads(p) : 256; }
+
+// Pass A/C LDS: [column][position (padded)] with a column stride that puts the columns of one position on
+// different banks (64 banks of 4 bytes).
+inline int column_stride(const FftPlan &p, int group, int es) {
+    const int bank_elems = 256 / es;
+    const int mp = padded_length(p);
+    const int up = (mp + bank_elems - 1) / bank_elems * bank_elems;
+    const int skew = bank_elems / group > 0 ? bank_elems / group : 1;
+    return up + skew;
+}
+inline size_t lds_ac_bytes(const FftPlan &p, int group, int es) {
+    return static_cast<size_t>(column_stride(p, group, es)) * group * es;
+}
+inline size_t lds_b_bytes(const FftPlan &p, int tile, int es) {
+    return static_cast<size_t>(2) * tile * padded_length(p) * es;
+}
+
+// Raises a kernel's dynamic LDS limit above the 64 KiB default (once per kernel and size); fails beyond the device.
+void set_lds(const void *kernel, size_t bytes);
+size_t lds_bytes_max();
+
+// Launchers (passes_a.hip, passes_b.hip, passes_c.hip, one translation unit per pass and precision).
+template <typename TC>
+void launch_pass_a(const Product &op, const PlanTables &t, const c128 *in, cplx<TC> *work, int nrhs, int group,
+                   hipStream_t s, const int *skip, int skip_stride);
+template <typename TC>
+void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
+                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride);
+template <typename TC>
+void launch_pass_c(const Product &op, const PlanTables &t, const cplx<TC> *work, c128 *out, int nrhs, int group,
+                   hipStream_t s, const int *skip, int skip_stride);
+
+// Calls F with the compile-time plan of (log2M, bits).
+#define RT_PLAN_CASE(L, F)                                      \
+    case L:                                                     \
+        if (bits >= 4) { F(L, 4) } else { F(L, 3) }             \
+        break;
+#define RT_BY_PLAN(LOG2M, BITS, F)                                                                   \
+    do {                                                                                             \
+        const int bits = (BITS);                                                                     \
+        switch (LOG2M) {                                                                             \
+            RT_PLAN_CASE(0, F) RT_PLAN_CASE(1, F) RT_PLAN_CASE(2, F) RT_PLAN_CASE(3, F)              \
+            RT_PLAN_CASE(4, F) RT_PLAN_CASE(5, F) RT_PLAN_CASE(6, F) RT_PLAN_CASE(7, F)              \
+            RT_PLAN_CASE(8, F) RT_PLAN_CASE(9, F) RT_PLAN_CASE(10, F) RT_PLAN_CASE(11, F)            \
+            RT_PLAN_CASE(12, F) RT_PLAN_CASE(13, F)                                                  \
+        default: fail(RT_EINVAL, "rooftop:product", "transform length 2^%d is not supported", LOG2M); \
+        }                                                                                            \
+    } while (0)
+
+}  // namespace rt

@@ -8,20 +8,13 @@
 //     out_x = Mxx X - Dx E (conj(Dx) X + conj(Dy) Y),   out_y = Myy Y - Dy E (conj(Dx) X + conj(Dy) Y),
 // the x and y current spectra X, Y and the face-charge spectrum conj(Dx) X + conj(Dy) Y ("Jx, Jy and charge").
 //
-// Passes (every transform is a team FFT, fft.hpp):
-//   A  forward x transform of `group` rooftop columns of one component: the first round reads the rooftop vector
-//      (threads along x, coalesced), the last round writes the work array with the threads along the columns, so
-//      the transposition costs no pass of its own;
-//   B  per x position: forward y transforms of both components, the three-plane multiply in registers between the
-//      last forward round and the first inverse round, inverse y transforms; contiguous work columns;
-//   C  inverse x transform of `group` columns, the mirror of A, cut to the rooftop rows.
-// Blocks are numbered XCD-major (blockIdx b runs on XCD b mod 8): consecutive logical blocks -- adjacent column
-// groups in A/C, the right-hand sides of one x position in B -- share an L2, so their partial cache lines of the
-// work array and their spectrum rows meet there.
+// The passes themselves are in passes.hpp and pass_{a,b,c}.inl; this file holds the tables, the spectra, the launch
+// shapes and their autotuner.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
-#include "rooftop.hpp"
+#include "passes.hpp"
 
 namespace rt {
 
@@ -61,307 +54,6 @@ void DimTables::build(Context *ctx, int M, double size, hipStream_t s) {
         RT_HIP(hipMemcpyAsync(t.d32.ptr, h_d32.data(), M * sizeof(c64), hipMemcpyHostToDevice, s));
         RT_HIP(hipStreamSynchronize(s));
     }
-}
-
-template <typename T> const cplx<T> *tw_of(const PlanTables &t);
-template <> const c128 *tw_of<double>(const PlanTables &t) { return t.tw.ptr; }
-template <> const c64 *tw_of<float>(const PlanTables &t) { return t.tw32.ptr; }
-template <typename T> const cplx<T> *d_of(const PlanTables &t);
-template <> const c128 *d_of<double>(const PlanTables &t) { return t.d.ptr; }
-template <> const c64 *d_of<float>(const PlanTables &t) { return t.d32.ptr; }
-
-// Work array of the passes, per right-hand side: [component][x position (plan order)][y column], the y columns of
-// one x position contiguous (row length ldj >= the columns of either component).
-__host__ __device__ inline int64_t work_index(const ProdDims &d, int r, int comp, int kx, int j) {
-    return ((static_cast<int64_t>(r) * 2 + comp) * d.Mx + kx) * d.ldj + j;
-}
-
-// Logical block of hardware block b in a grid of 8 * ceil(nblocks / 8): XCD b mod 8 runs the logical blocks
-// [(b mod 8) per, (b mod 8 + 1) per) in order. -1 for the padding blocks.
-__device__ __forceinline__ int xcd_block(int b, int nblocks) {
-    const int per = (nblocks + 7) >> 3;
-    const int lb = (b & 7) * per + (b >> 3);
-    return lb < nblocks ? lb : -1;
-}
-static unsigned xcd_grid(int64_t nblocks) { return static_cast<unsigned>((nblocks + 7) / 8 * 8); }
-
-// Decodes a pass-A/C logical block into (right-hand side, component, first rooftop column).
-__device__ inline void group_of(const ProdDims &d, int group, int blk, int &r, int &comp, int &j0) {
-    const int gx = (d.cols_x + group - 1) / group, gy = (d.cols_y + group - 1) / group;
-    const int per = gx + gy;
-    r = blk / per;
-    const int rem = blk - r * per;
-    comp = rem < gx ? 0 : 1;
-    j0 = (comp ? rem - gx : rem) * group;
-}
-
-// ------------------------------------------------------------------------------------------------------------------
-// Pass A: forward x transforms of `group` consecutive rooftop columns of one component (zero-padded from their rows
-// to Mx), written into the work array. LDS holds [x position (padded)][column].
-// ------------------------------------------------------------------------------------------------------------------
-template <typename TC, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) pass_a(const c128 *__restrict__ in, cplx<TC> *__restrict__ work,
-                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan p,
-                                               const cplx<TC> *__restrict__ tw, int G, int nblocks,
-                                               const int *__restrict__ skip, int skip_stride) {
-    const int lb = xcd_block(blockIdx.x, nblocks);
-    if (lb < 0) return;
-    int r, comp, j0;
-    group_of(d, G, lb, r, comp, j0);
-    if (skip && skip[r * skip_stride]) return;
-    extern __shared__ unsigned char smem_raw[];
-    cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
-    const int cols = comp ? d.cols_y : d.cols_x;
-    const int rows = comp ? d.rows_y : d.rows_x;
-    const int ncols = min(G, cols - j0);
-    const int T = team_threads(p);
-    const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
-    const uint8_t *ex = exists ? exists + (comp ? d.nx_count : 0) : nullptr;
-    const int K = p.K;
-    auto lidx = [&](int pos, int c) { return padded(p, pos) * G + c; };
-    {   // Round 0, threads along x: reads the rooftop rows of column c.
-        const int c = threadIdx.x / T, t = threadIdx.x - c * T;
-        const bool live = c < ncols;
-        const int64_t col = static_cast<int64_t>(j0 + c) * rows;
-        team_round<TC, E, false, true>(
-            p, 0, t, tw, K > 1,
-            [&](int pos) {
-                if (!live || pos >= rows) return mk<TC>(0, 0);
-                if (ex && !ex[col + pos]) return mk<TC>(0, 0);
-                return cvt<TC>(in[off + col + pos]);
-            },
-            [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
-    }
-    __syncthreads();
-    // Rounds 1 .. K-1, threads along the columns; the last one writes the work array.
-    const int c = threadIdx.x % G, t = threadIdx.x / G;
-    for (int s = 1; s < K - 1; ++s) {
-        team_round<TC, E, false, false>(
-            p, s, t, tw, true, [&](int pos) { return buf[lidx(pos, c)]; },
-            [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
-        __syncthreads();
-    }
-    if (K > 1) {
-        team_round<TC, E, false, false>(
-            p, K - 1, t, tw, false, [&](int pos) { return buf[lidx(pos, c)]; },
-            [&](int pos, cplx<TC> v) {
-                if (c < ncols) work[work_index(d, r, comp, pos, j0 + c)] = v;
-            });
-    } else if (c < ncols) {
-        for (int pos = t; pos < p.M; pos += T) work[work_index(d, r, comp, pos, j0 + c)] = buf[lidx(pos, c)];
-    }
-}
-
-// ------------------------------------------------------------------------------------------------------------------
-// Pass C: inverse x transforms of `group` consecutive columns read back from the work array; the first rows of each
-// column are the product (the cut of FFTApplicator.m:80-84).
-// ------------------------------------------------------------------------------------------------------------------
-template <typename TC, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) pass_c(const cplx<TC> *__restrict__ work, c128 *__restrict__ out,
-                                               const uint8_t *__restrict__ exists, ProdDims d, FftPlan p,
-                                               const cplx<TC> *__restrict__ tw, int G, int nblocks,
-                                               const int *__restrict__ skip, int skip_stride) {
-    const int lb = xcd_block(blockIdx.x, nblocks);
-    if (lb < 0) return;
-    int r, comp, j0;
-    group_of(d, G, lb, r, comp, j0);
-    if (skip && skip[r * skip_stride]) return;
-    extern __shared__ unsigned char smem_raw[];
-    cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw);
-    const int cols = comp ? d.cols_y : d.cols_x;
-    const int rows = comp ? d.rows_y : d.rows_x;
-    const int ncols = min(G, cols - j0);
-    const int T = team_threads(p);
-    const int K = p.K;
-    auto lidx = [&](int pos, int c) { return padded(p, pos) * G + c; };
-    {
-        const int c = threadIdx.x % G, t = threadIdx.x / G;
-        if (K > 1) {
-            // Round K-1 (no twiddles), threads along the columns, read from the work array.
-            team_round<TC, E, true, false>(
-                p, K - 1, t, tw, false,
-                [&](int pos) { return c < ncols ? work[work_index(d, r, comp, pos, j0 + c)] : mk<TC>(0, 0); },
-                [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
-            __syncthreads();
-            for (int s = K - 2; s >= 1; --s) {
-                team_round<TC, E, true, false>(
-                    p, s, t, tw, true, [&](int pos) { return buf[lidx(pos, c)]; },
-                    [&](int pos, cplx<TC> v) { buf[lidx(pos, c)] = v; });
-                __syncthreads();
-            }
-        } else {
-            for (int pos = t; pos < p.M; pos += T)
-                buf[lidx(pos, c)] = c < ncols ? work[work_index(d, r, comp, pos, j0 + c)] : mk<TC>(0, 0);
-            __syncthreads();
-        }
-    }
-    // Round 0, threads along x: writes the rooftop rows of column c.
-    const int c = threadIdx.x / T, t = threadIdx.x - c * T;
-    const bool live = c < ncols;
-    const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
-    const int64_t col = static_cast<int64_t>(j0 + c) * rows;
-    const uint8_t *ex = exists ? exists + (comp ? d.nx_count : 0) : nullptr;
-    team_round<TC, E, true, false>(
-        p, 0, t, tw, K > 1, [&](int pos) { return buf[lidx(pos, c)]; },
-        [&](int pos, cplx<TC> v) {
-            if (!live || pos >= rows) return;
-            if (ex && !ex[col + pos]) v = mk<TC>(0, 0);
-            out[off + col + pos] = cvt<double>(v);
-        });
-}
-
-// ------------------------------------------------------------------------------------------------------------------
-// Pass B: for `tile` consecutive x positions, the forward y transforms of both components (contiguous columns,
-// zero-padded to My), the three-plane spectral multiply, the inverse y transforms and the cut back to the columns.
-// A team of T threads owns one x position and both of its columns; LDS holds [position][component][y (padded)].
-// ------------------------------------------------------------------------------------------------------------------
-template <typename TC, int E, int R>
-__device__ __forceinline__ void multiply_round(const FftPlan &p, int t, int px, cplx<TC> *col0, cplx<TC> *col1,
-                                               cplx<TC> *__restrict__ w0, cplx<TC> *__restrict__ w1,
-                                               int n0, int n1, bool direct, const cplx<TC> *__restrict__ planes,
-                                               int64_t plane, int Hy, int fx, const int *__restrict__ fold_y,
-                                               cplx<TC> dxv, const cplx<TC> *__restrict__ dy) {
-    constexpr int U = E / R;
-    const int T = p.M >> p.log2E;
-    const cplx<TC> *rowp = planes + static_cast<int64_t>(fx) * Hy;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int base = (t + u * T) * R;   // the last round has stride 1: positions base + q
-        cplx<TC> X[R], Y[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int pos = base + q;
-            if (direct) {
-                X[q] = (R == 1 || q < R / 2) && pos < n0 ? w0[pos] : mk<TC>(0, 0);
-                Y[q] = (R == 1 || q < R / 2) && pos < n1 ? w1[pos] : mk<TC>(0, 0);
-            } else {
-                X[q] = col0[padded(p, pos)];
-                Y[q] = col1[padded(p, pos)];
-            }
-        }
-        dft<TC, R, false, false>(X);
-        dft<TC, R, false, false>(Y);
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int py = base + q;
-            const int f = fold_y[py];
-            const cplx<TC> a = rowp[f], b = rowp[plane + f], el = rowp[2 * plane + f];
-            const cplx<TC> dyv = dy[py];
-            const cplx<TC> phi = el * (cmulc(dxv, X[q]) + cmulc(dyv, Y[q]));
-            X[q] = a * X[q] - dxv * phi;
-            Y[q] = b * Y[q] - dyv * phi;
-        }
-        dft<TC, R, true, false>(X);
-        dft<TC, R, true, false>(Y);
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int pos = base + q;
-            if (direct) {
-                if (pos < n0) w0[pos] = X[q];
-                if (pos < n1) w1[pos] = Y[q];
-            } else {
-                col0[padded(p, pos)] = X[q];
-                col1[padded(p, pos)] = Y[q];
-            }
-        }
-    }
-}
-
-template <typename TC, int E, int MAXT>
-__global__ void __launch_bounds__(MAXT) pass_b(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ planes,
-                                               const int *__restrict__ fold_x, const int *__restrict__ fold_y,
-                                               const cplx<TC> *__restrict__ dx, const cplx<TC> *__restrict__ dy,
-                                               ProdDims d, FftPlan p, const cplx<TC> *__restrict__ tw, int tile,
-                                               int R, int nblocks, const int *__restrict__ skip, int skip_stride) {
-    const int lb = xcd_block(blockIdx.x, nblocks);
-    if (lb < 0) return;
-    const int r = lb % R, pt = lb / R;
-    if (skip && skip[r * skip_stride]) return;
-    extern __shared__ unsigned char smem_raw[];
-    const int T = team_threads(p);
-    const int tt = threadIdx.x / T, t = threadIdx.x - tt * T;
-    const int px = pt * tile + tt;
-    const int Mp = padded_length(p);
-    cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + static_cast<int64_t>(2 * tt) * Mp;
-    cplx<TC> *col1 = col0 + Mp;
-    cplx<TC> *w0 = work + work_index(d, r, 0, px, 0);
-    cplx<TC> *w1 = work + work_index(d, r, 1, px, 0);
-    const int n0 = d.cols_x, n1 = d.cols_y;
-    const int64_t plane = static_cast<int64_t>(d.Hx) * d.Hy;
-    const int fx = fold_x[px];
-    const cplx<TC> dxv = dx[px];
-    const int K = p.K;
-    auto middle = [&](bool direct) {
-        switch (p.lr[K - 1]) {
-        case 0:
-            multiply_round<TC, E, 1>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx, fold_y,
-                                     dxv, dy);
-            break;
-        case 1:
-            if constexpr (E >= 2)
-                multiply_round<TC, E, 2>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
-                                         fold_y, dxv, dy);
-            break;
-        case 2:
-            if constexpr (E >= 4)
-                multiply_round<TC, E, 4>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
-                                         fold_y, dxv, dy);
-            break;
-        case 3:
-            if constexpr (E >= 8)
-                multiply_round<TC, E, 8>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
-                                         fold_y, dxv, dy);
-            break;
-        default:
-            if constexpr (E >= 16)
-                multiply_round<TC, E, 16>(p, t, px, col0, col1, w0, w1, n0, n1, direct, planes, plane, d.Hy, fx,
-                                          fold_y, dxv, dy);
-            break;
-        }
-    };
-    if (K == 1) {   // one round: straight from the work array and back
-        middle(true);
-        return;
-    }
-    // Forward round 0 from the work array (the upper half of the embedding is zero).
-    team_round<TC, E, false, true>(
-        p, 0, t, tw, true, [&](int pos) { return pos < n0 ? w0[pos] : mk<TC>(0, 0); },
-        [&](int pos, cplx<TC> v) { col0[padded(p, pos)] = v; });
-    team_round<TC, E, false, true>(
-        p, 0, t, tw, true, [&](int pos) { return pos < n1 ? w1[pos] : mk<TC>(0, 0); },
-        [&](int pos, cplx<TC> v) { col1[padded(p, pos)] = v; });
-    __syncthreads();
-    for (int s = 1; s < K - 1; ++s) {
-        team_round<TC, E, false, false>(
-            p, s, t, tw, true, [&](int pos) { return col0[padded(p, pos)]; },
-            [&](int pos, cplx<TC> v) { col0[padded(p, pos)] = v; });
-        team_round<TC, E, false, false>(
-            p, s, t, tw, true, [&](int pos) { return col1[padded(p, pos)]; },
-            [&](int pos, cplx<TC> v) { col1[padded(p, pos)] = v; });
-        __syncthreads();
-    }
-    middle(false);
-    __syncthreads();
-    for (int s = K - 2; s >= 1; --s) {
-        team_round<TC, E, true, false>(
-            p, s, t, tw, true, [&](int pos) { return col0[padded(p, pos)]; },
-            [&](int pos, cplx<TC> v) { col0[padded(p, pos)] = v; });
-        team_round<TC, E, true, false>(
-            p, s, t, tw, true, [&](int pos) { return col1[padded(p, pos)]; },
-            [&](int pos, cplx<TC> v) { col1[padded(p, pos)] = v; });
-        __syncthreads();
-    }
-    // Inverse round 0 into the work array, cut to the columns.
-    team_round<TC, E, true, false>(
-        p, 0, t, tw, true, [&](int pos) { return col0[padded(p, pos)]; },
-        [&](int pos, cplx<TC> v) {
-            if (pos < n0) w0[pos] = v;
-        });
-    team_round<TC, E, true, false>(
-        p, 0, t, tw, true, [&](int pos) { return col1[padded(p, pos)]; },
-        [&](int pos, cplx<TC> v) {
-            if (pos < n1) w1[pos] = v;
-        });
 }
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -411,16 +103,11 @@ __global__ void __launch_bounds__(1024) spectra_b(const c128 *__restrict__ in, c
     }
 }
 
-__global__ void to_c64(const c128 *__restrict__ a, c64 *__restrict__ b, int64_t n) {
-    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        b[i] = cvt<float>(a[i]);
-}
 
 // ------------------------------------------------------------------------------------------------------------------
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
-static size_t lds_bytes_max() {
+size_t lds_bytes_max() {
     static size_t cached = 0;
     if (!cached) {
         int dev = 0;
@@ -433,8 +120,7 @@ static size_t lds_bytes_max() {
     return cached;
 }
 
-// Raises a kernel's dynamic LDS limit above the 64 KiB default (once per kernel and size).
-static void set_lds(const void *kernel, size_t bytes) {
+void set_lds(const void *kernel, size_t bytes) {
     if (bytes <= 64 * 1024) return;
     if (bytes > lds_bytes_max())
         fail(RT_EINVAL, "rooftop:product", "a pass needs %zu bytes of LDS, more than this device has", bytes);
@@ -447,25 +133,15 @@ static void set_lds(const void *kernel, size_t bytes) {
     have = bytes;
 }
 
-template <typename TC>
-static size_t lds_ac(const FftPlan &p, int group) {
-    return static_cast<size_t>(padded_length(p)) * group * sizeof(cplx<TC>);
-}
-template <typename TC>
-static size_t lds_b(const FftPlan &p, int tile) {
-    return static_cast<size_t>(2) * tile * padded_length(p) * sizeof(cplx<TC>);
-}
-static int64_t blocks_ac(const ProdDims &d, int nrhs, int g) {
-    return static_cast<int64_t>(nrhs) * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g);
-}
-
 // Whether a shape fits the device: threads per block and LDS.
 template <typename TC>
 static bool shape_fits(const Product &op, const PassShape &sh) {
     const FftPlan &px = op.tx.of(sh.bits).plan, &py = op.ty.of(sh.bits).plan;
+    const int es = sizeof(cplx<TC>);
     const size_t lmax = lds_bytes_max();
-    return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx && team_threads(px) * sh.group <= 1024 &&
-           team_threads(py) * sh.tile <= 1024 && lds_ac<TC>(px, sh.group) <= lmax && lds_b<TC>(py, sh.tile) <= lmax;
+    return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx &&
+           team_threads(px) * sh.group <= max_threads_ac(es, px) && team_threads(py) * sh.tile <= max_threads_b(py) &&
+           lds_ac_bytes(px, sh.group, es) <= lmax && lds_b_bytes(py, sh.tile, es) <= lmax;
 }
 
 // Default launch shape (the autotuner may replace it): radices up to 16 in complex64 and up to 8 in complex128;
@@ -478,11 +154,12 @@ static PassShape default_shape(const Product &op, int nrhs, int bits = 0) {
     const FftPlan &px = op.tx.of(s.bits).plan, &py = op.ty.of(s.bits).plan;
     const int es = sizeof(cplx<TC>);
     int g = std::max(1, 64 / es);
-    while (g > 1 && (team_threads(px) * g > 1024 || lds_ac<TC>(px, g) > 64 * 1024 || blocks_ac(d, nrhs, g) < 1024))
+    while (g > 1 && (team_threads(px) * g > std::min(1024, max_threads_ac(es, px)) ||
+                     lds_ac_bytes(px, g, es) > 64 * 1024 || blocks_ac(d, nrhs, g) < 1024))
         g >>= 1;
     s.group = g;
     int tile = 1;
-    while (tile * 2 <= d.Mx && team_threads(py) * tile * 2 <= 256 && lds_b<TC>(py, tile * 2) <= 48 * 1024 &&
+    while (tile * 2 <= d.Mx && team_threads(py) * tile * 2 <= 256 && lds_b_bytes(py, tile * 2, es) <= 48 * 1024 &&
            static_cast<int64_t>(d.Mx / (tile * 2)) * nrhs >= 2048)
         tile *= 2;
     s.tile = tile;
@@ -504,69 +181,51 @@ static void pass_bytes(const ProdDims &d, int nrhs, double *bytes) {
     const double vec = 16.0 * d.n * nrhs;
     const double work_bytes = sc * (static_cast<double>(d.cols_x) + d.cols_y) * d.Mx * nrhs;
     bytes[0] = vec + work_bytes;
-    bytes[1] = 2 * work_bytes + 3.0 * sc * d.Hx * d.Hy;
+    bytes[1] = 2 * work_bytes + 3.0 * sc * d.Hx * d.My;
     bytes[2] = work_bytes + vec;
 }
 
-// Kernels are instantiated per largest radix E of the plan and per bound MAXT on the block size (the register
-// budget of a thread grows as MAXT shrinks).
-#define RT_BY_E(E_VALUE, CALL)                      \
-    switch (E_VALUE) {                              \
-    case 1: { constexpr int E = 1; CALL; } break;   \
-    case 2: { constexpr int E = 2; CALL; } break;   \
-    case 4: { constexpr int E = 4; CALL; } break;   \
-    case 8: { constexpr int E = 8; CALL; } break;   \
-    default: { constexpr int E = 16; CALL; } break; \
+// bspec[(fx My + py) 3 + k] = plane k at (fx, fold_y(py)), in the precision of the pass.
+template <typename TC>
+__global__ void build_bspec(const c128 *__restrict__ planes, int Hx, int Hy, int My, const int *__restrict__ fold_y,
+                            cplx<TC> *__restrict__ bspec) {
+    const int64_t total = static_cast<int64_t>(Hx) * My;
+    const int64_t plane = static_cast<int64_t>(Hx) * Hy;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t fx = i / My;
+        const int py = static_cast<int>(i - fx * My);
+        const int64_t f = fx * Hy + fold_y[py];
+        for (int k = 0; k < 3; ++k) bspec[3 * i + k] = cvt<TC>(planes[k * plane + f]);
     }
-#define RT_BY_MAXT(THREADS, CALL)                                   \
-    if ((THREADS) <= 256) { constexpr int MAXT = 256; CALL; }       \
-    else if ((THREADS) <= 512) { constexpr int MAXT = 512; CALL; }  \
-    else { constexpr int MAXT = 1024; CALL; }
-
-template <typename TC, int E, int MAXT>
-static void launch_a(const Product &op, const PlanTables &t, const c128 *in, cplx<TC> *work, int nrhs, int g,
-                     hipStream_t s, const int *skip, int skip_stride) {
-    const FftPlan &p = t.plan;
-    const int64_t nb = blocks_ac(op.dims, nrhs, g);
-    const size_t lds = lds_ac<TC>(p, g);
-    set_lds(reinterpret_cast<const void *>(&pass_a<TC, E, MAXT>), lds);
-    hipLaunchKernelGGL((pass_a<TC, E, MAXT>), dim3(xcd_grid(nb)), dim3(team_threads(p) * g), lds, s, in, work,
-                       op.problem->exists.ptr, op.dims, p, tw_of<TC>(t), g, static_cast<int>(nb), skip, skip_stride);
-    RT_LAUNCH_CHECK();
 }
 
-template <typename TC, int E, int MAXT>
-static void launch_c(const Product &op, const PlanTables &t, const cplx<TC> *work, c128 *out, int nrhs, int g,
-                     hipStream_t s, const int *skip, int skip_stride) {
-    const FftPlan &p = t.plan;
-    const int64_t nb = blocks_ac(op.dims, nrhs, g);
-    const size_t lds = lds_ac<TC>(p, g);
-    set_lds(reinterpret_cast<const void *>(&pass_c<TC, E, MAXT>), lds);
-    hipLaunchKernelGGL((pass_c<TC, E, MAXT>), dim3(xcd_grid(nb)), dim3(team_threads(p) * g), lds, s, work, out,
-                       op.problem->exists.ptr, op.dims, p, tw_of<TC>(t), g, static_cast<int>(nb), skip, skip_stride);
-    RT_LAUNCH_CHECK();
-}
-
-template <typename TC, int E, int MAXT>
-static void launch_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
-                     const cplx<TC> *planes, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride) {
-    const FftPlan &p = ty.plan;
-    const int64_t nb = static_cast<int64_t>(op.dims.Mx / tile) * nrhs;
-    const size_t lds = lds_b<TC>(p, tile);
-    set_lds(reinterpret_cast<const void *>(&pass_b<TC, E, MAXT>), lds);
-    hipLaunchKernelGGL((pass_b<TC, E, MAXT>), dim3(xcd_grid(nb)), dim3(team_threads(p) * tile), lds, s, work,
-                       planes, tx.fold.ptr, ty.fold.ptr, d_of<TC>(tx), d_of<TC>(ty), op.dims, p, tw_of<TC>(ty), tile,
-                       nrhs, static_cast<int>(nb), skip, skip_stride);
-    RT_LAUNCH_CHECK();
+template <typename TC>
+static const cplx<TC> *bspec_of(Product &op, int bits, hipStream_t s) {
+    const int v = bits >= 4 ? 1 : 0;
+    const ProdDims &d = op.dims;
+    const size_t count = static_cast<size_t>(3) * d.Hx * d.My;
+    auto build = [&](auto &arr) {
+        using Elem = typename std::remove_reference<decltype(*arr.ptr)>::type;
+        if (!arr.ptr) {
+            arr.allocate(op.problem->ctx, count, "The spectra of the product");
+            hipLaunchKernelGGL((build_bspec<decltype(Elem{}.x)>), dim3(blocks_for(count / 3, 256, 8192)), dim3(256),
+                               0, s, op.planes64.ptr, d.Hx, d.Hy, d.My, op.ty.p[v].fold.ptr, arr.ptr);
+            RT_LAUNCH_CHECK();
+        }
+        return reinterpret_cast<const cplx<TC> *>(arr.ptr);
+    };
+    if (sizeof(TC) == sizeof(double)) return build(op.bspec64[v]);
+    return build(op.bspec32[v]);
 }
 
 template <typename TC>
 static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
-                       const cplx<TC> *planes, hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
+                       hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
     const ProdDims &d = op.dims;
     Context *ctx = op.problem->ctx;
     const PlanTables &tx = op.tx.of(sh.bits), &ty = op.ty.of(sh.bits);
-    const int ex = 1 << tx.plan.log2E, ey = 1 << ty.plan.log2E;
+    const cplx<TC> *bspec = bspec_of<TC>(op, sh.bits, s);
     Context::PassRecord rec{};
     const bool timed = ctx->timing;
     if (timed) {
@@ -575,12 +234,11 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
         pass_bytes<TC>(d, nrhs, rec.bytes);
         RT_HIP(hipEventRecord(rec.ev[0], s));
     }
-    const int ta = team_threads(tx.plan) * sh.group, tb = team_threads(ty.plan) * sh.tile;
-    RT_BY_E(ex, RT_BY_MAXT(ta, (launch_a<TC, E, MAXT>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride))));
+    launch_pass_a<TC>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride);
     if (timed) RT_HIP(hipEventRecord(rec.ev[1], s));
-    RT_BY_E(ey, RT_BY_MAXT(tb, (launch_b<TC, E, MAXT>(op, tx, ty, work, planes, nrhs, sh.tile, s, skip, skip_stride))));
+    launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, s, skip, skip_stride);
     if (timed) RT_HIP(hipEventRecord(rec.ev[2], s));
-    RT_BY_E(ex, RT_BY_MAXT(ta, (launch_c<TC, E, MAXT>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride))));
+    launch_pass_c<TC>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride);
     if (timed) {
         RT_HIP(hipEventRecord(rec.ev[3], s));
         ctx->pending.push_back(rec);
@@ -606,12 +264,6 @@ static cplx<TC> *work_of(Product &op, int nrhs) {
 }
 
 template <typename TC>
-static const cplx<TC> *planes_of(const Product &op) {
-    if (sizeof(TC) == sizeof(double)) return reinterpret_cast<const cplx<TC> *>(op.planes64.ptr);
-    return reinterpret_cast<const cplx<TC> *>(op.planes32.ptr);
-}
-
-template <typename TC>
 static void apply_as(Product &op, const c128 *in, c128 *out, int nrhs, hipStream_t s, const int *skip,
                      int skip_stride) {
     cplx<TC> *work = work_of<TC>(op, nrhs);
@@ -619,7 +271,7 @@ static void apply_as(Product &op, const c128 *in, c128 *out, int nrhs, hipStream
     if (!shape_fits<TC>(op, sh))
         fail(RT_EINVAL, "rooftop:product", "the FFT columns of this product (%d x %d) exceed the LDS of this device",
              op.dims.Mx, op.dims.My);
-    run_passes<TC>(op, in, out, work, nrhs, sh, planes_of<TC>(op), s, skip, skip_stride);
+    run_passes<TC>(op, in, out, work, nrhs, sh, s, skip, skip_stride);
 }
 
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
@@ -629,6 +281,16 @@ void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, 
     if (prec == RT_C128) apply_as<double>(op, in, out, nrhs, s, skip, skip_stride);
     else apply_as<float>(op, in, out, nrhs, s, skip, skip_stride);
 }
+
+// The spectra kernels are instantiated per largest radix E of their plan.
+#define RT_BY_E(E_VALUE, CALL)                      \
+    switch (E_VALUE) {                              \
+    case 1: { constexpr int E = 1; CALL; } break;   \
+    case 2: { constexpr int E = 2; CALL; } break;   \
+    case 4: { constexpr int E = 4; CALL; } break;   \
+    case 8: { constexpr int E = 8; CALL; } break;   \
+    default: { constexpr int E = 16; CALL; } break; \
+    }
 
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier) {
     Context *ctx = p->ctx;
@@ -696,16 +358,9 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
         }
         RT_HIP(hipStreamSynchronize(s));
     }
-    if (precision == RT_C64 || precision == RT_MIXED) {
-        op->planes32.allocate(ctx, 3 * plane_elems, "The spectra of the product");
-        hipLaunchKernelGGL(to_c64, dim3(blocks_for(3 * plane_elems, 256, 4096)), dim3(256), 0, s,
-                           op->planes64.ptr, op->planes32.ptr, static_cast<int64_t>(3 * plane_elems));
-        RT_LAUNCH_CHECK();
-        if (precision == RT_C64) {
-            RT_HIP(hipStreamSynchronize(s));
-            op->planes64.reset();
-        }
-    }
+    // The pass-B spectra of the default plans of the precisions held.
+    if (op->has(RT_C128)) bspec_of<double>(*op, 3, s);
+    if (op->has(RT_C64)) bspec_of<float>(*op, 4, s);
     RT_HIP(hipStreamSynchronize(s));
     return op;
 }
@@ -720,7 +375,7 @@ static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, con
                        double *bytes, hipStream_t s) {
     Context *ctx = op.problem->ctx;
     cplx<TC> *work = work_of<TC>(op, nrhs);
-    run_passes<TC>(op, in, out, work, nrhs, sh, planes_of<TC>(op), s);
+    run_passes<TC>(op, in, out, work, nrhs, sh, s);
     RT_HIP(hipStreamSynchronize(s));
     const bool was = ctx->timing;
     ctx->drain_timing();
@@ -732,7 +387,7 @@ static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, con
         by0[k] = ctx->pass_bytes[p][k];
     }
     ctx->timing = true;
-    for (int it = 0; it < reps; ++it) run_passes<TC>(op, in, out, work, nrhs, sh, planes_of<TC>(op), s);
+    for (int it = 0; it < reps; ++it) run_passes<TC>(op, in, out, work, nrhs, sh, s);
     ctx->drain_timing();
     ctx->timing = was;
     const double cnt = static_cast<double>(ctx->launches[p] - n0);
@@ -858,7 +513,8 @@ void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
 }
 
 uint64_t Product::stored_bytes() const {
-    return planes64.bytes() + planes32.bytes() + diag64.bytes() + diag32.bytes() + coef64.bytes() + coef32.bytes() +
+    return planes64.bytes() + bspec64[0].bytes() + bspec64[1].bytes() + bspec32[0].bytes() + bspec32[1].bytes() +
+           diag64.bytes() + diag32.bytes() + coef64.bytes() + coef32.bytes() +
            cls.bytes();
 }
 

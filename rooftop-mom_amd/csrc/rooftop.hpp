@@ -83,7 +83,10 @@ struct Product {
     // Operator.
     DimTables tx, ty;
     DeviceArray<c128> planes64;      // [3][Hx][Hy]: magneticXX, magneticYY, electric, scaled by 1/(Mx My)
-    DeviceArray<c64> planes32;
+    // The same spectra as pass B reads them, per plan (radices up to 8, up to 16): [Hx][My][3], x folded, y in the
+    // plan's position order, the three values of a position together. Built when a plan is first used.
+    DeviceArray<c128> bspec64[2];
+    DeviceArray<c64> bspec32[2];
     DeviceArray<c128> work64;        // [nrhs][2][Mx][ldj]
     DeviceArray<c64> work32;
     int work_rhs64 = 0, work_rhs32 = 0;
