@@ -13,7 +13,8 @@ roofline of the dominant matvec kernel (pass B: y transforms + fused spectral mu
 on the library's stream over the timed solves, and a CPU baseline: the oracle's FP64 split-FFT GMRES on a bounded
 sample of the same solve, timed on this host.
 
-A third form solves the same plate under 64 plane waves together (batched mixed and batched FP64 solves/s).
+Batched forms solve the same plate under 64 plane waves together (GMRES(50) and COCG, each mixed and FP64;
+batch solves/s, p50/p99 batch latency), and the matvec is profiled at 1 and 64 right-hand sides.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--batch 64] [--no-cpu]
 """
@@ -32,6 +33,63 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'solves/s + p50/p99 latency at rel.res 1e-6, mixed vs FP64; matvec HBM GB/s'
 PEAK_HBM_GBS = 8000.0
+
+
+# Timed kernel slots of the library (RT_TIMED_* of include/rooftop.h) and the kernel symbols they time.
+TIMED = ['pass_a_k<double> (x FFT)', 'pass_b_k<double> (y FFT + multiply)', 'pass_c_k<double> (inverse x FFT)',
+         'pass_a_k<float> (x FFT)', 'pass_b_k<float> (y FFT + multiply)', 'pass_c_k<float> (inverse x FFT)',
+         'k_cgs_dot (CGS2 h1 = V^H w)', 'k_cgs_update_dot (w -= V h1; h2 = V^H w)',
+         'k_cgs_update_norm (w -= V h2; |w|^2)', 'sai_apply<double> (near field)', 'sai_apply<float> (near field)']
+
+
+class KernelClock:
+    """The library's live kernel timing (HIP events around each timed launch on the library's stream)."""
+
+    def __init__(self, rt, ctx):
+        self.rt, self.ctx, self.lib = rt, ctx, rt.library()
+        self.n = len(TIMED)
+        self.launches = (C.c_int64 * self.n)()
+        self.ms = (C.c_double * self.n)()
+        self.by = (C.c_double * self.n)()
+
+    def start(self):
+        self.rt.check(self.lib.rt_ctx_set_timing(self.ctx.handle, 0))
+        self.rt.check(self.lib.rt_ctx_kernel_timing(self.ctx.handle, self.launches, self.ms, self.by))
+        self.rt.check(self.lib.rt_ctx_set_timing(self.ctx.handle, 1))
+
+    def stop(self):
+        """{slot name: {launches, ms_per_launch, bytes_per_launch, GBps, total_ms}} of the kernels that ran."""
+        self.rt.check(self.lib.rt_ctx_set_timing(self.ctx.handle, 0))
+        self.rt.check(self.lib.rt_ctx_kernel_timing(self.ctx.handle, self.launches, self.ms, self.by))
+        out = {}
+        for k, name in enumerate(TIMED):
+            if self.launches[k]:
+                out[name] = {'launches': int(self.launches[k]), 'ms_per_launch': round(self.ms[k], 6),
+                             'bytes_per_launch': round(self.by[k]),
+                             'GBps': round(self.by[k] / (self.ms[k] * 1e-3) / 1e9, 1) if self.ms[k] > 0 else 0.0,
+                             'total_ms': round(self.launches[k] * self.ms[k], 3)}
+        return out
+
+
+def roofline_of(kernels, name, traffic=None):
+    k = kernels[name]
+    achieved = k['GBps']
+    return {'bound': 'hbm', 'kernel': name, 'achieved': achieved, 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
+            'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': traffic, 'launches': k['launches'],
+            'ms_per_launch': k['ms_per_launch'], 'bytes_per_launch': k['bytes_per_launch']}
+
+
+def measured_traffic(kernel_symbol, workload):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary of this workload, if any:
+    profiles/traffic_<kernel>_<workload>.txt holds a line 'traffic_bytes_per_launch: <bytes>'."""
+    path = os.path.join(ROOT, 'profiles', 'traffic_%s_%s.txt' % (kernel_symbol, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith('traffic_bytes_per_launch:'):
+                return float(line.split(':', 1)[1])
+    return None
 
 
 def percentile(values, q):
@@ -136,6 +194,8 @@ def main():
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--batch', type=int, default=64, help='right-hand sides of the batched form (0: skip it)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-only', action='store_true',
+                    help='only the plain 1 GiB device copy (calibrates the HBM byte counters of rocprofv3)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: this build runs the path on a single MI355X and does not shard a solve '
@@ -147,6 +207,13 @@ def main():
     rt = ge.load()
     lib = rt.library()
     ctx = rt.Context.default()
+    copy_bytes = 1 << 30
+    copy_ms = C.c_double()
+    rt.check(lib.rt_bandwidth_probe(ctx.handle, copy_bytes, 1 if args.copy_only else 20, C.byref(copy_ms)))
+    copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
+    if args.copy_only:
+        print(json.dumps({'copy_bytes': copy_bytes, 'ms': copy_ms.value, 'GBps': round(copy_gbs, 1)}), flush=True)
+        return
     n = args.n
     print('bench: plate %dx%d, assembling generators and spectra' % (n, n), flush=True)
     t0 = time.perf_counter()
@@ -175,36 +242,32 @@ def main():
     print('bench: fp64 form', flush=True)
     t64, i64, o64, r64 = run_form(rt, ctx, op, pc, fp64, d_b.value, d_x64.value, N, args.steps, args.warmup)
     print('bench: mixed form', flush=True)
-    rt.check(lib.rt_ctx_set_timing(ctx.handle, 0))
-    launches = (C.c_int64 * 2)()
-    ms = (C.c_double * 6)()
-    by = (C.c_double * 6)()
-    rt.check(lib.rt_ctx_kernel_timing(ctx.handle, launches, ms, by))        # reset the counters
-    rt.check(lib.rt_ctx_set_timing(ctx.handle, 1))
     tmx, imx, omx, rmx = run_form(rt, ctx, op, pc, mixed, d_b.value, d_x.value, N, args.steps, args.warmup)
-    rt.check(lib.rt_ctx_set_timing(ctx.handle, 0))
-    rt.check(lib.rt_ctx_kernel_timing(ctx.handle, launches, ms, by))
     x64 = np.empty(N, np.complex128)
     xmx = np.empty(N, np.complex128)
     rt.check(lib.rt_copy_to_host(ctx.handle, x64.ctypes.data_as(C.c_void_p), d_x64, 16 * N))
     rt.check(lib.rt_copy_to_host(ctx.handle, xmx.ctypes.data_as(C.c_void_p), d_x, 16 * N))
     err = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+    # Kernel table: a separate run of the mixed form with the library's event timing on (the events themselves add
+    # launch gaps, so the headline solves above run without them). The dominant kernel by total GPU time gives the
+    # roofline.
+    print('bench: mixed form, kernel timing', flush=True)
+    clock = KernelClock(rt, ctx)
+    clock.start()
+    run_form(rt, ctx, op, pc, mixed, d_b.value, d_x.value, N, max(3, args.steps // 4), 0)
+    kernels = clock.stop()
+    dominant = max(kernels, key=lambda k: kernels[k]['total_ms'])
+    workload = 'plate%d' % n
+    roofline = roofline_of(kernels, dominant, measured_traffic(dominant.split(' ')[0].split('<')[0], workload))
 
-    # Live roofline: complex64 pass B launches of the timed mixed solves (inner products); FP64 ones are the outer
-    # residuals. ms/bytes are per launch averages.
-    b_ms, b_bytes = ms[4], by[4]
-    achieved = b_bytes / (b_ms * 1e-3) / 1e9 if b_ms > 0 else 0.0
-    prof_ms = (C.c_double * 3)()
-    prof_by = (C.c_double * 3)()
-    rt.check(lib.rt_product_profile(op.handle, 1, 50, rt._abi.RT_C64, prof_ms, prof_by))
-    matvec = {'c64': {}, 'c128': {}}
-    for name, prec in (('c64', rt._abi.RT_C64), ('c128', rt._abi.RT_C128)):
-        rt.check(lib.rt_product_profile(op.handle, 1, 50, prec, prof_ms, prof_by))
-        total_ms = sum(prof_ms)
-        total_bytes = sum(prof_by)
-        matvec[name] = {'ms': round(total_ms, 5), 'bytes': total_bytes,
-                        'GBps': round(total_bytes / (total_ms * 1e-3) / 1e9, 1),
-                        'pass_ms': [round(v, 5) for v in prof_ms]}
+    matvec = {}
+    for R in sorted({1, args.batch} - {0}):
+        for prec in ('c64', 'c128'):
+            pms, pby = op.profile(nrhs=R, reps=50 if R == 1 else 10, precision=prec)
+            matvec['%s x%d' % (prec, R)] = {'ms': round(sum(pms), 5), 'bytes': sum(pby),
+                                            'GBps': round(sum(pby) / (sum(pms) * 1e-3) / 1e9, 1),
+                                            'frac': round(sum(pby) / (sum(pms) * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                            'pass_ms': [round(v, 5) for v in pms]}
 
     # Batched form: the same plate under `batch` plane waves, solved together (one product launch per pass for all).
     batch = {}
@@ -216,7 +279,10 @@ def main():
             rt.check(lib.rt_device_alloc(ctx.handle, 16 * N * R, C.byref(ptr)))
         rt.check(lib.rt_copy_to_device(ctx.handle, d_B, B.ctypes.data_as(C.c_void_p), 16 * N * R))
         bsteps, bwarm = max(1, args.steps // 10), 1
-        for name, solver in (('mixed', mixed), ('fp64', fp64)):
+        forms = (('gmres mixed', mixed), ('gmres fp64', fp64),
+                 ('cocg mixed', rt.Solver('cg', tolerance=1e-6, maxIterationCount=2000, precision='c64')),
+                 ('cocg fp64', rt.Solver('cg', tolerance=1e-6, maxIterationCount=2000, refine=True)))
+        for name, solver in forms:
             print('bench: batched %s form, %d right-hand sides' % (name, R), flush=True)
             bt, outs = run_batch(rt, ctx, op, pc, solver, d_B.value, d_X.value, N, R, bsteps, bwarm)
             its = [q[0] for q in outs]
@@ -224,11 +290,18 @@ def main():
                            'batch_p50_ms': round(1e3 * percentile(bt, 50), 3),
                            'batch_p99_ms': round(1e3 * percentile(bt, 99), 3),
                            'iterations_median': int(np.median(its)), 'iterations_max': int(max(its)),
+                           'outer_max': int(max(q[1] for q in outs)),
                            'residual_max': max(q[2] for q in outs), 'steps': bsteps}
+        print('bench: batched gmres mixed form, kernel timing', flush=True)
+        clock.start()
+        run_batch(rt, ctx, op, pc, mixed, d_B.value, d_X.value, N, R, 1, 0)
+        batch['gmres mixed']['kernels'] = clock.stop()
         batch['rhs'] = R
         batch['excitations'] = ('plane waves, theta U[0,80] deg, phi U[0,360) deg, TE/TM at random (seed 1), '
                                 'solved together')
-        batch['mixed_vs_fp64_speedup'] = round(batch['mixed']['solves_per_s'] / batch['fp64']['solves_per_s'], 3)
+        for m in ('gmres', 'cocg'):
+            batch['%s mixed_vs_fp64_speedup' % m] = round(batch['%s mixed' % m]['solves_per_s'] /
+                                                          batch['%s fp64' % m]['solves_per_s'], 3)
         for ptr in (d_B, d_X):
             rt.check(lib.rt_device_free(ctx.handle, ptr))
 
@@ -250,11 +323,10 @@ def main():
                  'outer': int(np.median(o64)), 'residual': max(r64)},
         'mixed_vs_fp64_speedup': round(sum(t64) / sum(tmx), 3),
         'error_mixed_vs_fp64': err,
-        'roofline': {'bound': 'hbm', 'kernel': 'pass_b<float> (y FFT + spectral multiply + inverse y FFT)',
-                     'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
-                     'launches': int(launches[1]), 'ms_per_launch': round(b_ms, 5), 'bytes_per_launch': b_bytes},
+        'roofline': roofline,
+        'kernels': kernels,
         'matvec': matvec,
+        'copy_ceiling_GBps': round(copy_gbs, 1),
         'batched': batch,
         'setup_s': round(setup_s, 3),
         'tuning': {'seconds': round(tune_s, 3), 'shapes': tuned},

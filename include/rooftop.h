@@ -157,13 +157,31 @@ int rt_product_tune(rt_product *product, int64_t nrhs, int32_t precision, int32_
  * RT_EINVAL when the shape is malformed or does not fit the device for this product. */
 int rt_product_set_shape(rt_product *product, int64_t nrhs, int32_t precision, const int32_t *shape);
 
-/* Measurement (no counterpart in the reference): while timing is on, every operator application records HIP events
- * around its three kernels on the context's stream. rt_ctx_kernel_timing drains them (synchronizing the stream) and
- * reports, per precision p (0: RT_C128, 1: RT_C64) and pass k, the launches counted, the average duration of one
- * launch in ms (ms[3p + k]) and the average algorithmic bytes of one launch (bytes[3p + k]); then it resets the
- * counters. */
+/* Measurement (no counterpart in the reference): while timing is on, the timed kernels below record a pair of HIP
+ * events around each launch on the stream they run on, with the launch's algorithmic HBM bytes (every element it
+ * must read or write counted once). rt_ctx_kernel_timing drains them (synchronizing the stream) and reports, per
+ * slot, the launches counted, the average duration of one launch in ms and its average algorithmic bytes; then it
+ * resets the counters. The arrays hold RT_TIMED_COUNT entries. */
+enum {
+    RT_TIMED_PASS_A_C128 = 0,   /* matvec pass A (x transforms), complex128 */
+    RT_TIMED_PASS_B_C128 = 1,   /* matvec pass B (y transforms + spectral multiply) */
+    RT_TIMED_PASS_C_C128 = 2,   /* matvec pass C (inverse x transforms) */
+    RT_TIMED_PASS_A_C64 = 3,
+    RT_TIMED_PASS_B_C64 = 4,
+    RT_TIMED_PASS_C_C64 = 5,
+    RT_TIMED_CGS_DOT = 6,         /* GMRES CGS2 first pass: h1 = V^H w */
+    RT_TIMED_CGS_UPDATE_DOT = 7,  /* w -= V h1 fused with h2 = V^H w */
+    RT_TIMED_CGS_UPDATE_NORM = 8, /* w -= V h2 fused with |w|^2 */
+    RT_TIMED_PRECOND_C128 = 9,    /* preconditioner application */
+    RT_TIMED_PRECOND_C64 = 10,
+    RT_TIMED_COUNT = 11
+};
 int rt_ctx_set_timing(rt_ctx *ctx, int32_t on);
 int rt_ctx_kernel_timing(rt_ctx *ctx, int64_t *launches, double *ms, double *bytes);
+/* Measurement (no counterpart in the reference): a plain device copy kernel of `bytes` bytes (one read, one write
+ * per byte), `reps` times after one warm-up; *ms receives the average duration. 2 bytes / ms is the streaming HBM
+ * bandwidth of this device, and the copy calibrates the rocprofv3 FETCH_SIZE / WRITE_SIZE counters. */
+int rt_bandwidth_probe(rt_ctx *ctx, uint64_t bytes, int32_t reps, double *ms);
 
 /* rooftopMoM.Solver(...).solve(operator, rhs, 'preconditioner', P, 'initialGuess', x0) (Solver.m:42-92,
  * IterativeMethod.m:27-70). precond may be NULL; x0 may be NULL (zeros). The nrhs (1..1024) right-hand sides are

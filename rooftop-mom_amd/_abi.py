@@ -36,6 +36,7 @@ SIGNATURES = [
     ('rt_ctx_stream', C.c_int, [P, C.POINTER(P)]),
     ('rt_ctx_set_timing', C.c_int, [P, C.c_int32]),
     ('rt_ctx_kernel_timing', C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ('rt_bandwidth_probe', C.c_int, [P, C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
     ('rt_problem_create', C.c_int, [P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32,
                                     C.c_int32, P, C.POINTER(P)]),
     ('rt_problem_destroy', None, [P]),

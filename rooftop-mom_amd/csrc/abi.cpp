@@ -173,8 +173,10 @@ void rt_ctx_destroy(rt_ctx *ctx) {
         (void)hipStreamDestroy(ctx->c.stream);
     }
     for (auto &kv : ctx->c.sizes) (void)hipFree(kv.first);
-    for (auto &r : ctx->c.pending)
-        for (hipEvent_t e : r.ev) (void)hipEventDestroy(e);
+    for (auto &r : ctx->c.pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
     for (hipEvent_t e : ctx->c.event_pool) (void)hipEventDestroy(e);
     if (ctx->c.host_pinned) (void)hipHostFree(ctx->c.host_pinned);
     delete ctx;
@@ -216,17 +218,23 @@ int rt_ctx_kernel_timing(rt_ctx *ctx, int64_t *launches, double *ms, double *byt
         ctx->c.activate();
         RT_HIP(hipStreamSynchronize(ctx->c.stream));
         ctx->c.drain_timing();
-        for (int p = 0; p < 2; ++p) {
-            launches[p] = ctx->c.launches[p];
-            for (int k = 0; k < 3; ++k) {
-                const double n = ctx->c.launches[p] ? static_cast<double>(ctx->c.launches[p]) : 1.0;
-                ms[3 * p + k] = ctx->c.pass_ms[p][k] / n;
-                bytes[3 * p + k] = ctx->c.pass_bytes[p][k] / n;
-                ctx->c.pass_ms[p][k] = 0;
-                ctx->c.pass_bytes[p][k] = 0;
-            }
-            ctx->c.launches[p] = 0;
+        for (int k = 0; k < RT_TIMED_COUNT; ++k) {
+            const double n = ctx->c.timed_launches[k] ? static_cast<double>(ctx->c.timed_launches[k]) : 1.0;
+            launches[k] = ctx->c.timed_launches[k];
+            ms[k] = ctx->c.timed_ms[k] / n;
+            bytes[k] = ctx->c.timed_bytes[k] / n;
+            ctx->c.timed_launches[k] = 0;
+            ctx->c.timed_ms[k] = 0;
+            ctx->c.timed_bytes[k] = 0;
         }
+    });
+}
+
+int rt_bandwidth_probe(rt_ctx *ctx, uint64_t bytes, int32_t reps, double *ms) {
+    return guarded([&] {
+        need(ctx && ms, RT_EINVAL, "rooftop:ctx", "NULL argument");
+        ctx->c.activate();
+        *ms = bandwidth_probe(&ctx->c, bytes, reps);
     });
 }
 

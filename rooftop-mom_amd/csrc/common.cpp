@@ -107,16 +107,15 @@ hipEvent_t Context::take_event() {
 }
 
 void Context::drain_timing() {
-    for (PassRecord &r : pending) {
-        RT_HIP(hipEventSynchronize(r.ev[3]));
-        for (int k = 0; k < 3; ++k) {
-            float t = 0;
-            RT_HIP(hipEventElapsedTime(&t, r.ev[k], r.ev[k + 1]));
-            pass_ms[r.precision][k] += t;
-            pass_bytes[r.precision][k] += r.bytes[k];
-        }
-        launches[r.precision] += 1;
-        for (hipEvent_t e : r.ev) event_pool.push_back(e);
+    for (TimedRecord &r : pending) {
+        RT_HIP(hipEventSynchronize(r.b));
+        float t = 0;
+        RT_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        timed_ms[r.slot] += t;
+        timed_bytes[r.slot] += r.bytes;
+        timed_launches[r.slot] += 1;
+        event_pool.push_back(r.a);
+        event_pool.push_back(r.b);
     }
     pending.clear();
 }

@@ -84,21 +84,21 @@ struct Context {
     std::mutex lock;
     std::unordered_map<void *, uint64_t> sizes;
 
-    // Kernel timing of the operator passes (rt_ctx_set_timing): events recorded around each pass, drained later.
-    struct PassRecord {
-        hipEvent_t ev[4];
-        int precision;        // 0 FP64, 1 complex64
-        double bytes[3];
+    // Kernel timing (rt_ctx_set_timing): while on, the timed kernels (RT_TIMED_* slots of rooftop.h) record a pair of
+    // HIP events around their launch on the stream they run on, with their algorithmic bytes; drain_timing() reads
+    // the events and accumulates per slot.
+    struct TimedRecord {
+        hipEvent_t a, b;
+        int slot;
+        double bytes;
     };
-    // Set by a Krylov loop while it runs: the kernels of the products return at once when *skip != 0 (the loop has
-    // converged on the device and the host has not polled yet).
-    const int *skip = nullptr;
     bool timing = false;
-    std::vector<PassRecord> pending;
+    std::vector<TimedRecord> pending;
     std::vector<hipEvent_t> event_pool;
-    int64_t launches[2] = {0, 0};
-    double pass_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    double pass_bytes[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    double timed_scale = 1.0;   // share of a batch's systems still running (set by a Krylov loop at each poll)
+    int64_t timed_launches[RT_TIMED_COUNT] = {};
+    double timed_ms[RT_TIMED_COUNT] = {};
+    double timed_bytes[RT_TIMED_COUNT] = {};
     hipEvent_t take_event();
     void drain_timing();
 
@@ -117,6 +117,29 @@ struct Context {
 };
 
 // A device buffer owned through a context.
+// Brackets one timed kernel launch: construct before the launch, call done() after it.
+struct KernelTimer {
+    Context *ctx;
+    int slot;
+    double bytes;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    KernelTimer(Context *c, int sl, double by, hipStream_t st) : ctx(c), slot(sl), bytes(by), s(st) {
+        if (ctx->timing) {
+            a = ctx->take_event();
+            RT_HIP(hipEventRecord(a, s));
+        }
+    }
+    void done() {
+        if (!a) return;
+        hipEvent_t b = ctx->take_event();
+        RT_HIP(hipEventRecord(b, s));
+        ctx->pending.push_back(Context::TimedRecord{a, b, slot, bytes * ctx->timed_scale});
+        a = nullptr;
+        if (ctx->pending.size() > 8192) ctx->drain_timing();
+    }
+};
+
 template <typename T>
 struct DeviceArray {
     Context *ctx = nullptr;

@@ -31,12 +31,14 @@ namespace rt {
 constexpr int kThreads = 256;
 constexpr int kLdp = 64;        // partial columns per block
 constexpr int kPoll = 8;        // steps enqueued between two polls of the loop states
+constexpr int kStall = 40;      // COCG steps without a new best norm that end a complex64 inner solve
 
 // Scalar slots of a system.
 enum ScalSlot {
     kRho0 = 0, kRho1 = 1, kDad = 2, kRnormCg = 3,   // COCG
     kH2 = 0, kH1 = 64,                              // GMRES: second- and first-pass coefficients
     kRnorm = 128, kInvSub = 130, kReference = 132, kTol = 134, kBnorm = 136,
+    kMaxIt = 138, kBest = 140,                      // per-system iteration limit; COCG best monitored norm
     kY = 256,                                       // GMRES: solution coefficients of a cycle
     kOuterB = 400, kOuterRes = 401
 };
@@ -200,9 +202,12 @@ __global__ void __launch_bounds__(kThreads) k_cocg_update(c128 *__restrict__ x, 
     if (threadIdx.x == 0) partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp] = mk(s, 0.0);
 }
 
-// The residual norm of iteration `it`, the stopping test (ConjugateGradient.m:126-136). Grid (1, R).
+// The residual norm of iteration `it`, the stopping test (ConjugateGradient.m:126-136) with the system's own
+// iteration limit. stall > 0 (inner solves of the refinement loop only) also stops a system whose monitored norm has
+// not improved on its best for `stall` iterations: a complex64 short recurrence that has lost its complex symmetry to
+// rounding, which the outer FP64 loop then restarts from the true residual. Grid (1, R).
 __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restrict__ partial, int blocks,
-                                                            const c128 *__restrict__ scal, int it, int max_iter,
+                                                            c128 *__restrict__ scal, int it, int stall,
                                                             LoopState *st, double *history, int cap) {
     SYSTEM(r);
     __shared__ double sh[8];
@@ -210,14 +215,19 @@ __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restri
     for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[(static_cast<int64_t>(r) * blocks + b) * kLdp].x;
     s = block_sum(s, sh);
     if (threadIdx.x != 0) return;
-    const double res = sqrt(s) / scal[r * kScal + kBnorm].x;
+    c128 *sc = scal + r * kScal;
+    const double res = sqrt(s) / sc[kBnorm].x;
     history[static_cast<int64_t>(r) * cap + it] = res;
     st[r].iterations = it;
-    if (res < scal[r * kScal + kTol].x) {
+    if (res < sc[kBest].x) {
+        sc[kBest] = mk(res, 0.0);
+        st[r].pad[0] = it;
+    }
+    if (res < sc[kTol].x) {
         st[r].converged = st[r].finished = st[r].done = 1;
     } else if (!isfinite(res)) {
         st[r].finished = st[r].done = 1;
-    } else if (it >= max_iter) {
+    } else if (it >= static_cast<int>(sc[kMaxIt].x) || (stall > 0 && it - st[r].pad[0] >= stall)) {
         st[r].done = 1;
     }
 }
@@ -351,7 +361,7 @@ __global__ void __launch_bounds__(kThreads) k_cgs_update_norm(const c128 *__rest
 __global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restrict__ partial, int blocks,
                                                              c128 *__restrict__ scal, c128 *__restrict__ Hall,
                                                              c128 *__restrict__ gall, int m, int j, int it,
-                                                             int max_iter, LoopState *st, double *history, int cap) {
+                                                             LoopState *st, double *history, int cap) {
     SYSTEM(r);
     __shared__ double sh[8];
     double s = 0;
@@ -392,7 +402,7 @@ __global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restr
         st[r].converged = st[r].finished = st[r].done = 1;
     } else if (!isfinite(res)) {
         st[r].finished = st[r].done = 1;
-    } else if (it >= max_iter) {
+    } else if (it >= static_cast<int>(sc[kMaxIt].x)) {
         st[r].done = 1;
     }
     sc[kInvSub] = (!conv && sub != 0) ? mk(1.0 / sub, 0.0) : mk(0.0, 0.0);
@@ -542,6 +552,12 @@ void Workspace::reset_state(const std::vector<char> &finished) {
     RT_HIP(hipStreamSynchronize(s));
 }
 
+void Workspace::set_scale(const std::vector<LoopState> &st) {
+    int active = 0;
+    for (const auto &q : st) active += q.done ? 0 : 1;
+    ctx->timed_scale = static_cast<double>(active) / R;
+}
+
 std::vector<LoopState> Workspace::read_state() {
     RT_HIP(hipMemcpyAsync(host, state, sizeof(LoopState) * R, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
@@ -604,8 +620,21 @@ static InnerResult collect(Workspace &ws, const std::vector<LoopState> &state, c
 // ------------------------------------------------------------------------------------------------------------------
 // COCG (ConjugateGradient.m:32-59) on the systems not marked `skip_sys`. Tolerances per system.
 // ------------------------------------------------------------------------------------------------------------------
+// Iteration limits per system; returns the largest over the systems that run.
+static int put_limits(Workspace &ws, const std::vector<int> &limits, const std::vector<char> &skip_sys) {
+    std::vector<double> v(ws.R);
+    int top = 0;
+    for (int r = 0; r < ws.R; ++r) {
+        v[r] = limits[r];
+        if (!skip_sys[r]) top = std::max(top, limits[r]);
+    }
+    ws.put_re(kMaxIt, v);
+    return top;
+}
+
 static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c128 *b, c128 *x, bool has_x0,
-                        const std::vector<double> &tol, const std::vector<char> &skip_sys, int max_iter) {
+                        const std::vector<double> &tol, const std::vector<char> &skip_sys,
+                        const std::vector<int> &limits, int stall) {
     Context *ctx = ws.ctx;
     const int64_t n = ws.n;
     const int R = ws.R;
@@ -614,6 +643,8 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
     c128 *z = P ? vec(ctx, kSlotZ, R * n) : nullptr;
     ws.reset_state(skip_sys);
     ws.put_re(kTol, tol);
+    ws.put_re(kBest, std::vector<double>(R, HUGE_VAL));
+    const int max_iter = put_limits(ws, limits, skip_sys);
     if (has_x0) {
         A(x, ad, ws.skip());
         ws.sub(b, ad, r, st);
@@ -652,11 +683,13 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
                                i_new, n, ws.chunk, ws.partial, st);
             RT_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_cocg_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
-                               it, max_iter, ws.state, ws.history, ws.cap);
+                               it, stall, ws.state, ws.history, ws.cap);
             RT_LAUNCH_CHECK();
         }
         state = ws.read_state();
+        ws.set_scale(state);
     }
+    ctx->timed_scale = 1.0;
     return collect(ws, state, skip_sys, r0, b0);
 }
 
@@ -664,12 +697,13 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
 // GMRES(m), left preconditioned, CGS2 (iterate_.m:1-85), on the systems not marked `skip_sys`.
 // ------------------------------------------------------------------------------------------------------------------
 static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c128 *b, c128 *x, bool has_x0,
-                         const std::vector<double> &tol, const std::vector<char> &skip_sys, int max_iter,
-                         int restart) {
+                         const std::vector<double> &tol, const std::vector<char> &skip_sys,
+                         const std::vector<int> &limits, int restart) {
     Context *ctx = ws.ctx;
     const int64_t n = ws.n;
     const int R = ws.R;
     const LoopState *st = ws.state;
+    const int max_iter = put_limits(ws, limits, skip_sys);
     int64_t m64 = restart > 0 ? restart : n;
     m64 = std::min<int64_t>({m64, n, static_cast<int64_t>(max_iter)});
     const int m = static_cast<int>(m64);
@@ -722,29 +756,38 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 ++it;
                 A(V + static_cast<int64_t>(j) * R * n, t, ws.skip());   // column j of every system
                 precondition(t, w);
+                // Algorithmic bytes of the CGS2 passes: the live basis columns once, w read (and written).
+                const double col = 16.0 * n * R;
+                KernelTimer t1(ctx, RT_TIMED_CGS_DOT, (step + 1) * col, ws.s);
                 hipLaunchKernelGGL(k_cgs_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, w, n, ws.chunk,
                                    ws.partial, st);
                 RT_LAUNCH_CHECK();
+                t1.done();
                 hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
                                    kH1, 0, st);
                 RT_LAUNCH_CHECK();
+                KernelTimer t2(ctx, RT_TIMED_CGS_UPDATE_DOT, (step + 2) * col, ws.s);
                 hipLaunchKernelGGL(k_cgs_update_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
                                    w, n, ws.chunk, ws.partial, st);
                 RT_LAUNCH_CHECK();
+                t2.done();
                 hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
                                    kH2, 0, st);
                 RT_LAUNCH_CHECK();
+                KernelTimer t3(ctx, RT_TIMED_CGS_UPDATE_NORM, (step + 2) * col, ws.s);
                 hipLaunchKernelGGL(k_cgs_update_norm, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
                                    w, n, ws.chunk, ws.partial, st);
                 RT_LAUNCH_CHECK();
+                t3.done();
                 hipLaunchKernelGGL(k_gmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
-                                   ws.scal, H, g, m, j, it, max_iter, ws.state, ws.history, ws.cap);
+                                   ws.scal, H, g, m, j, it, ws.state, ws.history, ws.cap);
                 RT_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_next_basis, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, V, step, n, ws.scal,
                                    st);
                 RT_LAUNCH_CHECK();
             }
             state = ws.read_state();
+            ws.set_scale(state);
             if (all_done(state)) break;
         }
         // x += V y with y = H \ g over the steps of this cycle (systems idle this cycle have 0 steps).
@@ -763,6 +806,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
         ws.nrm2(w, kRnorm, st);
     }
     RT_HIP(hipStreamSynchronize(ws.s));
+    ctx->timed_scale = 1.0;
     return collect(ws, state, skip_sys, r0, ref);
 }
 
@@ -805,9 +849,9 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
     };
     const LinOp *P = pc ? &Pop : nullptr;
     auto run = [&](const c128 *rhs, c128 *sol, bool has_x0, const std::vector<double> &tol,
-                   const std::vector<char> &skip_sys, int max_it) {
-        if (o.method == RT_METHOD_CG) return cocg(ws, A, P, rhs, sol, has_x0, tol, skip_sys, max_it);
-        return gmres(ws, A, P, rhs, sol, has_x0, tol, skip_sys, max_it, o.restart);
+                   const std::vector<char> &skip_sys, const std::vector<int> &limits, int stall) {
+        if (o.method == RT_METHOD_CG) return cocg(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, stall);
+        return gmres(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
     };
     if (!any) {
         ws.zero(x);
@@ -818,7 +862,7 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
     } else if (!o.refine) {
         if (x0 && x0 != x) ws.copy(x0, x, nullptr);
         std::vector<double> tol(R, o.tolerance);
-        InnerResult ir = run(b, x, x0 != nullptr, tol, zero_rhs, o.max_iterations);
+        InnerResult ir = run(b, x, x0 != nullptr, tol, zero_rhs, std::vector<int>(R, o.max_iterations), 0);
         const std::vector<double> rel = true_residual(x);
         for (int r = 0; r < R; ++r) {
             if (zero_rhs[r]) {
@@ -848,10 +892,17 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
             done[r] = zero_rhs[r] || rel[r] < o.tolerance;
             out[r].norms.push_back(rel[r]);
         }
-        int outer = 0, used = 0;
-        while (outer < o.max_outer && used < o.max_iterations) {
+        // An inner complex64 COCG that stagnates (no new best norm for kStall iterations) hands back to this loop.
+        const int stall = o.method == RT_METHOD_CG && o.precision == RT_C64 ? kStall : 0;
+        int outer = 0;
+        while (outer < o.max_outer) {
             bool active = false;
-            for (int r = 0; r < R; ++r) active = active || !done[r];
+            std::vector<int> limits(R, 0);
+            for (int r = 0; r < R; ++r) {
+                if (!done[r] && total[r] >= o.max_iterations) done[r] = 1;   // this system's budget is spent
+                limits[r] = o.max_iterations - total[r];
+                active = active || !done[r];
+            }
             if (!active) break;
             ++outer;
             ws.copy(t, rv, nullptr);   // t holds b - A x from true_residual
@@ -859,16 +910,13 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
             std::vector<double> want(R, 0.5);
             for (int r = 0; r < R; ++r)
                 if (!done[r]) want[r] = std::min(0.5, std::max(inner_tol, 0.5 * o.tolerance / rel[r]));
-            InnerResult ir = run(rv, d, false, want, done, o.max_iterations - used);
-            int most = 0;
+            InnerResult ir = run(rv, d, false, want, done, limits, stall);
             for (int r = 0; r < R; ++r) {
                 if (done[r]) continue;
                 total[r] += ir.iterations[r];
-                most = std::max(most, ir.iterations[r]);
                 out[r].outer = outer;
                 for (size_t k = 1; k < ir.norms[r].size(); ++k) out[r].norms.push_back(ir.norms[r][k] * rel[r]);
             }
-            used += most;
             ws.add(d, x, nullptr);   // d is zero for the systems the inner solve left out
             rel = true_residual(x);
             for (int r = 0; r < R; ++r)

@@ -71,6 +71,7 @@ struct Workspace {
     void zero(c128 *x);
     void reset_state(const std::vector<char> &finished);
     std::vector<LoopState> read_state();
+    void set_scale(const std::vector<LoopState> &st);   // share of systems running, for the kernel timers
     std::vector<double> read_history(int r, int first, int count);
 };
 

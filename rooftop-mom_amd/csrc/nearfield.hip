@@ -344,6 +344,12 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     if (!pc.has(prec)) fail(RT_EINVAL, "rooftop:product", "the preconditioner does not hold the requested precision");
     const ProdDims &d = pc.dims;
     const int64_t total = d.n * nrhs;
+    // Algorithmic bytes: the vectors in and out, the class (or diagonal) of every rooftop, the coefficient table.
+    const int es = prec == RT_C128 ? 16 : 8;
+    const double bytes = 32.0 * total +
+                         (pc.type == kJacobi ? static_cast<double>(es) * d.n
+                                             : 4.0 * d.n + 18.0 * es * pc.class_count);
+    KernelTimer timer(pc.problem->ctx, prec == RT_C128 ? RT_TIMED_PRECOND_C128 : RT_TIMED_PRECOND_C64, bytes, s);
     if (pc.type == kJacobi) {
         const unsigned blocks = blocks_for(total, 256, 8192);
         if (prec == RT_C128)
@@ -353,6 +359,7 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
             hipLaunchKernelGGL(jacobi_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag32.ptr,
                                pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip, skip_stride);
         RT_LAUNCH_CHECK();
+        timer.done();
         return;
     }
     const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
@@ -364,6 +371,7 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
         hipLaunchKernelGGL(sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef32.ptr,
                            d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
     RT_LAUNCH_CHECK();
+    timer.done();
 }
 
 }  // namespace rt

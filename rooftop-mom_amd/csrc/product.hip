@@ -226,24 +226,18 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     Context *ctx = op.problem->ctx;
     const PlanTables &tx = op.tx.of(sh.bits), &ty = op.ty.of(sh.bits);
     const cplx<TC> *bspec = bspec_of<TC>(op, sh.bits, s);
-    Context::PassRecord rec{};
-    const bool timed = ctx->timing;
-    if (timed) {
-        for (auto &e : rec.ev) e = ctx->take_event();
-        rec.precision = sizeof(TC) == sizeof(float) ? 1 : 0;
-        pass_bytes<TC>(d, nrhs, rec.bytes);
-        RT_HIP(hipEventRecord(rec.ev[0], s));
-    }
+    double by[3];
+    pass_bytes<TC>(d, nrhs, by);
+    const int slot0 = sizeof(TC) == sizeof(float) ? RT_TIMED_PASS_A_C64 : RT_TIMED_PASS_A_C128;
+    KernelTimer ta(ctx, slot0, by[0], s);
     launch_pass_a<TC>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride);
-    if (timed) RT_HIP(hipEventRecord(rec.ev[1], s));
+    ta.done();
+    KernelTimer tb(ctx, slot0 + 1, by[1], s);
     launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, s, skip, skip_stride);
-    if (timed) RT_HIP(hipEventRecord(rec.ev[2], s));
+    tb.done();
+    KernelTimer tc(ctx, slot0 + 2, by[2], s);
     launch_pass_c<TC>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride);
-    if (timed) {
-        RT_HIP(hipEventRecord(rec.ev[3], s));
-        ctx->pending.push_back(rec);
-        if (ctx->pending.size() > 4096) ctx->drain_timing();
-    }
+    tc.done();
 }
 
 template <typename TC>
@@ -379,25 +373,26 @@ static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, con
     RT_HIP(hipStreamSynchronize(s));
     const bool was = ctx->timing;
     ctx->drain_timing();
-    const int p = sizeof(TC) == sizeof(float) ? 1 : 0;
-    const int64_t n0 = ctx->launches[p];
+    const int slot0 = sizeof(TC) == sizeof(float) ? RT_TIMED_PASS_A_C64 : RT_TIMED_PASS_A_C128;
+    int64_t n0[3];
     double ms0[3], by0[3];
     for (int k = 0; k < 3; ++k) {
-        ms0[k] = ctx->pass_ms[p][k];
-        by0[k] = ctx->pass_bytes[p][k];
+        n0[k] = ctx->timed_launches[slot0 + k];
+        ms0[k] = ctx->timed_ms[slot0 + k];
+        by0[k] = ctx->timed_bytes[slot0 + k];
     }
     ctx->timing = true;
     for (int it = 0; it < reps; ++it) run_passes<TC>(op, in, out, work, nrhs, sh, s);
     ctx->drain_timing();
     ctx->timing = was;
-    const double cnt = static_cast<double>(ctx->launches[p] - n0);
     for (int k = 0; k < 3; ++k) {
-        ms[k] = (ctx->pass_ms[p][k] - ms0[k]) / cnt;
-        bytes[k] = (ctx->pass_bytes[p][k] - by0[k]) / cnt;
-        ctx->pass_ms[p][k] = ms0[k];
-        ctx->pass_bytes[p][k] = by0[k];
+        const double cnt = static_cast<double>(ctx->timed_launches[slot0 + k] - n0[k]);
+        ms[k] = (ctx->timed_ms[slot0 + k] - ms0[k]) / cnt;
+        bytes[k] = (ctx->timed_bytes[slot0 + k] - by0[k]) / cnt;
+        ctx->timed_launches[slot0 + k] = n0[k];
+        ctx->timed_ms[slot0 + k] = ms0[k];
+        ctx->timed_bytes[slot0 + k] = by0[k];
     }
-    ctx->launches[p] = n0;
 }
 
 template <typename TC>

@@ -116,6 +116,8 @@ void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, dou
 // Times the launch shapes of the passes for nrhs right-hand sides and keeps the fastest for that batch size;
 // shape (may be NULL) receives {bits, group, tile}, ms the summed pass time of one application.
 void operator_tune(Product &op, int nrhs, int prec, int *shape, double *ms, hipStream_t s);
+// calibrate.hip: average ms of one plain device copy of `bytes`.
+double bandwidth_probe(Context *ctx, uint64_t bytes, int reps);
 // Sets the shape {bits, group, tile} used for nrhs right-hand sides (a saved tuning result).
 void operator_set_shape(Product &op, int nrhs, int prec, const int *shape);
 // nearfield.hip
