@@ -4,11 +4,11 @@
 
 namespace rt {
 
-// 16-byte elements, grid-stride, one read and one write per element.
+// 16-byte elements, one read and one write per element, one element per thread (the grid covers the buffer);
+// the stores bypass the caches' retention like any streaming output.
 __global__ void __launch_bounds__(256) copy_probe(const int4 *__restrict__ src, int4 *__restrict__ dst, int64_t n) {
-    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
-         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
-        dst[i] = src[i];
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(src[i], dst + i);
 }
 
 // Average milliseconds of one copy of `bytes` (rounded down to 16) over `reps` launches after one warm-up.
@@ -20,7 +20,9 @@ double bandwidth_probe(Context *ctx, uint64_t bytes, int reps) {
     b.allocate(ctx, n, "the bandwidth probe");
     hipStream_t s = ctx->stream;
     a.zero(s);
-    const unsigned blocks = blocks_for(n, 256, 256 * 8 * 16);
+    const int64_t grid = (n + 255) / 256;
+    if (grid > 0x7fffffff) fail(RT_EINVAL, "rooftop:ctx", "the probe buffer is too large");
+    const unsigned blocks = static_cast<unsigned>(grid);
     auto launch = [&] {
         hipLaunchKernelGGL(copy_probe, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const int4 *>(a.ptr),
                            reinterpret_cast<int4 *>(b.ptr), n);

@@ -275,59 +275,64 @@ __global__ void __launch_bounds__(kThreads) k_cgs_dot(const c128 *__restrict__ V
     }
 }
 
-// w_r -= V_r h1, then partial = conj(V_r[:, k]) w_r over the block's rows (w staged in LDS per tile of 256 rows).
+// w_r -= V_r h1, then partial = conj(V_r[:, k]) w_r over the block's rows. The live columns of a tile of 64 rows are
+// staged in LDS once (row stride 65 keeps both access patterns off shared banks) and serve both the update and the
+// dot, so the basis crosses HBM once. Wave q sums the columns k = q mod 4 of the update and the rows i = q mod 4 of
+// the dot; the four partial results meet in LDS. Dynamic LDS: cnt * 65 complex128.
+constexpr int kTileRows = 64;
+constexpr int kTileStride = 65;
 __global__ void __launch_bounds__(kThreads) k_cgs_update_dot(const c128 *__restrict__ V, int cnt,
                                                              const c128 *__restrict__ scal, c128 *__restrict__ w,
                                                              int64_t n, int64_t chunk, c128 *__restrict__ partial,
                                                              const LoopState *__restrict__ st) {
     SYSTEM(r);
     const int R = gridDim.y;
+    extern __shared__ c128 tile[];   // [cnt][kTileStride]
     __shared__ c128 hs[kLdp];
-    __shared__ c128 wt[kThreads];
+    __shared__ c128 red[4 * kTileRows];
+    __shared__ c128 wt[kTileRows];
     for (int k = threadIdx.x; k < cnt; k += kThreads) hs[k] = scal[r * kScal + kH1 + k];
-    __syncthreads();
-    c128 *wr = w + r * n;
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    c128 *wr = w + r * n;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
-    double re[16], im[16];
-#pragma unroll
-    for (int c = 0; c < 16; ++c) re[c] = im[c] = 0;
-    for (int64_t t0 = base; t0 < end; t0 += kThreads) {
-        const int64_t i = t0 + threadIdx.x;
-        c128 v = mk(0.0, 0.0);
-        if (i < end) {
-            v = wr[i];
-            for (int k = 0; k < cnt; ++k) v = v - hs[k] * basis_col(V, k, r, R, n)[i];
-            wr[i] = v;
+    double dre = 0, dim = 0;
+    for (int64_t t0 = base; t0 < end; t0 += kTileRows) {
+        const int rows = static_cast<int>(min(static_cast<int64_t>(kTileRows), end - t0));
+        __syncthreads();   // the previous tile is consumed (and hs is written, first time)
+        for (int e = threadIdx.x; e < cnt * kTileRows; e += kThreads) {
+            const int k = e >> 6, i = e & (kTileRows - 1);
+            tile[k * kTileStride + i] = i < rows ? basis_col(V, k, r, R, n)[t0 + i] : mk(0.0, 0.0);
         }
-        wt[threadIdx.x] = v;
         __syncthreads();
-        for (int u = 0; u < 4; ++u) {
-            const int row = lane + 64 * u;
-            const int64_t ii = t0 + row;
-            if (ii < end) {
-                const c128 wi = wt[row];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const int k = q + 4 * c;
-                    if (k < cnt) {
-                        const c128 vv = basis_col(V, k, r, R, n)[ii];
-                        re[c] += vv.x * wi.x + vv.y * wi.y;
-                        im[c] += vv.x * wi.y - vv.y * wi.x;
-                    }
-                }
+        c128 acc = mk(0.0, 0.0);
+        for (int k = q; k < cnt; k += 4) acc = acc + hs[k] * tile[k * kTileStride + lane];
+        red[q * kTileRows + lane] = acc;
+        __syncthreads();
+        if (q == 0) {
+            c128 v = mk(0.0, 0.0);
+            if (lane < rows) {
+                const c128 sum = (red[lane] + red[kTileRows + lane]) + (red[2 * kTileRows + lane] + red[3 * kTileRows + lane]);
+                v = wr[t0 + lane] - sum;
+                wr[t0 + lane] = v;
+            }
+            wt[lane] = v;
+        }
+        __syncthreads();
+        if (lane < cnt) {
+            const c128 *col = tile + lane * kTileStride;
+            for (int i = q; i < rows; i += 4) {
+                const c128 vv = col[i], wi = wt[i];
+                dre += vv.x * wi.x + vv.y * wi.y;
+                dim += vv.x * wi.y - vv.y * wi.x;
             }
         }
-        __syncthreads();
     }
-    c128 *out = partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const int k = q + 4 * c;
-        if (k < cnt) {
-            const double a = wave_sum(re[c]), b = wave_sum(im[c]);
-            if (lane == 0) out[k] = mk(a, b);
-        }
+    __syncthreads();
+    red[q * kTileRows + lane] = mk(dre, dim);
+    __syncthreads();
+    if (q == 0 && lane < cnt) {
+        const c128 sum = (red[lane] + red[kTileRows + lane]) + (red[2 * kTileRows + lane] + red[3 * kTileRows + lane]);
+        partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp + lane] = sum;
     }
 }
 
@@ -479,6 +484,14 @@ __global__ void k_gemv_add(const c128 *__restrict__ V, const LoopState *st, cons
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
 static unsigned vec_blocks(int64_t n) { return blocks_for(n, 256, 1024); }
+
+// Raises a kernel's dynamic LDS limit (the static LDS of the kernel counts against the same 64 KiB default).
+static void raise_lds(const void *kernel, size_t dynamic_bytes) {
+    static size_t raised = 0;
+    if (dynamic_bytes <= raised) return;
+    RT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamic_bytes)));
+    raised = dynamic_bytes;
+}
 
 static c128 *vec(Context *ctx, int slot, int64_t count) {
     return static_cast<c128 *>(ctx->scratch(slot, sizeof(c128) * count, "the solver's vectors"));
@@ -767,8 +780,10 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                                    kH1, 0, st);
                 RT_LAUNCH_CHECK();
                 KernelTimer t2(ctx, RT_TIMED_CGS_UPDATE_DOT, (step + 2) * col, ws.s);
-                hipLaunchKernelGGL(k_cgs_update_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
-                                   w, n, ws.chunk, ws.partial, st);
+                const size_t tile_lds = sizeof(c128) * step * kTileStride;
+                if (tile_lds > 32 * 1024) raise_lds(reinterpret_cast<const void *>(&k_cgs_update_dot), tile_lds);
+                hipLaunchKernelGGL(k_cgs_update_dot, dim3(ws.blocks, R), dim3(kThreads), tile_lds, ws.s, V, step,
+                                   ws.scal, w, n, ws.chunk, ws.partial, st);
                 RT_LAUNCH_CHECK();
                 t2.done();
                 hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
