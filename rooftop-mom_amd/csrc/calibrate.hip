@@ -6,7 +6,8 @@ namespace rt {
 
 // 16-byte elements, one read and one write per element, one element per thread (the grid covers the buffer);
 // the stores bypass the caches' retention like any streaming output.
-__global__ void __launch_bounds__(256) copy_probe(const int4 *__restrict__ src, int4 *__restrict__ dst, int64_t n) {
+typedef int v4i __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) copy_probe(const v4i *__restrict__ src, v4i *__restrict__ dst, int64_t n) {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i < n) __builtin_nontemporal_store(src[i], dst + i);
 }
@@ -24,8 +25,8 @@ double bandwidth_probe(Context *ctx, uint64_t bytes, int reps) {
     if (grid > 0x7fffffff) fail(RT_EINVAL, "rooftop:ctx", "the probe buffer is too large");
     const unsigned blocks = static_cast<unsigned>(grid);
     auto launch = [&] {
-        hipLaunchKernelGGL(copy_probe, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const int4 *>(a.ptr),
-                           reinterpret_cast<int4 *>(b.ptr), n);
+        hipLaunchKernelGGL(copy_probe, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const v4i *>(a.ptr),
+                           reinterpret_cast<v4i *>(b.ptr), n);
         RT_LAUNCH_CHECK();
     };
     launch();
