@@ -1,6 +1,7 @@
 // Shared types of librooftop: complex arithmetic, error state, device memory accounting and the context.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -117,28 +118,33 @@ struct Context {
 };
 
 // A device buffer owned through a context.
-// Brackets one timed kernel launch: construct before the launch, call done() after it.
+// Timing of one kernel launch: a pair of events that the launch itself stamps at the kernel's start and end
+// (hipExtLaunchKernelGGL), so the measured duration is that of the kernel alone. Inactive while timing is off.
 struct KernelTimer {
     Context *ctx;
     int slot;
     double bytes;
-    hipStream_t s;
-    hipEvent_t a = nullptr;
-    KernelTimer(Context *c, int sl, double by, hipStream_t st) : ctx(c), slot(sl), bytes(by), s(st) {
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(Context *c, int sl, double by) : ctx(c), slot(sl), bytes(by) {
         if (ctx->timing) {
             a = ctx->take_event();
-            RT_HIP(hipEventRecord(a, s));
+            b = ctx->take_event();
         }
     }
     void done() {
         if (!a) return;
-        hipEvent_t b = ctx->take_event();
-        RT_HIP(hipEventRecord(b, s));
         ctx->pending.push_back(Context::TimedRecord{a, b, slot, bytes * ctx->timed_scale});
-        a = nullptr;
+        a = b = nullptr;
         if (ctx->pending.size() > 8192) ctx->drain_timing();
     }
 };
+
+// Launches a kernel, stamping the events of t (when timing is on) at its start and end.
+template <typename F, typename... Args>
+void launch_timed(const KernelTimer *t, F kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t s, Args... args) {
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, s, t ? t->a : nullptr, t ? t->b : nullptr, 0, args...);
+    RT_LAUNCH_CHECK();
+}
 
 template <typename T>
 struct DeviceArray {

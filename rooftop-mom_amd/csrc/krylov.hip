@@ -369,23 +369,30 @@ __global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restr
                                                              LoopState *st, double *history, int cap) {
     SYSTEM(r);
     __shared__ double sh[8];
-    double s = 0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[(static_cast<int64_t>(r) * blocks + b) * kLdp].x;
-    s = block_sum(s, sh);
-    if (threadIdx.x != 0) return;
+    __shared__ c128 hcol[kLdp + 1], csv[kLdp], snv[kLdp];
     c128 *sc = scal + r * kScal;
     const int ldh = m + 1;
     c128 *col = Hall + static_cast<int64_t>(r) * ldh * m + static_cast<int64_t>(ldh) * j;
     c128 *g = gall + static_cast<int64_t>(r) * 3 * ldh, *cs = g + ldh, *sn = g + 2 * ldh;
-    for (int k = 0; k <= j; ++k) col[k] = sc[kH1 + k] + sc[kH2 + k];
-    const double sub = sqrt(s);
-    col[j + 1] = mk(sub, 0.0);
-    for (int k = 0; k < j; ++k) {
-        const c128 a0 = col[k], a1 = col[k + 1];
-        col[k] = cmulc(cs[k], a0) + cmulc(sn[k], a1);
-        col[k + 1] = cs[k] * a1 - sn[k] * a0;
+    // Stage the new column and the previous rotations with all threads; thread 0 then rotates in LDS.
+    for (int k = threadIdx.x; k <= j; k += kThreads) hcol[k] = sc[kH1 + k] + sc[kH2 + k];
+    for (int k = threadIdx.x; k < j; k += kThreads) {
+        csv[k] = cs[k];
+        snv[k] = sn[k];
     }
-    const c128 a = col[j];
+    double s = 0;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[(static_cast<int64_t>(r) * blocks + b) * kLdp].x;
+    s = block_sum(s, sh);   // synchronizes the block
+    if (threadIdx.x != 0) return;
+    const double sub = sqrt(s);
+    hcol[j + 1] = mk(sub, 0.0);
+    for (int k = 0; k < j; ++k) {
+        const c128 a0 = hcol[k], a1 = hcol[k + 1];
+        hcol[k] = cmulc(csv[k], a0) + cmulc(snv[k], a1);
+        hcol[k + 1] = csv[k] * a1 - snv[k] * a0;
+    }
+    for (int k = 0; k < j; ++k) col[k] = hcol[k];
+    const c128 a = hcol[j];
     const double radius = sqrt(abs2(a) + sub * sub);
     c128 c = mk(1.0, 0.0), sv = mk(0.0, 0.0);
     if (radius > 0) {
@@ -394,7 +401,7 @@ __global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restr
     }
     cs[j] = c;
     sn[j] = sv;
-    col[j] = cmulc(c, col[j]) + cmulc(sv, mk(sub, 0.0));
+    col[j] = cmulc(c, a) + cmulc(sv, mk(sub, 0.0));
     col[j + 1] = mk(0.0, 0.0);
     g[j + 1] = mk(0.0, 0.0) - sv * g[j];
     g[j] = cmulc(c, g[j]);
@@ -771,28 +778,25 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 precondition(t, w);
                 // Algorithmic bytes of the CGS2 passes: the live basis columns once, w read (and written).
                 const double col = 16.0 * n * R;
-                KernelTimer t1(ctx, RT_TIMED_CGS_DOT, (step + 1) * col, ws.s);
-                hipLaunchKernelGGL(k_cgs_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, w, n, ws.chunk,
-                                   ws.partial, st);
-                RT_LAUNCH_CHECK();
+                KernelTimer t1(ctx, RT_TIMED_CGS_DOT, (step + 1) * col);
+                launch_timed(&t1, k_cgs_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, w, n, ws.chunk,
+                             ws.partial, st);
                 t1.done();
                 hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
                                    kH1, 0, st);
                 RT_LAUNCH_CHECK();
-                KernelTimer t2(ctx, RT_TIMED_CGS_UPDATE_DOT, (step + 2) * col, ws.s);
+                KernelTimer t2(ctx, RT_TIMED_CGS_UPDATE_DOT, (step + 2) * col);
                 const size_t tile_lds = sizeof(c128) * step * kTileStride;
                 if (tile_lds > 32 * 1024) raise_lds(reinterpret_cast<const void *>(&k_cgs_update_dot), tile_lds);
-                hipLaunchKernelGGL(k_cgs_update_dot, dim3(ws.blocks, R), dim3(kThreads), tile_lds, ws.s, V, step,
-                                   ws.scal, w, n, ws.chunk, ws.partial, st);
-                RT_LAUNCH_CHECK();
+                launch_timed(&t2, k_cgs_update_dot, dim3(ws.blocks, R), dim3(kThreads),
+                             static_cast<uint32_t>(tile_lds), ws.s, V, step, ws.scal, w, n, ws.chunk, ws.partial, st);
                 t2.done();
                 hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
                                    kH2, 0, st);
                 RT_LAUNCH_CHECK();
-                KernelTimer t3(ctx, RT_TIMED_CGS_UPDATE_NORM, (step + 2) * col, ws.s);
-                hipLaunchKernelGGL(k_cgs_update_norm, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
-                                   w, n, ws.chunk, ws.partial, st);
-                RT_LAUNCH_CHECK();
+                KernelTimer t3(ctx, RT_TIMED_CGS_UPDATE_NORM, (step + 2) * col);
+                launch_timed(&t3, k_cgs_update_norm, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
+                             w, n, ws.chunk, ws.partial, st);
                 t3.done();
                 hipLaunchKernelGGL(k_gmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
                                    ws.scal, H, g, m, j, it, ws.state, ws.history, ws.cap);

@@ -349,28 +349,26 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     const double bytes = 32.0 * total +
                          (pc.type == kJacobi ? static_cast<double>(es) * d.n
                                              : 4.0 * d.n + 18.0 * es * pc.class_count);
-    KernelTimer timer(pc.problem->ctx, prec == RT_C128 ? RT_TIMED_PRECOND_C128 : RT_TIMED_PRECOND_C64, bytes, s);
+    KernelTimer timer(pc.problem->ctx, prec == RT_C128 ? RT_TIMED_PRECOND_C128 : RT_TIMED_PRECOND_C64, bytes);
     if (pc.type == kJacobi) {
         const unsigned blocks = blocks_for(total, 256, 8192);
         if (prec == RT_C128)
-            hipLaunchKernelGGL(jacobi_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag64.ptr,
-                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip, skip_stride);
+            launch_timed(&timer, jacobi_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag64.ptr,
+                         pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip, skip_stride);
         else
-            hipLaunchKernelGGL(jacobi_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag32.ptr,
-                               pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip, skip_stride);
-        RT_LAUNCH_CHECK();
+            launch_timed(&timer, jacobi_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.diag32.ptr,
+                         pc.problem->exists.ptr, d.nx_count, d.n, nrhs, skip, skip_stride);
         timer.done();
         return;
     }
     const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
     if (pc.class_count == 0) fail(RT_EINVAL, "rooftop:preconditioner", "the grid has no rooftop to precondition");
     if (prec == RT_C128)
-        hipLaunchKernelGGL(sai_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef64.ptr,
-                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
+        launch_timed(&timer, sai_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef64.ptr,
+                     d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
     else
-        hipLaunchKernelGGL(sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef32.ptr,
-                           d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
-    RT_LAUNCH_CHECK();
+        launch_timed(&timer, sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef32.ptr,
+                     d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
     timer.done();
 }
 

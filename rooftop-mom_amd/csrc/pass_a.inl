@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(MAXT) pass_a_k(const c128 *__restrict__ in, cp
 
 template <typename TC>
 void launch_pass_a(const Product &op, const PlanTables &t, const c128 *in, cplx<TC> *work, int nrhs, int g,
-                   hipStream_t s, const int *skip, int skip_stride) {
+                   hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer) {
     const FftPlan &p = t.plan;
     constexpr int es = sizeof(cplx<TC>);
     const int64_t nb = blocks_ac(op.dims, nrhs, g);
@@ -71,12 +71,11 @@ void launch_pass_a(const Product &op, const PlanTables &t, const c128 *in, cplx<
     {                                                                                                       \
         constexpr int MT = (es == 16 && CPlan<L, B>::log2E >= 4) ? 512 : 1024;                              \
         set_lds(reinterpret_cast<const void *>(&pass_a_k<TC, L, B, MT>), lds);                              \
-        hipLaunchKernelGGL((pass_a_k<TC, L, B, MT>), dim3(xcd_grid(nb)), dim3(threads), lds, s, in, work,   \
+        launch_timed(timer, pass_a_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads), static_cast<uint32_t>(lds), s, in, work,   \
                            op.problem->exists.ptr, op.dims, tw, g, cs, static_cast<int>(nb), skip, skip_stride); \
     }
     RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_A);
 #undef RT_LAUNCH_A
-    RT_LAUNCH_CHECK();
 }
 
 }  // namespace rt

@@ -2,5 +2,5 @@
 #include "pass_a.inl"
 
 namespace rt {
-template void launch_pass_a<float>(const Product &, const PlanTables &, const c128 *, cplx<float> *, int, int, hipStream_t, const int *, int);
+template void launch_pass_a<float>(const Product &, const PlanTables &, const c128 *, cplx<float> *, int, int, hipStream_t, const int *, int, const KernelTimer *);
 }  // namespace rt

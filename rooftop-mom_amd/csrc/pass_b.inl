@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
 
 template <typename TC>
 void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
-                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride) {
+                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer) {
     const FftPlan &p = ty.plan;
     constexpr int es = sizeof(cplx<TC>);
     const int64_t nb = static_cast<int64_t>(op.dims.Mx / tile) * nrhs;
@@ -149,12 +149,11 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
     {                                                                                                          \
         constexpr int MT = CPlan<L, B>::T > 256 ? CPlan<L, B>::T : 256;                                        \
         set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, B, MT>), lds);                                 \
-        hipLaunchKernelGGL((pass_b_k<TC, L, B, MT>), dim3(xcd_grid(nb)), dim3(threads), lds, s, work, bspec,   \
+        launch_timed(timer, pass_b_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads), static_cast<uint32_t>(lds), s, work, bspec,   \
                            tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, static_cast<int>(nb), skip, skip_stride); \
     }
     RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_B);
 #undef RT_LAUNCH_B
-    RT_LAUNCH_CHECK();
 }
 
 }  // namespace rt

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ wo
 
 template <typename TC>
 void launch_pass_c(const Product &op, const PlanTables &t, const cplx<TC> *work, c128 *out, int nrhs, int g,
-                   hipStream_t s, const int *skip, int skip_stride) {
+                   hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer) {
     const FftPlan &p = t.plan;
     constexpr int es = sizeof(cplx<TC>);
     const int64_t nb = blocks_ac(op.dims, nrhs, g);
@@ -75,12 +75,11 @@ void launch_pass_c(const Product &op, const PlanTables &t, const cplx<TC> *work,
     {                                                                                                       \
         constexpr int MT = (es == 16 && CPlan<L, B>::log2E >= 4) ? 512 : 1024;                              \
         set_lds(reinterpret_cast<const void *>(&pass_c_k<TC, L, B, MT>), lds);                              \
-        hipLaunchKernelGGL((pass_c_k<TC, L, B, MT>), dim3(xcd_grid(nb)), dim3(threads), lds, s, work, out,  \
+        launch_timed(timer, pass_c_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads), static_cast<uint32_t>(lds), s, work, out,  \
                            op.problem->exists.ptr, op.dims, tw, g, cs, static_cast<int>(nb), skip, skip_stride); \
     }
     RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_C);
 #undef RT_LAUNCH_C
-    RT_LAUNCH_CHECK();
 }
 
 }  // namespace rt

@@ -2,5 +2,5 @@
 #include "pass_c.inl"
 
 namespace rt {
-template void launch_pass_c<float>(const Product &, const PlanTables &, const cplx<float> *, c128 *, int, int, hipStream_t, const int *, int);
+template void launch_pass_c<float>(const Product &, const PlanTables &, const cplx<float> *, c128 *, int, int, hipStream_t, const int *, int, const KernelTimer *);
 }  // namespace rt

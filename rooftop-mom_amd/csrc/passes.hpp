@@ -170,13 +170,14 @@ size_t lds_bytes_max();
 // Launchers (passes_a.hip, passes_b.hip, passes_c.hip, one translation unit per pass and precision).
 template <typename TC>
 void launch_pass_a(const Product &op, const PlanTables &t, const c128 *in, cplx<TC> *work, int nrhs, int group,
-                   hipStream_t s, const int *skip, int skip_stride);
+                   hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer);
 template <typename TC>
 void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
-                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride);
+                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride,
+                   const KernelTimer *timer);
 template <typename TC>
 void launch_pass_c(const Product &op, const PlanTables &t, const cplx<TC> *work, c128 *out, int nrhs, int group,
-                   hipStream_t s, const int *skip, int skip_stride);
+                   hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer);
 
 // Calls F with the compile-time plan of (log2M, bits).
 #define RT_PLAN_CASE(L, F)                                      \

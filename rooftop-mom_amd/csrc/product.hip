@@ -229,14 +229,14 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     double by[3];
     pass_bytes<TC>(d, nrhs, by);
     const int slot0 = sizeof(TC) == sizeof(float) ? RT_TIMED_PASS_A_C64 : RT_TIMED_PASS_A_C128;
-    KernelTimer ta(ctx, slot0, by[0], s);
-    launch_pass_a<TC>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride);
+    KernelTimer ta(ctx, slot0, by[0]);
+    launch_pass_a<TC>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride, &ta);
     ta.done();
-    KernelTimer tb(ctx, slot0 + 1, by[1], s);
-    launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, s, skip, skip_stride);
+    KernelTimer tb(ctx, slot0 + 1, by[1]);
+    launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, s, skip, skip_stride, &tb);
     tb.done();
-    KernelTimer tc(ctx, slot0 + 2, by[2], s);
-    launch_pass_c<TC>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride);
+    KernelTimer tc(ctx, slot0 + 2, by[2]);
+    launch_pass_c<TC>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride, &tc);
     tc.done();
 }
 
