@@ -14,7 +14,8 @@ on the library's stream over the timed solves, and a CPU baseline: the oracle's 
 sample of the same solve, timed on this host.
 
 Batched forms solve the same plate under 64 plane waves together (GMRES(50) and COCG, each mixed and FP64;
-batch solves/s, p50/p99 batch latency), and the matvec is profiled at 1 and 64 right-hand sides.
+batch solves/s, p50/p99 batch latency), the matvec is profiled at 1 and 64 right-hand sides, and a memory-ceiling
+run solves the 4096 x 4096 plate (33.5 M unknowns) once.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--batch 64] [--no-cpu]
 """
@@ -193,6 +194,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--batch', type=int, default=64, help='right-hand sides of the batched form (0: skip it)')
+    ap.add_argument('--ceiling', type=int, default=4096,
+                    help='side of the memory-ceiling plate (one mixed GMRES(30) solve; 0: skip it)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-only', action='store_true',
                     help='only the plain 1 GiB device copy (calibrates the HBM byte counters of rocprofv3)')
@@ -305,6 +308,40 @@ def main():
         for ptr in (d_B, d_X):
             rt.check(lib.rt_device_free(ctx.handle, ptr))
 
+    # Memory ceiling: the 4096 x 4096 plate (33.5 M unknowns), mixed GMRES(30) + near field, FP64 refinement to 1e-6,
+    # one solve; the device bytes the library holds against the device's HBM.
+    ceiling = {}
+    if args.ceiling > 0:
+        L = args.ceiling
+        print('bench: memory ceiling, plate %dx%d' % (L, L), flush=True)
+        del pc, op
+        t0 = time.perf_counter()
+        big = rt.Problem(rt.Mesh(L, L, 0.1, 0.1), 2 * np.pi, quadratureOrder=(16, 8))
+        bop = rt.Operator(big, precision='mixed')
+        bpc = rt.Preconditioner(big, 'nearField', precision='mixed')
+        bsetup = time.perf_counter() - t0
+        mv = {}
+        for prec in ('c64', 'c128'):
+            bop.tune(nrhs=1, precision=prec)
+            pms, pby = bop.profile(nrhs=1, reps=5, precision=prec)
+            mv[prec] = {'ms': round(sum(pms), 4), 'bytes': sum(pby), 'GBps': round(sum(pby) / (sum(pms) * 1e-3) / 1e9, 1),
+                        'frac': round(sum(pby) / (sum(pms) * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                        'pass_ms': [round(v, 4) for v in pms]}
+        bb = rt.Excitation.planeWave(big).vector()
+        t0 = time.perf_counter()
+        res = rt.Solver('gmres', tolerance=1e-6, restart=30, precision='c64', maxIterationCount=3000).solve(
+            bop, bb, preconditioner=bpc)
+        bt = time.perf_counter() - t0
+        held, total = C.c_uint64(), C.c_uint64()
+        rt.check(lib.rt_ctx_memory(ctx.handle, C.byref(held), C.byref(total)))
+        ceiling = {'workload': 'plate %dx%d cells lambda/10, normal incidence, mixed GMRES(30) + near-field, FP64 '
+                               'refinement to 1e-6' % (L, L),
+                   'unknowns': big.rooftopCount(), 'solve_s': round(bt, 3), 'setup_s': round(bsetup, 3),
+                   'converged': res.converged(), 'iterations': res.iterationCount(), 'outer': res.outerCount,
+                   'residual': res.residualNorm(), 'held_GB': round(held.value / 1e9, 2),
+                   'device_GB': round(total.value / 1e9, 1), 'matvec': mv}
+        del bop, bpc, big
+
     steps = args.steps
     value = steps / sum(tmx)
     line = {
@@ -328,6 +365,7 @@ def main():
         'matvec': matvec,
         'copy_ceiling_GBps': round(copy_gbs, 1),
         'batched': batch,
+        'memory_ceiling': ceiling,
         'setup_s': round(setup_s, 3),
         'tuning': {'seconds': round(tune_s, 3), 'shapes': tuned},
     }

@@ -162,6 +162,8 @@ inline size_t lds_ac_bytes(const FftPlan &p, int group, int es) {
 inline size_t lds_b_bytes(const FftPlan &p, int tile, int es) {
     return static_cast<size_t>(2) * tile * padded_length(p) * es;
 }
+// Pass B in three launches (pass_b_split.inl) holds one column per block.
+inline size_t lds_b_split_bytes(const FftPlan &p, int es) { return static_cast<size_t>(padded_length(p)) * es; }
 
 // Raises a kernel's dynamic LDS limit above the 64 KiB default (once per kernel and size); fails beyond the device.
 void set_lds(const void *kernel, size_t bytes);
@@ -175,6 +177,10 @@ template <typename TC>
 void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
                    const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride,
                    const KernelTimer *timer);
+template <typename TC>
+void launch_pass_b_split(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
+                         cplx<TC> *spec, const cplx<TC> *bspec, int nrhs, hipStream_t s, const int *skip,
+                         int skip_stride);
 template <typename TC>
 void launch_pass_c(const Product &op, const PlanTables &t, const cplx<TC> *work, c128 *out, int nrhs, int group,
                    hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer);

@@ -133,15 +133,27 @@ void set_lds(const void *kernel, size_t bytes) {
     have = bytes;
 }
 
+// Whether pass B of a shape runs in one launch (both columns of `tile` x positions in the LDS of a block).
+template <typename TC>
+static bool b_fits(const Product &op, const PassShape &sh) {
+    const FftPlan &py = op.ty.of(sh.bits).plan;
+    return team_threads(py) * sh.tile <= max_threads_b(py) && lds_b_bytes(py, sh.tile, sizeof(cplx<TC>)) <= lds_bytes_max();
+}
+// Whether pass B can run in three launches instead (one column per block).
+template <typename TC>
+static bool b_split_fits(const Product &op, int bits) {
+    const FftPlan &py = op.ty.of(bits).plan;
+    return team_threads(py) <= 1024 && lds_b_split_bytes(py, sizeof(cplx<TC>)) <= lds_bytes_max();
+}
+
 // Whether a shape fits the device: threads per block and LDS.
 template <typename TC>
 static bool shape_fits(const Product &op, const PassShape &sh) {
-    const FftPlan &px = op.tx.of(sh.bits).plan, &py = op.ty.of(sh.bits).plan;
+    const FftPlan &px = op.tx.of(sh.bits).plan;
     const int es = sizeof(cplx<TC>);
-    const size_t lmax = lds_bytes_max();
-    return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx &&
-           team_threads(px) * sh.group <= max_threads_ac(es, px) && team_threads(py) * sh.tile <= max_threads_b(py) &&
-           lds_ac_bytes(px, sh.group, es) <= lmax && lds_b_bytes(py, sh.tile, es) <= lmax;
+    const bool ac = team_threads(px) * sh.group <= max_threads_ac(es, px) && lds_ac_bytes(px, sh.group, es) <= lds_bytes_max();
+    const bool b = b_fits<TC>(op, sh) || (sh.tile == 1 && b_split_fits<TC>(op, sh.bits));
+    return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx && ac && b;
 }
 
 // Default launch shape (the autotuner may replace it): radices up to 16 in complex64 and up to 8 in complex128;
@@ -219,6 +231,24 @@ static const cplx<TC> *bspec_of(Product &op, int bits, hipStream_t s) {
     return build(op.bspec32[v]);
 }
 
+// The full-period y spectra of the three-launch pass B, grown to nrhs right-hand sides.
+template <typename TC>
+static cplx<TC> *spec_of(Product &op, int nrhs) {
+    const size_t per = static_cast<size_t>(2) * op.dims.Mx * op.dims.My;
+    if (sizeof(TC) == sizeof(double)) {
+        if (op.spec_rhs64 < nrhs) {
+            op.spec64.allocate(op.problem->ctx, per * nrhs, "the product's working memory");
+            op.spec_rhs64 = nrhs;
+        }
+        return reinterpret_cast<cplx<TC> *>(op.spec64.ptr);
+    }
+    if (op.spec_rhs32 < nrhs) {
+        op.spec32.allocate(op.problem->ctx, per * nrhs, "the product's working memory");
+        op.spec_rhs32 = nrhs;
+    }
+    return reinterpret_cast<cplx<TC> *>(op.spec32.ptr);
+}
+
 template <typename TC>
 static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
                        hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
@@ -232,9 +262,24 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     KernelTimer ta(ctx, slot0, by[0]);
     launch_pass_a<TC>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride, &ta);
     ta.done();
-    KernelTimer tb(ctx, slot0 + 1, by[1]);
-    launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, s, skip, skip_stride, &tb);
-    tb.done();
+    if (b_fits<TC>(op, sh)) {
+        KernelTimer tb(ctx, slot0 + 1, by[1]);
+        launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, s, skip, skip_stride, &tb);
+        tb.done();
+    } else {
+        // Three launches, timed together as pass B (events recorded around them on the stream).
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (ctx->timing) {
+            e0 = ctx->take_event();
+            RT_HIP(hipEventRecord(e0, s));
+        }
+        launch_pass_b_split<TC>(op, tx, ty, work, spec_of<TC>(op, nrhs), bspec, nrhs, s, skip, skip_stride);
+        if (e0) {
+            e1 = ctx->take_event();
+            RT_HIP(hipEventRecord(e1, s));
+            ctx->pending.push_back(Context::TimedRecord{e0, e1, slot0 + 1, by[1] * ctx->timed_scale});
+        }
+    }
     KernelTimer tc(ctx, slot0 + 2, by[2]);
     launch_pass_c<TC>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride, &tc);
     tc.done();

@@ -87,6 +87,9 @@ struct Product {
     // plan's position order, the three values of a position together. Built when a plan is first used.
     DeviceArray<c128> bspec64[2];
     DeviceArray<c64> bspec32[2];
+    DeviceArray<c128> spec64;        // [nrhs][2][Mx][My]: full y spectra of the three-launch pass B (large periods)
+    DeviceArray<c64> spec32;
+    int spec_rhs64 = 0, spec_rhs32 = 0;
     DeviceArray<c128> work64;        // [nrhs][2][Mx][ldj]
     DeviceArray<c64> work32;
     int work_rhs64 = 0, work_rhs32 = 0;

@@ -189,3 +189,17 @@ def test_plate_512_product_and_tuning(rt):
         assert shape['bits'] in (3, 4) and ms > 0
     assert rel(op.apply(x), ref) < 1e-12
     assert rel(op.apply(x, precision='c64'), ref) < 5e-6
+
+
+def test_long_columns_three_launch_pass_b(rt):
+    """8192-point complex128 y transforms (cy = 4096): the two columns of a position exceed one block's LDS, so pass B
+    runs as forward transforms, multiply and inverse transforms in three launches; both precisions against the
+    oracle."""
+    cx, cy = 3, 4096
+    print('long columns', flush=True)
+    p = _problem(rt, cx, cy, 0.1, 0.1, (3, 3))
+    x = _random(p.rooftopCount(), 13, nrhs=2)
+    ref = _oracle_product(p, cx, cy, 0.1, 0.1, x)
+    op = rt.Operator(p, precision='mixed')
+    assert rel(op.apply(x), ref) < 1e-12
+    assert rel(op.apply(x, precision='c64'), ref) < 5e-6
