@@ -15,7 +15,8 @@ sample of the same solve, timed on this host.
 
 Batched forms solve the same plate under 64 plane waves together (GMRES(50) and COCG, each mixed and FP64;
 batch solves/s, p50/p99 batch latency), the matvec is profiled at 1 and 64 right-hand sides, and a memory-ceiling
-run solves the 4096 x 4096 plate (33.5 M unknowns) once.
+run solves the 4096 x 4096 plate (33.5 M unknowns) once. A served stream of 32 requests (random grids, frequencies,
+masks and feeds) runs through the library's request loop with Poisson arrivals (p50/p99 latency, solves/s).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--batch 64] [--no-cpu]
 """
@@ -154,6 +155,57 @@ def run_batch(rt, ctx, op, pc, solver, d_b, d_x, n, count, steps, warmup):
     return times, [(o_.iteration_count, o_.outer_count, o_.residual_norm) for o_ in outs]
 
 
+C0 = 299792458.0
+
+
+def served_request(rt, i, max_side=1024):
+    """Request i of the served stream (seed 1000 + i): a square grid of side L = round(exp(U[ln 32, ln max_side])),
+    frequency U[0.5, 3] GHz, cell lambda / U[10, 20], a 4-connected blob mask grown to a fill U[0.3, 0.9], and a 1 V
+    delta gap at a uniformly drawn existing rooftop (b = V * edge length there)."""
+    rng = np.random.default_rng(1000 + i)
+    L = int(round(np.exp(rng.uniform(np.log(32), np.log(max_side)))))
+    f = rng.uniform(0.5e9, 3e9)
+    a = C0 / f / rng.uniform(10, 20)
+    fill = rng.uniform(0.3, 0.9)
+    mask = rt.blob_mask(L, L, 1000 + i, fill)
+    ex = np.concatenate([(mask[:-1] & mask[1:]).ravel(order='F'), (mask[:, :-1] & mask[:, 1:]).ravel(order='F')])
+    port = int(rng.choice(np.flatnonzero(ex)))
+    rhs = np.zeros(ex.size, np.complex128)
+    rhs[port] = 1.0 * a   # square cells: every edge has length a
+    return {'L': L, 'a': a, 'k': 2 * np.pi * f / C0, 'mask': mask, 'rhs': rhs, 'n': ex.size, 'fill': fill,
+            'port': port}
+
+
+def serve_stream(rt, requests, solver, slots, arrivals):
+    """Submits requests[i] at arrivals[i] seconds after the start to a server with `slots` workers; returns the
+    per-request (solution, Result, times) and the makespan (first arrival to last completion)."""
+    import threading
+    server = rt.Server(slots=slots)
+    tickets = [None] * len(requests)
+    t_start = time.perf_counter()
+
+    def feeder():
+        for i, q in enumerate(requests):
+            delay = arrivals[i] - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = server.submit(q['L'], q['L'], q['a'], q['a'], q['k'], q['rhs'], mask=q['mask'],
+                                       solver=solver, preconditioner='nearField')
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    out = []
+    for i, q in enumerate(requests):
+        while tickets[i] is None:
+            time.sleep(0.0005)
+        out.append(server.wait(tickets[i], q['n']))
+    th.join()
+    makespan = time.perf_counter() - t_start - arrivals[0]
+    stats = server.stats()
+    server.close()
+    return out, makespan, stats
+
+
 def cpu_baseline(n, budget_s=10.0):
     """The oracle's FP64 path (scipy FFTs on the host cores of this GPU's share, split-FFT product, sparse near-field
     apply, GMRES(50) with CGS2 and the same tolerance) on the same plate: whole solves repeated until about budget_s
@@ -194,6 +246,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--batch', type=int, default=64, help='right-hand sides of the batched form (0: skip it)')
+    ap.add_argument('--serve', type=int, default=32, help='requests of the served stream (0: skip it)')
     ap.add_argument('--ceiling', type=int, default=4096,
                     help='side of the memory-ceiling plate (one mixed GMRES(30) solve; 0: skip it)')
     ap.add_argument('--no-cpu', action='store_true')
@@ -308,6 +361,52 @@ def main():
         for ptr in (d_B, d_X):
             rt.check(lib.rt_device_free(ctx.handle, ptr))
 
+    # Served stream: requests of random grids (32..1024), frequencies, blob masks and feeds; single-stream
+    # throughput with one slot (requests back to back), all requests at once into 4 slots (saturation), then Poisson
+    # arrivals at 0.7 x the mixed single-stream throughput into 4 slots, for the mixed and the FP64 GMRES(50) +
+    # near-field solve to 1e-6; each
+    # request's solution in the mix is compared with its solution alone (the single-stream run).
+    serving = {}
+    if args.serve > 0:
+        Q = args.serve
+        print('bench: served stream, %d requests' % Q, flush=True)
+        reqs = [served_request(rt, i) for i in range(Q)]
+        forms = (('mixed', rt.Solver('gmres', tolerance=1e-6, restart=50, maxIterationCount=3000, precision='c64')),
+                 ('fp64', rt.Solver('gmres', tolerance=1e-6, restart=50, maxIterationCount=3000, refine=True)))
+        alone, rate = {}, None
+        for name, solver in forms:
+            out1, span1, _ = serve_stream(rt, reqs, solver, 1, [0.0] * Q)
+            alone[name] = out1
+            serving[name] = {'single_stream_solves_per_s': round(Q / span1, 3),
+                             'service_ms_mean': round(1e3 * np.mean([o[2]['service'] for o in out1]), 3)}
+            if rate is None:
+                rate = 0.7 * Q / span1
+        arrivals = np.concatenate([[0.0], np.cumsum(np.random.default_rng(7).exponential(1.0 / rate, Q - 1))])
+        for name, solver in forms:
+            print('bench: served stream, %s form, saturation' % name, flush=True)
+            sat, sat_span, sat_st = serve_stream(rt, reqs, solver, 4, [0.0] * Q)
+            serving[name]['saturation_solves_per_s'] = round(Q / sat_span, 3)
+            serving[name]['saturation_peak_concurrent'] = sat_st['peak_running']
+            print('bench: served stream, %s form, Poisson arrivals' % name, flush=True)
+            out, span, st = serve_stream(rt, reqs, solver, 4, arrivals)
+            lat = [o[2]['latency'] for o in out]
+            diff = max(float(np.linalg.norm(o[0] - a[0]) / np.linalg.norm(a[0])) for o, a in zip(out, alone[name]))
+            serving[name].update({
+                'solves_per_s': round(Q / span, 3), 'p50_ms': round(1e3 * percentile(lat, 50), 3),
+                'p99_ms': round(1e3 * percentile(lat, 99), 3),
+                'converged': int(sum(o[1].converged() for o in out)),
+                'residual_max': max(o[1].residualNorm() for o in out),
+                'iterations_median': int(np.median([o[1].iterationCount() for o in out])),
+                'peak_concurrent': st['peak_running'], 'alone_vs_mix_max_rel_diff': diff})
+        serving['requests'] = Q
+        serving['arrival_rate_per_s'] = round(rate, 3)
+        serving['workload'] = ('square grids L = round(exp(U[ln 32, ln 1024])), f ~ U[0.5, 3] GHz, cell lambda/U[10,'
+                               ' 20], 4-connected blob masks filled U[0.3, 0.9], 1 V delta gap at a random existing '
+                               'rooftop (seeds 1000+i); Poisson arrivals (seed 7) at 0.7 x the mixed single-stream '
+                               'throughput; GMRES(50) + near field to 1e-6 (true residual); 4 slots; saturation = all '
+                               'requests at once')
+        serving['sides'] = [q['L'] for q in reqs]
+
     # Memory ceiling: the 4096 x 4096 plate (33.5 M unknowns), mixed GMRES(30) + near field, FP64 refinement to 1e-6,
     # one solve; the device bytes the library holds against the device's HBM.
     ceiling = {}
@@ -365,6 +464,7 @@ def main():
         'matvec': matvec,
         'copy_ceiling_GBps': round(copy_gbs, 1),
         'batched': batch,
+        'serving': serving,
         'memory_ceiling': ceiling,
         'setup_s': round(setup_s, 3),
         'tuning': {'seconds': round(tune_s, 3), 'shapes': tuned},

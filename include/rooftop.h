@@ -194,6 +194,41 @@ int rt_solve(rt_ctx *ctx, rt_product *op, rt_product *precond, const rt_solve_op
 int rt_solve_device(rt_ctx *ctx, rt_product *op, rt_product *precond, const rt_solve_opts *opts, const void *rhs,
                     int64_t n, int64_t nrhs, const void *x0, void *x, rt_output *out);
 
+/* Serving (no counterpart in the reference): a library-owned request loop. Requests of
+ * different grids, frequencies and masks are admitted first come, first served when a worker slot is free and their
+ * estimated device bytes fit the server's HBM budget (the semantics of Budget.require/hold/release,
+ * src/+rooftopMoM/+resources/Budget.m:59-106); each slot has its own HIP stream, so admitted requests run
+ * concurrently, and a request holds its generators, spectra, preconditioner and Krylov state only while it runs. A
+ * request's result is the one rt_solve gives on the same inputs, whatever runs beside it. */
+typedef struct rt_server rt_server;
+typedef struct rt_request {
+    int32_t count_x, count_y;    /* faces */
+    double size_x, size_y;       /* face sizes (m) */
+    double wavenumber;           /* k (rad/m) */
+    int32_t near_order, far_order;   /* quadrature orders, 0 = [16, 8] */
+    const uint8_t *mask;         /* count_x * count_y faces (x fastest), 1 = metal; NULL = full plate */
+    const void *rhs;             /* n complex128 values (copied at submission) */
+    int64_t n;                   /* rooftops of the grid with this mask's layout (all edges of the full grid) */
+    int32_t preconditioner;      /* 0, RT_PRECOND_JACOBI or RT_PRECOND_NEARFIELD */
+    rt_solve_opts opts;
+} rt_request;
+/* slots: concurrent requests (1..64); hbm_budget_bytes 0 = 90 % of the device's free memory. */
+int rt_server_create(int32_t device, uint64_t hbm_budget_bytes, int32_t slots, rt_server **out);
+/* Waits for the running requests; queued ones end with an error for their waiters. */
+void rt_server_destroy(rt_server *server);
+/* Copies the request and returns at once; RT_ENOMEM when it can never fit the budget. */
+int rt_server_submit(rt_server *server, const rt_request *request, int64_t *ticket);
+/* Blocks until the request has run; x (may be NULL) receives the solution (n complex128), out the solver output and
+ * times (may be NULL) {queue wait, service, total latency} in seconds. A ticket is waited for once. */
+int rt_server_wait(rt_server *server, int64_t ticket, void *x, int64_t n, rt_output *out, double *times);
+/* The device bytes the server reserves for a request while it runs (its admission size). */
+int rt_request_bytes(const rt_request *request, uint64_t *bytes);
+/* counts: {submitted, completed, queued, running, peak running}; bytes: {reserved, budget}. */
+int rt_server_stats(rt_server *server, int64_t *counts, uint64_t *bytes);
+/* A 4-connected blob of metal faces grown by random frontier growth from a uniformly drawn seed face until
+ * round(fill * count_x * count_y) faces are metal (seeded, portable; the masks of the served-stream workload). */
+int rt_blob_mask(int32_t count_x, int32_t count_y, uint64_t seed, double fill, uint8_t *mask);
+
 /* Device buffers owned by the caller through the library (counted in the context's budget). */
 int rt_device_alloc(rt_ctx *ctx, uint64_t bytes, void **ptr);
 int rt_device_free(rt_ctx *ctx, void *ptr);

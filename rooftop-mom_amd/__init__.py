@@ -518,3 +518,91 @@ class Solver:
                           norms[r, :min(cap, out.residual_norms_count)].copy(), out.residual_norm, out.time_s,
                           excitation, out.outer_count) for r, out in enumerate(outs)]
         return results[0] if b.ndim == 1 else results
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# Serving: the library's request loop (rt_server_*), and the connected-blob masks of the served-stream workload.
+# ---------------------------------------------------------------------------------------------------------------
+def blob_mask(countX, countY, seed, fill):
+    """A 4-connected blob of metal faces grown by random frontier growth from a uniformly drawn seed face until
+    round(fill * countX * countY) faces are metal (countX x countY booleans; rt_blob_mask)."""
+    m = np.zeros(countX * countY, np.uint8)
+    check(library().rt_blob_mask(int(countX), int(countY), int(seed), float(fill), m.ctypes.data_as(C.c_void_p)))
+    return m.reshape((countX, countY), order='F').astype(bool)
+
+
+class Server:
+    """Requests of different grids, frequencies and masks admitted first come, first served by an HBM budget and
+    solved concurrently by `slots` workers, each on its own HIP stream (rt_server_*). The result of a request is
+    the one Solver.solve gives on the same inputs."""
+
+    def __init__(self, slots=4, budget=0, device=0):
+        h = C.c_void_p()
+        check(library().rt_server_create(int(device), int(budget), int(slots), C.byref(h)))
+        self.handle = h
+
+    @staticmethod
+    def _request(countX, countY, sizeX, sizeY, wavenumber, rightHandSide, mask, solver, preconditioner,
+                 quadratureOrder):
+        solver = solver or Solver('cg', tolerance=1e-6)
+        b = np.ascontiguousarray(rightHandSide, dtype=np.complex128)
+        req = _abi.rt_request()
+        req.count_x, req.count_y = int(countX), int(countY)
+        req.size_x, req.size_y, req.wavenumber = float(sizeX), float(sizeY), float(wavenumber)
+        req.near_order, req.far_order = int(quadratureOrder[0]), int(quadratureOrder[-1])
+        m = None
+        if mask is not None:
+            m = np.asfortranarray(mask, dtype=np.uint8).ravel(order='F')
+            req.mask = m.ctypes.data
+        req.rhs = b.ctypes.data
+        req.n = b.size
+        req.preconditioner = _PRECONDITIONERS[preconditioner] if preconditioner else 0
+        req.opts = solver.options()
+        return req, (b, m)   # the arrays must outlive the call that reads the request
+
+    @staticmethod
+    def requestBytes(countX, countY, solver=None, preconditioner='nearField'):
+        """Device bytes a request of a countX x countY grid reserves while it runs (its admission size)."""
+        req, keep = Server._request(countX, countY, 1.0, 1.0, 1.0, np.zeros(1), None, solver, preconditioner,
+                                    (16, 8))
+        b = C.c_uint64()
+        check(library().rt_request_bytes(C.byref(req), C.byref(b)))
+        return b.value
+
+    def submit(self, countX, countY, sizeX, sizeY, wavenumber, rightHandSide, mask=None, solver=None,
+               preconditioner='nearField', quadratureOrder=(16, 8)):
+        """Queues one solve and returns its ticket at once (the inputs are copied)."""
+        req, keep = self._request(countX, countY, sizeX, sizeY, wavenumber, rightHandSide, mask, solver,
+                                  preconditioner, quadratureOrder)
+        ticket = C.c_int64()
+        check(library().rt_server_submit(self.handle, C.byref(req), C.byref(ticket)))
+        return ticket.value
+
+    def wait(self, ticket, n):
+        """(solution, Result, {'queue', 'service', 'latency'} seconds) of a ticket; blocks until it has run."""
+        x = np.empty(n, np.complex128)
+        cap = 8192
+        norms = np.zeros(cap)
+        out = _abi.rt_output()
+        out.residual_norms_capacity = cap
+        out.residual_norms = norms.ctypes.data_as(C.POINTER(C.c_double))
+        times = (C.c_double * 3)()
+        check(library().rt_server_wait(self.handle, int(ticket), _ptr(x), n, C.byref(out), times))
+        res = Result(x, bool(out.converged), out.iteration_count, norms[:min(cap, out.residual_norms_count)].copy(),
+                     out.residual_norm, out.time_s, None, out.outer_count)
+        return x, res, {'queue': times[0], 'service': times[1], 'latency': times[2]}
+
+    def stats(self):
+        counts = (C.c_int64 * 5)()
+        by = (C.c_uint64 * 2)()
+        check(library().rt_server_stats(self.handle, counts, by))
+        return {'submitted': counts[0], 'completed': counts[1], 'queued': counts[2], 'running': counts[3],
+                'peak_running': counts[4], 'reserved_bytes': by[0], 'budget_bytes': by[1]}
+
+    def close(self):
+        if getattr(self, 'handle', None) is not None and _abi._lib is not None:
+            _abi._lib.rt_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()

@@ -26,6 +26,13 @@ class rt_output(C.Structure):
                 ('residual_norms_capacity', C.c_int32), ('residual_norms', C.POINTER(C.c_double))]
 
 
+class rt_request(C.Structure):
+    _fields_ = [('count_x', C.c_int32), ('count_y', C.c_int32), ('size_x', C.c_double), ('size_y', C.c_double),
+                ('wavenumber', C.c_double), ('near_order', C.c_int32), ('far_order', C.c_int32),
+                ('mask', C.c_void_p), ('rhs', C.c_void_p), ('n', C.c_int64), ('preconditioner', C.c_int32),
+                ('opts', rt_solve_opts)]
+
+
 # Every exported function: (name, restype, argtypes). The CPU test suite checks this list against include/rooftop.h.
 P = C.c_void_p
 SIGNATURES = [
@@ -53,6 +60,13 @@ SIGNATURES = [
                                      C.POINTER(C.c_double)]),
     ('rt_product_tune', C.c_int, [P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     ('rt_product_set_shape', C.c_int, [P, C.c_int64, C.c_int32, C.POINTER(C.c_int32)]),
+    ('rt_server_create', C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.POINTER(P)]),
+    ('rt_server_destroy', None, [P]),
+    ('rt_server_submit', C.c_int, [P, C.POINTER(rt_request), C.POINTER(C.c_int64)]),
+    ('rt_server_wait', C.c_int, [P, C.c_int64, P, C.c_int64, C.POINTER(rt_output), C.POINTER(C.c_double)]),
+    ('rt_server_stats', C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
+    ('rt_request_bytes', C.c_int, [C.POINTER(rt_request), C.POINTER(C.c_uint64)]),
+    ('rt_blob_mask', C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_double, P]),
     ('rt_solve', C.c_int, [P, P, P, C.POINTER(rt_solve_opts), P, C.c_int64, C.c_int64, P, P,
                            C.POINTER(rt_output)]),
     ('rt_solve_device', C.c_int, [P, P, P, C.POINTER(rt_solve_opts), P, C.c_int64, C.c_int64, P, P,

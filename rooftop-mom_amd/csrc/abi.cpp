@@ -29,6 +29,42 @@ GridInfo GridInfo::make(int cx, int cy, double a, double b) {
     g.has_y = g.ny > 0;
     return g;
 }
+std::shared_ptr<Problem> make_problem(Context *ctx, int cx, int cy, double a, double b, double k, int near_order,
+                                      int far_order, const uint8_t *mask) {
+    if (cx < 1 || cy < 1) fail(RT_EINVAL, "MATLAB:validators:mustBePositive", "The face counts are positive.");
+    if (cx == 1 && cy == 1)
+        fail(RT_EINVAL, "rooftopMoM:mesh:noRooftops", "A single face has no interior edge, so it carries no rooftop.");
+    if (!(a > 0 && b > 0 && std::isfinite(a) && std::isfinite(b)))
+        fail(RT_EINVAL, "MATLAB:validators:mustBePositive", "The face sizes are positive and finite.");
+    if (!(k > 0 && std::isfinite(k)))
+        fail(RT_EINVAL, "rooftopMoM:problem:invalidWavenumber", "The wavenumber is positive and finite.");
+    if (near_order < 1 || far_order < 1 || far_order > 16 || near_order > 64)
+        fail(RT_EINVAL, "rooftopMoM:problem:invalidQuadratureOrder",
+             "The quadrature orders are 1..64 (near) and 1..16 (far).");
+    auto p = std::make_shared<Problem>();
+    p->ctx = ctx;
+    p->g = GridInfo::make(cx, cy, a, b);
+    p->k = k;
+    p->near_order = near_order;
+    p->far_order = far_order;
+    // Budget of the generators (estimateGenerators.m:11) checked up front, as Problem.m:210-219 does.
+    ctx->require(16ull * (p->g.n + static_cast<uint64_t>(cx) * cy), "The assembly of the generators");
+    if (mask) {
+        p->face_mask.assign(mask, mask + static_cast<size_t>(cx) * cy);
+        std::vector<uint8_t> ex(static_cast<size_t>(p->g.n));
+        auto face = [&](int x, int y) { return mask[x + static_cast<size_t>(cx) * y] != 0; };
+        for (int j = 0; j < p->g.cols_x; ++j)
+            for (int i = 0; i < p->g.rows_x; ++i)
+                ex[i + static_cast<size_t>(p->g.rows_x) * j] = face(i, j) && face(i + 1, j);
+        for (int j = 0; j < p->g.cols_y; ++j)
+            for (int i = 0; i < p->g.rows_y; ++i)
+                ex[p->g.nx + i + static_cast<size_t>(p->g.rows_y) * j] = face(i, j) && face(i, j + 1);
+        p->exists.allocate(ctx, ex.size(), "The rooftop mask");
+        RT_HIP(hipMemcpyAsync(p->exists.ptr, ex.data(), ex.size(), hipMemcpyHostToDevice, ctx->stream));
+        RT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return p;
+}
 }  // namespace rt
 
 using namespace rt;
@@ -151,34 +187,15 @@ extern "C" {
 int rt_ctx_create(int32_t device, uint64_t budget, rt_ctx **out) {
     return guarded([&] {
         need(out != nullptr, RT_EINVAL, "rooftop:ctx", "out is NULL");
-        int count = 0;
-        RT_HIP(hipGetDeviceCount(&count));
-        need(device >= 0 && device < count, RT_EINVAL, "rooftop:ctx", "no such device");
         auto c = std::make_unique<rt_ctx>();
-        c->c.device = device;
-        c->c.activate();
-        size_t free_b = 0, total_b = 0;
-        RT_HIP(hipMemGetInfo(&free_b, &total_b));
-        c->c.ceiling = budget ? budget : static_cast<uint64_t>(free_b);
-        RT_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        c->c.open(device, budget);
         *out = c.release();
     });
 }
 
 void rt_ctx_destroy(rt_ctx *ctx) {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->c.device);
-    if (ctx->c.stream) {
-        (void)hipStreamSynchronize(ctx->c.stream);
-        (void)hipStreamDestroy(ctx->c.stream);
-    }
-    for (auto &kv : ctx->c.sizes) (void)hipFree(kv.first);
-    for (auto &r : ctx->c.pending) {
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
-    }
-    for (hipEvent_t e : ctx->c.event_pool) (void)hipEventDestroy(e);
-    if (ctx->c.host_pinned) (void)hipHostFree(ctx->c.host_pinned);
+    ctx->c.close();
     delete ctx;
 }
 
@@ -242,39 +259,9 @@ int rt_problem_create(rt_ctx *ctx, int32_t cx, int32_t cy, double a, double b, d
                       int32_t far_order, const uint8_t *mask, rt_problem **out) {
     return guarded([&] {
         need(ctx && out, RT_EINVAL, "rooftop:problem", "ctx or out is NULL");
-        need(cx >= 1 && cy >= 1, RT_EINVAL, "MATLAB:validators:mustBePositive", "The face counts are positive.");
-        need(!(cx == 1 && cy == 1), RT_EINVAL, "rooftopMoM:mesh:noRooftops",
-             "A single face has no interior edge, so it carries no rooftop.");
-        need(a > 0 && b > 0 && std::isfinite(a) && std::isfinite(b), RT_EINVAL, "MATLAB:validators:mustBePositive",
-             "The face sizes are positive and finite.");
-        need(k > 0 && std::isfinite(k), RT_EINVAL, "rooftopMoM:problem:invalidWavenumber",
-             "The wavenumber is positive and finite.");
-        need(near_order >= 1 && far_order >= 1 && far_order <= 16 && near_order <= 64, RT_EINVAL,
-             "rooftopMoM:problem:invalidQuadratureOrder", "The quadrature orders are 1..64 (near) and 1..16 (far).");
         ctx->c.activate();
-        auto p = std::make_unique<Problem>();
-        p->ctx = &ctx->c;
-        p->g = GridInfo::make(cx, cy, a, b);
-        p->k = k;
-        p->near_order = near_order;
-        p->far_order = far_order;
-        // Budget of the generators (estimateGenerators.m:11) checked up front, as Problem.m:210-219 does.
-        ctx->c.require(16ull * (p->g.n + static_cast<uint64_t>(cx) * cy), "The assembly of the generators");
-        if (mask) {
-            p->face_mask.assign(mask, mask + static_cast<size_t>(cx) * cy);
-            std::vector<uint8_t> ex(static_cast<size_t>(p->g.n));
-            auto face = [&](int x, int y) { return mask[x + static_cast<size_t>(cx) * y] != 0; };
-            for (int j = 0; j < p->g.cols_x; ++j)
-                for (int i = 0; i < p->g.rows_x; ++i)
-                    ex[i + static_cast<size_t>(p->g.rows_x) * j] = face(i, j) && face(i + 1, j);
-            for (int j = 0; j < p->g.cols_y; ++j)
-                for (int i = 0; i < p->g.rows_y; ++i)
-                    ex[p->g.nx + i + static_cast<size_t>(p->g.rows_y) * j] = face(i, j) && face(i, j + 1);
-            p->exists.allocate(&ctx->c, ex.size(), "The rooftop mask");
-            RT_HIP(hipMemcpy(p->exists.ptr, ex.data(), ex.size(), hipMemcpyHostToDevice));
-        }
         auto h = std::make_unique<rt_problem>();
-        h->p = std::shared_ptr<Problem>(std::move(p));
+        h->p = make_problem(&ctx->c, cx, cy, a, b, k, near_order, far_order, mask);
         *out = h.release();
     });
 }

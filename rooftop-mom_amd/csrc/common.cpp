@@ -95,6 +95,48 @@ void *Context::scratch(int slot, uint64_t bytes, const char *what) {
     return e.first;
 }
 
+void Context::open(int dev, uint64_t budget) {
+    int count = 0;
+    RT_HIP(hipGetDeviceCount(&count));
+    if (dev < 0 || dev >= count) fail(RT_EINVAL, "rooftop:ctx", "no such device");
+    device = dev;
+    activate();
+    size_t free_b = 0, total_b = 0;
+    RT_HIP(hipMemGetInfo(&free_b, &total_b));
+    ceiling = budget ? budget : static_cast<uint64_t>(free_b);
+    RT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
+void Context::close() {
+    (void)hipSetDevice(device);
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    for (auto &kv : sizes) (void)hipFree(kv.first);
+    sizes.clear();
+    pool.clear();
+    held = 0;
+    for (auto &r : pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    pending.clear();
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    event_pool.clear();
+    if (host_pinned) (void)hipHostFree(host_pinned);
+    host_pinned = nullptr;
+    host_pinned_bytes = 0;
+}
+
+void Context::free_pool() {
+    if (stream) RT_HIP(hipStreamSynchronize(stream));
+    for (auto &e : pool)
+        if (e.first) release(e.first);
+    pool.clear();
+}
+
 hipEvent_t Context::take_event() {
     if (!event_pool.empty()) {
         hipEvent_t e = event_pool.back();

@@ -115,6 +115,12 @@ struct Context {
     // Raises rooftopMoM:resources:insufficientMemory if bytes more do not fit under the ceiling.
     void require(uint64_t bytes, const char *what);
     void activate() const { RT_HIP(hipSetDevice(device)); }
+    // Binds the context to a device with its own stream and a byte ceiling (0: the device's free memory).
+    void open(int dev, uint64_t budget);
+    // Frees everything the context still holds (device buffers, events, pinned staging) and its stream.
+    void close();
+    // Frees the solver buffers kept between solves.
+    void free_pool();
 };
 
 // A device buffer owned through a context.

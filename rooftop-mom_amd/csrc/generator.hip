@@ -352,12 +352,15 @@ __device__ void along_x(double a, double b, int power, const double *gl_p, const
                     double sum = 0;
                     for (int i = 0; i < 20; ++i) {
                         const double u = kk + gl_p[i];
+                        const double um = m == 0 ? 1.0 : m == 1 ? u : m == 2 ? u * u : u * u * u;
                         double inner = 0;
                         for (int j = 0; j < 20; ++j) {
                             const double v = ratio * (l + gl_p[j]);
-                            inner += gl_w[j] * pow(u, m) * pow(v, n) * pow(sqrt(u * u + v * v), power);
+                            const double r = sqrt(u * u + v * v);
+                            const double rp = power == -1 ? 1.0 / r : power == 1 ? r : r * r * r;
+                            inner += gl_w[j] * (n == 0 ? 1.0 : v) * rp;
                         }
-                        sum += gl_w[i] * inner;
+                        sum += gl_w[i] * um * inner;
                     }
                     rect[m][n][kk][l] = ratio * sum;
                 }
@@ -381,24 +384,31 @@ __device__ void along_x(double a, double b, int power, const double *gl_p, const
 }
 
 // The extracted moments 7x4 (generateExtractedMoments.m:31-33): P(-1) - k^2/2 P(1) + k^4/24 P(3).
+// Six threads evaluate the six power moment tables (three powers along x and along y) at once; thread 0 combines.
 __global__ void touching_moments(double a, double b, double k, const double *gl_p, const double *gl_w,
                                  c128 *__restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double tab[6][4][4];
     const int powers[3] = {-1, 1, 3};
+    const int t = threadIdx.x;
+    if (t < 6) {
+        double res[4][4];
+        if (t < 3) along_x(a, b, powers[t], gl_p, gl_w, res);
+        else along_x(b, a, powers[t - 3], gl_p, gl_w, res);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) tab[t][i][j] = res[i][j];
+    }
+    __syncthreads();
+    if (t != 0) return;
     const double factors[3] = {1.0, -k * k / 2, k * k * k * k / 24};
     double acc[7][4];
     for (int i = 0; i < 7; ++i)
         for (int j = 0; j < 4; ++j) acc[i][j] = 0;
-    for (int pi = 0; pi < 3; ++pi) {
-        double ax[4][4], ay[4][4];
-        along_x(a, b, powers[pi], gl_p, gl_w, ax);
-        along_x(b, a, powers[pi], gl_p, gl_w, ay);
-        const int perm[4] = {0, 2, 1, 3};
+    const int perm[4] = {0, 2, 1, 3};
+    for (int pi = 0; pi < 3; ++pi)
         for (int j = 0; j < 4; ++j) {
-            for (int i = 0; i < 4; ++i) acc[i][j] += factors[pi] * ax[i][j];
-            for (int i = 0; i < 3; ++i) acc[4 + i][j] += factors[pi] * ay[1 + i][perm[j]];
+            for (int i = 0; i < 4; ++i) acc[i][j] += factors[pi] * tab[pi][i][j];
+            for (int i = 0; i < 3; ++i) acc[4 + i][j] += factors[pi] * tab[3 + pi][1 + i][perm[j]];
         }
-    }
     for (int i = 0; i < 7; ++i)
         for (int j = 0; j < 4; ++j) out[i * 4 + j] = mk(acc[i][j], 0.0);
 }

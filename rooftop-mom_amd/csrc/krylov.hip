@@ -494,7 +494,9 @@ static unsigned vec_blocks(int64_t n) { return blocks_for(n, 256, 1024); }
 
 // Raises a kernel's dynamic LDS limit (the static LDS of the kernel counts against the same 64 KiB default).
 static void raise_lds(const void *kernel, size_t dynamic_bytes) {
+    static std::mutex lock;
     static size_t raised = 0;
+    std::lock_guard<std::mutex> g(lock);
     if (dynamic_bytes <= raised) return;
     RT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamic_bytes)));
     raised = dynamic_bytes;

@@ -11,6 +11,7 @@
 // The passes themselves are in passes.hpp and pass_{a,b,c}.inl; this file holds the tables, the spectra, the launch
 // shapes and their autotuner.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <type_traits>
 
@@ -108,16 +109,17 @@ __global__ void __launch_bounds__(1024) spectra_b(const c128 *__restrict__ in, c
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
 size_t lds_bytes_max() {
-    static size_t cached = 0;
-    if (!cached) {
-        int dev = 0;
+    static std::atomic<size_t> cached{0};
+    size_t v = cached.load(std::memory_order_relaxed);
+    if (!v) {
+        int dev = 0, got = 0;
         (void)hipGetDevice(&dev);
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
-            v = 64 * 1024;
-        cached = static_cast<size_t>(v);
+        if (hipDeviceGetAttribute(&got, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || got <= 0)
+            got = 64 * 1024;
+        v = static_cast<size_t>(got);
+        cached.store(v, std::memory_order_relaxed);
     }
-    return cached;
+    return v;
 }
 
 void set_lds(const void *kernel, size_t bytes) {

@@ -111,6 +111,9 @@ struct Product {
     uint64_t working_bytes() const;
 };
 
+// abi.cpp: a problem on a grid of cx x cy faces of a x b, wavenumber k, optional face mask (cx * cy, x fastest).
+std::shared_ptr<Problem> make_problem(Context *ctx, int cx, int cy, double a, double b, double k, int near_order,
+                                      int far_order, const uint8_t *mask);
 // product.hip
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier);
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,

@@ -1,0 +1,397 @@
+// The serving loop (no counterpart in the reference, whose Solver.solve serves one caller at a time): requests of
+// different grids, frequencies and masks are admitted by an HBM budget and solved by a pool of worker slots, each
+// with its own HIP stream and device buffers, so the requests on the device run concurrently and a request that
+// arrives while others run starts as soon as a slot and its bytes are free (first come, first served). A request
+// keeps its generators, spectra, near field and Krylov state resident while it runs and returns all of it when it
+// ends. The solve of a request is exactly the one of rt_solve on the same inputs: the kernels of one request read
+// only its own buffers and reduce in a fixed order, so its result does not depend on what runs beside it.
+//
+// The budget follows the reference's Budget.require/hold/release (src/+rooftopMoM/+resources/Budget.m:59-106): a
+// request's bytes are estimated before it starts (the byte models of estimateProduct.m:13-37, estimateSolver.m:19-26
+// and estimateGenerators.m:11, re-derived for this library's device buffers), held while it runs and released at its
+// end; a request larger than the whole budget is refused at submission with rooftopMoM:resources:insufficientMemory.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+#include "krylov.hpp"
+#include "rooftop.hpp"
+
+namespace rt {
+
+using Clock = std::chrono::steady_clock;
+
+struct ServeJob {
+    int64_t ticket = 0;
+    int cx = 0, cy = 0;
+    double a = 0, b = 0, k = 0;
+    int near_order = 16, far_order = 8;
+    std::vector<uint8_t> mask;
+    std::vector<c128> rhs;
+    int preconditioner = 0;
+    SolveOptions opts;
+    uint64_t bytes = 0;
+    Clock::time_point submitted, started, finished;
+    bool done = false;
+    int status = RT_OK;
+    std::string error_id, error_message;
+    std::vector<c128> x;
+    SolveResult result;
+};
+
+// Device bytes a request holds at its peak: generators, mask, spectra (master and the pass-B copies of both
+// precisions), FFT tables, the products' working arrays, the near field, the Krylov vectors (a GMRES(m) basis or the
+// COCG vectors) and the right-hand side and solution, with a margin for the one-off transform buffer of the spectra.
+uint64_t estimate_request_bytes(int cx, int cy, const SolveOptions &o, int preconditioner) {
+    const GridInfo g = GridInfo::make(cx, cy, 1.0, 1.0);
+    const uint64_t n = static_cast<uint64_t>(g.n), cells = static_cast<uint64_t>(cx) * cy;
+    const uint64_t Mx = cx > 1 ? next_pow2(2L * cx - 1) : 1, My = cy > 1 ? next_pow2(2L * cy - 1) : 1;
+    const uint64_t Hx = Mx / 2 + 1, Hy = My / 2 + 1;
+    const uint64_t ldj = (std::max(g.cols_x, g.cols_y) + 7) / 8 * 8;
+    const bool mixed = o.precision == RT_C64;
+    uint64_t b = 0;
+    b += 16 * (n + 3 * cells);                                    // generators
+    b += n;                                                       // mask
+    b += 3 * Hx * Hy * 16;                                        // spectra (FP64 master)
+    b += 3 * Hx * My * (16 + (mixed ? 8 : 0));                    // pass-B spectra
+    b += 2 * ldj * Mx * (16 + (mixed ? 8 : 0));                   // working arrays
+    b += (Mx + My) * 4 * 64;                                      // FFT tables
+    if (preconditioner == RT_PRECOND_NEARFIELD) b += 4 * n + 4096 * 18 * 24;
+    if (preconditioner == RT_PRECOND_JACOBI) b += 24 * n;
+    uint64_t vectors = o.method == RT_METHOD_GMRES ? static_cast<uint64_t>(std::min<int64_t>(o.restart > 0 ? o.restart
+                                                                                                   : 62, 62) + 1) + 6
+                                                   : 8;
+    if (o.refine) vectors += 3;
+    b += (vectors + 2) * n * 16;                                  // Krylov vectors, right-hand side, solution
+    const uint64_t spectra_transform = Mx * My * 16;              // freed before the solve starts
+    return b + spectra_transform + (b >> 3) + (16u << 20);
+}
+
+struct Server {
+    int device = 0;
+    uint64_t budget = 0;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<ServeJob>> queue;
+    std::map<int64_t, std::shared_ptr<ServeJob>> jobs;
+    uint64_t reserved = 0;
+    int running = 0, peak_running = 0;
+    int64_t next_ticket = 1, submitted = 0, completed = 0;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+
+    void run(int slot);
+    void solve(Context &ctx, ServeJob &job);
+};
+
+void Server::solve(Context &ctx, ServeJob &job) {
+    const SolveOptions &o = job.opts;
+    auto p = make_problem(&ctx, job.cx, job.cy, job.a, job.b, job.k, job.near_order, job.far_order,
+                          job.mask.empty() ? nullptr : job.mask.data());
+    const int held = o.precision == RT_C64 ? RT_MIXED : RT_C128;
+    auto op = create_operator(p.get(), held, RT_MODIFIER_NONE);
+    std::unique_ptr<Product> pc;
+    if (job.preconditioner) pc = create_preconditioner(p.get(), job.preconditioner, held);
+    const int64_t n = p->g.n;
+    DeviceArray<c128> b, x;
+    b.allocate(&ctx, n, "the right-hand side");
+    x.allocate(&ctx, n, "the solution");
+    RT_HIP(hipMemcpyAsync(b.ptr, job.rhs.data(), n * sizeof(c128), hipMemcpyHostToDevice, ctx.stream));
+    std::vector<SolveResult> r = solve_batch(&ctx, *op, pc.get(), o, b.ptr, nullptr, x.ptr, 1);
+    job.x.resize(n);
+    RT_HIP(hipMemcpyAsync(job.x.data(), x.ptr, n * sizeof(c128), hipMemcpyDeviceToHost, ctx.stream));
+    RT_HIP(hipStreamSynchronize(ctx.stream));
+    job.result = r[0];
+}
+
+void Server::run(int) {
+    Context ctx;
+    try {
+        ctx.open(device, budget);
+    } catch (const Error &) {
+        return;
+    }
+    for (;;) {
+        std::shared_ptr<ServeJob> job;
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv_work.wait(lk, [&] {
+                return stopping || (!queue.empty() && (reserved + queue.front()->bytes <= budget || running == 0));
+            });
+            if (stopping) break;
+            job = queue.front();
+            queue.pop_front();
+            reserved += job->bytes;
+            ++running;
+            peak_running = std::max(peak_running, running);
+        }
+        job->started = Clock::now();
+        try {
+            solve(ctx, *job);
+        } catch (const Error &e) {
+            job->status = e.status;
+            job->error_id = e.id;
+            job->error_message = e.what();
+        } catch (const std::exception &e) {
+            job->status = RT_EINVAL;
+            job->error_id = "rooftop:server";
+            job->error_message = e.what();
+        }
+        try {
+            ctx.free_pool();
+        } catch (const Error &) {
+        }
+        {
+            std::lock_guard<std::mutex> lk(m);
+            job->finished = Clock::now();
+            job->done = true;
+            reserved -= job->bytes;
+            --running;
+            ++completed;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+    ctx.close();
+}
+
+}  // namespace rt
+
+using namespace rt;
+
+struct rt_server {
+    Server s;
+};
+
+extern "C" {
+
+int rt_blob_mask(int32_t cx, int32_t cy, uint64_t seed, double fill, uint8_t *mask) {
+    try {
+        if (cx < 1 || cy < 1 || !mask || !(fill > 0 && fill <= 1))
+            fail(RT_EINVAL, "rooftop:mask", "positive counts, a mask buffer and a fill in (0, 1] are needed");
+        const int64_t cells = static_cast<int64_t>(cx) * cy;
+        std::fill(mask, mask + cells, 0);
+        // splitmix64: a seeded, portable generator.
+        uint64_t state = seed;
+        auto next = [&]() {
+            uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            return z ^ (z >> 31);
+        };
+        auto uniform = [&](uint64_t bound) { return static_cast<int64_t>(next() % bound); };
+        const int64_t target = std::max<int64_t>(1, static_cast<int64_t>(fill * static_cast<double>(cells) + 0.5));
+        std::vector<int64_t> frontier;
+        std::vector<uint8_t> queued(cells, 0);
+        auto push = [&](int64_t c) {
+            if (!mask[c] && !queued[c]) {
+                queued[c] = 1;
+                frontier.push_back(c);
+            }
+        };
+        frontier.push_back(uniform(cells));
+        queued[frontier[0]] = 1;
+        for (int64_t grown = 0; grown < target && !frontier.empty(); ++grown) {
+            const int64_t pick = uniform(frontier.size());
+            const int64_t c = frontier[pick];
+            frontier[pick] = frontier.back();
+            frontier.pop_back();
+            mask[c] = 1;
+            const int x = static_cast<int>(c % cx), y = static_cast<int>(c / cx);
+            if (x > 0) push(c - 1);
+            if (x + 1 < cx) push(c + 1);
+            if (y > 0) push(c - cx);
+            if (y + 1 < cy) push(c + cx);
+        }
+        set_last_error(RT_OK, "", "");
+        return RT_OK;
+    } catch (const Error &e) {
+        set_last_error(e.status, e.id, e.what());
+        return e.status;
+    }
+}
+
+int rt_server_create(int32_t device, uint64_t budget, int32_t slots, rt_server **out) {
+    try {
+        if (!out || slots < 1 || slots > 64) fail(RT_EINVAL, "rooftop:server", "1..64 slots and an output are needed");
+        int count = 0;
+        RT_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) fail(RT_EINVAL, "rooftop:server", "no such device");
+        RT_HIP(hipSetDevice(device));
+        size_t free_b = 0, total_b = 0;
+        RT_HIP(hipMemGetInfo(&free_b, &total_b));
+        auto h = std::make_unique<rt_server>();
+        h->s.device = device;
+        h->s.budget = budget ? budget : static_cast<uint64_t>(free_b) / 10 * 9;
+        for (int i = 0; i < slots; ++i) h->s.workers.emplace_back([srv = &h->s, i] { srv->run(i); });
+        *out = h.release();
+        set_last_error(RT_OK, "", "");
+        return RT_OK;
+    } catch (const Error &e) {
+        set_last_error(e.status, e.id, e.what());
+        return e.status;
+    }
+}
+
+void rt_server_destroy(rt_server *h) {
+    if (!h) return;
+    {
+        std::lock_guard<std::mutex> lk(h->s.m);
+        h->s.stopping = true;
+        for (auto &job : h->s.queue) {   // never started: their waiters get an error
+            job->done = true;
+            job->status = RT_EINVAL;
+            job->error_id = "rooftop:server";
+            job->error_message = "the server was destroyed before the request started";
+        }
+        h->s.queue.clear();
+    }
+    h->s.cv_work.notify_all();
+    h->s.cv_done.notify_all();
+    for (auto &t : h->s.workers) t.join();
+    delete h;
+}
+
+int rt_server_submit(rt_server *h, const rt_request *r, int64_t *ticket) {
+    try {
+        if (!h || !r || !ticket || !r->rhs) fail(RT_EINVAL, "rooftop:server", "NULL argument");
+        const GridInfo g = GridInfo::make(r->count_x, r->count_y, 1.0, 1.0);
+        if (r->count_x < 1 || r->count_y < 1 || g.n < 1)
+            fail(RT_EINVAL, "rooftopMoM:mesh:noRooftops", "The grid carries no rooftop.");
+        if (r->n != g.n)
+            fail(RT_ESIZE, "rooftopMoM:solver:sizeMismatch", "The right-hand side does not match the grid's rooftops.");
+        if (r->preconditioner != 0 && r->preconditioner != RT_PRECOND_JACOBI && r->preconditioner != RT_PRECOND_NEARFIELD)
+            fail(RT_EINVAL, "MATLAB:validators:mustBeMember", "The preconditioner is none, jacobi or nearField.");
+        auto job = std::make_shared<ServeJob>();
+        job->cx = r->count_x;
+        job->cy = r->count_y;
+        job->a = r->size_x;
+        job->b = r->size_y;
+        job->k = r->wavenumber;
+        job->near_order = r->near_order > 0 ? r->near_order : 16;
+        job->far_order = r->far_order > 0 ? r->far_order : 8;
+        if (r->mask) job->mask.assign(r->mask, r->mask + static_cast<size_t>(r->count_x) * r->count_y);
+        const c128 *v = static_cast<const c128 *>(r->rhs);
+        job->rhs.assign(v, v + g.n);
+        SolveOptions o;
+        o.method = r->opts.method;
+        o.tolerance = r->opts.tolerance;
+        o.max_iterations = r->opts.max_iterations;
+        o.restart = r->opts.restart;
+        o.precision = r->opts.precision;
+        o.refine = r->opts.refine;
+        o.inner_tolerance = r->opts.inner_tolerance;
+        o.max_outer = r->opts.max_outer > 0 ? r->opts.max_outer : 20;
+        if (o.method != RT_METHOD_CG && o.method != RT_METHOD_GMRES)
+            fail(RT_EINVAL, "MATLAB:validators:mustBeMember", "The method is 'cg' or 'gmres'.");
+        if (!(o.tolerance > 0 && o.tolerance < 1) || o.max_iterations < 1)
+            fail(RT_EINVAL, "MATLAB:validators:mustBePositive", "Invalid tolerance or iteration limit.");
+        if (o.precision != RT_C128 && o.precision != RT_C64)
+            fail(RT_EINVAL, "rooftop:solver:invalidPrecision", "The inner precision is RT_C128 or RT_C64.");
+        if (o.precision == RT_C64 && !o.refine)
+            fail(RT_EINVAL, "rooftop:solver:invalidPrecision",
+                 "A complex64 inner loop needs the FP64 refinement loop (refine = 1).");
+        job->opts = o;
+        job->preconditioner = r->preconditioner;
+        job->bytes = estimate_request_bytes(job->cx, job->cy, o, job->preconditioner);
+        {
+            std::lock_guard<std::mutex> lk(h->s.m);
+            if (job->bytes > h->s.budget)
+                fail(RT_ENOMEM, "rooftopMoM:resources:insufficientMemory",
+                     "The request needs about %llu bytes, more than the server's budget of %llu.",
+                     static_cast<unsigned long long>(job->bytes), static_cast<unsigned long long>(h->s.budget));
+            job->ticket = h->s.next_ticket++;
+            job->submitted = Clock::now();
+            h->s.jobs[job->ticket] = job;
+            h->s.queue.push_back(job);
+            ++h->s.submitted;
+            *ticket = job->ticket;
+        }
+        h->s.cv_work.notify_all();
+        set_last_error(RT_OK, "", "");
+        return RT_OK;
+    } catch (const Error &e) {
+        set_last_error(e.status, e.id, e.what());
+        return e.status;
+    }
+}
+
+int rt_request_bytes(const rt_request *r, uint64_t *bytes) {
+    if (!r || !bytes || r->count_x < 1 || r->count_y < 1) return RT_EINVAL;
+    SolveOptions o;
+    o.method = r->opts.method;
+    o.restart = r->opts.restart;
+    o.precision = r->opts.precision;
+    o.refine = r->opts.refine;
+    *bytes = estimate_request_bytes(r->count_x, r->count_y, o, r->preconditioner);
+    return RT_OK;
+}
+
+int rt_server_wait(rt_server *h, int64_t ticket, void *x, int64_t n, rt_output *out, double *times) {
+    try {
+        if (!h) fail(RT_EINVAL, "rooftop:server", "NULL server");
+        std::shared_ptr<ServeJob> job;
+        {
+            std::unique_lock<std::mutex> lk(h->s.m);
+            auto it = h->s.jobs.find(ticket);
+            if (it == h->s.jobs.end()) fail(RT_EINVAL, "rooftop:server", "unknown ticket %lld", (long long)ticket);
+            job = it->second;
+            h->s.cv_done.wait(lk, [&] { return job->done; });
+            h->s.jobs.erase(ticket);
+        }
+        if (job->status != RT_OK) fail(job->status, job->error_id.c_str(), "%s", job->error_message.c_str());
+        if (x) {
+            if (n != static_cast<int64_t>(job->x.size()))
+                fail(RT_ESIZE, "rooftopMoM:solver:sizeMismatch", "The solution buffer does not match the request.");
+            std::memcpy(x, job->x.data(), job->x.size() * sizeof(c128));
+        }
+        if (out) {
+            const SolveResult &r = job->result;
+            out->converged = r.converged;
+            out->iteration_count = r.iterations;
+            out->residual_norm = r.residual;
+            out->time_s = r.time;
+            out->outer_count = r.outer;
+            out->residual_norms_count = static_cast<int32_t>(r.norms.size());
+            if (out->residual_norms && out->residual_norms_capacity > 0) {
+                const size_t k = std::min<size_t>(r.norms.size(), static_cast<size_t>(out->residual_norms_capacity));
+                std::memcpy(out->residual_norms, r.norms.data(), k * sizeof(double));
+            }
+        }
+        if (times) {
+            auto sec = [](Clock::duration d) { return std::chrono::duration<double>(d).count(); };
+            times[0] = sec(job->started - job->submitted);
+            times[1] = sec(job->finished - job->started);
+            times[2] = sec(job->finished - job->submitted);
+        }
+        set_last_error(RT_OK, "", "");
+        return RT_OK;
+    } catch (const Error &e) {
+        set_last_error(e.status, e.id, e.what());
+        return e.status;
+    }
+}
+
+int rt_server_stats(rt_server *h, int64_t *counts, uint64_t *bytes) {
+    if (!h) return RT_EINVAL;
+    std::lock_guard<std::mutex> lk(h->s.m);
+    if (counts) {
+        counts[0] = h->s.submitted;
+        counts[1] = h->s.completed;
+        counts[2] = static_cast<int64_t>(h->s.queue.size());
+        counts[3] = h->s.running;
+        counts[4] = h->s.peak_running;
+    }
+    if (bytes) {
+        bytes[0] = h->s.reserved;
+        bytes[1] = h->s.budget;
+    }
+    return RT_OK;
+}
+
+}  // extern "C"

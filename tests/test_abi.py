@@ -4,6 +4,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -68,3 +69,22 @@ def test_missing_library_fails_loudly(rt, monkeypatch):
     monkeypatch.setattr(rt._abi, 'LIBRARY', '/nonexistent/librooftop.so')
     with pytest.raises(RuntimeError, match='no CPU fallback'):
         rt.library()
+
+
+def test_blob_mask_is_a_seeded_connected_blob(rt):
+    """rt_blob_mask (host code): the requested fill, 4-connected, reproducible per seed."""
+    import collections
+    for seed, (cx, cy), fill in ((1000, (40, 30), 0.3), (1001, (17, 64), 0.9), (7, (5, 5), 1.0)):
+        m = rt.blob_mask(cx, cy, seed, fill)
+        assert m.shape == (cx, cy) and m.sum() == max(1, int(fill * cx * cy + 0.5))
+        start = tuple(np.argwhere(m)[0])
+        seen, todo = {start}, collections.deque([start])
+        while todo:
+            i, j = todo.popleft()
+            for a, b in ((i + 1, j), (i - 1, j), (i, j + 1), (i, j - 1)):
+                if 0 <= a < cx and 0 <= b < cy and m[a, b] and (a, b) not in seen:
+                    seen.add((a, b))
+                    todo.append((a, b))
+        assert len(seen) == m.sum()
+        assert np.array_equal(m, rt.blob_mask(cx, cy, seed, fill))
+    assert not np.array_equal(rt.blob_mask(40, 30, 1, 0.5), rt.blob_mask(40, 30, 2, 0.5))
