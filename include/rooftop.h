@@ -142,18 +142,24 @@ int rt_product_memory(rt_product *product, uint64_t *stored, uint64_t *working);
 
 /* Measurement (no counterpart in the reference): runs `reps` applications of the operator on nrhs device vectors
  * and times each of its three kernels (pass A: x transforms, pass B: y transforms + spectral multiply, pass C:
- * inverse x transforms) with HIP events on the context's stream. ms[k] is the average duration of one launch of
- * pass k; bytes[k] its algorithmic HBM bytes per launch (vector and spectra traffic, each element once). */
+ * inverse x transforms) with HIP events on the streams they run on. ms (4 entries): ms[k] (k < 3) is the kernel time
+ * of pass k in one application, summed over the application's right-hand-side chunks, and ms[3] the wall time of one
+ * application (less than the sum when chunks overlap); bytes (3 entries): bytes[k] the algorithmic HBM bytes of pass
+ * k in one application (vector and spectra traffic, each element once). */
 int rt_product_profile(rt_product *product, int64_t nrhs, int32_t reps, int32_t precision, double *ms,
                        double *bytes);
 
 /* Autotuner (no counterpart in the reference; the north star's FFT tile and radix schedule): times the launch shapes
  * of the three passes for nrhs right-hand sides in the given precision -- plan radices up to 8 or 16, pass A/C
- * rooftop columns per block, pass B x positions per block -- and keeps the fastest for every later application with
- * that many right-hand sides. shape (may be NULL) receives {largest radix bits, columns per block, positions per
- * block}; ms (may be NULL) the summed pass time of one application with it. */
+ * rooftop columns per block, pass B x positions per block and teams per position, right-hand-side chunks -- and
+ * keeps the fastest for every later application with that many right-hand sides. shape (may be NULL, else 5
+ * entries) receives {largest radix bits, columns per block, positions per block, pass B teams per position (1: one
+ * team transforms both columns of a position, 2: a team per column), chunks (the right-hand sides split into chunks
+ * that alternate between two streams, so that one chunk's passes overlap another's)}; ms (may be NULL) the wall
+ * time of one application with it. */
 int rt_product_tune(rt_product *product, int64_t nrhs, int32_t precision, int32_t *shape, double *ms);
-/* Sets the launch shape used for nrhs right-hand sides in the given precision (a saved rt_product_tune result);
+/* Sets the launch shape (5 entries, as rt_product_tune returns them) used for nrhs right-hand sides in the given
+ * precision;
  * RT_EINVAL when the shape is malformed or does not fit the device for this product. */
 int rt_product_set_shape(rt_product *product, int64_t nrhs, int32_t precision, const int32_t *shape);
 

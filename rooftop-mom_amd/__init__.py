@@ -230,29 +230,33 @@ class Operator(Product):
 
     def tune(self, nrhs=1, precision=None):
         """Times the launch shapes of the matvec passes for nrhs right-hand sides and keeps the fastest (the FFT
-        tile and radix schedule); returns ({'bits', 'group', 'tile'}, milliseconds of one application)."""
+        tile and radix schedule); returns ({'bits', 'group', 'tile', 'teams', 'chunks'}, milliseconds of one
+        application)."""
         prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
                                                          else self.precision)
-        shape = (C.c_int32 * 3)()
+        shape = (C.c_int32 * 5)()
         ms = C.c_double()
         check(library().rt_product_tune(self.handle, nrhs, prec, shape, C.byref(ms)))
-        return {'bits': shape[0], 'group': shape[1], 'tile': shape[2]}, ms.value
+        return {'bits': shape[0], 'group': shape[1], 'tile': shape[2], 'teams': shape[3], 'chunks': shape[4]}, ms.value
 
     def setShape(self, shape, nrhs=1, precision=None):
-        """Uses shape {'bits', 'group', 'tile'} for nrhs right-hand sides (a saved tune() result)."""
+        """Uses shape {'bits', 'group', 'tile', 'teams', 'chunks'} for nrhs right-hand sides (a saved tune() result;
+        teams and chunks default to 1)."""
         prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
                                                          else self.precision)
-        arr = (C.c_int32 * 3)(shape['bits'], shape['group'], shape['tile'])
+        arr = (C.c_int32 * 5)(shape['bits'], shape['group'], shape['tile'], shape.get('teams', 1),
+                              shape.get('chunks', 1))
         check(library().rt_product_set_shape(self.handle, nrhs, prec, arr))
 
     def profile(self, nrhs=1, reps=20, precision=None):
-        """Average per-launch milliseconds and algorithmic bytes of the three passes (HIP events)."""
+        """Kernel milliseconds and algorithmic bytes of the three passes in one application, and the wall
+        milliseconds of one application (HIP events): (pass_ms, pass_bytes, wall_ms)."""
         prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
                                                          else self.precision)
-        ms = (C.c_double * 3)()
+        ms = (C.c_double * 4)()
         by = (C.c_double * 3)()
         check(library().rt_product_profile(self.handle, nrhs, reps, prec, ms, by))
-        return list(ms), list(by)
+        return list(ms)[:3], list(by), ms[3]
 
 
 class Preconditioner(Product):

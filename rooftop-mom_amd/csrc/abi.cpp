@@ -403,11 +403,11 @@ int rt_product_tune(rt_product *p, int64_t nrhs, int32_t precision, int32_t *sha
         need(p && p->p && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product", "invalid tuning arguments");
         Context *c = p->p->problem->ctx;
         c->activate();
-        int sh[3] = {0, 0, 0};
+        int sh[5] = {0, 0, 0, 0, 0};
         double t = 0;
         operator_tune(*p->p, static_cast<int>(nrhs), precision, sh, &t, c->stream);
         if (shape)
-            for (int k = 0; k < 3; ++k) shape[k] = sh[k];
+            for (int k = 0; k < 5; ++k) shape[k] = sh[k];
         if (ms) *ms = t;
     });
 }
@@ -417,7 +417,7 @@ int rt_product_set_shape(rt_product *p, int64_t nrhs, int32_t precision, const i
         need(p && p->p && shape && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product", "invalid shape arguments");
         Context *c = p->p->problem->ctx;
         c->activate();
-        const int sh[3] = {shape[0], shape[1], shape[2]};
+        const int sh[5] = {shape[0], shape[1], shape[2], shape[3], shape[4]};
         operator_set_shape(*p->p, static_cast<int>(nrhs), precision, sh);
     });
 }

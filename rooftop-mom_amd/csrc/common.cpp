@@ -107,6 +107,16 @@ void Context::open(int dev, uint64_t budget) {
     RT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 }
 
+hipStream_t Context::side_stream() {
+    if (!side) RT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    return side;
+}
+
+hipEvent_t Context::sync_event(int k) {
+    if (!sync_events[k]) RT_HIP(hipEventCreateWithFlags(&sync_events[k], hipEventDisableTiming));
+    return sync_events[k];
+}
+
 void Context::close() {
     (void)hipSetDevice(device);
     if (stream) {
@@ -114,6 +124,16 @@ void Context::close() {
         (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
+    if (side) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+        side = nullptr;
+    }
+    for (hipEvent_t &e : sync_events)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
     for (auto &kv : sizes) (void)hipFree(kv.first);
     sizes.clear();
     pool.clear();

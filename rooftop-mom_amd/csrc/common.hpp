@@ -80,6 +80,12 @@ void set_last_error(int status, const std::string &id, const std::string &messag
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // A second stream (default priority, created on first use) on which the matvec runs every other chunk of its
+    // right-hand sides beside the main stream, and the untimed events that order the two.
+    hipStream_t side = nullptr;
+    hipEvent_t sync_events[4] = {};
+    hipStream_t side_stream();
+    hipEvent_t sync_event(int k);
     uint64_t ceiling = 0;
     uint64_t held = 0;
     std::mutex lock;

@@ -32,6 +32,12 @@ constexpr int kThreads = 256;
 constexpr int kLdp = 64;        // partial columns per block
 constexpr int kPoll = 8;        // steps enqueued between two polls of the loop states
 constexpr int kStall = 40;      // COCG steps without a new best norm that end a complex64 inner solve
+// Mixed COCG: the first inner solve in complex64, to kCocgInner or a stall, and the later ones in FP64. Complex64
+// COCG loses the complex symmetry of its products on systems whose residual swings high (on the 512 x 512 plate
+// under grazing plane waves it stalls between 1e-2 and 1e-4); the FP64 inner solve restarted from the refined
+// residual converges where a single FP64 COCG from zero may not within its iteration limit.
+constexpr bool kCocgFallback = true;
+constexpr double kCocgInner = 1e-2;
 
 // Scalar slots of a system.
 enum ScalSlot {
@@ -861,7 +867,7 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
         for (int r = 0; r < R; ++r) rel[r] = zero_rhs[r] ? 0.0 : rel[r] / bnorm[r];
         return rel;
     };
-    const int inner_prec = o.precision;
+    int inner_prec = o.precision;
     LinOp A = [&](const c128 *in, c128 *res, const int *skip) {
         op.apply(in, res, R, inner_prec, s, skip, kStateInts);
     };
@@ -903,10 +909,16 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
         c128 *rv = vec(ctx, kSlotOuterR, R * n), *d = vec(ctx, kSlotOuterD, R * n);
         std::vector<double> rel = true_residual(x);
         // Inner tolerance: complex64 GMRES reaches about 1e-6 before the rounding of its products shows; COCG's
-        // short recurrence loses the complex symmetry of complex64 products near 1e-4.
+        // short recurrence loses the complex symmetry of complex64 products near 1e-4 (sooner on systems whose
+        // residual norm swings high on the way), so a mixed COCG hands what is left after its first inner solve to
+        // FP64 inner solves when the products hold FP64 (and, without them, refines with complex64 inner solves to
+        // 1e-4 at most).
         double inner_tol = o.tolerance;
         if (o.inner_tolerance > 0) inner_tol = o.inner_tolerance;
-        else if (o.precision == RT_C64) inner_tol = std::max(o.tolerance, o.method == RT_METHOD_CG ? 1e-4 : 1e-6);
+        const bool fallback = kCocgFallback && o.method == RT_METHOD_CG && o.precision == RT_C64 &&
+                              op.has(RT_C128) && (!pc || pc->has(RT_C128));
+        if (o.inner_tolerance <= 0 && o.precision == RT_C64)
+            inner_tol = std::max(o.tolerance, o.method == RT_METHOD_CG ? (fallback ? kCocgInner : 1e-4) : 1e-6);
         std::vector<char> done(R);
         std::vector<int> total(R, 0);
         for (int r = 0; r < R; ++r) {
@@ -914,7 +926,7 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
             out[r].norms.push_back(rel[r]);
         }
         // An inner complex64 COCG that stagnates (no new best norm for kStall iterations) hands back to this loop.
-        const int stall = o.method == RT_METHOD_CG && o.precision == RT_C64 ? kStall : 0;
+        int stall = o.method == RT_METHOD_CG && o.precision == RT_C64 ? kStall : 0;
         int outer = 0;
         while (outer < o.max_outer) {
             bool active = false;
@@ -926,6 +938,11 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
             }
             if (!active) break;
             ++outer;
+            if (fallback && outer == 2) {   // the rest of a mixed COCG in FP64
+                inner_prec = RT_C128;
+                inner_tol = o.tolerance;
+                stall = 0;
+            }
             ws.copy(t, rv, nullptr);   // t holds b - A x from true_residual
             // Each system asks the inner solve for what is left to reach the tolerance, no less than the floor.
             std::vector<double> want(R, 0.5);

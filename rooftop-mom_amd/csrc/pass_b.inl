@@ -108,6 +108,7 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
         });
         multiply_round<TC, P, false>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
         __syncthreads();
+        tw = opaque(tw);
         static_for_down<1, P::K - 1>([&](auto S) {
             constexpr int s = decltype(S)::value;
             cround<TC, P, s, true, false, true>(
@@ -130,9 +131,131 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
     }
 }
 
+// ------------------------------------------------------------------------------------------------------------------
+// Pass B with a team per component: the 2 T threads of an x position form a team for each of its two columns, which
+// doubles the waves that share the position's LDS (the LDS, not the registers, bounds the blocks per CU for long
+// columns). Each team runs its column's rounds; the last forward round, the multiply and the first inverse round
+// stay in the registers of the team's own column, and the multiply reads the other column's transformed values
+// from the LDS. Needs T >= 64, so that each wave belongs to one team and the role is wave-uniform.
+// ------------------------------------------------------------------------------------------------------------------
+template <typename TC, class P, int C>
+__device__ __forceinline__ void multiply_team(int t, cplx<TC> *mine, const cplx<TC> *other,
+                                              const cplx<TC> *__restrict__ row, cplx<TC> dxv,
+                                              const cplx<TC> *__restrict__ dy) {
+    constexpr int S = P::K - 1;
+    constexpr int R = 1 << P::lr(S), U = P::E / R;
+    constexpr int CH = sizeof(TC) == sizeof(float) ? 8 : 4;
+    cplx<TC> v[U][R];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int base = (t + u * P::T) * R;   // the last round has stride 1: positions base + q
+        const int pb = base + (base >> P::log2E);
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[u][q] = mine[pb + q];
+        dft<TC, R, false, false>(v[u]);
+#pragma unroll
+        for (int q = 0; q < R; ++q) mine[pb + q] = v[u][q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int base = (t + u * P::T) * R;
+        const int pb = base + (base >> P::log2E);
+        const cplx<TC> *sp = row + 3 * base;
+        const cplx<TC> *dyp = dy + base;
+#pragma unroll
+        for (int q0 = 0; q0 < R; q0 += CH) {
+#pragma unroll
+            for (int q = q0; q < q0 + CH && q < R; ++q) {
+                const cplx<TC> o = other[pb + q];
+                const cplx<TC> self = sp[3 * q + C], el = sp[3 * q + 2];
+                const cplx<TC> dyv = dyp[q];
+                cplx<TC> &w = v[u][q];
+                if constexpr (C == 0) {
+                    const cplx<TC> phi = el * (cmulc(dxv, w) + cmulc(dyv, o));
+                    w = self * w - dxv * phi;
+                } else {
+                    const cplx<TC> phi = el * (cmulc(dxv, o) + cmulc(dyv, w));
+                    w = self * w - dyv * phi;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        dft<TC, R, true, false>(v[u]);
+    }
+    __syncthreads();   // the other team has read this column
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int base = (t + u * P::T) * R;
+        const int pb = base + (base >> P::log2E);
+#pragma unroll
+        for (int q = 0; q < R; ++q) mine[pb + q] = v[u][q];
+    }
+}
+
+template <typename TC, class P, int C>
+__device__ __forceinline__ void pass_b_team(int t, cplx<TC> *mine, const cplx<TC> *other, cplx<TC> *w, int nc,
+                                            const cplx<TC> *tw, const cplx<TC> *row, cplx<TC> dxv,
+                                            const cplx<TC> *dy) {
+    cround<TC, P, 0, false, true, true>(
+        t, tw, [&](int pos, int) { return pos < nc ? w[pos] : mk<TC>(0, 0); },
+        [&](int, int pp, cplx<TC> v) { mine[pp] = v; });
+    __syncthreads();
+    static_for<1, P::K - 1>([&](auto S) {
+        constexpr int s = decltype(S)::value;
+        cround<TC, P, s, false, false, true>(
+            t, tw, [&](int, int pp) { return mine[pp]; }, [&](int, int pp, cplx<TC> v) { mine[pp] = v; });
+        __syncthreads();
+    });
+    multiply_team<TC, P, C>(t, mine, other, row, dxv, dy);
+    __syncthreads();
+    tw = opaque(tw);
+    static_for_down<1, P::K - 1>([&](auto S) {
+        constexpr int s = decltype(S)::value;
+        cround<TC, P, s, true, false, true>(
+            t, tw, [&](int, int pp) { return mine[pp]; }, [&](int, int pp, cplx<TC> v) { mine[pp] = v; });
+        __syncthreads();
+    });
+    cround<TC, P, 0, true, true, true>(
+        t, tw, [&](int, int pp) { return mine[pp]; },
+        [&](int pos, int, cplx<TC> v) {
+            if (pos < nc) w[pos] = v;
+        });
+}
+
+template <typename TC, int LOG2M, int BITS, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_b2_k(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ bspec,
+                                                  const int *__restrict__ fold_x, const cplx<TC> *__restrict__ dx,
+                                                  const cplx<TC> *__restrict__ dy, ProdDims d,
+                                                  const cplx<TC> *__restrict__ tw, int tile, int R, int nblocks,
+                                                  const int *__restrict__ skip, int skip_stride) {
+    using P = CPlan<LOG2M, BITS>;
+    static_assert(P::K >= 2 && P::T % 64 == 0, "a team per component needs two rounds and whole waves");
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    const int r = lb % R, pt = lb / R;
+    if (skip && skip[r * skip_stride]) return;
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int Mp = P::M + (P::M >> P::log2E);
+    const int tt = threadIdx.x / (2 * P::T);
+    const int rem = threadIdx.x - tt * 2 * P::T;
+    const int c = __builtin_amdgcn_readfirstlane(rem / P::T);   // the team's component, uniform in the wave
+    const int t = rem - c * P::T;
+    const int px = pt * tile + tt;
+    cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + 2 * tt * Mp;
+    cplx<TC> *col1 = col0 + Mp;
+    const cplx<TC> *row = bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+    const cplx<TC> dxv = dx[px];
+    if (c == 0)
+        pass_b_team<TC, P, 0>(t, col0, col1, work + work_index(d, r, 0, px, 0), d.cols_x, tw, row, dxv, dy);
+    else
+        pass_b_team<TC, P, 1>(t, col1, col0, work + work_index(d, r, 1, px, 0), d.cols_y, tw, row, dxv, dy);
+}
+
 template <typename TC>
 void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
-                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer) {
+                   const cplx<TC> *bspec, int nrhs, int tile, int teams, hipStream_t s, const int *skip,
+                   int skip_stride, const KernelTimer *timer) {
     const FftPlan &p = ty.plan;
     constexpr int es = sizeof(cplx<TC>);
     const int64_t nb = static_cast<int64_t>(op.dims.Mx / tile) * nrhs;
@@ -145,12 +268,23 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
                              : reinterpret_cast<const cplx<TC> *>(tx.d32.ptr);
     const cplx<TC> *dy = dbl ? reinterpret_cast<const cplx<TC> *>(ty.d.ptr)
                              : reinterpret_cast<const cplx<TC> *>(ty.d32.ptr);
-#define RT_LAUNCH_B(L, B)                                                                                      \
-    {                                                                                                          \
-        constexpr int MT = CPlan<L, B>::T > 256 ? CPlan<L, B>::T : 256;                                        \
-        set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, B, MT>), lds);                                 \
-        launch_timed(timer, pass_b_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads), static_cast<uint32_t>(lds), s, work, bspec,   \
-                           tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, static_cast<int>(nb), skip, skip_stride); \
+    if (!b_threads_fit(p, tile, teams)) fail(RT_EINVAL, "rooftop:product", "pass B shape does not fit this plan");
+#define RT_LAUNCH_B(L, B)                                                                                          \
+    {                                                                                                              \
+        using PL = CPlan<L, B>;                                                                                    \
+        if (teams == 1) {                                                                                          \
+            constexpr int MT = PL::T > 256 ? PL::T : 256;                                                          \
+            set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, B, MT>), lds);                                 \
+            launch_timed(timer, pass_b_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads),                         \
+                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, \
+                         static_cast<int>(nb), skip, skip_stride);                                                 \
+        } else if constexpr (PL::K >= 2 && PL::T % 64 == 0 && 2 * PL::T <= 1024) {                                 \
+            constexpr int MT = 2 * PL::T > 256 ? 2 * PL::T : 256;                                                  \
+            set_lds(reinterpret_cast<const void *>(&pass_b2_k<TC, L, B, MT>), lds);                                \
+            launch_timed(timer, pass_b2_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(2 * threads),                    \
+                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, \
+                         static_cast<int>(nb), skip, skip_stride);                                                 \
+        }                                                                                                          \
     }
     RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_B);
 #undef RT_LAUNCH_B

@@ -47,6 +47,15 @@ struct CPlan {
 template <int I>
 using ic = std::integral_constant<int, I>;
 
+// The same pointer, opaque to the compiler: loads through it are not merged with earlier loads of the same
+// addresses. The inverse rounds of pass B read the forward rounds' twiddles again through it instead of keeping
+// them in registers across the multiply, which would cost occupancy.
+template <typename T>
+__device__ __forceinline__ const T *opaque(const T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F &&f) {
     if constexpr (B < E) {
@@ -146,6 +155,14 @@ inline int64_t blocks_ac(const ProdDims &d, int nrhs, int g) {
 // (512 for complex128 with radix 16, whose butterflies need more registers), pass B one team of at least 256.
 inline int max_threads_ac(int es, const FftPlan &p) { return es == 16 && p.log2E >= 4 ? 512 : 1024; }
 inline int max_threads_b(const FftPlan &p) { return team_threads(p) > 256 ? team_threads(p) : 256; }
+// Whether pass B can run `tile` x positions per block with `teams` teams per position (2: a team per column, which
+// needs two rounds and whole waves per team, and at most 1024 threads).
+inline bool b_threads_fit(const FftPlan &p, int tile, int teams) {
+    const int T = team_threads(p);
+    if (teams == 1) return T * tile <= max_threads_b(p);
+    const int per = 2 * T;
+    return teams == 2 && p.K >= 2 && T % 64 == 0 && per <= 1024 && per * tile <= (per > 256 ? per : 256);
+}
 
 // Pass A/C LDS: [column][position (padded)] with a column stride that puts the columns of one position on
 // different banks (64 banks of 4 bytes).
@@ -175,8 +192,8 @@ void launch_pass_a(const Product &op, const PlanTables &t, const c128 *in, cplx<
                    hipStream_t s, const int *skip, int skip_stride, const KernelTimer *timer);
 template <typename TC>
 void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
-                   const cplx<TC> *bspec, int nrhs, int tile, hipStream_t s, const int *skip, int skip_stride,
-                   const KernelTimer *timer);
+                   const cplx<TC> *bspec, int nrhs, int tile, int teams, hipStream_t s, const int *skip,
+                   int skip_stride, const KernelTimer *timer);
 template <typename TC>
 void launch_pass_b_split(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
                          cplx<TC> *spec, const cplx<TC> *bspec, int nrhs, hipStream_t s, const int *skip,

@@ -139,7 +139,7 @@ void set_lds(const void *kernel, size_t bytes) {
 template <typename TC>
 static bool b_fits(const Product &op, const PassShape &sh) {
     const FftPlan &py = op.ty.of(sh.bits).plan;
-    return team_threads(py) * sh.tile <= max_threads_b(py) && lds_b_bytes(py, sh.tile, sizeof(cplx<TC>)) <= lds_bytes_max();
+    return b_threads_fit(py, sh.tile, sh.teams) && lds_b_bytes(py, sh.tile, sizeof(cplx<TC>)) <= lds_bytes_max();
 }
 // Whether pass B can run in three launches instead (one column per block).
 template <typename TC>
@@ -154,7 +154,7 @@ static bool shape_fits(const Product &op, const PassShape &sh) {
     const FftPlan &px = op.tx.of(sh.bits).plan;
     const int es = sizeof(cplx<TC>);
     const bool ac = team_threads(px) * sh.group <= max_threads_ac(es, px) && lds_ac_bytes(px, sh.group, es) <= lds_bytes_max();
-    const bool b = b_fits<TC>(op, sh) || (sh.tile == 1 && b_split_fits<TC>(op, sh.bits));
+    const bool b = b_fits<TC>(op, sh) || (sh.tile == 1 && sh.teams == 1 && b_split_fits<TC>(op, sh.bits));
     return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx && ac && b;
 }
 
@@ -251,9 +251,10 @@ static cplx<TC> *spec_of(Product &op, int nrhs) {
     return reinterpret_cast<cplx<TC> *>(op.spec32.ptr);
 }
 
+// The three passes of nrhs right-hand sides on stream s; after_a (if set) is recorded once pass A is queued.
 template <typename TC>
-static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
-                       hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
+static void run_chunk(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
+                      hipStream_t s, const int *skip, int skip_stride, hipEvent_t after_a) {
     const ProdDims &d = op.dims;
     Context *ctx = op.problem->ctx;
     const PlanTables &tx = op.tx.of(sh.bits), &ty = op.ty.of(sh.bits);
@@ -264,9 +265,10 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     KernelTimer ta(ctx, slot0, by[0]);
     launch_pass_a<TC>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride, &ta);
     ta.done();
+    if (after_a) RT_HIP(hipEventRecord(after_a, s));
     if (b_fits<TC>(op, sh)) {
         KernelTimer tb(ctx, slot0 + 1, by[1]);
-        launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, s, skip, skip_stride, &tb);
+        launch_pass_b<TC>(op, tx, ty, work, bspec, nrhs, sh.tile, sh.teams, s, skip, skip_stride, &tb);
         tb.done();
     } else {
         // Three launches, timed together as pass B (events recorded around them on the stream).
@@ -285,6 +287,37 @@ static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, i
     KernelTimer tc(ctx, slot0 + 2, by[2]);
     launch_pass_c<TC>(op, tx, work, out, nrhs, sh.group, s, skip, skip_stride, &tc);
     tc.done();
+}
+
+// The matvec of nrhs right-hand sides. With sh.chunks > 1 the right-hand sides run in chunks that alternate between
+// the main stream s and the context's side stream, the side stream one pass A behind: the HBM-bound passes A and C of
+// one chunk then overlap the VALU- and LDS-bound pass B of another. Chunks touch disjoint columns of the input,
+// output and work arrays; s waits for the side stream before it goes on.
+template <typename TC>
+static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
+                       hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
+    const int G = std::min(sh.chunks, nrhs);
+    if (G <= 1 || !b_fits<TC>(op, sh)) {
+        run_chunk<TC>(op, in, out, work, nrhs, sh, s, skip, skip_stride, nullptr);
+        return;
+    }
+    Context *ctx = op.problem->ctx;
+    hipStream_t side = ctx->side_stream();
+    hipEvent_t fork = ctx->sync_event(0), first_a = ctx->sync_event(1), join = ctx->sync_event(2);
+    RT_HIP(hipEventRecord(fork, s));
+    RT_HIP(hipStreamWaitEvent(side, fork, 0));
+    const int64_t n = op.dims.n, per = static_cast<int64_t>(2) * op.dims.ldj * op.dims.Mx;
+    int r0 = 0;
+    for (int c = 0; c < G; ++c) {
+        const int cnt = nrhs / G + (c < nrhs % G ? 1 : 0);
+        if (c == 1) RT_HIP(hipStreamWaitEvent(side, first_a, 0));
+        run_chunk<TC>(op, in + r0 * n, out + r0 * n, work + r0 * per, cnt, sh, (c & 1) ? side : s,
+                      skip ? skip + static_cast<int64_t>(r0) * skip_stride : nullptr, skip_stride,
+                      c == 0 ? first_a : nullptr);
+        r0 += cnt;
+    }
+    RT_HIP(hipEventRecord(join, side));
+    RT_HIP(hipStreamWaitEvent(s, join, 0));
 }
 
 template <typename TC>
@@ -413,7 +446,7 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
 // untimed application first), through the context's kernel timing.
 template <typename TC>
 static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, const c128 *in, c128 *out, double *ms,
-                       double *bytes, hipStream_t s) {
+                       double *bytes, hipStream_t s, double *wall = nullptr) {
     Context *ctx = op.problem->ctx;
     cplx<TC> *work = work_of<TC>(op, nrhs);
     run_passes<TC>(op, in, out, work, nrhs, sh, s);
@@ -431,11 +464,25 @@ static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, con
     ctx->timing = true;
     for (int it = 0; it < reps; ++it) run_passes<TC>(op, in, out, work, nrhs, sh, s);
     ctx->drain_timing();
+    ctx->timing = false;
+    if (wall) {
+        // The wall time of back-to-back applications, with the kernel timing off.
+        hipEvent_t w0 = ctx->take_event(), w1 = ctx->take_event();
+        RT_HIP(hipEventRecord(w0, s));
+        for (int it = 0; it < reps; ++it) run_passes<TC>(op, in, out, work, nrhs, sh, s);
+        RT_HIP(hipEventRecord(w1, s));
+        RT_HIP(hipEventSynchronize(w1));
+        float wall_ms = 0;
+        RT_HIP(hipEventElapsedTime(&wall_ms, w0, w1));
+        ctx->event_pool.push_back(w0);
+        ctx->event_pool.push_back(w1);
+        *wall = wall_ms / reps;
+    }
     ctx->timing = was;
     for (int k = 0; k < 3; ++k) {
-        const double cnt = static_cast<double>(ctx->timed_launches[slot0 + k] - n0[k]);
-        ms[k] = (ctx->timed_ms[slot0 + k] - ms0[k]) / cnt;
-        bytes[k] = (ctx->timed_bytes[slot0 + k] - by0[k]) / cnt;
+        // Per application: the chunks of one application add up.
+        ms[k] = (ctx->timed_ms[slot0 + k] - ms0[k]) / reps;
+        bytes[k] = (ctx->timed_bytes[slot0 + k] - by0[k]) / reps;
         ctx->timed_launches[slot0 + k] = n0[k];
         ctx->timed_ms[slot0 + k] = ms0[k];
         ctx->timed_bytes[slot0 + k] = by0[k];
@@ -453,7 +500,7 @@ static void profile_as(Product &op, int nrhs, int reps, double *ms, double *byte
     if (!shape_fits<TC>(op, sh))
         fail(RT_EINVAL, "rooftop:product", "the FFT columns of this product (%d x %d) exceed the LDS of this device",
              op.dims.Mx, op.dims.My);
-    time_shape<TC>(op, nrhs, reps, sh, in.ptr, out.ptr, ms, bytes, s);
+    time_shape<TC>(op, nrhs, reps, sh, in.ptr, out.ptr, ms, bytes, s, &ms[3]);
 }
 
 void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s) {
@@ -480,7 +527,7 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
     for (int bits = 3; bits <= 4; ++bits) {
         PassShape sh = default_shape<TC>(op, nrhs, bits);
         double ac = 1e300, b = 1e300;
-        int g_best = 0, t_best = 0;
+        int g_best = 0, t_best = 0, k_best = 1;
         for (int g = 1; g <= 16; g *= 2) {
             PassShape c = sh;
             c.group = g;
@@ -494,21 +541,41 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
         }
         if (!g_best) continue;
         sh.group = g_best;
-        for (int tile = 1; tile <= 16; tile *= 2) {
-            PassShape c = sh;
-            c.tile = tile;
-            if (!shape_fits<TC>(op, c)) continue;
-            double ms[3], by[3];
-            time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s);
-            if (ms[1] < b) {
-                b = ms[1];
-                t_best = tile;
+        for (int teams = 1; teams <= 2; ++teams)
+            for (int tile = 1; tile <= 16; tile *= 2) {
+                PassShape c = sh;
+                c.tile = tile;
+                c.teams = teams;
+                if (!shape_fits<TC>(op, c)) continue;
+                double ms[3], by[3];
+                time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s);
+                if (ms[1] < b) {
+                    b = ms[1];
+                    t_best = tile;
+                    k_best = teams;
+                }
             }
-        }
         if (!t_best) continue;
         if (ac + b < best_total) {
             best_total = ac + b;
-            best = PassShape{bits, g_best, t_best};
+            best = PassShape{bits, g_best, t_best, k_best};
+        }
+    }
+    // Then the chunk count, by the wall time of whole applications (the chunks overlap).
+    {
+        double ms[3], by[3], wall = 0;
+        time_shape<TC>(op, nrhs, reps, best, in.ptr, out.ptr, ms, by, s, &wall);
+        best_total = wall;
+        const PassShape one = best;
+        for (int chunks = 2; chunks <= 8 && chunks <= nrhs; chunks *= 2) {
+            PassShape c = one;
+            c.chunks = chunks;
+            if (!b_fits<TC>(op, c)) break;
+            time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s, &wall);
+            if (wall < best_total) {
+                best_total = wall;
+                best = c;
+            }
         }
     }
     auto &tuned = sizeof(TC) == sizeof(float) ? op.tuned32 : op.tuned64;
@@ -532,6 +599,8 @@ void operator_tune(Product &op, int nrhs, int prec, int *shape, double *ms, hipS
         shape[0] = sh.bits;
         shape[1] = sh.group;
         shape[2] = sh.tile;
+        shape[3] = sh.teams;
+        shape[4] = sh.chunks;
     }
 }
 
@@ -539,10 +608,12 @@ void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
     if (op.type != kOperator) fail(RT_EINVAL, "rooftop:product", "only an operator has matvec passes");
     if (prec == RT_MIXED) prec = RT_C128;
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
-    const PassShape sh{shape[0], shape[1], shape[2]};
+    const PassShape sh{shape[0], shape[1], shape[2], shape[3], shape[4]};
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
-    if ((sh.bits != 3 && sh.bits != 4) || !pow2(sh.group) || !pow2(sh.tile) || sh.group > 16 || sh.tile > 16)
-        fail(RT_EINVAL, "rooftop:product", "a shape is {3 or 4, a power of two <= 16, a power of two <= 16}");
+    if ((sh.bits != 3 && sh.bits != 4) || !pow2(sh.group) || !pow2(sh.tile) || sh.group > 16 || sh.tile > 16 ||
+        (sh.teams != 1 && sh.teams != 2) || sh.chunks < 1 || sh.chunks > 64)
+        fail(RT_EINVAL, "rooftop:product",
+             "a shape is {3 or 4, a power of two <= 16, a power of two <= 16, 1 or 2, 1 .. 64}");
     const bool fits = prec == RT_C128 ? shape_fits<double>(op, sh) : shape_fits<float>(op, sh);
     if (!fits) fail(RT_EINVAL, "rooftop:product", "the shape does not fit the device for this product");
     auto &tuned = prec == RT_C128 ? op.tuned64 : op.tuned32;

@@ -152,10 +152,11 @@ def _oracle_product(p, cx, cy, a, b, x):
     return np.stack([S.split_fft_product(spectra, grid, x[:, r]) for r in range(x.shape[1])], axis=1)
 
 
-@pytest.mark.parametrize('size', [(17, 33), (64, 3), (3, 100), (130, 40), (1, 40), (40, 1)])
+@pytest.mark.parametrize('size', [(17, 33), (64, 3), (3, 100), (130, 40), (1, 40), (40, 1), (7, 200), (20, 300)])
 def test_product_shapes_across_transform_lengths(rt, size):
-    """Every launch shape the autotuner may pick (both plans, all column groups and tiles that fit) gives the same
-    product, on grids whose embeddings cover transform lengths 1 .. 256 in both directions."""
+    """Every launch shape the autotuner may pick (both plans, all column groups, tiles and pass B teams that fit)
+    gives the same product, on grids whose embeddings cover transform lengths 1 .. 1024 (a team per column needs
+    y transforms of at least 512 points)."""
     cx, cy = size
     print('shapes', size, flush=True)
     p = _problem(rt, cx, cy, 0.1, 0.1, (3, 3))
@@ -165,12 +166,14 @@ def test_product_shapes_across_transform_lengths(rt, size):
     for prec, tol in (('c128', 1e-12), ('c64', 5e-6)):
         for bits in (3, 4):
             for group in (1, 2, 4, 8, 16):
-                for tile in (1, 4, 16):
+                for tile, teams, chunks in ((1, 1, 1), (4, 1, 1), (16, 1, 1), (1, 2, 1), (2, 2, 1), (1, 1, 2),
+                                            (1, 2, 3)):
                     try:
-                        op.setShape({'bits': bits, 'group': group, 'tile': tile}, nrhs=3, precision=prec)
+                        op.setShape({'bits': bits, 'group': group, 'tile': tile, 'teams': teams, 'chunks': chunks},
+                                    nrhs=3, precision=prec)
                     except rt.RooftopError:
                         continue   # does not fit this product
-                    assert rel(op.apply(x, precision=prec), ref) < tol, (prec, bits, group, tile)
+                    assert rel(op.apply(x, precision=prec), ref) < tol, (prec, bits, group, tile, teams, chunks)
 
 
 def test_plate_512_product_and_tuning(rt):
