@@ -258,15 +258,24 @@ __global__ void __launch_bounds__(kThreads) k_cgs_dot(const c128 *__restrict__ V
     double re[16], im[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) re[c] = im[c] = 0;
-    for (int64_t i = base + lane; i < end; i += 64) {
-        const c128 wi = wr[i];
+    // Four rows per lane per step (chunk is a multiple of 256), so that a column's four loads are in flight together.
+    for (int64_t i0 = base + lane; i0 < end; i0 += 256) {
+        c128 wi[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) wi[u] = i0 + 64 * u < end ? wr[i0 + 64 * u] : mk(0.0, 0.0);
 #pragma unroll
         for (int c = 0; c < 16; ++c) {
             const int k = q + 4 * c;
             if (k < cnt) {
-                const c128 v = basis_col(V, k, r, R, n)[i];
-                re[c] += v.x * wi.x + v.y * wi.y;
-                im[c] += v.x * wi.y - v.y * wi.x;
+                const c128 *col = basis_col(V, k, r, R, n);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (i0 + 64 * u < end) {
+                        const c128 v = col[i0 + 64 * u];
+                        re[c] += v.x * wi[u].x + v.y * wi[u].y;
+                        im[c] += v.x * wi[u].y - v.y * wi[u].x;
+                    }
+                }
             }
         }
     }

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(MAXT) pass_a_k(const c128 *__restrict__ in, cp
         cround<TC, P, P::K - 1, false, false, false>(
             t, tw, [&](int, int pp) { return lds[pp]; }, [&](int pos, int, cplx<TC> v) { dst[pos * ldj] = v; });
     } else {
-        for (int pos = t; pos < P::M; pos += P::T) dst[pos * ldj] = lds[pos + (pos >> P::log2E)];
+        for (int pos = t; pos < P::M; pos += P::T) dst[pos * ldj] = lds[P::pad(pos)];
     }
 }
 

@@ -21,7 +21,7 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
 #pragma unroll 1
     for (int u = 0; u < U; ++u) {
         const int base = (t + u * P::T) * R;   // the last round has stride 1: positions base + q
-        const int pb = base + (base >> P::log2E);
+        const int pb = P::pad(base);
         cplx<TC> X[R], Y[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
@@ -30,8 +30,8 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
                 X[q] = in_half && base + q < n0 ? w0[base + q] : mk<TC>(0, 0);
                 Y[q] = in_half && base + q < n1 ? w1[base + q] : mk<TC>(0, 0);
             } else {
-                X[q] = col0[pb + q];
-                Y[q] = col1[pb + q];
+                X[q] = col0[pb + P::pad(q)];
+                Y[q] = col1[pb + P::pad(q)];
             }
         }
         dft<TC, R, false, false>(X);
@@ -58,8 +58,8 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
                 if (base + q < n0) w0[base + q] = X[q];
                 if (base + q < n1) w1[base + q] = Y[q];
             } else {
-                col0[pb + q] = X[q];
-                col1[pb + q] = Y[q];
+                col0[pb + P::pad(q)] = X[q];
+                col1[pb + P::pad(q)] = Y[q];
             }
         }
     }
@@ -71,13 +71,13 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
                                                  const cplx<TC> *__restrict__ dy, ProdDims d,
                                                  const cplx<TC> *__restrict__ tw, int tile, int R, int nblocks,
                                                  const int *__restrict__ skip, int skip_stride) {
-    using P = CPlan<LOG2M, BITS>;
+    using P = BPlan<LOG2M, BITS, sizeof(cplx<TC>)>;
     const int lb = xcd_block(blockIdx.x, nblocks);
     if (lb < 0) return;
     const int r = lb % R, pt = lb / R;
     if (skip && skip[r * skip_stride]) return;
     extern __shared__ unsigned char smem_raw[];
-    constexpr int Mp = P::M + (P::M >> P::log2E);
+    constexpr int Mp = P::pad(P::M - 1) + 1;
     const int tt = threadIdx.x / P::T, t = threadIdx.x - tt * P::T;
     const int px = pt * tile + tt;
     cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + 2 * tt * Mp;
@@ -149,25 +149,25 @@ __device__ __forceinline__ void multiply_team(int t, cplx<TC> *mine, const cplx<
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int base = (t + u * P::T) * R;   // the last round has stride 1: positions base + q
-        const int pb = base + (base >> P::log2E);
+        const int pb = P::pad(base);
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[u][q] = mine[pb + q];
+        for (int q = 0; q < R; ++q) v[u][q] = mine[pb + P::pad(q)];
         dft<TC, R, false, false>(v[u]);
 #pragma unroll
-        for (int q = 0; q < R; ++q) mine[pb + q] = v[u][q];
+        for (int q = 0; q < R; ++q) mine[pb + P::pad(q)] = v[u][q];
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int base = (t + u * P::T) * R;
-        const int pb = base + (base >> P::log2E);
+        const int pb = P::pad(base);
         const cplx<TC> *sp = row + 3 * base;
         const cplx<TC> *dyp = dy + base;
 #pragma unroll
         for (int q0 = 0; q0 < R; q0 += CH) {
 #pragma unroll
             for (int q = q0; q < q0 + CH && q < R; ++q) {
-                const cplx<TC> o = other[pb + q];
+                const cplx<TC> o = other[pb + P::pad(q)];
                 const cplx<TC> self = sp[3 * q + C], el = sp[3 * q + 2];
                 const cplx<TC> dyv = dyp[q];
                 cplx<TC> &w = v[u][q];
@@ -187,9 +187,9 @@ __device__ __forceinline__ void multiply_team(int t, cplx<TC> *mine, const cplx<
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const int base = (t + u * P::T) * R;
-        const int pb = base + (base >> P::log2E);
+        const int pb = P::pad(base);
 #pragma unroll
-        for (int q = 0; q < R; ++q) mine[pb + q] = v[u][q];
+        for (int q = 0; q < R; ++q) mine[pb + P::pad(q)] = v[u][q];
     }
 }
 
@@ -229,14 +229,14 @@ __global__ void __launch_bounds__(MAXT) pass_b2_k(cplx<TC> *__restrict__ work, c
                                                   const cplx<TC> *__restrict__ dy, ProdDims d,
                                                   const cplx<TC> *__restrict__ tw, int tile, int R, int nblocks,
                                                   const int *__restrict__ skip, int skip_stride) {
-    using P = CPlan<LOG2M, BITS>;
+    using P = BPlan<LOG2M, BITS, sizeof(cplx<TC>)>;
     static_assert(P::K >= 2 && P::T % 64 == 0, "a team per component needs two rounds and whole waves");
     const int lb = xcd_block(blockIdx.x, nblocks);
     if (lb < 0) return;
     const int r = lb % R, pt = lb / R;
     if (skip && skip[r * skip_stride]) return;
     extern __shared__ unsigned char smem_raw[];
-    constexpr int Mp = P::M + (P::M >> P::log2E);
+    constexpr int Mp = P::pad(P::M - 1) + 1;
     const int tt = threadIdx.x / (2 * P::T);
     const int rem = threadIdx.x - tt * 2 * P::T;
     const int c = __builtin_amdgcn_readfirstlane(rem / P::T);   // the team's component, uniform in the wave

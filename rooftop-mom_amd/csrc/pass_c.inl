@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ wo
             });
         } else {
             for (int pos = t; pos < P::M; pos += P::T)
-                lds[pos + (pos >> P::log2E)] = live ? src[pos * ldj] : mk<TC>(0, 0);
+                lds[P::pad(pos)] = live ? src[pos * ldj] : mk<TC>(0, 0);
             __syncthreads();
         }
     }

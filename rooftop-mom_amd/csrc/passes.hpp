@@ -42,6 +42,28 @@ struct CPlan {
     static constexpr int log2E = m == 0 ? 0 : base_bits + (extra > 0 ? 1 : 0);
     static constexpr int E = 1 << log2E;
     static constexpr int T = M >> log2E;   // threads of a team
+    // LDS offset of position p: one element of padding after every E.
+    static constexpr int pad(int p) { return p + (p >> log2E); }
+};
+
+// LDS layout of pass B's columns: position p at p + c1 (p >> a1) + c2 (p >> a2). Chosen per plan and element size
+// from a model of the 64 LDS banks (8-byte accesses served 16 lanes at a time, 16-byte ones 8 at a time) over every
+// round's access pattern: one element of padding per 128 bytes keeps both the unit-stride rounds and the stride-R
+// last round conflict-free for most plans; 1024-point complex64 columns with radices up to 8 (rounds 8, 8, 4, 4) take
+// two per 128 bytes and one more per 256.
+struct BPad {
+    int a1, c1, a2, c2;
+};
+__host__ __device__ constexpr BPad b_pad_rule(int m, int bits, int es) {
+    if (m == 10 && bits == 3 && es == 8) return BPad{4, 2, 5, 1};
+    return BPad{es == 8 ? 4 : 3, 1, 0, 0};
+}
+__host__ __device__ constexpr int b_pad(const BPad &r, int p) { return p + r.c1 * (p >> r.a1) + r.c2 * (p >> r.a2); }
+
+template <int LOG2M, int BITS, int ES>
+struct BPlan : CPlan<LOG2M, BITS> {
+    static constexpr BPad rule = b_pad_rule(LOG2M, BITS, ES);
+    static constexpr int pad(int p) { return b_pad(rule, p); }
 };
 
 template <int I>
@@ -86,12 +108,12 @@ __device__ __forceinline__ void cround(int t, const cplx<T> *__restrict__ tw, Lo
         const int beta = t + u * P::T;
         const int n = beta & ((1 << ls) - 1);
         const int base = ((beta >> ls) << ll) + n;
-        const int pb = base + (base >> P::log2E);
+        const int pb = P::pad(base);   // base and the offsets o below have disjoint bits: pad(base + o) = pb + pad(o)
         cplx<T> v[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             const int o = q << ls;
-            const int po = o + (o >> P::log2E);
+            const int po = P::pad(o);
             v[q] = (!INV && HALF && R > 1 && q >= R / 2) ? mk<T>(0, 0) : load(base + o, pb + po);
         }
         if constexpr (!INV) {
@@ -113,7 +135,7 @@ __device__ __forceinline__ void cround(int t, const cplx<T> *__restrict__ tw, Lo
         for (int q = 0; q < R; ++q) {
             if (INV && HALF && R > 1 && q >= R / 2) continue;
             const int o = q << ls;
-            const int po = o + (o >> P::log2E);
+            const int po = P::pad(o);
             store(base + o, pb + po, v[q]);
         }
     }
@@ -176,8 +198,10 @@ inline int column_stride(const FftPlan &p, int group, int es) {
 inline size_t lds_ac_bytes(const FftPlan &p, int group, int es) {
     return static_cast<size_t>(column_stride(p, group, es)) * group * es;
 }
+// Padded length of a pass-B column (the BPlan layout).
+inline int b_padded_length(const FftPlan &p, int es) { return b_pad(b_pad_rule(p.log2M, p.bits, es), p.M - 1) + 1; }
 inline size_t lds_b_bytes(const FftPlan &p, int tile, int es) {
-    return static_cast<size_t>(2) * tile * padded_length(p) * es;
+    return static_cast<size_t>(2) * tile * b_padded_length(p, es) * es;
 }
 // Pass B in three launches (pass_b_split.inl) holds one column per block.
 inline size_t lds_b_split_bytes(const FftPlan &p, int es) { return static_cast<size_t>(padded_length(p)) * es; }
