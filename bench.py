@@ -120,17 +120,25 @@ def run_form(rt, ctx, op, pc, solver, d_b, d_x, n, steps, warmup):
     return times, iters, outers, residuals
 
 
-def plane_wave_batch(rt, problem, count, seed=0):
-    """count plane waves with (theta, phi) uniform in [0, 80] x [0, 360) degrees and TE or TM polarization drawn from
-    the seed; incidence from above the plate (z > 0)."""
+def plane_wave_angles(count, seed=0):
+    """(directions, polarizations), count x 3 each: (theta, phi) uniform in [0, 80] x [0, 360) degrees and TE or TM
+    polarization drawn from the seed; incidence from above the plate (z > 0)."""
     rng = np.random.default_rng(seed)
-    cols = []
+    dirs, pols = [], []
     for _ in range(count):
         th, ph = np.radians(rng.uniform(0, 80)), np.radians(rng.uniform(0, 360))
         d = -np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
         te = np.array([-np.sin(ph), np.cos(ph), 0.0])
         tm = np.array([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)])
-        cols.append(rt.Excitation.planeWave(problem, d, te if rng.random() < 0.5 else tm).vector())
+        dirs.append(d)
+        pols.append(te if rng.random() < 0.5 else tm)
+    return np.array(dirs), np.array(pols)
+
+
+def plane_wave_batch(rt, problem, count, seed=0):
+    """The plane waves of plane_wave_angles(count, seed) as a host N x count array (host formulas)."""
+    dirs, pols = plane_wave_angles(count, seed)
+    cols = [rt.Excitation.planeWave(problem, d, p).vector() for d, p in zip(dirs, pols)]
     return np.asfortranarray(np.stack(cols, axis=1), dtype=np.complex128)
 
 
@@ -347,11 +355,15 @@ def main():
                                                  'GBps': round(sum(pby) / (wall * 1e-3) / 1e9, 1),
                                                  'frac': round(sum(pby) / (wall * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                                                  'pass_ms': [round(v, 5) for v in pms]}
-        B = plane_wave_batch(rt, bproblem, R, seed=1)
-        d_B, d_X = C.c_void_p(), C.c_void_p()
-        for ptr in (d_B, d_X):
-            rt.check(lib.rt_device_alloc(ctx.handle, 16 * BN * R, C.byref(ptr)))
-        rt.check(lib.rt_copy_to_device(ctx.handle, d_B, B.ctypes.data_as(C.c_void_p), 16 * BN * R))
+        # The excitations are built on the device (rt_plane_waves_device), and the radar cross sections of the
+        # solutions are computed there too: no N x R transfer.
+        dirs, pols = plane_wave_angles(R, seed=1)
+        t0 = time.perf_counter()
+        b_batch = rt.plane_waves(bproblem, dirs, pols)
+        batch['excitation_build_ms'] = round(1e3 * (time.perf_counter() - t0), 3)
+        x_batch = rt.DeviceBatch(ctx, BN, R)
+        d_B, d_X = C.c_void_p(b_batch.ptr), C.c_void_p(x_batch.ptr)
+        rcs = {}
         bsteps, bwarm = max(1, args.steps // 10), 1
         forms = (('gmres mixed', mixed), ('gmres fp64', fp64),
                  ('cocg mixed', rt.Solver('cg', tolerance=1e-6, maxIterationCount=2000, precision='c64')),
@@ -362,6 +374,7 @@ def main():
                                  allow_unconverged=True)
             its = [q[0] for q in outs]
             conv = sum(q[3] for q in outs)
+            rcs[name] = rt.radar_cross_sections(bproblem, b_batch, x_batch)
             # Only the systems that reach the tolerance count as solves.
             batch[name] = {'solves_per_s': round(conv * bsteps / sum(bt), 3), 'converged': '%d of %d' % (conv, R),
                            'batch_p50_ms': round(1e3 * percentile(bt, 50), 3),
@@ -382,8 +395,12 @@ def main():
                                                           max(batch['%s fp64' % m]['solves_per_s'], 1e-9), 3)
         best = lambda kind: max(batch['%s %s' % (m, kind)]['solves_per_s'] for m in ('gmres', 'cocg'))
         batch['best mixed_vs_best_fp64'] = round(best('mixed') / max(best('fp64'), 1e-9), 3)
-        for ptr in (d_B, d_X):
-            rt.check(lib.rt_device_free(ctx.handle, ptr))
+        # Radar cross sections of the mixed forms against the FP64 GMRES ones (every system converged there).
+        for name in ('gmres mixed', 'cocg mixed', 'cocg fp64'):
+            batch[name]['rcs_max_rel_diff_vs_gmres_fp64'] = float(np.max(np.abs(rcs[name] - rcs['gmres fp64']) /
+                                                                         np.abs(rcs['gmres fp64'])))
+        b_batch.free()
+        x_batch.free()
 
     # Served stream: requests of random grids (32..1024), frequencies, blob masks and feeds; single-stream
     # throughput with one slot (requests back to back), all requests at once into 4 slots (saturation), then Poisson

@@ -235,6 +235,25 @@ int rt_server_stats(rt_server *server, int64_t *counts, uint64_t *bytes);
  * round(fill * count_x * count_y) faces are metal (seeded, portable; the masks of the served-stream workload). */
 int rt_blob_mask(int32_t count_x, int32_t count_y, uint64_t seed, double fill, uint8_t *mask);
 
+/* Batched right-hand sides and observables on the device. No counterpart in the reference, which builds one
+ * excitation and reads one result at a time on the host; they replace, for a batch already in HBM:
+ *   rt_plane_waves_device          generatePlaneWave.m:18-42 (Excitation.planeWave, Excitation.m:29-72) for `count`
+ *                                  incidences: directions and polarizations are host arrays [count][3] (normalised
+ *                                  here; rooftopMoM:excitation:zeroVector / notPerpendicular as the reference),
+ *                                  intensities host [count] or NULL (1); b the device batch, n x count complex128;
+ *   rt_radar_cross_sections_device computeRadarCrossSection.m:18-19: sigma[r] = (k Z0)^2 |b_r.' x_r|^2 / (4 pi E_r^4)
+ *                                  from the device batches b and x (n x count); sigma host [count];
+ *   rt_input_impedances_device     computeInputImpedance.m:17: Z_in[r] = V_r / (edge lengths . x_r(port)); port host
+ *                                  [port_len] distinct 0-based rooftop indices (rooftopMoM:excitation:invalidPort),
+ *                                  voltages host [count] or NULL (1 V), x the device batch; zin host [count][2]
+ *                                  (real, imaginary). */
+int rt_plane_waves_device(rt_problem *problem, int64_t count, const double *directions, const double *polarizations,
+                          const double *intensities, void *b);
+int rt_radar_cross_sections_device(rt_problem *problem, int64_t count, const void *b, const void *x,
+                                   const double *intensities, double *sigma);
+int rt_input_impedances_device(rt_problem *problem, int64_t count, const int64_t *port, int64_t port_len,
+                               const double *voltages, const void *x, double *zin);
+
 /* Device buffers owned by the caller through the library (counted in the context's budget). */
 int rt_device_alloc(rt_ctx *ctx, uint64_t bytes, void **ptr);
 int rt_device_free(rt_ctx *ctx, void *ptr);

@@ -338,6 +338,87 @@ class Excitation:
         return Excitation(problem, 'deltaGap', vector, {'port': port, 'voltage': voltage})
 
 
+class DeviceBatch:
+    """n x count complex128 in device memory (a column per right-hand side), owned through a Context."""
+
+    def __init__(self, context, n, count):
+        self.context, self.n, self.count = context, int(n), int(count)
+        p = C.c_void_p()
+        check(library().rt_device_alloc(context.handle, 16 * self.n * self.count, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, context, array):
+        a = np.asfortranarray(array, dtype=np.complex128)
+        if a.ndim == 1:
+            a = a[:, None]
+        out = cls(context, a.shape[0], a.shape[1])
+        check(library().rt_copy_to_device(context.handle, C.c_void_p(out.ptr), a.ctypes.data_as(C.c_void_p),
+                                          a.nbytes))
+        return out
+
+    def to_host(self):
+        a = np.empty((self.n, self.count), np.complex128, order='F')
+        check(library().rt_copy_to_host(self.context.handle, a.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                        a.nbytes))
+        return a
+
+    def free(self):
+        if getattr(self, 'ptr', None) and _abi._lib is not None:
+            _abi._lib.rt_device_free(self.context.handle, C.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def _rows3(v, count, name):
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float64), (count, 3)))
+    return a
+
+
+def plane_waves(problem, directions, polarizations, intensities=None, out=None):
+    """The plane waves of count incidences (directions, polarizations: count x 3) as a DeviceBatch, computed on the
+    device (rt_plane_waves_device; generatePlaneWave.m:18-42 per column)."""
+    d = np.atleast_2d(np.asarray(directions, dtype=np.float64))
+    count = d.shape[0]
+    d = _rows3(d, count, 'direction')
+    p = _rows3(polarizations, count, 'polarization')
+    e = None if intensities is None else np.ascontiguousarray(np.broadcast_to(np.asarray(intensities, float), count))
+    out = out or DeviceBatch(problem.context, problem.rooftopCount(), count)
+    dp = C.POINTER(C.c_double)
+    check(library().rt_plane_waves_device(problem.handle, count, d.ctypes.data_as(dp), p.ctypes.data_as(dp),
+                                          None if e is None else e.ctypes.data_as(dp), C.c_void_p(out.ptr)))
+    return out
+
+
+def radar_cross_sections(problem, b, x, intensities=None):
+    """sigma per column of the DeviceBatches b (plane waves) and x (their solutions), on the device
+    (rt_radar_cross_sections_device; computeRadarCrossSection.m:18-19)."""
+    count = b.count
+    e = None if intensities is None else np.ascontiguousarray(np.broadcast_to(np.asarray(intensities, float), count))
+    sigma = np.zeros(count)
+    dp = C.POINTER(C.c_double)
+    check(library().rt_radar_cross_sections_device(problem.handle, count, C.c_void_p(b.ptr), C.c_void_p(x.ptr),
+                                                   None if e is None else e.ctypes.data_as(dp),
+                                                   sigma.ctypes.data_as(dp)))
+    return sigma
+
+
+def input_impedances(problem, port, x, voltages=None):
+    """Z_in per column of the DeviceBatch x for one port (0-based rooftop indices), on the device
+    (rt_input_impedances_device; computeInputImpedance.m:17)."""
+    port = np.ascontiguousarray(port, dtype=np.int64)
+    count = x.count
+    v = None if voltages is None else np.ascontiguousarray(np.broadcast_to(np.asarray(voltages, float), count))
+    z = np.zeros(2 * count)
+    dp = C.POINTER(C.c_double)
+    check(library().rt_input_impedances_device(problem.handle, count, port.ctypes.data_as(C.POINTER(C.c_int64)),
+                                               port.size, None if v is None else v.ctypes.data_as(dp),
+                                               C.c_void_p(x.ptr), z.ctypes.data_as(dp)))
+    return z[0::2] + 1j * z[1::2]
+
+
 def edge_lengths(problem, idx):
     idx = np.asarray(idx)
     out = np.full(idx.shape, problem.mesh.sizeX)
@@ -522,6 +603,28 @@ class Solver:
                           norms[r, :min(cap, out.residual_norms_count)].copy(), out.residual_norm, out.time_s,
                           excitation, out.outer_count) for r, out in enumerate(outs)]
         return results[0] if b.ndim == 1 else results
+
+    def solveDevice(self, operator, b, preconditioner=None, x=None):
+        """Solves the columns of the DeviceBatch b together with inputs and outputs in HBM (rt_solve_device); returns
+        (x as a DeviceBatch, [Result without a host solution] per column)."""
+        n = operator.problem().rooftopCount()
+        if b.n != n:
+            raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
+                               'The operator does not match the %d unknowns of the right-hand side.' % b.n)
+        x = x or DeviceBatch(b.context, n, b.count)
+        cap = self.maxIterationCount * 2 + 64
+        norms = np.zeros((b.count, cap))
+        outs = (_abi.rt_output * b.count)()
+        for r in range(b.count):
+            outs[r].residual_norms_capacity = cap
+            outs[r].residual_norms = norms[r].ctypes.data_as(C.POINTER(C.c_double))
+        o = self.options()
+        check(library().rt_solve_device(b.context.handle, operator.handle,
+                                        preconditioner.handle if preconditioner is not None else None, C.byref(o),
+                                        C.c_void_p(b.ptr), n, b.count, None, C.c_void_p(x.ptr), outs))
+        return x, [Result(None, bool(out.converged), out.iteration_count,
+                          norms[r, :min(cap, out.residual_norms_count)].copy(), out.residual_norm, out.time_s, None,
+                          out.outer_count) for r, out in enumerate(outs)]
 
 
 # ---------------------------------------------------------------------------------------------------------------

@@ -75,6 +75,11 @@ SIGNATURES = [
     ('rt_device_free', C.c_int, [P, P]),
     ('rt_copy_to_device', C.c_int, [P, P, P, C.c_uint64]),
     ('rt_copy_to_host', C.c_int, [P, P, P, C.c_uint64]),
+    ('rt_plane_waves_device', C.c_int, [P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), P]),
+    ('rt_radar_cross_sections_device', C.c_int, [P, C.c_int64, P, P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ('rt_input_impedances_device', C.c_int, [P, C.c_int64, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_double),
+                                             P, C.POINTER(C.c_double)]),
     ('rt_last_error_id', C.c_char_p, []),
     ('rt_last_error_message', C.c_char_p, []),
     ('rt_version', C.c_char_p, []),

@@ -279,6 +279,39 @@ int rt_problem_count(rt_problem *p, int64_t *count) {
     });
 }
 
+int rt_plane_waves_device(rt_problem *p, int64_t count, const double *directions, const double *polarizations,
+                          const double *intensities, void *b) {
+    return guarded([&] {
+        need(p && directions && polarizations && b && count >= 1, RT_EINVAL, "rooftop:excitation",
+             "invalid plane-wave arguments");
+        Problem &q = *p->p;
+        q.ctx->activate();
+        plane_waves_device(q, count, directions, polarizations, intensities, static_cast<c128 *>(b), q.ctx->stream);
+    });
+}
+
+int rt_radar_cross_sections_device(rt_problem *p, int64_t count, const void *b, const void *x,
+                                   const double *intensities, double *sigma) {
+    return guarded([&] {
+        need(p && b && x && sigma && count >= 1, RT_EINVAL, "rooftop:excitation", "invalid observable arguments");
+        Problem &q = *p->p;
+        q.ctx->activate();
+        radar_cross_sections_device(q, count, static_cast<const c128 *>(b), static_cast<const c128 *>(x), intensities,
+                                    sigma, q.ctx->stream);
+    });
+}
+
+int rt_input_impedances_device(rt_problem *p, int64_t count, const int64_t *port, int64_t port_len,
+                               const double *voltages, const void *x, double *zin) {
+    return guarded([&] {
+        need(p && port && x && zin && count >= 1, RT_EINVAL, "rooftop:excitation", "invalid observable arguments");
+        Problem &q = *p->p;
+        q.ctx->activate();
+        input_impedances_device(q, count, port, port_len, voltages, static_cast<const c128 *>(x), zin,
+                                q.ctx->stream);
+    });
+}
+
 int rt_problem_generators(rt_problem *p, void *mxx, void *myy, void *e) {
     return guarded([&] {
         need(p != nullptr, RT_EINVAL, "rooftop:problem", "NULL problem");

@@ -24,7 +24,7 @@ static const double kZ0 = 376.730313668;   // +assembler/Constants.m:11
 // ------------------------------------------------------------------------------------------------------------------
 // Host: Gauss-Legendre rules (Newton on the Legendre recurrence in long double) and the reduced moment rule.
 // ------------------------------------------------------------------------------------------------------------------
-static void gauss_legendre(int n, std::vector<double> &pts, std::vector<double> &wts) {
+void gauss_legendre(int n, std::vector<double> &pts, std::vector<double> &wts) {
     pts.assign(n, 0.0);
     wts.assign(n, 0.0);
     const long double pi = 3.141592653589793238462643383279502884L;

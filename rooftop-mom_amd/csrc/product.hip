@@ -479,10 +479,14 @@ static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, con
         *wall = wall_ms / reps;
     }
     ctx->timing = was;
+    // Algorithmic bytes of one application, whatever its chunks (each chunk's launch of pass B reads the spectra
+    // again, which is traffic, not algorithmic bytes).
+    double app[3];
+    pass_bytes<TC>(op.dims, nrhs, app);
     for (int k = 0; k < 3; ++k) {
         // Per application: the chunks of one application add up.
         ms[k] = (ctx->timed_ms[slot0 + k] - ms0[k]) / reps;
-        bytes[k] = (ctx->timed_bytes[slot0 + k] - by0[k]) / reps;
+        bytes[k] = app[k];
         ctx->timed_launches[slot0 + k] = n0[k];
         ctx->timed_ms[slot0 + k] = ms0[k];
         ctx->timed_bytes[slot0 + k] = by0[k];

@@ -133,4 +133,15 @@ std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precisi
 void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s,
                           const int *skip, int skip_stride);
 
+// Gauss-Legendre rule of n points on [0, 1] (generator.hip).
+void gauss_legendre(int n, std::vector<double> &pts, std::vector<double> &wts);
+
+// Batched right-hand sides and observables on the device (excitation.hip).
+void plane_waves_device(Problem &p, int64_t count, const double *directions, const double *polarizations,
+                        const double *intensities, c128 *b, hipStream_t s);
+void radar_cross_sections_device(Problem &p, int64_t count, const c128 *b, const c128 *x, const double *intensities,
+                                 double *sigma, hipStream_t s);
+void input_impedances_device(Problem &p, int64_t count, const int64_t *port, int64_t port_len, const double *voltages,
+                             const c128 *x, double *zin, hipStream_t s);
+
 }  // namespace rt

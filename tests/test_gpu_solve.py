@@ -226,3 +226,57 @@ def test_batched_solve_with_chunked_products(rt, precision):
         y1 = op.apply(x, precision=prec)
         op.setShape(dict(base, chunks=chunks), nrhs=5, precision=prec)
         assert np.array_equal(op.apply(x, precision=prec), y1)
+
+def test_device_plane_waves_and_observables(rt):
+    """Plane waves of many incidences built on the device equal the oracle's (generatePlaneWave.m), masked and not;
+    RCS and input impedance computed on the device from a device-resident batch solve equal the host formulas."""
+    rng = np.random.default_rng(21)
+    cases = []
+    for q in range(6):
+        th, ph = rng.uniform(0, np.radians(85)), rng.uniform(0, 2 * np.pi)
+        d = -np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+        te = np.array([-np.sin(ph), np.cos(ph), 0.0])
+        cases.append((d, te * (1.5 if q % 2 else 1.0)))   # unnormalised polarization on odd columns
+    D = np.array([c[0] for c in cases])
+    Pz = np.array([c[1] for c in cases])
+    E = np.linspace(0.5, 2.0, len(cases))
+    for size, mask in (((13, 9, 0.1, 0.12), None), ((10, 8, 0.1, 0.1), 'blob')):
+        print('plane waves', size, mask, flush=True)
+        m = rt.blob_mask(size[0], size[1], 5, 0.6) if mask else None
+        p, grid, gen = _setup(rt, *size, (6, 5), mask=m)
+        batch = rt.plane_waves(p, D, Pz, E)
+        got = batch.to_host()
+        exist = rt.rooftop_existence(p)
+        for r, (d, e) in enumerate(cases):
+            ref = X.plane_wave(grid, 2 * np.pi, d, e / np.linalg.norm(e), E[r], order=5) * exist
+            assert rel(got[:, r], ref) < 1e-13, (size, r, rel(got[:, r], ref))
+            host = rt.Excitation.planeWave(p, d, e, E[r]).vector()
+            assert rel(got[:, r], host) < 1e-13
+        op = rt.Operator(p)
+        pc = rt.Preconditioner(p, 'nearField')
+        solver = rt.Solver('gmres', tolerance=1e-10, restart=30)
+        xb, res = solver.solveDevice(op, batch, preconditioner=pc)
+        assert all(q.converged() for q in res)
+        sol = xb.to_host()
+        sigma = rt.radar_cross_sections(p, batch, xb, E)
+        for r in range(len(cases)):
+            ref = X.radar_cross_section(2 * np.pi, got[:, r], sol[:, r], E[r])
+            assert abs(sigma[r] - ref) < 1e-12 * abs(ref), (r, sigma[r], ref)
+        if mask is None:
+            port = X.middle_port(grid, 'x')
+            V = np.array([1.0, 2.0, 0.5, 1.0, 1.0, 3.0])
+            zin = rt.input_impedances(p, port, xb, V)
+            for r in range(len(cases)):
+                ref = X.input_impedance(grid, sol[:, r], port, V[r])
+                assert abs(zin[r] - ref) < 1e-12 * abs(ref)
+    p, _, _ = _setup(rt, 5, 4, 0.1, 0.1, (3, 3))
+    with pytest.raises(rt.RooftopError) as e:
+        rt.plane_waves(p, [[0, 0, -1]], [[0, 0, 1]])
+    assert e.value.identifier == 'rooftopMoM:excitation:notPerpendicular'
+    with pytest.raises(rt.RooftopError) as e:
+        rt.plane_waves(p, [[0, 0, 0]], [[1, 0, 0]])
+    assert e.value.identifier == 'rooftopMoM:excitation:zeroVector'
+    xb = rt.DeviceBatch.from_host(p.context, np.ones((p.rooftopCount(), 2)))
+    with pytest.raises(rt.RooftopError) as e:
+        rt.input_impedances(p, [0, 0], xb)
+    assert e.value.identifier == 'rooftopMoM:excitation:invalidPort'
