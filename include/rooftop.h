@@ -53,6 +53,16 @@ extern "C" {
 #define RT_PRECOND_JACOBI 1
 #define RT_PRECOND_NEARFIELD 2
 
+/* Circulant kernels and builders (Preconditioner.m:31-48, +preconditioner/+kernels, generate*Spectra.m). */
+#define RT_CIRCULANT_STRANG 0
+#define RT_CIRCULANT_TCHAN 1
+#define RT_CIRCULANT_RCHAN 2
+#define RT_CIRCULANT_HUCKLE 3
+#define RT_CIRCULANT_SUPEROPTIMAL 4
+#define RT_BUILDER_PARTWISE 0
+#define RT_BUILDER_BLOCKWISE 1
+#define RT_BUILDER_HYBRID 2
+
 /* Methods (rooftopMoM.Solver 'method', Solver.m:27-32). */
 #define RT_METHOD_CG 0
 #define RT_METHOD_GMRES 1
@@ -127,6 +137,15 @@ int rt_product_create(rt_problem *problem, int32_t kind, int32_t precision, int3
 /* A preconditioner product (the role of rooftopMoM.Preconditioner, src/+rooftopMoM/Preconditioner.m:7-70);
  * RT_ESINGULAR mirrors rooftopMoM:preconditioner:singular. */
 int rt_precond_create(rt_problem *problem, int32_t kind, int32_t precision, rt_product **out);
+/* rooftopMoM.Preconditioner(problem, 'kernel', k, 'builder', b, 'bandwidth', [bx by], 'modifier', m)
+ * (src/+rooftopMoM/Preconditioner.m:7-70, +preconditioner/generateCirculant.m:16-24): the inverse of a block circulant
+ * approximation of the Green's matrix on the face grid, applied by FFTs of the face grid (FFTApplicator.m). On the GPU:
+ * the partwise builder (generatePartwiseSpectra.m:16-30) with the strang, tChan, rChan and huckle kernels, on face
+ * counts that are powers of two; other kernels, builders or grids return RT_EINVAL with
+ * rooftop:preconditioner:unsupported. bandwidth_x/y: Huckle's bandwidths in faces (<= 0: the grid's counts);
+ * RT_ESINGULAR with rooftopMoM:preconditioner:singular when the approximation is singular at some frequency. */
+int rt_precond_create_circulant(rt_problem *problem, int32_t kernel, int32_t builder, int32_t bandwidth_x,
+                                int32_t bandwidth_y, int32_t modifier, int32_t precision, rt_product **out);
 void rt_product_destroy(rt_product *product);
 /* Product.apply (src/+rooftopMoM/Product.m:25-35) on nrhs column-major host vectors of n rooftops each; n must be
  * the rooftop count (RT_ESIZE, rooftopMoM:product:sizeMismatch). Synchronous. precision selects the stored

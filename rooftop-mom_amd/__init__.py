@@ -259,19 +259,53 @@ class Operator(Product):
         return list(ms)[:3], list(by), ms[3]
 
 
+_KERNELS = {'strang': 0, 'tChan': 1, 'rChan': 2, 'huckle': 3, 'superoptimal': 4}
+_BUILDERS = {'partwise': 0, 'blockwise': 1, 'hybrid': 2}
+
+
 class Preconditioner(Product):
-    """A preconditioner product: 'nearField' (3x3-cell neighbourhoods) or 'jacobi' (diag(Z)).
+    """A preconditioner product: 'nearField' (3x3-cell neighbourhoods), 'jacobi' (diag(Z)) or 'circulant', the
+    reference's rooftopMoM.Preconditioner(problem, 'kernel', k, 'builder', b, 'bandwidth', [bx by], 'modifier', m)
+    (Preconditioner.m:7-70): the inverse of a block circulant approximation on the face grid. Passing kernel= (or
+    builder=, bandwidth=) selects 'circulant', whose defaults are the reference's ('rChan', 'partwise').
 
     Errors rooftopMoM:preconditioner:singular like Preconditioner.m:60-67.
     """
 
-    def __init__(self, problem, kind='nearField', precision='c128'):
-        if kind not in _PRECONDITIONERS:
-            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember',
-                               "'kind' must be 'nearField' or 'jacobi'.")
-        h = C.c_void_p()
-        check(library().rt_precond_create(problem.handle, _PRECONDITIONERS[kind], _PRECISIONS[precision],
-                                          C.byref(h)))
+    def __init__(self, problem, kind=None, precision='c128', kernel=None, builder=None, bandwidth=None,
+                 modifier=None):
+        if kind is None:
+            kind = 'circulant' if (kernel or builder or bandwidth is not None) else 'nearField'
+        if kind == 'circulant':
+            kernel, builder = kernel or 'rChan', builder or 'partwise'
+            if kernel not in _KERNELS:
+                raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember',
+                                   "'kernel' must be 'strang', 'tChan', 'rChan', 'huckle' or 'superoptimal'.")
+            if builder not in _BUILDERS:
+                raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember',
+                                   "'builder' must be 'partwise', 'blockwise' or 'hybrid'.")
+            bw = (0, 0)
+            if bandwidth is not None:
+                bw = tuple(np.atleast_1d(bandwidth))
+                if len(bw) != 2:
+                    raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:preconditioner:invalidBandwidth',
+                                       'The bandwidth is a pair [bandwidthX, bandwidthY].')
+                if any(int(v) != v or v <= 0 for v in bw):
+                    raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:preconditioner:invalidBandwidth',
+                                       'The bandwidth is a pair of positive integers.')
+            h = C.c_void_p()
+            check(library().rt_precond_create_circulant(problem.handle, _KERNELS[kernel], _BUILDERS[builder],
+                                                        int(bw[0]), int(bw[1]), _MODIFIERS[modifier],
+                                                        _PRECISIONS[precision], C.byref(h)))
+            self.kernel, self.builder = kernel, builder
+        else:
+            if kind not in _PRECONDITIONERS:
+                raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember',
+                                   "'kind' must be 'nearField', 'jacobi' or 'circulant'.")
+            h = C.c_void_p()
+            check(library().rt_precond_create(problem.handle, _PRECONDITIONERS[kind], _PRECISIONS[precision],
+                                              C.byref(h)))
+        self.kind = kind
         super().__init__(problem, h, _PRECISIONS[precision])
 
 

@@ -75,6 +75,8 @@ SIGNATURES = [
     ('rt_device_free', C.c_int, [P, P]),
     ('rt_copy_to_device', C.c_int, [P, P, P, C.c_uint64]),
     ('rt_copy_to_host', C.c_int, [P, P, P, C.c_uint64]),
+    ('rt_precond_create_circulant', C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                              C.POINTER(P)]),
     ('rt_plane_waves_device', C.c_int, [P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), P]),
     ('rt_radar_cross_sections_device', C.c_int, [P, C.c_int64, P, P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -363,6 +363,28 @@ int rt_product_create(rt_problem *p, int32_t kind, int32_t precision, int32_t mo
     });
 }
 
+int rt_precond_create_circulant(rt_problem *p, int32_t kernel, int32_t builder, int32_t bandwidth_x,
+                                int32_t bandwidth_y, int32_t modifier, int32_t precision, rt_product **out) {
+    return guarded([&] {
+        need(p && out, RT_EINVAL, "rooftop:preconditioner", "NULL argument");
+        need(kernel >= RT_CIRCULANT_STRANG && kernel <= RT_CIRCULANT_SUPEROPTIMAL, RT_EINVAL,
+             "MATLAB:validators:mustBeMember", "The kernel is 'strang', 'tChan', 'rChan', 'huckle' or 'superoptimal'.");
+        need(builder >= RT_BUILDER_PARTWISE && builder <= RT_BUILDER_HYBRID, RT_EINVAL,
+             "MATLAB:validators:mustBeMember", "The builder is 'partwise', 'blockwise' or 'hybrid'.");
+        need(modifier >= RT_MODIFIER_NONE && modifier <= RT_MODIFIER_IMAG, RT_EINVAL, "rooftop:product",
+             "The modifier is none, real or imag.");
+        need(precision == RT_C128 || precision == RT_C64 || precision == RT_MIXED, RT_EINVAL,
+             "rooftop:product:invalidPrecision", "The precision is RT_C128, RT_C64 or RT_MIXED.");
+        p->p->ctx->activate();
+        const GridInfo &g = p->p->g;
+        auto h = std::make_unique<rt_product>();
+        h->owner = p->p;
+        h->p = create_circulant(p->p.get(), kernel, builder, bandwidth_x > 0 ? bandwidth_x : g.cx,
+                                bandwidth_y > 0 ? bandwidth_y : g.cy, modifier, precision);
+        *out = h.release();
+    });
+}
+
 int rt_precond_create(rt_problem *p, int32_t kind, int32_t precision, rt_product **out) {
     return guarded([&] {
         need(p && out, RT_EINVAL, "rooftop:preconditioner", "NULL argument");

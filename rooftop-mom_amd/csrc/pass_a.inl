@@ -4,7 +4,9 @@
 
 namespace rt {
 
-template <typename TC, int LOG2M, int BITS, int MAXT>
+// EMB: the rows are the lower half of an embedding (the upper half of the period zero), as in the Toeplitz product;
+// false for rows that fill the period (a circulant on the face grid).
+template <typename TC, int LOG2M, int BITS, int MAXT, bool EMB = true>
 __global__ void __launch_bounds__(MAXT) pass_a_k(const c128 *__restrict__ in, cplx<TC> *__restrict__ work,
                                                  const uint8_t *__restrict__ exists, ProdDims d,
                                                  const cplx<TC> *__restrict__ tw, int G, int CS, int nblocks,
@@ -28,7 +30,7 @@ __global__ void __launch_bounds__(MAXT) pass_a_k(const c128 *__restrict__ in, cp
         const c128 *src = in + off + col;
         const uint8_t *exc = exists ? exists + (comp ? d.nx_count : 0) + col : nullptr;
         cplx<TC> *lds = buf + c * CS;
-        cround<TC, P, 0, false, true, (P::K > 1)>(
+        cround<TC, P, 0, false, EMB, (P::K > 1)>(
             t, tw,
             [&](int pos, int) {
                 if (!live || pos >= rows || (exc && !exc[pos])) return mk<TC>(0, 0);
@@ -67,12 +69,22 @@ void launch_pass_a(const Product &op, const PlanTables &t, const c128 *in, cplx<
     const int threads = team_threads(p) * g;
     const cplx<TC> *tw = sizeof(TC) == sizeof(double) ? reinterpret_cast<const cplx<TC> *>(t.tw.ptr)
                                                       : reinterpret_cast<const cplx<TC> *>(t.tw32.ptr);
+    if (!op.embedding && p.bits != 3) fail(RT_EINVAL, "rooftop:product", "a face-grid product runs the radix-8 plan");
 #define RT_LAUNCH_A(L, B)                                                                                   \
     {                                                                                                       \
         constexpr int MT = (es == 16 && CPlan<L, B>::log2E >= 4) ? 512 : 1024;                              \
-        set_lds(reinterpret_cast<const void *>(&pass_a_k<TC, L, B, MT>), lds);                              \
-        launch_timed(timer, pass_a_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads), static_cast<uint32_t>(lds), s, in, work,   \
-                           op.problem->exists.ptr, op.dims, tw, g, cs, static_cast<int>(nb), skip, skip_stride); \
+        constexpr int MF = (es == 16 && CPlan<L, 3>::log2E >= 4) ? 512 : 1024;                              \
+        if (op.embedding) {                                                                                 \
+            set_lds(reinterpret_cast<const void *>(&pass_a_k<TC, L, B, MT>), lds);                          \
+            launch_timed(timer, pass_a_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads),                  \
+                         static_cast<uint32_t>(lds), s, in, work, op.problem->exists.ptr, op.dims, tw, g, cs, \
+                         static_cast<int>(nb), skip, skip_stride);                                          \
+        } else {                                                                                            \
+            set_lds(reinterpret_cast<const void *>(&pass_a_k<TC, L, 3, MF, false>), lds);                   \
+            launch_timed(timer, pass_a_k<TC, L, 3, MF, false>, dim3(xcd_grid(nb)), dim3(threads),           \
+                         static_cast<uint32_t>(lds), s, in, work, op.problem->exists.ptr, op.dims, tw, g, cs, \
+                         static_cast<int>(nb), skip, skip_stride);                                          \
+        }                                                                                                   \
     }
     RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_A);
 #undef RT_LAUNCH_A

@@ -10,7 +10,7 @@ namespace rt {
 // Spectra: row = bspec + fold(kx) My 3, position py holds (Mxx, Myy, E) at row[3 py ..]; dy in position order.
 // The multiply runs in chunks of CH positions behind scheduling barriers, which bounds the spectrum loads in flight
 // (and the registers they hold).
-template <typename TC, class P, bool DIRECT>
+template <typename TC, class P, bool DIRECT, bool EMB>
 __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *col1, cplx<TC> *__restrict__ w0,
                                                cplx<TC> *__restrict__ w1, int n0, int n1,
                                                const cplx<TC> *__restrict__ row, cplx<TC> dxv,
@@ -26,7 +26,7 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             if constexpr (DIRECT) {
-                const bool in_half = R == 1 || q < R / 2;
+                const bool in_half = !EMB || R == 1 || q < R / 2;   // an embedding's upper half is zero
                 X[q] = in_half && base + q < n0 ? w0[base + q] : mk<TC>(0, 0);
                 Y[q] = in_half && base + q < n1 ? w1[base + q] : mk<TC>(0, 0);
             } else {
@@ -65,7 +65,9 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
     }
 }
 
-template <typename TC, int LOG2M, int BITS, int MAXT>
+// EMB: the columns are the lower halves of an embedding (the upper half of every input zero, only the lower half of
+// every output kept), as in the Toeplitz product; false for full-period columns (a circulant on the face grid).
+template <typename TC, int LOG2M, int BITS, int MAXT, bool EMB = true>
 __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ bspec,
                                                  const int *__restrict__ fold_x, const cplx<TC> *__restrict__ dx,
                                                  const cplx<TC> *__restrict__ dy, ProdDims d,
@@ -88,13 +90,13 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
     const cplx<TC> *row = bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
     const cplx<TC> dxv = dx[px];
     if constexpr (P::K == 1) {   // one round: straight from the work array and back
-        multiply_round<TC, P, true>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
+        multiply_round<TC, P, true, EMB>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
     } else {
         // Forward round 0 from the work array (the upper half of the embedding is zero).
-        cround<TC, P, 0, false, true, true>(
+        cround<TC, P, 0, false, EMB, true>(
             t, tw, [&](int pos, int) { return pos < n0 ? w0[pos] : mk<TC>(0, 0); },
             [&](int, int pp, cplx<TC> v) { col0[pp] = v; });
-        cround<TC, P, 0, false, true, true>(
+        cround<TC, P, 0, false, EMB, true>(
             t, tw, [&](int pos, int) { return pos < n1 ? w1[pos] : mk<TC>(0, 0); },
             [&](int, int pp, cplx<TC> v) { col1[pp] = v; });
         __syncthreads();
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
                 t, tw, [&](int, int pp) { return col1[pp]; }, [&](int, int pp, cplx<TC> v) { col1[pp] = v; });
             __syncthreads();
         });
-        multiply_round<TC, P, false>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
+        multiply_round<TC, P, false, EMB>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
         __syncthreads();
         tw = opaque(tw);
         static_for_down<1, P::K - 1>([&](auto S) {
@@ -118,12 +120,12 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
             __syncthreads();
         });
         // Inverse round 0 into the work array, cut to the columns (the upper half of the period is never kept).
-        cround<TC, P, 0, true, true, true>(
+        cround<TC, P, 0, true, EMB, true>(
             t, tw, [&](int, int pp) { return col0[pp]; },
             [&](int pos, int, cplx<TC> v) {
                 if (pos < n0) w0[pos] = v;
             });
-        cround<TC, P, 0, true, true, true>(
+        cround<TC, P, 0, true, EMB, true>(
             t, tw, [&](int, int pp) { return col1[pp]; },
             [&](int pos, int, cplx<TC> v) {
                 if (pos < n1) w1[pos] = v;
@@ -269,10 +271,19 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
     const cplx<TC> *dy = dbl ? reinterpret_cast<const cplx<TC> *>(ty.d.ptr)
                              : reinterpret_cast<const cplx<TC> *>(ty.d32.ptr);
     if (!b_threads_fit(p, tile, teams)) fail(RT_EINVAL, "rooftop:product", "pass B shape does not fit this plan");
+    if (!op.embedding && (p.bits != 3 || teams != 1))
+        fail(RT_EINVAL, "rooftop:product", "a face-grid product runs the radix-8 plan with one team per position");
 #define RT_LAUNCH_B(L, B)                                                                                          \
     {                                                                                                              \
         using PL = CPlan<L, B>;                                                                                    \
-        if (teams == 1) {                                                                                          \
+        if (!op.embedding) {                                                                                       \
+            using PF = CPlan<L, 3>;                                                                                \
+            constexpr int MT = PF::T > 256 ? PF::T : 256;                                                          \
+            set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, 3, MT, false>), lds);                          \
+            launch_timed(timer, pass_b_k<TC, L, 3, MT, false>, dim3(xcd_grid(nb)), dim3(threads),                  \
+                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, \
+                         static_cast<int>(nb), skip, skip_stride);                                                 \
+        } else if (teams == 1) {                                                                                   \
             constexpr int MT = PL::T > 256 ? PL::T : 256;                                                          \
             set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, B, MT>), lds);                                 \
             launch_timed(timer, pass_b_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads),                         \

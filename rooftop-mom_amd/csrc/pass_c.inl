@@ -4,7 +4,8 @@
 
 namespace rt {
 
-template <typename TC, int LOG2M, int BITS, int MAXT>
+// EMB: only the lower half of every column is kept (the cut of an embedding); false keeps the full period.
+template <typename TC, int LOG2M, int BITS, int MAXT, bool EMB = true>
 __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ work, c128 *__restrict__ out,
                                                  const uint8_t *__restrict__ exists, ProdDims d,
                                                  const cplx<TC> *__restrict__ tw, int G, int CS, int nblocks,
@@ -51,7 +52,7 @@ __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ wo
     c128 *dst = out + off + col;
     const uint8_t *exc = exists ? exists + (comp ? d.nx_count : 0) + col : nullptr;
     const cplx<TC> *lds = buf + c * CS;
-    cround<TC, P, 0, true, (P::M >= 2), (P::K > 1)>(
+    cround<TC, P, 0, true, EMB && (P::M >= 2), (P::K > 1)>(
         t, tw, [&](int, int pp) { return lds[pp]; },
         [&](int pos, int, cplx<TC> v) {
             if (pos >= rows) return;
@@ -71,12 +72,22 @@ void launch_pass_c(const Product &op, const PlanTables &t, const cplx<TC> *work,
     const int threads = team_threads(p) * g;
     const cplx<TC> *tw = sizeof(TC) == sizeof(double) ? reinterpret_cast<const cplx<TC> *>(t.tw.ptr)
                                                       : reinterpret_cast<const cplx<TC> *>(t.tw32.ptr);
+    if (!op.embedding && p.bits != 3) fail(RT_EINVAL, "rooftop:product", "a face-grid product runs the radix-8 plan");
 #define RT_LAUNCH_C(L, B)                                                                                   \
     {                                                                                                       \
         constexpr int MT = (es == 16 && CPlan<L, B>::log2E >= 4) ? 512 : 1024;                              \
-        set_lds(reinterpret_cast<const void *>(&pass_c_k<TC, L, B, MT>), lds);                              \
-        launch_timed(timer, pass_c_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads), static_cast<uint32_t>(lds), s, work, out,  \
-                           op.problem->exists.ptr, op.dims, tw, g, cs, static_cast<int>(nb), skip, skip_stride); \
+        constexpr int MF = (es == 16 && CPlan<L, 3>::log2E >= 4) ? 512 : 1024;                              \
+        if (op.embedding) {                                                                                 \
+            set_lds(reinterpret_cast<const void *>(&pass_c_k<TC, L, B, MT>), lds);                          \
+            launch_timed(timer, pass_c_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads),                  \
+                         static_cast<uint32_t>(lds), s, work, out, op.problem->exists.ptr, op.dims, tw, g, cs, \
+                         static_cast<int>(nb), skip, skip_stride);                                          \
+        } else {                                                                                            \
+            set_lds(reinterpret_cast<const void *>(&pass_c_k<TC, L, 3, MF, false>), lds);                   \
+            launch_timed(timer, pass_c_k<TC, L, 3, MF, false>, dim3(xcd_grid(nb)), dim3(threads),           \
+                         static_cast<uint32_t>(lds), s, work, out, op.problem->exists.ptr, op.dims, tw, g, cs, \
+                         static_cast<int>(nb), skip, skip_stride);                                          \
+        }                                                                                                   \
     }
     RT_BY_PLAN(p.log2M, p.bits, RT_LAUNCH_C);
 #undef RT_LAUNCH_C

@@ -165,6 +165,7 @@ static PassShape default_shape(const Product &op, int nrhs, int bits = 0) {
     const ProdDims &d = op.dims;
     PassShape s;
     s.bits = bits ? bits : sizeof(TC) == sizeof(float) ? 4 : 3;
+    if (!op.embedding) s.bits = 3;   // the face-grid passes are built for the radix-8 plans
     const FftPlan &px = op.tx.of(s.bits).plan, &py = op.ty.of(s.bits).plan;
     const int es = sizeof(cplx<TC>);
     int g = std::max(1, 64 / es);
@@ -440,6 +441,197 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
 }
 
 // ------------------------------------------------------------------------------------------------------------------
+// Circulant preconditioners (Preconditioner.m:7-70, +preconditioner/generateCirculant.m:16-24 with the partwise
+// builder generatePartwiseSpectra.m:16-30): every generator, zero-padded to the countX x countY face grid, is folded
+// onto that period by the window of a kernel (foldGenerator.m:27-61: Strang, T. Chan, R. Chan, Huckle); the block
+// spectra of the three circulants follow as for the Green's matrix (generateBlockSpectra.m) and are inverted frequency
+// by frequency (@Spectra/invert.m); the product with the inverse runs the matvec passes on the face grid.
+// ------------------------------------------------------------------------------------------------------------------
+// Weight of the offset o of a period P (the windows of +kernels/generate*.m).
+__device__ double circulant_window(int o, int P, int kind, double bandwidth) {
+    const int a = o < 0 ? -o : o;
+    switch (kind) {
+    case RT_CIRCULANT_STRANG: return 2 * a < P ? 1.0 : (2 * a == P ? 0.5 : 0.0);
+    case RT_CIRCULANT_TCHAN: return static_cast<double>(P - a) / P;
+    case RT_CIRCULANT_RCHAN: return 1.0;
+    default: return fmax(0.0, 1.0 - a / bandwidth);   // Huckle
+    }
+}
+
+// c(kx, ky) for kx <= P/2, ky <= Q/2 (the rest follows from the symmetry c(k) = c(P - k) of the fold of an even
+// generator): the sum over the offsets ox in {kx, kx - P} (only 0 for kx = 0), oy likewise, of
+// w(ox) w(oy) g(|ox|, |oy|); g is the generator (rows x cols, column-major) zero-padded to P x Q. Output [Hy][Hx].
+__global__ void circulant_fold(const c128 *__restrict__ g, int rows, int cols, int P, int Q, int kind, double bwx,
+                               double bwy, int modifier, c128 *__restrict__ out, int Hx, int Hy) {
+    const int64_t total = static_cast<int64_t>(Hx) * Hy;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int kx = static_cast<int>(i % Hx), ky = static_cast<int>(i / Hx);
+        c128 sum = mk(0.0, 0.0);
+        for (int sx = 0; sx < (kx == 0 ? 1 : 2); ++sx) {
+            const int ox = sx ? kx - P : kx;
+            const double wx = circulant_window(ox, P, kind, bwx);
+            for (int sy = 0; sy < (ky == 0 ? 1 : 2); ++sy) {
+                const int oy = sy ? ky - Q : ky;
+                const double w = wx * circulant_window(oy, Q, kind, bwy);
+                const int px = ox < 0 ? -ox : ox, py = oy < 0 ? -oy : oy;
+                if (w == 0 || px >= rows || py >= cols) continue;
+                c128 v = g[static_cast<int64_t>(py) * rows + px];
+                if (modifier == RT_MODIFIER_REAL) v.y = 0;
+                if (modifier == RT_MODIFIER_IMAG) v = mk(v.y, 0.0);
+                sum = sum + w * v;
+            }
+        }
+        out[i] = sum;
+    }
+}
+
+// In place on the folded planes [3][Hx][Hy] (the circulants' spectra A, B, E): the inverse of the 2x2 block
+// [XX, XY; YX, YY] (XX = A - |Dx|^2 E, YY = B - |Dy|^2 E, XY = -Dx conj(Dy) E, YX = -conj(Dx) Dy E) written in the
+// same three-plane form as the product, (A', B', E') = ((YY - |Dx|^2 E) / det, (XX - |Dy|^2 E) / det, -E / det), so
+// that the product's passes apply it; 1 / XX (1 / YY) alone for a strip. Scaled by 1/(Mx My) for the inverse FFT.
+// A zero or non-finite determinant raises *singular (the reference's allfinite test of the inverse).
+__device__ __forceinline__ c128 reciprocal(c128 z) {
+    const double m2 = z.x * z.x + z.y * z.y;
+    return mk(z.x / m2, -z.y / m2);
+}
+__global__ void circulant_invert(c128 *__restrict__ planes, int Hx, int Hy, int Mx, int My, double a, double b,
+                                 int has_x, int has_y, double scale, int *__restrict__ singular) {
+    const int64_t plane = static_cast<int64_t>(Hx) * Hy;
+    const double pi = 3.141592653589793;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < plane;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int fx = static_cast<int>(i / Hy), fy = static_cast<int>(i - static_cast<int64_t>(fx) * Hy);
+        const double sx = 2 * sin(pi * fx / Mx) / a, sy = 2 * sin(pi * fy / My) / b;
+        const double dx2 = sx * sx, dy2 = sy * sy;
+        const c128 A = planes[i], B = planes[plane + i], E = planes[2 * plane + i];
+        c128 a2 = mk(0.0, 0.0), b2 = mk(0.0, 0.0), e2 = mk(0.0, 0.0), det;
+        if (has_x && has_y) {
+            const c128 xx = A - dx2 * E, yy = B - dy2 * E;
+            det = xx * yy - (dx2 * dy2) * (E * E);
+            const c128 inv = reciprocal(det);
+            a2 = (yy - dx2 * E) * inv;
+            b2 = (xx - dy2 * E) * inv;
+            e2 = mk(0.0, 0.0) - E * inv;
+        } else if (has_x) {
+            det = A - dx2 * E;
+            a2 = reciprocal(det);
+        } else {
+            det = B - dy2 * E;
+            b2 = reciprocal(det);
+        }
+        const bool ok = (det.x != 0 || det.y != 0) && isfinite(a2.x) && isfinite(a2.y) && isfinite(b2.x) &&
+                        isfinite(b2.y) && isfinite(e2.x) && isfinite(e2.y);
+        if (!ok) atomicExch(singular, 1);
+        planes[i] = scale * a2;
+        planes[plane + i] = scale * b2;
+        planes[2 * plane + i] = scale * e2;
+    }
+}
+
+std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, int bandwidth_x, int bandwidth_y,
+                                          int modifier, int precision) {
+    Context *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const GridInfo &g = p->g;
+    if (builder != RT_BUILDER_PARTWISE || kernel < RT_CIRCULANT_STRANG || kernel > RT_CIRCULANT_HUCKLE)
+        fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
+             "The GPU circulant preconditioner takes the partwise builder with the strang, tChan, rChan or huckle "
+             "kernel.");
+    auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
+    if (!pow2(g.cx) || !pow2(g.cy))
+        fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
+             "The GPU circulant preconditioner needs face counts that are powers of two (the grid is %d x %d).", g.cx,
+             g.cy);
+    if (kernel == RT_CIRCULANT_HUCKLE && (bandwidth_x < 1 || bandwidth_y < 1))
+        fail(RT_EINVAL, "rooftopMoM:preconditioner:invalidBandwidth", "The bandwidth is a pair of positive integers.");
+    auto pc = std::make_unique<Product>();
+    pc->type = kCirculant;
+    pc->embedding = false;
+    pc->problem = p;
+    pc->precision = precision;
+    pc->modifier = modifier;
+    ProdDims &d = pc->dims;
+    d.rows_x = g.rows_x;
+    d.rows_y = g.rows_y;
+    d.cols_x = g.has_x ? g.cols_x : 0;
+    d.cols_y = g.has_y ? g.cols_y : 0;
+    d.ny = d.cols_x > d.cols_y ? d.cols_x : d.cols_y;
+    d.ldj = (d.ny + 7) / 8 * 8;
+    d.nx_count = g.nx;
+    d.n = g.n;
+    d.Mx = g.cx;
+    d.My = g.cy;
+    d.Hx = d.Mx / 2 + 1;
+    d.Hy = d.My / 2 + 1;
+    const uint64_t plane_elems = static_cast<uint64_t>(d.Hx) * d.Hy;
+    ctx->require(static_cast<uint64_t>(d.Mx) * d.My * sizeof(c128), "The circulant preconditioner (transform buffer)");
+    p->ensure_generators();
+    pc->tx.build(ctx, d.Mx, g.a, s);
+    pc->ty.build(ctx, d.My, g.b, s);
+    pc->planes64.allocate(ctx, 3 * plane_elems, "The circulant preconditioner");
+    pc->planes64.zero(s);
+    {
+        const PlanTables &tx = pc->tx.of(3), &ty = pc->ty.of(3);
+        DeviceArray<c128> tmp, folded;
+        tmp.allocate(ctx, static_cast<size_t>(d.Mx) * d.My, "The circulant preconditioner (transform buffer)");
+        folded.allocate(ctx, plane_elems, "The circulant preconditioner (fold)");
+        std::vector<int> fx = output_frequencies(tx.plan), fy = output_frequencies(ty.plan);
+        DeviceArray<int> dfx, dfy;
+        dfx.allocate(ctx, d.Mx, "FFT frequency table");
+        dfy.allocate(ctx, d.My, "FFT frequency table");
+        RT_HIP(hipMemcpyAsync(dfx.ptr, fx.data(), d.Mx * sizeof(int), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(dfy.ptr, fy.data(), d.My * sizeof(int), hipMemcpyHostToDevice, s));
+        const c128 *gens[3] = {g.has_x ? p->mxx.ptr : nullptr, g.has_y ? p->myy.ptr : nullptr, p->e.ptr};
+        const int rows[3] = {g.cx - 1, g.cx, g.cx};
+        const int cols[3] = {g.cy, g.cy - 1, g.cy};
+        const size_t la = static_cast<size_t>(padded_length(tx.plan)) * sizeof(c128);
+        const size_t lb = static_cast<size_t>(padded_length(ty.plan)) * sizeof(c128);
+        const int ex = 1 << tx.plan.log2E, ey = 1 << ty.plan.log2E;
+        for (int k = 0; k < 3; ++k) {
+            if (!gens[k]) continue;
+            hipLaunchKernelGGL(circulant_fold, dim3(blocks_for(plane_elems, 256, 4096)), dim3(256), 0, s, gens[k],
+                               rows[k], cols[k], d.Mx, d.My, kernel, static_cast<double>(bandwidth_x),
+                               static_cast<double>(bandwidth_y), modifier, folded.ptr, d.Hx, d.Hy);
+            RT_LAUNCH_CHECK();
+            // The fold is even on the period, so the product's spectrum kernels transform it as the even extension
+            // of its half [0, P/2] x [0, Q/2].
+            RT_BY_E(ex, ({
+                        set_lds(reinterpret_cast<const void *>(&spectra_a<E>), la);
+                        hipLaunchKernelGGL((spectra_a<E>), dim3(d.My), dim3(std::max(64, team_threads(tx.plan))), la,
+                                           s, folded.ptr, d.Hx, d.Hy, RT_MODIFIER_NONE, tmp.ptr, d.Mx, d.My, tx.plan,
+                                           tx.tw.ptr);
+                    }));
+            RT_LAUNCH_CHECK();
+            RT_BY_E(ey, ({
+                        set_lds(reinterpret_cast<const void *>(&spectra_b<E>), lb);
+                        hipLaunchKernelGGL((spectra_b<E>), dim3(d.Mx), dim3(std::max(64, team_threads(ty.plan))), lb,
+                                           s, tmp.ptr, pc->planes64.ptr + k * plane_elems, d.Mx, d.My, d.Hy, ty.plan,
+                                           ty.tw.ptr, dfx.ptr, dfy.ptr, 1.0);
+                    }));
+            RT_LAUNCH_CHECK();
+        }
+        DeviceArray<int> flag;
+        flag.allocate(ctx, 1, "singularity flag");
+        flag.zero(s);
+        hipLaunchKernelGGL(circulant_invert, dim3(blocks_for(plane_elems, 256, 4096)), dim3(256), 0, s,
+                           pc->planes64.ptr, d.Hx, d.Hy, d.Mx, d.My, g.a, g.b, g.has_x ? 1 : 0, g.has_y ? 1 : 0,
+                           1.0 / (static_cast<double>(d.Mx) * d.My), flag.ptr);
+        RT_LAUNCH_CHECK();
+        int singular = 0;
+        RT_HIP(hipMemcpyAsync(&singular, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+        RT_HIP(hipStreamSynchronize(s));
+        if (singular)
+            fail(RT_ESINGULAR, "rooftopMoM:preconditioner:singular",
+                 "The circulant approximation is singular at some frequency. Use another kernel or builder.");
+    }
+    if (pc->has(RT_C128)) bspec_of<double>(*pc, 3, s);
+    if (pc->has(RT_C64)) bspec_of<float>(*pc, 3, s);
+    RT_HIP(hipStreamSynchronize(s));
+    return pc;
+}
+
+// ------------------------------------------------------------------------------------------------------------------
 // Measurement and tuning.
 // ------------------------------------------------------------------------------------------------------------------
 // Average per-launch milliseconds and algorithmic bytes of each pass over `reps` applications with shape sh (one
@@ -636,13 +828,13 @@ uint64_t Product::stored_bytes() const {
 }
 
 uint64_t Product::working_bytes() const {
-    if (type != kOperator) return 0;
+    if (type != kOperator && type != kCirculant) return 0;
     const uint64_t per = static_cast<uint64_t>(2) * dims.ldj * dims.Mx;
     return per * (has(RT_C128) ? sizeof(c128) : sizeof(c64));
 }
 
 void Product::apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip, int skip_stride) {
-    if (type == kOperator)
+    if (type == kOperator || type == kCirculant)
         operator_apply(*this, in, out, nrhs, prec, s, skip, skip_stride);
     else
         preconditioner_apply(*this, in, out, nrhs, prec, s, skip, skip_stride);

@@ -74,13 +74,16 @@ struct PassShape {
     int chunks = 1;  // right-hand-side chunks, alternating between the main and the side stream
 };
 
-enum ProductType { kOperator = 0, kJacobi = 1, kNearField = 2 };
+enum ProductType { kOperator = 0, kJacobi = 1, kNearField = 2, kCirculant = 3 };
 
 struct Product {
     int type = kOperator;
     Problem *problem = nullptr;
     int precision = RT_C128;
     int modifier = RT_MODIFIER_NONE;
+    // The FFT passes run on an even embedding of period >= 2n - 1 (the Toeplitz product: half of every column is
+    // zero padding or cut away) or, false, on the face grid itself (a circulant preconditioner).
+    bool embedding = true;
     ProdDims dims{};
     // Operator.
     DimTables tx, ty;
@@ -118,6 +121,9 @@ std::shared_ptr<Problem> make_problem(Context *ctx, int cx, int cy, double a, do
                                       int far_order, const uint8_t *mask);
 // product.hip
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier);
+// The circulant preconditioner of the problem on its face grid (RT_CIRCULANT_* kernel, RT_BUILDER_* builder).
+std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, int bandwidth_x, int bandwidth_y,
+                                          int modifier, int precision);
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
                     int skip_stride);
 void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s);

@@ -280,3 +280,73 @@ def test_device_plane_waves_and_observables(rt):
     with pytest.raises(rt.RooftopError) as e:
         rt.input_impedances(p, [0, 0], xb)
     assert e.value.identifier == 'rooftopMoM:excitation:invalidPort'
+
+
+@pytest.mark.parametrize('size', [(8, 8, 0.1, 0.1), (16, 4, 0.1, 0.12), (4, 16, 0.05, 0.1), (32, 32, 0.1, 0.1),
+                                  (8, 1, 0.1, 0.02), (1, 8, 0.03, 0.1)])
+def test_circulant_preconditioner_matches_oracle(rt, size):
+    """rooftopMoM.Preconditioner(problem, 'kernel', k, 'builder', 'partwise') on the GPU (Preconditioner.m:7-70)
+    against the oracle's partwise circulant (pinned to reference.mat), every GPU kernel, both precisions."""
+    print('circulant', size, flush=True)
+    from oracle import circulant as CI
+    p, grid, gen = _setup(rt, *size, (4, 4))
+    g = p.generators()   # the oracle gets the GPU's generators: this test is about the preconditioner
+    x = np.random.default_rng(7).standard_normal((grid.count, 2)) + 1j * np.random.default_rng(8).standard_normal(
+        (grid.count, 2))
+    for kernel, bw in (('strang', None), ('tChan', None), ('rChan', None), ('huckle', (3, 5)), ('huckle', None)):
+        pc = rt.Preconditioner(p, kernel=kernel, bandwidth=bw, precision='mixed')
+        inv = CI.circulant_preconditioner(g, grid, kernel, bw)
+        ref = np.stack([CI.apply(inv, grid, x[:, r]) for r in range(2)], axis=1)
+        assert rel(pc.apply(x), ref) < 1e-12, (kernel, bw)
+        assert rel(pc.apply(x, precision='c64'), ref) < 2e-5, (kernel, bw)
+    imag = {k: None if v is None else v.imag + 0j for k, v in g.items()}
+    pc = rt.Preconditioner(p, kernel='tChan', modifier='imag')
+    assert rel(pc.apply(x[:, 0]), CI.apply(CI.circulant_preconditioner(imag, grid, 'tChan'), grid, x[:, 0])) < 1e-12
+
+
+def test_circulant_preconditioner_errors_and_solve(rt):
+    """A singular approximation raises rooftopMoM:preconditioner:singular (Preconditioner.m:57-65); the
+    kernels and grids the GPU lacks are refused; preconditioned COCG and GMRES converge in fewer iterations
+    (PreconditionerTest.m:44-55) and GMRES follows the oracle's iterations with the same preconditioner."""
+    from oracle import circulant as CI
+    p, grid, gen = _setup(rt, 16, 16, 0.1, 0.1, (4, 4))
+    with pytest.raises(rt.RooftopError) as e:
+        rt.Preconditioner(p, kernel='superoptimal')
+    assert e.value.identifier == 'rooftop:preconditioner:unsupported'
+    with pytest.raises(rt.RooftopError) as e:
+        rt.Preconditioner(p, kernel='rChan', builder='blockwise')
+    assert e.value.identifier == 'rooftop:preconditioner:unsupported'
+    with pytest.raises(rt.RooftopError) as e:
+        rt.Preconditioner(p, kernel='huckle', bandwidth=(2,))
+    assert e.value.identifier == 'rooftopMoM:preconditioner:invalidBandwidth'
+    q, _, _ = _setup(rt, 12, 8, 0.1, 0.1, (4, 4))
+    with pytest.raises(rt.RooftopError) as e:
+        rt.Preconditioner(q, kernel='rChan')
+    assert e.value.identifier == 'rooftop:preconditioner:unsupported'
+    z, zgrid, _ = _setup(rt, 8, 4, 0.1, 0.1, (3, 3))
+    zeros = z.generators()
+    z.setGenerators(*[None if zeros[k] is None else np.zeros_like(zeros[k])
+                      for k in ('magneticXX', 'magneticYY', 'electric')])
+    with pytest.raises(rt.RooftopError) as e:
+        rt.Preconditioner(z, kernel='rChan')
+    assert e.value.identifier == 'rooftopMoM:preconditioner:singular'
+    Z = S.dense_matrix(gen, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [1, 0, 0], 1.0, order=4)
+    op = rt.Operator(p)
+    pc = rt.Preconditioner(p)   # without a kernel this library's default, the near field
+    circ = rt.Preconditioner(p, kernel='rChan')
+    inv = CI.circulant_preconditioner(p.generators(), grid, 'rChan')
+    for method in ('cg', 'gmres'):
+        plain = rt.Solver(method, tolerance=1e-8, restart=30).solve(op, b)
+        res = rt.Solver(method, tolerance=1e-8, restart=30).solve(op, b, preconditioner=circ)
+        assert res.converged() and res.iterationCount() < plain.iterationCount()
+        assert rel(Z @ res.solution(), b) < 1e-8
+        if method == 'gmres':
+            ref = SV.solve(method, Z, b, tol=1e-8, max_iter=1000, restart=30,
+                           preconditioner=lambda v: CI.apply(inv, grid, v))
+            # The monitored norms agree; the last step may land on either side of 1e-8 (a flat tail).
+            k = min(res.residualNorms().size, ref['residualNorms'].size)
+            keep = ref['residualNorms'][:k] > 1e-3
+            np.testing.assert_allclose(res.residualNorms()[:k][keep], ref['residualNorms'][:k][keep], rtol=1e-6)
+            assert abs(res.iterationCount() - ref['iterationCount']) <= 1
+    assert pc.kind == 'nearField' and circ.kind == 'circulant'

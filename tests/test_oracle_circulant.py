@@ -1,0 +1,103 @@
+"""Pins the oracle's circulant preconditioners (CPU): reference.mat, brute-force folds, optimality, the inverse."""
+import numpy as np
+import pytest
+
+from conftest import rel
+from oracle import assembly as A
+from oracle import circulant as C
+from oracle import spectra as S
+
+
+def test_partwise_strang_and_tchan_match_reference_mat(reference_data):
+    """tests/+reference/+assembler/+preconditioner/GenerateCirculantTest.m:5-25 at 1e-12."""
+    for case in reference_data:
+        grid = A.Grid(case['countX'], case['countY'], case['sizeX'], case['sizeY'])
+        g = {'magneticXX': case['magneticXX'], 'magneticYY': case['magneticYY'], 'electric': case['electric']}
+        for kernel, key in (('strang', 'preconditionedStrang'), ('tChan', 'preconditionedTChan')):
+            inv = C.circulant_preconditioner(g, grid, kernel)
+            assert rel(C.apply(inv, grid, case['vector']), case[key]) < 1e-12, (kernel, grid.content)
+
+
+def _even_generator(cx, cy, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((cx, cy)) + 1j * rng.standard_normal((cx, cy))
+
+
+def _brute_fold(two_sided, weight):
+    """c(p, q) = sum over the offsets p' = p (mod cx), |p'| < cx, and q' likewise of w(p', q') t(p', q')."""
+    cx, cy = (two_sided.shape[0] + 1) // 2, (two_sided.shape[1] + 1) // 2
+    c = np.zeros((cx, cy), complex)
+    for ox in range(-(cx - 1), cx):
+        for oy in range(-(cy - 1), cy):
+            c[ox % cx, oy % cy] += weight(ox, oy) * two_sided[ox + cx - 1, oy + cy - 1]
+    return c
+
+
+@pytest.mark.parametrize('shape', [(5, 4), (4, 4), (1, 6), (7, 1), (6, 3)])
+def test_folds_match_brute_force(shape):
+    """R. Chan wraps every offset, Strang keeps the offsets below half the period (half weight at half), T. Chan
+    weights by how often the offset occurs; Huckle at full bandwidth is T. Chan."""
+    cx, cy = shape
+    g = _even_generator(cx, cy, 3)
+    t = S.extend_evenly(g)
+    rchan = _brute_fold(t, lambda ox, oy: 1.0)
+    assert np.allclose(C.fold_generator(g, C.window_rchan, even=True), rchan, rtol=0, atol=1e-13)
+    assert np.allclose(C.fold_generator(t, C.window_rchan), rchan, rtol=0, atol=1e-13)
+
+    def strang_w(o, n):
+        return 1.0 if 2 * abs(o) < n else (0.5 if 2 * abs(o) == n else 0.0)
+    strang = _brute_fold(t, lambda ox, oy: strang_w(ox, cx) * strang_w(oy, cy))
+    assert np.allclose(C.fold_generator(t, C.window_strang), strang, rtol=0, atol=1e-13)
+    tchan = _brute_fold(t, lambda ox, oy: (cx - abs(ox)) / cx * (cy - abs(oy)) / cy)
+    assert np.allclose(C.fold_generator(g, C.window_tchan, even=True), tchan, rtol=0, atol=1e-13)
+    assert np.allclose(C.fold_generator(t, C.window_tchan), tchan, rtol=0, atol=1e-13)
+    assert np.allclose(C.fold_generator(t, C.window_huckle((cx, cy))), tchan, rtol=0, atol=1e-13)
+    # Huckle with bandwidth 1 keeps the offset 0 only.
+    h1 = C.fold_generator(t, C.window_huckle((1, 1)))
+    assert h1[0, 0] == g[0, 0] and np.count_nonzero(h1) == 1
+
+
+def test_tchan_is_the_frobenius_optimal_circulant():
+    """T. Chan's circulant is the average of the Toeplitz matrix over every wrapped diagonal, the circulant closest
+    in the Frobenius norm: perturbing its generator only moves away."""
+    cx, cy = 5, 3
+    g = _even_generator(cx, cy, 4)
+    T = C.dense_toeplitz(S.extend_evenly(g))
+    c = C.fold_generator(g, C.window_tchan, even=True)
+    best = np.linalg.norm(T - C.dense_circulant(c))
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        d = 1e-3 * (rng.standard_normal((cx, cy)) + 1j * rng.standard_normal((cx, cy)))
+        assert np.linalg.norm(T - C.dense_circulant(c + d)) > best
+    # The average itself, class by class.
+    i = np.arange(cx * cy)
+    ix, iy = i % cx, i // cx
+    cls_x = (ix[:, None] - ix[None, :]) % cx
+    cls_y = (iy[:, None] - iy[None, :]) % cy
+    for p in range(cx):
+        for q in range(cy):
+            sel = (cls_x == p) & (cls_y == q)
+            assert abs(T[sel].mean() - c[p, q]) < 1e-13
+
+
+def test_rchan_of_a_circulant_is_itself_and_inverse_blocks():
+    """R. Chan of the Toeplitz generator of a circulant returns that circulant; the inverse spectra are the 2x2
+    inverses frequency by frequency; a zero block is singular."""
+    cx, cy = 4, 6
+    rng = np.random.default_rng(6)
+    c = rng.standard_normal((cx, cy)) + 1j * rng.standard_normal((cx, cy))
+    # A Toeplitz generator holding c at the offsets 0 .. count-1 and zero at the negative ones: every wrapped class
+    # is hit once, so R. Chan's fold gives c back.
+    t = np.zeros((2 * cx - 1, 2 * cy - 1), complex)
+    t[cx - 1:, cy - 1:] = c
+    assert np.allclose(C.fold_generator(t, C.window_rchan), c, rtol=0, atol=1e-14)
+    grid = A.Grid(7, 5, 0.1, 0.12)
+    g = A.generate(grid, 2 * np.pi, (3, 3))
+    sp = C.partwise_spectra(g, grid, 'rChan')
+    inv = C.invert(sp, grid.content)
+    one = sp['XX'] * inv['XX'] + sp['XY'] * inv['YX']
+    zero = sp['XX'] * inv['XY'] + sp['XY'] * inv['YY']
+    assert np.allclose(one, 1, atol=1e-11) and np.allclose(zero, 0, atol=1e-11)
+    assert not C.is_singular(inv)
+    zeros = {k: None if v is None else np.zeros_like(v) for k, v in g.items()}
+    assert C.is_singular(C.invert(C.partwise_spectra(zeros, grid, 'rChan'), grid.content))
