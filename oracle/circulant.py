@@ -2,12 +2,14 @@
 infrastructure only).
 
 Follows src/+rooftopMoM/+assembler/+preconditioner/: generateCirculant.m:16-24, generatePartwiseSpectra.m:16-30,
-+kernels/generate{Strang,TChan,RChan,Huckle}.m, +utils/foldGenerator.m:27-61 and @Spectra/invert.m:12-20, applied by
-the FFTApplicator of the face grid (FFTApplicator.m:14-87).
+generateHybridSpectra.m, +kernels/generate{Strang,TChan,RChan,Huckle}.m, +utils/foldGenerator.m:27-61 and
+@Spectra/invert.m:12-20, applied by the FFTApplicator of the face grid (FFTApplicator.m:14-87).
 
 Pinned: reference.mat preconditioned{1,2} = partwise Strang and T. Chan at 1e-12 (GenerateCirculantTest.m:5-25); the
 folds against dense circulants built entry by entry, T. Chan's Frobenius optimality, Huckle at full bandwidth = T. Chan,
-R. Chan of a circulant = that circulant, and the inverse against a dense solve.
+R. Chan of a circulant = that circulant, the inverse blocks frequency by frequency; the hybrid builder from those pinned
+parts (its coupling is the partwise one, its diagonal blocks folds of the spatial blocks, pinned against the dense
+matrix).
 """
 
 import numpy as np
@@ -106,6 +108,30 @@ def partwise_spectra(generators, grid, kernel, bandwidth=None):
                            grid.content, grid.size_x, grid.size_y, sx, sy)
 
 
+def _fold_two_sided(t, kernel, bandwidth, count_x, count_y):
+    """A kernel applied to a two-sided generator (the blocks of the blockwise and hybrid builders)."""
+    if t is None:
+        return None
+    windows = {'strang': window_strang, 'tChan': window_tchan, 'rChan': window_rchan,
+               'huckle': window_huckle(bandwidth if bandwidth is not None else (count_x, count_y))}
+    return fold_generator(t, windows[kernel])
+
+
+def hybrid_spectra(generators, grid, kernel, bandwidth=None):
+    """generateHybridSpectra.m: the kernel's circulants of the spatial diagonal blocks (generateSpatialBlocks.m) and
+    the coupling of the electric generator's circulant through the block spectra."""
+    e = generators['electric']
+    cx, cy = e.shape
+    xx, _, _, yy = S.spatial_blocks(generators, grid)
+    sx, sy = S.shift_phases(cx, cy, 0, 0)
+    electric = np.fft.fft2(_fold_two_sided(S.extend_evenly(e), kernel, bandwidth, cx, cy))
+    coupling = S.block_spectra(None, None, electric, grid.content, grid.size_x, grid.size_y, sx, sy)
+    fx = _fold_two_sided(xx, kernel, bandwidth, cx, cy)
+    fy = _fold_two_sided(yy, kernel, bandwidth, cx, cy)
+    return {'XX': None if fx is None else np.fft.fft2(fx), 'XY': coupling['XY'], 'YX': coupling['YX'],
+            'YY': None if fy is None else np.fft.fft2(fy)}
+
+
 def invert(spectra, content):
     """@Spectra/invert.m:12-20: the 2x2 inverse per frequency, or the reciprocal of the single block of a strip."""
     with np.errstate(divide='ignore', invalid='ignore'):
@@ -123,9 +149,10 @@ def is_singular(inverse):
     return not all(np.all(np.isfinite(b)) for b in inverse.values() if b is not None)
 
 
-def circulant_preconditioner(generators, grid, kernel='rChan', bandwidth=None):
-    """The inverse spectra of generateCirculant.m:16-24 with the partwise builder."""
-    return invert(partwise_spectra(generators, grid, kernel, bandwidth), grid.content)
+def circulant_preconditioner(generators, grid, kernel='rChan', bandwidth=None, builder='partwise'):
+    """The inverse spectra of generateCirculant.m:16-24 with the partwise or hybrid builder."""
+    build = partwise_spectra if builder == 'partwise' else hybrid_spectra
+    return invert(build(generators, grid, kernel, bandwidth), grid.content)
 
 
 def apply(inverse, grid, x):

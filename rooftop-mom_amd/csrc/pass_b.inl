@@ -69,7 +69,8 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
 // every output kept), as in the Toeplitz product; false for full-period columns (a circulant on the face grid).
 template <typename TC, int LOG2M, int BITS, int MAXT, bool EMB = true>
 __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ bspec,
-                                                 const int *__restrict__ fold_x, const cplx<TC> *__restrict__ dx,
+                                                 const int *__restrict__ fold_x, const int *__restrict__ border,
+                                                 const cplx<TC> *__restrict__ dx,
                                                  const cplx<TC> *__restrict__ dy, ProdDims d,
                                                  const cplx<TC> *__restrict__ tw, int tile, int R, int nblocks,
                                                  const int *__restrict__ skip, int skip_stride) {
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
     extern __shared__ unsigned char smem_raw[];
     constexpr int Mp = P::pad(P::M - 1) + 1;
     const int tt = threadIdx.x / P::T, t = threadIdx.x - tt * P::T;
-    const int px = pt * tile + tt;
+    const int px = border[pt * tile + tt];   // partners k and M - k run together
     cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + 2 * tt * Mp;
     cplx<TC> *col1 = col0 + Mp;
     cplx<TC> *w0 = work + work_index(d, r, 0, px, 0);
@@ -227,7 +228,8 @@ __device__ __forceinline__ void pass_b_team(int t, cplx<TC> *mine, const cplx<TC
 
 template <typename TC, int LOG2M, int BITS, int MAXT>
 __global__ void __launch_bounds__(MAXT) pass_b2_k(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ bspec,
-                                                  const int *__restrict__ fold_x, const cplx<TC> *__restrict__ dx,
+                                                  const int *__restrict__ fold_x, const int *__restrict__ border,
+                                                  const cplx<TC> *__restrict__ dx,
                                                   const cplx<TC> *__restrict__ dy, ProdDims d,
                                                   const cplx<TC> *__restrict__ tw, int tile, int R, int nblocks,
                                                   const int *__restrict__ skip, int skip_stride) {
@@ -243,7 +245,7 @@ __global__ void __launch_bounds__(MAXT) pass_b2_k(cplx<TC> *__restrict__ work, c
     const int rem = threadIdx.x - tt * 2 * P::T;
     const int c = __builtin_amdgcn_readfirstlane(rem / P::T);   // the team's component, uniform in the wave
     const int t = rem - c * P::T;
-    const int px = pt * tile + tt;
+    const int px = border[pt * tile + tt];   // partners k and M - k run together
     cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + 2 * tt * Mp;
     cplx<TC> *col1 = col0 + Mp;
     const cplx<TC> *row = bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
@@ -281,19 +283,19 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
             constexpr int MT = PF::T > 256 ? PF::T : 256;                                                          \
             set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, 3, MT, false>), lds);                          \
             launch_timed(timer, pass_b_k<TC, L, 3, MT, false>, dim3(xcd_grid(nb)), dim3(threads),                  \
-                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, \
+                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, nrhs, \
                          static_cast<int>(nb), skip, skip_stride);                                                 \
         } else if (teams == 1) {                                                                                   \
             constexpr int MT = PL::T > 256 ? PL::T : 256;                                                          \
             set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, B, MT>), lds);                                 \
             launch_timed(timer, pass_b_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads),                         \
-                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, \
+                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, nrhs, \
                          static_cast<int>(nb), skip, skip_stride);                                                 \
         } else if constexpr (PL::K >= 2 && PL::T % 64 == 0 && 2 * PL::T <= 1024) {                                 \
             constexpr int MT = 2 * PL::T > 256 ? 2 * PL::T : 256;                                                  \
             set_lds(reinterpret_cast<const void *>(&pass_b2_k<TC, L, B, MT>), lds);                                \
             launch_timed(timer, pass_b2_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(2 * threads),                    \
-                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, dx, dy, op.dims, tw, tile, nrhs, \
+                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, nrhs, \
                          static_cast<int>(nb), skip, skip_stride);                                                 \
         }                                                                                                          \
     }

@@ -81,7 +81,8 @@ __global__ void __launch_bounds__(MAXT) pass_by_inv_k(cplx<TC> *__restrict__ wor
 // The three-plane multiply at every (x position, y position) of every right-hand side (the spectra of pass B).
 template <typename TC>
 __global__ void __launch_bounds__(256) pass_by_mul_k(cplx<TC> *__restrict__ spec, const cplx<TC> *__restrict__ bspec,
-                                                     const int *__restrict__ fold_x, const cplx<TC> *__restrict__ dx,
+                                                     const int *__restrict__ fold_x, const int *__restrict__ border,
+                                                     const cplx<TC> *__restrict__ dx,
                                                      const cplx<TC> *__restrict__ dy, int Mx, int My, int R,
                                                      const int *__restrict__ skip, int skip_stride) {
     const int64_t plane = static_cast<int64_t>(Mx) * My;
@@ -90,8 +91,9 @@ __global__ void __launch_bounds__(256) pass_by_mul_k(cplx<TC> *__restrict__ spec
         const int64_t r = e / plane;
         if (skip && skip[r * skip_stride]) continue;
         const int64_t p = e - r * plane;
-        const int px = static_cast<int>(p / My), py = static_cast<int>(p - static_cast<int64_t>(px) * My);
-        cplx<TC> *X = spec + (r * 2) * plane + p;
+        const int q = static_cast<int>(p / My), py = static_cast<int>(p - static_cast<int64_t>(q) * My);
+        const int px = border[q];   // partners k and M - k run together and share their spectrum row
+        cplx<TC> *X = spec + (r * 2) * plane + static_cast<int64_t>(px) * My + py;
         cplx<TC> *Y = X + plane;
         const cplx<TC> *sp = bspec + (static_cast<int64_t>(fold_x[px]) * My + py) * 3;
         const cplx<TC> dxv = dx[px], dyv = dy[py];
@@ -128,7 +130,8 @@ void launch_pass_b_split(const Product &op, const PlanTables &tx, const PlanTabl
                      static_cast<int>(nb), skip, skip_stride);                                                 \
         launch_timed(nullptr, pass_by_mul_k<TC>, dim3(blocks_for(static_cast<size_t>(op.dims.Mx) * op.dims.My * \
                                                                  nrhs, 256, 65536)),                          \
-                     dim3(256), 0u, s, spec, bspec, tx.fold.ptr, dx, dy, op.dims.Mx, op.dims.My, nrhs, skip,   \
+                     dim3(256), 0u, s, spec, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims.Mx, op.dims.My,  \
+                     nrhs, skip,                                                                               \
                      skip_stride);                                                                             \
         launch_timed(nullptr, pass_by_inv_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads),                  \
                      static_cast<uint32_t>(lds), s, work, static_cast<const cplx<TC> *>(spec), op.dims, tw,    \

@@ -46,11 +46,16 @@ void DimTables::build(Context *ctx, int M, double size, hipStream_t s) {
         t.tw.allocate(ctx, h_tw.size(), "FFT twiddles");
         t.tw32.allocate(ctx, h_tw.size(), "FFT twiddles");
         t.fold.allocate(ctx, M, "FFT fold table");
+        std::vector<int> h_order(M);
+        for (int q = 0; q < M; ++q) h_order[q] = q;
+        std::stable_sort(h_order.begin(), h_order.end(), [&](int u, int v) { return h_fold[u] < h_fold[v]; });
+        t.border.allocate(ctx, M, "pass B position order");
         t.d.allocate(ctx, M, "divergence table");
         t.d32.allocate(ctx, M, "divergence table");
         RT_HIP(hipMemcpyAsync(t.tw.ptr, h_tw.data(), h_tw.size() * sizeof(c128), hipMemcpyHostToDevice, s));
         RT_HIP(hipMemcpyAsync(t.tw32.ptr, h_tw32.data(), h_tw.size() * sizeof(c64), hipMemcpyHostToDevice, s));
         RT_HIP(hipMemcpyAsync(t.fold.ptr, h_fold.data(), M * sizeof(int), hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(t.border.ptr, h_order.data(), M * sizeof(int), hipMemcpyHostToDevice, s));
         RT_HIP(hipMemcpyAsync(t.d.ptr, h_d.data(), M * sizeof(c128), hipMemcpyHostToDevice, s));
         RT_HIP(hipMemcpyAsync(t.d32.ptr, h_d32.data(), M * sizeof(c64), hipMemcpyHostToDevice, s));
         RT_HIP(hipStreamSynchronize(s));
@@ -458,11 +463,42 @@ __device__ double circulant_window(int o, int P, int kind, double bandwidth) {
     }
 }
 
+// Value of an even generator (rows x cols, column-major, zero-padded) at the offset (|p|, |q|), after the modifier.
+__device__ __forceinline__ c128 generator_at(const c128 *__restrict__ g, int rows, int cols, int p, int q,
+                                              int modifier) {
+    if (p < 0) p = -p;
+    if (q < 0) q = -q;
+    if (p >= rows || q >= cols) return mk(0.0, 0.0);
+    c128 v = g[static_cast<int64_t>(q) * rows + p];
+    if (modifier == RT_MODIFIER_REAL) v.y = 0;
+    if (modifier == RT_MODIFIER_IMAG) v = mk(v.y, 0.0);
+    return v;
+}
+
+// The two-sided spatial generator of a diagonal block at the offset (p, q) (generateSpatialBlocks.m:19-61):
+// XX(p, q) = mXX(p, q) - (2 e(p, q) - e(p - 1, q) - e(p + 1, q)) / a^2, zero beyond |p| = P - 2; YY along q with b.
+// block 1: XX from g = mXX ((P-1) x Q); block 2: YY from g = mYY (P x (Q-1)); e is P x Q.
+__device__ __forceinline__ c128 spatial_block_at(int block, const c128 *__restrict__ g, const c128 *__restrict__ e,
+                                                 int P, int Q, double size, int p, int q, int modifier) {
+    if (block == 1) {
+        if ((p < 0 ? -p : p) > P - 2) return mk(0.0, 0.0);
+        const c128 d = 2.0 * generator_at(e, P, Q, p, q, modifier) - generator_at(e, P, Q, p - 1, q, modifier) -
+                       generator_at(e, P, Q, p + 1, q, modifier);
+        return generator_at(g, P - 1, Q, p, q, modifier) - (1.0 / (size * size)) * d;
+    }
+    if ((q < 0 ? -q : q) > Q - 2) return mk(0.0, 0.0);
+    const c128 d = 2.0 * generator_at(e, P, Q, p, q, modifier) - generator_at(e, P, Q, p, q - 1, modifier) -
+                   generator_at(e, P, Q, p, q + 1, modifier);
+    return generator_at(g, P, Q - 1, p, q, modifier) - (1.0 / (size * size)) * d;
+}
+
 // c(kx, ky) for kx <= P/2, ky <= Q/2 (the rest follows from the symmetry c(k) = c(P - k) of the fold of an even
 // generator): the sum over the offsets ox in {kx, kx - P} (only 0 for kx = 0), oy likewise, of
-// w(ox) w(oy) g(|ox|, |oy|); g is the generator (rows x cols, column-major) zero-padded to P x Q. Output [Hy][Hx].
+// w(ox) w(oy) t(ox, oy). block 0: t = the generator g (rows x cols, column-major) zero-padded to P x Q; blocks 1, 2:
+// t = the spatial XX or YY block of g and the electric generator e (the hybrid builder). Output [Hy][Hx].
 __global__ void circulant_fold(const c128 *__restrict__ g, int rows, int cols, int P, int Q, int kind, double bwx,
-                               double bwy, int modifier, c128 *__restrict__ out, int Hx, int Hy) {
+                               double bwy, int modifier, c128 *__restrict__ out, int Hx, int Hy, int block,
+                               const c128 *__restrict__ e, double size) {
     const int64_t total = static_cast<int64_t>(Hx) * Hy;
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -474,11 +510,9 @@ __global__ void circulant_fold(const c128 *__restrict__ g, int rows, int cols, i
             for (int sy = 0; sy < (ky == 0 ? 1 : 2); ++sy) {
                 const int oy = sy ? ky - Q : ky;
                 const double w = wx * circulant_window(oy, Q, kind, bwy);
-                const int px = ox < 0 ? -ox : ox, py = oy < 0 ? -oy : oy;
-                if (w == 0 || px >= rows || py >= cols) continue;
-                c128 v = g[static_cast<int64_t>(py) * rows + px];
-                if (modifier == RT_MODIFIER_REAL) v.y = 0;
-                if (modifier == RT_MODIFIER_IMAG) v = mk(v.y, 0.0);
+                if (w == 0) continue;
+                const c128 v = block == 0 ? generator_at(g, rows, cols, ox, oy, modifier)
+                                          : spatial_block_at(block, g, e, P, Q, size, ox, oy, modifier);
                 sum = sum + w * v;
             }
         }
@@ -486,7 +520,8 @@ __global__ void circulant_fold(const c128 *__restrict__ g, int rows, int cols, i
     }
 }
 
-// In place on the folded planes [3][Hx][Hy] (the circulants' spectra A, B, E): the inverse of the 2x2 block
+// In place on the folded planes [3][Hx][Hy] (the circulants' spectra A, B, E; with blocks_given the first two are the
+// diagonal blocks XX, YY themselves, the hybrid builder's): the inverse of the 2x2 block
 // [XX, XY; YX, YY] (XX = A - |Dx|^2 E, YY = B - |Dy|^2 E, XY = -Dx conj(Dy) E, YX = -conj(Dx) Dy E) written in the
 // same three-plane form as the product, (A', B', E') = ((YY - |Dx|^2 E) / det, (XX - |Dy|^2 E) / det, -E / det), so
 // that the product's passes apply it; 1 / XX (1 / YY) alone for a strip. Scaled by 1/(Mx My) for the inverse FFT.
@@ -496,7 +531,7 @@ __device__ __forceinline__ c128 reciprocal(c128 z) {
     return mk(z.x / m2, -z.y / m2);
 }
 __global__ void circulant_invert(c128 *__restrict__ planes, int Hx, int Hy, int Mx, int My, double a, double b,
-                                 int has_x, int has_y, double scale, int *__restrict__ singular) {
+                                 int has_x, int has_y, int blocks_given, double scale, int *__restrict__ singular) {
     const int64_t plane = static_cast<int64_t>(Hx) * Hy;
     const double pi = 3.141592653589793;
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < plane;
@@ -507,17 +542,17 @@ __global__ void circulant_invert(c128 *__restrict__ planes, int Hx, int Hy, int 
         const c128 A = planes[i], B = planes[plane + i], E = planes[2 * plane + i];
         c128 a2 = mk(0.0, 0.0), b2 = mk(0.0, 0.0), e2 = mk(0.0, 0.0), det;
         if (has_x && has_y) {
-            const c128 xx = A - dx2 * E, yy = B - dy2 * E;
+            const c128 xx = blocks_given ? A : A - dx2 * E, yy = blocks_given ? B : B - dy2 * E;
             det = xx * yy - (dx2 * dy2) * (E * E);
             const c128 inv = reciprocal(det);
             a2 = (yy - dx2 * E) * inv;
             b2 = (xx - dy2 * E) * inv;
             e2 = mk(0.0, 0.0) - E * inv;
         } else if (has_x) {
-            det = A - dx2 * E;
+            det = blocks_given ? A : A - dx2 * E;
             a2 = reciprocal(det);
         } else {
-            det = B - dy2 * E;
+            det = blocks_given ? B : B - dy2 * E;
             b2 = reciprocal(det);
         }
         const bool ok = (det.x != 0 || det.y != 0) && isfinite(a2.x) && isfinite(a2.y) && isfinite(b2.x) &&
@@ -534,10 +569,12 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
     Context *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const GridInfo &g = p->g;
-    if (builder != RT_BUILDER_PARTWISE || kernel < RT_CIRCULANT_STRANG || kernel > RT_CIRCULANT_HUCKLE)
+    if ((builder != RT_BUILDER_PARTWISE && builder != RT_BUILDER_HYBRID) || kernel < RT_CIRCULANT_STRANG ||
+        kernel > RT_CIRCULANT_HUCKLE)
         fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
-             "The GPU circulant preconditioner takes the partwise builder with the strang, tChan, rChan or huckle "
-             "kernel.");
+             "The GPU circulant preconditioner takes the partwise or hybrid builder with the strang, tChan, rChan or "
+             "huckle kernel.");
+    const bool hybrid = builder == RT_BUILDER_HYBRID;
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
     if (!pow2(g.cx) || !pow2(g.cy))
         fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
@@ -590,9 +627,13 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
         const int ex = 1 << tx.plan.log2E, ey = 1 << ty.plan.log2E;
         for (int k = 0; k < 3; ++k) {
             if (!gens[k]) continue;
+            // Partwise: the folds of the generators; hybrid (generateHybridSpectra.m): the folds of the spatial
+            // diagonal blocks XX, YY (generateSpatialBlocks.m) and of the electric generator.
+            const int block = hybrid && k < 2 ? k + 1 : 0;
             hipLaunchKernelGGL(circulant_fold, dim3(blocks_for(plane_elems, 256, 4096)), dim3(256), 0, s, gens[k],
                                rows[k], cols[k], d.Mx, d.My, kernel, static_cast<double>(bandwidth_x),
-                               static_cast<double>(bandwidth_y), modifier, folded.ptr, d.Hx, d.Hy);
+                               static_cast<double>(bandwidth_y), modifier, folded.ptr, d.Hx, d.Hy, block, p->e.ptr,
+                               k == 0 ? g.a : g.b);
             RT_LAUNCH_CHECK();
             // The fold is even on the period, so the product's spectrum kernels transform it as the even extension
             // of its half [0, P/2] x [0, Q/2].
@@ -616,7 +657,7 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
         flag.zero(s);
         hipLaunchKernelGGL(circulant_invert, dim3(blocks_for(plane_elems, 256, 4096)), dim3(256), 0, s,
                            pc->planes64.ptr, d.Hx, d.Hy, d.Mx, d.My, g.a, g.b, g.has_x ? 1 : 0, g.has_y ? 1 : 0,
-                           1.0 / (static_cast<double>(d.Mx) * d.My), flag.ptr);
+                           hybrid ? 1 : 0, 1.0 / (static_cast<double>(d.Mx) * d.My), flag.ptr);
         RT_LAUNCH_CHECK();
         int singular = 0;
         RT_HIP(hipMemcpyAsync(&singular, flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));

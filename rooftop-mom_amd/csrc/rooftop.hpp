@@ -54,6 +54,8 @@ struct PlanTables {
     DeviceArray<c128> tw;     // twiddles w_L^(q n) of the rounds
     DeviceArray<c64> tw32;
     DeviceArray<int> fold;    // fold index min(k, M - k) of the frequency k held at each position
+    DeviceArray<int> border;  // the positions ordered by fold index: the two positions of a folded spectrum row
+                              // (frequencies k and M - k) are neighbours, so pass B reads the row for both at once
     DeviceArray<c128> d;      // divergence factor (1 - exp(2 pi i k / M)) / size at each position
     DeviceArray<c64> d32;
 };

@@ -293,12 +293,14 @@ def test_circulant_preconditioner_matches_oracle(rt, size):
     g = p.generators()   # the oracle gets the GPU's generators: this test is about the preconditioner
     x = np.random.default_rng(7).standard_normal((grid.count, 2)) + 1j * np.random.default_rng(8).standard_normal(
         (grid.count, 2))
-    for kernel, bw in (('strang', None), ('tChan', None), ('rChan', None), ('huckle', (3, 5)), ('huckle', None)):
-        pc = rt.Preconditioner(p, kernel=kernel, bandwidth=bw, precision='mixed')
-        inv = CI.circulant_preconditioner(g, grid, kernel, bw)
+    for kernel, bw, builder in (('strang', None, 'partwise'), ('tChan', None, 'partwise'), ('rChan', None, 'partwise'),
+                                ('huckle', (3, 5), 'partwise'), ('huckle', None, 'partwise'),
+                                ('rChan', None, 'hybrid'), ('strang', None, 'hybrid'), ('huckle', (2, 4), 'hybrid')):
+        pc = rt.Preconditioner(p, kernel=kernel, bandwidth=bw, builder=builder, precision='mixed')
+        inv = CI.circulant_preconditioner(g, grid, kernel, bw, builder)
         ref = np.stack([CI.apply(inv, grid, x[:, r]) for r in range(2)], axis=1)
-        assert rel(pc.apply(x), ref) < 1e-12, (kernel, bw)
-        assert rel(pc.apply(x, precision='c64'), ref) < 2e-5, (kernel, bw)
+        assert rel(pc.apply(x), ref) < 1e-12, (kernel, bw, builder)
+        assert rel(pc.apply(x, precision='c64'), ref) < 2e-5, (kernel, bw, builder)
     imag = {k: None if v is None else v.imag + 0j for k, v in g.items()}
     pc = rt.Preconditioner(p, kernel='tChan', modifier='imag')
     assert rel(pc.apply(x[:, 0]), CI.apply(CI.circulant_preconditioner(imag, grid, 'tChan'), grid, x[:, 0])) < 1e-12

@@ -101,3 +101,24 @@ def test_rchan_of_a_circulant_is_itself_and_inverse_blocks():
     assert not C.is_singular(inv)
     zeros = {k: None if v is None else np.zeros_like(v) for k, v in g.items()}
     assert C.is_singular(C.invert(C.partwise_spectra(zeros, grid, 'rChan'), grid.content))
+
+
+def test_hybrid_builder_from_pinned_parts():
+    """generateHybridSpectra.m: the coupling blocks are the partwise ones (the same electric circulant), the
+    diagonal blocks the kernel's fold of the spatial blocks (pinned against the dense matrix) by brute force."""
+    grid = A.Grid(6, 5, 0.1, 0.12)
+    g = A.generate(grid, 2 * np.pi, (3, 3))
+    for kernel in ('strang', 'tChan', 'rChan', 'huckle'):
+        hy = C.hybrid_spectra(g, grid, kernel, (2, 3) if kernel == 'huckle' else None)
+        pw = C.partwise_spectra(g, grid, kernel, (2, 3) if kernel == 'huckle' else None)
+        assert np.allclose(hy['XY'], pw['XY'], rtol=0, atol=1e-13 * np.abs(pw['XY']).max())
+        assert np.allclose(hy['YX'], pw['YX'], rtol=0, atol=1e-13 * np.abs(pw['YX']).max())
+    xx, _, _, yy = S.spatial_blocks(g, grid)
+    for blk, key in ((xx, 'XX'), (yy, 'YY')):
+        brute = _brute_fold(blk, lambda ox, oy: 1.0)
+        assert np.allclose(C.hybrid_spectra(g, grid, 'rChan')[key], np.fft.fft2(brute), rtol=0,
+                           atol=1e-12 * np.abs(np.fft.fft2(brute)).max())
+    strip = A.Grid(8, 1, 0.1, 0.02)
+    gs = A.generate(strip, 2 * np.pi, (3, 3))
+    inv = C.circulant_preconditioner(gs, strip, 'tChan', builder='hybrid')
+    assert inv['XX'] is not None and inv['YY'] is None and not C.is_singular(inv)
