@@ -311,12 +311,24 @@ __global__ void __launch_bounds__(kThreads) k_cgs_update_dot(const c128 *__restr
     c128 *wr = w + r * n;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
     double dre = 0, dim = 0;
+    constexpr int kPer = kLdp * kTileRows / kThreads;
     for (int64_t t0 = base; t0 < end; t0 += kTileRows) {
         const int rows = static_cast<int>(min(static_cast<int64_t>(kTileRows), end - t0));
-        __syncthreads();   // the previous tile is consumed (and hs is written, first time)
-        for (int e = threadIdx.x; e < cnt * kTileRows; e += kThreads) {
+        // The tile's loads are all issued before any is stored (cnt <= 64 columns: at most 16 per thread), so the
+        // staging costs one memory latency, not one per column.
+        c128 vals[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = threadIdx.x + u * kThreads;
             const int k = e >> 6, i = e & (kTileRows - 1);
-            tile[k * kTileStride + i] = i < rows ? basis_col(V, k, r, R, n)[t0 + i] : mk(0.0, 0.0);
+            vals[u] = (k < cnt && i < rows) ? basis_col(V, k, r, R, n)[t0 + i] : mk(0.0, 0.0);
+        }
+        __syncthreads();   // the previous tile is consumed (and hs is written, first time)
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            const int k = e >> 6, i = e & (kTileRows - 1);
+            if (k < cnt) tile[k * kTileStride + i] = vals[u];
         }
         __syncthreads();
         c128 acc = mk(0.0, 0.0);
@@ -367,7 +379,14 @@ __global__ void __launch_bounds__(kThreads) k_cgs_update_norm(const c128 *__rest
     double s = 0;
     for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
         c128 v = wr[i];
-        for (int k = 0; k < cnt; ++k) v = v - hs[k] * basis_col(V, k, r, R, n)[i];
+        for (int k0 = 0; k0 < cnt; k0 += 8) {   // eight columns' loads in flight at once, the same order of terms
+            c128 vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vv[u] = k0 + u < cnt ? basis_col(V, k0 + u, r, R, n)[i] : mk(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k0 + u < cnt) v = v - hs[k0 + u] * vv[u];
+        }
         wr[i] = v;
         s += abs2(v);
     }
