@@ -9,6 +9,7 @@
 // +preconditioner/+utils/generateSpatialBlocks.m:19-61. Z depends on offsets only, so every rooftop whose
 // neighbourhood has the same existing rooftops shares its row: rows are fitted once per class, and P is applied as
 // an 18-point stencil over the two rooftop grids with the class coefficients.
+#include <algorithm>
 #include <unordered_map>
 #include <vector>
 
@@ -119,10 +120,13 @@ __global__ void sai_fit(GenView v, const int *__restrict__ dir, const unsigned l
     for (int j = 0; j < n; ++j) coef[static_cast<int64_t>(c) * kSlots + slot_of_col[j]] = x[j];
 }
 
-// z = P r with P = (M + M^T)/2 as an 18-point stencil; one thread per rooftop and right-hand side.
+// z = P r with P = (M + M^T)/2 as an 18-point stencil; one thread per rooftop and right-hand side. The stencil of a
+// rooftop depends only on its class and its neighbours' classes, so the rooftops are grouped into stencil classes
+// whose 18 symmetrized coefficients (M_ij + M_ji)/2 are tabulated once (stencil_table): the apply reads its stencil
+// class, then the 18 coefficients and the 18 neighbours' values, which do not depend on each other.
 template <typename TC>
 __global__ void __launch_bounds__(256) sai_apply(const c128 *__restrict__ in, c128 *__restrict__ out,
-                                                 const int *__restrict__ cls, const cplx<TC> *__restrict__ coef,
+                                                 const int *__restrict__ scls, const cplx<TC> *__restrict__ scoef,
                                                  int rows_x, int cols_x, int rows_y, int cols_y, int64_t nx,
                                                  int64_t n, int nrhs, const int *__restrict__ skip,
                                                  int skip_stride) {
@@ -131,33 +135,55 @@ __global__ void __launch_bounds__(256) sai_apply(const c128 *__restrict__ in, c1
     const int64_t r = id / n;
     if (skip && skip[r * skip_stride]) return;
     const int64_t i = id - r * n;
-    const int ci = cls[i];
-    if (ci < 0) {
+    int fx, fy;
+    if (i < nx) { fx = static_cast<int>(i % rows_x); fy = static_cast<int>(i / rows_x); }
+    else { const int64_t k = i - nx; fx = static_cast<int>(k % rows_y); fy = static_cast<int>(k / rows_y); }
+    const c128 *x = in + r * n;
+    const int st = scls[i];
+    if (st < 0) {
         out[id] = mk(0.0, 0.0);
         return;
     }
-    int di, fx, fy;
-    if (i < nx) { di = 0; fx = static_cast<int>(i % rows_x); fy = static_cast<int>(i / rows_x); }
-    else { di = 1; const int64_t k = i - nx; fx = static_cast<int>(k % rows_y); fy = static_cast<int>(k / rows_y); }
-    const c128 *x = in + r * n;
+    const cplx<TC> *cf = scoef + static_cast<int64_t>(st) * kSlots;
     cplx<TC> acc = mk<TC>(0, 0);
+#pragma unroll
     for (int d = 0; d < 2; ++d) {
         const int rows = d ? rows_y : rows_x, cols = d ? cols_y : cols_x;
         const int64_t base = d ? nx : 0;
+#pragma unroll
         for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
             for (int dx = -1; dx <= 1; ++dx) {
                 const int gx = fx + dx, gy = fy + dy;
                 if (gx < 0 || gy < 0 || gx >= rows || gy >= cols) continue;
-                const int64_t j = base + gx + static_cast<int64_t>(rows) * gy;
-                const int cj = cls[j];
-                if (cj < 0) continue;
                 const int s = d * 9 + (dx + 1) + 3 * (dy + 1);
-                const int sm = di * 9 + (1 - dx) + 3 * (1 - dy);
-                const cplx<TC> cf = coef[static_cast<int64_t>(ci) * kSlots + s] + coef[static_cast<int64_t>(cj) * kSlots + sm];
-                acc = acc + cf * cvt<TC>(x[j]);
+                const cplx<TC> c = cf[s], v = cvt<TC>(x[base + gx + static_cast<int64_t>(rows) * gy]);
+                if (c.x != 0 || c.y != 0) acc = acc + c * v;   // an absent neighbour's value is never used
             }
     }
-    out[id] = cvt<double>(static_cast<TC>(0.5) * acc);
+    out[id] = cvt<double>(acc);
+}
+
+// scoef[t][s] = (coef[ci][s] + coef[cj][s']) / 2 for the stencil class t of tuple (ci, cj_0 .. cj_17), s' the slot
+// of the rooftop seen from its neighbour; 0 for an absent neighbour (cj < 0). The halving is exact, so the apply
+// equals the one that sums (coef + coef) x and halves the sum.
+template <typename TC>
+__global__ void stencil_table(const int *__restrict__ tuples, int count, const cplx<TC> *__restrict__ coef,
+                              cplx<TC> *__restrict__ scoef) {
+    const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (e >= static_cast<int64_t>(count) * kSlots) return;
+    const int t = static_cast<int>(e / kSlots), s = static_cast<int>(e - static_cast<int64_t>(t) * kSlots);
+    const int *tp = tuples + static_cast<int64_t>(t) * (kSlots + 2);
+    const int ci = tp[0], di = tp[1], cj = tp[2 + s];
+    if (cj < 0) {
+        scoef[e] = mk<TC>(0, 0);
+        return;
+    }
+    const int d = s / 9, dx = s % 3 - 1, dy = (s / 3) % 3 - 1;
+    const int sm = di * 9 + (1 - dx) + 3 * (1 - dy);
+    (void)d;
+    const cplx<TC> sum = coef[static_cast<int64_t>(ci) * kSlots + s] + coef[static_cast<int64_t>(cj) * kSlots + sm];
+    scoef[e] = static_cast<TC>(0.5) * sum;
 }
 
 // Jacobi: z = r / diag(Z); diag is XX(0,0) on x rooftops and YY(0,0) on y rooftops (generateSpatialBlocks.m:35-45).
@@ -326,6 +352,72 @@ std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precisi
                                0, s, pc->coef64.ptr, pc->coef32.ptr, static_cast<int64_t>(classes) * kSlots);
             RT_LAUNCH_CHECK();
         }
+        // Stencil classes: (own class, own direction, the 18 neighbours' classes), deduplicated.
+        {
+            std::vector<int> scls(static_cast<size_t>(g.n), -1), tuples;
+            std::unordered_map<uint64_t, std::vector<int>> buckets;   // hash -> stencil classes with that hash
+            int key[kSlots + 2];
+            for (int64_t i = 0; i < g.n; ++i) {
+                if (cls[i] < 0) continue;
+                const int di = i < g.nx ? 0 : 1;
+                const int64_t k = di ? i - g.nx : i;
+                const int rows_i = di ? g.rows_y : g.rows_x;
+                const int fx = static_cast<int>(k % rows_i), fy = static_cast<int>(k / rows_i);
+                key[0] = cls[i];
+                key[1] = di;
+                for (int d = 0; d < 2; ++d) {
+                    const int rows = d ? g.rows_y : g.rows_x, cols = d ? g.cols_y : g.cols_x;
+                    const bool present = d ? g.has_y : g.has_x;
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int gx = fx + dx, gy = fy + dy;
+                            int cj = -1;
+                            if (present && gx >= 0 && gy >= 0 && gx < rows && gy < cols)
+                                cj = cls[(d ? g.nx : 0) + gx + static_cast<int64_t>(rows) * gy];
+                            key[2 + d * 9 + (dx + 1) + 3 * (dy + 1)] = cj;
+                        }
+                }
+                uint64_t h = 1469598103934665603ull;
+                for (int q = 0; q < kSlots + 2; ++q) h = (h ^ static_cast<uint32_t>(key[q])) * 1099511628211ull;
+                auto &bucket = buckets[h];
+                int id = -1;
+                for (int t : bucket)
+                    if (std::equal(key, key + kSlots + 2, tuples.begin() + static_cast<int64_t>(t) * (kSlots + 2))) {
+                        id = t;
+                        break;
+                    }
+                if (id < 0) {
+                    id = static_cast<int>(tuples.size() / (kSlots + 2));
+                    tuples.insert(tuples.end(), key, key + kSlots + 2);
+                    bucket.push_back(id);
+                }
+                scls[i] = id;
+            }
+            const int stencils = static_cast<int>(tuples.size() / (kSlots + 2));
+            pc->stencil_count = stencils;
+            pc->scls.allocate(ctx, scls.size(), "The near-field stencil classes");
+            RT_HIP(hipMemcpyAsync(pc->scls.ptr, scls.data(), scls.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            if (stencils > 0) {
+                DeviceArray<int> d_tuples;
+                d_tuples.allocate(ctx, tuples.size(), "near-field stencil tuples");
+                RT_HIP(hipMemcpyAsync(d_tuples.ptr, tuples.data(), tuples.size() * sizeof(int), hipMemcpyHostToDevice,
+                                      s));
+                const unsigned nb = blocks_for(static_cast<size_t>(stencils) * kSlots, 256, 65535);
+                if (precision != RT_C64) {
+                    pc->scoef64.allocate(ctx, static_cast<size_t>(stencils) * kSlots, "The near-field stencils");
+                    hipLaunchKernelGGL(stencil_table<double>, dim3(nb), dim3(256), 0, s, d_tuples.ptr, stencils,
+                                       pc->coef64.ptr, pc->scoef64.ptr);
+                    RT_LAUNCH_CHECK();
+                }
+                if (precision != RT_C128) {
+                    pc->scoef32.allocate(ctx, static_cast<size_t>(stencils) * kSlots, "The near-field stencils");
+                    hipLaunchKernelGGL(stencil_table<float>, dim3(nb), dim3(256), 0, s, d_tuples.ptr, stencils,
+                                       pc->coef32.ptr, pc->scoef32.ptr);
+                    RT_LAUNCH_CHECK();
+                }
+                RT_HIP(hipStreamSynchronize(s));
+            }
+        }
     } else {
         fail(RT_EINVAL, "rooftop:preconditioner", "unknown preconditioner kind %d", kind);
     }
@@ -348,7 +440,7 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     const int es = prec == RT_C128 ? 16 : 8;
     const double bytes = 32.0 * total +
                          (pc.type == kJacobi ? static_cast<double>(es) * d.n
-                                             : 4.0 * d.n + 18.0 * es * pc.class_count);
+                                             : 4.0 * d.n + 18.0 * es * pc.stencil_count);
     KernelTimer timer(pc.problem->ctx, prec == RT_C128 ? RT_TIMED_PRECOND_C128 : RT_TIMED_PRECOND_C64, bytes);
     if (pc.type == kJacobi) {
         const unsigned blocks = blocks_for(total, 256, 8192);
@@ -364,10 +456,10 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
     if (pc.class_count == 0) fail(RT_EINVAL, "rooftop:preconditioner", "the grid has no rooftop to precondition");
     if (prec == RT_C128)
-        launch_timed(&timer, sai_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef64.ptr,
+        launch_timed(&timer, sai_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.scls.ptr, pc.scoef64.ptr,
                      d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
     else
-        launch_timed(&timer, sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.cls.ptr, pc.coef32.ptr,
+        launch_timed(&timer, sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.scls.ptr, pc.scoef32.ptr,
                      d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
     timer.done();
 }

@@ -864,8 +864,8 @@ void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
 
 uint64_t Product::stored_bytes() const {
     return planes64.bytes() + bspec64[0].bytes() + bspec64[1].bytes() + bspec32[0].bytes() + bspec32[1].bytes() +
-           diag64.bytes() + diag32.bytes() + coef64.bytes() + coef32.bytes() +
-           cls.bytes();
+           diag64.bytes() + diag32.bytes() + coef64.bytes() + coef32.bytes() + cls.bytes() + scls.bytes() +
+           scoef64.bytes() + scoef32.bytes();
 }
 
 uint64_t Product::working_bytes() const {

@@ -108,6 +108,10 @@ struct Product {
     DeviceArray<c64> coef32;
     DeviceArray<int> cls;                     // near field: class of every rooftop (-1: no rooftop)
     int class_count = 0;
+    DeviceArray<int> scls;                    // near field: stencil class of every rooftop (-1: no rooftop)
+    DeviceArray<c128> scoef64;                // near field: [stencil classes][18] symmetrized coefficients
+    DeviceArray<c64> scoef32;
+    int stencil_count = 0;
 
     bool has(int prec) const { return precision == RT_MIXED || precision == prec; }
     // skip: optional device flags, one per right-hand side r at skip[r * skip_stride] (stride 0: one flag for all);
