@@ -120,48 +120,63 @@ __global__ void sai_fit(GenView v, const int *__restrict__ dir, const unsigned l
     for (int j = 0; j < n; ++j) coef[static_cast<int64_t>(c) * kSlots + slot_of_col[j]] = x[j];
 }
 
-// z = P r with P = (M + M^T)/2 as an 18-point stencil; one thread per rooftop and right-hand side. The stencil of a
-// rooftop depends only on its class and its neighbours' classes, so the rooftops are grouped into stencil classes
-// whose 18 symmetrized coefficients (M_ij + M_ji)/2 are tabulated once (stencil_table): the apply reads its stencil
-// class, then the 18 coefficients and the 18 neighbours' values, which do not depend on each other.
+// z = P r with P = (M + M^T)/2 as an 18-point stencil. The stencil of a rooftop depends only on its class and its
+// neighbours' classes, so the rooftops are grouped into stencil classes whose 18 symmetrized coefficients
+// (M_ij + M_ji)/2 are tabulated once (stencil_table): the apply reads its stencil class, then the 18 coefficients and
+// the 18 neighbours' values, which do not depend on each other.
+// One block per tile of kTX x kTY face positions (both components) and right-hand side: the tile's values with a
+// one-cell halo are staged in LDS once (coalesced along x), and the 18 neighbours of every rooftop are read from
+// there, summed in slot order.
+constexpr int kTX = 32, kTY = 8;
 template <typename TC>
-__global__ void __launch_bounds__(256) sai_apply(const c128 *__restrict__ in, c128 *__restrict__ out,
-                                                 const int *__restrict__ scls, const cplx<TC> *__restrict__ scoef,
-                                                 int rows_x, int cols_x, int rows_y, int cols_y, int64_t nx,
-                                                 int64_t n, int nrhs, const int *__restrict__ skip,
-                                                 int skip_stride) {
-    const int64_t id = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (id >= n * nrhs) return;
-    const int64_t r = id / n;
-    if (skip && skip[r * skip_stride]) return;
-    const int64_t i = id - r * n;
-    int fx, fy;
-    if (i < nx) { fx = static_cast<int>(i % rows_x); fy = static_cast<int>(i / rows_x); }
-    else { const int64_t k = i - nx; fx = static_cast<int>(k % rows_y); fy = static_cast<int>(k / rows_y); }
-    const c128 *x = in + r * n;
-    const int st = scls[i];
-    if (st < 0) {
-        out[id] = mk(0.0, 0.0);
-        return;
-    }
-    const cplx<TC> *cf = scoef + static_cast<int64_t>(st) * kSlots;
-    cplx<TC> acc = mk<TC>(0, 0);
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
+__global__ void __launch_bounds__(kTX * kTY) sai_apply_tiled(const c128 *__restrict__ in, c128 *__restrict__ out,
+                                                              const int *__restrict__ scls,
+                                                              const cplx<TC> *__restrict__ scoef, int rows_x,
+                                                              int cols_x, int rows_y, int cols_y, int64_t nx, int64_t n,
+                                                              int tiles_x, const int *__restrict__ skip,
+                                                              int skip_stride) {
+    const int r = blockIdx.y;
+    if (skip && skip[static_cast<int64_t>(r) * skip_stride]) return;
+    constexpr int LX = kTX + 2, LY = kTY + 2;
+    __shared__ cplx<TC> tile[2][LY][LX];
+    const int X0 = (blockIdx.x % tiles_x) * kTX, Y0 = (blockIdx.x / tiles_x) * kTY;
+    const c128 *x = in + static_cast<int64_t>(r) * n;
+    for (int e = threadIdx.x; e < 2 * LY * LX; e += kTX * kTY) {
+        const int d = e / (LY * LX), rem = e - d * (LY * LX), ly = rem / LX, lx = rem - ly * LX;
+        const int gx = X0 + lx - 1, gy = Y0 + ly - 1;
         const int rows = d ? rows_y : rows_x, cols = d ? cols_y : cols_x;
-        const int64_t base = d ? nx : 0;
-#pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int gx = fx + dx, gy = fy + dy;
-                if (gx < 0 || gy < 0 || gx >= rows || gy >= cols) continue;
-                const int s = d * 9 + (dx + 1) + 3 * (dy + 1);
-                const cplx<TC> c = cf[s], v = cvt<TC>(x[base + gx + static_cast<int64_t>(rows) * gy]);
-                if (c.x != 0 || c.y != 0) acc = acc + c * v;   // an absent neighbour's value is never used
-            }
+        cplx<TC> v = mk<TC>(0, 0);
+        if (gx >= 0 && gy >= 0 && gx < rows && gy < cols)
+            v = cvt<TC>(x[(d ? nx : 0) + gx + static_cast<int64_t>(rows) * gy]);
+        tile[d][ly][lx] = v;
     }
-    out[id] = cvt<double>(acc);
+    __syncthreads();
+    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int fx = X0 + tx, fy = Y0 + ty;
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+        const int rows_i = di ? rows_y : rows_x, cols_i = di ? cols_y : cols_x;
+        if (fx >= rows_i || fy >= cols_i) continue;
+        const int64_t i = (di ? nx : 0) + fx + static_cast<int64_t>(rows_i) * fy;
+        const int st = scls[i];
+        c128 res = mk(0.0, 0.0);
+        if (st >= 0) {
+            const cplx<TC> *cf = scoef + static_cast<int64_t>(st) * kSlots;
+            cplx<TC> acc = mk<TC>(0, 0);
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int s = d * 9 + (dx + 1) + 3 * (dy + 1);
+                        const cplx<TC> c = cf[s];
+                        if (c.x != 0 || c.y != 0) acc = acc + c * tile[d][ty + 1 + dy][tx + 1 + dx];
+                    }
+            res = cvt<double>(acc);
+        }
+        out[static_cast<int64_t>(r) * n + i] = res;
+    }
 }
 
 // scoef[t][s] = (coef[ci][s] + coef[cj][s']) / 2 for the stencil class t of tuple (ci, cj_0 .. cj_17), s' the slot
@@ -453,14 +468,19 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
         timer.done();
         return;
     }
-    const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
     if (pc.class_count == 0) fail(RT_EINVAL, "rooftop:preconditioner", "the grid has no rooftop to precondition");
+    // Tiles cover the face positions 0 .. countX-1 by 0 .. countY-1 (the y rooftops' rows and the x rooftops' columns).
+    const int cx = d.rows_y > 0 ? d.rows_y : d.rows_x + 1, cy = d.cols_x > 0 ? d.cols_x : d.cols_y + 1;
+    const int tiles_x = (cx + kTX - 1) / kTX, tiles_y = (cy + kTY - 1) / kTY;
+    const dim3 grid(static_cast<unsigned>(tiles_x) * tiles_y, nrhs);
     if (prec == RT_C128)
-        launch_timed(&timer, sai_apply<double>, dim3(blocks), dim3(256), 0, s, in, out, pc.scls.ptr, pc.scoef64.ptr,
-                     d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
+        launch_timed(&timer, sai_apply_tiled<double>, grid, dim3(kTX * kTY), 0, s, in, out, pc.scls.ptr,
+                     pc.scoef64.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, skip,
+                     skip_stride);
     else
-        launch_timed(&timer, sai_apply<float>, dim3(blocks), dim3(256), 0, s, in, out, pc.scls.ptr, pc.scoef32.ptr,
-                     d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, nrhs, skip, skip_stride);
+        launch_timed(&timer, sai_apply_tiled<float>, grid, dim3(kTX * kTY), 0, s, in, out, pc.scls.ptr,
+                     pc.scoef32.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, skip,
+                     skip_stride);
     timer.done();
 }
 
