@@ -40,8 +40,8 @@ PEAK_HBM_GBS = 8000.0
 # Timed kernel slots of the library (RT_TIMED_* of include/rooftop.h) and the kernel symbols they time.
 TIMED = ['pass_a_k<double> (x FFT)', 'pass_b_k<double> (y FFT + multiply)', 'pass_c_k<double> (inverse x FFT)',
          'pass_a_k<float> (x FFT)', 'pass_b_k<float> (y FFT + multiply)', 'pass_c_k<float> (inverse x FFT)',
-         'k_cgs_dot (CGS2 h1 = V^H w)', 'k_cgs_update_dot (w -= V h1; h2 = V^H w)',
-         'k_cgs_update_norm (w -= V h2; |w|^2)', 'sai_apply<double> (near field)', 'sai_apply<float> (near field)']
+         'k_dcgs_dots ([s z] = V^H [p w])', 'k_dcgs_update (q = (p - V s)/beta; p = (w - V z)/(beta nu))',
+         'sai_apply<double> (near field)', 'sai_apply<float> (near field)']
 
 
 class KernelClock:
@@ -217,7 +217,7 @@ def serve_stream(rt, requests, solver, slots, arrivals):
 
 def cpu_baseline(n, budget_s=10.0):
     """The oracle's FP64 path (scipy FFTs on the host cores of this GPU's share, split-FFT product, sparse near-field
-    apply, GMRES(50) with CGS2 and the same tolerance) on the same plate: whole solves repeated until about budget_s
+    apply, GMRES(50) with two-pass Gram-Schmidt and the same tolerance) on the same plate: whole solves repeated until about budget_s
     seconds of CPU work (at least one), setup excluded."""
     from oracle import assembly as A
     from oracle import excitation as X

@@ -194,12 +194,11 @@ enum {
     RT_TIMED_PASS_A_C64 = 3,
     RT_TIMED_PASS_B_C64 = 4,
     RT_TIMED_PASS_C_C64 = 5,
-    RT_TIMED_CGS_DOT = 6,         /* GMRES CGS2 first pass: h1 = V^H w */
-    RT_TIMED_CGS_UPDATE_DOT = 7,  /* w -= V h1 fused with h2 = V^H w */
-    RT_TIMED_CGS_UPDATE_NORM = 8, /* w -= V h2 fused with |w|^2 */
-    RT_TIMED_PRECOND_C128 = 9,    /* preconditioner application */
-    RT_TIMED_PRECOND_C64 = 10,
-    RT_TIMED_COUNT = 11
+    RT_TIMED_GS_DOTS = 6,         /* GMRES Gram-Schmidt dot pass: [s z] = V^H [p w] and the norms of p and w */
+    RT_TIMED_GS_UPDATE = 7,       /* update pass: q = (p - V s) / beta, p = (w - V z - z_j q) / (beta nu) */
+    RT_TIMED_PRECOND_C128 = 8,    /* preconditioner application */
+    RT_TIMED_PRECOND_C64 = 9,
+    RT_TIMED_COUNT = 10
 };
 int rt_ctx_set_timing(rt_ctx *ctx, int32_t on);
 int rt_ctx_kernel_timing(rt_ctx *ctx, int64_t *launches, double *ms, double *bytes);

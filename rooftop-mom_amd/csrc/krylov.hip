@@ -1,4 +1,4 @@
-// The preconditioned Krylov loops on gfx950: COCG and restarted GMRES(m) with CGS2 on R right-hand sides at once, FP64
+// The preconditioned Krylov loops on gfx950: COCG and restarted GMRES(m) on R right-hand sides at once, FP64
 // vectors on the device, and the FP64 iterative-refinement loop around an inner solve whose products run in
 // complex64.
 //
@@ -15,13 +15,15 @@
 // -- returns at once, so each system's result is exactly that of stopping at its own step. Reductions are
 // deterministic: fixed row blocks, wave64 shuffles and a fixed-order second stage.
 //
-// CGS2 (orthogonalize_.m:15-23) runs as three row-partitioned passes over the live columns only (the reference takes
-// the whole basis with zeroed coefficients): h1 = V^H w; w -= V h1 fused with h2 = V^H w; w -= V h2 fused with |w|^2.
-// The bases are stored column by column across the systems, V[k][r][n], so column k of every system is one batch of
-// R vectors for the product.
+// The Arnoldi step orthogonalizes twice, as CGS2 does (orthogonalize_.m:15-23), but with the second pass delayed by
+// one step (DCGS2, below): two row-partitioned passes over the live columns only per step instead of three. The bases
+// are stored column by column across the systems, V[k][r][n]; the vector the product is applied to is the pending
+// FP64 vector p of every system, one batch of R vectors.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "krylov.hpp"
@@ -45,6 +47,7 @@ enum ScalSlot {
     kH2 = 0, kH1 = 64,                              // GMRES: second- and first-pass coefficients
     kRnorm = 128, kInvSub = 130, kReference = 132, kTol = 134, kBnorm = 136,
     kMaxIt = 138, kBest = 140,                      // per-system iteration limit; COCG best monitored norm
+    kAlpha = 142, kT = 143, kOmega = 144, kInvBeta = 145,   // GMRES: p^H p, p^H w, w^H w, 1 / beta
     kY = 256,                                       // GMRES: solution coefficients of a cycle
     kOuterB = 400, kOuterRes = 401
 };
@@ -52,7 +55,7 @@ enum ScalSlot {
 // Pool slots of the context's solver buffers.
 enum PoolSlot {
     kSlotPartial, kSlotScal, kSlotState, kSlotHistory, kSlotV, kSlotW, kSlotT, kSlotR, kSlotZ, kSlotD, kSlotAd,
-    kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags
+    kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags, kSlotP, kSlotHraw
 };
 
 // ------------------------------------------------------------------------------------------------------------------
@@ -159,7 +162,7 @@ __global__ void k_reset_state(LoopState *st, const int *__restrict__ finished, i
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int f = finished[r];
-    st[r] = LoopState{f, 0, f, 0, 0, {0, 0, 0}};
+    st[r] = LoopState{f, 0, f, 0, 0, 0, -1, 0};
 }
 // Marks the systems whose starting residual is already below their tolerance as converged at iteration 0.
 __global__ void k_initial_check(LoopState *st, const c128 *__restrict__ scal, int num_slot, int den_slot, int R) {
@@ -227,222 +230,306 @@ __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restri
     st[r].iterations = it;
     if (res < sc[kBest].x) {
         sc[kBest] = mk(res, 0.0);
-        st[r].pad[0] = it;
+        st[r].best = it;
     }
     if (res < sc[kTol].x) {
         st[r].converged = st[r].finished = st[r].done = 1;
     } else if (!isfinite(res)) {
         st[r].finished = st[r].done = 1;
-    } else if (it >= static_cast<int>(sc[kMaxIt].x) || (stall > 0 && it - st[r].pad[0] >= stall)) {
+    } else if (it >= static_cast<int>(sc[kMaxIt].x) || (stall > 0 && it - st[r].best >= stall)) {
         st[r].done = 1;
     }
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// GMRES kernels. Column k of the basis of system r is V + (k * R + r) * n (R = gridDim.y).
+// GMRES kernels. Column k of the basis of system r is V + (k * R + r) * n (R = gridDim.y). The basis is held in
+// complex128, or in complex64 for a complex64 inner solve (VT): its columns are the rounded Arnoldi vectors, every
+// coefficient and update is computed in FP64 from them, and the operator is applied to the FP64 vector q.
 // ------------------------------------------------------------------------------------------------------------------
-__device__ __forceinline__ const c128 *basis_col(const c128 *V, int k, int r, int R, int64_t n) {
+template <typename VT>
+__device__ __forceinline__ const VT *basis_col(const VT *V, int k, int r, int R, int64_t n) {
     return V + (static_cast<int64_t>(k) * R + r) * n;
 }
-
-// partial[(r * blocks + b) * kLdp + k] = sum over block b's rows of conj(V_r[:, k]) w_r, k < cnt <= 64. Wave q takes
-// the columns q, q + 4, ...; its lanes run over the block's rows (coalesced column reads).
-__global__ void __launch_bounds__(kThreads) k_cgs_dot(const c128 *__restrict__ V, int cnt,
-                                                      const c128 *__restrict__ w, int64_t n, int64_t chunk,
-                                                      c128 *__restrict__ partial, const LoopState *__restrict__ st) {
-    SYSTEM(r);
-    const int R = gridDim.y;
-    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-    const c128 *wr = w + r * n;
-    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
-    double re[16], im[16];
-#pragma unroll
-    for (int c = 0; c < 16; ++c) re[c] = im[c] = 0;
-    // Four rows per lane per step (chunk is a multiple of 256), so that a column's four loads are in flight together.
-    for (int64_t i0 = base + lane; i0 < end; i0 += 256) {
-        c128 wi[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) wi[u] = i0 + 64 * u < end ? wr[i0 + 64 * u] : mk(0.0, 0.0);
-#pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const int k = q + 4 * c;
-            if (k < cnt) {
-                const c128 *col = basis_col(V, k, r, R, n);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (i0 + 64 * u < end) {
-                        const c128 v = col[i0 + 64 * u];
-                        re[c] += v.x * wi[u].x + v.y * wi[u].y;
-                        im[c] += v.x * wi[u].y - v.y * wi[u].x;
-                    }
-                }
-            }
-        }
-    }
-    c128 *out = partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const int k = q + 4 * c;
-        if (k < cnt) {
-            const double a = wave_sum(re[c]), b = wave_sum(im[c]);
-            if (lane == 0) out[k] = mk(a, b);
-        }
-    }
+__device__ __forceinline__ c128 widen(c128 v) { return v; }
+__device__ __forceinline__ c128 widen(c64 v) { return mk(static_cast<double>(v.x), static_cast<double>(v.y)); }
+template <typename VT>
+__device__ __forceinline__ VT narrow(c128 v) {
+    if constexpr (sizeof(VT) == sizeof(c128)) return v;
+    else return mk(static_cast<float>(v.x), static_cast<float>(v.y));
 }
 
-// w_r -= V_r h1, then partial = conj(V_r[:, k]) w_r over the block's rows. The live columns of a tile of 64 rows are
-// staged in LDS once (row stride 65 keeps both access patterns off shared banks) and serve both the update and the
-// dot, so the basis crosses HBM once. Wave q sums the columns k = q mod 4 of the update and the rows i = q mod 4 of
-// the dot; the four partial results meet in LDS. Dynamic LDS: cnt * 65 complex128.
+// ------------------------------------------------------------------------------------------------------------------
+// Arnoldi with delayed reorthogonalization (DCGS2: Swirydowicz, Langou, Ananthan, Yang and Thomas, Numer. Linear
+// Algebra Appl. 2021; Bielich, Langou, Thomas, Swirydowicz, Yamazaki and Boman, Parallel Computing 2022). At step j
+// the basis holds the final orthonormal columns q_0 .. q_{j-1}, and the FP64 vector p the once-orthogonalized,
+// approximately normalized next one. One pass over the basis forms the inner products of p and of w = M A p,
+//   s = V^H p, z = V^H w, alpha = p^H p, t = p^H w, omega = w^H w,
+// from which (V = [q_0 .. q_{j-1}], H the Hessenberg matrix of the final columns)
+//   beta = sqrt(alpha - |s|^2), q_j = (p - V s) / beta        the reorthogonalization of p, one step late,
+//   z_j = (t - s^H z) / beta, h_j = (z - H s) / beta          column j of H (z extended by z_j),
+//   nu = sqrt(omega - |z|^2) / beta, p' = (w - V z - z_j q_j) / (beta nu),
+// and column j - 1 of H becomes final: + nu_{j-1} s above the diagonal, subdiagonal nu_{j-1} beta. A second pass
+// writes q_j and p'. Two passes over the basis and one reduction per step, against the three passes and two
+// reductions of CGS2, with the same orthogonality.
+// ------------------------------------------------------------------------------------------------------------------
 constexpr int kTileRows = 64;
 constexpr int kTileStride = 65;
-__global__ void __launch_bounds__(kThreads) k_cgs_update_dot(const c128 *__restrict__ V, int cnt,
-                                                             const c128 *__restrict__ scal, c128 *__restrict__ w,
-                                                             int64_t n, int64_t chunk, c128 *__restrict__ partial,
-                                                             const LoopState *__restrict__ st) {
+constexpr int kLdp2 = 128;                           // partial slots per block of the dot pass
+constexpr int kPartZ = 64, kPartScalars = 125;       // s at k, z at kPartZ + k, then alpha, t, omega
+
+// partial[(r * blocks + b) * kLdp2 + ...] = the inner products above over block b's rows. The live columns of a tile of
+// 64 rows are staged in LDS with p and w (row stride 65 keeps the column-wise reads off shared banks); lane k of wave
+// q sums column k over the rows i = q mod 4, wave 0 sums alpha, t and omega. Dynamic LDS: cnt * 65 basis elements.
+template <typename VT>
+__global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V, int cnt, const c128 *__restrict__ p,
+                                                        const c128 *__restrict__ w, int64_t n, int64_t chunk,
+                                                        c128 *__restrict__ partial, const LoopState *__restrict__ st) {
     SYSTEM(r);
     const int R = gridDim.y;
-    extern __shared__ c128 tile[];   // [cnt][kTileStride]
-    __shared__ c128 hs[kLdp];
-    __shared__ c128 red[4 * kTileRows];
-    __shared__ c128 wt[kTileRows];
-    for (int k = threadIdx.x; k < cnt; k += kThreads) hs[k] = scal[r * kScal + kH1 + k];
+    extern __shared__ unsigned char tile_raw[];
+    VT *tile = reinterpret_cast<VT *>(tile_raw);   // [cnt][kTileStride]
+    __shared__ c128 pt[kTileRows], wt[kTileRows];
+    __shared__ c128 red[2][4 * kTileRows];
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-    c128 *wr = w + r * n;
+    const c128 *pr = p + r * n, *wr = w + r * n;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
-    double dre = 0, dim = 0;
+    double sre = 0, sim = 0, zre = 0, zim = 0;   // column `lane`
+    double al = 0, tre = 0, tim = 0, om = 0;     // wave 0: row `lane` of every tile
     constexpr int kPer = kLdp * kTileRows / kThreads;
     for (int64_t t0 = base; t0 < end; t0 += kTileRows) {
         const int rows = static_cast<int>(min(static_cast<int64_t>(kTileRows), end - t0));
-        // The tile's loads are all issued before any is stored (cnt <= 64 columns: at most 16 per thread), so the
-        // staging costs one memory latency, not one per column.
-        c128 vals[kPer];
+        // All loads of the tile are issued before any is stored: the staging costs one memory latency.
+        VT vals[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int e = threadIdx.x + u * kThreads;
             const int k = e >> 6, i = e & (kTileRows - 1);
-            vals[u] = (k < cnt && i < rows) ? basis_col(V, k, r, R, n)[t0 + i] : mk(0.0, 0.0);
+            vals[u] = (k < cnt && i < rows) ? basis_col(V, k, r, R, n)[t0 + i] : VT{0, 0};
         }
-        __syncthreads();   // the previous tile is consumed (and hs is written, first time)
+        c128 pv = mk(0.0, 0.0), wv = mk(0.0, 0.0);
+        if (q == 0 && lane < rows) {
+            pv = pr[t0 + lane];
+            wv = wr[t0 + lane];
+        }
+        __syncthreads();   // the previous tile is consumed
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int e = threadIdx.x + u * kThreads;
             const int k = e >> 6, i = e & (kTileRows - 1);
             if (k < cnt) tile[k * kTileStride + i] = vals[u];
         }
-        __syncthreads();
-        c128 acc = mk(0.0, 0.0);
-        for (int k = q; k < cnt; k += 4) acc = acc + hs[k] * tile[k * kTileStride + lane];
-        red[q * kTileRows + lane] = acc;
-        __syncthreads();
         if (q == 0) {
-            c128 v = mk(0.0, 0.0);
-            if (lane < rows) {
-                const c128 sum = (red[lane] + red[kTileRows + lane]) + (red[2 * kTileRows + lane] + red[3 * kTileRows + lane]);
-                v = wr[t0 + lane] - sum;
-                wr[t0 + lane] = v;
-            }
-            wt[lane] = v;
+            pt[lane] = pv;
+            wt[lane] = wv;
+            const c128 pw = cmulc(pv, wv);
+            al += abs2(pv);
+            tre += pw.x;
+            tim += pw.y;
+            om += abs2(wv);
         }
         __syncthreads();
         if (lane < cnt) {
-            const c128 *col = tile + lane * kTileStride;
+            const VT *col = tile + lane * kTileStride;
             for (int i = q; i < rows; i += 4) {
-                const c128 vv = col[i], wi = wt[i];
-                dre += vv.x * wi.x + vv.y * wi.y;
-                dim += vv.x * wi.y - vv.y * wi.x;
+                const c128 v = widen(col[i]), pi = pt[i], wi = wt[i];
+                sre += v.x * pi.x + v.y * pi.y;
+                sim += v.x * pi.y - v.y * pi.x;
+                zre += v.x * wi.x + v.y * wi.y;
+                zim += v.x * wi.y - v.y * wi.x;
             }
         }
     }
+    red[0][q * kTileRows + lane] = mk(sre, sim);
+    red[1][q * kTileRows + lane] = mk(zre, zim);
+    if (q == 0) {
+        al = wave_sum(al);
+        tre = wave_sum(tre);
+        tim = wave_sum(tim);
+        om = wave_sum(om);
+    }
     __syncthreads();
-    red[q * kTileRows + lane] = mk(dre, dim);
-    __syncthreads();
-    if (q == 0 && lane < cnt) {
-        const c128 sum = (red[lane] + red[kTileRows + lane]) + (red[2 * kTileRows + lane] + red[3 * kTileRows + lane]);
-        partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp + lane] = sum;
+    c128 *out = partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp2;
+    if (q < 2 && lane < cnt) {
+        const c128 *v = red[q];
+        out[q * kPartZ + lane] = (v[lane] + v[kTileRows + lane]) + (v[2 * kTileRows + lane] + v[3 * kTileRows + lane]);
+    }
+    if (threadIdx.x == 0) {
+        out[kPartScalars] = mk(al, 0.0);
+        out[kPartScalars + 1] = mk(tre, tim);
+        out[kPartScalars + 2] = mk(om, 0.0);
     }
 }
 
-// w_r -= V_r h2; partial = |w_r|^2 over the block's rows.
-__global__ void __launch_bounds__(kThreads) k_cgs_update_norm(const c128 *__restrict__ V, int cnt,
-                                                              const c128 *__restrict__ scal, c128 *__restrict__ w,
-                                                              int64_t n, int64_t chunk, c128 *__restrict__ partial,
-                                                              const LoopState *__restrict__ st) {
+// The dot pass's sums over the blocks: block x of the grid (2 cnt + 3, R) gives s_x (x < cnt) into kH2 + x, z into
+// kH1, then alpha, t and omega into kAlpha, kT, kOmega.
+__global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict__ partial, int blocks, int cnt,
+                                                          c128 *__restrict__ scal, const LoopState *__restrict__ st) {
     SYSTEM(r);
-    const int R = gridDim.y;
-    __shared__ c128 hs[kLdp];
     __shared__ double sh[8];
-    for (int k = threadIdx.x; k < cnt; k += kThreads) hs[k] = scal[r * kScal + kH2 + k];
-    __syncthreads();
-    c128 *wr = w + r * n;
-    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
-    double s = 0;
-    for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
-        c128 v = wr[i];
-        for (int k0 = 0; k0 < cnt; k0 += 8) {   // eight columns' loads in flight at once, the same order of terms
-            c128 vv[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) vv[u] = k0 + u < cnt ? basis_col(V, k0 + u, r, R, n)[i] : mk(0.0, 0.0);
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (k0 + u < cnt) v = v - hs[k0 + u] * vv[u];
-        }
-        wr[i] = v;
-        s += abs2(v);
+    const int x = blockIdx.x;
+    int src, dst;
+    if (x < cnt) {
+        src = x;
+        dst = kH2 + x;
+    } else if (x < 2 * cnt) {
+        src = kPartZ + x - cnt;
+        dst = kH1 + x - cnt;
+    } else {
+        src = kPartScalars + x - 2 * cnt;
+        dst = kAlpha + x - 2 * cnt;
     }
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[(static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp] = mk(s, 0.0);
+    double re = 0, im = 0;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) {
+        const c128 v = partial[(static_cast<int64_t>(r) * blocks + b) * kLdp2 + src];
+        re += v.x;
+        im += v.y;
+    }
+    re = block_sum(re, sh);
+    im = block_sum(im, sh);
+    if (threadIdx.x == 0) scal[r * kScal + dst] = mk(re, im);
 }
 
-// Step j (0-based) of the cycle, iteration `it` overall: the new Hessenberg column h1 + h2 and |w|, the previous
-// rotations, the new rotation, the rotated right-hand side g, the monitored residual |g(j+1)| / reference and the
-// stopping test (iterate_.m:252-278); inv_sub = 1 / H(j+1, j) for the next basis vector. Grid (1, R).
-__global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restrict__ partial, int blocks,
-                                                             c128 *__restrict__ scal, c128 *__restrict__ Hall,
-                                                             c128 *__restrict__ gall, int m, int j, int it,
-                                                             LoopState *st, double *history, int cap) {
-    SYSTEM(r);
-    __shared__ double sh[8];
-    __shared__ c128 hcol[kLdp + 1], csv[kLdp], snv[kLdp];
-    c128 *sc = scal + r * kScal;
-    const int ldh = m + 1;
-    c128 *col = Hall + static_cast<int64_t>(r) * ldh * m + static_cast<int64_t>(ldh) * j;
-    c128 *g = gall + static_cast<int64_t>(r) * 3 * ldh, *cs = g + ldh, *sn = g + 2 * ldh;
-    // Stage the new column and the previous rotations with all threads; thread 0 then rotates in LDS.
-    for (int k = threadIdx.x; k <= j; k += kThreads) hcol[k] = sc[kH1 + k] + sc[kH2 + k];
-    for (int k = threadIdx.x; k < j; k += kThreads) {
-        csv[k] = cs[k];
-        snv[k] = sn[k];
-    }
-    double s = 0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[(static_cast<int64_t>(r) * blocks + b) * kLdp].x;
-    s = block_sum(s, sh);   // synchronizes the block
-    if (threadIdx.x != 0) return;
-    const double sub = sqrt(s);
-    hcol[j + 1] = mk(sub, 0.0);
+// Applies the rotations 0 .. j-1 to col[0 .. j] (LDS) and forms rotation j from col[j] and the subdiagonal sub;
+// col[j] becomes the rotated diagonal.
+__device__ __forceinline__ void rotate_column(c128 *col, int j, double sub, c128 *csv, c128 *snv) {
     for (int k = 0; k < j; ++k) {
-        const c128 a0 = hcol[k], a1 = hcol[k + 1];
-        hcol[k] = cmulc(csv[k], a0) + cmulc(snv[k], a1);
-        hcol[k + 1] = csv[k] * a1 - snv[k] * a0;
+        const c128 a0 = col[k], a1 = col[k + 1];
+        col[k] = cmulc(csv[k], a0) + cmulc(snv[k], a1);
+        col[k + 1] = csv[k] * a1 - snv[k] * a0;
     }
-    for (int k = 0; k < j; ++k) col[k] = hcol[k];
-    const c128 a = hcol[j];
+    const c128 a = col[j];
     const double radius = sqrt(abs2(a) + sub * sub);
     c128 c = mk(1.0, 0.0), sv = mk(0.0, 0.0);
     if (radius > 0) {
         c = (1.0 / radius) * a;
         sv = mk(sub / radius, 0.0);
     }
-    cs[j] = c;
-    sn[j] = sv;
+    csv[j] = c;
+    snv[j] = sv;
     col[j] = cmulc(c, a) + cmulc(sv, mk(sub, 0.0));
-    col[j + 1] = mk(0.0, 0.0);
-    g[j + 1] = mk(0.0, 0.0) - sv * g[j];
-    g[j] = cmulc(c, g[j]);
-    const double res = sqrt(abs2(g[j + 1])) / sc[kReference].x;
+}
+
+// Step j (0-based) of the cycle, iteration `it` overall: beta and z_j; column j - 1 of H in its final form, its
+// rotation formed again and the monitored residual of iteration it - 1 refined; column j with the subdiagonal nu
+// (final one step later), its rotation, the rotated right-hand side g, the monitored residual |g(j+1)| / reference
+// and the stopping test (iterate_.m:252-278); 1 / beta and 1 / (beta nu) for the update pass. H holds the columns as
+// formed (Hraw, for H s) and rotated (Hall, for the triangular solve); gall holds g, the cosines, the sines and g
+// before each rotation. Grid (1, R).
+__global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__ scal, c128 *__restrict__ Hraw,
+                                                              c128 *__restrict__ Hall, c128 *__restrict__ gall, int m,
+                                                              int j, int it, LoopState *st, double *history, int cap) {
+    SYSTEM(r);
+    __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp + 1], coln[kLdp + 2], csv[kLdp], snv[kLdp];
+    __shared__ double shb[2];
+    c128 *sc = scal + r * kScal;
+    const int ldh = m + 1;
+    c128 *Hr = Hraw + static_cast<int64_t>(r) * ldh * m, *Hc = Hall + static_cast<int64_t>(r) * ldh * m;
+    c128 *g = gall + static_cast<int64_t>(r) * 4 * ldh, *cs = g + ldh, *sn = g + 2 * ldh, *gs = g + 3 * ldh;
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < j; k += kThreads) {
+        s[k] = sc[kH2 + k];
+        z[k] = sc[kH1 + k];
+        csv[k] = cs[k];
+        snv[k] = sn[k];
+    }
+    if (j >= 1)
+        for (int k = threadIdx.x; k < j; k += kThreads) colp[k] = Hr[static_cast<int64_t>(j - 1) * ldh + k];
+    __syncthreads();
+    if (q == 0) {   // |s|^2 and s^H z
+        double ss = 0, szr = 0, szi = 0;
+        for (int k = lane; k < j; k += 64) {
+            ss += abs2(s[k]);
+            const c128 v = cmulc(s[k], z[k]);
+            szr += v.x;
+            szi += v.y;
+        }
+        ss = wave_sum(ss);
+        szr = wave_sum(szr);
+        szi = wave_sum(szi);
+        if (lane == 0) {
+            const double b2 = sc[kAlpha].x - ss;
+            const double beta = b2 > 0 && isfinite(b2) ? sqrt(b2) : 0.0;
+            shb[0] = beta;
+            z[j] = beta > 0 ? (1.0 / beta) * (sc[kT] - mk(szr, szi)) : mk(0.0, 0.0);
+            if (j >= 1) {   // column j - 1 final: + nu_{j-1} s, subdiagonal nu_{j-1} beta
+                const double nu = Hr[static_cast<int64_t>(j - 1) * ldh + j].x;
+                shb[1] = nu * beta;
+            }
+        }
+    }
+    __syncthreads();
+    const double beta = shb[0];
+    if (!(beta > 0)) {   // p vanished or is not finite: the system stops without a new column
+        if (threadIdx.x == 0) st[r].finished = st[r].done = 1;
+        return;
+    }
+    if (j >= 1) {
+        const double nu = Hr[static_cast<int64_t>(j - 1) * ldh + j].x;
+        for (int k = threadIdx.x; k < j; k += kThreads) {
+            colp[k] = colp[k] + nu * s[k];
+            Hr[static_cast<int64_t>(j - 1) * ldh + k] = colp[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && j >= 1) {
+        colp[j] = mk(shb[1], 0.0);
+        Hr[static_cast<int64_t>(j - 1) * ldh + j] = colp[j];
+    }
+    // (H s)_k over the final columns l < j, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4. Column
+    // j - 1 comes from LDS, the earlier ones from Hraw.
+    for (int k = lane; k <= j; k += 64) {
+        c128 acc = mk(0.0, 0.0);
+        for (int l = max(0, k - 1) + ((q - max(0, k - 1)) & 3); l < j; l += 4) {
+            const c128 h = l == j - 1 ? (k < j ? colp[k] : mk(shb[1], 0.0)) : Hr[static_cast<int64_t>(l) * ldh + k];
+            acc = acc + h * s[l];
+        }
+        part[q][k] = acc;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= j; k += kThreads) {
+        const c128 hs = (part[0][k] + part[1][k]) + (part[2][k] + part[3][k]);
+        coln[k] = (1.0 / beta) * (z[k] - hs);
+        Hr[static_cast<int64_t>(j) * ldh + k] = coln[k];
+    }
+    __syncthreads();
+    if (q == 0) {   // |z|^2 (z extended by z_j)
+        double zz = 0;
+        for (int k = lane; k <= j; k += 64) zz += abs2(z[k]);
+        zz = wave_sum(zz);
+        if (lane == 0) {
+            const double n2 = sc[kOmega].x - zz;
+            shb[1] = n2 > 0 ? sqrt(n2) / beta : 0.0;   // nu; 0: w lies in the span (breakdown)
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double nu = shb[1];
+    sc[kH1 + j] = z[j];
+    Hr[static_cast<int64_t>(j) * ldh + j + 1] = mk(nu, 0.0);
+    const double ref = sc[kReference].x;
+    if (j >= 1) {
+        rotate_column(colp, j - 1, colp[j].x, csv, snv);
+        for (int k = 0; k < j; ++k) Hc[static_cast<int64_t>(j - 1) * ldh + k] = colp[k];
+        cs[j - 1] = csv[j - 1];
+        sn[j - 1] = snv[j - 1];
+        const c128 g0 = gs[j - 1];
+        g[j - 1] = cmulc(csv[j - 1], g0);
+        g[j] = mk(0.0, 0.0) - snv[j - 1] * g0;
+        history[static_cast<int64_t>(r) * cap + it - 1] = sqrt(abs2(g[j])) / ref;
+    }
+    rotate_column(coln, j, nu, csv, snv);
+    for (int k = 0; k <= j; ++k) Hc[static_cast<int64_t>(j) * ldh + k] = coln[k];
+    cs[j] = csv[j];
+    sn[j] = snv[j];
+    const c128 g0 = g[j];
+    gs[j] = g0;
+    g[j + 1] = mk(0.0, 0.0) - snv[j] * g0;
+    g[j] = cmulc(csv[j], g0);
+    const double res = sqrt(abs2(g[j + 1])) / ref;
     history[static_cast<int64_t>(r) * cap + it] = res;
     st[r].steps = j + 1;
     st[r].iterations = it;
+    st[r].tag = it;
     const bool conv = res < sc[kTol].x;
     if (conv) {
         st[r].converged = st[r].finished = st[r].done = 1;
@@ -451,29 +538,63 @@ __global__ void __launch_bounds__(kThreads) k_gmres_finalize(const c128 *__restr
     } else if (it >= static_cast<int>(sc[kMaxIt].x)) {
         st[r].done = 1;
     }
-    sc[kInvSub] = (!conv && sub != 0) ? mk(1.0 / sub, 0.0) : mk(0.0, 0.0);
+    sc[kInvBeta] = mk(1.0 / beta, 0.0);
+    sc[kInvSub] = (!conv && nu > 0) ? mk(1.0 / (beta * nu), 0.0) : mk(0.0, 0.0);
 }
 
-// v_{col} = w * inv_sub.
-__global__ void k_next_basis(const c128 *__restrict__ w, c128 *__restrict__ V, int col, int64_t n,
-                             const c128 *__restrict__ scal, const LoopState *__restrict__ st) {
-    SYSTEM(r);
-    const c128 f = scal[r * kScal + kInvSub];
+// The update pass: q_j = (p - V s) / beta into column j of the basis, p = (w - V z - z_j q_j) / (beta nu). Runs for
+// the systems whose finalize ran in iteration `it`, the ones that stopped in it included (q_j enters their solution).
+template <typename VT>
+__global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, int cnt, c128 *__restrict__ p,
+                                                          const c128 *__restrict__ w, int64_t n, int64_t chunk,
+                                                          const c128 *__restrict__ scal, int it,
+                                                          const LoopState *__restrict__ st) {
+    const int r = blockIdx.y;
+    if (st[r].tag != it) return;
+    const int R = gridDim.y;
+    __shared__ c128 hs[kLdp], hz[kLdp];
+    const c128 *sc = scal + r * kScal;
+    for (int k = threadIdx.x; k < cnt; k += kThreads) {
+        hs[k] = sc[kH2 + k];
+        hz[k] = sc[kH1 + k];
+    }
+    __syncthreads();
+    const double ib = sc[kInvBeta].x, isub = sc[kInvSub].x;
+    const c128 zj = sc[kH1 + cnt];
+    VT *vj = V + (static_cast<int64_t>(cnt) * R + r) * n;
+    c128 *pr = p + r * n;
     const c128 *wr = w + r * n;
-    c128 *v = V + (static_cast<int64_t>(col) * gridDim.y + r) * n;
-    ROWS(i, n) v[i] = f * wr[i];
+    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
+        c128 as = mk(0.0, 0.0), az = mk(0.0, 0.0);
+        for (int k0 = 0; k0 < cnt; k0 += 8) {   // eight columns' loads in flight at once
+            VT vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vv[u] = k0 + u < cnt ? basis_col(V, k0 + u, r, R, n)[i] : VT{0, 0};
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k0 + u < cnt) {
+                    const c128 v = widen(vv[u]);
+                    as = as + hs[k0 + u] * v;
+                    az = az + hz[k0 + u] * v;
+                }
+        }
+        const c128 qj = ib * (pr[i] - as);
+        vj[i] = narrow<VT>(qj);
+        pr[i] = isub * (wr[i] - az - zj * qj);
+    }
 }
 
-// Start of a cycle: v_0 = w / |w|, g = (|w|, 0, ..., 0).
-__global__ void k_gmres_start(const c128 *__restrict__ w, c128 *__restrict__ V, int m1, int64_t n,
+// Start of a cycle: p = w / |w|, g = (|w|, 0, ..., 0).
+__global__ void k_gmres_start(const c128 *__restrict__ w, c128 *__restrict__ p, int m1, int64_t n,
                               const c128 *__restrict__ scal, c128 *__restrict__ gall, const LoopState *__restrict__ st) {
     SYSTEM(r);
     const double rn = scal[r * kScal + kRnorm].x;
     const c128 *wr = w + r * n;
-    c128 *v = V + static_cast<int64_t>(r) * n;
-    ROWS(i, n) v[i] = (1.0 / rn) * wr[i];
+    c128 *pr = p + r * n;
+    ROWS(i, n) pr[i] = (1.0 / rn) * wr[i];
     if (blockIdx.x == 0) {
-        c128 *g = gall + static_cast<int64_t>(r) * 3 * m1;
+        c128 *g = gall + static_cast<int64_t>(r) * 4 * m1;
         for (int k = threadIdx.x; k < m1; k += blockDim.x) g[k] = mk(k == 0 ? rn : 0.0, 0.0);
     }
 }
@@ -494,7 +615,7 @@ __global__ void k_gmres_solve(const c128 *__restrict__ Hall, const c128 *__restr
     const int steps = st[r].steps;
     const int ldh = m + 1;
     const c128 *H = Hall + static_cast<int64_t>(r) * ldh * m;
-    const c128 *g = gall + static_cast<int64_t>(r) * 3 * ldh;
+    const c128 *g = gall + static_cast<int64_t>(r) * 4 * ldh;
     c128 *y = scal + r * kScal + kY;
     for (int k = steps - 1; k >= 0; --k) {
         c128 acc = g[k];
@@ -505,7 +626,8 @@ __global__ void k_gmres_solve(const c128 *__restrict__ Hall, const c128 *__restr
 }
 
 // x_r += V_r[:, 0:steps] y_r. Grid (blocks, R); no-op at steps = 0.
-__global__ void k_gemv_add(const c128 *__restrict__ V, const LoopState *st, const c128 *__restrict__ scal,
+template <typename VT>
+__global__ void k_gemv_add(const VT *__restrict__ V, const LoopState *st, const c128 *__restrict__ scal,
                            c128 *__restrict__ x, int64_t n) {
     __shared__ c128 ys[kLdp];
     const int r = blockIdx.y, R = gridDim.y;
@@ -516,7 +638,7 @@ __global__ void k_gemv_add(const c128 *__restrict__ V, const LoopState *st, cons
     c128 *xr = x + r * n;
     ROWS(i, n) {
         c128 v = xr[i];
-        for (int k = 0; k < cnt; ++k) v = v + ys[k] * basis_col(V, k, r, R, n)[i];
+        for (int k = 0; k < cnt; ++k) v = v + ys[k] * widen(basis_col(V, k, r, R, n)[i]);
         xr[i] = v;
     }
 }
@@ -529,11 +651,12 @@ static unsigned vec_blocks(int64_t n) { return blocks_for(n, 256, 1024); }
 // Raises a kernel's dynamic LDS limit (the static LDS of the kernel counts against the same 64 KiB default).
 static void raise_lds(const void *kernel, size_t dynamic_bytes) {
     static std::mutex lock;
-    static size_t raised = 0;
+    static std::map<const void *, size_t> raised;
     std::lock_guard<std::mutex> g(lock);
-    if (dynamic_bytes <= raised) return;
+    size_t &have = raised[kernel];
+    if (dynamic_bytes <= have) return;
     RT_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(dynamic_bytes)));
-    raised = dynamic_bytes;
+    have = dynamic_bytes;
 }
 
 static c128 *vec(Context *ctx, int slot, int64_t count) {
@@ -548,7 +671,7 @@ Workspace::Workspace(Context *c, int64_t n_, int R_, int max_iter) : ctx(c), s(c
     const int64_t per = (n + b - 1) / b;
     chunk = (per + kThreads - 1) / kThreads * kThreads;
     blocks = static_cast<int>((n + chunk - 1) / chunk);
-    partial = vec(c, kSlotPartial, static_cast<int64_t>(R) * blocks * kLdp);
+    partial = vec(c, kSlotPartial, static_cast<int64_t>(R) * blocks * kLdp2);
     scal = vec(c, kSlotScal, static_cast<int64_t>(R) * kScal);
     state = static_cast<LoopState *>(c->scratch(kSlotState, sizeof(LoopState) * R, "the solver's state"));
     cap = max_iter + 2;
@@ -752,9 +875,11 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
 // ------------------------------------------------------------------------------------------------------------------
 // GMRES(m), left preconditioned, CGS2 (iterate_.m:1-85), on the systems not marked `skip_sys`.
 // ------------------------------------------------------------------------------------------------------------------
+template <typename VT>
 static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c128 *b, c128 *x, bool has_x0,
                          const std::vector<double> &tol, const std::vector<char> &skip_sys,
                          const std::vector<int> &limits, int restart) {
+    constexpr bool low = sizeof(VT) == sizeof(c64);
     Context *ctx = ws.ctx;
     const int64_t n = ws.n;
     const int R = ws.R;
@@ -765,10 +890,12 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
     const int m = static_cast<int>(m64);
     if (m > 62) fail(RT_EINVAL, "rooftopMoM:solver:invalidRestart", "restarts above 62 are not supported on the GPU");
     const int m1 = m + 1;
-    c128 *V = vec(ctx, kSlotV, static_cast<int64_t>(R) * m1 * n);
+    VT *V = static_cast<VT *>(ctx->scratch(kSlotV, sizeof(VT) * R * m * n, "the solver's vectors"));
+    c128 *pv = vec(ctx, kSlotP, R * n);   // the pending vector p (FP64), the one the operator is applied to
     c128 *w = vec(ctx, kSlotW, R * n), *t = vec(ctx, kSlotT, R * n);
     c128 *H = vec(ctx, kSlotH, static_cast<int64_t>(R) * m1 * m);
-    c128 *g = vec(ctx, kSlotG, static_cast<int64_t>(R) * 3 * m1);
+    c128 *Hr = vec(ctx, kSlotHraw, static_cast<int64_t>(R) * m1 * m);
+    c128 *g = vec(ctx, kSlotG, static_cast<int64_t>(R) * 4 * m1);
     auto precondition = [&](const c128 *in, c128 *out) {
         if (P) (*P)(in, out, ws.skip());
         else ws.copy(in, out, st);
@@ -801,7 +928,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
     while (it < max_iter && any_active()) {
         hipLaunchKernelGGL(k_cycle_reset, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, R);
         RT_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gmres_start, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, V, m1, n, ws.scal, g, st);
+        hipLaunchKernelGGL(k_gmres_start, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, pv, m1, n, ws.scal, g, st);
         RT_LAUNCH_CHECK();
         int step = 0;
         while (step < m && it < max_iter) {
@@ -810,36 +937,26 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 const int j = step;
                 ++step;
                 ++it;
-                A(V + static_cast<int64_t>(j) * R * n, t, ws.skip());   // column j of every system
+                A(pv, t, ws.skip());
                 precondition(t, w);
-                // Algorithmic bytes of the CGS2 passes: the live basis columns once, w read (and written).
-                const double col = 16.0 * n * R;
-                KernelTimer t1(ctx, RT_TIMED_CGS_DOT, (step + 1) * col);
-                launch_timed(&t1, k_cgs_dot, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, w, n, ws.chunk,
-                             ws.partial, st);
+                // Algorithmic bytes of the two passes: the live basis columns once, p and w read, q_j and p written.
+                const double col = static_cast<double>(sizeof(VT)) * n * R, wb = 16.0 * n * R;
+                KernelTimer t1(ctx, RT_TIMED_GS_DOTS, j * col + 2 * wb);
+                const size_t tile_lds = sizeof(VT) * j * kTileStride;
+                if (tile_lds > 32 * 1024) raise_lds(reinterpret_cast<const void *>(&k_dcgs_dots<VT>), tile_lds);
+                launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.blocks, R), dim3(kThreads), static_cast<uint32_t>(tile_lds),
+                             ws.s, V, j, pv, w, n, ws.chunk, ws.partial, st);
                 t1.done();
-                hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
-                                   kH1, 0, st);
+                hipLaunchKernelGGL(k_dcgs_reduce, dim3(2 * j + 3, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
+                                   j, ws.scal, st);
                 RT_LAUNCH_CHECK();
-                KernelTimer t2(ctx, RT_TIMED_CGS_UPDATE_DOT, (step + 2) * col);
-                const size_t tile_lds = sizeof(c128) * step * kTileStride;
-                if (tile_lds > 32 * 1024) raise_lds(reinterpret_cast<const void *>(&k_cgs_update_dot), tile_lds);
-                launch_timed(&t2, k_cgs_update_dot, dim3(ws.blocks, R), dim3(kThreads),
-                             static_cast<uint32_t>(tile_lds), ws.s, V, step, ws.scal, w, n, ws.chunk, ws.partial, st);
+                hipLaunchKernelGGL(k_dgmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.scal, Hr, H, g, m, j, it,
+                                   ws.state, ws.history, ws.cap);
+                RT_LAUNCH_CHECK();
+                KernelTimer t2(ctx, RT_TIMED_GS_UPDATE, j * col + 3 * wb + col);
+                launch_timed(&t2, k_dcgs_update<VT>, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, j, pv, w, n,
+                             ws.chunk, ws.scal, it, st);
                 t2.done();
-                hipLaunchKernelGGL(k_reduce, dim3(step, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
-                                   kH2, 0, st);
-                RT_LAUNCH_CHECK();
-                KernelTimer t3(ctx, RT_TIMED_CGS_UPDATE_NORM, (step + 2) * col);
-                launch_timed(&t3, k_cgs_update_norm, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, step, ws.scal,
-                             w, n, ws.chunk, ws.partial, st);
-                t3.done();
-                hipLaunchKernelGGL(k_gmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
-                                   ws.scal, H, g, m, j, it, ws.state, ws.history, ws.cap);
-                RT_LAUNCH_CHECK();
-                hipLaunchKernelGGL(k_next_basis, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, V, step, n, ws.scal,
-                                   st);
-                RT_LAUNCH_CHECK();
             }
             state = ws.read_state();
             ws.set_scale(state);
@@ -848,7 +965,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
         // x += V y with y = H \ g over the steps of this cycle (systems idle this cycle have 0 steps).
         hipLaunchKernelGGL(k_gmres_solve, dim3(1, R), dim3(64), 0, ws.s, H, g, m, ws.state, ws.scal);
         RT_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gemv_add, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, V, ws.state, ws.scal, x, n);
+        hipLaunchKernelGGL(k_gemv_add<VT>, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, V, ws.state, ws.scal, x, n);
         RT_LAUNCH_CHECK();
         state = ws.read_state();
         if (it >= max_iter || !any_active()) break;
@@ -906,7 +1023,9 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
     auto run = [&](const c128 *rhs, c128 *sol, bool has_x0, const std::vector<double> &tol,
                    const std::vector<char> &skip_sys, const std::vector<int> &limits, int stall) {
         if (o.method == RT_METHOD_CG) return cocg(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, stall);
-        return gmres(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
+        // A complex64 inner GMRES keeps its basis in complex64: half the basis traffic of the Gram-Schmidt passes.
+        if (inner_prec == RT_C64) return gmres<c64>(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
+        return gmres<c128>(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
     };
     if (!any) {
         ws.zero(x);

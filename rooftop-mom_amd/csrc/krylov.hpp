@@ -38,7 +38,9 @@ struct LoopState {
     int finished;     // converged or broke down: stays done across GMRES cycles
     int iterations;   // iterations completed
     int steps;        // GMRES: steps of the current cycle completed
-    int pad[3];
+    int best;         // COCG: iteration of the best monitored norm
+    int tag;          // GMRES: the last iteration whose finalize ran for this system
+    int pad;
 };
 constexpr int kStateInts = sizeof(LoopState) / sizeof(int);
 constexpr int kScal = 512;   // device scalar slots per system

@@ -72,10 +72,14 @@ using ic = std::integral_constant<int, I>;
 // The same pointer, opaque to the compiler: loads through it are not merged with earlier loads of the same
 // addresses. The inverse rounds of pass B read the forward rounds' twiddles again through it instead of keeping
 // them in registers across the multiply, which would cost occupancy.
+// The pointer goes through the asm as a global-memory pointer, so the loads stay global loads (a flat load would
+// also count against the LDS wait counter).
 template <typename T>
 __device__ __forceinline__ const T *opaque(const T *p) {
-    asm volatile("" : "+s"(p));
-    return p;
+    using G = const T __attribute__((address_space(1))) *;
+    G g = (G)p;
+    asm volatile("" : "+s"(g));
+    return (const T *)g;
 }
 
 template <int B, int E, typename F>
