@@ -671,7 +671,13 @@ Workspace::Workspace(Context *c, int64_t n_, int R_, int max_iter) : ctx(c), s(c
     const int64_t per = (n + b - 1) / b;
     chunk = (per + kThreads - 1) / kThreads * kThreads;
     blocks = static_cast<int>((n + chunk - 1) / chunk);
-    partial = vec(c, kSlotPartial, static_cast<int64_t>(R) * blocks * kLdp2);
+    // The Gram-Schmidt dot pass runs on 64-row tiles: one tile per block while the blocks stay within 16384, so that
+    // small systems are not left with a few blocks walking many tiles one memory latency at a time.
+    const int64_t tiles = (n + kTileRows - 1) / kTileRows;
+    gs_blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(tiles, std::max(16, 16384 / R))));
+    gs_chunk = (tiles + gs_blocks - 1) / gs_blocks * kTileRows;
+    gs_blocks = static_cast<int>((n + gs_chunk - 1) / gs_chunk);
+    partial = vec(c, kSlotPartial, static_cast<int64_t>(R) * std::max(blocks, gs_blocks) * kLdp2);
     scal = vec(c, kSlotScal, static_cast<int64_t>(R) * kScal);
     state = static_cast<LoopState *>(c->scratch(kSlotState, sizeof(LoopState) * R, "the solver's state"));
     cap = max_iter + 2;
@@ -944,11 +950,11 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 KernelTimer t1(ctx, RT_TIMED_GS_DOTS, j * col + 2 * wb);
                 const size_t tile_lds = sizeof(VT) * j * kTileStride;
                 if (tile_lds > 32 * 1024) raise_lds(reinterpret_cast<const void *>(&k_dcgs_dots<VT>), tile_lds);
-                launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.blocks, R), dim3(kThreads), static_cast<uint32_t>(tile_lds),
-                             ws.s, V, j, pv, w, n, ws.chunk, ws.partial, st);
+                launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.gs_blocks, R), dim3(kThreads),
+                             static_cast<uint32_t>(tile_lds), ws.s, V, j, pv, w, n, ws.gs_chunk, ws.partial, st);
                 t1.done();
-                hipLaunchKernelGGL(k_dcgs_reduce, dim3(2 * j + 3, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks,
-                                   j, ws.scal, st);
+                hipLaunchKernelGGL(k_dcgs_reduce, dim3(2 * j + 3, R), dim3(kThreads), 0, ws.s, ws.partial,
+                                   ws.gs_blocks, j, ws.scal, st);
                 RT_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_dgmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.scal, Hr, H, g, m, j, it,
                                    ws.state, ws.history, ws.cap);

@@ -54,7 +54,9 @@ struct Workspace {
     int R;
     int blocks;          // row blocks per system of every reduction
     int64_t chunk;       // rows per block (a multiple of 256)
-    c128 *partial;       // [R][blocks][64]
+    int gs_blocks;       // row blocks per system of the GMRES Gram-Schmidt dot pass
+    int64_t gs_chunk;    // their rows per block (a multiple of 64)
+    c128 *partial;       // [R][blocks][128]
     c128 *scal;          // [R][kScal]
     LoopState *state;    // [R]
     double *history;     // [R][cap]
