@@ -348,15 +348,18 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
         om = wave_sum(om);
     }
     __syncthreads();
-    c128 *out = partial + (static_cast<int64_t>(r) * gridDim.x + blockIdx.x) * kLdp2;
+    // Slot-major partials, [R][slot][block]: the reduction reads each slot's blocks contiguously.
+    const int nb = gridDim.x;
+    c128 *out = partial + static_cast<int64_t>(r) * kLdp2 * nb + blockIdx.x;
     if (q < 2 && lane < cnt) {
         const c128 *v = red[q];
-        out[q * kPartZ + lane] = (v[lane] + v[kTileRows + lane]) + (v[2 * kTileRows + lane] + v[3 * kTileRows + lane]);
+        out[static_cast<int64_t>(q * kPartZ + lane) * nb] =
+            (v[lane] + v[kTileRows + lane]) + (v[2 * kTileRows + lane] + v[3 * kTileRows + lane]);
     }
     if (threadIdx.x == 0) {
-        out[kPartScalars] = mk(al, 0.0);
-        out[kPartScalars + 1] = mk(tre, tim);
-        out[kPartScalars + 2] = mk(om, 0.0);
+        out[static_cast<int64_t>(kPartScalars) * nb] = mk(al, 0.0);
+        out[static_cast<int64_t>(kPartScalars + 1) * nb] = mk(tre, tim);
+        out[static_cast<int64_t>(kPartScalars + 2) * nb] = mk(om, 0.0);
     }
 }
 
@@ -379,8 +382,9 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict
         dst = kAlpha + x - 2 * cnt;
     }
     double re = 0, im = 0;
+    const c128 *row = partial + (static_cast<int64_t>(r) * kLdp2 + src) * blocks;
     for (int b = threadIdx.x; b < blocks; b += kThreads) {
-        const c128 v = partial[(static_cast<int64_t>(r) * blocks + b) * kLdp2 + src];
+        const c128 v = row[b];
         re += v.x;
         im += v.y;
     }
@@ -389,38 +393,32 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict
     if (threadIdx.x == 0) scal[r * kScal + dst] = mk(re, im);
 }
 
-// Applies the rotations 0 .. j-1 to col[0 .. j] (LDS) and forms rotation j from col[j] and the subdiagonal sub;
-// col[j] becomes the rotated diagonal.
-__device__ __forceinline__ void rotate_column(c128 *col, int j, double sub, c128 *csv, c128 *snv) {
-    for (int k = 0; k < j; ++k) {
-        const c128 a0 = col[k], a1 = col[k + 1];
-        col[k] = cmulc(csv[k], a0) + cmulc(snv[k], a1);
-        col[k + 1] = csv[k] * a1 - snv[k] * a0;
-    }
-    const c128 a = col[j];
+// Givens rotation of (a, sub) with sub real: c = a / radius, s = sub / radius; returns the rotated diagonal.
+__device__ __forceinline__ c128 givens(c128 a, double sub, c128 &c, c128 &sv) {
     const double radius = sqrt(abs2(a) + sub * sub);
-    c128 c = mk(1.0, 0.0), sv = mk(0.0, 0.0);
+    c = mk(1.0, 0.0);
+    sv = mk(0.0, 0.0);
     if (radius > 0) {
         c = (1.0 / radius) * a;
         sv = mk(sub / radius, 0.0);
     }
-    csv[j] = c;
-    snv[j] = sv;
-    col[j] = cmulc(c, a) + cmulc(sv, mk(sub, 0.0));
+    return cmulc(c, a) + cmulc(sv, mk(sub, 0.0));
 }
 
-// Step j (0-based) of the cycle, iteration `it` overall: beta and z_j; column j - 1 of H in its final form, its
+// Step j (0-based) of the cycle, iteration `it` overall: beta, z_j and nu; column j - 1 of H in its final form, its
 // rotation formed again and the monitored residual of iteration it - 1 refined; column j with the subdiagonal nu
 // (final one step later), its rotation, the rotated right-hand side g, the monitored residual |g(j+1)| / reference
 // and the stopping test (iterate_.m:252-278); 1 / beta and 1 / (beta nu) for the update pass. H holds the columns as
 // formed (Hraw, for H s) and rotated (Hall, for the triangular solve); gall holds g, the cosines, the sines and g
-// before each rotation. Grid (1, R).
+// before each rotation. Everything the step reads is loaded at once; the norms are one wave's sums; the rotations of
+// both columns run in one pass with the running entry in registers. Grid (1, R).
 __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__ scal, c128 *__restrict__ Hraw,
                                                               c128 *__restrict__ Hall, c128 *__restrict__ gall, int m,
                                                               int j, int it, LoopState *st, double *history, int cap) {
     SYSTEM(r);
-    __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp + 1], coln[kLdp + 2], csv[kLdp], snv[kLdp];
-    __shared__ double shb[2];
+    __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp], colf[kLdp + 1], coln[kLdp + 1];
+    __shared__ c128 csv[kLdp], snv[kLdp];
+    __shared__ double shb[3];   // nu_{j-1}, beta, nu
     c128 *sc = scal + r * kScal;
     const int ldh = m + 1;
     c128 *Hr = Hraw + static_cast<int64_t>(r) * ldh * m, *Hc = Hall + static_cast<int64_t>(r) * ldh * m;
@@ -431,56 +429,63 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__
         z[k] = sc[kH1 + k];
         csv[k] = cs[k];
         snv[k] = sn[k];
+        colp[k] = Hr[static_cast<int64_t>(j - 1) * ldh + k];
     }
-    if (j >= 1)
-        for (int k = threadIdx.x; k < j; k += kThreads) colp[k] = Hr[static_cast<int64_t>(j - 1) * ldh + k];
+    double alpha = 0, omega = 0, ref = 0, tol = 0, maxit = 0;
+    c128 t = mk(0.0, 0.0), gprev = mk(0.0, 0.0);
+    if (threadIdx.x == 0) {
+        shb[0] = j >= 1 ? Hr[static_cast<int64_t>(j - 1) * ldh + j].x : 0.0;
+        alpha = sc[kAlpha].x;
+        t = sc[kT];
+        omega = sc[kOmega].x;
+        ref = sc[kReference].x;
+        tol = sc[kTol].x;
+        maxit = sc[kMaxIt].x;
+        gprev = j >= 1 ? gs[j - 1] : g[0];
+    }
     __syncthreads();
-    if (q == 0) {   // |s|^2 and s^H z
-        double ss = 0, szr = 0, szi = 0;
+    if (q == 0) {   // |s|^2, s^H z, |z|^2
+        double ss = 0, szr = 0, szi = 0, zz = 0;
         for (int k = lane; k < j; k += 64) {
             ss += abs2(s[k]);
             const c128 v = cmulc(s[k], z[k]);
             szr += v.x;
             szi += v.y;
+            zz += abs2(z[k]);
         }
         ss = wave_sum(ss);
         szr = wave_sum(szr);
         szi = wave_sum(szi);
+        zz = wave_sum(zz);
         if (lane == 0) {
-            const double b2 = sc[kAlpha].x - ss;
+            const double b2 = alpha - ss;
             const double beta = b2 > 0 && isfinite(b2) ? sqrt(b2) : 0.0;
-            shb[0] = beta;
-            z[j] = beta > 0 ? (1.0 / beta) * (sc[kT] - mk(szr, szi)) : mk(0.0, 0.0);
-            if (j >= 1) {   // column j - 1 final: + nu_{j-1} s, subdiagonal nu_{j-1} beta
-                const double nu = Hr[static_cast<int64_t>(j - 1) * ldh + j].x;
-                shb[1] = nu * beta;
-            }
+            shb[1] = beta;
+            const c128 zj = beta > 0 ? (1.0 / beta) * (t - mk(szr, szi)) : mk(0.0, 0.0);
+            z[j] = zj;
+            const double n2 = omega - (zz + abs2(zj));
+            shb[2] = beta > 0 && n2 > 0 ? sqrt(n2) / beta : 0.0;   // 0: w lies in the span (breakdown)
         }
     }
     __syncthreads();
-    const double beta = shb[0];
+    const double nup = shb[0], beta = shb[1];
     if (!(beta > 0)) {   // p vanished or is not finite: the system stops without a new column
         if (threadIdx.x == 0) st[r].finished = st[r].done = 1;
         return;
     }
+    // Column j - 1 final: + nu_{j-1} s above the subdiagonal, subdiagonal nu_{j-1} beta.
     if (j >= 1) {
-        const double nu = Hr[static_cast<int64_t>(j - 1) * ldh + j].x;
-        for (int k = threadIdx.x; k < j; k += kThreads) {
-            colp[k] = colp[k] + nu * s[k];
-            Hr[static_cast<int64_t>(j - 1) * ldh + k] = colp[k];
+        for (int k = threadIdx.x; k <= j; k += kThreads) {
+            colf[k] = k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0);
+            Hr[static_cast<int64_t>(j - 1) * ldh + k] = colf[k];
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && j >= 1) {
-        colp[j] = mk(shb[1], 0.0);
-        Hr[static_cast<int64_t>(j - 1) * ldh + j] = colp[j];
-    }
-    // (H s)_k over the final columns l < j, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4. Column
-    // j - 1 comes from LDS, the earlier ones from Hraw.
+    // (H s)_k over the final columns l < j, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4.
     for (int k = lane; k <= j; k += 64) {
         c128 acc = mk(0.0, 0.0);
         for (int l = max(0, k - 1) + ((q - max(0, k - 1)) & 3); l < j; l += 4) {
-            const c128 h = l == j - 1 ? (k < j ? colp[k] : mk(shb[1], 0.0)) : Hr[static_cast<int64_t>(l) * ldh + k];
+            const c128 h = l == j - 1 ? (k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0))
+                                      : Hr[static_cast<int64_t>(l) * ldh + k];
             acc = acc + h * s[l];
         }
         part[q][k] = acc;
@@ -492,50 +497,53 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__
         Hr[static_cast<int64_t>(j) * ldh + k] = coln[k];
     }
     __syncthreads();
-    if (q == 0) {   // |z|^2 (z extended by z_j)
-        double zz = 0;
-        for (int k = lane; k <= j; k += 64) zz += abs2(z[k]);
-        zz = wave_sum(zz);
-        if (lane == 0) {
-            const double n2 = sc[kOmega].x - zz;
-            shb[1] = n2 > 0 ? sqrt(n2) / beta : 0.0;   // nu; 0: w lies in the span (breakdown)
-        }
-    }
-    __syncthreads();
     if (threadIdx.x != 0) return;
-    const double nu = shb[1];
+    const double nu = shb[2];
     sc[kH1 + j] = z[j];
     Hr[static_cast<int64_t>(j) * ldh + j + 1] = mk(nu, 0.0);
-    const double ref = sc[kReference].x;
+    // The final rotations 0 .. j-2 on both columns, the running entries in registers.
+    c128 cb = coln[0], gj = gprev;
     if (j >= 1) {
-        rotate_column(colp, j - 1, colp[j].x, csv, snv);
-        for (int k = 0; k < j; ++k) Hc[static_cast<int64_t>(j - 1) * ldh + k] = colp[k];
-        cs[j - 1] = csv[j - 1];
-        sn[j - 1] = snv[j - 1];
-        const c128 g0 = gs[j - 1];
-        g[j - 1] = cmulc(csv[j - 1], g0);
-        g[j] = mk(0.0, 0.0) - snv[j - 1] * g0;
-        history[static_cast<int64_t>(r) * cap + it - 1] = sqrt(abs2(g[j])) / ref;
+        c128 ca = colf[0];
+        for (int k = 0; k + 1 < j; ++k) {
+            const c128 c = csv[k], sv = snv[k], a1 = colf[k + 1], b1 = coln[k + 1];
+            colf[k] = cmulc(c, ca) + cmulc(sv, a1);
+            ca = c * a1 - sv * ca;
+            coln[k] = cmulc(c, cb) + cmulc(sv, b1);
+            cb = c * b1 - sv * cb;
+        }
+        c128 c, sv;
+        colf[j - 1] = givens(ca, nup * beta, c, sv);   // rotation j - 1, formed again on the final column
+        const c128 b1 = coln[j];
+        coln[j - 1] = cmulc(c, cb) + cmulc(sv, b1);
+        cb = c * b1 - sv * cb;
+        for (int k = 0; k < j; ++k) Hc[static_cast<int64_t>(j - 1) * ldh + k] = colf[k];
+        cs[j - 1] = c;
+        sn[j - 1] = sv;
+        g[j - 1] = cmulc(c, gprev);
+        gj = mk(0.0, 0.0) - sv * gprev;
+        history[static_cast<int64_t>(r) * cap + it - 1] = sqrt(abs2(gj)) / ref;
     }
-    rotate_column(coln, j, nu, csv, snv);
+    c128 c, sv;
+    coln[j] = givens(cb, nu, c, sv);
     for (int k = 0; k <= j; ++k) Hc[static_cast<int64_t>(j) * ldh + k] = coln[k];
-    cs[j] = csv[j];
-    sn[j] = snv[j];
-    const c128 g0 = g[j];
-    gs[j] = g0;
-    g[j + 1] = mk(0.0, 0.0) - snv[j] * g0;
-    g[j] = cmulc(csv[j], g0);
-    const double res = sqrt(abs2(g[j + 1])) / ref;
+    cs[j] = c;
+    sn[j] = sv;
+    gs[j] = gj;
+    const c128 gnext = mk(0.0, 0.0) - sv * gj;
+    g[j + 1] = gnext;
+    g[j] = cmulc(c, gj);
+    const double res = sqrt(abs2(gnext)) / ref;
     history[static_cast<int64_t>(r) * cap + it] = res;
     st[r].steps = j + 1;
     st[r].iterations = it;
     st[r].tag = it;
-    const bool conv = res < sc[kTol].x;
+    const bool conv = res < tol;
     if (conv) {
         st[r].converged = st[r].finished = st[r].done = 1;
     } else if (!isfinite(res)) {
         st[r].finished = st[r].done = 1;
-    } else if (it >= static_cast<int>(sc[kMaxIt].x)) {
+    } else if (it >= static_cast<int>(maxit)) {
         st[r].done = 1;
     }
     sc[kInvBeta] = mk(1.0 / beta, 0.0);
