@@ -431,6 +431,15 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__
         snv[k] = sn[k];
         colp[k] = Hr[static_cast<int64_t>(j - 1) * ldh + k];
     }
+    // H(k, l) of the columns l < j - 1, l = q mod 4, for (H s)_k with k = lane: loaded with everything else, so that
+    // the step costs one memory latency.
+    const int k0 = max(0, lane - 1), l0 = k0 + ((q - k0) & 3);
+    c128 hk[kLdp / 4];
+#pragma unroll
+    for (int u = 0; u < kLdp / 4; ++u) {
+        const int l = l0 + 4 * u;
+        hk[u] = lane <= j && l < j - 1 ? Hr[static_cast<int64_t>(l) * ldh + lane] : mk(0.0, 0.0);
+    }
     double alpha = 0, omega = 0, ref = 0, tol = 0, maxit = 0;
     c128 t = mk(0.0, 0.0), gprev = mk(0.0, 0.0);
     if (threadIdx.x == 0) {
@@ -480,13 +489,17 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__
             Hr[static_cast<int64_t>(j - 1) * ldh + k] = colf[k];
         }
     }
-    // (H s)_k over the final columns l < j, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4.
-    for (int k = lane; k <= j; k += 64) {
+    // (H s)_k over the final columns l < j, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4, lane k.
+    if (lane <= j) {
+        const int k = lane;
         c128 acc = mk(0.0, 0.0);
-        for (int l = max(0, k - 1) + ((q - max(0, k - 1)) & 3); l < j; l += 4) {
-            const c128 h = l == j - 1 ? (k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0))
-                                      : Hr[static_cast<int64_t>(l) * ldh + k];
-            acc = acc + h * s[l];
+#pragma unroll
+        for (int u = 0; u < kLdp / 4; ++u) {
+            const int l = l0 + 4 * u;
+            if (l < j) {
+                const c128 h = l == j - 1 ? (k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0)) : hk[u];
+                acc = acc + h * s[l];
+            }
         }
         part[q][k] = acc;
     }
