@@ -276,6 +276,7 @@ constexpr int kTileRows = 64;
 constexpr int kTileStride = 65;
 constexpr int kLdp2 = 128;                           // partial slots per block of the dot pass
 constexpr int kPartZ = 64, kPartScalars = 125;       // s at k, z at kPartZ + k, then alpha, t, omega
+constexpr size_t kDotsRedBytes = 2 * 4 * kTileRows * sizeof(c128);
 
 // partial[(r * blocks + b) * kLdp2 + ...] = the inner products above over block b's rows. The live columns of a tile of
 // 64 rows are staged in LDS with p and w (row stride 65 keeps the column-wise reads off shared banks); lane k of wave
@@ -286,10 +287,11 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
                                                         c128 *__restrict__ partial, const LoopState *__restrict__ st) {
     SYSTEM(r);
     const int R = gridDim.y;
+    // Dynamic LDS: the tile [cnt][kTileStride], then the same bytes for the four waves' sums (kDotsRedBytes at least).
     extern __shared__ unsigned char tile_raw[];
-    VT *tile = reinterpret_cast<VT *>(tile_raw);   // [cnt][kTileStride]
+    VT *tile = reinterpret_cast<VT *>(tile_raw);
+    c128(*red)[4 * kTileRows] = reinterpret_cast<c128(*)[4 * kTileRows]>(tile_raw);
     __shared__ c128 pt[kTileRows], wt[kTileRows];
-    __shared__ c128 red[2][4 * kTileRows];
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const c128 *pr = p + r * n, *wr = w + r * n;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
@@ -339,6 +341,7 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
             }
         }
     }
+    __syncthreads();   // the tile is consumed: its bytes take the sums
     red[0][q * kTileRows + lane] = mk(sre, sim);
     red[1][q * kTileRows + lane] = mk(zre, zim);
     if (q == 0) {
@@ -969,7 +972,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 // Algorithmic bytes of the two passes: the live basis columns once, p and w read, q_j and p written.
                 const double col = static_cast<double>(sizeof(VT)) * n * R, wb = 16.0 * n * R;
                 KernelTimer t1(ctx, RT_TIMED_GS_DOTS, j * col + 2 * wb);
-                const size_t tile_lds = sizeof(VT) * j * kTileStride;
+                const size_t tile_lds = std::max(sizeof(VT) * j * kTileStride, kDotsRedBytes);
                 if (tile_lds > 32 * 1024) raise_lds(reinterpret_cast<const void *>(&k_dcgs_dots<VT>), tile_lds);
                 launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.gs_blocks, R), dim3(kThreads),
                              static_cast<uint32_t>(tile_lds), ws.s, V, j, pv, w, n, ws.gs_chunk, ws.partial, st);
