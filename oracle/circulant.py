@@ -1,15 +1,19 @@
-"""Circulant preconditioners: the kernels' folds, the partwise builder, the inverse and its product (oracle; test
-infrastructure only).
+"""Circulant preconditioners: the kernels' folds, the superoptimal kernel, the partwise, blockwise and hybrid builders,
+the inverse and its product (oracle; test infrastructure only).
 
 Follows src/+rooftopMoM/+assembler/+preconditioner/: generateCirculant.m:16-24, generatePartwiseSpectra.m:16-30,
-generateHybridSpectra.m, +kernels/generate{Strang,TChan,RChan,Huckle}.m, +utils/foldGenerator.m:27-61 and
+generateBlockwiseSpectra.m, generateHybridSpectra.m, +kernels/generate{Strang,TChan,RChan,Huckle,Superoptimal}.m,
++utils/foldGenerator.m:27-61, +utils/generatePartSpectra.m, +utils/generateOptimalOfProduct.m and
 @Spectra/invert.m:12-20, applied by the FFTApplicator of the face grid (FFTApplicator.m:14-87).
 
 Pinned: reference.mat preconditioned{1,2} = partwise Strang and T. Chan at 1e-12 (GenerateCirculantTest.m:5-25); the
 folds against dense circulants built entry by entry, T. Chan's Frobenius optimality, Huckle at full bandwidth = T. Chan,
 R. Chan of a circulant = that circulant, the inverse blocks frequency by frequency; the hybrid builder from those pinned
 parts (its coupling is the partwise one, its diagonal blocks folds of the spatial blocks, pinned against the dense
-matrix).
+matrix). The superoptimal kernel against its definition, the eigenvalues of c(T T^H) / c(T^H) from dense matrices in
+the Fourier basis (KernelsTest.m:104-115), and its evenness for an even generator (KernelsTest.m:117-123); the
+blockwise builder's transposed coupling against the dense transpose, and every kernel and builder's preconditioner
+against complex symmetry (PreconditionerTest.m).
 """
 
 import numpy as np
@@ -73,9 +77,86 @@ def fold_generator(generator, window, even=False):
     return fold_rows(fold_rows(weighted).T).T
 
 
+# --- the superoptimal kernel of Tyrtyshnikov --------------------------------------------------------------------
+
+def part_spectra(t):
+    """+utils/generatePartSpectra.m: along each direction a Toeplitz matrix is a circulant plus a skew-circulant,
+    c(k) = (t(k) + t(k - count)) / 2, s(k) = (t(k) - t(k - count)) / 2, k = 0 .. count - 1 (t(-count) = 0); the skew
+    part is diagonal in the Fourier basis modulated by exp(i pi k / count). Returns {(m, n): eigenvalues}, m along x
+    and n along y, 0 circulant and 1 skew."""
+    t = np.asarray(t, dtype=np.complex128)
+    cx, cy = (t.shape[0] + 1) // 2, (t.shape[1] + 1) // 2
+
+    def split(h):   # along axis 0; row k of a part holds its generator at the offset k
+        count = (h.shape[0] + 1) // 2
+        positive = h[count - 1:]
+        negative = np.vstack([np.zeros((1, h.shape[1]), dtype=np.complex128), h[:count - 1]])
+        return (positive + negative) / 2, (positive - negative) / 2
+
+    mod_x = np.exp(1j * np.pi * np.arange(cx) / cx)[:, None]
+    mod_y = np.exp(1j * np.pi * np.arange(cy) / cy)[None, :]
+    out = {}
+    for m, along_x in enumerate(split(t)):
+        for n, along_y in enumerate(split(along_x.T)):
+            part = along_y.T
+            if m == 1:
+                part = mod_x * part
+            if n == 1:
+                part = part * mod_y
+            out[m, n] = np.fft.fft2(part)
+    return out
+
+
+def _skew_kernel(count):
+    """The spectrum of K(d) = 1 / (count^2 sin^2(pi (2 d + 1) / (2 count))) (generateOptimalOfProduct.m)."""
+    d = np.arange(count)
+    return np.fft.fft(1.0 / (count ** 2 * np.sin(np.pi * (2 * d + 1) / (2 * count)) ** 2))
+
+
+def _convolve(values, kernel, axis):
+    if axis == 0:
+        return np.fft.ifft(kernel[:, None] * np.fft.fft(values, axis=0), axis=0)
+    return np.fft.ifft(kernel[None, :] * np.fft.fft(values, axis=1), axis=1)
+
+
+def optimal_of_product(parts_a, parts_b):
+    """+utils/generateOptimalOfProduct.m: the eigenvalues of T. Chan's c(A B^H) from the part spectra, without the
+    product: along a direction a circulant factor passes through c, a skew-circulant one is projected by the circular
+    convolution of its eigenvalues with K; every pair of parts adds the A part's eigenvalues times the conjugate of the
+    B part's, convolved first where one of the two is skew, the product convolved where both are."""
+    cx, cy = parts_a[0, 0].shape
+    kernels = (_skew_kernel(cx), _skew_kernel(cy))
+    types = ((0, 0), (1, 0), (0, 1), (1, 1))
+    total = np.zeros((cx, cy), dtype=np.complex128)
+    for ta in types:
+        for tb in types:
+            va, vb = parts_a[ta], parts_b[tb]
+            for d in range(2):
+                if ta[d] == 1 and tb[d] == 0:
+                    va = _convolve(va, kernels[d], d)
+                if ta[d] == 0 and tb[d] == 1:
+                    vb = _convolve(vb, kernels[d], d)
+            product = va * np.conj(vb)
+            for d in range(2):
+                if ta[d] == 1 and tb[d] == 1:
+                    product = _convolve(product, kernels[d], d)
+            total = total + product
+    return total
+
+
+def superoptimal(t):
+    """+kernels/generateSuperoptimal.m: the circulant C minimizing |I - C^-1 T|_F, C = c(T T^H) / c(T^H), from a
+    two-sided generator; c(T^H) has the conjugate eigenvalues of c(T). Returns the count_x x count_y generator."""
+    parts = part_spectra(t)
+    squared = optimal_of_product(parts, parts)
+    optimal = np.fft.fft2(fold_generator(t, window_tchan))
+    return np.fft.ifft2(squared / np.conj(optimal))
+
+
 # --- partwise builder, inverse, product -------------------------------------------------------------------------
 
-KERNELS = ('strang', 'tChan', 'rChan', 'huckle')
+KERNELS = ('strang', 'tChan', 'rChan', 'huckle', 'superoptimal')
+BUILDERS = ('partwise', 'blockwise', 'hybrid')
 
 
 def approximate(generator, kernel, count_x, count_y, bandwidth=None):
@@ -93,6 +174,8 @@ def approximate(generator, kernel, count_x, count_y, bandwidth=None):
     elif kernel == 'huckle':
         bw = bandwidth if bandwidth is not None else (count_x, count_y)
         c = fold_generator(S.extend_evenly(g), window_huckle(bw))
+    elif kernel == 'superoptimal':
+        c = superoptimal(S.extend_evenly(g))
     else:
         raise ValueError(kernel)
     return np.fft.fft2(c)
@@ -112,9 +195,35 @@ def _fold_two_sided(t, kernel, bandwidth, count_x, count_y):
     """A kernel applied to a two-sided generator (the blocks of the blockwise and hybrid builders)."""
     if t is None:
         return None
+    if kernel == 'superoptimal':
+        return superoptimal(t)
     windows = {'strang': window_strang, 'tChan': window_tchan, 'rChan': window_rchan,
                'huckle': window_huckle(bandwidth if bandwidth is not None else (count_x, count_y))}
     return fold_generator(t, windows[kernel])
+
+
+def negate_frequencies(spectrum):
+    """The spectrum at the negative frequencies (generateBlockwiseSpectra.m transpose_): the transposed circulant has
+    the generator c(-k, -l)."""
+    if spectrum is None:
+        return None
+    rows = np.r_[0, np.arange(spectrum.shape[0] - 1, 0, -1)]
+    cols = np.r_[0, np.arange(spectrum.shape[1] - 1, 0, -1)]
+    return spectrum[rows][:, cols]
+
+
+def blockwise_spectra(generators, grid, kernel, bandwidth=None):
+    """generateBlockwiseSpectra.m: each of the spatial blocks (generateSpatialBlocks.m) replaced by its kernel's
+    circulant; the YX block is the transpose of the XY block, and so is its circulant."""
+    cx, cy = generators['electric'].shape
+    xx, xy, _, yy = S.spatial_blocks(generators, grid)
+
+    def spectrum(t):
+        c = _fold_two_sided(t, kernel, bandwidth, cx, cy)
+        return None if c is None else np.fft.fft2(c)
+
+    coupling = spectrum(xy)
+    return {'XX': spectrum(xx), 'XY': coupling, 'YX': negate_frequencies(coupling), 'YY': spectrum(yy)}
 
 
 def hybrid_spectra(generators, grid, kernel, bandwidth=None):
@@ -150,8 +259,8 @@ def is_singular(inverse):
 
 
 def circulant_preconditioner(generators, grid, kernel='rChan', bandwidth=None, builder='partwise'):
-    """The inverse spectra of generateCirculant.m:16-24 with the partwise or hybrid builder."""
-    build = partwise_spectra if builder == 'partwise' else hybrid_spectra
+    """The inverse spectra of generateCirculant.m:16-24 with the partwise, blockwise or hybrid builder."""
+    build = {'partwise': partwise_spectra, 'blockwise': blockwise_spectra, 'hybrid': hybrid_spectra}[builder]
     return invert(build(generators, grid, kernel, bandwidth), grid.content)
 
 

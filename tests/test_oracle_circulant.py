@@ -122,3 +122,80 @@ def test_hybrid_builder_from_pinned_parts():
     gs = A.generate(strip, 2 * np.pi, (3, 3))
     inv = C.circulant_preconditioner(gs, strip, 'tChan', builder='hybrid')
     assert inv['XX'] is not None and inv['YY'] is None and not C.is_singular(inv)
+
+
+def _fourier(cx, cy):
+    """The 2D DFT matrix on vectors indexed x fastest."""
+    fx = np.exp(-2j * np.pi * np.outer(np.arange(cx), np.arange(cx)) / cx)
+    fy = np.exp(-2j * np.pi * np.outer(np.arange(cy), np.arange(cy)) / cy)
+    return np.kron(fy, fx)
+
+
+def test_superoptimal_matches_its_definition():
+    """KernelsTest.m:104-115: the eigenvalues of C are diag(F T T^H F^H) / conj(diag(F T F^H)), from the dense
+    two-level Toeplitz matrix of a random complex generator in the Fourier basis."""
+    rng = np.random.default_rng(1)
+    t = rng.standard_normal((7, 5)) + 1j * rng.standard_normal((7, 5))
+    T = C.dense_toeplitz(t)
+    F = _fourier(4, 3)
+    expected = np.diag(F @ (T @ T.conj().T) @ F.conj().T) / np.conj(np.diag(F @ T @ F.conj().T))
+    got = np.fft.fft2(C.superoptimal(t)).reshape(-1, order='F')
+    assert np.abs(got - expected).max() < 1e-12 * np.abs(expected).max()
+
+
+def test_superoptimal_of_an_even_generator_is_even():
+    """KernelsTest.m:117-123: C(p, q) = C(-p, -q) for an even generator."""
+    rng = np.random.default_rng(1)
+    t = S.extend_evenly(rng.standard_normal((5, 4)) + 1j * rng.standard_normal((5, 4)))
+    c = C.superoptimal(t)
+    mirrored = c[np.r_[0, 4:0:-1]][:, np.r_[0, 3:0:-1]]
+    assert np.abs(c - mirrored).max() < 1e-13 * np.abs(c).max()
+
+
+def test_blockwise_coupling_is_the_transposed_circulant():
+    """generateBlockwiseSpectra.m: YX is the transpose of the XY circulant, its spectrum taken at the negative
+    frequencies; checked against the dense transpose."""
+    grid = A.Grid(5, 4, 0.1, 0.15)
+    g = A.generate(grid, 4, (3, 3))
+    _, xy, yx, _ = S.spatial_blocks(g, grid)
+    for kernel in ('strang', 'rChan', 'superoptimal'):
+        sp = C.blockwise_spectra(g, grid, kernel)
+        cxy = np.fft.ifft2(sp['XY'])
+        cyx = np.fft.ifft2(sp['YX'])
+        assert np.abs(C.dense_circulant(cyx) - C.dense_circulant(cxy).T).max() < 1e-13 * np.abs(cxy).max()
+    # R. Chan wraps every offset: the XY circulant is the brute-force fold of the spatial XY block.
+    brute = _brute_fold(xy, lambda ox, oy: 1.0)
+    assert np.allclose(np.fft.ifft2(C.blockwise_spectra(g, grid, 'rChan')['XY']), brute, rtol=0,
+                       atol=1e-13 * np.abs(brute).max())
+
+
+@pytest.mark.parametrize('builder', ['partwise', 'blockwise', 'hybrid'])
+def test_every_kernel_and_builder_is_complex_symmetric(builder):
+    """PreconditionerTest.m testIsSymmetric on its 5 x 4 problem (wavenumber 4, faces 0.1 x 0.15): x.' P y = y.' P x,
+    which the conjugate gradients need."""
+    grid = A.Grid(5, 4, 0.1, 0.15)
+    g = A.generate(grid, 4, (3, 3))
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(grid.count) + 1j * rng.standard_normal(grid.count)
+    y = rng.standard_normal(grid.count) + 1j * rng.standard_normal(grid.count)
+    for kernel in C.KERNELS:
+        inv = C.circulant_preconditioner(g, grid, kernel, (5, 4) if kernel == 'huckle' else None, builder)
+        assert not C.is_singular(inv)
+        a, b = x @ C.apply(inv, grid, y), y @ C.apply(inv, grid, x)
+        assert abs(a - b) < 1e-12 * abs(a), (kernel, builder)
+
+
+def test_blockwise_approximation_cuts_the_iterations():
+    """PreconditionerTest.m testBlockwiseApproximationCutsTheIterations: 16 x 16 plate, R. Chan blockwise, GMRES to
+    1e-8 needs fewer iterations than without."""
+    from oracle import excitation as X
+    from oracle import solvers as SV
+    grid = A.Grid(16, 16, 0.05, 0.05)
+    g = A.generate(grid, 2 * np.pi, (3, 3))
+    Z = S.dense_matrix(g, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [1, 0, 0], 1.0, order=3)
+    inv = C.circulant_preconditioner(g, grid, 'rChan', builder='blockwise')
+    plain = SV.solve('gmres', Z, b, tol=1e-8, max_iter=1000, restart=50)
+    pre = SV.solve('gmres', Z, b, tol=1e-8, max_iter=1000, restart=50,
+                   preconditioner=lambda v: C.apply(inv, grid, v))
+    assert pre['converged'] and pre['iterationCount'] < plain['iterationCount']
