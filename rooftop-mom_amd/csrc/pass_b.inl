@@ -9,8 +9,10 @@ namespace rt {
 // The last forward round, the multiply and the first inverse round on the registers of one butterfly at a time.
 // Spectra: row = bspec + fold(kx) My 3, position py holds (Mxx, Myy, E) at row[3 py ..]; dy in position order.
 // The multiply runs in chunks of CH positions behind scheduling barriers, which bounds the spectrum loads in flight
-// (and the registers they hold).
-template <typename TC, class P, bool DIRECT, bool EMB>
+// (and the registers they hold). FOUR: a general 2x2 block per frequency instead, row = bspec + px My 4 holding
+// (XX, XY, YX, YY) at row[4 py ..] (the blockwise circulant preconditioner, whose coupling is not a multiple of one
+// plane).
+template <typename TC, class P, bool DIRECT, bool EMB, bool FOUR = false>
 __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *col1, cplx<TC> *__restrict__ w0,
                                                cplx<TC> *__restrict__ w1, int n0, int n1,
                                                const cplx<TC> *__restrict__ row, cplx<TC> dxv,
@@ -36,17 +38,25 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
         }
         dft<TC, R, false, false>(X);
         dft<TC, R, false, false>(Y);
-        const cplx<TC> *sp = row + 3 * base;
+        constexpr int NP = FOUR ? 4 : 3;
+        const cplx<TC> *sp = row + NP * base;
         const cplx<TC> *dyp = dy + base;
 #pragma unroll
         for (int q0 = 0; q0 < R; q0 += CH) {
 #pragma unroll
             for (int q = q0; q < q0 + CH && q < R; ++q) {
-                const cplx<TC> a = sp[3 * q], b = sp[3 * q + 1], el = sp[3 * q + 2];
-                const cplx<TC> dyv = dyp[q];
-                const cplx<TC> phi = el * (cmulc(dxv, X[q]) + cmulc(dyv, Y[q]));
-                X[q] = a * X[q] - dxv * phi;
-                Y[q] = b * Y[q] - dyv * phi;
+                if constexpr (FOUR) {
+                    const cplx<TC> xx = sp[4 * q], xy = sp[4 * q + 1], yx = sp[4 * q + 2], yy = sp[4 * q + 3];
+                    const cplx<TC> x = X[q];
+                    X[q] = xx * x + xy * Y[q];
+                    Y[q] = yx * x + yy * Y[q];
+                } else {
+                    const cplx<TC> a = sp[3 * q], b = sp[3 * q + 1], el = sp[3 * q + 2];
+                    const cplx<TC> dyv = dyp[q];
+                    const cplx<TC> phi = el * (cmulc(dxv, X[q]) + cmulc(dyv, Y[q]));
+                    X[q] = a * X[q] - dxv * phi;
+                    Y[q] = b * Y[q] - dyv * phi;
+                }
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -67,7 +77,8 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
 
 // EMB: the columns are the lower halves of an embedding (the upper half of every input zero, only the lower half of
 // every output kept), as in the Toeplitz product; false for full-period columns (a circulant on the face grid).
-template <typename TC, int LOG2M, int BITS, int MAXT, bool EMB = true>
+// FOUR: the four-plane multiply (unfolded blocks per x position).
+template <typename TC, int LOG2M, int BITS, int MAXT, bool EMB = true, bool FOUR = false>
 __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ bspec,
                                                  const int *__restrict__ fold_x, const int *__restrict__ border,
                                                  const cplx<TC> *__restrict__ dx,
@@ -88,10 +99,11 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
     cplx<TC> *w0 = work + work_index(d, r, 0, px, 0);
     cplx<TC> *w1 = work + work_index(d, r, 1, px, 0);
     const int n0 = d.cols_x, n1 = d.cols_y;
-    const cplx<TC> *row = bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+    const cplx<TC> *row = FOUR ? bspec + static_cast<int64_t>(px) * d.My * 4
+                               : bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
     const cplx<TC> dxv = dx[px];
     if constexpr (P::K == 1) {   // one round: straight from the work array and back
-        multiply_round<TC, P, true, EMB>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
+        multiply_round<TC, P, true, EMB, FOUR>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
     } else {
         // Forward round 0 from the work array (the upper half of the embedding is zero).
         cround<TC, P, 0, false, EMB, true>(
@@ -109,7 +121,7 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
                 t, tw, [&](int, int pp) { return col1[pp]; }, [&](int, int pp, cplx<TC> v) { col1[pp] = v; });
             __syncthreads();
         });
-        multiply_round<TC, P, false, EMB>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
+        multiply_round<TC, P, false, EMB, FOUR>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
         __syncthreads();
         tw = opaque(tw);
         static_for_down<1, P::K - 1>([&](auto S) {
@@ -278,7 +290,14 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
 #define RT_LAUNCH_B(L, B)                                                                                          \
     {                                                                                                              \
         using PL = CPlan<L, B>;                                                                                    \
-        if (!op.embedding) {                                                                                       \
+        if (!op.embedding && op.four_plane) {                                                                      \
+            using PF = CPlan<L, 3>;                                                                                \
+            constexpr int MT = PF::T > 256 ? PF::T : 256;                                                          \
+            set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, 3, MT, false, true>), lds);                    \
+            launch_timed(timer, pass_b_k<TC, L, 3, MT, false, true>, dim3(xcd_grid(nb)), dim3(threads),            \
+                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, nrhs, \
+                         static_cast<int>(nb), skip, skip_stride);                                                 \
+        } else if (!op.embedding) {                                                                                \
             using PF = CPlan<L, 3>;                                                                                \
             constexpr int MT = PF::T > 256 ? PF::T : 256;                                                          \
             set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, 3, MT, false>), lds);                          \

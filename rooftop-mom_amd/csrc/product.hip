@@ -15,6 +15,7 @@
 #include <cmath>
 #include <type_traits>
 
+#include "circulant_host.hpp"
 #include "passes.hpp"
 
 namespace rt {
@@ -196,12 +197,12 @@ static PassShape shape_for(const Product &op, int nrhs) {
 
 // Algorithmic HBM bytes of one launch of each pass: every vector element, work element and spectrum value once.
 template <typename TC>
-static void pass_bytes(const ProdDims &d, int nrhs, double *bytes) {
+static void pass_bytes(const ProdDims &d, int nrhs, double *bytes, bool four_plane = false) {
     const double sc = sizeof(cplx<TC>);
     const double vec = 16.0 * d.n * nrhs;
     const double work_bytes = sc * (static_cast<double>(d.cols_x) + d.cols_y) * d.Mx * nrhs;
     bytes[0] = vec + work_bytes;
-    bytes[1] = 2 * work_bytes + 3.0 * sc * d.Hx * d.My;
+    bytes[1] = 2 * work_bytes + (four_plane ? 4.0 * sc * d.Mx * d.My : 3.0 * sc * d.Hx * d.My);
     bytes[2] = work_bytes + vec;
 }
 
@@ -264,9 +265,15 @@ static void run_chunk(Product &op, const c128 *in, c128 *out, cplx<TC> *work, in
     const ProdDims &d = op.dims;
     Context *ctx = op.problem->ctx;
     const PlanTables &tx = op.tx.of(sh.bits), &ty = op.ty.of(sh.bits);
-    const cplx<TC> *bspec = bspec_of<TC>(op, sh.bits, s);
+    const cplx<TC> *bspec = op.four_plane ? reinterpret_cast<const cplx<TC> *>(sizeof(TC) == sizeof(double)
+                                                                                    ? static_cast<const void *>(op.b4spec64.ptr)
+                                                                                    : static_cast<const void *>(op.b4spec32.ptr))
+                                          : bspec_of<TC>(op, sh.bits, s);
+    if (op.four_plane && !b_fits<TC>(op, sh))
+        fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
+             "The blockwise circulant preconditioner needs the face grid's columns to fit one block's LDS.");
     double by[3];
-    pass_bytes<TC>(d, nrhs, by);
+    pass_bytes<TC>(d, nrhs, by, op.four_plane);
     const int slot0 = sizeof(TC) == sizeof(float) ? RT_TIMED_PASS_A_C64 : RT_TIMED_PASS_A_C128;
     KernelTimer ta(ctx, slot0, by[0]);
     launch_pass_a<TC>(op, tx, in, work, nrhs, sh.group, s, skip, skip_stride, &ta);
@@ -564,17 +571,85 @@ __global__ void circulant_invert(c128 *__restrict__ planes, int Hx, int Hy, int 
     }
 }
 
+// pc->b4spec{64,32}[(px My + py) 4 + c] = block c of the inverse at (freq_x[px], freq_y[py]) from planes
+// [4][Mx][My] (unfolded frequencies), in the precision of the pass.
+template <typename TC>
+__global__ void build_b4spec(const c128 *__restrict__ planes, int Mx, int My, const int *__restrict__ freq_x,
+                             const int *__restrict__ freq_y, cplx<TC> *__restrict__ out) {
+    const int64_t total = static_cast<int64_t>(Mx) * My;
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int px = static_cast<int>(i / My), py = static_cast<int>(i - static_cast<int64_t>(px) * My);
+        const int64_t f = static_cast<int64_t>(freq_x[px]) * My + freq_y[py];
+        for (int c = 0; c < 4; ++c) out[4 * i + c] = cvt<TC>(planes[c * total + f]);
+    }
+}
+
+// Host twins of generator_at and spatial_block_at for the spectra formed on the host: the superoptimal kernel and
+// the blockwise builder (whose XY block is not even).
+static cd host_at(const std::vector<cd> &g, int rows, int cols, int p, int q) {
+    if (p < 0) p = -p;
+    if (q < 0) q = -q;
+    if (p >= rows || q >= cols) return 0.0;
+    return g[static_cast<size_t>(q) * rows + p];
+}
+
+static std::vector<cd> host_generator(const DeviceArray<c128> &g, size_t count, int modifier, hipStream_t s) {
+    std::vector<c128> raw(count);
+    RT_HIP(hipMemcpyAsync(raw.data(), g.ptr, sizeof(c128) * count, hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    std::vector<cd> out(count);
+    for (size_t i = 0; i < count; ++i) {
+        cd v(raw[i].x, raw[i].y);
+        if (modifier == RT_MODIFIER_REAL) v = cd(v.real(), 0.0);
+        if (modifier == RT_MODIFIER_IMAG) v = cd(v.imag(), 0.0);
+        out[i] = v;
+    }
+    return out;
+}
+
+template <class F>
+static TwoSided two_sided(int cx, int cy, F f) {
+    TwoSided t;
+    t.cx = cx;
+    t.cy = cy;
+    t.v.resize(static_cast<size_t>(2 * cx - 1) * (2 * cy - 1));
+    for (int q = -(cy - 1); q < cy; ++q)
+        for (int p = -(cx - 1); p < cx; ++p) t.v[(p + cx - 1) + static_cast<size_t>(2 * cx - 1) * (q + cy - 1)] = f(p, q);
+    return t;
+}
+
+// The two-sided spatial blocks (generateSpatialBlocks.m:19-61) of the host generators: 1 XX, 2 YY, 3 XY.
+static TwoSided spatial_block(int block, const std::vector<cd> &mxx, const std::vector<cd> &myy,
+                              const std::vector<cd> &e, int cx, int cy, double a, double b) {
+    auto E = [&](int p, int q) { return host_at(e, cx, cy, p, q); };
+    return two_sided(cx, cy, [&](int p, int q) -> cd {
+        if (block == 1) {
+            if (std::abs(p) > cx - 2) return 0.0;
+            return host_at(mxx, cx - 1, cy, p, q) - (2.0 * E(p, q) - E(p - 1, q) - E(p + 1, q)) / (a * a);
+        }
+        if (block == 2) {
+            if (std::abs(q) > cy - 2) return 0.0;
+            return host_at(myy, cx, cy - 1, p, q) - (2.0 * E(p, q) - E(p, q - 1) - E(p, q + 1)) / (b * b);
+        }
+        if (p > cx - 2 || q < -(cy - 2)) return 0.0;
+        return -(E(p, q) - E(p, q - 1) - E(p + 1, q) + E(p + 1, q - 1)) / (a * b);
+    });
+}
+
 std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, int bandwidth_x, int bandwidth_y,
                                           int modifier, int precision) {
     Context *ctx = p->ctx;
     hipStream_t s = ctx->stream;
     const GridInfo &g = p->g;
-    if ((builder != RT_BUILDER_PARTWISE && builder != RT_BUILDER_HYBRID) || kernel < RT_CIRCULANT_STRANG ||
-        kernel > RT_CIRCULANT_HUCKLE)
+    if (builder < RT_BUILDER_PARTWISE || builder > RT_BUILDER_HYBRID || kernel < RT_CIRCULANT_STRANG ||
+        kernel > RT_CIRCULANT_SUPEROPTIMAL)
         fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
-             "The GPU circulant preconditioner takes the partwise or hybrid builder with the strang, tChan, rChan or "
-             "huckle kernel.");
+             "The circulant preconditioner takes the partwise, blockwise or hybrid builder with the strang, tChan, "
+             "rChan, huckle or superoptimal kernel.");
     const bool hybrid = builder == RT_BUILDER_HYBRID;
+    const bool blockwise = builder == RT_BUILDER_BLOCKWISE;
+    const bool on_host = blockwise || kernel == RT_CIRCULANT_SUPEROPTIMAL;
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
     if (!pow2(g.cx) || !pow2(g.cy))
         fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
@@ -606,8 +681,125 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
     p->ensure_generators();
     pc->tx.build(ctx, d.Mx, g.a, s);
     pc->ty.build(ctx, d.My, g.b, s);
-    pc->planes64.allocate(ctx, 3 * plane_elems, "The circulant preconditioner");
-    pc->planes64.zero(s);
+    if (on_host) {
+        // Spectra formed on the host: the generators (after the modifier), their or their spatial blocks'
+        // circulants, then the inverse.
+        const std::vector<cd> e = host_generator(p->e, static_cast<size_t>(g.cx) * g.cy, modifier, s);
+        const std::vector<cd> mxx = g.has_x ? host_generator(p->mxx, static_cast<size_t>(g.cx - 1) * g.cy, modifier, s)
+                                            : std::vector<cd>();
+        const std::vector<cd> myy = g.has_y ? host_generator(p->myy, static_cast<size_t>(g.cx) * (g.cy - 1), modifier, s)
+                                            : std::vector<cd>();
+        const double bwx = bandwidth_x, bwy = bandwidth_y;
+        const int64_t full = static_cast<int64_t>(d.Mx) * d.My;
+        auto unfold = [&](const std::vector<cd> &spec, std::vector<cd> &out, int64_t offset) {   // [fx + Mx fy]
+            for (int fx = 0; fx < d.Mx; ++fx)
+                for (int fy = 0; fy < d.My; ++fy) out[offset + static_cast<int64_t>(fx) * d.My + fy] = spec[fx + static_cast<size_t>(d.Mx) * fy];
+        };
+        auto upload = [&](const std::vector<cd> &h, c128 *dst) {
+            std::vector<c128> raw(h.size());
+            for (size_t i = 0; i < h.size(); ++i) raw[i] = mk(h[i].real(), h[i].imag());
+            RT_HIP(hipMemcpyAsync(dst, raw.data(), sizeof(c128) * raw.size(), hipMemcpyHostToDevice, s));
+            RT_HIP(hipStreamSynchronize(s));
+        };
+        if (blockwise) {
+            // generateBlockwiseSpectra.m: the kernel's circulant of every spatial block; YX is the transpose of XY,
+            // its spectrum taken at the negative frequencies. Inverted frequency by frequency (@Spectra/invert.m) and
+            // kept unfolded, four planes, for the four-plane multiply of pass B.
+            std::vector<cd> blocks(4 * full, 0.0);
+            if (g.has_x) unfold(kernel_spectrum(spatial_block(1, mxx, myy, e, g.cx, g.cy, g.a, g.b), kernel, bwx, bwy), blocks, 0);
+            if (g.has_y) unfold(kernel_spectrum(spatial_block(2, mxx, myy, e, g.cx, g.cy, g.a, g.b), kernel, bwx, bwy), blocks, 3 * full);
+            if (g.has_x && g.has_y) {
+                unfold(kernel_spectrum(spatial_block(3, mxx, myy, e, g.cx, g.cy, g.a, g.b), kernel, bwx, bwy), blocks, full);
+                for (int fx = 0; fx < d.Mx; ++fx)
+                    for (int fy = 0; fy < d.My; ++fy)
+                        blocks[2 * full + static_cast<int64_t>(fx) * d.My + fy] =
+                            blocks[full + static_cast<int64_t>((d.Mx - fx) % d.Mx) * d.My + (d.My - fy) % d.My];
+            }
+            const double scale = 1.0 / static_cast<double>(full);
+            bool singular = false;
+            for (int64_t f = 0; f < full; ++f) {
+                cd &xx = blocks[f], &xy = blocks[full + f], &yx = blocks[2 * full + f], &yy = blocks[3 * full + f];
+                cd det;
+                if (g.has_x && g.has_y) {
+                    det = xx * yy - xy * yx;
+                    const cd a2 = yy / det, b2 = -xy / det, c2 = -yx / det, d2 = xx / det;
+                    xx = a2;
+                    xy = b2;
+                    yx = c2;
+                    yy = d2;
+                } else if (g.has_x) {
+                    det = xx;
+                    xx = 1.0 / det;
+                } else {
+                    det = yy;
+                    yy = 1.0 / det;
+                }
+                singular = singular || det == cd(0.0) || !std::isfinite(xx.real()) || !std::isfinite(xx.imag()) ||
+                           !std::isfinite(xy.real()) || !std::isfinite(xy.imag()) || !std::isfinite(yx.real()) ||
+                           !std::isfinite(yx.imag()) || !std::isfinite(yy.real()) || !std::isfinite(yy.imag());
+                xx *= scale;
+                xy *= scale;
+                yx *= scale;
+                yy *= scale;
+            }
+            if (singular)
+                fail(RT_ESINGULAR, "rooftopMoM:preconditioner:singular",
+                     "The circulant approximation is singular at some frequency. Use another kernel or builder.");
+            if (!b_fits<double>(*pc, PassShape{3, 1, 1, 1, 1}))
+                fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
+                     "The blockwise circulant preconditioner needs the face grid's columns (%d) to fit one block's LDS.",
+                     d.My);
+            DeviceArray<c128> planes4;
+            planes4.allocate(ctx, 4 * full, "The circulant preconditioner (blocks)");
+            upload(blocks, planes4.ptr);
+            const PlanTables &tx = pc->tx.of(3), &ty = pc->ty.of(3);
+            const std::vector<int> fx = output_frequencies(tx.plan), fy = output_frequencies(ty.plan);
+            DeviceArray<int> dfx, dfy;
+            dfx.allocate(ctx, d.Mx, "FFT frequency table");
+            dfy.allocate(ctx, d.My, "FFT frequency table");
+            RT_HIP(hipMemcpyAsync(dfx.ptr, fx.data(), d.Mx * sizeof(int), hipMemcpyHostToDevice, s));
+            RT_HIP(hipMemcpyAsync(dfy.ptr, fy.data(), d.My * sizeof(int), hipMemcpyHostToDevice, s));
+            pc->four_plane = true;
+            if (pc->has(RT_C128)) {
+                pc->b4spec64.allocate(ctx, 4 * full, "The circulant preconditioner");
+                hipLaunchKernelGGL(build_b4spec<double>, dim3(blocks_for(full, 256, 4096)), dim3(256), 0, s,
+                                   planes4.ptr, d.Mx, d.My, dfx.ptr, dfy.ptr, pc->b4spec64.ptr);
+                RT_LAUNCH_CHECK();
+            }
+            if (pc->has(RT_C64)) {
+                pc->b4spec32.allocate(ctx, 4 * full, "The circulant preconditioner");
+                hipLaunchKernelGGL(build_b4spec<float>, dim3(blocks_for(full, 256, 4096)), dim3(256), 0, s,
+                                   planes4.ptr, d.Mx, d.My, dfx.ptr, dfy.ptr, pc->b4spec32.ptr);
+                RT_LAUNCH_CHECK();
+            }
+            RT_HIP(hipStreamSynchronize(s));
+            return pc;
+        }
+        // The superoptimal kernel with the partwise or hybrid builder: the three-plane form as for the folds, the
+        // spectra formed on the host (even for even generators: their [0, M/2] quarter is kept).
+        pc->planes64.allocate(ctx, 3 * plane_elems, "The circulant preconditioner");
+        pc->planes64.zero(s);
+        std::vector<cd> folded(plane_elems);
+        for (int k = 0; k < 3; ++k) {
+            if ((k == 0 && !g.has_x) || (k == 1 && !g.has_y)) continue;
+            TwoSided t;
+            if (hybrid && k < 2) {
+                t = spatial_block(k + 1, mxx, myy, e, g.cx, g.cy, g.a, g.b);
+            } else {
+                const std::vector<cd> &gen = k == 0 ? mxx : (k == 1 ? myy : e);
+                const int rows = k == 0 ? g.cx - 1 : g.cx, cols = k == 1 ? g.cy - 1 : g.cy;
+                t = two_sided(g.cx, g.cy, [&](int pp, int qq) { return host_at(gen, rows, cols, pp, qq); });
+            }
+            const std::vector<cd> spec = superoptimal_spectrum(t);
+            for (int fx = 0; fx < d.Hx; ++fx)
+                for (int fy = 0; fy < d.Hy; ++fy)
+                    folded[static_cast<size_t>(fx) * d.Hy + fy] = spec[fx + static_cast<size_t>(d.Mx) * fy];
+            upload(folded, pc->planes64.ptr + k * plane_elems);
+        }
+    } else {
+        pc->planes64.allocate(ctx, 3 * plane_elems, "The circulant preconditioner");
+        pc->planes64.zero(s);
+    }
     {
         const PlanTables &tx = pc->tx.of(3), &ty = pc->ty.of(3);
         DeviceArray<c128> tmp, folded;
@@ -625,7 +817,7 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
         const size_t la = static_cast<size_t>(padded_length(tx.plan)) * sizeof(c128);
         const size_t lb = static_cast<size_t>(padded_length(ty.plan)) * sizeof(c128);
         const int ex = 1 << tx.plan.log2E, ey = 1 << ty.plan.log2E;
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 3 && !on_host; ++k) {
             if (!gens[k]) continue;
             // Partwise: the folds of the generators; hybrid (generateHybridSpectra.m): the folds of the spatial
             // diagonal blocks XX, YY (generateSpatialBlocks.m) and of the electric generator.
@@ -715,7 +907,7 @@ static void time_shape(Product &op, int nrhs, int reps, const PassShape &sh, con
     // Algorithmic bytes of one application, whatever its chunks (each chunk's launch of pass B reads the spectra
     // again, which is traffic, not algorithmic bytes).
     double app[3];
-    pass_bytes<TC>(op.dims, nrhs, app);
+    pass_bytes<TC>(op.dims, nrhs, app, op.four_plane);
     for (int k = 0; k < 3; ++k) {
         // Per application: the chunks of one application add up.
         ms[k] = (ctx->timed_ms[slot0 + k] - ms0[k]) / reps;
@@ -864,7 +1056,7 @@ void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
 
 uint64_t Product::stored_bytes() const {
     return planes64.bytes() + bspec64[0].bytes() + bspec64[1].bytes() + bspec32[0].bytes() + bspec32[1].bytes() +
-           diag64.bytes() + diag32.bytes() + coef64.bytes() + coef32.bytes() + cls.bytes() + scls.bytes() +
+           b4spec64.bytes() + b4spec32.bytes() + diag64.bytes() + diag32.bytes() + coef64.bytes() + coef32.bytes() + cls.bytes() + scls.bytes() +
            scoef64.bytes() + scoef32.bytes();
 }
 

@@ -94,6 +94,11 @@ struct Product {
     // plan's position order, the three values of a position together. Built when a plan is first used.
     DeviceArray<c128> bspec64[2];
     DeviceArray<c64> bspec32[2];
+    // A blockwise circulant preconditioner: the inverse blocks (XX, XY, YX, YY) per frequency, unfolded, as pass B
+    // reads them: [x position][y position][4] in the radix-8 plans' orders.
+    bool four_plane = false;
+    DeviceArray<c128> b4spec64;
+    DeviceArray<c64> b4spec32;
     DeviceArray<c128> spec64;        // [nrhs][2][Mx][My]: full y spectra of the three-launch pass B (large periods)
     DeviceArray<c64> spec32;
     int spec_rhs64 = 0, spec_rhs32 = 0;

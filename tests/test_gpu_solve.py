@@ -285,8 +285,9 @@ def test_device_plane_waves_and_observables(rt):
 @pytest.mark.parametrize('size', [(8, 8, 0.1, 0.1), (16, 4, 0.1, 0.12), (4, 16, 0.05, 0.1), (32, 32, 0.1, 0.1),
                                   (8, 1, 0.1, 0.02), (1, 8, 0.03, 0.1)])
 def test_circulant_preconditioner_matches_oracle(rt, size):
-    """rooftopMoM.Preconditioner(problem, 'kernel', k, 'builder', 'partwise') on the GPU (Preconditioner.m:7-70)
-    against the oracle's partwise circulant (pinned to reference.mat), every GPU kernel, both precisions."""
+    """rooftopMoM.Preconditioner(problem, 'kernel', k, 'builder', b) on the GPU (Preconditioner.m:7-70) against the
+    oracle's circulants (pinned to reference.mat, the superoptimal definition and the dense transpose): every kernel,
+    the three builders, both precisions."""
     print('circulant', size, flush=True)
     from oracle import circulant as CI
     p, grid, gen = _setup(rt, *size, (4, 4))
@@ -295,7 +296,10 @@ def test_circulant_preconditioner_matches_oracle(rt, size):
         (grid.count, 2))
     for kernel, bw, builder in (('strang', None, 'partwise'), ('tChan', None, 'partwise'), ('rChan', None, 'partwise'),
                                 ('huckle', (3, 5), 'partwise'), ('huckle', None, 'partwise'),
-                                ('rChan', None, 'hybrid'), ('strang', None, 'hybrid'), ('huckle', (2, 4), 'hybrid')):
+                                ('rChan', None, 'hybrid'), ('strang', None, 'hybrid'), ('huckle', (2, 4), 'hybrid'),
+                                ('superoptimal', None, 'partwise'), ('superoptimal', None, 'hybrid'),
+                                ('rChan', None, 'blockwise'), ('strang', None, 'blockwise'), ('tChan', None, 'blockwise'),
+                                ('huckle', (2, 4), 'blockwise'), ('superoptimal', None, 'blockwise')):
         pc = rt.Preconditioner(p, kernel=kernel, bandwidth=bw, builder=builder, precision='mixed')
         inv = CI.circulant_preconditioner(g, grid, kernel, bw, builder)
         ref = np.stack([CI.apply(inv, grid, x[:, r]) for r in range(2)], axis=1)
@@ -307,17 +311,12 @@ def test_circulant_preconditioner_matches_oracle(rt, size):
 
 
 def test_circulant_preconditioner_errors_and_solve(rt):
-    """A singular approximation raises rooftopMoM:preconditioner:singular (Preconditioner.m:57-65); the
-    kernels and grids the GPU lacks are refused; preconditioned COCG and GMRES converge in fewer iterations
-    (PreconditionerTest.m:44-55) and GMRES follows the oracle's iterations with the same preconditioner."""
+    """A singular approximation raises rooftopMoM:preconditioner:singular (Preconditioner.m:57-65), with every
+    builder; face grids that are not powers of two are refused; preconditioned COCG and GMRES converge in fewer
+    iterations (PreconditionerTest.m:44-55, the partwise and the blockwise builders) and GMRES follows the oracle's
+    iterations with the same preconditioner."""
     from oracle import circulant as CI
     p, grid, gen = _setup(rt, 16, 16, 0.1, 0.1, (4, 4))
-    with pytest.raises(rt.RooftopError) as e:
-        rt.Preconditioner(p, kernel='superoptimal')
-    assert e.value.identifier == 'rooftop:preconditioner:unsupported'
-    with pytest.raises(rt.RooftopError) as e:
-        rt.Preconditioner(p, kernel='rChan', builder='blockwise')
-    assert e.value.identifier == 'rooftop:preconditioner:unsupported'
     with pytest.raises(rt.RooftopError) as e:
         rt.Preconditioner(p, kernel='huckle', bandwidth=(2,))
     assert e.value.identifier == 'rooftopMoM:preconditioner:invalidBandwidth'
@@ -329,16 +328,20 @@ def test_circulant_preconditioner_errors_and_solve(rt):
     zeros = z.generators()
     z.setGenerators(*[None if zeros[k] is None else np.zeros_like(zeros[k])
                       for k in ('magneticXX', 'magneticYY', 'electric')])
-    with pytest.raises(rt.RooftopError) as e:
-        rt.Preconditioner(z, kernel='rChan')
-    assert e.value.identifier == 'rooftopMoM:preconditioner:singular'
+    for builder in ('partwise', 'blockwise', 'hybrid'):
+        with pytest.raises(rt.RooftopError) as e:
+            rt.Preconditioner(z, kernel='rChan', builder=builder)
+        assert e.value.identifier == 'rooftopMoM:preconditioner:singular', builder
     Z = S.dense_matrix(gen, grid)
     b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [1, 0, 0], 1.0, order=4)
     op = rt.Operator(p)
     pc = rt.Preconditioner(p)   # without a kernel this library's default, the near field
     circ = rt.Preconditioner(p, kernel='rChan')
     inv = CI.circulant_preconditioner(p.generators(), grid, 'rChan')
+    block = rt.Preconditioner(p, kernel='rChan', builder='blockwise')
     for method in ('cg', 'gmres'):
+        res = rt.Solver(method, tolerance=1e-8, restart=30).solve(op, b, preconditioner=block)
+        assert res.converged() and rel(Z @ res.solution(), b) < 1e-8
         plain = rt.Solver(method, tolerance=1e-8, restart=30).solve(op, b)
         res = rt.Solver(method, tolerance=1e-8, restart=30).solve(op, b, preconditioner=circ)
         assert res.converged() and res.iterationCount() < plain.iterationCount()
