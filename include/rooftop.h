@@ -142,8 +142,9 @@ int rt_precond_create(rt_problem *problem, int32_t kind, int32_t precision, rt_p
  * approximation of the Green's matrix on the face grid, applied by FFTs of the face grid (FFTApplicator.m): the
  * partwise, blockwise and hybrid builders (generatePartwiseSpectra.m:16-30, generateBlockwiseSpectra.m,
  * generateHybridSpectra.m) with the strang, tChan, rChan, huckle and superoptimal kernels (+preconditioner/+kernels),
- * on face counts that are powers of two; other grids, and blockwise face grids whose columns do not fit one block's
- * LDS, return RT_EINVAL with rooftop:preconditioner:unsupported. bandwidth_x/y: Huckle's bandwidths in faces (<= 0: the grid's counts);
+ * on any face grid (power-of-two counts run on the face grid, others through a re-embedded Toeplitz product); a
+ * four-plane product whose columns do not fit one block's LDS returns RT_EINVAL with rooftop:preconditioner:unsupported.
+ * bandwidth_x/y: Huckle's bandwidths in faces (<= 0: the grid's counts);
  * RT_ESINGULAR with rooftopMoM:preconditioner:singular when the approximation is singular at some frequency. */
 int rt_precond_create_circulant(rt_problem *problem, int32_t kernel, int32_t builder, int32_t bandwidth_x,
                                 int32_t bandwidth_y, int32_t modifier, int32_t precision, rt_product **out);

@@ -1,8 +1,9 @@
 // Host side of the circulant preconditioners that the device folds do not cover: the superoptimal kernel of
 // Tyrtyshnikov (+preconditioner/+kernels/generateSuperoptimal.m with +utils/generatePartSpectra.m and
 // +utils/generateOptimalOfProduct.m) and the kernels applied to generators that are not even (the XY coupling block of
-// the blockwise builder, generateBlockwiseSpectra.m). These are one-off set-up computations on count_x x count_y
-// grids (powers of two), in FP64, with a radix-2 FFT of their own.
+// the blockwise builder, generateBlockwiseSpectra.m), and every kernel and builder on face grids that are not powers of
+// two. These are one-off set-up computations on count_x x count_y grids, in FP64, with an FFT of their own (radix 2,
+// Bluestein's chirp transform for other lengths).
 #include "circulant_host.hpp"
 
 #include <cmath>
@@ -17,7 +18,7 @@ constexpr double kPi = 3.141592653589793;
 
 // In-place radix-2 FFT of n (a power of two) values at a[0], a[stride], ...: forward sum x(k) exp(-2 pi i k j / n),
 // inverse with the opposite sign and no scaling.
-void fft1(cd *a, int n, int64_t stride, bool inverse) {
+void fft_pow2(cd *a, int n, int64_t stride, bool inverse) {
     for (int i = 1, j = 0; i < n; ++i) {
         int bit = n >> 1;
         for (; j & bit; bit >>= 1) j ^= bit;
@@ -36,6 +37,32 @@ void fft1(cd *a, int n, int64_t stride, bool inverse) {
             }
         }
     }
+}
+
+// The same for any n: Bluestein's chirp transform, j k = (j^2 + k^2 - (k - j)^2) / 2, turns the DFT into a
+// convolution with the chirp exp(+-i pi m^2 / n), done by power-of-two FFTs of length >= 2 n - 1.
+void fft1(cd *a, int n, int64_t stride, bool inverse) {
+    if ((n & (n - 1)) == 0) {
+        fft_pow2(a, n, stride, inverse);
+        return;
+    }
+    const double sgn = inverse ? 1.0 : -1.0;
+    int L = 1;
+    while (L < 2 * n - 1) L <<= 1;
+    std::vector<cd> chirp(n), u(L, 0.0), v(L, 0.0);
+    for (int m = 0; m < n; ++m) {
+        // m^2 mod 2n keeps the angle small and exact.
+        const long long m2 = (static_cast<long long>(m) * m) % (2LL * n);
+        chirp[m] = std::polar(1.0, sgn * kPi * static_cast<double>(m2) / n);
+    }
+    for (int m = 0; m < n; ++m) u[m] = a[m * stride] * chirp[m];
+    v[0] = std::conj(chirp[0]);
+    for (int m = 1; m < n; ++m) v[m] = v[L - m] = std::conj(chirp[m]);
+    fft_pow2(u.data(), L, 1, false);
+    fft_pow2(v.data(), L, 1, false);
+    for (int i = 0; i < L; ++i) u[i] *= v[i];
+    fft_pow2(u.data(), L, 1, true);
+    for (int k = 0; k < n; ++k) a[k * stride] = u[k] * chirp[k] / static_cast<double>(L);
 }
 
 // One-dimensional transforms of every column (axis 0, along x) or every row (axis 1, along y) of a[x + cx y].
@@ -205,6 +232,111 @@ std::vector<cd> kernel_spectrum(const TwoSided &t, int kind, double bwx, double 
     std::vector<cd> c = fold_two_sided(t, kind, bwx, bwy);
     fft2_host(c, t.cx, t.cy, false);
     return c;
+}
+
+cd host_at(const std::vector<cd> &g, int rows, int cols, int p, int q) {
+    if (p < 0) p = -p;
+    if (q < 0) q = -q;
+    if (p >= rows || q >= cols) return 0.0;
+    return g[static_cast<size_t>(q) * rows + p];
+}
+
+TwoSided even_extension(const std::vector<cd> &g, int rows, int cols, int cx, int cy) {
+    return two_sided(cx, cy, [&](int p, int q) { return host_at(g, rows, cols, p, q); });
+}
+
+TwoSided spatial_block(int block, const HostGenerators &g) {
+    const int cx = g.cx, cy = g.cy;
+    const double a = g.a, b = g.b;
+    auto E = [&](int p, int q) { return host_at(g.e, cx, cy, p, q); };
+    return two_sided(cx, cy, [&](int p, int q) -> cd {
+        if (block == 1) {
+            if (std::abs(p) > cx - 2) return 0.0;
+            return host_at(g.mxx, cx - 1, cy, p, q) - (2.0 * E(p, q) - E(p - 1, q) - E(p + 1, q)) / (a * a);
+        }
+        if (block == 2) {
+            if (std::abs(q) > cy - 2) return 0.0;
+            return host_at(g.myy, cx, cy - 1, p, q) - (2.0 * E(p, q) - E(p, q - 1) - E(p, q + 1)) / (b * b);
+        }
+        if (p > cx - 2 || q < -(cy - 2)) return 0.0;
+        return -(E(p, q) - E(p, q - 1) - E(p + 1, q) + E(p + 1, q - 1)) / (a * b);
+    });
+}
+
+std::array<std::vector<cd>, 4> circulant_blocks(const HostGenerators &g, int kernel, int builder, double bwx,
+                                                double bwy) {
+    const int cx = g.cx, cy = g.cy;
+    const size_t n = static_cast<size_t>(cx) * cy;
+    std::array<std::vector<cd>, 4> out;   // XX, XY, YX, YY
+    auto spectrum = [&](const TwoSided &t) { return kernel_spectrum(t, kernel, bwx, bwy); };
+    if (builder == RT_BUILDER_BLOCKWISE) {
+        // generateBlockwiseSpectra.m: every spatial block's circulant; YX the transpose of XY (negative frequencies).
+        if (g.has_x) out[0] = spectrum(spatial_block(1, g));
+        if (g.has_y) out[3] = spectrum(spatial_block(2, g));
+        if (g.has_x && g.has_y) {
+            out[1] = spectrum(spatial_block(3, g));
+            out[2].resize(n);
+            for (int fy = 0; fy < cy; ++fy)
+                for (int fx = 0; fx < cx; ++fx)
+                    out[2][fx + static_cast<size_t>(cx) * fy] = out[1][(cx - fx) % cx + static_cast<size_t>(cx) * ((cy - fy) % cy)];
+        }
+        return out;
+    }
+    // Partwise (generatePartwiseSpectra.m) and hybrid (generateHybridSpectra.m): the electric circulant couples the
+    // blocks as in generateBlockSpectra.m, Dx = (1 - exp(2 pi i fx / cx)) / a; the diagonal blocks from the magnetic
+    // generators' circulants (partwise) or the spatial blocks' (hybrid).
+    const std::vector<cd> E = spectrum(even_extension(g.e, cx, cy, cx, cy));
+    std::vector<cd> A, B;
+    if (builder == RT_BUILDER_HYBRID) {
+        if (g.has_x) A = spectrum(spatial_block(1, g));
+        if (g.has_y) B = spectrum(spatial_block(2, g));
+    } else {
+        if (g.has_x) A = spectrum(even_extension(g.mxx, cx - 1, cy, cx, cy));
+        if (g.has_y) B = spectrum(even_extension(g.myy, cx, cy - 1, cx, cy));
+    }
+    if (g.has_x) out[0].resize(n);
+    if (g.has_y) out[3].resize(n);
+    if (g.has_x && g.has_y) {
+        out[1].resize(n);
+        out[2].resize(n);
+    }
+    for (int fy = 0; fy < cy; ++fy)
+        for (int fx = 0; fx < cx; ++fx) {
+            const size_t i = fx + static_cast<size_t>(cx) * fy;
+            const cd dx = (1.0 - std::polar(1.0, 2 * kPi * fx / cx)) / g.a;
+            const cd dy = (1.0 - std::polar(1.0, 2 * kPi * fy / cy)) / g.b;
+            const bool given = builder == RT_BUILDER_HYBRID;
+            if (g.has_x) out[0][i] = given ? A[i] : A[i] - std::norm(dx) * E[i];
+            if (g.has_y) out[3][i] = given ? B[i] : B[i] - std::norm(dy) * E[i];
+            if (g.has_x && g.has_y) {
+                out[1][i] = -dx * std::conj(dy) * E[i];
+                out[2][i] = -std::conj(dx) * dy * E[i];
+            }
+        }
+    return out;
+}
+
+bool invert_blocks(std::array<std::vector<cd>, 4> &blk, bool has_x, bool has_y) {
+    const size_t n = has_x ? blk[0].size() : blk[3].size();
+    bool ok = true;
+    auto finite = [](cd v) { return std::isfinite(v.real()) && std::isfinite(v.imag()); };
+    for (size_t f = 0; f < n; ++f) {
+        if (has_x && has_y) {
+            const cd xx = blk[0][f], xy = blk[1][f], yx = blk[2][f], yy = blk[3][f];
+            const cd det = xx * yy - xy * yx;
+            blk[0][f] = yy / det;
+            blk[1][f] = -xy / det;
+            blk[2][f] = -yx / det;
+            blk[3][f] = xx / det;
+            ok = ok && det != cd(0.0) && finite(blk[0][f]) && finite(blk[1][f]) && finite(blk[2][f]) && finite(blk[3][f]);
+        } else {
+            cd &v = has_x ? blk[0][f] : blk[3][f];
+            const cd det = v;
+            v = 1.0 / det;
+            ok = ok && det != cd(0.0) && finite(v);
+        }
+    }
+    return ok;
 }
 
 }  // namespace rt

@@ -285,18 +285,26 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
     const cplx<TC> *dy = dbl ? reinterpret_cast<const cplx<TC> *>(ty.d.ptr)
                              : reinterpret_cast<const cplx<TC> *>(ty.d32.ptr);
     if (!b_threads_fit(p, tile, teams)) fail(RT_EINVAL, "rooftop:product", "pass B shape does not fit this plan");
-    if (!op.embedding && (p.bits != 3 || teams != 1))
-        fail(RT_EINVAL, "rooftop:product", "a face-grid product runs the radix-8 plan with one team per position");
+    if ((!op.embedding || op.four_plane) && (p.bits != 3 || teams != 1))
+        fail(RT_EINVAL, "rooftop:product",
+             "a face-grid or four-plane product runs the radix-8 plan with one team per position");
 #define RT_LAUNCH_B(L, B)                                                                                          \
     {                                                                                                              \
         using PL = CPlan<L, B>;                                                                                    \
-        if (!op.embedding && op.four_plane) {                                                                      \
+        if (op.four_plane) {                                                                                       \
             using PF = CPlan<L, 3>;                                                                                \
             constexpr int MT = PF::T > 256 ? PF::T : 256;                                                          \
-            set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, 3, MT, false, true>), lds);                    \
-            launch_timed(timer, pass_b_k<TC, L, 3, MT, false, true>, dim3(xcd_grid(nb)), dim3(threads),            \
-                         static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, nrhs, \
-                         static_cast<int>(nb), skip, skip_stride);                                                 \
+            if (op.embedding) {                                                                                    \
+                set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, 3, MT, true, true>), lds);                 \
+                launch_timed(timer, pass_b_k<TC, L, 3, MT, true, true>, dim3(xcd_grid(nb)), dim3(threads),         \
+                             static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, \
+                             nrhs, static_cast<int>(nb), skip, skip_stride);                                       \
+            } else {                                                                                               \
+                set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, 3, MT, false, true>), lds);                \
+                launch_timed(timer, pass_b_k<TC, L, 3, MT, false, true>, dim3(xcd_grid(nb)), dim3(threads),        \
+                             static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, \
+                             nrhs, static_cast<int>(nb), skip, skip_stride);                                       \
+            }                                                                                                      \
         } else if (!op.embedding) {                                                                                \
             using PF = CPlan<L, 3>;                                                                                \
             constexpr int MT = PF::T > 256 ? PF::T : 256;                                                          \

@@ -171,7 +171,7 @@ static PassShape default_shape(const Product &op, int nrhs, int bits = 0) {
     const ProdDims &d = op.dims;
     PassShape s;
     s.bits = bits ? bits : sizeof(TC) == sizeof(float) ? 4 : 3;
-    if (!op.embedding) s.bits = 3;   // the face-grid passes are built for the radix-8 plans
+    if (!op.embedding || op.four_plane) s.bits = 3;   // face-grid and four-plane passes: the radix-8 plans
     const FftPlan &px = op.tx.of(s.bits).plan, &py = op.ty.of(s.bits).plan;
     const int es = sizeof(cplx<TC>);
     int g = std::max(1, 64 / es);
@@ -585,15 +585,7 @@ __global__ void build_b4spec(const c128 *__restrict__ planes, int Mx, int My, co
     }
 }
 
-// Host twins of generator_at and spatial_block_at for the spectra formed on the host: the superoptimal kernel and
-// the blockwise builder (whose XY block is not even).
-static cd host_at(const std::vector<cd> &g, int rows, int cols, int p, int q) {
-    if (p < 0) p = -p;
-    if (q < 0) q = -q;
-    if (p >= rows || q >= cols) return 0.0;
-    return g[static_cast<size_t>(q) * rows + p];
-}
-
+// The generators on the host, after the modifier, for the spectra formed there (circulant_host.cpp).
 static std::vector<cd> host_generator(const DeviceArray<c128> &g, size_t count, int modifier, hipStream_t s) {
     std::vector<c128> raw(count);
     RT_HIP(hipMemcpyAsync(raw.data(), g.ptr, sizeof(c128) * count, hipMemcpyDeviceToHost, s));
@@ -608,35 +600,6 @@ static std::vector<cd> host_generator(const DeviceArray<c128> &g, size_t count, 
     return out;
 }
 
-template <class F>
-static TwoSided two_sided(int cx, int cy, F f) {
-    TwoSided t;
-    t.cx = cx;
-    t.cy = cy;
-    t.v.resize(static_cast<size_t>(2 * cx - 1) * (2 * cy - 1));
-    for (int q = -(cy - 1); q < cy; ++q)
-        for (int p = -(cx - 1); p < cx; ++p) t.v[(p + cx - 1) + static_cast<size_t>(2 * cx - 1) * (q + cy - 1)] = f(p, q);
-    return t;
-}
-
-// The two-sided spatial blocks (generateSpatialBlocks.m:19-61) of the host generators: 1 XX, 2 YY, 3 XY.
-static TwoSided spatial_block(int block, const std::vector<cd> &mxx, const std::vector<cd> &myy,
-                              const std::vector<cd> &e, int cx, int cy, double a, double b) {
-    auto E = [&](int p, int q) { return host_at(e, cx, cy, p, q); };
-    return two_sided(cx, cy, [&](int p, int q) -> cd {
-        if (block == 1) {
-            if (std::abs(p) > cx - 2) return 0.0;
-            return host_at(mxx, cx - 1, cy, p, q) - (2.0 * E(p, q) - E(p - 1, q) - E(p + 1, q)) / (a * a);
-        }
-        if (block == 2) {
-            if (std::abs(q) > cy - 2) return 0.0;
-            return host_at(myy, cx, cy - 1, p, q) - (2.0 * E(p, q) - E(p, q - 1) - E(p, q + 1)) / (b * b);
-        }
-        if (p > cx - 2 || q < -(cy - 2)) return 0.0;
-        return -(E(p, q) - E(p, q - 1) - E(p + 1, q) + E(p + 1, q - 1)) / (a * b);
-    });
-}
-
 std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, int bandwidth_x, int bandwidth_y,
                                           int modifier, int precision) {
     Context *ctx = p->ctx;
@@ -649,17 +612,18 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
              "rChan, huckle or superoptimal kernel.");
     const bool hybrid = builder == RT_BUILDER_HYBRID;
     const bool blockwise = builder == RT_BUILDER_BLOCKWISE;
-    const bool on_host = blockwise || kernel == RT_CIRCULANT_SUPEROPTIMAL;
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
-    if (!pow2(g.cx) || !pow2(g.cy))
-        fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
-             "The GPU circulant preconditioner needs face counts that are powers of two (the grid is %d x %d).", g.cx,
-             g.cy);
+    // On a face grid of power-of-two counts the passes run on the face grid itself; on any other the inverse's block
+    // circulants are re-embedded as Toeplitz products (generator c(p mod cx, q mod cy) at the offsets |p| < cx,
+    // |q| < cy) and applied by the operator's power-of-two embedding, whose inputs are zero and outputs cut beyond the
+    // face grid, so the product is the circulant product.
+    const bool face_grid = pow2(g.cx) && pow2(g.cy);
+    const bool on_host = blockwise || kernel == RT_CIRCULANT_SUPEROPTIMAL || !face_grid;
     if (kernel == RT_CIRCULANT_HUCKLE && (bandwidth_x < 1 || bandwidth_y < 1))
         fail(RT_EINVAL, "rooftopMoM:preconditioner:invalidBandwidth", "The bandwidth is a pair of positive integers.");
     auto pc = std::make_unique<Product>();
     pc->type = kCirculant;
-    pc->embedding = false;
+    pc->embedding = !face_grid;
     pc->problem = p;
     pc->precision = precision;
     pc->modifier = modifier;
@@ -672,8 +636,8 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
     d.ldj = (d.ny + 7) / 8 * 8;
     d.nx_count = g.nx;
     d.n = g.n;
-    d.Mx = g.cx;
-    d.My = g.cy;
+    d.Mx = face_grid ? g.cx : (g.cx > 1 ? next_pow2(2L * g.cx - 1) : 1);
+    d.My = face_grid ? g.cy : (g.cy > 1 ? next_pow2(2L * g.cy - 1) : 1);
     d.Hx = d.Mx / 2 + 1;
     d.Hy = d.My / 2 + 1;
     const uint64_t plane_elems = static_cast<uint64_t>(d.Hx) * d.Hy;
@@ -684,74 +648,56 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
     if (on_host) {
         // Spectra formed on the host: the generators (after the modifier), their or their spatial blocks'
         // circulants, then the inverse.
-        const std::vector<cd> e = host_generator(p->e, static_cast<size_t>(g.cx) * g.cy, modifier, s);
-        const std::vector<cd> mxx = g.has_x ? host_generator(p->mxx, static_cast<size_t>(g.cx - 1) * g.cy, modifier, s)
-                                            : std::vector<cd>();
-        const std::vector<cd> myy = g.has_y ? host_generator(p->myy, static_cast<size_t>(g.cx) * (g.cy - 1), modifier, s)
-                                            : std::vector<cd>();
+        HostGenerators hg;
+        hg.cx = g.cx;
+        hg.cy = g.cy;
+        hg.a = g.a;
+        hg.b = g.b;
+        hg.has_x = g.has_x;
+        hg.has_y = g.has_y;
+        hg.e = host_generator(p->e, static_cast<size_t>(g.cx) * g.cy, modifier, s);
+        if (g.has_x) hg.mxx = host_generator(p->mxx, static_cast<size_t>(g.cx - 1) * g.cy, modifier, s);
+        if (g.has_y) hg.myy = host_generator(p->myy, static_cast<size_t>(g.cx) * (g.cy - 1), modifier, s);
         const double bwx = bandwidth_x, bwy = bandwidth_y;
-        const int64_t full = static_cast<int64_t>(d.Mx) * d.My;
-        auto unfold = [&](const std::vector<cd> &spec, std::vector<cd> &out, int64_t offset) {   // [fx + Mx fy]
-            for (int fx = 0; fx < d.Mx; ++fx)
-                for (int fy = 0; fy < d.My; ++fy) out[offset + static_cast<int64_t>(fx) * d.My + fy] = spec[fx + static_cast<size_t>(d.Mx) * fy];
-        };
         auto upload = [&](const std::vector<cd> &h, c128 *dst) {
             std::vector<c128> raw(h.size());
             for (size_t i = 0; i < h.size(); ++i) raw[i] = mk(h[i].real(), h[i].imag());
             RT_HIP(hipMemcpyAsync(dst, raw.data(), sizeof(c128) * raw.size(), hipMemcpyHostToDevice, s));
             RT_HIP(hipStreamSynchronize(s));
         };
-        if (blockwise) {
-            // generateBlockwiseSpectra.m: the kernel's circulant of every spatial block; YX is the transpose of XY,
-            // its spectrum taken at the negative frequencies. Inverted frequency by frequency (@Spectra/invert.m) and
-            // kept unfolded, four planes, for the four-plane multiply of pass B.
-            std::vector<cd> blocks(4 * full, 0.0);
-            if (g.has_x) unfold(kernel_spectrum(spatial_block(1, mxx, myy, e, g.cx, g.cy, g.a, g.b), kernel, bwx, bwy), blocks, 0);
-            if (g.has_y) unfold(kernel_spectrum(spatial_block(2, mxx, myy, e, g.cx, g.cy, g.a, g.b), kernel, bwx, bwy), blocks, 3 * full);
-            if (g.has_x && g.has_y) {
-                unfold(kernel_spectrum(spatial_block(3, mxx, myy, e, g.cx, g.cy, g.a, g.b), kernel, bwx, bwy), blocks, full);
-                for (int fx = 0; fx < d.Mx; ++fx)
-                    for (int fy = 0; fy < d.My; ++fy)
-                        blocks[2 * full + static_cast<int64_t>(fx) * d.My + fy] =
-                            blocks[full + static_cast<int64_t>((d.Mx - fx) % d.Mx) * d.My + (d.My - fy) % d.My];
-            }
-            const double scale = 1.0 / static_cast<double>(full);
-            bool singular = false;
-            for (int64_t f = 0; f < full; ++f) {
-                cd &xx = blocks[f], &xy = blocks[full + f], &yx = blocks[2 * full + f], &yy = blocks[3 * full + f];
-                cd det;
-                if (g.has_x && g.has_y) {
-                    det = xx * yy - xy * yx;
-                    const cd a2 = yy / det, b2 = -xy / det, c2 = -yx / det, d2 = xx / det;
-                    xx = a2;
-                    xy = b2;
-                    yx = c2;
-                    yy = d2;
-                } else if (g.has_x) {
-                    det = xx;
-                    xx = 1.0 / det;
-                } else {
-                    det = yy;
-                    yy = 1.0 / det;
-                }
-                singular = singular || det == cd(0.0) || !std::isfinite(xx.real()) || !std::isfinite(xx.imag()) ||
-                           !std::isfinite(xy.real()) || !std::isfinite(xy.imag()) || !std::isfinite(yx.real()) ||
-                           !std::isfinite(yx.imag()) || !std::isfinite(yy.real()) || !std::isfinite(yy.imag());
-                xx *= scale;
-                xy *= scale;
-                yx *= scale;
-                yy *= scale;
-            }
-            if (singular)
+        if (blockwise || !face_grid) {
+            // The inverse blocks (XX', XY', YX', YY') per frequency of the cx x cy period, kept unfolded as four
+            // planes for the four-plane multiply of pass B: on the face grid directly, or re-embedded.
+            std::array<std::vector<cd>, 4> blocks = circulant_blocks(hg, kernel, builder, bwx, bwy);
+            if (!invert_blocks(blocks, g.has_x, g.has_y))
                 fail(RT_ESINGULAR, "rooftopMoM:preconditioner:singular",
                      "The circulant approximation is singular at some frequency. Use another kernel or builder.");
             if (!b_fits<double>(*pc, PassShape{3, 1, 1, 1, 1}))
                 fail(RT_EINVAL, "rooftop:preconditioner:unsupported",
-                     "The blockwise circulant preconditioner needs the face grid's columns (%d) to fit one block's LDS.",
-                     d.My);
+                     "The four-plane circulant product needs columns of %d that fit one block's LDS.", d.My);
+            const int64_t full = static_cast<int64_t>(d.Mx) * d.My;
+            const double scale = 1.0 / static_cast<double>(full);
+            std::vector<cd> planes(4 * full, 0.0);   // [c][fx My + fy]
+            for (int c = 0; c < 4; ++c) {
+                if (blocks[c].empty()) continue;
+                std::vector<cd> spec = std::move(blocks[c]);   // [fx + cx fy]
+                if (!face_grid) {
+                    fft2_host(spec, g.cx, g.cy, true);   // the block circulant's generator
+                    std::vector<cd> emb(full, 0.0);
+                    for (int q = -(g.cy - 1); q < g.cy; ++q)
+                        for (int pp = -(g.cx - 1); pp < g.cx; ++pp)
+                            emb[(pp + d.Mx) % d.Mx + static_cast<size_t>(d.Mx) * ((q + d.My) % d.My)] =
+                                spec[(pp + g.cx) % g.cx + static_cast<size_t>(g.cx) * ((q + g.cy) % g.cy)];
+                    fft2_host(emb, d.Mx, d.My, false);
+                    spec = std::move(emb);
+                }
+                for (int fx = 0; fx < d.Mx; ++fx)
+                    for (int fy = 0; fy < d.My; ++fy)
+                        planes[c * full + static_cast<int64_t>(fx) * d.My + fy] = scale * spec[fx + static_cast<size_t>(d.Mx) * fy];
+            }
             DeviceArray<c128> planes4;
             planes4.allocate(ctx, 4 * full, "The circulant preconditioner (blocks)");
-            upload(blocks, planes4.ptr);
+            upload(planes, planes4.ptr);
             const PlanTables &tx = pc->tx.of(3), &ty = pc->ty.of(3);
             const std::vector<int> fx = output_frequencies(tx.plan), fy = output_frequencies(ty.plan);
             DeviceArray<int> dfx, dfy;
@@ -784,11 +730,10 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
             if ((k == 0 && !g.has_x) || (k == 1 && !g.has_y)) continue;
             TwoSided t;
             if (hybrid && k < 2) {
-                t = spatial_block(k + 1, mxx, myy, e, g.cx, g.cy, g.a, g.b);
+                t = spatial_block(k + 1, hg);
             } else {
-                const std::vector<cd> &gen = k == 0 ? mxx : (k == 1 ? myy : e);
-                const int rows = k == 0 ? g.cx - 1 : g.cx, cols = k == 1 ? g.cy - 1 : g.cy;
-                t = two_sided(g.cx, g.cy, [&](int pp, int qq) { return host_at(gen, rows, cols, pp, qq); });
+                const std::vector<cd> &gen = k == 0 ? hg.mxx : (k == 1 ? hg.myy : hg.e);
+                t = even_extension(gen, k == 0 ? g.cx - 1 : g.cx, k == 1 ? g.cy - 1 : g.cy, g.cx, g.cy);
             }
             const std::vector<cd> spec = superoptimal_spectrum(t);
             for (int fx = 0; fx < d.Hx; ++fx)
