@@ -279,8 +279,8 @@ constexpr int kPartZ = 64, kPartScalars = 125;       // s at k, z at kPartZ + k,
 constexpr size_t kDotsRedBytes = 2 * 4 * kTileRows * sizeof(c128);
 
 // partial[(r * blocks + b) * kLdp2 + ...] = the inner products above over block b's rows. The live columns of a tile of
-// 64 rows are staged in LDS with p and w (row stride 65 keeps the column-wise reads off shared banks); lane k of wave
-// q sums column k over the rows i = q mod 4, wave 0 sums alpha, t and omega. Dynamic LDS: cnt * 65 basis elements.
+// 64 rows are staged in LDS with p and w (row stride 65 keeps the column-wise reads off shared banks); the columns' sums
+// are split over row groups (below), wave 0 sums alpha, t and omega. Dynamic LDS: cnt * 65 basis elements.
 template <typename VT>
 __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V, int cnt, const c128 *__restrict__ p,
                                                         const c128 *__restrict__ w, int64_t n, int64_t chunk,
@@ -292,21 +292,27 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
     VT *tile = reinterpret_cast<VT *>(tile_raw);
     c128(*red)[4 * kTileRows] = reinterpret_cast<c128(*)[4 * kTileRows]>(tile_raw);
     __shared__ c128 pt[kTileRows], wt[kTileRows];
-    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    // The wave index is uniform: the columns' base addresses live in scalar registers and the 16 loads of a thread
+    // share one 32-bit row offset.
+    const int lane = threadIdx.x & 63, q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const c128 *pr = p + r * n, *wr = w + r * n;
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
-    double sre = 0, sim = 0, zre = 0, zim = 0;   // column `lane`
+    // The dot phase: thread (k, g) sums column k over the rows i = g mod G, with G = 256 / cnt row groups, so that
+    // every thread of the block works however few the columns.
+    const int groups = cnt > 0 ? min(kTileRows, kThreads / cnt) : 1;
+    const int dk = cnt > 0 ? static_cast<int>(threadIdx.x) % cnt : 0, dg = cnt > 0 ? static_cast<int>(threadIdx.x) / cnt : groups;
+    double sre = 0, sim = 0, zre = 0, zim = 0;   // column dk, rows dg mod groups
     double al = 0, tre = 0, tim = 0, om = 0;     // wave 0: row `lane` of every tile
     constexpr int kPer = kLdp * kTileRows / kThreads;
     for (int64_t t0 = base; t0 < end; t0 += kTileRows) {
         const int rows = static_cast<int>(min(static_cast<int64_t>(kTileRows), end - t0));
         // All loads of the tile are issued before any is stored: the staging costs one memory latency.
+        // Thread (q, lane) loads the columns k = q + 4 u (uniform over the wave) at the row `lane`.
         VT vals[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            const int k = e >> 6, i = e & (kTileRows - 1);
-            vals[u] = (k < cnt && i < rows) ? basis_col(V, k, r, R, n)[t0 + i] : VT{0, 0};
+            const int k = q + 4 * u;
+            vals[u] = (k < cnt && lane < rows) ? basis_col(V, k, r, R, n)[t0 + lane] : VT{0, 0};
         }
         c128 pv = mk(0.0, 0.0), wv = mk(0.0, 0.0);
         if (q == 0 && lane < rows) {
@@ -316,9 +322,8 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
         __syncthreads();   // the previous tile is consumed
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
-            const int e = threadIdx.x + u * kThreads;
-            const int k = e >> 6, i = e & (kTileRows - 1);
-            if (k < cnt) tile[k * kTileStride + i] = vals[u];
+            const int k = q + 4 * u;
+            if (k < cnt) tile[k * kTileStride + lane] = vals[u];
         }
         if (q == 0) {
             pt[lane] = pv;
@@ -330,9 +335,9 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
             om += abs2(wv);
         }
         __syncthreads();
-        if (lane < cnt) {
-            const VT *col = tile + lane * kTileStride;
-            for (int i = q; i < rows; i += 4) {
+        if (dg < groups) {
+            const VT *col = tile + dk * kTileStride;
+            for (int i = dg; i < rows; i += groups) {
                 const c128 v = widen(col[i]), pi = pt[i], wi = wt[i];
                 sre += v.x * pi.x + v.y * pi.y;
                 sim += v.x * pi.y - v.y * pi.x;
@@ -342,8 +347,10 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
         }
     }
     __syncthreads();   // the tile is consumed: its bytes take the sums
-    red[0][q * kTileRows + lane] = mk(sre, sim);
-    red[1][q * kTileRows + lane] = mk(zre, zim);
+    if (dg < groups) {
+        red[0][dg * cnt + dk] = mk(sre, sim);
+        red[1][dg * cnt + dk] = mk(zre, zim);
+    }
     if (q == 0) {
         al = wave_sum(al);
         tre = wave_sum(tre);
@@ -354,10 +361,11 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
     // Slot-major partials, [R][slot][block]: the reduction reads each slot's blocks contiguously.
     const int nb = gridDim.x;
     c128 *out = partial + static_cast<int64_t>(r) * kLdp2 * nb + blockIdx.x;
-    if (q < 2 && lane < cnt) {
+    if (q < 2 && lane < cnt) {   // the groups' sums in a fixed order
         const c128 *v = red[q];
-        out[static_cast<int64_t>(q * kPartZ + lane) * nb] =
-            (v[lane] + v[kTileRows + lane]) + (v[2 * kTileRows + lane] + v[3 * kTileRows + lane]);
+        c128 sum = v[lane];
+        for (int gi = 1; gi < groups; ++gi) sum = sum + v[gi * cnt + lane];
+        out[static_cast<int64_t>(q * kPartZ + lane) * nb] = sum;
     }
     if (threadIdx.x == 0) {
         out[static_cast<int64_t>(kPartScalars) * nb] = mk(al, 0.0);
