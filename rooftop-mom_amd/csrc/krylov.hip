@@ -576,11 +576,15 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__
 
 // The update pass: q_j = (p - V s) / beta into column j of the basis, p = (w - V z - z_j q_j) / (beta nu). Runs for
 // the systems whose finalize ran in iteration `it`, the ones that stopped in it included (q_j enters their solution).
-template <typename VT>
+// S lanes share a row (S = 1 for batches, whose many rows fill the chip; S = 4 for a few systems): lane group s sums
+// the columns k = s mod S, so that each lane's chain of column loads is S times shorter, and the S partial sums meet
+// by lane shuffles in a fixed order.
+template <typename VT, int S>
 __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, int cnt, c128 *__restrict__ p,
                                                           const c128 *__restrict__ w, int64_t n, int64_t chunk,
                                                           const c128 *__restrict__ scal, int it,
                                                           const LoopState *__restrict__ st) {
+    static_assert(S == 1 || S == 2 || S == 4, "one, two or four lanes per row");
     const int r = blockIdx.y;
     if (st[r].tag != it) return;
     const int R = gridDim.y;
@@ -596,24 +600,68 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, in
     VT *vj = V + (static_cast<int64_t>(cnt) * R + r) * n;
     c128 *pr = p + r * n;
     const c128 *wr = w + r * n;
+    if constexpr (S == 1) {
+        const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+        for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
+            c128 as = mk(0.0, 0.0), az = mk(0.0, 0.0);
+            for (int k0 = 0; k0 < cnt; k0 += 8) {   // eight columns' loads in flight at once
+                VT vv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) vv[u] = k0 + u < cnt ? basis_col(V, k0 + u, r, R, n)[i] : VT{0, 0};
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (k0 + u < cnt) {
+                        const c128 v = widen(vv[u]);
+                        as = as + hs[k0 + u] * v;
+                        az = az + hz[k0 + u] * v;
+                    }
+            }
+            const c128 qj = ib * (pr[i] - as);
+            vj[i] = narrow<VT>(qj);
+            pr[i] = isub * (wr[i] - az - zj * qj);
+        }
+        return;
+    }
+    constexpr int kRowsPerWave = 64 / S;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int sub = lane / kRowsPerWave;                                  // this lane's column group
+    const int row = wave * kRowsPerWave + (lane & (kRowsPerWave - 1));   // row within the block's step
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
-    for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
+    for (int64_t i0 = base; i0 < end; i0 += kThreads / S) {
+        const int64_t i = i0 + row;
+        const bool in = i < end;
         c128 as = mk(0.0, 0.0), az = mk(0.0, 0.0);
-        for (int k0 = 0; k0 < cnt; k0 += 8) {   // eight columns' loads in flight at once
+        for (int k0 = sub; k0 < cnt; k0 += 8 * S) {   // eight columns' loads in flight at once
             VT vv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) vv[u] = k0 + u < cnt ? basis_col(V, k0 + u, r, R, n)[i] : VT{0, 0};
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u * S;
+                vv[u] = in && k < cnt ? basis_col(V, k, r, R, n)[i] : VT{0, 0};
+            }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (k0 + u < cnt) {
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u * S;
+                if (k < cnt) {
                     const c128 v = widen(vv[u]);
-                    as = as + hs[k0 + u] * v;
-                    az = az + hz[k0 + u] * v;
+                    as = as + hs[k] * v;
+                    az = az + hz[k] * v;
                 }
+            }
         }
-        const c128 qj = ib * (pr[i] - as);
-        vj[i] = narrow<VT>(qj);
-        pr[i] = isub * (wr[i] - az - zj * qj);
+        if constexpr (S > 1) {   // the column groups' sums, lane l gathering l ^ 32 (and l ^ 16)
+#pragma unroll
+            for (int o = 32; o >= kRowsPerWave; o >>= 1) {
+                as.x += __shfl_xor(as.x, o, 64);
+                as.y += __shfl_xor(as.y, o, 64);
+                az.x += __shfl_xor(az.x, o, 64);
+                az.y += __shfl_xor(az.y, o, 64);
+            }
+        }
+        if (in && sub == 0) {
+            const c128 qj = ib * (pr[i] - as);
+            vj[i] = narrow<VT>(qj);
+            pr[i] = isub * (wr[i] - az - zj * qj);
+        }
     }
 }
 
@@ -992,8 +1040,13 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                                    ws.state, ws.history, ws.cap);
                 RT_LAUNCH_CHECK();
                 KernelTimer t2(ctx, RT_TIMED_GS_UPDATE, j * col + 3 * wb + col);
-                launch_timed(&t2, k_dcgs_update<VT>, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, j, pv, w, n,
-                             ws.chunk, ws.scal, it, st);
+                // A few systems leave the chip short of rows: four lanes per row then.
+                if (static_cast<int64_t>(ws.blocks) * R < 2048)
+                    launch_timed(&t2, k_dcgs_update<VT, 4>, dim3(ws.gs_blocks, R), dim3(kThreads), 0, ws.s, V, j, pv,
+                                 w, n, ws.gs_chunk, ws.scal, it, st);
+                else
+                    launch_timed(&t2, k_dcgs_update<VT, 1>, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, j, pv, w,
+                                 n, ws.chunk, ws.scal, it, st);
                 t2.done();
             }
             state = ws.read_state();
