@@ -133,13 +133,27 @@ __global__ void __launch_bounds__(kTX * kTY) sai_apply_tiled(const c128 *__restr
                                                               const int *__restrict__ scls,
                                                               const cplx<TC> *__restrict__ scoef, int rows_x,
                                                               int cols_x, int rows_y, int cols_y, int64_t nx, int64_t n,
-                                                              int tiles_x, const int *__restrict__ skip,
-                                                              int skip_stride) {
+                                                              int tiles_x, int table_count,
+                                                              const int *__restrict__ skip, int skip_stride) {
     const int r = blockIdx.y;
     if (skip && skip[static_cast<int64_t>(r) * skip_stride]) return;
     constexpr int LX = kTX + 2, LY = kTY + 2;
     __shared__ cplx<TC> tile[2][LY][LX];
+    // With table_count > 0 the class coefficients [table_count][18] are staged in LDS too (dynamic), so that the
+    // neighbour values, the rooftops' stencil classes and the coefficients are all loaded at once: one memory latency
+    // before the stencil instead of three in a row (values, then classes, then the coefficients they select).
+    extern __shared__ unsigned char coef_raw[];
+    cplx<TC> *coef_lds = reinterpret_cast<cplx<TC> *>(coef_raw);
     const int X0 = (blockIdx.x % tiles_x) * kTX, Y0 = (blockIdx.x / tiles_x) * kTY;
+    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int fx = X0 + tx, fy = Y0 + ty;
+    int st_of[2];
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+        const int rows_i = di ? rows_y : rows_x, cols_i = di ? cols_y : cols_x;
+        st_of[di] = fx < rows_i && fy < cols_i ? scls[(di ? nx : 0) + fx + static_cast<int64_t>(rows_i) * fy] : -1;
+    }
+    for (int e = threadIdx.x; e < table_count * kSlots; e += kTX * kTY) coef_lds[e] = scoef[e];
     const c128 *x = in + static_cast<int64_t>(r) * n;
     for (int e = threadIdx.x; e < 2 * LY * LX; e += kTX * kTY) {
         const int d = e / (LY * LX), rem = e - d * (LY * LX), ly = rem / LX, lx = rem - ly * LX;
@@ -151,17 +165,15 @@ __global__ void __launch_bounds__(kTX * kTY) sai_apply_tiled(const c128 *__restr
         tile[d][ly][lx] = v;
     }
     __syncthreads();
-    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-    const int fx = X0 + tx, fy = Y0 + ty;
 #pragma unroll
     for (int di = 0; di < 2; ++di) {
         const int rows_i = di ? rows_y : rows_x, cols_i = di ? cols_y : cols_x;
         if (fx >= rows_i || fy >= cols_i) continue;
         const int64_t i = (di ? nx : 0) + fx + static_cast<int64_t>(rows_i) * fy;
-        const int st = scls[i];
+        const int st = st_of[di];
         c128 res = mk(0.0, 0.0);
         if (st >= 0) {
-            const cplx<TC> *cf = scoef + static_cast<int64_t>(st) * kSlots;
+            const cplx<TC> *cf = (table_count > 0 ? coef_lds : scoef) + static_cast<int64_t>(st) * kSlots;
             cplx<TC> acc = mk<TC>(0, 0);
 #pragma unroll
             for (int d = 0; d < 2; ++d)
@@ -473,13 +485,17 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     const int cx = d.rows_y > 0 ? d.rows_y : d.rows_x + 1, cy = d.cols_x > 0 ? d.cols_x : d.cols_y + 1;
     const int tiles_x = (cx + kTX - 1) / kTX, tiles_y = (cy + kTY - 1) / kTY;
     const dim3 grid(static_cast<unsigned>(tiles_x) * tiles_y, nrhs);
+    // The coefficient table goes to LDS when it is small (a plate has a few dozen stencil classes).
+    const size_t table = static_cast<size_t>(pc.stencil_count) * kSlots * es;
+    const int staged = table <= 16 * 1024 ? pc.stencil_count : 0;
+    const uint32_t lds = staged ? static_cast<uint32_t>(table) : 0;
     if (prec == RT_C128)
-        launch_timed(&timer, sai_apply_tiled<double>, grid, dim3(kTX * kTY), 0, s, in, out, pc.scls.ptr,
-                     pc.scoef64.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, skip,
+        launch_timed(&timer, sai_apply_tiled<double>, grid, dim3(kTX * kTY), lds, s, in, out, pc.scls.ptr,
+                     pc.scoef64.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, staged, skip,
                      skip_stride);
     else
-        launch_timed(&timer, sai_apply_tiled<float>, grid, dim3(kTX * kTY), 0, s, in, out, pc.scls.ptr,
-                     pc.scoef32.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, skip,
+        launch_timed(&timer, sai_apply_tiled<float>, grid, dim3(kTX * kTY), lds, s, in, out, pc.scls.ptr,
+                     pc.scoef32.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, staged, skip,
                      skip_stride);
     timer.done();
 }
