@@ -124,12 +124,15 @@ __global__ void sai_fit(GenView v, const int *__restrict__ dir, const unsigned l
 // neighbours' classes, so the rooftops are grouped into stencil classes whose 18 symmetrized coefficients
 // (M_ij + M_ji)/2 are tabulated once (stencil_table): the apply reads its stencil class, then the 18 coefficients and
 // the 18 neighbours' values, which do not depend on each other.
-// One block per tile of kTX x kTY face positions (both components) and right-hand side: the tile's values with a
-// one-cell halo are staged in LDS once (coalesced along x), and the 18 neighbours of every rooftop are read from
-// there, summed in slot order.
-constexpr int kTX = 32, kTY = 8;
-template <typename TC>
-__global__ void __launch_bounds__(kTX * kTY) sai_apply_tiled(const c128 *__restrict__ in, c128 *__restrict__ out,
+// One block per tile of TX x kTY face positions and right-hand side: the tile's values with a one-cell halo are
+// staged in LDS once (coalesced along x), and the 18 neighbours of every rooftop are read from there, summed in slot
+// order. SPLIT: a thread per rooftop (the first two waves the x rooftops, the last two the y rooftops of a 16 x 8
+// tile), twice the threads of a small problem in flight; otherwise a thread per face position of a 32 x 8 tile, both
+// of its rooftops (less halo per rooftop, for batches that fill the chip anyway).
+constexpr int kTY = 8;
+constexpr int kApplyThreads = 256;
+template <typename TC, bool SPLIT>
+__global__ void __launch_bounds__(kApplyThreads) sai_apply_tiled(const c128 *__restrict__ in, c128 *__restrict__ out,
                                                               const int *__restrict__ scls,
                                                               const cplx<TC> *__restrict__ scoef, int rows_x,
                                                               int cols_x, int rows_y, int cols_y, int64_t nx, int64_t n,
@@ -137,6 +140,7 @@ __global__ void __launch_bounds__(kTX * kTY) sai_apply_tiled(const c128 *__restr
                                                               const int *__restrict__ skip, int skip_stride) {
     const int r = blockIdx.y;
     if (skip && skip[static_cast<int64_t>(r) * skip_stride]) return;
+    constexpr int kTX = SPLIT ? kApplyThreads / (2 * kTY) : kApplyThreads / kTY;
     constexpr int LX = kTX + 2, LY = kTY + 2;
     __shared__ cplx<TC> tile[2][LY][LX];
     // With table_count > 0 the class coefficients [table_count][18] are staged in LDS too (dynamic), so that the
@@ -145,17 +149,25 @@ __global__ void __launch_bounds__(kTX * kTY) sai_apply_tiled(const c128 *__restr
     extern __shared__ unsigned char coef_raw[];
     cplx<TC> *coef_lds = reinterpret_cast<cplx<TC> *>(coef_raw);
     const int X0 = (blockIdx.x % tiles_x) * kTX, Y0 = (blockIdx.x / tiles_x) * kTY;
-    const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+    const int d0 = SPLIT ? static_cast<int>(threadIdx.x) / (kTX * kTY) : 0;   // the first (or only) component
+    const int f = threadIdx.x - d0 * (kTX * kTY);
+    const int tx = f % kTX, ty = f / kTX;
     const int fx = X0 + tx, fy = Y0 + ty;
-    int st_of[2];
+    constexpr int kComps = SPLIT ? 1 : 2;
+    bool mine[kComps];
+    int64_t idx[kComps];
+    int stc[kComps];
 #pragma unroll
-    for (int di = 0; di < 2; ++di) {
+    for (int c = 0; c < kComps; ++c) {
+        const int di = d0 + c;
         const int rows_i = di ? rows_y : rows_x, cols_i = di ? cols_y : cols_x;
-        st_of[di] = fx < rows_i && fy < cols_i ? scls[(di ? nx : 0) + fx + static_cast<int64_t>(rows_i) * fy] : -1;
+        mine[c] = fx < rows_i && fy < cols_i;
+        idx[c] = (di ? nx : 0) + fx + static_cast<int64_t>(rows_i) * fy;
+        stc[c] = mine[c] ? scls[idx[c]] : -1;
     }
-    for (int e = threadIdx.x; e < table_count * kSlots; e += kTX * kTY) coef_lds[e] = scoef[e];
+    for (int e = threadIdx.x; e < table_count * kSlots; e += kApplyThreads) coef_lds[e] = scoef[e];
     const c128 *x = in + static_cast<int64_t>(r) * n;
-    for (int e = threadIdx.x; e < 2 * LY * LX; e += kTX * kTY) {
+    for (int e = threadIdx.x; e < 2 * LY * LX; e += kApplyThreads) {
         const int d = e / (LY * LX), rem = e - d * (LY * LX), ly = rem / LX, lx = rem - ly * LX;
         const int gx = X0 + lx - 1, gy = Y0 + ly - 1;
         const int rows = d ? rows_y : rows_x, cols = d ? cols_y : cols_x;
@@ -166,11 +178,10 @@ __global__ void __launch_bounds__(kTX * kTY) sai_apply_tiled(const c128 *__restr
     }
     __syncthreads();
 #pragma unroll
-    for (int di = 0; di < 2; ++di) {
-        const int rows_i = di ? rows_y : rows_x, cols_i = di ? cols_y : cols_x;
-        if (fx >= rows_i || fy >= cols_i) continue;
-        const int64_t i = (di ? nx : 0) + fx + static_cast<int64_t>(rows_i) * fy;
-        const int st = st_of[di];
+    for (int c = 0; c < kComps; ++c) {
+        if (!mine[c]) continue;
+        const int64_t i = idx[c];
+        const int st = stc[c];
         c128 res = mk(0.0, 0.0);
         if (st >= 0) {
             const cplx<TC> *cf = (table_count > 0 ? coef_lds : scoef) + static_cast<int64_t>(st) * kSlots;
@@ -483,20 +494,26 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     if (pc.class_count == 0) fail(RT_EINVAL, "rooftop:preconditioner", "the grid has no rooftop to precondition");
     // Tiles cover the face positions 0 .. countX-1 by 0 .. countY-1 (the y rooftops' rows and the x rooftops' columns).
     const int cx = d.rows_y > 0 ? d.rows_y : d.rows_x + 1, cy = d.cols_x > 0 ? d.cols_x : d.cols_y + 1;
-    const int tiles_x = (cx + kTX - 1) / kTX, tiles_y = (cy + kTY - 1) / kTY;
+    // A thread per rooftop on 16 x 8 tiles while the 32 x 8 tiles would leave fewer than 2048 blocks.
+    const bool split = static_cast<int64_t>((cx + 31) / 32) * ((cy + kTY - 1) / kTY) * nrhs < 2048;
+    const int tx_w = split ? kApplyThreads / (2 * kTY) : kApplyThreads / kTY;
+    const int tiles_x = (cx + tx_w - 1) / tx_w, tiles_y = (cy + kTY - 1) / kTY;
     const dim3 grid(static_cast<unsigned>(tiles_x) * tiles_y, nrhs);
     // The coefficient table goes to LDS when it is small (a plate has a few dozen stencil classes).
     const size_t table = static_cast<size_t>(pc.stencil_count) * kSlots * es;
     const int staged = table <= 16 * 1024 ? pc.stencil_count : 0;
     const uint32_t lds = staged ? static_cast<uint32_t>(table) : 0;
-    if (prec == RT_C128)
-        launch_timed(&timer, sai_apply_tiled<double>, grid, dim3(kTX * kTY), lds, s, in, out, pc.scls.ptr,
-                     pc.scoef64.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, staged, skip,
-                     skip_stride);
-    else
-        launch_timed(&timer, sai_apply_tiled<float>, grid, dim3(kTX * kTY), lds, s, in, out, pc.scls.ptr,
-                     pc.scoef32.ptr, d.rows_x, d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, staged, skip,
-                     skip_stride);
+    auto launch = [&](auto kernel, const auto *coef) {
+        launch_timed(&timer, kernel, grid, dim3(kApplyThreads), lds, s, in, out, pc.scls.ptr, coef, d.rows_x,
+                     d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, staged, skip, skip_stride);
+    };
+    if (prec == RT_C128) {
+        if (split) launch(sai_apply_tiled<double, true>, pc.scoef64.ptr);
+        else launch(sai_apply_tiled<double, false>, pc.scoef64.ptr);
+    } else {
+        if (split) launch(sai_apply_tiled<float, true>, pc.scoef32.ptr);
+        else launch(sai_apply_tiled<float, false>, pc.scoef32.ptr);
+    }
     timer.done();
 }
 
