@@ -374,13 +374,14 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
     }
 }
 
-// The dot pass's sums over the blocks: block x of the grid (2 cnt + 3, R) gives s_x (x < cnt) into kH2 + x, z into
-// kH1, then alpha, t and omega into kAlpha, kT, kOmega.
+// The dot pass's sums over the blocks, a wave per slot (no barrier): slot x = 4 blockIdx.x + wave gives s_x (x < cnt)
+// into kH2 + x, z into kH1, then alpha, t and omega into kAlpha, kT, kOmega. Grid (ceil((2 cnt + 3) / 4), R).
 __global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict__ partial, int blocks, int cnt,
                                                           c128 *__restrict__ scal, const LoopState *__restrict__ st) {
     SYSTEM(r);
-    __shared__ double sh[8];
-    const int x = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int x = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (x >= 2 * cnt + 3) return;
     int src, dst;
     if (x < cnt) {
         src = x;
@@ -392,16 +393,24 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict
         src = kPartScalars + x - 2 * cnt;
         dst = kAlpha + x - 2 * cnt;
     }
-    double re = 0, im = 0;
+    double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};   // four loads in flight per lane
     const c128 *row = partial + (static_cast<int64_t>(r) * kLdp2 + src) * blocks;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) {
+    int b = lane;
+    for (; b + 192 < blocks; b += 256)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const c128 v = row[b + 64 * u];
+            re[u] += v.x;
+            im[u] += v.y;
+        }
+    for (; b < blocks; b += 64) {
         const c128 v = row[b];
-        re += v.x;
-        im += v.y;
+        re[0] += v.x;
+        im[0] += v.y;
     }
-    re = block_sum(re, sh);
-    im = block_sum(im, sh);
-    if (threadIdx.x == 0) scal[r * kScal + dst] = mk(re, im);
+    const double sr = wave_sum((re[0] + re[1]) + (re[2] + re[3]));
+    const double si = wave_sum((im[0] + im[1]) + (im[2] + im[3]));
+    if (lane == 0) scal[r * kScal + dst] = mk(sr, si);
 }
 
 // Givens rotation of (a, sub) with sub real: c = a / radius, s = sub / radius; returns the rotated diagonal.
@@ -1033,7 +1042,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.gs_blocks, R), dim3(kThreads),
                              static_cast<uint32_t>(tile_lds), ws.s, V, j, pv, w, n, ws.gs_chunk, ws.partial, st);
                 t1.done();
-                hipLaunchKernelGGL(k_dcgs_reduce, dim3(2 * j + 3, R), dim3(kThreads), 0, ws.s, ws.partial,
+                hipLaunchKernelGGL(k_dcgs_reduce, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s, ws.partial,
                                    ws.gs_blocks, j, ws.scal, st);
                 RT_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_dgmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.scal, Hr, H, g, m, j, it,
