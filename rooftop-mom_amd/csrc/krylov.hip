@@ -374,14 +374,10 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
     }
 }
 
-// The dot pass's sums over the blocks, a wave per slot (no barrier): slot x = 4 blockIdx.x + wave gives s_x (x < cnt)
-// into kH2 + x, z into kH1, then alpha, t and omega into kAlpha, kT, kOmega. Grid (ceil((2 cnt + 3) / 4), R).
-__global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict__ partial, int blocks, int cnt,
-                                                          c128 *__restrict__ scal, const LoopState *__restrict__ st) {
-    SYSTEM(r);
-    const int lane = threadIdx.x & 63;
-    const int x = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-    if (x >= 2 * cnt + 3) return;
+// One slot of the dot pass summed over its blocks by one wave: slot x gives s_x (x < cnt) into kH2 + x, z into kH1,
+// then alpha, t and omega into kAlpha, kT, kOmega.
+__device__ __forceinline__ void dcgs_slot_sum(const c128 *__restrict__ partial, int r, int blocks, int cnt, int x,
+                                              int lane, c128 *__restrict__ scal) {
     int src, dst;
     if (x < cnt) {
         src = x;
@@ -393,12 +389,12 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict
         src = kPartScalars + x - 2 * cnt;
         dst = kAlpha + x - 2 * cnt;
     }
-    double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};   // four loads in flight per lane
+    double re[8] = {0, 0, 0, 0, 0, 0, 0, 0}, im[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // eight loads in flight per lane
     const c128 *row = partial + (static_cast<int64_t>(r) * kLdp2 + src) * blocks;
     int b = lane;
-    for (; b + 192 < blocks; b += 256)
+    for (; b + 448 < blocks; b += 512)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const c128 v = row[b + 64 * u];
             re[u] += v.x;
             im[u] += v.y;
@@ -408,10 +404,27 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_reduce(const c128 *__restrict
         re[0] += v.x;
         im[0] += v.y;
     }
-    const double sr = wave_sum((re[0] + re[1]) + (re[2] + re[3]));
-    const double si = wave_sum((im[0] + im[1]) + (im[2] + im[3]));
+    const double sr = wave_sum(((re[0] + re[1]) + (re[2] + re[3])) + ((re[4] + re[5]) + (re[6] + re[7])));
+    const double si = wave_sum(((im[0] + im[1]) + (im[2] + im[3])) + ((im[4] + im[5]) + (im[6] + im[7])));
     if (lane == 0) scal[r * kScal + dst] = mk(sr, si);
 }
+
+// Lane shuffles of FP64 values as two 32-bit halves: from lane - d (own value below d), and from the uniform lane k.
+__device__ __forceinline__ double shfl_up_f64(double v, int d) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __shfl_up(static_cast<unsigned>(b), d, 64);
+    const unsigned hi = __shfl_up(static_cast<unsigned>(b >> 32), d, 64);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(b), k);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), k);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// a where c holds, else b, component by component (a select of whole structures would go through memory).
+__device__ __forceinline__ c128 pick(bool c, c128 a, c128 b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
 
 // Givens rotation of (a, sub) with sub real: c = a / radius, s = sub / radius; returns the rotated diagonal.
 __device__ __forceinline__ c128 givens(c128 a, double sub, c128 &c, c128 &sv) {
@@ -425,17 +438,32 @@ __device__ __forceinline__ c128 givens(c128 a, double sub, c128 &c, c128 &sv) {
     return cmulc(c, a) + cmulc(sv, mk(sub, 0.0));
 }
 
-// Step j (0-based) of the cycle, iteration `it` overall: beta, z_j and nu; column j - 1 of H in its final form, its
-// rotation formed again and the monitored residual of iteration it - 1 refined; column j with the subdiagonal nu
-// (final one step later), its rotation, the rotated right-hand side g, the monitored residual |g(j+1)| / reference
-// and the stopping test (iterate_.m:252-278); 1 / beta and 1 / (beta nu) for the update pass. H holds the columns as
-// formed (Hraw, for H s) and rotated (Hall, for the triangular solve); gall holds g, the cosines, the sines and g
-// before each rotation. Everything the step reads is loaded at once; the norms are one wave's sums; the rotations of
-// both columns run in one pass with the running entry in registers. Grid (1, R).
-__global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__ scal, c128 *__restrict__ Hraw,
+// Step j (0-based) of the cycle, iteration `it` overall, in two phases. First every block sums slots of the dot pass
+// (a wave per slot: s, z, alpha, t, omega into scal); the last block of a system to finish them (counted on its
+// LoopState, which it returns to zero) then finalizes the step: beta, z_j and nu; column j - 1 of H in its final
+// form, its rotation formed again and the monitored residual of iteration it - 1 refined; column j with the
+// subdiagonal nu (final one step later), its rotation, the rotated right-hand side g, the monitored residual
+// |g(j+1)| / reference and the stopping test (iterate_.m:252-278); 1 / beta and 1 / (beta nu) for the update pass.
+// H holds the columns as formed (Hraw, for H s) and rotated (Hall, for the triangular solve); gall holds g, the
+// cosines, the sines and g before each rotation. Everything the step reads is loaded at once; the norms are one
+// wave's sums; the earlier rotations reach both columns by one wave's scan. Grid (ceil((2 j + 3) / 4), R).
+__global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__restrict__ partial, int blocks,
+                                                              c128 *__restrict__ scal, c128 *__restrict__ Hraw,
                                                               c128 *__restrict__ Hall, c128 *__restrict__ gall, int m,
                                                               int j, int it, LoopState *st, double *history, int cap) {
     SYSTEM(r);
+    {
+        __shared__ int last;
+        const int x = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+        if (x < 2 * j + 3) dcgs_slot_sum(partial, r, blocks, j, x, threadIdx.x & 63, scal);
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) last = atomicAdd(&st[r].arrivals, 1) == static_cast<int>(gridDim.x) - 1;
+        __syncthreads();
+        if (!last) return;
+        __threadfence();
+        if (threadIdx.x == 0) st[r].arrivals = 0;
+    }
     __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp], colf[kLdp + 1], coln[kLdp + 1];
     __shared__ c128 csv[kLdp], snv[kLdp];
     __shared__ double shb[3];   // nu_{j-1}, beta, nu
@@ -530,36 +558,68 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(c128 *__restrict__
         Hr[static_cast<int64_t>(j) * ldh + k] = coln[k];
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (q != 0) return;
     const double nu = shb[2];
-    sc[kH1 + j] = z[j];
-    Hr[static_cast<int64_t>(j) * ldh + j + 1] = mk(nu, 0.0);
-    // The final rotations 0 .. j-2 on both columns, the running entries in registers.
-    c128 cb = coln[0], gj = gprev;
-    if (j >= 1) {
-        c128 ca = colf[0];
-        for (int k = 0; k + 1 < j; ++k) {
-            const c128 c = csv[k], sv = snv[k], a1 = colf[k + 1], b1 = coln[k + 1];
-            colf[k] = cmulc(c, ca) + cmulc(sv, a1);
-            ca = c * a1 - sv * ca;
-            coln[k] = cmulc(c, cb) + cmulc(sv, b1);
-            cb = c * b1 - sv * cb;
+    // The rotations 0 .. j-2 on both columns. With run_0 = a_0, rotation k gives the final entry
+    // conj(c_k) run_k + s_k a_{k+1} and the running entry run_{k+1} = -s_k run_k + c_k a_{k+1} (s_k real): an affine
+    // recurrence, whose prefix compositions (M, B) (run_{k+1} = M a_0 + B) one wave scans, lane k holding step k.
+    const int K = j >= 1 ? j - 1 : 0;
+    c128 ck = mk(0.0, 0.0), af = mk(0.0, 0.0), an = mk(0.0, 0.0), bf = mk(0.0, 0.0), bn = mk(0.0, 0.0);
+    double sk = 0, M = 1;
+    if (lane < K) {
+        ck = csv[lane];
+        sk = snv[lane].x;
+        af = colf[lane + 1];
+        an = coln[lane + 1];
+        M = -sk;
+        bf = ck * af;
+        bn = ck * an;
+    }
+    for (int d = 1; d < K; d <<= 1) {
+        const double Mp = shfl_up_f64(M, d);
+        const c128 fp = mk(shfl_up_f64(bf.x, d), shfl_up_f64(bf.y, d));
+        const c128 np = mk(shfl_up_f64(bn.x, d), shfl_up_f64(bn.y, d));
+        if (lane >= d) {
+            bf = M * fp + bf;
+            bn = M * np + bn;
+            M = M * Mp;
         }
+    }
+    // run_k on lane k from the composition of the steps before it.
+    double Me = shfl_up_f64(M, 1);
+    c128 fe = mk(shfl_up_f64(bf.x, 1), shfl_up_f64(bf.y, 1));
+    c128 ne = mk(shfl_up_f64(bn.x, 1), shfl_up_f64(bn.y, 1));
+    if (lane == 0) {
+        Me = 1;
+        fe = ne = mk(0.0, 0.0);
+    }
+    const c128 runf = Me * colf[0] + fe, runn = Me * coln[0] + ne;
+    const c128 outf = cmulc(ck, runf) + sk * af, outn = cmulc(ck, runn) + sk * an;   // final entries k < K
+    c128 cb = mk(readlane_f64(runn.x, K), readlane_f64(runn.y, K));
+    c128 gj = gprev;   // the values thread 0 loaded
+    c128 colnj1 = mk(0.0, 0.0);
+    if (j >= 1) {
+        const c128 ca = mk(readlane_f64(runf.x, K), readlane_f64(runf.y, K));
         c128 c, sv;
-        colf[j - 1] = givens(ca, nup * beta, c, sv);   // rotation j - 1, formed again on the final column
+        const c128 diag = givens(ca, nup * beta, c, sv);   // rotation j - 1, formed again on the final column
         const c128 b1 = coln[j];
-        coln[j - 1] = cmulc(c, cb) + cmulc(sv, b1);
+        colnj1 = cmulc(c, cb) + cmulc(sv, b1);
         cb = c * b1 - sv * cb;
-        for (int k = 0; k < j; ++k) Hc[static_cast<int64_t>(j - 1) * ldh + k] = colf[k];
-        cs[j - 1] = c;
-        sn[j - 1] = sv;
-        g[j - 1] = cmulc(c, gprev);
-        gj = mk(0.0, 0.0) - sv * gprev;
-        history[static_cast<int64_t>(r) * cap + it - 1] = sqrt(abs2(gj)) / ref;
+        if (lane < j) Hc[static_cast<int64_t>(j - 1) * ldh + lane] = pick(lane < K, outf, diag);
+        if (lane == 0) {
+            cs[j - 1] = c;
+            sn[j - 1] = sv;
+            g[j - 1] = cmulc(c, gprev);
+            gj = mk(0.0, 0.0) - sv * gprev;
+            history[static_cast<int64_t>(r) * cap + it - 1] = sqrt(abs2(gj)) / ref;
+        }
     }
     c128 c, sv;
-    coln[j] = givens(cb, nu, c, sv);
-    for (int k = 0; k <= j; ++k) Hc[static_cast<int64_t>(j) * ldh + k] = coln[k];
+    const c128 diag = givens(cb, nu, c, sv);
+    if (lane <= j) Hc[static_cast<int64_t>(j) * ldh + lane] = pick(lane < K, outn, pick(lane < j, colnj1, diag));
+    if (lane != 0) return;
+    sc[kH1 + j] = z[j];
+    Hr[static_cast<int64_t>(j) * ldh + j + 1] = mk(nu, 0.0);
     cs[j] = c;
     sn[j] = sv;
     gs[j] = gj;
@@ -697,21 +757,35 @@ __global__ void k_cycle_reset(LoopState *st, int R) {
 }
 
 // y = H(0:steps, 0:steps) \ g(0:steps) (upper triangular after the rotations). Grid (1, R); no-op at steps = 0.
-__global__ void k_gmres_solve(const c128 *__restrict__ Hall, const c128 *__restrict__ gall, int m,
-                              const LoopState *st, c128 *__restrict__ scal) {
+__global__ __launch_bounds__(64) void k_gmres_solve(const c128 *__restrict__ Hall, const c128 *__restrict__ gall,
+                                                    int m, const LoopState *st, c128 *__restrict__ scal) {
+    // Column-oriented back substitution, one wave per right-hand side, lane l owning row l (steps <= m <= 62): the
+    // rows a lane reads are staged in LDS first (all loads in flight at once), then each step finishes y_k on lane
+    // k, broadcasts it and subtracts column k from the rows above. Every lane reads only the LDS it wrote itself.
+    __shared__ c128 hs[62 * 64];
     const int r = blockIdx.y;
-    if (threadIdx.x != 0) return;
+    const int lane = threadIdx.x;
     const int steps = st[r].steps;
     const int ldh = m + 1;
     const c128 *H = Hall + static_cast<int64_t>(r) * ldh * m;
     const c128 *g = gall + static_cast<int64_t>(r) * 4 * ldh;
-    c128 *y = scal + r * kScal + kY;
+    const bool row = lane < steps;
+    for (int k = 0; k < steps; ++k)
+        if (row && k >= lane) hs[k * 64 + lane] = H[lane + static_cast<int64_t>(ldh) * k];
+    c128 acc = row ? g[lane] : mk(0.0, 0.0);
+    c128 mine = mk(0.0, 0.0);
     for (int k = steps - 1; k >= 0; --k) {
-        c128 acc = g[k];
-        for (int l = k + 1; l < steps; ++l) acc = acc - H[k + static_cast<int64_t>(ldh) * l] * y[l];
-        const c128 d = H[k + static_cast<int64_t>(ldh) * k];
-        y[k] = (1.0 / abs2(d)) * cmulc(d, acc);
+        c128 yk = mk(0.0, 0.0);
+        if (lane == k) {
+            const c128 d = hs[k * 64 + k];
+            yk = (1.0 / abs2(d)) * cmulc(d, acc);
+            mine = yk;
+        }
+        yk.x = __shfl(yk.x, k, 64);
+        yk.y = __shfl(yk.y, k, 64);
+        if (lane < k) acc = acc - hs[k * 64 + lane] * yk;
     }
+    if (row) scal[r * kScal + kY + lane] = mine;
 }
 
 // x_r += V_r[:, 0:steps] y_r. Grid (blocks, R); no-op at steps = 0.
@@ -1042,11 +1116,9 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.gs_blocks, R), dim3(kThreads),
                              static_cast<uint32_t>(tile_lds), ws.s, V, j, pv, w, n, ws.gs_chunk, ws.partial, st);
                 t1.done();
-                hipLaunchKernelGGL(k_dcgs_reduce, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s, ws.partial,
-                                   ws.gs_blocks, j, ws.scal, st);
-                RT_LAUNCH_CHECK();
-                hipLaunchKernelGGL(k_dgmres_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.scal, Hr, H, g, m, j, it,
-                                   ws.state, ws.history, ws.cap);
+                hipLaunchKernelGGL(k_dgmres_finalize, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s,
+                                   ws.partial, ws.gs_blocks, ws.scal, Hr, H, g, m, j, it, ws.state, ws.history,
+                                   ws.cap);
                 RT_LAUNCH_CHECK();
                 KernelTimer t2(ctx, RT_TIMED_GS_UPDATE, j * col + 3 * wb + col);
                 // A few systems leave the chip short of rows: four lanes per row then.

@@ -40,7 +40,7 @@ struct LoopState {
     int steps;        // GMRES: steps of the current cycle completed
     int best;         // COCG: iteration of the best monitored norm
     int tag;          // GMRES: the last iteration whose finalize ran for this system
-    int pad;
+    int arrivals;     // GMRES: blocks of the running step's reduction done so far (zero between launches)
 };
 constexpr int kStateInts = sizeof(LoopState) / sizeof(int);
 constexpr int kScal = 512;   // device scalar slots per system
