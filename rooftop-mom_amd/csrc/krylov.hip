@@ -33,6 +33,7 @@ namespace rt {
 constexpr int kThreads = 256;
 constexpr int kLdp = 64;        // partial columns per block
 constexpr int kPoll = 8;        // steps enqueued between two polls of the loop states
+constexpr int kPollMax = 32;    // GMRES: the most steps enqueued between two polls
 constexpr int kStall = 40;      // COCG steps without a new best norm that end a complex64 inner solve
 // Mixed COCG: the first inner solve in complex64, to kCocgInner or a stall, and the later ones in FP64. Complex64
 // COCG loses the complex symmetry of its products on systems whose residual swings high (on the 512 x 512 plate
@@ -162,13 +163,14 @@ __global__ void k_reset_state(LoopState *st, const int *__restrict__ finished, i
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int f = finished[r];
-    st[r] = LoopState{f, 0, f, 0, 0, 0, -1, 0};
+    st[r] = LoopState{f, 0, f, 0, 0, 0, -1, 0, 0.f};
 }
 // Marks the systems whose starting residual is already below their tolerance as converged at iteration 0.
 __global__ void k_initial_check(LoopState *st, const c128 *__restrict__ scal, int num_slot, int den_slot, int R) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R || st[r].done) return;
     const double res = scal[r * kScal + num_slot].x / scal[r * kScal + den_slot].x;
+    st[r].monitored = static_cast<float>(res);
     if (res < scal[r * kScal + kTol].x) st[r].done = st[r].finished = st[r].converged = 1;
 }
 __global__ void k_copy_slot(c128 *scal, int from, int to, int R) {
@@ -278,12 +280,15 @@ constexpr int kLdp2 = 128;                           // partial slots per block 
 constexpr int kPartZ = 64, kPartScalars = 125;       // s at k, z at kPartZ + k, then alpha, t, omega
 constexpr size_t kDotsRedBytes = 2 * 4 * kTileRows * sizeof(c128);
 
-// partial[(r * blocks + b) * kLdp2 + ...] = the inner products above over block b's rows. The live columns of a tile of
-// 64 rows are staged in LDS with p and w (row stride 65 keeps the column-wise reads off shared banks); the columns' sums
-// are split over row groups (below), wave 0 sums alpha, t and omega. Dynamic LDS: cnt * 65 basis elements.
+// partial[(r * blocks + b) * kLdp2 + ...] = the inner products above over block b's rows: `per` tiles of 64 rows,
+// tiles b * per .. b * per + per - 1, or, strided, b, b + blocks, b + 2 blocks ... (with the blocks a multiple of 8,
+// every tile is then read on the XCD that runs block `tile` of a one-tile-per-block pass: the update pass finds the
+// basis rows in its own L2). The live columns of a tile are staged in LDS with p and w (row stride 65 keeps the
+// column-wise reads off shared banks); the columns' sums are split over row groups (below), wave 0 sums alpha, t and
+// omega. Dynamic LDS: cnt * 65 basis elements.
 template <typename VT>
 __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V, int cnt, const c128 *__restrict__ p,
-                                                        const c128 *__restrict__ w, int64_t n, int64_t chunk,
+                                                        const c128 *__restrict__ w, int64_t n, int per, int strided,
                                                         c128 *__restrict__ partial, const LoopState *__restrict__ st) {
     SYSTEM(r);
     const int R = gridDim.y;
@@ -296,7 +301,7 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
     // share one 32-bit row offset.
     const int lane = threadIdx.x & 63, q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const c128 *pr = p + r * n, *wr = w + r * n;
-    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    const int64_t tiles = (n + kTileRows - 1) / kTileRows;
     // The dot phase: thread (k, g) sums column k over the rows i = g mod G, with G = 256 / cnt row groups, so that
     // every thread of the block works however few the columns.
     const int groups = cnt > 0 ? min(kTileRows, kThreads / cnt) : 1;
@@ -304,8 +309,12 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_dots(const VT *__restrict__ V
     double sre = 0, sim = 0, zre = 0, zim = 0;   // column dk, rows dg mod groups
     double al = 0, tre = 0, tim = 0, om = 0;     // wave 0: row `lane` of every tile
     constexpr int kPer = kLdp * kTileRows / kThreads;
-    for (int64_t t0 = base; t0 < end; t0 += kTileRows) {
-        const int rows = static_cast<int>(min(static_cast<int64_t>(kTileRows), end - t0));
+    for (int tk = 0; tk < per; ++tk) {
+        const int64_t tb = strided ? blockIdx.x + static_cast<int64_t>(tk) * gridDim.x
+                                   : static_cast<int64_t>(blockIdx.x) * per + tk;
+        if (tb >= tiles) break;
+        const int64_t t0 = tb * kTileRows;
+        const int rows = static_cast<int>(min(static_cast<int64_t>(kTileRows), n - t0));
         // All loads of the tile are issued before any is stored: the staging costs one memory latency.
         // Thread (q, lane) loads the columns k = q + 4 u (uniform over the wave) at the row `lane`.
         VT vals[kPer];
@@ -628,6 +637,7 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
     g[j] = cmulc(c, gj);
     const double res = sqrt(abs2(gnext)) / ref;
     history[static_cast<int64_t>(r) * cap + it] = res;
+    st[r].monitored = static_cast<float>(res);
     st[r].steps = j + 1;
     st[r].iterations = it;
     st[r].tag = it;
@@ -834,13 +844,29 @@ Workspace::Workspace(Context *c, int64_t n_, int R_, int max_iter) : ctx(c), s(c
     const int64_t per = (n + b - 1) / b;
     chunk = (per + kThreads - 1) / kThreads * kThreads;
     blocks = static_cast<int>((n + chunk - 1) / chunk);
-    // The Gram-Schmidt dot pass runs on 64-row tiles: one tile per block while the blocks stay within 16384, so that
-    // small systems are not left with a few blocks walking many tiles one memory latency at a time.
+    // The Gram-Schmidt passes run on 64-row tiles. With few systems (under 2048 reduction blocks in all) the update
+    // pass takes one tile per block while the blocks stay within 16384, so that small systems are not left with a few
+    // blocks walking many tiles one memory latency at a time; otherwise it runs on the reduction blocks' rows. The dot
+    // pass takes at least three tiles per block: its blocks' partial sums are summed again on the critical path of
+    // every step, and with one system a third of the blocks is faster overall (256 x 256 plate: 680 instead of 2040
+    // blocks, 4 % more solves per second mixed, 7 % in FP64). Beside the one-tile update its tiles are strided over a
+    // multiple of 8 blocks (each tile on the XCD of the update block that reads it next); beside the row-block update
+    // it takes the same rows as the update's blocks (runs of at least four tiles).
     const int64_t tiles = (n + kTileRows - 1) / kTileRows;
+    gs_tile_update = static_cast<int64_t>(blocks) * R < 2048;
     gs_blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(tiles, std::max(16, 16384 / R))));
     gs_chunk = (tiles + gs_blocks - 1) / gs_blocks * kTileRows;
     gs_blocks = static_cast<int>((n + gs_chunk - 1) / gs_chunk);
-    partial = vec(c, kSlotPartial, static_cast<int64_t>(R) * std::max(blocks, gs_blocks) * kLdp2);
+    if (gs_tile_update) {
+        dot_strided = 1;
+        dot_blocks = static_cast<int>(std::min<int64_t>(tiles, ((tiles + 2) / 3 + 7) / 8 * 8));
+        dot_per = static_cast<int>((tiles + dot_blocks - 1) / dot_blocks);
+    } else {
+        dot_strided = 0;
+        dot_per = static_cast<int>(chunk / kTileRows);   // chunk: a multiple of 256 rows
+        dot_blocks = static_cast<int>((tiles + dot_per - 1) / dot_per);
+    }
+    partial = vec(c, kSlotPartial, static_cast<int64_t>(R) * std::max(blocks, dot_blocks) * kLdp2);
     scal = vec(c, kSlotScal, static_cast<int64_t>(R) * kScal);
     state = static_cast<LoopState *>(c->scratch(kSlotState, sizeof(LoopState) * R, "the solver's state"));
     cap = max_iter + 2;
@@ -939,6 +965,45 @@ struct InnerResult {
     std::vector<char> converged;
     std::vector<int> iterations;
     std::vector<std::vector<double>> norms;   // monitored norms from iteration 0
+};
+
+// GMRES: the steps to enqueue before the next poll of the loop states. A poll drains the stream (about as long as
+// a step), and steps enqueued past a system's convergence run as no-ops (as long again): the window is the
+// iterations the slowest running system still needs at the rate its monitored residual fell since the previous
+// poll (log-linear), between 1 and kPollMax; kPoll for a system without such a rate, and the previous estimate for
+// one that made no step since (a poll at the end of a cycle).
+struct PollPlan {
+    std::vector<int> it0, est;
+    std::vector<float> res0;
+    int next(const std::vector<LoopState> &st, const std::vector<double> &tol) {
+        const size_t R = st.size();
+        if (it0.size() != R) {
+            it0.assign(R, -1);
+            est.assign(R, kPoll);
+            res0.assign(R, 0.f);
+        }
+        int need = 1;
+        for (size_t r = 0; r < R; ++r) {
+            const LoopState &q = st[r];
+            if (q.done) continue;
+            const double res = q.monitored, prev = res0[r];
+            if (q.iterations == it0[r]) {
+                need = std::max(need, est[r]);
+                continue;
+            }
+            int k = kPoll;
+            if (it0[r] >= 0 && q.iterations > it0[r] && res > 0 && res < prev && tol[r] > 0) {
+                const double slope = std::log(res / prev) / (q.iterations - it0[r]);
+                const double left = std::ceil(std::log(tol[r] / res) / slope);
+                k = static_cast<int>(std::min<double>(kPollMax, std::max(1.0, left)));
+            }
+            it0[r] = q.iterations;
+            res0[r] = q.monitored;
+            est[r] = k;
+            need = std::max(need, k);
+        }
+        return std::min(kPollMax, need);
+    }
 };
 
 static bool all_done(const std::vector<LoopState> &st) {
@@ -1093,6 +1158,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
             if (!q.finished) return true;
         return false;
     };
+    PollPlan plan;
     int it = 0;
     while (it < max_iter && any_active()) {
         hipLaunchKernelGGL(k_cycle_reset, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, R);
@@ -1101,7 +1167,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
         RT_LAUNCH_CHECK();
         int step = 0;
         while (step < m && it < max_iter) {
-            const int window = std::min({kPoll, m - step, max_iter - it});
+            const int window = std::min({plan.next(state, tol), m - step, max_iter - it});
             for (int k = 0; k < window; ++k) {
                 const int j = step;
                 ++step;
@@ -1113,16 +1179,17 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 KernelTimer t1(ctx, RT_TIMED_GS_DOTS, j * col + 2 * wb);
                 const size_t tile_lds = std::max(sizeof(VT) * j * kTileStride, kDotsRedBytes);
                 if (tile_lds > 32 * 1024) raise_lds(reinterpret_cast<const void *>(&k_dcgs_dots<VT>), tile_lds);
-                launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.gs_blocks, R), dim3(kThreads),
-                             static_cast<uint32_t>(tile_lds), ws.s, V, j, pv, w, n, ws.gs_chunk, ws.partial, st);
+                launch_timed(&t1, k_dcgs_dots<VT>, dim3(ws.dot_blocks, R), dim3(kThreads),
+                             static_cast<uint32_t>(tile_lds), ws.s, V, j, pv, w, n, ws.dot_per, ws.dot_strided,
+                             ws.partial, st);
                 t1.done();
                 hipLaunchKernelGGL(k_dgmres_finalize, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s,
-                                   ws.partial, ws.gs_blocks, ws.scal, Hr, H, g, m, j, it, ws.state, ws.history,
+                                   ws.partial, ws.dot_blocks, ws.scal, Hr, H, g, m, j, it, ws.state, ws.history,
                                    ws.cap);
                 RT_LAUNCH_CHECK();
                 KernelTimer t2(ctx, RT_TIMED_GS_UPDATE, j * col + 3 * wb + col);
                 // A few systems leave the chip short of rows: four lanes per row then.
-                if (static_cast<int64_t>(ws.blocks) * R < 2048)
+                if (ws.gs_tile_update)
                     launch_timed(&t2, k_dcgs_update<VT, 4>, dim3(ws.gs_blocks, R), dim3(kThreads), 0, ws.s, V, j, pv,
                                  w, n, ws.gs_chunk, ws.scal, it, st);
                 else

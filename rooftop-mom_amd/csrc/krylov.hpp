@@ -41,6 +41,7 @@ struct LoopState {
     int best;         // COCG: iteration of the best monitored norm
     int tag;          // GMRES: the last iteration whose finalize ran for this system
     int arrivals;     // GMRES: blocks of the running step's reduction done so far (zero between launches)
+    float monitored;  // GMRES: monitored residual of the last finalized iteration (the host's poll schedule)
 };
 constexpr int kStateInts = sizeof(LoopState) / sizeof(int);
 constexpr int kScal = 512;   // device scalar slots per system
@@ -54,8 +55,12 @@ struct Workspace {
     int R;
     int blocks;          // row blocks per system of every reduction
     int64_t chunk;       // rows per block (a multiple of 256)
-    int gs_blocks;       // row blocks per system of the GMRES Gram-Schmidt dot pass
+    bool gs_tile_update; // GMRES: the Gram-Schmidt update pass runs on gs_blocks x gs_chunk (else blocks x chunk)
+    int gs_blocks;       // its row blocks per system
     int64_t gs_chunk;    // their rows per block (a multiple of 64)
+    int dot_blocks;      // row blocks per system of the GMRES Gram-Schmidt dot pass
+    int dot_per;         // their 64-row tiles per block
+    int dot_strided;     // 1: block b takes the tiles b, b + dot_blocks, ...; 0: a run of dot_per tiles
     c128 *partial;       // [R][blocks][128]
     c128 *scal;          // [R][kScal]
     LoopState *state;    // [R]
