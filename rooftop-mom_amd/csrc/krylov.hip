@@ -398,23 +398,32 @@ __device__ __forceinline__ void dcgs_slot_sum(const c128 *__restrict__ partial, 
         src = kPartScalars + x - 2 * cnt;
         dst = kAlpha + x - 2 * cnt;
     }
-    double re[8] = {0, 0, 0, 0, 0, 0, 0, 0}, im[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // eight loads in flight per lane
+    // Up to 768 blocks in one round of loads (twelve per lane), the rest after. The loads past the last block read
+    // the last one again and are dropped by a select: a load under a branch would be waited for inside it.
+    constexpr int kRound = 12;
+    double re[kRound], im[kRound];
     const c128 *row = partial + (static_cast<int64_t>(r) * kLdp2 + src) * blocks;
-    int b = lane;
-    for (; b + 448 < blocks; b += 512)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const c128 v = row[b + 64 * u];
-            re[u] += v.x;
-            im[u] += v.y;
-        }
-    for (; b < blocks; b += 64) {
+    for (int u = 0; u < kRound; ++u) {
+        const int b = lane + 64 * u;
+        const c128 v = row[min(b, blocks - 1)];
+        re[u] = b < blocks ? v.x : 0.0;
+        im[u] = b < blocks ? v.y : 0.0;
+    }
+    for (int b = lane + 64 * kRound; b < blocks; b += 64) {
         const c128 v = row[b];
         re[0] += v.x;
         im[0] += v.y;
     }
-    const double sr = wave_sum(((re[0] + re[1]) + (re[2] + re[3])) + ((re[4] + re[5]) + (re[6] + re[7])));
-    const double si = wave_sum(((im[0] + im[1]) + (im[2] + im[3])) + ((im[4] + im[5]) + (im[6] + im[7])));
+#pragma unroll
+    for (int w = kRound / 2; w >= 3; w /= 2)
+#pragma unroll
+        for (int u = 0; u < w; ++u) {
+            re[u] += re[u + w];
+            im[u] += im[u + w];
+        }
+    const double sr = wave_sum((re[0] + re[1]) + re[2]);
+    const double si = wave_sum((im[0] + im[1]) + im[2]);
     if (lane == 0) scal[r * kScal + dst] = mk(sr, si);
 }
 
@@ -441,8 +450,9 @@ __device__ __forceinline__ c128 givens(c128 a, double sub, c128 &c, c128 &sv) {
     c = mk(1.0, 0.0);
     sv = mk(0.0, 0.0);
     if (radius > 0) {
-        c = (1.0 / radius) * a;
-        sv = mk(sub / radius, 0.0);
+        const double inv = 1.0 / radius;
+        c = inv * a;
+        sv = mk(sub * inv, 0.0);
     }
     return cmulc(c, a) + cmulc(sv, mk(sub, 0.0));
 }
@@ -669,24 +679,29 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, in
     const int R = gridDim.y;
     __shared__ c128 hs[kLdp], hz[kLdp];
     const c128 *sc = scal + r * kScal;
-    for (int k = threadIdx.x; k < cnt; k += kThreads) {
-        hs[k] = sc[kH2 + k];
-        hz[k] = sc[kH1 + k];
-    }
-    __syncthreads();
-    const double ib = sc[kInvBeta].x, isub = sc[kInvSub].x;
-    const c128 zj = sc[kH1 + cnt];
     VT *vj = V + (static_cast<int64_t>(cnt) * R + r) * n;
     c128 *pr = p + r * n;
     const c128 *wr = w + r * n;
+    auto stage = [&]() {   // s and z of this step into LDS
+        for (int k = threadIdx.x; k < cnt; k += kThreads) {
+            hs[k] = sc[kH2 + k];
+            hz[k] = sc[kH1 + k];
+        }
+        __syncthreads();
+    };
+    const double ib = sc[kInvBeta].x, isub = sc[kInvSub].x;
+    const c128 zj = sc[kH1 + cnt];
     if constexpr (S == 1) {
+        stage();
         const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
         for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
             c128 as = mk(0.0, 0.0), az = mk(0.0, 0.0);
             for (int k0 = 0; k0 < cnt; k0 += 8) {   // eight columns' loads in flight at once
+                // Loads past the last column read it again and are not summed: a load under a branch would be
+                // waited for inside it.
                 VT vv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) vv[u] = k0 + u < cnt ? basis_col(V, k0 + u, r, R, n)[i] : VT{0, 0};
+                for (int u = 0; u < 8; ++u) vv[u] = basis_col(V, min(k0 + u, cnt - 1), r, R, n)[i];
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     if (k0 + u < cnt) {
@@ -706,17 +721,21 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, in
     const int sub = lane / kRowsPerWave;                                  // this lane's column group
     const int row = wave * kRowsPerWave + (lane & (kRowsPerWave - 1));   // row within the block's step
     const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    const int last = max(cnt - 1, 0);   // column `cnt` is allocated: a valid address when cnt = 0
     for (int64_t i0 = base; i0 < end; i0 += kThreads / S) {
         const int64_t i = i0 + row;
         const bool in = i < end;
-        c128 as = mk(0.0, 0.0), az = mk(0.0, 0.0);
-        for (int k0 = sub; k0 < cnt; k0 += 8 * S) {   // eight columns' loads in flight at once
-            VT vv[8];
+        // Loads past the last column or row read the last one again and are not summed or stored (a load under a
+        // branch would be waited for inside it). The row's p and w and its first eight columns are requested before
+        // s and z are staged, so that the step costs one memory latency.
+        const int64_t ic = in ? i : end - 1;
+        const c128 pv = pr[ic], wv = wr[ic];
+        VT vv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = k0 + u * S;
-                vv[u] = in && k < cnt ? basis_col(V, k, r, R, n)[i] : VT{0, 0};
-            }
+        for (int u = 0; u < 8; ++u) vv[u] = basis_col(V, min(sub + u * S, last), r, R, n)[ic];
+        if (i0 == base) stage();
+        c128 as = mk(0.0, 0.0), az = mk(0.0, 0.0);
+        for (int k0 = sub;; ) {   // eight columns' loads in flight at once
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int k = k0 + u * S;
@@ -726,6 +745,10 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, in
                     az = az + hz[k] * v;
                 }
             }
+            k0 += 8 * S;
+            if (k0 >= cnt) break;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vv[u] = basis_col(V, min(k0 + u * S, last), r, R, n)[ic];
         }
         if constexpr (S > 1) {   // the column groups' sums, lane l gathering l ^ 32 (and l ^ 16)
 #pragma unroll
@@ -737,9 +760,9 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, in
             }
         }
         if (in && sub == 0) {
-            const c128 qj = ib * (pr[i] - as);
+            const c128 qj = ib * (pv - as);
             vj[i] = narrow<VT>(qj);
-            pr[i] = isub * (wr[i] - az - zj * qj);
+            pr[i] = isub * (wv - az - zj * qj);
         }
     }
 }
