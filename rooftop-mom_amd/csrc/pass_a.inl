@@ -23,20 +23,34 @@ __global__ void __launch_bounds__(MAXT) pass_a_k(const c128 *__restrict__ in, cp
     const int rows = comp ? d.rows_y : d.rows_x;
     const int ncols = min(G, cols - j0);
     const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
-    {   // Round 0, threads along x: reads the rooftop rows of column c.
+    {   // Round 0, threads along x: reads the rooftop rows of column c. Every thread loads from a valid row of a valid
+        // column (the last one again past the end) and zeroes what lies outside: a load under a branch would be
+        // waited for inside it, one at a time.
         const int c = threadIdx.x / P::T, t = threadIdx.x - c * P::T;
         const bool live = c < ncols;
-        const int64_t col = static_cast<int64_t>(j0 + c) * rows;
+        const int64_t col = static_cast<int64_t>(j0 + (live ? c : 0)) * rows;
         const c128 *src = in + off + col;
-        const uint8_t *exc = exists ? exists + (comp ? d.nx_count : 0) + col : nullptr;
         cplx<TC> *lds = buf + c * CS;
-        cround<TC, P, 0, false, EMB, (P::K > 1)>(
-            t, tw,
-            [&](int pos, int) {
-                if (!live || pos >= rows || (exc && !exc[pos])) return mk<TC>(0, 0);
-                return cvt<TC>(src[pos]);
-            },
-            [&](int, int pp, cplx<TC> v) { lds[pp] = v; });
+        auto store = [&](int, int pp, cplx<TC> v) { lds[pp] = v; };
+        if (exists) {   // a masked plate: the mask bytes are loaded beside the values
+            const uint8_t *exc = exists + (comp ? d.nx_count : 0) + col;
+            cround<TC, P, 0, false, EMB, (P::K > 1)>(
+                t, tw,
+                [&](int pos, int) {
+                    const int pc = min(pos, rows - 1);
+                    const cplx<TC> v = cvt<TC>(src[pc]);
+                    return live && pos < rows && exc[pc] ? v : mk<TC>(0, 0);
+                },
+                store);
+        } else {
+            cround<TC, P, 0, false, EMB, (P::K > 1)>(
+                t, tw,
+                [&](int pos, int) {
+                    const cplx<TC> v = cvt<TC>(src[min(pos, rows - 1)]);
+                    return live && pos < rows ? v : mk<TC>(0, 0);
+                },
+                store);
+        }
     }
     __syncthreads();
     // Rounds 1 .. K-1, threads along the columns; the last one writes the work array.

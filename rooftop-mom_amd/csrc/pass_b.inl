@@ -29,8 +29,10 @@ __device__ __forceinline__ void multiply_round(int t, cplx<TC> *col0, cplx<TC> *
         for (int q = 0; q < R; ++q) {
             if constexpr (DIRECT) {
                 const bool in_half = !EMB || R == 1 || q < R / 2;   // an embedding's upper half is zero
-                X[q] = in_half && base + q < n0 ? w0[base + q] : mk<TC>(0, 0);
-                Y[q] = in_half && base + q < n1 ? w1[base + q] : mk<TC>(0, 0);
+                // Loads from a valid column (clamped), then a select: a load under a branch is waited for inside it.
+                const cplx<TC> x = w0[min(base + q, max(n0 - 1, 0))], y = w1[min(base + q, max(n1 - 1, 0))];
+                X[q] = in_half && base + q < n0 ? x : mk<TC>(0, 0);
+                Y[q] = in_half && base + q < n1 ? y : mk<TC>(0, 0);
             } else {
                 X[q] = col0[pb + P::pad(q)];
                 Y[q] = col1[pb + P::pad(q)];
@@ -107,10 +109,10 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
     } else {
         // Forward round 0 from the work array (the upper half of the embedding is zero).
         cround<TC, P, 0, false, EMB, true>(
-            t, tw, [&](int pos, int) { return pos < n0 ? w0[pos] : mk<TC>(0, 0); },
+            t, tw, [&](int pos, int) { return clamped_load(w0, pos, n0); },
             [&](int, int pp, cplx<TC> v) { col0[pp] = v; });
         cround<TC, P, 0, false, EMB, true>(
-            t, tw, [&](int pos, int) { return pos < n1 ? w1[pos] : mk<TC>(0, 0); },
+            t, tw, [&](int pos, int) { return clamped_load(w1, pos, n1); },
             [&](int, int pp, cplx<TC> v) { col1[pp] = v; });
         __syncthreads();
         static_for<1, P::K - 1>([&](auto S) {
@@ -213,7 +215,7 @@ __device__ __forceinline__ void pass_b_team(int t, cplx<TC> *mine, const cplx<TC
                                             const cplx<TC> *tw, const cplx<TC> *row, cplx<TC> dxv,
                                             const cplx<TC> *dy) {
     cround<TC, P, 0, false, true, true>(
-        t, tw, [&](int pos, int) { return pos < nc ? w[pos] : mk<TC>(0, 0); },
+        t, tw, [&](int pos, int) { return clamped_load(w, pos, nc); },
         [&](int, int pp, cplx<TC> v) { mine[pp] = v; });
     __syncthreads();
     static_for<1, P::K - 1>([&](auto S) {

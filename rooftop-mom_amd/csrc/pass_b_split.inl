@@ -23,11 +23,11 @@ __global__ void __launch_bounds__(MAXT) pass_by_fwd_k(const cplx<TC> *__restrict
     cplx<TC> *out = spec + ((static_cast<int64_t>(r) * 2 + comp) * d.Mx + px) * d.My;
     if constexpr (P::K == 1) {
         cround<TC, P, 0, false, true, false>(
-            t, tw, [&](int pos, int) { return pos < cols ? w[pos] : mk<TC>(0, 0); },
+            t, tw, [&](int pos, int) { return clamped_load(w, pos, cols); },
             [&](int pos, int, cplx<TC> v) { out[pos] = v; });
     } else {
         cround<TC, P, 0, false, true, true>(
-            t, tw, [&](int pos, int) { return pos < cols ? w[pos] : mk<TC>(0, 0); },
+            t, tw, [&](int pos, int) { return clamped_load(w, pos, cols); },
             [&](int, int pp, cplx<TC> v) { col[pp] = v; });
         __syncthreads();
         static_for<1, P::K - 1>([&](auto S) {

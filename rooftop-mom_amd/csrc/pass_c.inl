@@ -30,7 +30,11 @@ __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ wo
         if constexpr (P::K > 1) {
             // Round K-1 (no twiddles), threads along the columns, read from the work array.
             cround<TC, P, P::K - 1, true, false, false>(
-                t, tw, [&](int pos, int) { return live ? src[pos * ldj] : mk<TC>(0, 0); },
+                t, tw,
+                [&](int pos, int) {   // src is a valid column: load, then select (no load under a branch)
+                    const cplx<TC> v = src[pos * ldj];
+                    return live ? v : mk<TC>(0, 0);
+                },
                 [&](int, int pp, cplx<TC> v) { lds[pp] = v; });
             __syncthreads();
             static_for_down<1, P::K - 1>([&](auto S) {
@@ -39,8 +43,10 @@ __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ wo
                 __syncthreads();
             });
         } else {
-            for (int pos = t; pos < P::M; pos += P::T)
-                lds[P::pad(pos)] = live ? src[pos * ldj] : mk<TC>(0, 0);
+            for (int pos = t; pos < P::M; pos += P::T) {
+                const cplx<TC> v = src[pos * ldj];
+                lds[P::pad(pos)] = live ? v : mk<TC>(0, 0);
+            }
             __syncthreads();
         }
     }
@@ -50,15 +56,19 @@ __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ wo
     const int64_t off = static_cast<int64_t>(r) * d.n + (comp ? d.nx_count : 0);
     const int64_t col = static_cast<int64_t>(j0 + c) * rows;
     c128 *dst = out + off + col;
-    const uint8_t *exc = exists ? exists + (comp ? d.nx_count : 0) + col : nullptr;
     const cplx<TC> *lds = buf + c * CS;
-    cround<TC, P, 0, true, EMB && (P::M >= 2), (P::K > 1)>(
-        t, tw, [&](int, int pp) { return lds[pp]; },
-        [&](int pos, int, cplx<TC> v) {
-            if (pos >= rows) return;
-            if (exc && !exc[pos]) v = mk<TC>(0, 0);
-            dst[pos] = cvt<double>(v);
+    auto load = [&](int, int pp) { return lds[pp]; };
+    if (exists) {   // a masked plate: the mask bytes are loaded before the branch around each store
+        const uint8_t *exc = exists + (comp ? d.nx_count : 0) + col;
+        cround<TC, P, 0, true, EMB && (P::M >= 2), (P::K > 1)>(t, tw, load, [&](int pos, int, cplx<TC> v) {
+            const bool keep = exc[min(pos, rows - 1)] != 0;
+            if (pos < rows) dst[pos] = keep ? cvt<double>(v) : mk(0.0, 0.0);
         });
+    } else {
+        cround<TC, P, 0, true, EMB && (P::M >= 2), (P::K > 1)>(t, tw, load, [&](int pos, int, cplx<TC> v) {
+            if (pos < rows) dst[pos] = cvt<double>(v);
+        });
+    }
 }
 
 template <typename TC>

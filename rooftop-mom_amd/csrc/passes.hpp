@@ -18,6 +18,14 @@
 
 namespace rt {
 
+// a[pos] for pos < count, else zero, as a load from a valid element (the last one again) and a select: a load under
+// a branch is waited for inside it, so the loads of a round would go out one at a time.
+template <typename T>
+__device__ __forceinline__ cplx<T> clamped_load(const cplx<T> *__restrict__ a, int pos, int count) {
+    const cplx<T> v = a[min(pos, max(count - 1, 0))];
+    return pos < count ? v : cplx<T>{0, 0};
+}
+
 // ------------------------------------------------------------------------------------------------------------------
 // Compile-time plan: the same schedule as make_plan(1 << LOG2M, BITS) (fft_plan.cpp), whose tables it reads.
 // ------------------------------------------------------------------------------------------------------------------
