@@ -475,12 +475,12 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
         __shared__ int last;
         const int x = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
         if (x < 2 * j + 3) dcgs_slot_sum(partial, r, blocks, j, x, threadIdx.x & 63, scal);
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // the sums are visible to every XCD ...
         __syncthreads();
         if (threadIdx.x == 0) last = atomicAdd(&st[r].arrivals, 1) == static_cast<int>(gridDim.x) - 1;
         __syncthreads();
         if (!last) return;
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // ... before the last block reads them
         if (threadIdx.x == 0) st[r].arrivals = 0;
     }
     __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp], colf[kLdp + 1], coln[kLdp + 1];
@@ -491,34 +491,33 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
     c128 *Hr = Hraw + static_cast<int64_t>(r) * ldh * m, *Hc = Hall + static_cast<int64_t>(r) * ldh * m;
     c128 *g = gall + static_cast<int64_t>(r) * 4 * ldh, *cs = g + ldh, *sn = g + 2 * ldh, *gs = g + 3 * ldh;
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-    for (int k = threadIdx.x; k < j; k += kThreads) {
-        s[k] = sc[kH2 + k];
-        z[k] = sc[kH1 + k];
-        csv[k] = cs[k];
-        snv[k] = sn[k];
-        colp[k] = Hr[static_cast<int64_t>(j - 1) * ldh + k];
-    }
-    // H(k, l) of the columns l < j - 1, l = q mod 4, for (H s)_k with k = lane: loaded with everything else, so that
-    // the step costs one memory latency.
+    // Everything the step reads is loaded before any of it is stored, from addresses clamped into range (a load under
+    // a branch would be waited for inside it), so that the step costs one memory latency: entry k = thread of s, z,
+    // the rotations and column j - 1 as formed (j <= 62 entries), H(k, l) of the columns l < j - 1, l = q mod 4, for
+    // (H s)_k with k = lane, and the scalars.
+    const int kk = min(static_cast<int>(threadIdx.x), max(j - 1, 0)), jm = max(j - 1, 0);
+    const c128 sv0 = sc[kH2 + kk], zv0 = sc[kH1 + kk], cv0 = cs[kk], nv0 = sn[kk];
+    const c128 pv0 = Hr[static_cast<int64_t>(jm) * ldh + kk];
     const int k0 = max(0, lane - 1), l0 = k0 + ((q - k0) & 3);
     c128 hk[kLdp / 4];
 #pragma unroll
     for (int u = 0; u < kLdp / 4; ++u) {
         const int l = l0 + 4 * u;
-        hk[u] = lane <= j && l < j - 1 ? Hr[static_cast<int64_t>(l) * ldh + lane] : mk(0.0, 0.0);
+        const c128 v = Hr[static_cast<int64_t>(min(l, max(j - 2, 0))) * ldh + min(lane, j)];
+        hk[u] = pick(lane <= j && l < j - 1, v, mk(0.0, 0.0));
     }
-    double alpha = 0, omega = 0, ref = 0, tol = 0, maxit = 0;
-    c128 t = mk(0.0, 0.0), gprev = mk(0.0, 0.0);
-    if (threadIdx.x == 0) {
-        shb[0] = j >= 1 ? Hr[static_cast<int64_t>(j - 1) * ldh + j].x : 0.0;
-        alpha = sc[kAlpha].x;
-        t = sc[kT];
-        omega = sc[kOmega].x;
-        ref = sc[kReference].x;
-        tol = sc[kTol].x;
-        maxit = sc[kMaxIt].x;
-        gprev = j >= 1 ? gs[j - 1] : g[0];
+    const double hsub = Hr[static_cast<int64_t>(jm) * ldh + j].x;
+    const double alpha = sc[kAlpha].x, omega = sc[kOmega].x, ref = sc[kReference].x, tol = sc[kTol].x;
+    const double maxit = sc[kMaxIt].x;
+    const c128 t = sc[kT], gprev = (j >= 1 ? gs + j - 1 : g)[0];
+    if (static_cast<int>(threadIdx.x) < j) {
+        s[threadIdx.x] = sv0;
+        z[threadIdx.x] = zv0;
+        csv[threadIdx.x] = cv0;
+        snv[threadIdx.x] = nv0;
+        colp[threadIdx.x] = pv0;
     }
+    if (threadIdx.x == 0) shb[0] = j >= 1 ? hsub : 0.0;
     __syncthreads();
     if (q == 0) {   // |s|^2, s^H z, |z|^2
         double ss = 0, szr = 0, szi = 0, zz = 0;
