@@ -157,24 +157,51 @@ __global__ void __launch_bounds__(kApplyThreads) sai_apply_tiled(const c128 *__r
     bool mine[kComps];
     int64_t idx[kComps];
     int stc[kComps];
+    // Every load below reads a valid address (clamped into range) and a select drops what lies outside, so that
+    // all of them are in flight together: a load under a branch is waited for inside it, one at a time.
 #pragma unroll
     for (int c = 0; c < kComps; ++c) {
         const int di = d0 + c;
         const int rows_i = di ? rows_y : rows_x, cols_i = di ? cols_y : cols_x;
         mine[c] = fx < rows_i && fy < cols_i;
         idx[c] = (di ? nx : 0) + fx + static_cast<int64_t>(rows_i) * fy;
-        stc[c] = mine[c] ? scls[idx[c]] : -1;
+        const int sv = scls[min(idx[c], n - 1)];
+        stc[c] = mine[c] ? sv : -1;
     }
-    for (int e = threadIdx.x; e < table_count * kSlots; e += kApplyThreads) coef_lds[e] = scoef[e];
+    constexpr int kStage = 16 * 1024 / static_cast<int>(sizeof(cplx<TC>)) / kApplyThreads;   // the LDS table's cap
+    const int tc = table_count * kSlots;
+    cplx<TC> cv[kStage];   // (the table holds at least one class: entry 0 is always valid)
+#pragma unroll
+    for (int u = 0; u < kStage; ++u)
+        cv[u] = scoef[min(static_cast<int>(threadIdx.x) + u * kApplyThreads, max(tc - 1, 0))];
     const c128 *x = in + static_cast<int64_t>(r) * n;
-    for (int e = threadIdx.x; e < 2 * LY * LX; e += kApplyThreads) {
+    constexpr int kHalo = (2 * LY * LX + kApplyThreads - 1) / kApplyThreads;
+    cplx<TC> hv[kHalo];
+#pragma unroll
+    for (int u = 0; u < kHalo; ++u) {
+        const int e = min(static_cast<int>(threadIdx.x) + u * kApplyThreads, 2 * LY * LX - 1);
         const int d = e / (LY * LX), rem = e - d * (LY * LX), ly = rem / LX, lx = rem - ly * LX;
         const int gx = X0 + lx - 1, gy = Y0 + ly - 1;
         const int rows = d ? rows_y : rows_x, cols = d ? cols_y : cols_x;
-        cplx<TC> v = mk<TC>(0, 0);
-        if (gx >= 0 && gy >= 0 && gx < rows && gy < cols)
-            v = cvt<TC>(x[(d ? nx : 0) + gx + static_cast<int64_t>(rows) * gy]);
-        tile[d][ly][lx] = v;
+        const bool in_grid = gx >= 0 && gy >= 0 && gx < rows && gy < cols;
+        const int cx = min(max(gx, 0), max(rows - 1, 0)), cy = min(max(gy, 0), max(cols - 1, 0));
+        const cplx<TC> v = cvt<TC>(x[min((d ? nx : 0) + cx + static_cast<int64_t>(rows) * cy, n - 1)]);
+        hv[u] = in_grid ? v : mk<TC>(0, 0);
+    }
+    if (tc > 0) {
+#pragma unroll
+        for (int u = 0; u < kStage; ++u) {
+            const int e = static_cast<int>(threadIdx.x) + u * kApplyThreads;
+            if (e < tc) coef_lds[e] = cv[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kHalo; ++u) {
+        const int e = static_cast<int>(threadIdx.x) + u * kApplyThreads;
+        if (e < 2 * LY * LX) {
+            const int d = e / (LY * LX), rem = e - d * (LY * LX), ly = rem / LX, lx = rem - ly * LX;
+            tile[d][ly][lx] = hv[u];
+        }
     }
     __syncthreads();
 #pragma unroll
