@@ -415,13 +415,17 @@ __device__ __forceinline__ void dcgs_slot_sum(const c128 *__restrict__ partial, 
         re[0] += v.x;
         im[0] += v.y;
     }
+    static_assert(kRound == 12, "the pairwise sum below is written for twelve");
 #pragma unroll
-    for (int w = kRound / 2; w >= 3; w /= 2)
+    for (int u = 0; u < 6; ++u) {
+        re[u] += re[u + 6];
+        im[u] += im[u + 6];
+    }
 #pragma unroll
-        for (int u = 0; u < w; ++u) {
-            re[u] += re[u + w];
-            im[u] += im[u + w];
-        }
+    for (int u = 0; u < 3; ++u) {
+        re[u] += re[u + 3];
+        im[u] += im[u + 3];
+    }
     const double sr = wave_sum((re[0] + re[1]) + re[2]);
     const double si = wave_sum((im[0] + im[1]) + im[2]);
     if (lane == 0) scal[r * kScal + dst] = mk(sr, si);
