@@ -85,6 +85,9 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     const int r = blockIdx.y; \
     if (st && st[r].done) return
 
+// a where c holds, else b, component by component (a select of whole structures would go through memory).
+__device__ __forceinline__ c128 pick(bool c, c128 a, c128 b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
+
 // partial[(r * blocks + b) * kLdp] = sum over block b's rows of system r of x^T y (or x^H y).
 template <bool CONJ>
 __global__ void __launch_bounds__(kThreads) k_dot(const c128 *__restrict__ x, const c128 *__restrict__ y, int64_t n,
@@ -124,11 +127,22 @@ __global__ void __launch_bounds__(kThreads) k_reduce(const c128 *__restrict__ pa
     SYSTEM(r);
     __shared__ double sh[8];
     const int k = blockIdx.x;
+    // Rounds of eight loads per thread from addresses clamped into range (a load under a branch is waited for inside
+    // it), summed in the order of the blocks.
+    const c128 *part = partial + static_cast<int64_t>(r) * blocks * kLdp + k;
     double re = 0, im = 0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) {
-        const c128 v = partial[(static_cast<int64_t>(r) * blocks + b) * kLdp + k];
-        re += v.x;
-        im += v.y;
+    for (int b0 = 0; b0 < blocks; b0 += 8 * kThreads) {
+        c128 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = b0 + static_cast<int>(threadIdx.x) + u * kThreads;
+            v[u] = pick(b < blocks, part[static_cast<int64_t>(min(b, blocks - 1)) * kLdp], mk(0.0, 0.0));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            re += v[u].x;
+            im += v[u].y;
+        }
     }
     re = block_sum(re, sh);
     im = block_sum(im, sh);
@@ -222,23 +236,37 @@ __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restri
                                                             LoopState *st, double *history, int cap) {
     SYSTEM(r);
     __shared__ double sh[8];
+    // Every load is issued before any is used, from addresses clamped into range (a load under a branch is waited
+    // for inside it): the scalars, then the partials in rounds of eight per thread, summed in the order of the blocks.
+    c128 *sc = scal + r * kScal;
+    const double bnorm = sc[kBnorm].x, best = sc[kBest].x, tol = sc[kTol].x, maxit = sc[kMaxIt].x;
+    const c128 *part = partial + static_cast<int64_t>(r) * blocks * kLdp;
     double s = 0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) s += partial[(static_cast<int64_t>(r) * blocks + b) * kLdp].x;
+    for (int b0 = 0; b0 < blocks; b0 += 8 * kThreads) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = b0 + static_cast<int>(threadIdx.x) + u * kThreads;
+            const double x = part[static_cast<int64_t>(min(b, blocks - 1)) * kLdp].x;
+            v[u] = b < blocks ? x : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
     s = block_sum(s, sh);
     if (threadIdx.x != 0) return;
-    c128 *sc = scal + r * kScal;
-    const double res = sqrt(s) / sc[kBnorm].x;
+    const double res = sqrt(s) / bnorm;
     history[static_cast<int64_t>(r) * cap + it] = res;
     st[r].iterations = it;
-    if (res < sc[kBest].x) {
+    if (res < best) {
         sc[kBest] = mk(res, 0.0);
         st[r].best = it;
     }
-    if (res < sc[kTol].x) {
+    if (res < tol) {
         st[r].converged = st[r].finished = st[r].done = 1;
     } else if (!isfinite(res)) {
         st[r].finished = st[r].done = 1;
-    } else if (it >= static_cast<int>(sc[kMaxIt].x) || (stall > 0 && it - st[r].best >= stall)) {
+    } else if (it >= static_cast<int>(maxit) || (stall > 0 && it - st[r].best >= stall)) {
         st[r].done = 1;
     }
 }
@@ -445,8 +473,6 @@ __device__ __forceinline__ double readlane_f64(double v, int k) {
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
-// a where c holds, else b, component by component (a select of whole structures would go through memory).
-__device__ __forceinline__ c128 pick(bool c, c128 a, c128 b) { return mk(c ? a.x : b.x, c ? a.y : b.y); }
 
 // Givens rotation of (a, sub) with sub real: c = a / radius, s = sub / radius; returns the rotated diagonal.
 __device__ __forceinline__ c128 givens(c128 a, double sub, c128 &c, c128 &sv) {
