@@ -210,9 +210,9 @@ __global__ void __launch_bounds__(kApplyThreads) sai_apply_tiled(const c128 *__r
         const int64_t i = idx[c];
         const int st = stc[c];
         c128 res = mk(0.0, 0.0);
-        if (st >= 0) {
-            const cplx<TC> *cf = (table_count > 0 ? coef_lds : scoef) + static_cast<int64_t>(st) * kSlots;
-            cplx<TC> acc = mk<TC>(0, 0);
+        {   // every coefficient and neighbour read first, then the sum over the non-zero coefficients in slot order
+            const cplx<TC> *cf = (table_count > 0 ? coef_lds : scoef) + static_cast<int64_t>(max(st, 0)) * kSlots;
+            cplx<TC> cs[kSlots], vs[kSlots];
 #pragma unroll
             for (int d = 0; d < 2; ++d)
 #pragma unroll
@@ -220,10 +220,17 @@ __global__ void __launch_bounds__(kApplyThreads) sai_apply_tiled(const c128 *__r
 #pragma unroll
                     for (int dx = -1; dx <= 1; ++dx) {
                         const int s = d * 9 + (dx + 1) + 3 * (dy + 1);
-                        const cplx<TC> c = cf[s];
-                        if (c.x != 0 || c.y != 0) acc = acc + c * tile[d][ty + 1 + dy][tx + 1 + dx];
+                        cs[s] = cf[s];
+                        vs[s] = tile[d][ty + 1 + dy][tx + 1 + dx];
                     }
-            res = cvt<double>(acc);
+            cplx<TC> acc = mk<TC>(0, 0);
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const cplx<TC> nx = acc + cs[s] * vs[s];
+                const bool use = cs[s].x != 0 || cs[s].y != 0;
+                acc = mk<TC>(use ? nx.x : acc.x, use ? nx.y : acc.y);
+            }
+            if (st >= 0) res = cvt<double>(acc);
         }
         out[static_cast<int64_t>(r) * n + i] = res;
     }
