@@ -199,6 +199,42 @@ def test_batched_solve_matches_single_solves(rt, method, precision):
 
 
 
+def test_row_block_gram_schmidt_matches_single_solves(rt):
+    """Enough systems at once (a 128 x 128 plate, 17 right-hand sides) that the GMRES dot and update passes run on
+    row blocks give, column by column, the results of the single solves, whose dot pass runs on strided 64-row tiles
+    and whose update pass takes one tile per block: the same iteration counts, monitored norms and solutions up to
+    the order of the sums (FP64 basis), and the same solutions to the mixed form's accuracy (complex64 basis)."""
+    cx = cy = 128
+    p = rt.Problem(rt.Mesh(cx, cy, 0.1, 0.1), 2 * np.pi, quadratureOrder=(6, 4))
+    grid = A.Grid(cx, cy, 0.1, 0.1)
+    rng = np.random.default_rng(17)
+    cols = []
+    for q in range(17):
+        d = rng.standard_normal(3)
+        d[2] = -abs(d[2]) - 0.5
+        d /= np.linalg.norm(d)
+        e = np.cross(d, [0.0, 1.0, 0.0])
+        cols.append(X.plane_wave(grid, 2 * np.pi, d, e / np.linalg.norm(e), 1.0, order=4))
+    B = np.stack(cols, axis=1)
+    op = rt.Operator(p, precision='mixed')
+    pc = rt.Preconditioner(p, 'nearField', precision='mixed')
+    for precision in ('c128', 'c64'):
+        print('precision', precision, flush=True)
+        solver = rt.Solver('gmres', tolerance=1e-8, restart=30, precision=precision)
+        batch = solver.solve(op, B, preconditioner=pc)
+        for q in (0, 7, 16):
+            alone = solver.solve(op, B[:, q], preconditioner=pc)
+            got = batch[q]
+            assert got.converged() and alone.converged()
+            if precision == 'c128':
+                assert got.iterationCount() == alone.iterationCount()
+                np.testing.assert_allclose(got.residualNorms(), alone.residualNorms(), rtol=1e-9)
+                assert rel(got.solution(), alone.solution()) < 1e-10
+            else:
+                assert abs(got.iterationCount() - alone.iterationCount()) <= 1
+                assert rel(got.solution(), alone.solution()) < 1e-6
+
+
 @pytest.mark.parametrize('precision', ['c128', 'c64'])
 def test_batched_solve_with_chunked_products(rt, precision):
     """A batch whose matvec runs in right-hand-side chunks on two streams gives bitwise the results of the unchunked
