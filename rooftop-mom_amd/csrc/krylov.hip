@@ -459,12 +459,28 @@ __device__ __forceinline__ void dcgs_slot_sum(const c128 *__restrict__ partial, 
     if (lane == 0) scal[r * kScal + dst] = mk(sr, si);
 }
 
-// Lane shuffles of FP64 values as two 32-bit halves: from lane - d (own value below d), and from the uniform lane k.
-__device__ __forceinline__ double shfl_up_f64(double v, int d) {
+// Cross-lane moves of FP64 values as two 32-bit halves. dpp_f64: a DPP move (CTRL) into the rows of ROW_MASK; a
+// lane without a source, or outside the rows, gets `old`. readlane_f64: the value of the uniform lane k.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double old, double v) {
+    const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __shfl_up(static_cast<unsigned>(b), d, 64);
-    const unsigned hi = __shfl_up(static_cast<unsigned>(b >> 32), d, 64);
+    const unsigned lo =
+        __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(b), CTRL, ROW_MASK, 0xf, false);
+    const unsigned hi =
+        __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(b >> 32), CTRL, ROW_MASK, 0xf, false);
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+// One step of a wave's inclusive scan of the affine maps x -> M x + B (two B at once, the later map applied after
+// the earlier): every lane composes its map after the one the DPP move brings (the identity where none).
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ void affine_scan_step(double &M, c128 &bf, c128 &bn) {
+    const double Mp = dpp_f64<CTRL, ROW_MASK>(1.0, M);
+    const c128 fp = mk(dpp_f64<CTRL, ROW_MASK>(0.0, bf.x), dpp_f64<CTRL, ROW_MASK>(0.0, bf.y));
+    const c128 np = mk(dpp_f64<CTRL, ROW_MASK>(0.0, bn.x), dpp_f64<CTRL, ROW_MASK>(0.0, bn.y));
+    bf = M * fp + bf;
+    bn = M * np + bn;
+    M = M * Mp;
 }
 __device__ __forceinline__ double readlane_f64(double v, int k) {
     const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
@@ -623,24 +639,18 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
         bf = ck * af;
         bn = ck * an;
     }
-    for (int d = 1; d < K; d <<= 1) {
-        const double Mp = shfl_up_f64(M, d);
-        const c128 fp = mk(shfl_up_f64(bf.x, d), shfl_up_f64(bf.y, d));
-        const c128 np = mk(shfl_up_f64(bn.x, d), shfl_up_f64(bn.y, d));
-        if (lane >= d) {
-            bf = M * fp + bf;
-            bn = M * np + bn;
-            M = M * Mp;
-        }
-    }
-    // run_k on lane k from the composition of the steps before it.
-    double Me = shfl_up_f64(M, 1);
-    c128 fe = mk(shfl_up_f64(bf.x, 1), shfl_up_f64(bf.y, 1));
-    c128 ne = mk(shfl_up_f64(bn.x, 1), shfl_up_f64(bn.y, 1));
-    if (lane == 0) {
-        Me = 1;
-        fe = ne = mk(0.0, 0.0);
-    }
+    // The scan by DPP moves: row_shr 1, 2, 4, 8 within the rows of 16 lanes, then lane 15 of rows 0 and 2 into rows 1
+    // and 3 (row_bcast:15) and lane 31 into rows 2 and 3 (row_bcast:31).
+    affine_scan_step<0x111>(M, bf, bn);
+    affine_scan_step<0x112>(M, bf, bn);
+    affine_scan_step<0x114>(M, bf, bn);
+    affine_scan_step<0x118>(M, bf, bn);
+    affine_scan_step<0x142, 0xa>(M, bf, bn);
+    affine_scan_step<0x143, 0xc>(M, bf, bn);
+    // run_k on lane k from the composition of the steps before it (wave_shr:1; lane 0: the identity).
+    const double Me = dpp_f64<0x138>(1.0, M);
+    const c128 fe = mk(dpp_f64<0x138>(0.0, bf.x), dpp_f64<0x138>(0.0, bf.y));
+    const c128 ne = mk(dpp_f64<0x138>(0.0, bn.x), dpp_f64<0x138>(0.0, bn.y));
     const c128 runf = Me * colf[0] + fe, runn = Me * coln[0] + ne;
     const c128 outf = cmulc(ck, runf) + sk * af, outn = cmulc(ck, runn) + sk * an;   // final entries k < K
     c128 cb = mk(readlane_f64(runn.x, K), readlane_f64(runn.y, K));
