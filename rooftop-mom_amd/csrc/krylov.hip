@@ -62,14 +62,47 @@ enum PoolSlot {
 // ------------------------------------------------------------------------------------------------------------------
 // Reductions.
 // ------------------------------------------------------------------------------------------------------------------
+// Cross-lane moves of FP64 values as two 32-bit halves. dpp_f64: a DPP move (CTRL) into the rows of ROW_MASK; a
+// lane without a source, or outside the rows, gets `old`. readlane_f64: the value of the uniform lane k.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double old, double v) {
+    const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo =
+        __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(b), CTRL, ROW_MASK, 0xf, false);
+    const unsigned hi =
+        __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(b), k);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), k);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// Sum over the wave, in every lane (the GMRES Gram-Schmidt kernels): an inclusive scan by DPP moves (row_shr 1, 2, 4, 8 within the rows of 16 lanes,
+// then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3) leaves the total in lane 63.
 __device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0x111>(0.0, v);
+    v += dpp_f64<0x112>(0.0, v);
+    v += dpp_f64<0x114>(0.0, v);
+    v += dpp_f64<0x118>(0.0, v);
+    v += dpp_f64<0x142, 0xa>(0.0, v);
+    v += dpp_f64<0x143, 0xc>(0.0, v);
+    return readlane_f64(v, 63);
+}
+
+// Sum over the wave by a shuffle tree, valid in lane 0: the vector reductions and COCG keep this order (COCG's
+// short recurrence on these systems is sensitive to the rounding of its inner products).
+__device__ __forceinline__ double wave_sum_tree(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
 
 // Sum over the block (all threads call it); valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double *sh) {
-    v = wave_sum(v);
+    v = wave_sum_tree(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) sh[wave] = v;
@@ -459,18 +492,6 @@ __device__ __forceinline__ void dcgs_slot_sum(const c128 *__restrict__ partial, 
     if (lane == 0) scal[r * kScal + dst] = mk(sr, si);
 }
 
-// Cross-lane moves of FP64 values as two 32-bit halves. dpp_f64: a DPP move (CTRL) into the rows of ROW_MASK; a
-// lane without a source, or outside the rows, gets `old`. readlane_f64: the value of the uniform lane k.
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ __forceinline__ double dpp_f64(double old, double v) {
-    const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo =
-        __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(b), CTRL, ROW_MASK, 0xf, false);
-    const unsigned hi =
-        __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(b >> 32), CTRL, ROW_MASK, 0xf, false);
-    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
-}
 // One step of a wave's inclusive scan of the affine maps x -> M x + B (two B at once, the later map applied after
 // the earlier): every lane composes its map after the one the DPP move brings (the identity where none).
 template <int CTRL, int ROW_MASK = 0xf>
@@ -481,12 +502,6 @@ __device__ __forceinline__ void affine_scan_step(double &M, c128 &bf, c128 &bn) 
     bf = M * fp + bf;
     bn = M * np + bn;
     M = M * Mp;
-}
-__device__ __forceinline__ double readlane_f64(double v, int k) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __builtin_amdgcn_readlane(static_cast<int>(b), k);
-    const unsigned hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), k);
-    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
 
