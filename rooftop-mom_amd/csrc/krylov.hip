@@ -868,8 +868,7 @@ __global__ __launch_bounds__(64) void k_gmres_solve(const c128 *__restrict__ Hal
             yk = (1.0 / abs2(d)) * cmulc(d, acc);
             mine = yk;
         }
-        yk.x = __shfl(yk.x, k, 64);
-        yk.y = __shfl(yk.y, k, 64);
+        yk = mk(readlane_f64(yk.x, k), readlane_f64(yk.y, k));   // k is uniform: a scalar read of lane k
         if (lane < k) acc = acc - hs[k * 64 + lane] * yk;
     }
     if (row) scal[r * kScal + kY + lane] = mine;
