@@ -71,7 +71,7 @@ typedef struct rt_ctx rt_ctx;
 typedef struct rt_problem rt_problem;
 typedef struct rt_product rt_product;
 
-/* Solver options: rooftopMoM.Solver name-value inputs (Solver.m:389-410) plus the precision of the inner loop.
+/* Solver options: rooftopMoM.Solver name-value inputs (Solver.m:17-38) plus the precision of the inner loop.
  *   precision       RT_C128 or RT_C64: arithmetic of the products inside the Krylov loop.
  *   refine          0: the reference's loop as is (the true residual is reported, not enforced);
  *                   1: an FP64 outer loop on the true residual norm(b - A x)/norm(b) until it is below tolerance
@@ -109,13 +109,17 @@ int rt_ctx_create(int32_t device, uint64_t hbm_budget_bytes, rt_ctx **out);
 void rt_ctx_destroy(rt_ctx *ctx);
 /* Resources.held()/ceiling() (src/+rooftopMoM/Resources.m:55-71), in device bytes. */
 int rt_ctx_memory(rt_ctx *ctx, uint64_t *held, uint64_t *ceiling);
+/* The most device bytes the context held at once since it was created or since the last call with reset != 0 (a
+ * reset starts the next interval at the bytes held now): the peak that Resources.held() (Resources.m:55-71) would
+ * have shown, for checking a byte estimate against what a solve actually holds. */
+int rt_ctx_memory_peak(rt_ctx *ctx, int32_t reset, uint64_t *peak);
 /* Blocks until the context's stream is idle. */
 int rt_ctx_synchronize(rt_ctx *ctx);
 /* Device address of the context's HIP stream (hipStream_t), for callers that time or order work on it. */
 int rt_ctx_stream(rt_ctx *ctx, void **stream);
 
 /* rooftopMoM.Problem(mesh, k, 'quadratureOrder', [near far]) (src/+rooftopMoM/Problem.m:25-92) on a uniform
- * countX x countY grid of sizeX x sizeY faces (rooftopMoM.Mesh, Mesh.m:163-185). The generators are assembled on the
+ * countX x countY grid of sizeX x sizeY faces (rooftopMoM.Mesh, Mesh.m:10-30). The generators are assembled on the
  * GPU on the first call that needs them (Problem.generators, Problem.m:135-145). mask (countX*countY bytes,
  * column-major, NULL = full plate) keeps the rooftops whose two faces are both metal. */
 int rt_problem_create(rt_ctx *ctx, int32_t count_x, int32_t count_y, double size_x, double size_y,
@@ -160,6 +164,19 @@ int rt_product_apply_device(rt_product *product, const void *input, void *output
 /* Product.memory / workingMemory (Product.m:47-57): device bytes of the stored spectra or coefficients, and of the
  * buffers one application uses. */
 int rt_product_memory(rt_product *product, uint64_t *stored, uint64_t *working);
+
+/* Applicator.spectra() (src/+rooftopMoM/+assembler/+greensMatrix/Applicator.m:20-22) of a Green's matrix product:
+ * dims (4 entries, may be NULL) receives the embedding periods Mx, My and the folded extents Hx = Mx/2 + 1,
+ * Hy = My/2 + 1; planes (may be NULL, else 3 Hx Hy complex128) the stored spectra in this library's three-plane form
+ * (magneticXX, magneticYY, electric; each the 2D DFT of the generator's even embedding of period Mx x My, scaled by
+ * 1/(Mx My), folded to frequencies [0, Mx/2] x [0, My/2], column-major). RT_EINVAL for a preconditioner. */
+int rt_product_spectra(rt_product *product, int32_t *dims, void *planes);
+/* The static Applicator.estimateWorkingMemory(rooftops, sizeX, sizeY, content) (Applicator.m:44-46,
+ * FFTApplicator.m:36-49) with the stored bytes beside it, before any spectra exist: the device bytes that
+ * rt_product_create(precision) of a count_x x count_y grid stores (stored) and that one application of it uses for
+ * one right-hand side (working), as rt_product_memory reports them once it exists. */
+int rt_product_estimate_memory(int32_t count_x, int32_t count_y, int32_t precision, uint64_t *stored,
+                               uint64_t *working);
 
 /* Measurement (no counterpart in the reference): runs `reps` applications of the operator on nrhs device vectors
  * and times each of its three kernels (pass A: x transforms, pass B: y transforms + spectral multiply, pass C:

@@ -60,5 +60,7 @@ def input_impedance(grid, current, port, voltage=1.0):
 
 
 def radar_cross_section(wavenumber, excitation, current, intensity=1.0):
-    """(k Z0)^2 |b.' I|^2 / (4 pi E^4). +operator/computeRadarCrossSection.m:18-19."""
+    """(k Z0)^2 |b.' I|^2 / (4 pi E^4). +operator/computeRadarCrossSection.m:18-19.
+
+    Pinned: the closed form of ComputeRadarCrossSectionTest.m:5-11 at 1e-15 and intensity independence (:13-23)."""
     return (wavenumber * Z0) ** 2 * abs(np.asarray(excitation) @ np.asarray(current)) ** 2 / (4 * np.pi * intensity ** 4)

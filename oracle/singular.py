@@ -110,7 +110,7 @@ _TOUCHING = ((0, 0), (1, 0), (0, 1), (1, 1))
 def _compose_part(weight, part, offset, cell_size, count):
     """One weight on one part of [-1, 1] as a polynomial in u = sigma*cellSize*(nu - offset) >= 0, and its interval.
 
-    Follows generatePowerMoments.m:155-190 (composePart_). part is 0 for [-1, 0] and 1 for [0, 1]; the returned
+    Follows generatePowerMoments.m:121-155 (composePart_). part is 0 for [-1, 0] and 1 for [0, 1]; the returned
     interval is 0 for [0, cellSize] and 1 for [cellSize, 2 cellSize].
     """
     ends = ((-1.0, 0.0), (0.0, 1.0))[part]
@@ -181,7 +181,7 @@ def power_moments(size_x, size_y, power):
 def extracted_moments(size_x, size_y, wavenumber):
     """Moments of S(R) = 1/R - k^2 R/2 + k^4 R^3/24 at the touching offsets (7x4 complex).
 
-    Follows src/+rooftopMoM/+assembler/+singularIntegrals/generateExtractedMoments.m:31-33.
+    Follows src/+rooftopMoM/+assembler/+singularIntegrals/generateExtractedMoments.m:15-18.
     """
     k = float(wavenumber)
     return (power_moments(size_x, size_y, -1) - k * k / 2 * power_moments(size_x, size_y, 1)

@@ -6,7 +6,7 @@ load() in this repository's __graft_entry__.py). Every numerical step of the pat
 preconditioners, Krylov loops -- runs in librooftop.so on the GPU; without the library every class raises.
 
 Names, argument meanings, defaults and error identifiers follow the reference MATLAB package rooftopMoM 3.5.0:
-  Mesh           src/+rooftopMoM/Mesh.m:163-214
+  Mesh           src/+rooftopMoM/Mesh.m:1-63
   Problem        src/+rooftopMoM/Problem.m:25-206
   Operator       src/+rooftopMoM/Operator.m:7-54 and Product.m:25-57
   Preconditioner src/+rooftopMoM/Preconditioner.m:7-70 (kinds here: 'nearField', 'jacobi')
@@ -63,6 +63,12 @@ class Context:
         check(library().rt_ctx_memory(self.handle, C.byref(held), C.byref(ceiling)))
         return held.value, ceiling.value
 
+    def peakMemory(self, reset=False):
+        """The most device bytes held at once since the context was created or the last reset."""
+        peak = C.c_uint64()
+        check(library().rt_ctx_memory_peak(self.handle, int(bool(reset)), C.byref(peak)))
+        return peak.value
+
     def synchronize(self):
         check(library().rt_ctx_synchronize(self.handle))
 
@@ -73,7 +79,7 @@ class Context:
 
 
 class Mesh:
-    """Uniform grid of countX x countY faces of sizeX x sizeY m (Mesh.m:163-185)."""
+    """Uniform grid of countX x countY faces of sizeX x sizeY m (Mesh.m:10-30)."""
 
     def __init__(self, countX, countY, sizeX, sizeY):
         if int(countX) != countX or int(countY) != countY or countX < 1 or countY < 1:
@@ -215,6 +221,19 @@ class Product:
         check(library().rt_product_memory(self.handle, None, C.byref(w)))
         return w.value
 
+    def spectra(self):
+        """Applicator.spectra() (Applicator.m:20-22): {'period': (Mx, My), 'magneticXX', 'magneticYY', 'electric'}, each
+        plane the DFT of the generator's even embedding scaled by 1/(Mx My), on the folded frequencies
+        [0, Mx/2] x [0, My/2] (Hx x Hy, complex128)."""
+        dims = (C.c_int32 * 4)()
+        check(library().rt_product_spectra(self.handle, dims, None))
+        hx, hy = dims[2], dims[3]
+        planes = np.zeros(3 * hx * hy, np.complex128)
+        check(library().rt_product_spectra(self.handle, dims, _ptr(planes)))
+        planes = planes.reshape(3, hy, hx).transpose(0, 2, 1)
+        return {'period': (dims[0], dims[1]), 'magneticXX': planes[0], 'magneticYY': planes[1],
+                'electric': planes[2]}
+
 
 class Operator(Product):
     """rooftopMoM.Operator(problem, 'product', ..., 'modifier', ...) (Operator.m:7-54)."""
@@ -227,6 +246,15 @@ class Operator(Product):
         check(library().rt_product_create(problem.handle, _PRODUCTS[product], _PRECISIONS[precision],
                                           _MODIFIERS[modifier], C.byref(h)))
         super().__init__(problem, h, _PRECISIONS[precision])
+
+    @staticmethod
+    def estimateMemory(countX, countY, precision='c128'):
+        """(stored, working) device bytes of the product of a countX x countY grid before it exists (the static
+        Applicator.estimateWorkingMemory, Applicator.m:44-46)."""
+        stored, working = C.c_uint64(), C.c_uint64()
+        check(library().rt_product_estimate_memory(int(countX), int(countY), _PRECISIONS[precision],
+                                                   C.byref(stored), C.byref(working)))
+        return stored.value, working.value
 
     def tune(self, nrhs=1, precision=None):
         """Times the launch shapes of the matvec passes for nrhs right-hand sides and keeps the fastest (the FFT
@@ -557,7 +585,7 @@ class Result:
 
 
 class Solver:
-    """rooftopMoM.Solver('method', ..., 'tolerance', ..., 'maxIterationCount', ..., 'restart', ...) (Solver.m:389-410).
+    """rooftopMoM.Solver('method', ..., 'tolerance', ..., 'maxIterationCount', ..., 'restart', ...) (Solver.m:17-38).
 
     precision 'c64' runs the products of the Krylov loop in complex64 inside an FP64 refinement loop on the true
     residual (refine is then implied); refine=True alone wraps the FP64 loop the same way.
@@ -693,7 +721,11 @@ class Server:
         req.near_order, req.far_order = int(quadratureOrder[0]), int(quadratureOrder[-1])
         m = None
         if mask is not None:
-            m = np.asfortranarray(mask, dtype=np.uint8).ravel(order='F')
+            mask = np.asarray(mask, dtype=np.uint8)
+            if mask.size != int(countX) * int(countY):
+                raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
+                                   'The mask must hold countX x countY = %d faces.' % (int(countX) * int(countY)))
+            m = np.asfortranarray(mask.reshape(int(countX), int(countY))).ravel(order='F')
             req.mask = m.ctypes.data
         req.rhs = b.ctypes.data
         req.n = b.size

@@ -39,6 +39,7 @@ SIGNATURES = [
     ('rt_ctx_create', C.c_int, [C.c_int32, C.c_uint64, C.POINTER(P)]),
     ('rt_ctx_destroy', None, [P]),
     ('rt_ctx_memory', C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ('rt_ctx_memory_peak', C.c_int, [P, C.c_int32, C.POINTER(C.c_uint64)]),
     ('rt_ctx_synchronize', C.c_int, [P]),
     ('rt_ctx_stream', C.c_int, [P, C.POINTER(P)]),
     ('rt_ctx_set_timing', C.c_int, [P, C.c_int32]),
@@ -56,6 +57,9 @@ SIGNATURES = [
     ('rt_product_apply', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
     ('rt_product_apply_device', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
     ('rt_product_memory', C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ('rt_product_spectra', C.c_int, [P, C.POINTER(C.c_int32), P]),
+    ('rt_product_estimate_memory', C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint64)]),
     ('rt_product_profile', C.c_int, [P, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
     ('rt_product_tune', C.c_int, [P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),

@@ -207,6 +207,15 @@ int rt_ctx_memory(rt_ctx *ctx, uint64_t *held, uint64_t *ceiling) {
     });
 }
 
+int rt_ctx_memory_peak(rt_ctx *ctx, int32_t reset, uint64_t *peak) {
+    return guarded([&] {
+        need(ctx, RT_EINVAL, "rooftop:ctx", "ctx is NULL");
+        std::lock_guard<std::mutex> g(ctx->c.lock);
+        if (peak) *peak = ctx->c.peak;
+        if (reset) ctx->c.peak = ctx->c.held;
+    });
+}
+
 int rt_ctx_synchronize(rt_ctx *ctx) {
     return guarded([&] {
         need(ctx, RT_EINVAL, "rooftop:ctx", "ctx is NULL");
@@ -440,6 +449,42 @@ int rt_product_memory(rt_product *p, uint64_t *stored, uint64_t *working) {
         need(p != nullptr, RT_EINVAL, "rooftop:product", "NULL product");
         if (stored) *stored = p->p->stored_bytes();
         if (working) *working = p->p->working_bytes();
+    });
+}
+
+int rt_product_spectra(rt_product *p, int32_t *dims, void *planes) {
+    return guarded([&] {
+        need(p && p->p, RT_EINVAL, "rooftop:product", "NULL product");
+        const Product &q = *p->p;
+        need(q.type == kOperator, RT_EINVAL, "rooftop:product",
+             "Only the Green's matrix products hold the three spectral planes.");
+        if (dims) {
+            dims[0] = q.dims.Mx;
+            dims[1] = q.dims.My;
+            dims[2] = q.dims.Hx;
+            dims[3] = q.dims.Hy;
+        }
+        if (planes) {
+            Context *c = q.problem->ctx;
+            c->activate();
+            RT_HIP(hipStreamSynchronize(c->stream));
+            RT_HIP(hipMemcpy(planes, q.planes64.ptr, q.planes64.bytes(), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int rt_product_estimate_memory(int32_t count_x, int32_t count_y, int32_t precision, uint64_t *stored,
+                               uint64_t *working) {
+    return guarded([&] {
+        need(count_x >= 1 && count_y >= 1 && (count_x > 1 || count_y > 1), RT_EINVAL, "rooftopMoM:mesh:noRooftops",
+             "The grid carries no rooftop.");
+        need(precision == RT_C128 || precision == RT_C64 || precision == RT_MIXED, RT_EINVAL,
+             "rooftop:product:invalidPrecision", "The precision is RT_C128, RT_C64 or RT_MIXED.");
+        const ProdDims d = operator_dims(GridInfo::make(count_x, count_y, 1.0, 1.0));
+        const bool c128 = precision != RT_C64, c64 = precision != RT_C128;
+        const uint64_t hx = d.Hx, my = d.My, plane = static_cast<uint64_t>(d.Hx) * d.Hy;
+        if (stored) *stored = 3 * plane * 16 + 3 * hx * my * ((c128 ? 16 : 0) + (c64 ? 8 : 0));
+        if (working) *working = static_cast<uint64_t>(2) * d.ldj * d.Mx * (c128 ? 16 : 8);
     });
 }
 

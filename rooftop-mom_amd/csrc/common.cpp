@@ -61,6 +61,7 @@ void *Context::alloc(uint64_t bytes, const char *what) {
     }
     std::lock_guard<std::mutex> g(lock);
     held += bytes;
+    if (held > peak) peak = held;
     sizes[p] = bytes;
     return p;
 }
@@ -138,6 +139,7 @@ void Context::close() {
     sizes.clear();
     pool.clear();
     held = 0;
+    peak = 0;
     for (auto &r : pending) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);

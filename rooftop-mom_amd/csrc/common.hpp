@@ -88,6 +88,7 @@ struct Context {
     hipEvent_t sync_event(int k);
     uint64_t ceiling = 0;
     uint64_t held = 0;
+    uint64_t peak = 0;   // the most bytes held at once since the context opened or the last reset_peak()
     std::mutex lock;
     std::unordered_map<void *, uint64_t> sizes;
 

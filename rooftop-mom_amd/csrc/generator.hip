@@ -310,7 +310,7 @@ __constant__ double kWeights[4][2][4] = {
 __constant__ int kWeightLen[4] = {2, 3, 3, 4};
 
 // The weight as a polynomial in u = sigma * size * (nu - offset) >= 0 on one part, and its interval (0: [0, size],
-// 1: [size, 2 size]) -- generatePowerMoments.m:155-190.
+// 1: [size, 2 size]) -- generatePowerMoments.m:121-155.
 __device__ int compose_part(int k, int part, int offset, double size, double *coef, int count) {
     const double lo = (part ? 0.0 : -1.0) - offset, hi = (part ? 1.0 : 0.0) - offset;
     const double sigma = hi <= 0 ? -1.0 : 1.0;
@@ -383,7 +383,7 @@ __device__ void along_x(double a, double b, int power, const double *gl_p, const
         }
 }
 
-// The extracted moments 7x4 (generateExtractedMoments.m:31-33): P(-1) - k^2/2 P(1) + k^4/24 P(3).
+// The extracted moments 7x4 (generateExtractedMoments.m:15-18): P(-1) - k^2/2 P(1) + k^4/24 P(3).
 // Six threads evaluate the six power moment tables (three powers along x and along y) at once; thread 0 combines.
 __global__ void touching_moments(double a, double b, double k, const double *gl_p, const double *gl_w,
                                  c128 *__restrict__ out) {
@@ -445,7 +445,7 @@ void Problem::assemble() {
     const c128 s_e = mk(0.0, kZ0 / (4 * M_PI) * (a * b) * (a * b) / k);
     const c128 s_m = mk(0.0, kZ0 / (4 * M_PI) * (a * b) * (a * b) * k);
 
-    // Far quadrant weights [q][7][order^2], x fastest (integrateLattice_.m:275-294).
+    // Far quadrant weights [q][7][order^2], x fastest (integrateLattice_.m:92-111).
     const int o = far_order;
     const int P2 = 2 * o;
     std::vector<double> qw(static_cast<size_t>(28) * o * o);

@@ -523,7 +523,7 @@ __device__ __forceinline__ c128 givens(c128 a, double sub, c128 &c, c128 &sv) {
 // LoopState, which it returns to zero) then finalizes the step: beta, z_j and nu; column j - 1 of H in its final
 // form, its rotation formed again and the monitored residual of iteration it - 1 refined; column j with the
 // subdiagonal nu (final one step later), its rotation, the rotated right-hand side g, the monitored residual
-// |g(j+1)| / reference and the stopping test (iterate_.m:252-278); 1 / beta and 1 / (beta nu) for the update pass.
+// |g(j+1)| / reference and the stopping test (iterate_.m:51-66); 1 / beta and 1 / (beta nu) for the update pass.
 // H holds the columns as formed (Hraw, for H s) and rotated (Hall, for the triangular solve); gall holds g, the
 // cosines, the sines and g before each rotation. Everything the step reads is loaded at once; the norms are one
 // wave's sums; the earlier rotations reach both columns by one wave's scan. Grid (ceil((2 j + 3) / 4), R).
@@ -546,7 +546,7 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
     }
     __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp], colf[kLdp + 1], coln[kLdp + 1];
     __shared__ c128 csv[kLdp], snv[kLdp];
-    __shared__ double shb[3];   // nu_{j-1}, beta, nu
+    __shared__ double shb[4];   // nu_{j-1}, beta, nu, breakdown (p in the span of the basis)
     c128 *sc = scal + r * kScal;
     const int ldh = m + 1;
     c128 *Hr = Hraw + static_cast<int64_t>(r) * ldh * m, *Hc = Hall + static_cast<int64_t>(r) * ldh * m;
@@ -597,6 +597,7 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
             const double b2 = alpha - ss;
             const double beta = b2 > 0 && isfinite(b2) ? sqrt(b2) : 0.0;
             shb[1] = beta;
+            shb[3] = j >= 1 && isfinite(b2) && !(b2 > 0) ? 1.0 : 0.0;
             const c128 zj = beta > 0 ? (1.0 / beta) * (t - mk(szr, szi)) : mk(0.0, 0.0);
             z[j] = zj;
             const double n2 = omega - (zz + abs2(zj));
@@ -605,7 +606,11 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
     }
     __syncthreads();
     const double nup = shb[0], beta = shb[1];
-    if (!(beta > 0)) {   // p vanished or is not finite: the system stops without a new column
+    // A finite p that vanishes after its reorthogonalization lies in the span of the basis: the subdiagonal of column
+    // j - 1 is zero, so the residual of iteration it - 1 is zero and the system has converged there (a zero
+    // subdiagonal in iterate_.m:62-66). Column j - 1 is finalized and rotated below; column j is never formed.
+    const bool brk = shb[3] != 0;
+    if (!(beta > 0) && !brk) {   // p is not finite: the system stops without a new column
         if (threadIdx.x == 0) st[r].finished = st[r].done = 1;
         return;
     }
@@ -617,7 +622,7 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
         }
     }
     // (H s)_k over the final columns l < j, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4, lane k.
-    if (lane <= j) {
+    if (lane <= j && !brk) {
         const int k = lane;
         c128 acc = mk(0.0, 0.0);
 #pragma unroll
@@ -632,6 +637,10 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
     }
     __syncthreads();
     for (int k = threadIdx.x; k <= j; k += kThreads) {
+        if (brk) {
+            coln[k] = mk(0.0, 0.0);
+            continue;
+        }
         const c128 hs = (part[0][k] + part[1][k]) + (part[2][k] + part[3][k]);
         coln[k] = (1.0 / beta) * (z[k] - hs);
         Hr[static_cast<int64_t>(j) * ldh + k] = coln[k];
@@ -686,6 +695,15 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
             gj = mk(0.0, 0.0) - sv * gprev;
             history[static_cast<int64_t>(r) * cap + it - 1] = sqrt(abs2(gj)) / ref;
         }
+    }
+    if (brk) {   // converged at iteration it - 1 with the j columns formed; no update pass for iteration it
+        if (lane == 0) {
+            st[r].monitored = 0.f;
+            st[r].steps = j;
+            st[r].iterations = it - 1;
+            st[r].converged = st[r].finished = st[r].done = 1;
+        }
+        return;
     }
     c128 c, sv;
     const c128 diag = givens(cb, nu, c, sv);

@@ -379,16 +379,8 @@ void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, 
     default: { constexpr int E = 16; CALL; } break; \
     }
 
-std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier) {
-    Context *ctx = p->ctx;
-    hipStream_t s = ctx->stream;
-    const GridInfo &g = p->g;
-    auto op = std::make_unique<Product>();
-    op->type = kOperator;
-    op->problem = p;
-    op->precision = precision;
-    op->modifier = modifier;
-    ProdDims &d = op->dims;
+ProdDims operator_dims(const GridInfo &g) {
+    ProdDims d{};
     d.rows_x = g.rows_x;
     d.rows_y = g.rows_y;
     d.cols_x = g.has_x ? g.cols_x : 0;
@@ -401,6 +393,20 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
     d.My = g.cy > 1 ? next_pow2(2L * g.cy - 1) : 1;
     d.Hx = d.Mx / 2 + 1;
     d.Hy = d.My / 2 + 1;
+    return d;
+}
+
+std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier) {
+    Context *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const GridInfo &g = p->g;
+    auto op = std::make_unique<Product>();
+    op->type = kOperator;
+    op->problem = p;
+    op->precision = precision;
+    op->modifier = modifier;
+    op->dims = operator_dims(g);
+    const ProdDims &d = op->dims;
     const uint64_t plane_elems = static_cast<uint64_t>(d.Hx) * d.Hy;
     const uint64_t temp = static_cast<uint64_t>(d.Mx) * d.My * sizeof(c128);
     ctx->require(temp, "The spectra of the product (transform buffer)");

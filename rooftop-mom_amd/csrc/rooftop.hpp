@@ -131,6 +131,9 @@ struct Product {
 std::shared_ptr<Problem> make_problem(Context *ctx, int cx, int cy, double a, double b, double k, int near_order,
                                       int far_order, const uint8_t *mask);
 // product.hip
+// The dimensions of the Green's matrix product of a grid: rooftop rows and columns, the power-of-two embedding
+// periods M >= 2 count - 1 and the folded spectrum extents M / 2 + 1.
+ProdDims operator_dims(const GridInfo &g);
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier);
 // The circulant preconditioner of the problem on its face grid (RT_CIRCULANT_* kernel, RT_BUILDER_* builder).
 std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, int bandwidth_x, int bandwidth_y,

@@ -75,6 +75,9 @@ uint64_t estimate_request_bytes(int cx, int cy, const SolveOptions &o, int preco
 struct Server {
     int device = 0;
     uint64_t budget = 0;
+    // One context per worker slot, opened by rt_server_create on the calling thread (a slot that cannot open its
+    // context fails the creation instead of leaving queued requests without a worker).
+    std::vector<std::unique_ptr<Context>> contexts;
     std::mutex m;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<ServeJob>> queue;
@@ -109,12 +112,11 @@ void Server::solve(Context &ctx, ServeJob &job) {
     job.result = r[0];
 }
 
-void Server::run(int) {
-    Context ctx;
+void Server::run(int slot) {
+    Context &ctx = *contexts[slot];
     try {
-        ctx.open(device, budget);
+        ctx.activate();   // the device of this worker thread
     } catch (const Error &) {
-        return;
     }
     for (;;) {
         std::shared_ptr<ServeJob> job;
@@ -157,15 +159,16 @@ void Server::run(int) {
         cv_done.notify_all();
         cv_work.notify_all();
     }
-    ctx.close();
 }
 
 }  // namespace rt
 
 using namespace rt;
 
+// The server state is shared with the callers blocked in rt_server_wait: rt_server_destroy drops the handle's
+// reference after the workers have ended, and a waiter it woke still holds its own until it returns.
 struct rt_server {
-    Server s;
+    std::shared_ptr<Server> s;
 };
 
 extern "C" {
@@ -226,9 +229,20 @@ int rt_server_create(int32_t device, uint64_t budget, int32_t slots, rt_server *
         size_t free_b = 0, total_b = 0;
         RT_HIP(hipMemGetInfo(&free_b, &total_b));
         auto h = std::make_unique<rt_server>();
-        h->s.device = device;
-        h->s.budget = budget ? budget : static_cast<uint64_t>(free_b) / 10 * 9;
-        for (int i = 0; i < slots; ++i) h->s.workers.emplace_back([srv = &h->s, i] { srv->run(i); });
+        h->s = std::make_shared<Server>();
+        Server &srv = *h->s;
+        srv.device = device;
+        srv.budget = budget ? budget : static_cast<uint64_t>(free_b) / 10 * 9;
+        try {
+            for (int i = 0; i < slots; ++i) {
+                srv.contexts.push_back(std::make_unique<Context>());
+                srv.contexts.back()->open(device, srv.budget);
+            }
+        } catch (...) {
+            for (auto &c : srv.contexts) c->close();
+            throw;
+        }
+        for (int i = 0; i < slots; ++i) srv.workers.emplace_back([p = &srv, i] { p->run(i); });
         *out = h.release();
         set_last_error(RT_OK, "", "");
         return RT_OK;
@@ -240,21 +254,23 @@ int rt_server_create(int32_t device, uint64_t budget, int32_t slots, rt_server *
 
 void rt_server_destroy(rt_server *h) {
     if (!h) return;
+    Server &srv = *h->s;
     {
-        std::lock_guard<std::mutex> lk(h->s.m);
-        h->s.stopping = true;
-        for (auto &job : h->s.queue) {   // never started: their waiters get an error
+        std::lock_guard<std::mutex> lk(srv.m);
+        srv.stopping = true;
+        for (auto &job : srv.queue) {   // never started: their waiters get an error
             job->done = true;
             job->status = RT_EINVAL;
             job->error_id = "rooftop:server";
             job->error_message = "the server was destroyed before the request started";
         }
-        h->s.queue.clear();
+        srv.queue.clear();
     }
-    h->s.cv_work.notify_all();
-    h->s.cv_done.notify_all();
-    for (auto &t : h->s.workers) t.join();
-    delete h;
+    srv.cv_work.notify_all();
+    srv.cv_done.notify_all();
+    for (auto &t : srv.workers) t.join();
+    for (auto &c : srv.contexts) c->close();
+    delete h;   // a waiter still returning keeps the server state alive through its own reference
 }
 
 int rt_server_submit(rt_server *h, const rt_request *r, int64_t *ticket) {
@@ -299,20 +315,21 @@ int rt_server_submit(rt_server *h, const rt_request *r, int64_t *ticket) {
         job->opts = o;
         job->preconditioner = r->preconditioner;
         job->bytes = estimate_request_bytes(job->cx, job->cy, o, job->preconditioner);
+        Server &srv = *h->s;
         {
-            std::lock_guard<std::mutex> lk(h->s.m);
-            if (job->bytes > h->s.budget)
+            std::lock_guard<std::mutex> lk(srv.m);
+            if (job->bytes > srv.budget)
                 fail(RT_ENOMEM, "rooftopMoM:resources:insufficientMemory",
                      "The request needs about %llu bytes, more than the server's budget of %llu.",
-                     static_cast<unsigned long long>(job->bytes), static_cast<unsigned long long>(h->s.budget));
-            job->ticket = h->s.next_ticket++;
+                     static_cast<unsigned long long>(job->bytes), static_cast<unsigned long long>(srv.budget));
+            job->ticket = srv.next_ticket++;
             job->submitted = Clock::now();
-            h->s.jobs[job->ticket] = job;
-            h->s.queue.push_back(job);
-            ++h->s.submitted;
+            srv.jobs[job->ticket] = job;
+            srv.queue.push_back(job);
+            ++srv.submitted;
             *ticket = job->ticket;
         }
-        h->s.cv_work.notify_all();
+        srv.cv_work.notify_all();
         set_last_error(RT_OK, "", "");
         return RT_OK;
     } catch (const Error &e) {
@@ -336,13 +353,14 @@ int rt_server_wait(rt_server *h, int64_t ticket, void *x, int64_t n, rt_output *
     try {
         if (!h) fail(RT_EINVAL, "rooftop:server", "NULL server");
         std::shared_ptr<ServeJob> job;
+        const std::shared_ptr<Server> srv = h->s;   // keeps the state alive while this call waits
         {
-            std::unique_lock<std::mutex> lk(h->s.m);
-            auto it = h->s.jobs.find(ticket);
-            if (it == h->s.jobs.end()) fail(RT_EINVAL, "rooftop:server", "unknown ticket %lld", (long long)ticket);
+            std::unique_lock<std::mutex> lk(srv->m);
+            auto it = srv->jobs.find(ticket);
+            if (it == srv->jobs.end()) fail(RT_EINVAL, "rooftop:server", "unknown ticket %lld", (long long)ticket);
             job = it->second;
-            h->s.cv_done.wait(lk, [&] { return job->done; });
-            h->s.jobs.erase(ticket);
+            srv->cv_done.wait(lk, [&] { return job->done; });
+            srv->jobs.erase(ticket);
         }
         if (job->status != RT_OK) fail(job->status, job->error_id.c_str(), "%s", job->error_message.c_str());
         if (x) {
@@ -379,17 +397,18 @@ int rt_server_wait(rt_server *h, int64_t ticket, void *x, int64_t n, rt_output *
 
 int rt_server_stats(rt_server *h, int64_t *counts, uint64_t *bytes) {
     if (!h) return RT_EINVAL;
-    std::lock_guard<std::mutex> lk(h->s.m);
+    Server &srv = *h->s;
+    std::lock_guard<std::mutex> lk(srv.m);
     if (counts) {
-        counts[0] = h->s.submitted;
-        counts[1] = h->s.completed;
-        counts[2] = static_cast<int64_t>(h->s.queue.size());
-        counts[3] = h->s.running;
-        counts[4] = h->s.peak_running;
+        counts[0] = srv.submitted;
+        counts[1] = srv.completed;
+        counts[2] = static_cast<int64_t>(srv.queue.size());
+        counts[3] = srv.running;
+        counts[4] = srv.peak_running;
     }
     if (bytes) {
-        bytes[0] = h->s.reserved;
-        bytes[1] = h->s.budget;
+        bytes[0] = srv.reserved;
+        bytes[1] = srv.budget;
     }
     return RT_OK;
 }

@@ -32,4 +32,9 @@ def reference_data():
 
 
 def rel(a, b):
-    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+    """norm(a - b) / norm(b); 0 when both vanish, inf when only b does."""
+    diff = float(np.linalg.norm(np.asarray(a) - np.asarray(b)))
+    ref = float(np.linalg.norm(np.asarray(b)))
+    if ref == 0:
+        return 0.0 if diff == 0 else float('inf')
+    return diff / ref
