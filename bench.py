@@ -167,12 +167,15 @@ def run_batch(rt, ctx, op, pc, solver, d_b, d_x, n, count, steps, warmup, allow_
 C0 = 299792458.0
 
 
-def served_request(rt, i, max_side=1024):
+def served_request(rt, i, max_side=1024, side=None):
     """Request i of the served stream (seed 1000 + i): a square grid of side L = round(exp(U[ln 32, ln max_side])),
     frequency U[0.5, 3] GHz, cell lambda / U[10, 20], a 4-connected blob mask grown to a fill U[0.3, 0.9], and a 1 V
-    delta gap at a uniformly drawn existing rooftop (b = V * edge length there)."""
+    delta gap at a uniformly drawn existing rooftop (b = V * edge length there). side, if given, replaces the drawn L
+    (the other draws are unchanged)."""
     rng = np.random.default_rng(1000 + i)
     L = int(round(np.exp(rng.uniform(np.log(32), np.log(max_side)))))
+    if side is not None:
+        L = int(side)
     f = rng.uniform(0.5e9, 3e9)
     a = C0 / f / rng.uniform(10, 20)
     fill = rng.uniform(0.3, 0.9)
