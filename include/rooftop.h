@@ -161,6 +161,18 @@ int rt_product_apply(rt_product *product, const void *input, void *output, int64
 /* The same on device buffers (complex128), enqueued on the context's stream. */
 int rt_product_apply_device(rt_product *product, const void *input, void *output, int64_t n, int64_t nrhs,
                             int32_t precision);
+/* rt_product_apply_device without any device allocation (no counterpart in the reference; for callers that cannot
+ * allocate: code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The matvec passes
+ * run with their working memory in a file-scope device array of the library, 64 MiB per device (held from the first
+ * context on the device until the process ends, outside every context's budget), instead of the product's own work
+ * buffer; the caller passes the device input and output. Limits: the FFT products only (operators and circulant
+ * preconditioners); nrhs times the working bytes of one right-hand side in the requested precision (complex64 half
+ * of complex128; rt_product_memory reports the working bytes of the precision held) within the 64 MiB (RT_EINVAL,
+ * rooftop:product:noAlloc, beyond it); y transforms that fit one block's LDS; and the product applied once by
+ * rt_product_apply_device with the same batch size and precision before (which builds the spectra of its launch
+ * shape). Every call on one device shares that array: calls that overlap on one device race. */
+int rt_product_apply_device_noalloc(rt_product *product, const void *input, void *output, int64_t n, int64_t nrhs,
+                                    int32_t precision);
 /* Product.memory / workingMemory (Product.m:47-57): device bytes of the stored spectra or coefficients, and of the
  * buffers one application uses. */
 int rt_product_memory(rt_product *product, uint64_t *stored, uint64_t *working);

@@ -56,6 +56,7 @@ SIGNATURES = [
     ('rt_product_destroy', None, [P]),
     ('rt_product_apply', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
     ('rt_product_apply_device', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
+    ('rt_product_apply_device_noalloc', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
     ('rt_product_memory', C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ('rt_product_spectra', C.c_int, [P, C.POINTER(C.c_int32), P]),
     ('rt_product_estimate_memory', C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint64),

@@ -189,6 +189,7 @@ int rt_ctx_create(int32_t device, uint64_t budget, rt_ctx **out) {
         need(out != nullptr, RT_EINVAL, "rooftop:ctx", "out is NULL");
         auto c = std::make_unique<rt_ctx>();
         c->c.open(device, budget);
+        noalloc_setup(device);   // the device address of the allocation-free working memory
         *out = c.release();
     });
 }
@@ -424,6 +425,19 @@ int rt_product_apply_device(rt_product *p, const void *in, void *out, int64_t n,
         c->activate();
         p->p->apply(static_cast<const c128 *>(in), static_cast<c128 *>(out), static_cast<int>(nrhs), precision,
                     c->stream);
+    });
+}
+
+int rt_product_apply_device_noalloc(rt_product *p, const void *in, void *out, int64_t n, int64_t nrhs,
+                                    int32_t precision) {
+    return guarded([&] {
+        need(p && in && out, RT_EINVAL, "rooftop:product", "NULL argument");
+        need(n == p->p->dims.n && nrhs >= 1 && nrhs <= 65535, RT_ESIZE, "rooftopMoM:product:sizeMismatch",
+             "The input must be a numeric Nx1 vector (N rooftops) per right-hand side.");
+        Context *c = p->p->problem->ctx;
+        c->activate();
+        product_apply_noalloc(*p->p, static_cast<const c128 *>(in), static_cast<c128 *>(out), static_cast<int>(nrhs),
+                              precision, c->stream);
     });
 }
 

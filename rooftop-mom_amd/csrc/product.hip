@@ -369,6 +369,56 @@ void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, 
     else apply_as<float>(op, in, out, nrhs, s, skip, skip_stride);
 }
 
+// The matvec of nrhs right-hand sides with a caller's working memory (work, work_bytes), allocating nothing: the
+// launch shape the product uses for nrhs, every right-hand side on stream s (no chunks on the side stream, whose
+// results are bitwise those of one chunk), no kernel timing. Products whose pass B needs its full-period spectrum
+// buffer, working memory beyond work_bytes, or pass-B spectra of a plan not built yet are refused.
+template <typename TC>
+static void apply_noalloc_as(Product &op, const c128 *in, c128 *out, int nrhs, hipStream_t s, void *work,
+                             size_t work_bytes) {
+    const PassShape sh = shape_for<TC>(op, nrhs);
+    if (!shape_fits<TC>(op, sh))
+        fail(RT_EINVAL, "rooftop:product", "the FFT columns of this product (%d x %d) exceed the LDS of this device",
+             op.dims.Mx, op.dims.My);
+    if (!b_fits<TC>(op, sh))
+        fail(RT_EINVAL, "rooftop:product:noAlloc",
+             "the %d-point y transforms of this product need the full-period spectrum buffer of the three-launch pass "
+             "B, which the allocation-free application does not have", op.dims.My);
+    const size_t need = static_cast<size_t>(2) * op.dims.ldj * op.dims.Mx * nrhs * sizeof(cplx<TC>);
+    if (need > work_bytes)
+        fail(RT_EINVAL, "rooftop:product:noAlloc",
+             "%d right-hand sides of this product need %zu bytes of working memory, more than the %zu bytes of the "
+             "allocation-free application", nrhs, need, work_bytes);
+    const cplx<TC> *bspec;
+    if (op.four_plane) {
+        bspec = reinterpret_cast<const cplx<TC> *>(sizeof(TC) == sizeof(double)
+                                                       ? static_cast<const void *>(op.b4spec64.ptr)
+                                                       : static_cast<const void *>(op.b4spec32.ptr));
+    } else {
+        const int v = sh.bits >= 4 ? 1 : 0;
+        bspec = reinterpret_cast<const cplx<TC> *>(sizeof(TC) == sizeof(double)
+                                                       ? static_cast<const void *>(op.bspec64[v].ptr)
+                                                       : static_cast<const void *>(op.bspec32[v].ptr));
+    }
+    if (!bspec)
+        fail(RT_EINVAL, "rooftop:product:noAlloc",
+             "the pass-B spectra of this launch shape are not built yet: apply the product once with "
+             "rt_product_apply_device for this many right-hand sides first");
+    cplx<TC> *w = static_cast<cplx<TC> *>(work);
+    const PlanTables &tx = op.tx.of(sh.bits), &ty = op.ty.of(sh.bits);
+    launch_pass_a<TC>(op, tx, in, w, nrhs, sh.group, s, nullptr, 0, nullptr);
+    launch_pass_b<TC>(op, tx, ty, w, bspec, nrhs, sh.tile, sh.teams, s, nullptr, 0, nullptr);
+    launch_pass_c<TC>(op, tx, w, out, nrhs, sh.group, s, nullptr, 0, nullptr);
+}
+
+void operator_apply_noalloc(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, void *work,
+                            size_t work_bytes) {
+    if (prec == RT_MIXED) prec = RT_C128;
+    if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
+    if (prec == RT_C128) apply_noalloc_as<double>(op, in, out, nrhs, s, work, work_bytes);
+    else apply_noalloc_as<float>(op, in, out, nrhs, s, work, work_bytes);
+}
+
 // The spectra kernels are instantiated per largest radix E of their plan.
 #define RT_BY_E(E_VALUE, CALL)                      \
     switch (E_VALUE) {                              \

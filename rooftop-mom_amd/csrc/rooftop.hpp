@@ -140,6 +140,13 @@ std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, i
                                           int modifier, int precision);
 void operator_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
                     int skip_stride);
+// The same with the caller's working memory (work_bytes at work), allocating nothing.
+void operator_apply_noalloc(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, void *work,
+                            size_t work_bytes);
+// noalloc.hip: the allocation-free application with the library's file-scope working memory; noalloc_setup reads
+// that memory's address on the current device once (at context creation).
+void noalloc_setup(int device);
+void product_apply_noalloc(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s);
 void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, double *bytes, hipStream_t s);
 // Times the launch shapes of the passes for nrhs right-hand sides and keeps the fastest for that batch size;
 // shape (may be NULL) receives {bits, group, tile}, ms the summed pass time of one application.
