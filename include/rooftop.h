@@ -152,6 +152,17 @@ int rt_precond_create(rt_problem *problem, int32_t kind, int32_t precision, rt_p
  * RT_ESINGULAR with rooftopMoM:preconditioner:singular when the approximation is singular at some frequency. */
 int rt_precond_create_circulant(rt_problem *problem, int32_t kernel, int32_t builder, int32_t bandwidth_x,
                                 int32_t bandwidth_y, int32_t modifier, int32_t precision, rt_product **out);
+/* The operator forms of rooftopMoM.solver.utils.convertToHandle (src/+rooftopMoM/+solver/+utils/convertToHandle.m:10-16,
+ * used by Solver.solve, Solver.m:146-154) for rt_solve, as operator or preconditioner:
+ *   rt_product_create_dense     a matrix: n x n complex128, column-major (copied to the device; RT_C64 / RT_MIXED keep
+ *                               a complex64 copy for complex64 products);
+ *   rt_product_create_callback  a function handle or an object with apply: apply(user, input, output, n, nrhs) is
+ *                               called on the calling thread of rt_solve / rt_product_apply with host arrays of nrhs
+ *                               column-major vectors (output zeroed before the call), once per product application
+ *                               of the Krylov loop; it computes in its own arithmetic whatever the loop's precision. */
+typedef void (*rt_apply_fn)(void *user, const void *input, void *output, int64_t n, int64_t nrhs);
+int rt_product_create_dense(rt_ctx *ctx, int64_t n, const void *matrix, int32_t precision, rt_product **out);
+int rt_product_create_callback(rt_ctx *ctx, int64_t n, rt_apply_fn apply, void *user, rt_product **out);
 void rt_product_destroy(rt_product *product);
 /* Product.apply (src/+rooftopMoM/Product.m:25-35) on nrhs column-major host vectors of n rooftops each; n must be
  * the rooftop count (RT_ESIZE, rooftopMoM:product:sizeMismatch). Synchronous. precision selects the stored

@@ -175,11 +175,12 @@ class Product:
 
     handle = None
 
-    def __init__(self, problem, handle, precision):
+    def __init__(self, problem, handle, precision, count=None, context=None):
         self._problem = problem
         self.handle = handle
         self.precision = precision
-        self._count = problem.rooftopCount()
+        self._count = problem.rooftopCount() if count is None else int(count)
+        self.context = problem.context if context is None else context
 
     def __del__(self):
         if getattr(self, 'handle', None) is not None and _abi._lib is not None:
@@ -201,7 +202,10 @@ class Product:
         out = np.empty_like(a)
         prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
                                                          else self.precision)
-        check(library().rt_product_apply(self.handle, _ptr(a), _ptr(out), n, nrhs, prec))
+        status = library().rt_product_apply(self.handle, _ptr(a), _ptr(out), n, nrhs, prec)
+        if isinstance(self, CallbackOperator):
+            self.raise_pending()
+        check(status)
         return out
 
     def apply_device(self, x_ptr, y_ptr, nrhs=1, precision=None):
@@ -285,6 +289,75 @@ class Operator(Product):
         by = (C.c_double * 3)()
         check(library().rt_product_profile(self.handle, nrhs, reps, prec, ms, by))
         return list(ms)[:3], list(by), ms[3]
+
+
+class DenseOperator(Product):
+    """A matrix operator on the GPU (the matrix form of convertToHandle, +solver/+utils/convertToHandle.m:10-16):
+    the N x N complex matrix is copied to the device and applied by a batched matrix-vector product."""
+
+    def __init__(self, matrix, precision='c128', context=None):
+        a = np.asarray(matrix)
+        if a.ndim != 2 or a.shape[0] != a.shape[1] or not np.issubdtype(a.dtype, np.number):
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidOperator',
+                               'A matrix operator is a square numeric matrix.')
+        a = np.asfortranarray(a, dtype=np.complex128)
+        context = context or Context.default()
+        h = C.c_void_p()
+        check(library().rt_product_create_dense(context.handle, a.shape[0], _ptr(a), _PRECISIONS[precision],
+                                                C.byref(h)))
+        super().__init__(None, h, _PRECISIONS[precision], count=a.shape[0], context=context)
+
+
+class CallbackOperator(Product):
+    """A function handle or an object with apply as an operator of the GPU solver (convertToHandle.m:10-16): the
+    Krylov loop calls it on the host with every product, one N x 1 vector at a time."""
+
+    def __init__(self, function, count, context=None):
+        fn = function.apply if hasattr(function, 'apply') and not isinstance(function, Product) else function
+        if not callable(fn):
+            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidOperator',
+                               'An operator is a matrix, a function handle or an object with apply.')
+        self._function = fn
+        self.error = None
+        count = int(count)
+
+        def trampoline(user, inp, out, n, nrhs):
+            try:
+                size = int(n) * int(nrhs)
+                x = np.ctypeslib.as_array(C.cast(inp, C.POINTER(C.c_double)), (2 * size,)).view(np.complex128)
+                y = np.ctypeslib.as_array(C.cast(out, C.POINTER(C.c_double)), (2 * size,)).view(np.complex128)
+                x = x.reshape((int(n), int(nrhs)), order='F')
+                y = y.reshape((int(n), int(nrhs)), order='F')
+                for r in range(int(nrhs)):
+                    v = np.asarray(fn(x[:, r].copy()), dtype=np.complex128).reshape(-1)
+                    if v.size != n:
+                        raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
+                                           'The operator returned %d values for %d unknowns.' % (v.size, n))
+                    y[:, r] = v
+            except BaseException as e:   # reported by the call that applied the operator
+                if self.error is None:
+                    self.error = e
+
+        self._trampoline = _abi.APPLY_FN(trampoline)   # kept alive with the product
+        context = context or Context.default()
+        h = C.c_void_p()
+        check(library().rt_product_create_callback(context.handle, count, self._trampoline, None, C.byref(h)))
+        super().__init__(None, h, _abi.RT_MIXED, count=count, context=context)
+
+    def raise_pending(self):
+        if self.error is not None:
+            e, self.error = self.error, None
+            raise e
+
+
+def as_product(operator, count, context=None):
+    """A Product for what the reference's solver accepts as an operator or preconditioner (convertToHandle.m:10-16):
+    a product as is, a matrix (DenseOperator), a function handle or an object with apply (CallbackOperator)."""
+    if operator is None or isinstance(operator, Product):
+        return operator
+    if isinstance(operator, np.ndarray) or (hasattr(operator, 'shape') and hasattr(operator, 'dtype')):
+        return DenseOperator(operator, context=context)
+    return CallbackOperator(operator, count, context=context)
 
 
 _KERNELS = {'strang': 0, 'tChan': 1, 'rChan': 2, 'huckle': 3, 'superoptimal': 4}
@@ -626,9 +699,6 @@ class Solver:
         A matrix right-hand side (N x R) solves its R columns together, in lockstep, and returns one Result per column,
         each equal to the Result of solving that column alone.
         """
-        if not isinstance(operator, Product):
-            raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidOperator',
-                               'An operator is a rooftop_mom_amd product on the GPU.')
         excitation = rightHandSide if isinstance(rightHandSide, Excitation) else None
         b = excitation.vector() if excitation else rightHandSide
         b = np.asarray(b)
@@ -636,12 +706,16 @@ class Solver:
             raise RooftopError(_abi.RT_EINVAL, 'rooftopMoM:solver:invalidRightHandSide',
                                'The right-hand side is a rooftopMoM.Excitation or a numeric vector of finite values.')
         b = np.asfortranarray(b, dtype=np.complex128)
-        n = operator.problem().rooftopCount()
+        # Matrices, function handles and objects with apply become device products (Solver.m:146-154).
+        context = next((q.context for q in (operator, preconditioner) if isinstance(q, Product)), None)
+        operator = as_product(operator, b.shape[0], context)
+        preconditioner = as_product(preconditioner, b.shape[0], operator.context)
+        n = operator._count
         nrhs = 1 if b.ndim == 1 else b.shape[1]
         if b.shape[0] != n:
             raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
                                'The operator does not match the %d unknowns of the right-hand side.' % b.shape[0])
-        if preconditioner is not None and preconditioner.problem().rooftopCount() != n:
+        if preconditioner is not None and preconditioner._count != n:
             raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
                                'The preconditioner does not match the %d unknowns of the right-hand side.' % n)
         x0 = None
@@ -658,9 +732,13 @@ class Solver:
             outs[r].residual_norms_capacity = cap
             outs[r].residual_norms = norms[r].ctypes.data_as(C.POINTER(C.c_double))
         o = self.options()
-        check(library().rt_solve(operator.problem().context.handle, operator.handle,
-                                 preconditioner.handle if preconditioner is not None else None, C.byref(o),
-                                 _ptr(b), n, nrhs, _ptr(x0), _ptr(x), outs))
+        status = library().rt_solve(operator.context.handle, operator.handle,
+                                    preconditioner.handle if preconditioner is not None else None, C.byref(o),
+                                    _ptr(b), n, nrhs, _ptr(x0), _ptr(x), outs)
+        for q in (operator, preconditioner):
+            if isinstance(q, CallbackOperator):
+                q.raise_pending()
+        check(status)
         results = [Result(x if b.ndim == 1 else x[:, r].copy(), bool(out.converged), out.iteration_count,
                           norms[r, :min(cap, out.residual_norms_count)].copy(), out.residual_norm, out.time_s,
                           excitation, out.outer_count) for r, out in enumerate(outs)]
@@ -669,7 +747,7 @@ class Solver:
     def solveDevice(self, operator, b, preconditioner=None, x=None):
         """Solves the columns of the DeviceBatch b together with inputs and outputs in HBM (rt_solve_device); returns
         (x as a DeviceBatch, [Result without a host solution] per column)."""
-        n = operator.problem().rooftopCount()
+        n = operator._count
         if b.n != n:
             raise RooftopError(_abi.RT_ESIZE, 'rooftopMoM:solver:sizeMismatch',
                                'The operator does not match the %d unknowns of the right-hand side.' % b.n)

@@ -33,6 +33,9 @@ class rt_request(C.Structure):
                 ('opts', rt_solve_opts)]
 
 
+# rt_apply_fn: apply(user, input, output, n, nrhs) on host arrays (rt_product_create_callback).
+APPLY_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64)
+
 # Every exported function: (name, restype, argtypes). The CPU test suite checks this list against include/rooftop.h.
 P = C.c_void_p
 SIGNATURES = [
@@ -53,6 +56,8 @@ SIGNATURES = [
     ('rt_problem_set_generators', C.c_int, [P, P, P, P]),
     ('rt_product_create', C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(P)]),
     ('rt_precond_create', C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(P)]),
+    ('rt_product_create_dense', C.c_int, [P, C.c_int64, P, C.c_int32, C.POINTER(P)]),
+    ('rt_product_create_callback', C.c_int, [P, C.c_int64, APPLY_FN, P, C.POINTER(P)]),
     ('rt_product_destroy', None, [P]),
     ('rt_product_apply', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),
     ('rt_product_apply_device', C.c_int, [P, P, P, C.c_int64, C.c_int64, C.c_int32]),

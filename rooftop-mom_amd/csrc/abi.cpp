@@ -149,8 +149,7 @@ std::vector<SolveResult> solve_device(rt_ctx *ctx, rt_product *op, rt_product *p
                                       const c128 *rhs, int64_t n, int64_t nrhs, const c128 *x0, c128 *x) {
     need(ctx && op && op->p, RT_EINVAL, "rooftopMoM:solver:invalidOperator",
          "An operator is a product created by rt_product_create.");
-    need(op->p->type == kOperator || op->p->type == kJacobi || op->p->type == kNearField, RT_EINVAL,
-         "rooftopMoM:solver:invalidOperator", "An operator is a product.");
+
     need(n == op->p->dims.n, RT_ESIZE, "rooftopMoM:solver:sizeMismatch",
          "The operator does not match the unknowns of the right-hand side.");
     need(!pc || (pc->p && pc->p->dims.n == n), RT_ESIZE, "rooftopMoM:solver:sizeMismatch",
@@ -173,11 +172,10 @@ void fill_outputs(const std::vector<SolveResult> &r, rt_output *out) {
     for (size_t k = 0; k < r.size(); ++k) fill_output(r[k], out + k);
 }
 
-void check_finite(const c128 *v, int64_t count) {
+void check_finite(const c128 *v, int64_t count, const char *id = "rooftopMoM:solver:invalidRightHandSide",
+                  const char *msg = "The right-hand side is a numeric vector of finite values.") {
     for (int64_t i = 0; i < count; ++i)
-        if (!std::isfinite(v[i].x) || !std::isfinite(v[i].y))
-            fail(RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide",
-                 "The right-hand side is a numeric vector of finite values.");
+        if (!std::isfinite(v[i].x) || !std::isfinite(v[i].y)) fail(RT_EINVAL, id, "%s", msg);
 }
 
 }  // namespace
@@ -410,6 +408,33 @@ int rt_precond_create(rt_problem *p, int32_t kind, int32_t precision, rt_product
     });
 }
 
+int rt_product_create_dense(rt_ctx *ctx, int64_t n, const void *matrix, int32_t precision, rt_product **out) {
+    return guarded([&] {
+        need(ctx && matrix && out, RT_EINVAL, "rooftopMoM:solver:invalidOperator", "NULL argument");
+        need(n >= 1 && n <= (int64_t(1) << 20), RT_ESIZE, "rooftopMoM:solver:sizeMismatch",
+             "A dense operator is a square matrix of 1 .. 2^20 rows.");
+        need(precision == RT_C128 || precision == RT_C64 || precision == RT_MIXED, RT_EINVAL,
+             "rooftop:product:invalidPrecision", "The precision is RT_C128, RT_C64 or RT_MIXED.");
+        check_finite(static_cast<const c128 *>(matrix), n * n, "rooftopMoM:solver:invalidOperator",
+                     "The operator matrix holds finite values.");
+        ctx->c.activate();
+        auto h = std::make_unique<rt_product>();
+        h->p = create_dense(&ctx->c, n, static_cast<const c128 *>(matrix), precision, h->owner);
+        *out = h.release();
+    });
+}
+
+int rt_product_create_callback(rt_ctx *ctx, int64_t n, rt_apply_fn apply, void *user, rt_product **out) {
+    return guarded([&] {
+        need(ctx && apply && out, RT_EINVAL, "rooftopMoM:solver:invalidOperator", "NULL argument");
+        need(n >= 1, RT_ESIZE, "rooftopMoM:solver:sizeMismatch", "An operator has at least one unknown.");
+        ctx->c.activate();
+        auto h = std::make_unique<rt_product>();
+        h->p = create_callback(&ctx->c, n, apply, user, h->owner);
+        *out = h.release();
+    });
+}
+
 void rt_product_destroy(rt_product *p) {
     if (!p) return;
     if (p->p && p->p->problem) (void)hipStreamSynchronize(p->p->problem->ctx->stream);
@@ -506,6 +531,8 @@ int rt_product_profile(rt_product *p, int64_t nrhs, int32_t reps, int32_t precis
     return guarded([&] {
         need(p && ms && bytes && reps >= 1 && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product",
              "invalid profile arguments");
+        need(p->p->type == kOperator || p->p->type == kCirculant, RT_EINVAL, "rooftop:product",
+             "Only the FFT products have launch shapes to profile or tune.");
         Context *c = p->p->problem->ctx;
         c->activate();
         operator_profile(*p->p, static_cast<int>(nrhs), reps, precision, ms, bytes, c->stream);
@@ -515,6 +542,8 @@ int rt_product_profile(rt_product *p, int64_t nrhs, int32_t reps, int32_t precis
 int rt_product_tune(rt_product *p, int64_t nrhs, int32_t precision, int32_t *shape, double *ms) {
     return guarded([&] {
         need(p && p->p && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product", "invalid tuning arguments");
+        need(p->p->type == kOperator || p->p->type == kCirculant, RT_EINVAL, "rooftop:product",
+             "Only the FFT products have launch shapes to profile or tune.");
         Context *c = p->p->problem->ctx;
         c->activate();
         int sh[5] = {0, 0, 0, 0, 0};
@@ -529,6 +558,8 @@ int rt_product_tune(rt_product *p, int64_t nrhs, int32_t precision, int32_t *sha
 int rt_product_set_shape(rt_product *p, int64_t nrhs, int32_t precision, const int32_t *shape) {
     return guarded([&] {
         need(p && p->p && shape && nrhs >= 1 && nrhs <= 65535, RT_EINVAL, "rooftop:product", "invalid shape arguments");
+        need(p->p->type == kOperator || p->p->type == kCirculant, RT_EINVAL, "rooftop:product",
+             "Only the FFT products have launch shapes to profile or tune.");
         Context *c = p->p->problem->ctx;
         c->activate();
         const int sh[5] = {shape[0], shape[1], shape[2], shape[3], shape[4]};

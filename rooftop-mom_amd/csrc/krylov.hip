@@ -56,8 +56,13 @@ enum ScalSlot {
 // Pool slots of the context's solver buffers.
 enum PoolSlot {
     kSlotPartial, kSlotScal, kSlotState, kSlotHistory, kSlotV, kSlotW, kSlotT, kSlotR, kSlotZ, kSlotD, kSlotAd,
-    kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags, kSlotP, kSlotHraw
+    kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags, kSlotP, kSlotHraw,
+    kSlotPartialL, kSlotHc, kSlotY
 };
+
+// Restarts up to this length run the delayed-reorthogonalization kernels, whose Gram-Schmidt coefficients, rotations
+// and triangular solve fit one wave; longer ones (up to the iteration limit, restart = Inf) the CGS2 kernels below.
+constexpr int kFastRestart = 62;
 
 // ------------------------------------------------------------------------------------------------------------------
 // Reductions.
@@ -911,6 +916,190 @@ __global__ void k_gemv_add(const VT *__restrict__ V, const LoopState *st, const 
 }
 
 // ------------------------------------------------------------------------------------------------------------------
+// GMRES with restarts longer than kFastRestart (up to the iteration limit, restart = Inf): the reference's classical
+// Gram-Schmidt run twice (orthogonalize_.m:15-23) on an FP64 basis V[k][r][n] of normalized columns. A step forms
+// w = M A v_j, then twice h = V^H w (a dot pass over the j + 1 live columns and its reduction) and w -= V h, the norm
+// of w as the subdiagonal, the Hessenberg column with all earlier rotations (one thread, in order), its own rotation
+// and the stopping test, and v_{j+1} = w / norm. Slower per step than the delayed-reorthogonalization kernels (three
+// passes over the basis and three reductions), but with no bound on the restart but memory.
+// ------------------------------------------------------------------------------------------------------------------
+constexpr int kCgsCols = 16;   // columns per wave of the dot pass (a block: 64 columns, blockIdx.z the column group)
+
+// partial[(r * ldp + c) * gridDim.x + b] = sum over block b's rows of conj(V_c) w, c < cnt. Every wave runs over all
+// rows of the block for its 16 columns; the sums are one wave's DPP reduction per column, in a fixed order.
+__global__ void __launch_bounds__(kThreads) k_cgs_dots(const c128 *__restrict__ V, int cnt, const c128 *__restrict__ w,
+                                                       int64_t n, int64_t chunk, c128 *__restrict__ partial, int ldp,
+                                                       const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int R = gridDim.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = blockIdx.z * 64 + wave * kCgsCols;
+    if (c0 >= cnt) return;   // uniform over the wave; no barrier follows
+    double re[kCgsCols], im[kCgsCols];
+#pragma unroll
+    for (int u = 0; u < kCgsCols; ++u) re[u] = im[u] = 0;
+    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    const c128 *wr = w + r * n;
+    for (int64_t i = base + lane; i < end; i += 64) {
+        const c128 wv = wr[i];
+        c128 v[kCgsCols];
+#pragma unroll
+        for (int u = 0; u < kCgsCols; ++u) v[u] = basis_col(V, min(c0 + u, cnt - 1), r, R, n)[i];
+#pragma unroll
+        for (int u = 0; u < kCgsCols; ++u) {
+            re[u] += v[u].x * wv.x + v[u].y * wv.y;
+            im[u] += v[u].x * wv.y - v[u].y * wv.x;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kCgsCols; ++u) {
+        const double sr = wave_sum(re[u]), si = wave_sum(im[u]);
+        if (lane == 0 && c0 + u < cnt)
+            partial[(static_cast<int64_t>(r) * ldp + c0 + u) * gridDim.x + blockIdx.x] = mk(sr, si);
+    }
+}
+
+// hc[r][c] = sum over the blocks of partial slot c (a wave per slot); the first pass stores it as column j of the
+// Hessenberg matrix as formed (Hraw), the second adds it. Grid (ceil(cnt / 4), R).
+__global__ void __launch_bounds__(kThreads) k_cgs_reduce(const c128 *__restrict__ partial, int blocks, int ldp, int cnt,
+                                                         c128 *__restrict__ hc, c128 *__restrict__ Hraw, int m, int j,
+                                                         int first, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (c >= cnt) return;
+    const c128 *row = partial + (static_cast<int64_t>(r) * ldp + c) * blocks;
+    double re = 0, im = 0;
+    for (int b = lane; b < blocks; b += 64) {
+        re += row[b].x;
+        im += row[b].y;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if (lane != 0) return;
+    const c128 h = mk(re, im);
+    hc[static_cast<int64_t>(r) * (m + 1) + c] = h;
+    c128 *col = Hraw + (static_cast<int64_t>(r) * m + j) * (m + 1);
+    col[c] = first ? h : col[c] + h;
+}
+
+// y[i] += sign * sum_k V_k[i] h[k] over k < cnt (cnt < 0: the system's steps), h staged in LDS 256 at a time.
+// skip_done: return for stopped systems (the Gram-Schmidt updates; the solution update at a cycle's end runs for all).
+__global__ void __launch_bounds__(kThreads) k_cgs_gemv(const c128 *__restrict__ V, int cnt, const c128 *__restrict__ h,
+                                                       int ldh, double sign, c128 *__restrict__ y, int64_t n,
+                                                       int skip_done, const LoopState *__restrict__ st) {
+    const int r = blockIdx.y, R = gridDim.y;
+    if (skip_done && st[r].done) return;
+    if (cnt < 0) cnt = st[r].steps;
+    if (cnt == 0) return;
+    __shared__ c128 hs[kThreads];
+    const c128 *hr = h + static_cast<int64_t>(r) * ldh;
+    c128 *yr = y + r * n;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * kThreads; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        c128 acc = mk(0.0, 0.0);
+        for (int k0 = 0; k0 < cnt; k0 += kThreads) {
+            __syncthreads();
+            if (k0 + static_cast<int>(threadIdx.x) < cnt) hs[threadIdx.x] = hr[k0 + threadIdx.x];
+            __syncthreads();
+            const int kn = min(kThreads, cnt - k0);
+            if (i < n)
+                for (int k = 0; k < kn; ++k) acc = acc + hs[k] * basis_col(V, k0 + k, r, R, n)[i];
+        }
+        if (i < n) yr[i] = yr[i] + sign * acc;
+    }
+}
+
+// Step j of the cycle, iteration `it` overall: the Hessenberg column j as formed (Hraw, rows <= j) and the subdiagonal
+// norm (scal kRnorm2 slot) rotated by the earlier rotations in order, its own rotation (generateRotation_.m:11-18), the
+// rotated right-hand side g, the monitored residual |g(j+1)| / reference and the stopping test (iterate_.m:51-66);
+// 1 / norm for the next basis vector. One thread per system. Grid (1, R).
+constexpr int kSubSlot = 146;   // scal slot of the subdiagonal norm of the step
+__global__ void k_cgs_finalize(const c128 *__restrict__ Hraw, c128 *__restrict__ Hall, c128 *__restrict__ gall,
+                               c128 *__restrict__ scal, int m, int j, int it, LoopState *st, double *history,
+                               int cap) {
+    SYSTEM(r);
+    if (threadIdx.x != 0) return;
+    const int ldh = m + 1;
+    c128 *sc = scal + r * kScal;
+    const c128 *raw = Hraw + (static_cast<int64_t>(r) * m + j) * ldh;
+    c128 *col = Hall + (static_cast<int64_t>(r) * m + j) * ldh;
+    c128 *g = gall + static_cast<int64_t>(r) * 4 * ldh, *cs = g + ldh, *sn = g + 2 * ldh;
+    const double sub = sc[kSubSlot].x;
+    for (int k = 0; k <= j; ++k) col[k] = raw[k];
+    for (int k = 0; k < j; ++k) {   // [conj(c) conj(s); -s c] on rows k, k + 1
+        const c128 a = col[k], b = col[k + 1], c = cs[k], s = sn[k];
+        col[k] = cmulc(c, a) + cmulc(s, b);
+        col[k + 1] = c * b - s * a;
+    }
+    c128 c, sv;
+    col[j] = givens(col[j], sub, c, sv);
+    col[j + 1] = mk(0.0, 0.0);
+    cs[j] = c;
+    sn[j] = sv;
+    const c128 gj = g[j];
+    g[j + 1] = mk(0.0, 0.0) - sv * gj;
+    g[j] = cmulc(c, gj);
+    const double res = sqrt(abs2(g[j + 1])) / sc[kReference].x;
+    history[static_cast<int64_t>(r) * cap + it] = res;
+    st[r].monitored = static_cast<float>(res);
+    st[r].steps = j + 1;
+    st[r].iterations = it;
+    st[r].tag = it;
+    const bool conv = res < sc[kTol].x;
+    if (conv) {
+        st[r].converged = st[r].finished = st[r].done = 1;
+    } else if (!isfinite(res)) {
+        st[r].finished = st[r].done = 1;
+    } else if (it >= static_cast<int>(sc[kMaxIt].x)) {
+        st[r].done = 1;
+    }
+    // A zero subdiagonal means the basis spans the solution: the residual above is zero and the system converged.
+    sc[kInvSub] = (!conv && sub > 0) ? mk(1.0 / sub, 0.0) : mk(0.0, 0.0);
+}
+
+// v_{j+1} = w / norm for the systems still running after step j.
+__global__ void k_cgs_next(const c128 *__restrict__ w, c128 *__restrict__ V, int j1, int64_t n,
+                           const c128 *__restrict__ scal, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    const int R = gridDim.y;
+    const double inv = scal[r * kScal + kInvSub].x;
+    const c128 *wr = w + r * n;
+    c128 *vr = V + (static_cast<int64_t>(j1) * R + r) * n;
+    ROWS(i, n) vr[i] = inv * wr[i];
+}
+
+// y = H(0:steps, 0:steps) \ g(0:steps), any number of steps: back substitution, one block per system, the right-hand
+// side in LDS (dynamic, steps entries), y into Y[r][m]. Grid (1, R).
+__global__ void __launch_bounds__(kThreads) k_cgs_solve(const c128 *__restrict__ Hall, const c128 *__restrict__ gall,
+                                                        int m, const LoopState *st, c128 *__restrict__ Y) {
+    extern __shared__ unsigned char solve_raw[];
+    c128 *acc = reinterpret_cast<c128 *>(solve_raw);
+    __shared__ c128 yk_s;
+    const int r = blockIdx.y;
+    const int steps = st[r].steps;
+    const int ldh = m + 1;
+    const c128 *H = Hall + static_cast<int64_t>(r) * ldh * m;
+    const c128 *g = gall + static_cast<int64_t>(r) * 4 * ldh;
+    c128 *y = Y + static_cast<int64_t>(r) * m;
+    for (int k = threadIdx.x; k < steps; k += kThreads) acc[k] = g[k];
+    __syncthreads();
+    for (int k = steps - 1; k >= 0; --k) {
+        if (threadIdx.x == 0) {
+            const c128 d = H[static_cast<int64_t>(k) * ldh + k];
+            const c128 yk = (1.0 / abs2(d)) * cmulc(d, acc[k]);
+            yk_s = yk;
+            y[k] = yk;
+        }
+        __syncthreads();
+        const c128 yk = yk_s;
+        for (int i = threadIdx.x; i < k; i += kThreads) acc[i] = acc[i] - H[static_cast<int64_t>(k) * ldh + i] * yk;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------------
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
 static unsigned vec_blocks(int64_t n) { return blocks_for(n, 256, 1024); }
@@ -1201,6 +1390,124 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
 }
 
 // ------------------------------------------------------------------------------------------------------------------
+// GMRES(m) with m > kFastRestart, left preconditioned, CGS2 exactly as the reference orthogonalizes
+// (iterate_.m:1-85, orthogonalize_.m:15-23), on an FP64 basis of m + 1 columns.
+// ------------------------------------------------------------------------------------------------------------------
+constexpr int kMaxRestart = 8192;   // the right-hand side of the cycle's triangular solve stays in one block's LDS
+
+static InnerResult gmres_cgs(Workspace &ws, const LinOp &A, const LinOp *P, const c128 *b, c128 *x, bool has_x0,
+                             const std::vector<double> &tol, const std::vector<char> &skip_sys, int max_iter, int m) {
+    if (m > kMaxRestart)
+        fail(RT_EINVAL, "rooftopMoM:solver:invalidRestart",
+             "A GMRES cycle of %d steps exceeds the %d this GPU solver supports; set a smaller restart.", m,
+             kMaxRestart);
+    Context *ctx = ws.ctx;
+    const int64_t n = ws.n;
+    const int R = ws.R;
+    const LoopState *st = ws.state;
+    const int m1 = m + 1;
+    c128 *V = vec(ctx, kSlotV, static_cast<int64_t>(R) * m1 * n);
+    c128 *w = vec(ctx, kSlotW, R * n), *t = vec(ctx, kSlotT, R * n);
+    c128 *H = vec(ctx, kSlotH, static_cast<int64_t>(R) * m1 * m);
+    c128 *Hr = vec(ctx, kSlotHraw, static_cast<int64_t>(R) * m1 * m);
+    c128 *g = vec(ctx, kSlotG, static_cast<int64_t>(R) * 4 * m1);
+    c128 *hc = vec(ctx, kSlotHc, static_cast<int64_t>(R) * m1);
+    c128 *Y = vec(ctx, kSlotY, static_cast<int64_t>(R) * m);
+    c128 *partial = vec(ctx, kSlotPartialL, static_cast<int64_t>(R) * m1 * ws.blocks);
+    const size_t solve_lds = static_cast<size_t>(m) * sizeof(c128);
+    if (solve_lds > 48 * 1024) raise_lds(reinterpret_cast<const void *>(&k_cgs_solve), solve_lds);
+    auto precondition = [&](const c128 *in, c128 *out) {
+        if (P) (*P)(in, out, ws.skip());
+        else ws.copy(in, out, st);
+    };
+    ws.reset_state(skip_sys);
+    ws.put_re(kTol, tol);
+    precondition(b, w);   // reference = norm(P b)
+    ws.nrm2(w, kReference, st);
+    if (has_x0) {
+        A(x, t, ws.skip());
+        ws.sub(b, t, t, st);
+        precondition(t, w);
+        ws.nrm2(w, kRnorm, st);
+    } else {
+        ws.zero(x);
+        hipLaunchKernelGGL(k_copy_slot, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.scal, kReference, kRnorm, R);
+        RT_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnorm, kReference,
+                       R);
+    RT_LAUNCH_CHECK();
+    const std::vector<double> r0 = ws.fetch_re(kRnorm), ref = ws.fetch_re(kReference);
+    std::vector<LoopState> state = ws.read_state();
+    auto any_active = [&]() {
+        for (const auto &q : state)
+            if (!q.finished) return true;
+        return false;
+    };
+    PollPlan plan;
+    int it = 0;
+    while (it < max_iter && any_active()) {
+        hipLaunchKernelGGL(k_cycle_reset, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, R);
+        RT_LAUNCH_CHECK();
+        // v_0 = w / |w| into column 0 of the basis, g = (|w|, 0, ...).
+        hipLaunchKernelGGL(k_gmres_start, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, V, m1, n, ws.scal, g, st);
+        RT_LAUNCH_CHECK();
+        int step = 0;
+        while (step < m && it < max_iter) {
+            const int window = std::min({plan.next(state, tol), m - step, max_iter - it});
+            for (int k = 0; k < window; ++k) {
+                const int j = step;
+                ++step;
+                ++it;
+                A(V + static_cast<int64_t>(j) * R * n, t, ws.skip());
+                precondition(t, w);
+                const int cnt = j + 1;
+                for (int pass = 0; pass < 2; ++pass) {   // classical Gram-Schmidt, twice
+                    hipLaunchKernelGGL(k_cgs_dots, dim3(ws.blocks, R, (cnt + 63) / 64), dim3(kThreads), 0, ws.s, V,
+                                       cnt, w, n, ws.chunk, partial, m1, st);
+                    RT_LAUNCH_CHECK();
+                    hipLaunchKernelGGL(k_cgs_reduce, dim3((cnt + 3) / 4, R), dim3(kThreads), 0, ws.s, partial,
+                                       ws.blocks, m1, cnt, hc, Hr, m, j, pass == 0 ? 1 : 0, st);
+                    RT_LAUNCH_CHECK();
+                    hipLaunchKernelGGL(k_cgs_gemv, dim3(vec_blocks(n), R), dim3(kThreads), 0, ws.s, V, cnt, hc, m1,
+                                       -1.0, w, n, 1, st);
+                    RT_LAUNCH_CHECK();
+                }
+                ws.nrm2(w, kSubSlot, st);
+                hipLaunchKernelGGL(k_cgs_finalize, dim3(1, R), dim3(64), 0, ws.s, Hr, H, g, ws.scal, m, j, it,
+                                   ws.state, ws.history, ws.cap);
+                RT_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_cgs_next, dim3(vec_blocks(n), R), dim3(256), 0, ws.s, w, V, j + 1, n, ws.scal,
+                                   st);
+                RT_LAUNCH_CHECK();
+            }
+            state = ws.read_state();
+            ws.set_scale(state);
+            if (all_done(state)) break;
+        }
+        // x += V y with y = H \ g over the steps of this cycle (systems idle this cycle have 0 steps).
+        hipLaunchKernelGGL(k_cgs_solve, dim3(1, R), dim3(kThreads), static_cast<uint32_t>(solve_lds), ws.s, H, g, m,
+                           ws.state, Y);
+        RT_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_cgs_gemv, dim3(vec_blocks(n), R), dim3(kThreads), 0, ws.s, V, -1, Y, m, 1.0, x, n, 0,
+                           ws.state);
+        RT_LAUNCH_CHECK();
+        state = ws.read_state();
+        if (it >= max_iter || !any_active()) break;
+        // The next cycle starts from the true preconditioned residual (finished systems are skipped).
+        hipLaunchKernelGGL(k_cycle_reset, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, R);
+        RT_LAUNCH_CHECK();
+        A(x, t, ws.skip());
+        ws.sub(b, t, t, st);
+        precondition(t, w);
+        ws.nrm2(w, kRnorm, st);
+    }
+    RT_HIP(hipStreamSynchronize(ws.s));
+    ws.ctx->timed_scale = 1.0;
+    return collect(ws, state, skip_sys, r0, ref);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
 // GMRES(m), left preconditioned, CGS2 (iterate_.m:1-85), on the systems not marked `skip_sys`.
 // ------------------------------------------------------------------------------------------------------------------
 template <typename VT>
@@ -1216,7 +1523,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
     int64_t m64 = restart > 0 ? restart : n;
     m64 = std::min<int64_t>({m64, n, static_cast<int64_t>(max_iter)});
     const int m = static_cast<int>(m64);
-    if (m > 62) fail(RT_EINVAL, "rooftopMoM:solver:invalidRestart", "restarts above 62 are not supported on the GPU");
+    if (m > kFastRestart) return gmres_cgs(ws, A, P, b, x, has_x0, tol, skip_sys, max_iter, m);
     const int m1 = m + 1;
     VT *V = static_cast<VT *>(ctx->scratch(kSlotV, sizeof(VT) * R * m * n, "the solver's vectors"));
     c128 *pv = vec(ctx, kSlotP, R * n);   // the pending vector p (FP64), the one the operator is applied to

@@ -1058,7 +1058,7 @@ void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
 uint64_t Product::stored_bytes() const {
     return planes64.bytes() + bspec64[0].bytes() + bspec64[1].bytes() + bspec32[0].bytes() + bspec32[1].bytes() +
            b4spec64.bytes() + b4spec32.bytes() + diag64.bytes() + diag32.bytes() + coef64.bytes() + coef32.bytes() + cls.bytes() + scls.bytes() +
-           scoef64.bytes() + scoef32.bytes();
+           scoef64.bytes() + scoef32.bytes() + dense64.bytes() + dense32.bytes();
 }
 
 uint64_t Product::working_bytes() const {
@@ -1070,6 +1070,8 @@ uint64_t Product::working_bytes() const {
 void Product::apply(const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip, int skip_stride) {
     if (type == kOperator || type == kCirculant)
         operator_apply(*this, in, out, nrhs, prec, s, skip, skip_stride);
+    else if (type == kDense || type == kCallback)
+        dense_apply(*this, in, out, nrhs, prec, s, skip, skip_stride);
     else
         preconditioner_apply(*this, in, out, nrhs, prec, s, skip, skip_stride);
 }

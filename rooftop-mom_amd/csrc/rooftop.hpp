@@ -76,7 +76,7 @@ struct PassShape {
     int chunks = 1;  // right-hand-side chunks, alternating between the main and the side stream
 };
 
-enum ProductType { kOperator = 0, kJacobi = 1, kNearField = 2, kCirculant = 3 };
+enum ProductType { kOperator = 0, kJacobi = 1, kNearField = 2, kCirculant = 3, kDense = 4, kCallback = 5 };
 
 struct Product {
     int type = kOperator;
@@ -117,7 +117,20 @@ struct Product {
     DeviceArray<c128> scoef64;                // near field: [stencil classes][18] symmetrized coefficients
     DeviceArray<c64> scoef32;
     int stencil_count = 0;
+    // A dense operator (n x n, column-major) and a host callback (convertToHandle.m:10-16).
+    DeviceArray<c128> dense64;
+    DeviceArray<c64> dense32;
+    rt_apply_fn callback = nullptr;
+    void *callback_user = nullptr;
+    void *staging = nullptr;   // pinned host copies of a callback's input and output
+    size_t staging_bytes = 0;
 
+    Product() = default;
+    Product(const Product &) = delete;
+    Product &operator=(const Product &) = delete;
+    ~Product() {
+        if (staging) (void)hipHostFree(staging);
+    }
     bool has(int prec) const { return precision == RT_MIXED || precision == prec; }
     // skip: optional device flags, one per right-hand side r at skip[r * skip_stride] (stride 0: one flag for all);
     // the kernels of a right-hand side whose flag is non-zero at run time return at once.
@@ -155,6 +168,14 @@ void operator_tune(Product &op, int nrhs, int prec, int *shape, double *ms, hipS
 double bandwidth_probe(Context *ctx, uint64_t bytes, int reps);
 // Sets the shape {bits, group, tile} used for nrhs right-hand sides (a saved tuning result).
 void operator_set_shape(Product &op, int nrhs, int prec, const int *shape);
+// dense.hip: a dense n x n operator from a host matrix (complex128, column-major), a host-callback operator, and
+// their application; owner receives the product's bare problem (the context and n).
+std::unique_ptr<Product> create_dense(Context *ctx, int64_t n, const c128 *host_matrix, int precision,
+                                      std::shared_ptr<Problem> &owner);
+std::unique_ptr<Product> create_callback(Context *ctx, int64_t n, rt_apply_fn fn, void *user,
+                                         std::shared_ptr<Problem> &owner);
+void dense_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
+                 int skip_stride);
 // nearfield.hip
 std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precision);
 void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s,

@@ -63,11 +63,14 @@ uint64_t estimate_request_bytes(int cx, int cy, const SolveOptions &o, int preco
     b += (Mx + My) * 4 * 64;                                      // FFT tables
     if (preconditioner == RT_PRECOND_NEARFIELD) b += 4 * n + 4096 * 18 * 24;
     if (preconditioner == RT_PRECOND_JACOBI) b += 24 * n;
-    uint64_t vectors = o.method == RT_METHOD_GMRES ? static_cast<uint64_t>(std::min<int64_t>(o.restart > 0 ? o.restart
-                                                                                                   : 62, 62) + 1) + 6
-                                                   : 8;
+    // GMRES: a basis of m + 1 columns (m = min(restart or the iteration limit, n, the iteration limit)), the pending
+    // and work vectors, the Hessenberg matrices and the reduction partials; COCG: its vectors.
+    const uint64_t maxit = static_cast<uint64_t>(std::max(o.max_iterations, 1));
+    const uint64_t m = std::min<uint64_t>({o.restart > 0 ? static_cast<uint64_t>(o.restart) : maxit, n, maxit});
+    uint64_t vectors = o.method == RT_METHOD_GMRES ? m + 1 + 6 : 8;
     if (o.refine) vectors += 3;
     b += (vectors + 2) * n * 16;                                  // Krylov vectors, right-hand side, solution
+    if (o.method == RT_METHOD_GMRES) b += (2 * m * (m + 1) + 6 * (m + 1)) * 16 + (m + 1) * 2048 * 16;
     const uint64_t spectra_transform = Mx * My * 16;              // freed before the solve starts
     return b + spectra_transform + (b >> 3) + (16u << 20);
 }
@@ -343,6 +346,7 @@ int rt_request_bytes(const rt_request *r, uint64_t *bytes) {
     SolveOptions o;
     o.method = r->opts.method;
     o.restart = r->opts.restart;
+    o.max_iterations = r->opts.max_iterations > 0 ? r->opts.max_iterations : 1000;
     o.precision = r->opts.precision;
     o.refine = r->opts.refine;
     *bytes = estimate_request_bytes(r->count_x, r->count_y, o, r->preconditioner);

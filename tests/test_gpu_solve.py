@@ -389,3 +389,40 @@ def test_circulant_preconditioner_errors_and_solve(rt):
             np.testing.assert_allclose(res.residualNorms()[:k][keep], ref['residualNorms'][:k][keep], rtol=1e-6)
             assert abs(res.iterationCount() - ref['iterationCount']) <= 1
     assert pc.kind == 'nearField' and circ.kind == 'circulant'
+
+
+@pytest.mark.parametrize('restart', [100, np.inf])
+def test_gmres_long_restarts_match_oracle(rt, restart):
+    """Solver('gmres', 'restart', 100 or Inf) (Solver.m:31; Inf is min(N, maxIterationCount),
+    GeneralizedMinimumResidual.m:15-23, iterate_.m:7): cycles longer than 62 steps run the CGS2 kernels. Without a
+    preconditioner (208 and 134 iterations) the monitored norms follow the oracle's while they are large and the
+    counts agree to rounding; with the near field the count is the oracle's exactly; a batch of three equals its
+    single solves."""
+    p, grid, gen = _setup(rt, 16, 14, 0.1, 0.1, (6, 4))
+    Z = S.dense_matrix(gen, grid)
+    P = NF.NearField(gen, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0.3, 0.4, -np.sqrt(0.75)], [0.8, -0.6, 0], 1.0, order=4)
+    op = rt.Operator(p)
+    pc = rt.Preconditioner(p, 'nearField')
+    solver = rt.Solver('gmres', tolerance=1e-8, restart=restart)
+    print('plain, restart', restart, flush=True)
+    res = solver.solve(op, b)
+    ref = SV.solve('gmres', Z, b, tol=1e-8, max_iter=1000, restart=restart)
+    assert res.converged() and rel(Z @ res.solution(), b) < 1e-8
+    k = min(res.residualNorms().size, ref['residualNorms'].size)
+    keep = ref['residualNorms'][:k] > 1e-3
+    np.testing.assert_allclose(res.residualNorms()[:k][keep], ref['residualNorms'][:k][keep], rtol=1e-6)
+    assert abs(res.iterationCount() - ref['iterationCount']) <= max(2, ref['iterationCount'] // 20)
+    print('near field, restart', restart, flush=True)
+    res = solver.solve(op, b, preconditioner=pc)
+    ref = SV.solve('gmres', Z, b, tol=1e-8, max_iter=1000, restart=restart, preconditioner=P.apply)
+    assert res.converged() and res.iterationCount() == ref['iterationCount']
+    np.testing.assert_allclose(res.residualNorms(), ref['residualNorms'], rtol=1e-7, atol=1e-300)
+    assert rel(res.solution(), ref['solution']) < 1e-6
+    B = np.stack([b, 1j * b[::-1].copy(), np.zeros_like(b)], axis=1)
+    B[:5, 2] = 1.0
+    batch = solver.solve(op, B, preconditioner=pc)
+    for q in range(3):
+        alone = solver.solve(op, B[:, q], preconditioner=pc)
+        assert batch[q].converged() and batch[q].iterationCount() == alone.iterationCount()
+        assert rel(batch[q].solution(), alone.solution()) < 1e-12
