@@ -188,11 +188,12 @@ def served_request(rt, i, max_side=1024, side=None):
             'port': port}
 
 
-def serve_stream(rt, requests, solver, slots, arrivals):
-    """Submits requests[i] at arrivals[i] seconds after the start to a server with `slots` workers; returns the
-    per-request (solution, Result, times) and the makespan (first arrival to last completion)."""
+def serve_stream(rt, requests, solver, slots, arrivals, batch=1):
+    """Submits requests[i] at arrivals[i] seconds after the start to a server with `slots` workers (batching up to
+    `batch` queued requests of one embedding size per solve); returns the per-request (solution, Result, times) and
+    the makespan (first arrival to last completion)."""
     import threading
-    server = rt.Server(slots=slots)
+    server = rt.Server(slots=slots, batch=batch)
     tickets = [None] * len(requests)
     t_start = time.perf_counter()
 
@@ -442,6 +443,22 @@ def main():
                 'residual_max': max(o[1].residualNorm() for o in out),
                 'iterations_median': int(np.median([o[1].iterationCount() for o in out])),
                 'peak_concurrent': st['peak_running'], 'alone_vs_mix_max_rel_diff': diff})
+        # Batching by embedding size: queued requests of one FFT period join one batched product and Krylov loop
+        # (up to 8), on the same arrivals; their results against the ones alone (equal to the solve's accuracy).
+        name, solver = forms[0]
+        print('bench: served stream, mixed form batched by embedding size, saturation', flush=True)
+        sat, sat_span, sat_st = serve_stream(rt, reqs, solver, 4, [0.0] * Q, batch=8)
+        print('bench: served stream, mixed form batched by embedding size, Poisson arrivals', flush=True)
+        out, span, st = serve_stream(rt, reqs, solver, 4, arrivals, batch=8)
+        lat = [o[2]['latency'] for o in out]
+        diff = max(float(np.linalg.norm(o[0] - a[0]) / np.linalg.norm(a[0])) for o, a in zip(out, alone[name]))
+        serving['mixed batched'] = {
+            'saturation_solves_per_s': round(Q / sat_span, 3), 'saturation_batched_requests':
+                sat_st['batched_requests'], 'solves_per_s': round(Q / span, 3),
+            'p50_ms': round(1e3 * percentile(lat, 50), 3), 'p99_ms': round(1e3 * percentile(lat, 99), 3),
+            'converged': int(sum(o[1].converged() for o in out)),
+            'residual_max': max(o[1].residualNorm() for o in out), 'batched_requests': st['batched_requests'],
+            'batched_solves': st['batched_solves'], 'alone_vs_mix_max_rel_diff': diff}
         serving['requests'] = Q
         serving['arrival_rate_per_s'] = round(rate, 3)
         serving['workload'] = ('square grids L = round(exp(U[ln 32, ln 1024])), f ~ U[0.5, 3] GHz, cell lambda/U[10,'

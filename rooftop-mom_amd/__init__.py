@@ -781,12 +781,15 @@ def blob_mask(countX, countY, seed, fill):
 class Server:
     """Requests of different grids, frequencies and masks admitted first come, first served by an HBM budget and
     solved concurrently by `slots` workers, each on its own HIP stream (rt_server_*). The result of a request is
-    the one Solver.solve gives on the same inputs."""
+    the one Solver.solve gives on the same inputs. batch > 1: queued requests of the same embedding size (and solver
+    options) are solved together, up to `batch` in one batched product and Krylov loop (rt_server_set_batching)."""
 
-    def __init__(self, slots=4, budget=0, device=0):
+    def __init__(self, slots=4, budget=0, device=0, batch=1):
         h = C.c_void_p()
         check(library().rt_server_create(int(device), int(budget), int(slots), C.byref(h)))
         self.handle = h
+        if batch != 1:
+            check(library().rt_server_set_batching(self.handle, int(batch)))
 
     @staticmethod
     def _request(countX, countY, sizeX, sizeY, wavenumber, rightHandSide, mask, solver, preconditioner,
@@ -847,8 +850,11 @@ class Server:
         counts = (C.c_int64 * 5)()
         by = (C.c_uint64 * 2)()
         check(library().rt_server_stats(self.handle, counts, by))
+        groups, grouped = C.c_int64(), C.c_int64()
+        check(library().rt_server_batch_stats(self.handle, C.byref(groups), C.byref(grouped)))
         return {'submitted': counts[0], 'completed': counts[1], 'queued': counts[2], 'running': counts[3],
-                'peak_running': counts[4], 'reserved_bytes': by[0], 'budget_bytes': by[1]}
+                'peak_running': counts[4], 'reserved_bytes': by[0], 'budget_bytes': by[1],
+                'batched_solves': groups.value, 'batched_requests': grouped.value}
 
     def close(self):
         if getattr(self, 'handle', None) is not None and _abi._lib is not None:

@@ -75,6 +75,8 @@ SIGNATURES = [
     ('rt_server_submit', C.c_int, [P, C.POINTER(rt_request), C.POINTER(C.c_int64)]),
     ('rt_server_wait', C.c_int, [P, C.c_int64, P, C.c_int64, C.POINTER(rt_output), C.POINTER(C.c_double)]),
     ('rt_server_stats', C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_uint64)]),
+    ('rt_server_set_batching', C.c_int, [P, C.c_int32]),
+    ('rt_server_batch_stats', C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ('rt_request_bytes', C.c_int, [C.POINTER(rt_request), C.POINTER(C.c_uint64)]),
     ('rt_blob_mask', C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_double, P]),
     ('rt_solve', C.c_int, [P, P, P, C.POINTER(rt_solve_opts), P, C.c_int64, C.c_int64, P, P,

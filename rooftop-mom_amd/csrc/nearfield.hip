@@ -17,7 +17,7 @@
 
 namespace rt {
 
-constexpr int kSlots = 18;    // 2 directions x 3 x 3 first-face offsets
+constexpr int kSlots = kNearFieldSlots;   // 2 directions x 3 x 3 first-face offsets
 constexpr int kFit = 50;      // 2 directions x 5 x 5
 
 // Generator value at absolute offsets (the generators are even in each offset); zero outside.
@@ -136,10 +136,13 @@ __global__ void __launch_bounds__(kApplyThreads) sai_apply_tiled(const c128 *__r
                                                               const int *__restrict__ scls,
                                                               const cplx<TC> *__restrict__ scoef, int rows_x,
                                                               int cols_x, int rows_y, int cols_y, int64_t nx, int64_t n,
-                                                              int tiles_x, int table_count,
-                                                              const int *__restrict__ skip, int skip_stride) {
+                                                              int tiles_x, int table_count, int64_t cls_stride,
+                                                              int64_t coef_stride, const int *__restrict__ skip,
+                                                              int skip_stride) {
     const int r = blockIdx.y;
     if (skip && skip[static_cast<int64_t>(r) * skip_stride]) return;
+    scls += r * cls_stride;     // a batch of problems: this right-hand side's classes and coefficient table
+    scoef += r * coef_stride;
     constexpr int kTX = SPLIT ? kApplyThreads / (2 * kTY) : kApplyThreads / kTY;
     constexpr int LX = kTX + 2, LY = kTY + 2;
     __shared__ cplx<TC> tile[2][LY][LX];
@@ -539,7 +542,8 @@ void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int 
     const uint32_t lds = staged ? static_cast<uint32_t>(table) : 0;
     auto launch = [&](auto kernel, const auto *coef) {
         launch_timed(&timer, kernel, grid, dim3(kApplyThreads), lds, s, in, out, pc.scls.ptr, coef, d.rows_x,
-                     d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, staged, skip, skip_stride);
+                     d.cols_x, d.rows_y, d.cols_y, d.nx_count, d.n, tiles_x, staged, pc.cls_stride,
+                     pc.coef_stride, skip, skip_stride);
     };
     if (prec == RT_C128) {
         if (split) launch(sai_apply_tiled<double, true>, pc.scoef64.ptr);

@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(MAXT) pass_a_k(const c128 *__restrict__ in, cp
         cplx<TC> *lds = buf + c * CS;
         auto store = [&](int, int pp, cplx<TC> v) { lds[pp] = v; };
         if (exists) {   // a masked plate: the mask bytes are loaded beside the values
-            const uint8_t *exc = exists + (comp ? d.nx_count : 0) + col;
+            const uint8_t *exc = exists + r * d.mask_stride + (comp ? d.nx_count : 0) + col;
             cround<TC, P, 0, false, EMB, (P::K > 1)>(
                 t, tw,
                 [&](int pos, int) {

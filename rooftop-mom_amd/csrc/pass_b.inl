@@ -102,7 +102,9 @@ __global__ void __launch_bounds__(MAXT) pass_b_k(cplx<TC> *__restrict__ work, co
     cplx<TC> *w1 = work + work_index(d, r, 1, px, 0);
     const int n0 = d.cols_x, n1 = d.cols_y;
     const cplx<TC> *row = FOUR ? bspec + static_cast<int64_t>(px) * d.My * 4
-                               : bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+                               : bspec + r * d.spec_stride + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+    dx += r * d.dx_stride;
+    dy += r * d.dy_stride;
     const cplx<TC> dxv = dx[px];
     if constexpr (P::K == 1) {   // one round: straight from the work array and back
         multiply_round<TC, P, true, EMB, FOUR>(t, col0, col1, w0, w1, n0, n1, row, dxv, dy);
@@ -262,7 +264,9 @@ __global__ void __launch_bounds__(MAXT) pass_b2_k(cplx<TC> *__restrict__ work, c
     const int px = border[pt * tile + tt];   // partners k and M - k run together
     cplx<TC> *col0 = reinterpret_cast<cplx<TC> *>(smem_raw) + 2 * tt * Mp;
     cplx<TC> *col1 = col0 + Mp;
-    const cplx<TC> *row = bspec + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+    const cplx<TC> *row = bspec + r * d.spec_stride + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+    dx += r * d.dx_stride;
+    dy += r * d.dy_stride;
     const cplx<TC> dxv = dx[px];
     if (c == 0)
         pass_b_team<TC, P, 0>(t, col0, col1, work + work_index(d, r, 0, px, 0), d.cols_x, tw, row, dxv, dy);

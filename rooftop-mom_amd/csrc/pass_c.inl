@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(MAXT) pass_c_k(const cplx<TC> *__restrict__ wo
     const cplx<TC> *lds = buf + c * CS;
     auto load = [&](int, int pp) { return lds[pp]; };
     if (exists) {   // a masked plate: the mask bytes are loaded before the branch around each store
-        const uint8_t *exc = exists + (comp ? d.nx_count : 0) + col;
+        const uint8_t *exc = exists + r * d.mask_stride + (comp ? d.nx_count : 0) + col;
         cround<TC, P, 0, true, EMB && (P::M >= 2), (P::K > 1)>(t, tw, load, [&](int pos, int, cplx<TC> v) {
             const bool keep = exc[min(pos, rows - 1)] != 0;
             if (pos < rows) dst[pos] = keep ? cvt<double>(v) : mk(0.0, 0.0);

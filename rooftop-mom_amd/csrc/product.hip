@@ -229,6 +229,8 @@ static const cplx<TC> *bspec_of(Product &op, int bits, hipStream_t s) {
     auto build = [&](auto &arr) {
         using Elem = typename std::remove_reference<decltype(*arr.ptr)>::type;
         if (!arr.ptr) {
+            if (!op.planes64.ptr || d.spec_stride)
+                fail(RT_EINVAL, "rooftop:product", "the pass-B spectra of this launch shape are not built");
             arr.allocate(op.problem->ctx, count, "The spectra of the product");
             hipLaunchKernelGGL((build_bspec<decltype(Elem{}.x)>), dim3(blocks_for(count / 3, 256, 8192)), dim3(256),
                                0, s, op.planes64.ptr, d.Hx, d.Hy, d.My, op.ty.p[v].fold.ptr, arr.ptr);
@@ -285,6 +287,8 @@ static void run_chunk(Product &op, const c128 *in, c128 *out, cplx<TC> *work, in
         tb.done();
     } else {
         // Three launches, timed together as pass B (events recorded around them on the stream).
+        if (d.spec_stride || d.dx_stride)
+            fail(RT_EINVAL, "rooftop:product", "a batch of problems needs y transforms that fit one block's LDS");
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ctx->timing) {
             e0 = ctx->take_event();
@@ -309,7 +313,9 @@ static void run_chunk(Product &op, const c128 *in, c128 *out, cplx<TC> *work, in
 template <typename TC>
 static void run_passes(Product &op, const c128 *in, c128 *out, cplx<TC> *work, int nrhs, const PassShape &sh,
                        hipStream_t s, const int *skip = nullptr, int skip_stride = 0) {
-    const int G = std::min(sh.chunks, nrhs);
+    const ProdDims &d = op.dims;
+    const bool batch_of_problems = d.spec_stride || d.mask_stride || d.dx_stride;   // chunk offsets not supported
+    const int G = batch_of_problems ? 1 : std::min(sh.chunks, nrhs);
     if (G <= 1 || !b_fits<TC>(op, sh)) {
         run_chunk<TC>(op, in, out, work, nrhs, sh, s, skip, skip_stride, nullptr);
         return;

@@ -31,6 +31,11 @@ struct ProdDims {
     int64_t n;                // rooftops per right-hand side
     int Mx, My;               // embedding periods
     int Hx, Hy;               // folded spectrum extents M/2 + 1
+    // A batch of problems on one grid (the served requests of one embedding size): right-hand side r reads its own
+    // pass-B spectra at spec_stride r elements and its own rooftop mask at mask_stride r bytes (0: shared by all).
+    int64_t spec_stride = 0;
+    int64_t mask_stride = 0;
+    int64_t dx_stride = 0, dy_stride = 0;   // and its own divergence tables (the face sizes differ)
 };
 
 struct Problem {
@@ -117,6 +122,7 @@ struct Product {
     DeviceArray<c128> scoef64;                // near field: [stencil classes][18] symmetrized coefficients
     DeviceArray<c64> scoef32;
     int stencil_count = 0;
+    int64_t cls_stride = 0, coef_stride = 0;   // a batch of problems: per right-hand side classes and tables
     // A dense operator (n x n, column-major) and a host callback (convertToHandle.m:10-16).
     DeviceArray<c128> dense64;
     DeviceArray<c64> dense32;
@@ -177,7 +183,16 @@ std::unique_ptr<Product> create_callback(Context *ctx, int64_t n, rt_apply_fn fn
 void dense_apply(Product &op, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s, const int *skip,
                  int skip_stride);
 // nearfield.hip
+constexpr int kNearFieldSlots = 18;   // coefficients of a near-field stencil class
 std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precision);
+// stack.hip: a batch of problems on one padded face grid (same embedding) as one operator and one near-field
+// preconditioner (preconditioner 0: none), each right-hand side reading its own problem's data.
+struct StackedProducts {
+    std::shared_ptr<Problem> problem;   // the common grid and the stacked masks
+    std::unique_ptr<Product> op, pc;
+};
+StackedProducts create_stacked(Context *ctx, const std::vector<std::shared_ptr<Problem>> &members, int held,
+                               int preconditioner);
 void preconditioner_apply(Product &pc, const c128 *in, c128 *out, int nrhs, int prec, hipStream_t s,
                           const int *skip, int skip_stride);
 

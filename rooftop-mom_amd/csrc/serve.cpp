@@ -90,10 +90,41 @@ struct Server {
     int64_t next_ticket = 1, submitted = 0, completed = 0;
     bool stopping = false;
     std::vector<std::thread> workers;
+    // Requests of one embedding size solved together (rt_server_set_batching): up to max_group per solve.
+    int max_group = 1;
+    int64_t groups = 0, grouped = 0;   // solves of more than one request, and the requests they held
 
     void run(int slot);
     void solve(Context &ctx, ServeJob &job);
+    void solve_group(Context &ctx, const std::vector<std::shared_ptr<ServeJob>> &jobs);
 };
+
+// Whether two requests can share one batched solve: the same FFT periods (embedding size), the same face aspect and
+// quadrature orders, the near-field preconditioner or none on both, and equal solver options. Their grids, masks,
+// wavenumbers, face sizes and right-hand sides may differ.
+static bool batchable(const ServeJob &a, const ServeJob &b) {
+    auto period = [](int c) { return c > 1 ? next_pow2(2L * c - 1) : 1L; };
+    const SolveOptions &p = a.opts, &q = b.opts;
+    return period(a.cx) == period(b.cx) && period(a.cy) == period(b.cy) && a.cx > 1 && a.cy > 1 && b.cx > 1 &&
+           b.cy > 1 && a.b * b.a == b.b * a.a && a.near_order == b.near_order && a.far_order == b.far_order &&
+           a.preconditioner == b.preconditioner && a.preconditioner != RT_PRECOND_JACOBI &&
+           p.method == q.method && p.tolerance == q.tolerance && p.max_iterations == q.max_iterations &&
+           p.restart == q.restart && p.precision == q.precision && p.refine == q.refine &&
+           p.inner_tolerance == q.inner_tolerance && p.max_outer == q.max_outer;
+}
+
+// The device bytes of a group: every request as if on the group's padded grid, and the stacked copies of the spectra
+// and near field beside the members' own while the batch product is built.
+static uint64_t group_bytes(const std::vector<std::shared_ptr<ServeJob>> &jobs) {
+    if (jobs.size() == 1) return jobs[0]->bytes;
+    int cx = 0, cy = 0;
+    for (const auto &j : jobs) {
+        cx = std::max(cx, j->cx);
+        cy = std::max(cy, j->cy);
+    }
+    const uint64_t one = estimate_request_bytes(cx, cy, jobs[0]->opts, jobs[0]->preconditioner);
+    return (one + one / 2) * jobs.size();
+}
 
 void Server::solve(Context &ctx, ServeJob &job) {
     const SolveOptions &o = job.opts;
@@ -115,6 +146,56 @@ void Server::solve(Context &ctx, ServeJob &job) {
     job.result = r[0];
 }
 
+void Server::solve_group(Context &ctx, const std::vector<std::shared_ptr<ServeJob>> &jobs) {
+    // Every request on the common face grid (the largest counts of the group), its mask padded with empty faces, so
+    // that one product serves them all; its right-hand side and solution are moved between the two rooftop layouts.
+    const ServeJob &first = *jobs[0];
+    const SolveOptions &o = first.opts;
+    int cx = 0, cy = 0;
+    for (const auto &j : jobs) {
+        cx = std::max(cx, j->cx);
+        cy = std::max(cy, j->cy);
+    }
+    const GridInfo pad = GridInfo::make(cx, cy, 1.0, 1.0);
+    const int R = static_cast<int>(jobs.size());
+    const int held = o.precision == RT_C64 ? RT_MIXED : RT_C128;
+    std::vector<std::shared_ptr<Problem>> problems;
+    for (const auto &j : jobs) {
+        std::vector<uint8_t> mask(static_cast<size_t>(cx) * cy, 0);
+        for (int y = 0; y < j->cy; ++y)
+            for (int x = 0; x < j->cx; ++x)
+                mask[x + static_cast<size_t>(cx) * y] = j->mask.empty() ? 1 : j->mask[x + static_cast<size_t>(j->cx) * y];
+        problems.push_back(make_problem(&ctx, cx, cy, j->a, j->b, j->k, j->near_order, j->far_order, mask.data()));
+    }
+    StackedProducts batch = create_stacked(&ctx, problems, held, first.preconditioner);
+    problems.clear();   // the batch holds copies of what the solve reads
+    // Index of rooftop i of a request's own grid in the padded layout (x-directed first, x fastest).
+    auto padded = [&](const ServeJob &j, int64_t i) -> int64_t {
+        const GridInfo own = GridInfo::make(j.cx, j.cy, 1.0, 1.0);
+        if (i < own.nx) return i % own.rows_x + static_cast<int64_t>(pad.rows_x) * (i / own.rows_x);
+        const int64_t k = i - own.nx;
+        return pad.nx + k % own.rows_y + static_cast<int64_t>(pad.rows_y) * (k / own.rows_y);
+    };
+    const int64_t n = pad.n;
+    std::vector<c128> host(static_cast<size_t>(R) * n, mk(0.0, 0.0));
+    for (int r = 0; r < R; ++r)
+        for (int64_t i = 0; i < static_cast<int64_t>(jobs[r]->rhs.size()); ++i)
+            host[r * n + padded(*jobs[r], i)] = jobs[r]->rhs[i];
+    DeviceArray<c128> b, x;
+    b.allocate(&ctx, host.size(), "the right-hand sides");
+    x.allocate(&ctx, host.size(), "the solutions");
+    RT_HIP(hipMemcpyAsync(b.ptr, host.data(), host.size() * sizeof(c128), hipMemcpyHostToDevice, ctx.stream));
+    std::vector<SolveResult> res = solve_batch(&ctx, *batch.op, batch.pc.get(), o, b.ptr, nullptr, x.ptr, R);
+    RT_HIP(hipMemcpyAsync(host.data(), x.ptr, host.size() * sizeof(c128), hipMemcpyDeviceToHost, ctx.stream));
+    RT_HIP(hipStreamSynchronize(ctx.stream));
+    for (int r = 0; r < R; ++r) {
+        ServeJob &j = *jobs[r];
+        j.x.resize(j.rhs.size());
+        for (int64_t i = 0; i < static_cast<int64_t>(j.x.size()); ++i) j.x[i] = host[r * n + padded(j, i)];
+        j.result = res[r];
+    }
+}
+
 void Server::run(int slot) {
     Context &ctx = *contexts[slot];
     try {
@@ -122,30 +203,56 @@ void Server::run(int slot) {
     } catch (const Error &) {
     }
     for (;;) {
-        std::shared_ptr<ServeJob> job;
+        std::vector<std::shared_ptr<ServeJob>> group;
+        uint64_t bytes = 0;
         {
             std::unique_lock<std::mutex> lk(m);
             cv_work.wait(lk, [&] {
                 return stopping || (!queue.empty() && (reserved + queue.front()->bytes <= budget || running == 0));
             });
             if (stopping) break;
-            job = queue.front();
+            group.push_back(queue.front());
             queue.pop_front();
-            reserved += job->bytes;
-            ++running;
+            bytes = group_bytes(group);
+            // Queued requests of the same embedding size join it while the group fits the budget (first come, first
+            // served among them).
+            for (auto it = queue.begin(); it != queue.end() && static_cast<int>(group.size()) < max_group;) {
+                if (!batchable(*group[0], **it)) {
+                    ++it;
+                    continue;
+                }
+                group.push_back(*it);
+                const uint64_t more = group_bytes(group);
+                if (reserved + more > budget) {
+                    group.pop_back();
+                    break;
+                }
+                bytes = more;
+                it = queue.erase(it);
+            }
+            reserved += bytes;
+            running += static_cast<int>(group.size());
             peak_running = std::max(peak_running, running);
+            if (group.size() > 1) {
+                ++groups;
+                grouped += static_cast<int64_t>(group.size());
+            }
         }
-        job->started = Clock::now();
+        const Clock::time_point started = Clock::now();
+        for (auto &job : group) job->started = started;
+        int status = RT_OK;
+        std::string error_id, error_message;
         try {
-            solve(ctx, *job);
+            if (group.size() == 1) solve(ctx, *group[0]);
+            else solve_group(ctx, group);
         } catch (const Error &e) {
-            job->status = e.status;
-            job->error_id = e.id;
-            job->error_message = e.what();
+            status = e.status;
+            error_id = e.id;
+            error_message = e.what();
         } catch (const std::exception &e) {
-            job->status = RT_EINVAL;
-            job->error_id = "rooftop:server";
-            job->error_message = e.what();
+            status = RT_EINVAL;
+            error_id = "rooftop:server";
+            error_message = e.what();
         }
         try {
             ctx.free_pool();
@@ -153,11 +260,17 @@ void Server::run(int slot) {
         }
         {
             std::lock_guard<std::mutex> lk(m);
-            job->finished = Clock::now();
-            job->done = true;
-            reserved -= job->bytes;
-            --running;
-            ++completed;
+            const Clock::time_point finished = Clock::now();
+            for (auto &job : group) {
+                job->finished = finished;
+                job->status = status;
+                job->error_id = error_id;
+                job->error_message = error_message;
+                job->done = true;
+            }
+            reserved -= bytes;
+            running -= static_cast<int>(group.size());
+            completed += static_cast<int64_t>(group.size());
         }
         cv_done.notify_all();
         cv_work.notify_all();
@@ -274,6 +387,25 @@ void rt_server_destroy(rt_server *h) {
     for (auto &t : srv.workers) t.join();
     for (auto &c : srv.contexts) c->close();
     delete h;   // a waiter still returning keeps the server state alive through its own reference
+}
+
+int rt_server_set_batching(rt_server *h, int32_t max_requests) {
+    if (!h || max_requests < 1 || max_requests > 64) {
+        set_last_error(RT_EINVAL, "rooftop:server", "1..64 requests per batched solve");
+        return RT_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(h->s->m);
+    h->s->max_group = max_requests;
+    set_last_error(RT_OK, "", "");
+    return RT_OK;
+}
+
+int rt_server_batch_stats(rt_server *h, int64_t *groups, int64_t *grouped) {
+    if (!h) return RT_EINVAL;
+    std::lock_guard<std::mutex> lk(h->s->m);
+    if (groups) *groups = h->s->groups;
+    if (grouped) *grouped = h->s->grouped;
+    return RT_OK;
 }
 
 int rt_server_submit(rt_server *h, const rt_request *r, int64_t *ticket) {

@@ -65,3 +65,43 @@ def test_budget_admission(rt):
         tiny.submit(qs[0]['L'], qs[0]['L'], qs[0]['a'], qs[0]['a'], qs[0]['k'], qs[0]['rhs'], mask=qs[0]['mask'])
     assert e.value.identifier == 'rooftopMoM:resources:insufficientMemory'
     tiny.close()
+
+
+def test_requests_of_one_embedding_size_are_solved_together(rt):
+    """Batching by embedding size (rt_server_set_batching): six queued requests of sides 65 .. 128 (one FFT period,
+    256) are solved as one group on the largest grid, each with its own mask, frequency, cell size and feed, and a
+    request of another embedding alone; every result converges, equals its solve alone to the solve's accuracy, and
+    has the oracle's residual on its own grid."""
+    import bench
+    from oracle import assembly as A
+    from oracle import spectra as S
+    sides = (70, 90, 128, 100, 120, 65, 40)
+    qs = [bench.served_request(rt, 20 + i, side=s) for i, s in enumerate(sides)]
+    solver = rt.Solver('gmres', tolerance=1e-6, restart=50, maxIterationCount=3000, precision='c64')
+    server = rt.Server(slots=1, batch=8)
+    # Submitted while the only slot is busy with a first request, so that the rest queue up together.
+    first = server.submit(qs[-1]['L'], qs[-1]['L'], qs[-1]['a'], qs[-1]['a'], qs[-1]['k'], qs[-1]['rhs'],
+                          mask=qs[-1]['mask'], solver=solver, preconditioner='nearField')
+    tickets = [server.submit(q['L'], q['L'], q['a'], q['a'], q['k'], q['rhs'], mask=q['mask'], solver=solver,
+                             preconditioner='nearField') for q in qs[:-1]]
+    out = [server.wait(t, q['n']) for t, q in zip(tickets, qs[:-1])]
+    server.wait(first, qs[-1]['n'])
+    stats = server.stats()
+    print('batched solves %d holding %d requests' % (stats['batched_solves'], stats['batched_requests']), flush=True)
+    assert stats['batched_solves'] >= 1 and stats['batched_requests'] >= 4
+    server.close()
+    for q, (x, res, times) in zip(qs[:-1], out):
+        p = rt.Problem(rt.Mesh(q['L'], q['L'], q['a'], q['a']), q['k'], mask=q['mask'])
+        alone = solver.solve(rt.Operator(p, precision='mixed'), q['rhs'],
+                             preconditioner=rt.Preconditioner(p, 'nearField', precision='mixed'))
+        print('L=%d: %d iterations batched, %d alone; difference %.2e' % (
+            q['L'], res.iterationCount(), alone.iterationCount(), rel(x, alone.solution())), flush=True)
+        assert res.converged() and res.residualNorm() < 1e-6
+        assert rel(x, alone.solution()) < 1e-6
+        assert abs(res.iterationCount() - alone.iterationCount()) <= 2
+        grid = A.Grid(q['L'], q['L'], q['a'], q['a'])
+        m = q['mask']
+        ex = np.concatenate([(m[:-1] & m[1:]).ravel(order='F'), (m[:, :-1] & m[:, 1:]).ravel(order='F')])
+        ax = ex * S.split_fft_product(S.subband_spectra(p.generators(), grid), grid, x)
+        assert rel(ax, q['rhs']) < 1.01e-6
+        assert np.all(x[ex == 0] == 0)
