@@ -255,7 +255,7 @@ __global__ void gather_generators(const c128 *__restrict__ terms, int cx, int cy
 }
 
 // ------------------------------------------------------------------------------------------------------------------
-// Touching closed forms (one thread; FP64).
+// Touching closed forms (FP64).
 // ------------------------------------------------------------------------------------------------------------------
 __device__ double monomial(int power, int m, int n, double a, double b) {
     if (a == 0 || b == 0) return 0;
@@ -332,73 +332,86 @@ __device__ int compose_part(int k, int part, int offset, double size, double *co
     return interval;
 }
 
-// M00, M10, M01, M11 at the four touching offsets of an a x b cell (generatePowerMoments.m:57-153).
-__device__ void along_x(double a, double b, int power, const double *gl_p, const double *gl_w, double out[4][4]) {
-    const double ratio = b / a;
-    double rect[4][2][2][2];
-    for (int m = 0; m < 4; ++m)
-        for (int n = 0; n < 2; ++n) {
-            double G[3][3];
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) G[i][j] = monomial(power, m, n, static_cast<double>(i), ratio * j);
-            for (int s = 0; s < 2; ++s)
-                for (int t = 0; t < 2; ++t) rect[m][n][s][t] = G[s + 1][t + 1] - G[s][t + 1] - G[s + 1][t] + G[s][t];
-        }
-    for (int kk = 0; kk < 2; ++kk)
-        for (int l = 0; l < 2; ++l) {
-            if (hypot(static_cast<double>(kk), ratio * l) < fmax(1.0, ratio)) continue;
-            for (int m = 0; m < 4; ++m)
-                for (int n = 0; n < 2; ++n) {
-                    double sum = 0;
-                    for (int i = 0; i < 20; ++i) {
-                        const double u = kk + gl_p[i];
-                        const double um = m == 0 ? 1.0 : m == 1 ? u : m == 2 ? u * u : u * u * u;
-                        double inner = 0;
-                        for (int j = 0; j < 20; ++j) {
-                            const double v = ratio * (l + gl_p[j]);
-                            const double r = sqrt(u * u + v * v);
-                            const double rp = power == -1 ? 1.0 / r : power == 1 ? r : r * r * r;
-                            inner += gl_w[j] * (n == 0 ? 1.0 : v) * rp;
-                        }
-                        sum += gl_w[i] * um * inner;
-                    }
-                    rect[m][n][kk][l] = ratio * sum;
-                }
-        }
-    const int offs[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
-    for (int k = 0; k < 4; ++k)
-        for (int l = 0; l < 4; ++l) {
-            double acc = 0;
-            for (int s = 0; s < 2; ++s) {
-                double ca[4];
-                const int ia = compose_part(k, s, offs[l][0], 1.0, ca, 4);
-                for (int t = 0; t < 2; ++t) {
-                    double cb[2];
-                    const int ib = compose_part(0, t, offs[l][1], ratio, cb, 2);
-                    for (int m = 0; m < 4; ++m)
-                        for (int n = 0; n < 2; ++n) acc += ca[m] * rect[m][n][ia][ib] * cb[n];
-                }
-            }
-            out[k][l] = pow(a, power) * acc / ratio;
-        }
+// Sum over the wave, the same in every lane (a butterfly of lane exchanges, in a fixed order).
+__device__ __forceinline__ double wave_total(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// The extracted moments 7x4 (generateExtractedMoments.m:15-18): P(-1) - k^2/2 P(1) + k^4/24 P(3).
-// Six threads evaluate the six power moment tables (three powers along x and along y) at once; thread 0 combines.
-__global__ void touching_moments(double a, double b, double k, const double *gl_p, const double *gl_w,
-                                 c128 *__restrict__ out) {
-    __shared__ double tab[6][4][4];
-    const int powers[3] = {-1, 1, 3};
-    const int t = threadIdx.x;
-    if (t < 6) {
-        double res[4][4];
-        if (t < 3) along_x(a, b, powers[t], gl_p, gl_w, res);
-        else along_x(b, a, powers[t - 3], gl_p, gl_w, res);
-        for (int i = 0; i < 4; ++i)
-            for (int j = 0; j < 4; ++j) tab[t][i][j] = res[i][j];
+// M00, M10, M01, M11 at the four touching offsets of an a x b cell (generatePowerMoments.m:57-153), computed by one
+// wave into out[4][4] (LDS). The corner values of the mixed antiderivatives (72, two per lane), the rectangle integrals
+// as their corner differences (64, one per lane), then the rectangles at least their longer side away from the origin
+// by a 20 x 20 Gauss-Legendre rule whose 400 points are spread over the lanes (generatePowerMoments.m:94-118: there
+// the closed form is a difference of large, nearly equal terms), and the 16 moments, one per lane.
+__device__ void along_x_wave(double a, double b, int power, const double *gl_p, const double *gl_w, double *G,
+                             double *rect, double out[4][4]) {
+    const int lane = threadIdx.x & 63;
+    const double ratio = b / a;
+    // G[((m * 2 + n) * 3 + i) * 3 + j] = monomial(power, m, n, i, ratio j).
+    for (int e = lane; e < 72; e += 64) {
+        const int j = e % 3, i = (e / 3) % 3, mn = e / 9;
+        G[e] = monomial(power, mn / 2, mn % 2, static_cast<double>(i), ratio * j);
     }
     __syncthreads();
-    if (t != 0) return;
+    {   // rect[((m * 2 + n) * 2 + s) * 2 + t]
+        const int t = lane & 1, s = (lane >> 1) & 1, mn = lane >> 2;
+        const double *g = G + mn * 9;
+        rect[lane] = g[(s + 1) * 3 + t + 1] - g[s * 3 + t + 1] - g[(s + 1) * 3 + t] + g[s * 3 + t];
+    }
+    __syncthreads();
+    for (int kk = 0; kk < 2; ++kk)
+        for (int l = 0; l < 2; ++l) {
+            if (hypot(static_cast<double>(kk), ratio * l) < fmax(1.0, ratio)) continue;   // uniform
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int e = lane; e < 400; e += 64) {
+                const int i = e / 20, j = e % 20;
+                const double u = kk + gl_p[i], v = ratio * (l + gl_p[j]);
+                const double r = sqrt(u * u + v * v);
+                const double rp = power == -1 ? 1.0 / r : power == 1 ? r : r * r * r;
+                const double w = gl_w[i] * gl_w[j] * rp;
+                double um = 1.0;
+                for (int m = 0; m < 4; ++m) {
+                    acc[m * 2] += w * um;
+                    acc[m * 2 + 1] += w * um * v;
+                    um *= u;
+                }
+            }
+            for (int mn = 0; mn < 8; ++mn) {
+                const double total = wave_total(acc[mn]);
+                if (lane == 0) rect[(mn * 2 + kk) * 2 + l] = ratio * total;
+            }
+        }
+    __syncthreads();
+    if (lane < 16) {
+        const int offs[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+        const int k = lane / 4, l = lane % 4;
+        double acc = 0;
+        for (int s = 0; s < 2; ++s) {
+            double ca[4];
+            const int ia = compose_part(k, s, offs[l][0], 1.0, ca, 4);
+            for (int t = 0; t < 2; ++t) {
+                double cb[2];
+                const int ib = compose_part(0, t, offs[l][1], ratio, cb, 2);
+                for (int m = 0; m < 4; ++m)
+                    for (int n = 0; n < 2; ++n) acc += ca[m] * rect[((m * 2 + n) * 2 + ia) * 2 + ib] * cb[n];
+            }
+        }
+        out[k][l] = pow(a, power) * acc / ratio;
+    }
+}
+
+// The extracted moments 7x4 (generateExtractedMoments.m:15-18): P(-1) - k^2/2 P(1) + k^4/24 P(3). Six waves
+// evaluate the six power moment tables (three powers along x and along y) at once; thread 0 combines.
+__global__ void __launch_bounds__(384) touching_moments(double a, double b, double k, const double *gl_p,
+                                                        const double *gl_w, c128 *__restrict__ out) {
+    __shared__ double tab[6][4][4];
+    __shared__ double G[6][72], rect[6][64];
+    const int powers[3] = {-1, 1, 3};
+    const int w = threadIdx.x >> 6;
+    if (w < 3) along_x_wave(a, b, powers[w], gl_p, gl_w, G[w], rect[w], tab[w]);
+    else along_x_wave(b, a, powers[w - 3], gl_p, gl_w, G[w], rect[w], tab[w]);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     const double factors[3] = {1.0, -k * k / 2, k * k * k * k / 24};
     double acc[7][4];
     for (int i = 0; i < 7; ++i)
@@ -493,7 +506,7 @@ void Problem::assemble() {
     hipLaunchKernelGGL(lattice_sums, dim3(static_cast<unsigned>((blocks + 255) / 256)), dim3(256), lds, s, d_pts.ptr,
                        d_qw.ptr, o, bx, by, a, b, k, lat.ptr);
     RT_LAUNCH_CHECK();
-    hipLaunchKernelGGL(touching_moments, dim3(1), dim3(64), 0, s, a, b, k, d_glp.ptr, d_glw.ptr, extracted.ptr);
+    hipLaunchKernelGGL(touching_moments, dim3(1), dim3(384), 0, s, a, b, k, d_glp.ptr, d_glw.ptr, extracted.ptr);
     RT_LAUNCH_CHECK();
     hipLaunchKernelGGL(offset_terms, dim3(static_cast<unsigned>((offsets + 255) / 256)), dim3(256), 0, s, near.ptr,
                        nearc[0], nearc[1], lat.ptr, bx, by, extracted.ptr, cx, cy, s_e, s_m, terms.ptr);

@@ -46,78 +46,117 @@ __device__ c128 z_entry(const GenView &v, int dr, int dc, int p, int q) {
     return (-1.0 / (v.a * v.b)) * (E(p, q) - E(p, q - 1) - E(p + 1, q) + E(p + 1, q - 1));
 }
 
-// One thread per class: build the fit matrix of the class's representative neighbourhood, complex Householder QR,
-// back substitution; coefficients by slot (0 where no rooftop). scratch holds kFit x (kSlots + 1) per class.
-__global__ void sai_fit(GenView v, const int *__restrict__ dir, const unsigned long long *__restrict__ bits,
-                        int classes, c128 *__restrict__ scratch, c128 *__restrict__ coef, int *__restrict__ bad) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= classes) return;
+// Sums over the wave, the same in every lane (a butterfly of lane exchanges, in a fixed order).
+__device__ __forceinline__ double wave_total(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ c128 wave_total(c128 v) { return mk(wave_total(v.x), wave_total(v.y)); }
+__device__ __forceinline__ c128 lane_value(c128 v, int k) {
+    return mk(__shfl(v.x, k, 64), __shfl(v.y, k, 64));
+}
+
+// One wave per class: the fit matrix of the class's representative neighbourhood with a row per lane (the m <= 50
+// rooftops of its 5 x 5 first-face neighbourhood; columns: the n <= 18 of its inner 3 x 3, then the right-hand side
+// e_self), complex Householder QR on the lanes' registers (a column's norm and each reflection's inner products as
+// wave sums), and the back substitution in every lane; coefficients by slot (0 where no rooftop).
+constexpr int kFitWaves = 4;
+__global__ void __launch_bounds__(64 * kFitWaves) sai_fit(GenView v, const int *__restrict__ dir,
+                                                          const unsigned long long *__restrict__ bits, int classes,
+                                                          c128 *__restrict__ coef, int *__restrict__ bad) {
+    const int c = blockIdx.x * kFitWaves + (threadIdx.x >> 6);
+    if (c >= classes) return;   // uniform over the wave; no barrier follows
+    const int lane = threadIdx.x & 63;
     const unsigned long long mask = bits[c];
     const int d0 = dir[c];
-    int slot_of_col[kSlots];
-    int col_dir[kSlots], col_x[kSlots], col_y[kSlots];
-    int row_dir[kFit], row_x[kFit], row_y[kFit];
-    int n = 0, m = 0, self = -1;
-    for (int d = 0; d < 2; ++d)
-        for (int dy = -2; dy <= 2; ++dy)
-            for (int dx = -2; dx <= 2; ++dx) {
-                const int bit = d * 25 + (dx + 2) + 5 * (dy + 2);
-                if (!((mask >> bit) & 1ull)) continue;
-                row_dir[m] = d;
-                row_x[m] = dx;
-                row_y[m] = dy;
-                if (d == d0 && dx == 0 && dy == 0) self = m;
-                ++m;
-                if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) {
-                    slot_of_col[n] = d * 9 + (dx + 1) + 3 * (dy + 1);
-                    col_dir[n] = d;
-                    col_x[n] = dx;
-                    col_y[n] = dy;
-                    ++n;
-                }
+    // The lane's row: the lane-th set bit of the mask (bit d * 25 + (dx + 2) + 5 (dy + 2)); the columns: the set bits
+    // of the inner 3 x 3, in the same order.
+    const int m = __popcll(mask);
+    int rd = 0, rx = 0, ry = 0;
+    {
+        int seen = 0;
+        for (int bit = 0; bit < 50; ++bit) {
+            if (!((mask >> bit) & 1ull)) continue;
+            if (seen == lane) {
+                rd = bit / 25;
+                rx = bit % 5 - 2;
+                ry = (bit % 25) / 5 - 2;
             }
-    c128 *A = scratch + static_cast<int64_t>(c) * kFit * (kSlots + 1);   // column-major m x (n + 1), last = rhs
-    for (int j = 0; j < n; ++j)
-        for (int i = 0; i < m; ++i)
-            A[i + kFit * j] = z_entry(v, row_dir[i], col_dir[j], row_x[i] - col_x[j], row_y[i] - col_y[j]);
-    for (int i = 0; i < m; ++i) A[i + kFit * n] = mk(i == self ? 1.0 : 0.0, 0.0);
+            ++seen;
+        }
+    }
+    const bool row = lane < m;
+    int cd[kSlots], cx[kSlots], cy[kSlots], slot[kSlots];
+    int n = 0, self = -1;
+    {
+        int seen = 0;
+        for (int bit = 0; bit < 50; ++bit) {
+            if (!((mask >> bit) & 1ull)) continue;
+            const int d = bit / 25, dx = bit % 5 - 2, dy = (bit % 25) / 5 - 2;
+            if (d == d0 && dx == 0 && dy == 0) self = seen;
+            ++seen;
+            if (dx < -1 || dx > 1 || dy < -1 || dy > 1) continue;
+            cd[n] = d;
+            cx[n] = dx;
+            cy[n] = dy;
+            slot[n] = d * 9 + (dx + 1) + 3 * (dy + 1);
+            ++n;
+        }
+    }
+    c128 A[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j)
+        A[j] = (row && j < n) ? z_entry(v, rd, cd[j], rx - cx[j], ry - cy[j]) : mk(0.0, 0.0);
+    c128 rhs = mk(row && lane == self ? 1.0 : 0.0, 0.0);
     bool ok = self >= 0 && n > 0;
-    for (int k = 0; k < n && ok; ++k) {
-        double nrm2 = 0;
-        for (int i = k; i < m; ++i) nrm2 += abs2(A[i + kFit * k]);
-        const double nrm = sqrt(nrm2);
-        if (!(nrm > 0)) { ok = false; break; }
-        const c128 x0 = A[k + kFit * k];
+#pragma unroll
+    for (int k = 0; k < kSlots; ++k) {
+        if (k >= n || !ok) break;   // uniform over the wave
+        const c128 xk = A[k];
+        const double nrm = sqrt(wave_total(lane >= k ? abs2(xk) : 0.0));
+        if (!(nrm > 0)) {
+            ok = false;
+            break;
+        }
+        const c128 x0 = lane_value(xk, k);
         const double ax0 = sqrt(abs2(x0));
         const c128 phase = ax0 > 0 ? mk(x0.x / ax0, x0.y / ax0) : mk(1.0, 0.0);
-        // v = x + phase * nrm e_0, stored in place of column k; tau = 1 / (nrm (nrm + |x0|)).
-        A[k + kFit * k] = x0 + nrm * phase;
+        // The reflector v = x + phase nrm e_k on the lanes >= k; tau = 1 / (nrm (nrm + |x0|)).
+        const c128 vk = lane < k ? mk(0.0, 0.0) : (lane == k ? x0 + nrm * phase : xk);
         const double tau = 1.0 / (nrm * (nrm + ax0));
-        for (int j = k + 1; j <= n; ++j) {
-            c128 w = mk(0.0, 0.0);
-            for (int i = k; i < m; ++i) w = w + cmulc(A[i + kFit * k], A[i + kFit * j]);
-            w = tau * w;
-            for (int i = k; i < m; ++i) A[i + kFit * j] = A[i + kFit * j] - A[i + kFit * k] * w;
+#pragma unroll
+        for (int j = k + 1; j < kSlots; ++j) {
+            if (j >= n) break;
+            const c128 w = tau * wave_total(cmulc(vk, A[j]));
+            A[j] = A[j] - vk * w;
         }
-        A[k + kFit * k] = -nrm * phase;   // R(k, k)
+        const c128 w = tau * wave_total(cmulc(vk, rhs));
+        rhs = rhs - vk * w;
+        if (lane == k) A[k] = mk(0.0, 0.0) - nrm * phase;   // R(k, k)
     }
+    // Back substitution with R in lanes 0 .. n-1: every lane runs it on the broadcast entries.
     c128 x[kSlots];
-    if (ok) {
-        for (int k = n - 1; k >= 0; --k) {
-            c128 s = A[k + kFit * n];
-            for (int j = k + 1; j < n; ++j) s = s - A[k + kFit * j] * x[j];
-            const c128 r = A[k + kFit * k];
-            const double den = abs2(r);
-            x[k] = (1.0 / den) * cmulc(r, s);
-            if (!(den > 0) || !isfinite(x[k].x) || !isfinite(x[k].y)) ok = false;
-        }
+#pragma unroll
+    for (int k = kSlots - 1; k >= 0; --k) {
+        x[k] = mk(0.0, 0.0);
+        if (k >= n || !ok) continue;
+        c128 sum = lane_value(rhs, k);
+#pragma unroll
+        for (int j = k + 1; j < kSlots; ++j)
+            if (j < n) sum = sum - lane_value(A[j], k) * x[j];
+        const c128 r = lane_value(A[k], k);
+        const double den = abs2(r);
+        x[k] = (1.0 / den) * cmulc(r, sum);
+        if (!(den > 0) || !isfinite(x[k].x) || !isfinite(x[k].y)) ok = false;
     }
-    for (int s = 0; s < kSlots; ++s) coef[static_cast<int64_t>(c) * kSlots + s] = mk(0.0, 0.0);
+    if (lane < kSlots) coef[static_cast<int64_t>(c) * kSlots + lane] = mk(0.0, 0.0);
     if (!ok) {
-        atomicAdd(bad, 1);
+        if (lane == 0) atomicAdd(bad, 1);
         return;
     }
-    for (int j = 0; j < n; ++j) coef[static_cast<int64_t>(c) * kSlots + slot_of_col[j]] = x[j];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j)
+        if (j < n && lane == 0) coef[static_cast<int64_t>(c) * kSlots + slot[j]] = x[j];
 }
 
 // z = P r with P = (M + M^T)/2 as an 18-point stencil. The stencil of a rooftop depends only on its class and its
@@ -403,15 +442,13 @@ std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precisi
         if (classes > 0) {
             DeviceArray<int> d_dir;
             DeviceArray<unsigned long long> d_bits;
-            DeviceArray<c128> scratch;
             d_dir.allocate(ctx, classes, "near-field fit");
             d_bits.allocate(ctx, classes, "near-field fit");
-            scratch.allocate(ctx, static_cast<size_t>(classes) * kFit * (kSlots + 1), "near-field fit");
             RT_HIP(hipMemcpyAsync(d_dir.ptr, rep_dir.data(), classes * sizeof(int), hipMemcpyHostToDevice, s));
             RT_HIP(hipMemcpyAsync(d_bits.ptr, rep_bits.data(), classes * sizeof(unsigned long long),
                                   hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(sai_fit, dim3((classes + 63) / 64), dim3(64), 0, s, v, d_dir.ptr, d_bits.ptr, classes,
-                               scratch.ptr, pc->coef64.ptr, bad.ptr);
+            hipLaunchKernelGGL(sai_fit, dim3((classes + kFitWaves - 1) / kFitWaves), dim3(64 * kFitWaves), 0, s, v,
+                               d_dir.ptr, d_bits.ptr, classes, pc->coef64.ptr, bad.ptr);
             RT_LAUNCH_CHECK();
             int h_bad = 0;
             RT_HIP(hipMemcpyAsync(&h_bad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
