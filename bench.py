@@ -9,8 +9,8 @@ The right-hand side and the solution stay in HBM (rt_solve_device); one solve is
 convergence. value = mixed solves/s over the timed steps.
 
 The JSON line also carries p50/p99 latency per form, the error of the mixed solution against the FP64 one, a live
-roofline of the dominant matvec kernel (pass B: y transforms + fused spectral multiply), measured with HIP events
-on the library's stream over the timed solves, and a CPU baseline: the oracle's FP64 split-FFT GMRES on a bounded
+roofline of the dominant kernel (by total GPU time in the mixed form), measured with HIP events on the library's
+stream over the timed solves, and a CPU baseline: the oracle's FP64 split-FFT GMRES on a bounded
 sample of the same solve, timed on this host.
 
 Batched forms solve the same plate under 64 plane waves together (GMRES(50) and COCG, each mixed and FP64;
@@ -41,7 +41,8 @@ PEAK_HBM_GBS = 8000.0
 TIMED = ['pass_a_k<double> (x FFT)', 'pass_b_k<double> (y FFT + multiply)', 'pass_c_k<double> (inverse x FFT)',
          'pass_a_k<float> (x FFT)', 'pass_b_k<float> (y FFT + multiply)', 'pass_c_k<float> (inverse x FFT)',
          'k_dcgs_dots ([s z] = V^H [p w])', 'k_dcgs_update (q = (p - V s)/beta; p = (w - V z)/(beta nu))',
-         'sai_apply<double> (near field)', 'sai_apply<float> (near field)']
+         'sai_apply<double> (near field)', 'sai_apply<float> (near field)',
+         'k_dgmres_finalize (block sums, Hessenberg columns, rotations, stopping test)']
 
 
 class KernelClock:
@@ -266,6 +267,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-only', action='store_true',
                     help='only the plain 1 GiB device copy (calibrates the HBM byte counters of rocprofv3)')
+    ap.add_argument('--matvec-only', nargs=3, metavar=('SIDE', 'RHS', 'PRECISION'),
+                    help='only the matvec of the SIDE x SIDE plate (lambda/10, orders [16, 8]) on RHS right-hand '
+                         'sides in c64 or c128, tuned, then applied --steps times (the counter profiles)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: this build runs the path on a single MI355X and does not shard a solve '
@@ -283,6 +287,18 @@ def main():
     copy_gbs = 2 * copy_bytes / (copy_ms.value * 1e-3) / 1e9
     if args.copy_only:
         print(json.dumps({'copy_bytes': copy_bytes, 'ms': copy_ms.value, 'GBps': round(copy_gbs, 1)}), flush=True)
+        return
+    if args.matvec_only:
+        side, R, prec = int(args.matvec_only[0]), int(args.matvec_only[1]), args.matvec_only[2]
+        print('bench: matvec of the %dx%d plate, %d right-hand sides, %s' % (side, side, R, prec), flush=True)
+        mp = rt.Problem(rt.Mesh(side, side, 0.1, 0.1), 2 * np.pi, quadratureOrder=(16, 8))
+        mop = rt.Operator(mp, precision='mixed')
+        shape, _ = mop.tune(nrhs=R, precision=prec)
+        pms, pby, wall = mop.profile(nrhs=R, reps=max(1, args.steps), precision=prec)
+        print(json.dumps({'workload': 'plate %dx%d, %d right-hand sides, %s' % (side, side, R, prec),
+                          'shape': shape, 'pass_ms': pms, 'pass_bytes': pby, 'wall_ms': wall,
+                          'GBps': sum(pby) / (wall * 1e-3) / 1e9,
+                          'frac': sum(pby) / (wall * 1e-3) / 1e9 / PEAK_HBM_GBS}), flush=True)
         return
     n = args.n
     print('bench: plate %dx%d, assembling generators and spectra' % (n, n), flush=True)

@@ -240,7 +240,8 @@ enum {
     RT_TIMED_GS_UPDATE = 7,       /* update pass: q = (p - V s) / beta, p = (w - V z - z_j q) / (beta nu) */
     RT_TIMED_PRECOND_C128 = 8,    /* preconditioner application */
     RT_TIMED_PRECOND_C64 = 9,
-    RT_TIMED_COUNT = 10
+    RT_TIMED_GS_FINALIZE = 10,    /* the dot pass's block sums, the Hessenberg columns, rotations and stopping test */
+    RT_TIMED_COUNT = 11
 };
 int rt_ctx_set_timing(rt_ctx *ctx, int32_t on);
 int rt_ctx_kernel_timing(rt_ctx *ctx, int64_t *launches, double *ms, double *bytes);

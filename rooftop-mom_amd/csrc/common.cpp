@@ -165,8 +165,10 @@ hipEvent_t Context::take_event() {
         event_pool.pop_back();
         return e;
     }
+    // Timing only: no system-scope fence when the event is recorded, so that the cache writeback it would add does not
+    // lengthen the measured kernel (its results are read after the stream is synchronized).
     hipEvent_t e;
-    RT_HIP(hipEventCreate(&e));
+    RT_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     return e;
 }
 

@@ -1584,10 +1584,12 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                              static_cast<uint32_t>(tile_lds), ws.s, V, j, pv, w, n, ws.dot_per, ws.dot_strided,
                              ws.partial, st);
                 t1.done();
-                hipLaunchKernelGGL(k_dgmres_finalize, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s,
-                                   ws.partial, ws.dot_blocks, ws.scal, Hr, H, g, m, j, it, ws.state, ws.history,
-                                   ws.cap);
-                RT_LAUNCH_CHECK();
+                // Algorithmic bytes: the dot pass's partial sums once (the Hessenberg entries and scalars are few).
+                KernelTimer tf(ctx, RT_TIMED_GS_FINALIZE, 16.0 * (2 * j + 3) * ws.dot_blocks * R);
+                launch_timed(&tf, k_dgmres_finalize, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s,
+                             static_cast<const c128 *>(ws.partial), ws.dot_blocks, ws.scal, Hr, H, g, m, j, it,
+                             ws.state, ws.history, ws.cap);
+                tf.done();
                 KernelTimer t2(ctx, RT_TIMED_GS_UPDATE, j * col + 3 * wb + col);
                 // A few systems leave the chip short of rows: four lanes per row then.
                 if (ws.gs_tile_update)
