@@ -247,6 +247,7 @@ def cpu_baseline(n, budget_s=10.0):
         res = out['residualNorm']
     dt = time.perf_counter() - t0
     S.set_workers(1)
+    cpu_baseline.solution = out['solution']   # the oracle's FP64 solution, for the GPU forms' error against it
     return {'value': count / dt, 'unit': 'solves/s', 'cores': cores, 'kind': 'port',
             'sample': '%d whole solves of the same %dx%d plate in %.1f s: oracle FP64 split-FFT GMRES(50) + '
                       'near-field, %d iterations each, true residual %.2e (scipy FFT on %d threads)'
@@ -334,6 +335,7 @@ def main():
 
     fp64 = rt.Solver('gmres', tolerance=1e-6, restart=50, refine=True)
     mixed = rt.Solver('gmres', tolerance=1e-6, restart=50, precision='c64')
+    mixed_b64 = rt.Solver('gmres', tolerance=1e-6, restart=50, precision='c64', basis='c128')
     print('bench: fp64 form', flush=True)
     t64, i64, o64, r64 = run_form(rt, ctx, op, pc, fp64, d_b.value, d_x64.value, N, args.steps, args.warmup)
     print('bench: mixed form', flush=True)
@@ -343,6 +345,15 @@ def main():
     rt.check(lib.rt_copy_to_host(ctx.handle, x64.ctypes.data_as(C.c_void_p), d_x64, 16 * N))
     rt.check(lib.rt_copy_to_host(ctx.handle, xmx.ctypes.data_as(C.c_void_p), d_x, 16 * N))
     err = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+    # The mixed form with its GMRES basis in complex128 (every Gram-Schmidt product and the basis in FP64).
+    print('bench: mixed form, complex128 basis', flush=True)
+    d_xb = C.c_void_p()
+    rt.check(lib.rt_device_alloc(ctx.handle, 16 * N, C.byref(d_xb)))
+    tb6, ib6, ob6, rb6 = run_form(rt, ctx, op, pc, mixed_b64, d_b.value, d_xb.value, N, max(5, args.steps // 3),
+                                  args.warmup)
+    xb6 = np.empty(N, np.complex128)
+    rt.check(lib.rt_copy_to_host(ctx.handle, xb6.ctypes.data_as(C.c_void_p), d_xb, 16 * N))
+    rt.check(lib.rt_device_free(ctx.handle, d_xb))
     # Kernel table: a separate run of the mixed form with the library's event timing on (the events themselves add
     # launch gaps, so the headline solves above run without them). The dominant kernel by total GPU time gives the
     # roofline.
@@ -534,6 +545,10 @@ def main():
         'fp64': {'solves_per_s': round(steps / sum(t64), 3), 'p50_ms': round(1e3 * percentile(t64, 50), 3),
                  'p99_ms': round(1e3 * percentile(t64, 99), 3), 'iterations': int(np.median(i64)),
                  'outer': int(np.median(o64)), 'residual': max(r64)},
+        'mixed_complex128_basis': {'solves_per_s': round(len(tb6) / sum(tb6), 3),
+                                   'p50_ms': round(1e3 * percentile(tb6, 50), 3), 'iterations': int(np.median(ib6)),
+                                   'residual': max(rb6),
+                                   'error_vs_fp64': float(np.linalg.norm(xb6 - x64) / np.linalg.norm(x64))},
         'mixed_vs_fp64_speedup': round(sum(t64) / sum(tmx), 3),
         'error_mixed_vs_fp64': err,
         'roofline': roofline,
@@ -549,6 +564,11 @@ def main():
     if not args.no_cpu:
         print('bench: cpu baseline', flush=True)
         line['cpu_baseline'] = cpu_baseline(n)
+        # Each GPU form's solution against the oracle's FP64 solution of the same plate (all at residual 1e-6).
+        xo = cpu_baseline.solution
+        line['error_vs_fp64_oracle'] = {
+            name: float(np.linalg.norm(v - xo) / np.linalg.norm(xo))
+            for name, v in (('mixed', xmx), ('fp64', x64), ('mixed_complex128_basis', xb6))}
     print(json.dumps(line), flush=True)
 
 

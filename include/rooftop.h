@@ -41,6 +41,9 @@ extern "C" {
 #define RT_C128 0
 #define RT_C64 1
 #define RT_MIXED 2
+/* Solver precision only: complex64 products as RT_C64, with the GMRES basis kept in complex128 (by default a
+ * complex64 inner GMRES stores its basis in complex64; every Gram-Schmidt coefficient is FP64 either way). */
+#define RT_C64_FP64_BASIS 3
 
 /* Generator modifiers (rooftopMoM.Operator 'modifier', Generators.m:25-37). */
 #define RT_MODIFIER_NONE 0
@@ -72,7 +75,7 @@ typedef struct rt_problem rt_problem;
 typedef struct rt_product rt_product;
 
 /* Solver options: rooftopMoM.Solver name-value inputs (Solver.m:17-38) plus the precision of the inner loop.
- *   precision       RT_C128 or RT_C64: arithmetic of the products inside the Krylov loop.
+ *   precision       RT_C128, RT_C64 or RT_C64_FP64_BASIS: arithmetic of the products inside the Krylov loop.
  *   refine          0: the reference's loop as is (the true residual is reported, not enforced);
  *                   1: an FP64 outer loop on the true residual norm(b - A x)/norm(b) until it is below tolerance
  *                      (iterative refinement; required with precision RT_C64).

@@ -661,11 +661,12 @@ class Solver:
     """rooftopMoM.Solver('method', ..., 'tolerance', ..., 'maxIterationCount', ..., 'restart', ...) (Solver.m:17-38).
 
     precision 'c64' runs the products of the Krylov loop in complex64 inside an FP64 refinement loop on the true
-    residual (refine is then implied); refine=True alone wraps the FP64 loop the same way.
+    residual (refine is then implied); refine=True alone wraps the FP64 loop the same way. basis 'c128' keeps the
+    GMRES basis of a complex64 inner loop in complex128 (by default it is stored in complex64).
     """
 
     def __init__(self, method='cg', tolerance=1e-6, maxIterationCount=1000, restart=50, precision='c128',
-                 refine=False, innerTolerance=0.0, maxOuter=20):
+                 refine=False, innerTolerance=0.0, maxOuter=20, basis=None):
         if method not in ('cg', 'gmres'):
             raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember', "'method' must be 'cg' or 'gmres'.")
         if not (0 < tolerance < 1):
@@ -679,6 +680,9 @@ class Solver:
         self.restart = restart
         self.precision = precision
         self.refine = bool(refine) or precision == 'c64'
+        if basis not in (None, 'c64', 'c128'):
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember', "'basis' is 'c64' or 'c128'.")
+        self.basis = basis
         self.innerTolerance, self.maxOuter = float(innerTolerance), int(maxOuter)
 
     def options(self):
@@ -688,6 +692,8 @@ class Solver:
         o.max_iterations = self.maxIterationCount
         o.restart = 0 if self.restart == math.inf else int(self.restart)
         o.precision = _PRECISIONS[self.precision]
+        if self.precision == 'c64' and self.basis == 'c128':
+            o.precision = _abi.RT_C64_FP64_BASIS
         o.refine = int(self.refine)
         o.inner_tolerance = self.innerTolerance
         o.max_outer = self.maxOuter

@@ -8,7 +8,7 @@ LIBRARY = os.path.join(HERE, 'librooftop.so')
 
 RT_OK, RT_EINVAL, RT_ESIZE, RT_ENOMEM, RT_ESINGULAR, RT_EHIP = 0, -1, -2, -3, -4, -5
 RT_PRODUCT_FFT, RT_PRODUCT_SPLIT_FFT, RT_PRODUCT_LEAN_SPLIT_FFT = 0, 1, 2
-RT_C128, RT_C64, RT_MIXED = 0, 1, 2
+RT_C128, RT_C64, RT_MIXED, RT_C64_FP64_BASIS = 0, 1, 2, 3
 RT_MODIFIER_NONE, RT_MODIFIER_REAL, RT_MODIFIER_IMAG = 0, 1, 2
 RT_PRECOND_JACOBI, RT_PRECOND_NEARFIELD = 1, 2
 RT_METHOD_CG, RT_METHOD_GMRES = 0, 1

@@ -116,6 +116,10 @@ SolveOptions options_of(const rt_solve_opts *o) {
     s.refine = o->refine;
     s.inner_tolerance = o->inner_tolerance;
     s.max_outer = o->max_outer > 0 ? o->max_outer : 20;
+    if (s.precision == RT_C64_FP64_BASIS) {
+        s.precision = RT_C64;
+        s.fp64_basis = true;
+    }
     if (s.method != RT_METHOD_CG && s.method != RT_METHOD_GMRES)
         fail(RT_EINVAL, "MATLAB:validators:mustBeMember", "The method is 'cg' or 'gmres'.");
     if (!(s.tolerance > 0 && s.tolerance < 1))

@@ -1666,7 +1666,8 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
                    const std::vector<char> &skip_sys, const std::vector<int> &limits, int stall) {
         if (o.method == RT_METHOD_CG) return cocg(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, stall);
         // A complex64 inner GMRES keeps its basis in complex64: half the basis traffic of the Gram-Schmidt passes.
-        if (inner_prec == RT_C64) return gmres<c64>(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
+        if (inner_prec == RT_C64 && !o.fp64_basis)
+            return gmres<c64>(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
         return gmres<c128>(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
     };
     if (!any) {

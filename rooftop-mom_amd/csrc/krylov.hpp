@@ -20,6 +20,7 @@ struct SolveOptions {
     int refine = 0;
     double inner_tolerance = 0;
     int max_outer = 20;
+    bool fp64_basis = false;   // a complex64 inner GMRES keeps its basis in complex128 (RT_C64_FP64_BASIS)
 };
 
 struct SolveResult {

@@ -110,7 +110,7 @@ static bool batchable(const ServeJob &a, const ServeJob &b) {
            a.preconditioner == b.preconditioner && a.preconditioner != RT_PRECOND_JACOBI &&
            p.method == q.method && p.tolerance == q.tolerance && p.max_iterations == q.max_iterations &&
            p.restart == q.restart && p.precision == q.precision && p.refine == q.refine &&
-           p.inner_tolerance == q.inner_tolerance && p.max_outer == q.max_outer;
+           p.inner_tolerance == q.inner_tolerance && p.max_outer == q.max_outer && p.fp64_basis == q.fp64_basis;
 }
 
 // The device bytes of a group: every request as if on the group's padded grid, and the stacked copies of the spectra
@@ -438,6 +438,10 @@ int rt_server_submit(rt_server *h, const rt_request *r, int64_t *ticket) {
         o.refine = r->opts.refine;
         o.inner_tolerance = r->opts.inner_tolerance;
         o.max_outer = r->opts.max_outer > 0 ? r->opts.max_outer : 20;
+        if (o.precision == RT_C64_FP64_BASIS) {
+            o.precision = RT_C64;
+            o.fp64_basis = true;
+        }
         if (o.method != RT_METHOD_CG && o.method != RT_METHOD_GMRES)
             fail(RT_EINVAL, "MATLAB:validators:mustBeMember", "The method is 'cg' or 'gmres'.");
         if (!(o.tolerance > 0 && o.tolerance < 1) || o.max_iterations < 1)
@@ -479,7 +483,7 @@ int rt_request_bytes(const rt_request *r, uint64_t *bytes) {
     o.method = r->opts.method;
     o.restart = r->opts.restart;
     o.max_iterations = r->opts.max_iterations > 0 ? r->opts.max_iterations : 1000;
-    o.precision = r->opts.precision;
+    o.precision = r->opts.precision == RT_C64_FP64_BASIS ? RT_C64 : r->opts.precision;
     o.refine = r->opts.refine;
     *bytes = estimate_request_bytes(r->count_x, r->count_y, o, r->preconditioner);
     return RT_OK;

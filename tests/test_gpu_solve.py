@@ -426,3 +426,18 @@ def test_gmres_long_restarts_match_oracle(rt, restart):
         alone = solver.solve(op, B[:, q], preconditioner=pc)
         assert batch[q].converged() and batch[q].iterationCount() == alone.iterationCount()
         assert rel(batch[q].solution(), alone.solution()) < 1e-12
+
+
+def test_mixed_gmres_with_complex128_basis(rt):
+    """precision='c64' with basis='c128' (RT_C64_FP64_BASIS): complex64 products, the Gram-Schmidt basis in FP64;
+    it reaches the tolerance by the dense matrix and agrees with the complex64-basis and the FP64 solves."""
+    p, grid, gen = _setup(rt, 24, 20, 0.1, 0.1, (8, 4))
+    Z = S.dense_matrix(gen, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0, 0, -1], [1, 0, 0], 1.0, order=4)
+    op = rt.Operator(p, precision='mixed')
+    pc = rt.Preconditioner(p, 'nearField', precision='mixed')
+    wide = rt.Solver('gmres', tolerance=1e-10, precision='c64', basis='c128').solve(op, b, preconditioner=pc)
+    narrow = rt.Solver('gmres', tolerance=1e-10, precision='c64').solve(op, b, preconditioner=pc)
+    fp64 = rt.Solver('gmres', tolerance=1e-10, refine=True).solve(op, b, preconditioner=pc)
+    assert wide.converged() and rel(Z @ wide.solution(), b) < 1e-10
+    assert rel(wide.solution(), fp64.solution()) < 1e-8 and rel(narrow.solution(), fp64.solution()) < 1e-8
