@@ -69,6 +69,9 @@ extern "C" {
 /* Methods (rooftopMoM.Solver 'method', Solver.m:27-32). */
 #define RT_METHOD_CG 0
 #define RT_METHOD_GMRES 1
+/* Block COCG (no counterpart in the reference, which solves one right-hand side at a time): the nrhs <= 64
+ * right-hand sides of rt_solve share one block Krylov space; a system that converges leaves the block. */
+#define RT_METHOD_BLOCK_CG 2
 
 typedef struct rt_ctx rt_ctx;
 typedef struct rt_problem rt_problem;

@@ -57,6 +57,99 @@ def cocg(apply, b, x0, tol, max_iter, precondition=None):
     return x, bool(converged), np.array(norms)
 
 
+def block_orth(W, tau):
+    """A basis of the columns of W (n x s): the Cholesky factor L of the Gram matrix W^H W, taken column by column,
+    keeps column j when what is left of it after the earlier kept columns exceeds tau^2 of its own squared norm (the
+    rank-revealing drop), and P = W C with C = L^-H on the kept columns, zero elsewhere. Returns (P, C, kept).
+    Pinned: P^H P = I on the kept columns and W in the span of P, on full-rank and exactly rank-deficient blocks."""
+    s = W.shape[1]
+    G = W.conj().T @ W
+    L = np.zeros((s, s), dtype=np.complex128)
+    kept = np.zeros(s, dtype=bool)
+    for j in range(s):
+        d = G[j, j].real - np.sum(np.abs(L[j, :j]) ** 2)
+        if np.isfinite(d) and d > tau * tau * G[j, j].real:
+            kept[j] = True
+            L[j, j] = np.sqrt(d)
+            for i in range(j + 1, s):
+                L[i, j] = (G[i, j] - np.sum(L[i, :j] * L[j, :j].conj())) / L[j, j]
+    C = np.zeros((s, s), dtype=np.complex128)
+    k = np.flatnonzero(kept)
+    if k.size:
+        C[np.ix_(k, k)] = np.linalg.inv(L[np.ix_(k, k)].conj().T)
+    return W @ C, C, kept
+
+
+BLOCK_DROP = 1e-6   # the rank-revealing threshold of the direction block
+
+
+def block_cocg(apply, B, tol, max_iter, precondition=None, tau=BLOCK_DROP):
+    """Block COCG on the columns of B (n x s), the complex symmetric form of block conjugate gradients with the
+    unconjugated products of COCG (ConjugateGradient.m:32-59 per column), in the breakdown-free arrangement of Ji and
+    Li (2017): the direction block P is an orthonormal basis (block_orth) of Z + P beta, and
+        Q = A P, D = P^T Q, alpha = D^-1 P^T R, X += P alpha, R -= Q alpha,
+        Z = M R, beta = -D^-1 Q^T Z, P = orth(Z + P beta),
+    so a right-hand side that converges, or a block that loses rank, drops directions rather than making a
+    coefficient matrix singular. Column j stops (its solution and record frozen) once its monitored norm(r_j) /
+    norm(b_j) is below tol, or at max_iter; its residual stays in the block. A singular D (a zero pivot on the kept
+    directions) ends every running column with a NaN norm, as COCG's breakdown (ConjugateGradient.m:56-59).
+    No reference counterpart (the reference solves one right-hand side at a time); pinned by: one column is COCG,
+    finite termination within ceil(n / s) iterations on a well-conditioned system, backslash, and block_orth.
+    Returns (X, converged per column, iterations per column, monitored norms per column).
+    """
+    B = np.asarray(B, dtype=np.complex128)
+    n, s = B.shape
+    apply_block = lambda P: np.stack([apply(P[:, j]) for j in range(P.shape[1])], axis=1)
+    prec_block = (lambda R: R.copy()) if precondition is None else (
+        lambda R: np.stack([precondition(R[:, j]) for j in range(R.shape[1])], axis=1))
+    X = np.zeros_like(B)
+    R = B.copy()
+    bnorm = np.linalg.norm(B, axis=0)
+    live = bnorm > 0
+    R[:, ~live] = 0
+    norms = [[np.linalg.norm(R[:, j]) / bnorm[j] if live[j] else 0.0] for j in range(s)]
+    conv = np.array([not live[j] or norms[j][0] < tol for j in range(s)])
+    running = ~conv
+    iters = np.zeros(s, dtype=int)
+    P, _, kept = block_orth(prec_block(R), tau)
+    it = 0
+    while running.any() and it < max_iter and kept.any():
+        it += 1
+        Q = apply_block(P)
+        D = P.T @ Q
+        k = np.flatnonzero(kept)
+        Dk = D[np.ix_(k, k)]
+        try:
+            singular = not np.all(np.isfinite(Dk)) or np.linalg.cond(Dk) > 1 / np.finfo(float).eps
+        except np.linalg.LinAlgError:
+            singular = True
+        if singular:
+            for j in np.flatnonzero(running):
+                norms[j].append(np.nan)
+                iters[j] = it
+            running[:] = False
+            break
+        alpha = np.zeros((s, s), dtype=np.complex128)
+        alpha[k] = np.linalg.solve(Dk, (P.T @ R)[k])
+        X[:, running] += (P @ alpha)[:, running]
+        R -= Q @ alpha
+        for j in np.flatnonzero(running):
+            norms[j].append(np.linalg.norm(R[:, j]) / bnorm[j])
+            iters[j] = it
+            if norms[j][-1] < tol:
+                conv[j] = True
+                running[j] = False
+            elif not np.isfinite(norms[j][-1]) or it >= max_iter:
+                running[j] = False
+        if not running.any():
+            break
+        Z = prec_block(R)
+        beta = np.zeros((s, s), dtype=np.complex128)
+        beta[k] = -np.linalg.solve(Dk, (Q.T @ Z)[k])
+        P, _, kept = block_orth(Z + P @ beta, tau)
+    return X, conv, iters, [np.array(v) for v in norms]
+
+
 def givens(a, b):
     """Complex Givens [conj(c), conj(s); -s, c] mapping [a; b] to [norm; 0].
     @GeneralizedMinimumResidual/private/generateRotation_.m:11-18."""

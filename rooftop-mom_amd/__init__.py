@@ -667,8 +667,9 @@ class Solver:
 
     def __init__(self, method='cg', tolerance=1e-6, maxIterationCount=1000, restart=50, precision='c128',
                  refine=False, innerTolerance=0.0, maxOuter=20, basis=None):
-        if method not in ('cg', 'gmres'):
-            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember', "'method' must be 'cg' or 'gmres'.")
+        if method not in ('cg', 'gmres', 'blockcg'):
+            raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeMember',
+                               "'method' must be 'cg', 'gmres' or 'blockcg'.")
         if not (0 < tolerance < 1):
             raise RooftopError(_abi.RT_EINVAL, 'MATLAB:validators:mustBeLessThan', "'tolerance' must be below 1.")
         if not (restart == math.inf or (restart > 0 and restart == int(restart))):
@@ -687,7 +688,8 @@ class Solver:
 
     def options(self):
         o = _abi.rt_solve_opts()
-        o.method = _abi.RT_METHOD_CG if self.method == 'cg' else _abi.RT_METHOD_GMRES
+        o.method = {'cg': _abi.RT_METHOD_CG, 'gmres': _abi.RT_METHOD_GMRES,
+                    'blockcg': _abi.RT_METHOD_BLOCK_CG}[self.method]
         o.tolerance = self.tolerance
         o.max_iterations = self.maxIterationCount
         o.restart = 0 if self.restart == math.inf else int(self.restart)

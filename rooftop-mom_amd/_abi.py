@@ -11,7 +11,7 @@ RT_PRODUCT_FFT, RT_PRODUCT_SPLIT_FFT, RT_PRODUCT_LEAN_SPLIT_FFT = 0, 1, 2
 RT_C128, RT_C64, RT_MIXED, RT_C64_FP64_BASIS = 0, 1, 2, 3
 RT_MODIFIER_NONE, RT_MODIFIER_REAL, RT_MODIFIER_IMAG = 0, 1, 2
 RT_PRECOND_JACOBI, RT_PRECOND_NEARFIELD = 1, 2
-RT_METHOD_CG, RT_METHOD_GMRES = 0, 1
+RT_METHOD_CG, RT_METHOD_GMRES, RT_METHOD_BLOCK_CG = 0, 1, 2
 
 
 class rt_solve_opts(C.Structure):

@@ -120,8 +120,8 @@ SolveOptions options_of(const rt_solve_opts *o) {
         s.precision = RT_C64;
         s.fp64_basis = true;
     }
-    if (s.method != RT_METHOD_CG && s.method != RT_METHOD_GMRES)
-        fail(RT_EINVAL, "MATLAB:validators:mustBeMember", "The method is 'cg' or 'gmres'.");
+    if (s.method != RT_METHOD_CG && s.method != RT_METHOD_GMRES && s.method != RT_METHOD_BLOCK_CG)
+        fail(RT_EINVAL, "MATLAB:validators:mustBeMember", "The method is 'cg', 'gmres' or 'blockcg'.");
     if (!(s.tolerance > 0 && s.tolerance < 1))
         fail(RT_EINVAL, "MATLAB:validators:mustBeLessThan", "The tolerance is a real number in (0, 1).");
     if (s.max_iterations <= 0)

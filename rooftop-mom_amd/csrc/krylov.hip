@@ -57,7 +57,7 @@ enum ScalSlot {
 enum PoolSlot {
     kSlotPartial, kSlotScal, kSlotState, kSlotHistory, kSlotV, kSlotW, kSlotT, kSlotR, kSlotZ, kSlotD, kSlotAd,
     kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags, kSlotP, kSlotHraw,
-    kSlotPartialL, kSlotHc, kSlotY
+    kSlotPartialL, kSlotHc, kSlotY, kSlotBlockCoef, kSlotBlockPartial
 };
 
 // Restarts up to this length run the delayed-reorthogonalization kernels, whose Gram-Schmidt coefficients, rotations
@@ -268,11 +268,14 @@ __global__ void __launch_bounds__(kThreads) k_cocg_update(c128 *__restrict__ x, 
 // The residual norm of iteration `it`, the stopping test (ConjugateGradient.m:126-136) with the system's own
 // iteration limit. stall > 0 (inner solves of the refinement loop only) also stops a system whose monitored norm has
 // not improved on its best for `stall` iterations: a complex64 short recurrence that has lost its complex symmetry to
-// rounding, which the outer FP64 loop then restarts from the true residual. Grid (1, R).
+// rounding, which the outer FP64 loop then restarts from the true residual. Grid (1, R). In a block loop (block = 1)
+// a system that stops is marked finished only: its residual stays in the block's search space, and it is skipped here
+// from then on.
 __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restrict__ partial, int blocks,
                                                             c128 *__restrict__ scal, int it, int stall,
-                                                            LoopState *st, double *history, int cap) {
+                                                            LoopState *st, double *history, int cap, int block) {
     SYSTEM(r);
+    if (block && st[r].finished) return;   // uniform over the block, before its first barrier
     __shared__ double sh[8];
     // Every load is issued before any is used, from addresses clamped into range (a load under a branch is waited
     // for inside it): the scalars, then the partials in rounds of eight per thread, summed in the order of the blocks.
@@ -300,12 +303,285 @@ __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restri
         sc[kBest] = mk(res, 0.0);
         st[r].best = it;
     }
+    const int stop = block ? 0 : 1;
     if (res < tol) {
-        st[r].converged = st[r].finished = st[r].done = 1;
+        st[r].converged = st[r].finished = 1;
+        st[r].done = stop;
     } else if (!isfinite(res)) {
-        st[r].finished = st[r].done = 1;
+        st[r].finished = 1;
+        st[r].done = stop;
     } else if (it >= static_cast<int>(maxit) || (stall > 0 && it - st[r].best >= stall)) {
+        if (block) st[r].finished = 1;
+        else st[r].done = 1;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// Block COCG kernels: the R <= 64 right-hand sides of one complex symmetric system share their search space. The
+// block vectors are the R systems' vectors (system-major, [r][n]); the R x R coefficient matrices are column-major
+// with leading dimension kBlockMax, element (i, j) at [j * kBlockMax + i]. keep[j].x = 1 marks the directions kept
+// by the last orthonormalization, 0 the dropped ones (zero columns of the direction block).
+// ------------------------------------------------------------------------------------------------------------------
+constexpr int kBlockMax = 64;   // right-hand sides of one block solve
+constexpr int kBtRows = 16;     // rows of one staged tile of the transposed product
+constexpr int kBlockLd = kBlockMax + 1;
+constexpr size_t kBlockLds = 2 * kBlockMax * kBlockLd * sizeof(c128);   // two padded s x s matrices
+
+// partial[blk] (element (i, j)) = sum over block blk's rows of U_i[k] V_j[k] (CONJ: conj(U_i[k]) V_j[k]), i, j < s:
+// a tile of 16 rows of all s columns of U and V staged in LDS (coalesced runs of 16 rows per column), each thread
+// accumulating 4 x 4 entries.
+template <bool CONJ>
+__global__ void __launch_bounds__(kThreads) k_block_tn(const c128 *__restrict__ U, const c128 *__restrict__ V,
+                                                       int64_t n, int s, int64_t chunk, c128 *__restrict__ partial) {
+    __shared__ c128 us[kBlockMax][kBtRows + 1], vs[kBlockMax][kBtRows + 1];
+    const int t = threadIdx.x;
+    const int ti = t >> 4, tj = t & 15;   // entries (4 ti + a, 4 tj + b)
+    c128 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = mk(0.0, 0.0);
+    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    for (int64_t k0 = base; k0 < end; k0 += kBtRows) {
+        __syncthreads();   // the previous tile is consumed
+#pragma unroll
+        for (int u = 0; u < kBlockMax * kBtRows / kThreads; ++u) {
+            const int e = t + u * kThreads, i = e / kBtRows, kk = e % kBtRows;
+            const int64_t k = k0 + kk;
+            const bool in = i < s && k < end;
+            const int64_t at = static_cast<int64_t>(min(i, s - 1)) * n + min(k, end - 1);
+            const c128 uv = U[at], vv = V[at];
+            us[i][kk] = pick(in, uv, mk(0.0, 0.0));
+            vs[i][kk] = pick(in, vv, mk(0.0, 0.0));
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kBtRows; ++kk) {
+            c128 a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = us[4 * ti + q][kk];
+                b[q] = vs[4 * tj + q][kk];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] = acc[p][q] + (CONJ ? cmulc(a[p], b[q]) : a[p] * b[q]);
+        }
+    }
+    c128 *out = partial + static_cast<int64_t>(blockIdx.x) * kBlockMax * kBlockMax;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[(4 * tj + q) * kBlockMax + 4 * ti + p] = acc[p][q];
+}
+
+// C = sum over the blocks of partial, in the blocks' order.
+__global__ void k_block_sum(const c128 *__restrict__ partial, int blocks, c128 *__restrict__ C) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kBlockMax * kBlockMax) return;
+    c128 acc = mk(0.0, 0.0);
+    for (int b = 0; b < blocks; ++b) acc = acc + partial[static_cast<int64_t>(b) * kBlockMax * kBlockMax + e];
+    C[e] = acc;
+}
+
+// Every system still running ends (finished, and done for every system): with it >= 0 iteration it counts with a NaN
+// monitored norm, as COCG's Inf or NaN residual ends its loop (ConjugateGradient.m:56-59). Thread-strided, s <= R.
+__device__ void block_stop_all(LoopState *st, int s, int it, double *history, int cap) {
+    for (int r = threadIdx.x; r < s; r += blockDim.x) {
+        if (!st[r].finished) {
+            if (it >= 0) {
+                history[static_cast<int64_t>(r) * cap + it] = __builtin_nan("");
+                st[r].iterations = it;
+            }
+            st[r].finished = 1;
+        }
         st[r].done = 1;
+    }
+}
+
+// X = M^-1 B on the kept directions: M's rows and columns of dropped directions are the identity's and B's rows there
+// zero, so X is zero in those rows; every column of B (one per system) is solved. Gauss-Jordan with partial pivoting
+// in LDS (kBlockLds dynamic), one block. A zero or non-finite pivot is a block breakdown: block_stop_all (it >= 0
+// for the solve for alpha of iteration it).
+__global__ void __launch_bounds__(kThreads) k_block_solve(const c128 *__restrict__ M, const c128 *__restrict__ B,
+                                                          c128 *__restrict__ X, const c128 *__restrict__ keep, int s,
+                                                          LoopState *st, int it, double *history, int cap) {
+    extern __shared__ unsigned char solve_lds[];
+    constexpr int ld = kBlockLd;
+    c128 *m = reinterpret_cast<c128 *>(solve_lds), *b = m + kBlockMax * ld;   // row-major [row][col]
+    __shared__ int piv;
+    __shared__ int broken;
+    const int t = threadIdx.x;
+    for (int e = t; e < s * s; e += kThreads) {
+        const int i = e % s, j = e / s;   // row i, column j (column-major input)
+        const bool ki = keep[i].x > 0.5, kj = keep[j].x > 0.5;
+        m[i * ld + j] = ki && kj ? M[j * kBlockMax + i] : mk(i == j ? 1.0 : 0.0, 0.0);
+        b[i * ld + j] = ki ? B[j * kBlockMax + i] : mk(0.0, 0.0);
+    }
+    if (t == 0) broken = 0;
+    __syncthreads();
+    for (int k = 0; k < s; ++k) {
+        if (t == 0) {
+            int p = k;
+            double best = abs2(m[k * ld + k]);
+            for (int i = k + 1; i < s; ++i) {
+                const double v = abs2(m[i * ld + k]);
+                if (v > best) {
+                    best = v;
+                    p = i;
+                }
+            }
+            piv = p;
+            if (!(best > 0) || !isfinite(best)) broken = 1;
+        }
+        __syncthreads();
+        if (broken) break;   // uniform: read after the barrier
+        const int p = piv;
+        if (p != k)
+            for (int j = t; j < s; j += kThreads) {
+                const c128 a = m[k * ld + j], c = b[k * ld + j];
+                m[k * ld + j] = m[p * ld + j];
+                b[k * ld + j] = b[p * ld + j];
+                m[p * ld + j] = a;
+                b[p * ld + j] = c;
+            }
+        __syncthreads();
+        const c128 d = m[k * ld + k];
+        const c128 inv = (1.0 / abs2(d)) * conjg(d);
+        __syncthreads();
+        for (int j = t; j < s; j += kThreads) {
+            m[k * ld + j] = inv * m[k * ld + j];
+            b[k * ld + j] = inv * b[k * ld + j];
+        }
+        __syncthreads();
+        for (int e = t; e < s * s; e += kThreads) {   // every other row minus its multiple of row k
+            const int i = e / s, j = e % s;
+            if (i == k) continue;
+            const c128 f = m[i * ld + k];
+            if (j != k) m[i * ld + j] = m[i * ld + j] - f * m[k * ld + j];
+            b[i * ld + j] = b[i * ld + j] - f * b[k * ld + j];
+        }
+        __syncthreads();
+        for (int i = t; i < s; i += kThreads)
+            if (i != k) m[i * ld + k] = mk(0.0, 0.0);
+        __syncthreads();
+    }
+    if (broken) {
+        block_stop_all(st, s, it, history, cap);
+        return;
+    }
+    for (int e = t; e < s * s; e += kThreads) {
+        const int i = e % s, j = e / s;
+        X[j * kBlockMax + i] = b[i * ld + j];
+    }
+}
+
+// The rank-revealing orthonormalization of a block W from its Gram matrix G = W^H W: the Cholesky factor L (G = L
+// L^H), column by column, keeps column j when what is left of it after the earlier kept columns, d = G_jj - sum
+// |L_jk|^2, exceeds tau^2 G_jj; C = L^-H on the kept columns (upper triangular, zero elsewhere), so that W C is
+// orthonormal there and zero in the dropped columns. No kept column ends every running system (nothing is left to
+// search with). One block; L and C in LDS (kBlockLds dynamic).
+__global__ void __launch_bounds__(kThreads) k_block_orth(const c128 *__restrict__ G, c128 *__restrict__ C,
+                                                         c128 *__restrict__ keep, int s, double tau, LoopState *st,
+                                                         double *history, int cap) {
+    extern __shared__ unsigned char orth_lds[];
+    constexpr int ld = kBlockLd;
+    c128 *L = reinterpret_cast<c128 *>(orth_lds), *Cs = L + kBlockMax * ld;   // row-major [i][j]
+    __shared__ double diag[kBlockMax];
+    __shared__ int kept[kBlockMax];
+    __shared__ int count;
+    const int t = threadIdx.x;
+    for (int e = t; e < s * s; e += kThreads) {
+        const int i = e % s, j = e / s;
+        L[i * ld + j] = G[j * kBlockMax + i];
+    }
+    if (t == 0) count = 0;
+    __syncthreads();
+    for (int j = 0; j < s; ++j) {
+        if (t == 0) {
+            const double g = L[j * ld + j].x;
+            double d = g;
+            for (int k = 0; k < j; ++k)
+                if (kept[k]) d -= abs2(L[j * ld + k]);
+            const int kj = isfinite(d) && d > tau * tau * g;
+            kept[j] = kj;
+            diag[j] = kj ? sqrt(d) : 0.0;
+            count += kj;
+        }
+        __syncthreads();
+        if (kept[j]) {
+            const double inv = 1.0 / diag[j];
+            for (int i = j + 1 + t; i < s; i += kThreads) {
+                c128 v = L[i * ld + j];
+                for (int k = 0; k < j; ++k)
+                    if (kept[k]) v = v - L[i * ld + k] * conjg(L[j * ld + k]);
+                L[i * ld + j] = inv * v;
+            }
+        }
+        __syncthreads();
+    }
+    if (count == 0) {
+        block_stop_all(st, s, -1, history, cap);
+        return;
+    }
+    // Column j of C solves L^H c = e_j (back substitution over the kept rows), one thread per column.
+    if (t < s) {
+        const int j = t;
+        for (int i = 0; i < s; ++i) Cs[i * ld + j] = mk(0.0, 0.0);
+        if (kept[j]) {
+            Cs[j * ld + j] = mk(1.0 / diag[j], 0.0);
+            for (int i = j - 1; i >= 0; --i) {
+                if (!kept[i]) continue;
+                c128 v = mk(0.0, 0.0);
+                for (int k = i + 1; k <= j; ++k)
+                    if (kept[k]) v = v + conjg(L[k * ld + i]) * Cs[k * ld + j];
+                Cs[i * ld + j] = (-1.0 / diag[i]) * v;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < s * s; e += kThreads) {
+        const int i = e % s, j = e / s;
+        C[j * kBlockMax + i] = Cs[i * ld + j];
+    }
+    for (int j = t; j < s; j += kThreads) keep[j] = mk(kept[j] ? 1.0 : 0.0, 0.0);
+}
+
+// Y_j = [Y_j +] sign sum_i W_i C(i, j) for the columns j the mode selects (kBlockAll: every column; kBlockRunning:
+// the systems not done; kBlockSearching: not done and not finished), a thread per row, C in LDS a quarter at a time.
+enum BlockColumns { kBlockAll = 0, kBlockRunning = 1, kBlockSearching = 2 };
+__global__ void __launch_bounds__(kThreads) k_block_nn(c128 *__restrict__ Y, const c128 *__restrict__ W,
+                                                       const c128 *__restrict__ C, double sign, int64_t n, int s,
+                                                       const LoopState *__restrict__ st, int mode, int accumulate) {
+    __shared__ c128 cs[kBlockMax * kBlockMax / 4];   // 16 columns j of C
+    const int t = threadIdx.x;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * kThreads + t;
+    const int64_t rr = min(row, n - 1);
+    for (int j0 = 0; j0 < s; j0 += 16) {
+        __syncthreads();
+        for (int e = t; e < 16 * kBlockMax; e += kThreads) {
+            const int jj = e / kBlockMax, i = e % kBlockMax;
+            cs[e] = j0 + jj < s ? C[(j0 + jj) * kBlockMax + i] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        c128 acc[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = mk(0.0, 0.0);
+        for (int i = 0; i < s; ++i) {
+            const c128 w = W[static_cast<int64_t>(i) * n + rr];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] = acc[q] + w * cs[q * kBlockMax + i];
+        }
+        if (row < n)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int j = j0 + q;
+                if (j >= s) continue;
+                if (mode != kBlockAll && (st[j].done || (mode == kBlockSearching && st[j].finished))) continue;
+                c128 *y = Y + static_cast<int64_t>(j) * n + row;
+                *y = accumulate ? *y + sign * acc[q] : sign * acc[q];
+            }
     }
 }
 
@@ -1379,8 +1655,135 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
                                i_new, n, ws.chunk, ws.partial, st);
             RT_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_cocg_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
-                               it, stall, ws.state, ws.history, ws.cap);
+                               it, stall, ws.state, ws.history, ws.cap, 0);
             RT_LAUNCH_CHECK();
+        }
+        state = ws.read_state();
+        ws.set_scale(state);
+    }
+    ctx->timed_scale = 1.0;
+    return collect(ws, state, skip_sys, r0, b0);
+}
+
+// ------------------------------------------------------------------------------------------------------------------
+// Block COCG: the R right-hand sides of one complex symmetric system (A^T = A, a symmetric preconditioner M) share
+// one block Krylov space: block conjugate gradients with the unconjugated products of ConjugateGradient.m:32-59, in
+// the breakdown-free arrangement of Ji and Li (2017), whose direction block P is an orthonormal basis of Z + P beta:
+//   Q = A P, D = P^T Q, alpha = D^-1 P^T R, X += P alpha, R -= Q alpha,
+//   Z = M R, beta = -D^-1 Q^T Z, P = orth(Z + P beta)   (rank-revealing: dependent directions are dropped).
+// A system whose monitored residual norm(r_j) / norm(b_j) falls below its tolerance, or that reaches its iteration
+// limit, stops there (its solution and record frozen) while its residual stays in the block; a right-hand side that
+// converges, or a block that loses rank, then drops directions instead of making a coefficient matrix singular.
+// ------------------------------------------------------------------------------------------------------------------
+constexpr double kBlockDrop = 1e-6;   // rank-revealing threshold of the direction block
+
+static bool all_finished(const std::vector<LoopState> &st) {
+    for (const auto &x : st)
+        if (!x.finished && !x.done) return false;
+    return true;
+}
+
+static InnerResult block_cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c128 *b, c128 *x, bool has_x0,
+                              const std::vector<double> &tol, const std::vector<char> &skip_sys,
+                              const std::vector<int> &limits, int stall) {
+    Context *ctx = ws.ctx;
+    const int64_t n = ws.n;
+    const int R = ws.R;
+    if (R > kBlockMax)
+        fail(RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide",
+             "Block COCG solves at most %d right-hand sides together.", kBlockMax);
+    LoopState *st = ws.state;
+    c128 *r = vec(ctx, kSlotR, R * n), *z = vec(ctx, kSlotZ, R * n), *pd = vec(ctx, kSlotD, R * n);
+    c128 *q = vec(ctx, kSlotAd, R * n);
+    constexpr int kSq = kBlockMax * kBlockMax;
+    c128 *coef = vec(ctx, kSlotBlockCoef, 8 * kSq);
+    c128 *D = coef, *T = coef + kSq, *alpha = coef + 2 * kSq, *E = coef + 3 * kSq, *beta = coef + 4 * kSq;
+    c128 *G = coef + 5 * kSq, *Cm = coef + 6 * kSq, *keep = coef + 7 * kSq;
+    const int tn_blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(512, (n + kBtRows - 1) / kBtRows)));
+    const int64_t tn_chunk = ((n + tn_blocks - 1) / tn_blocks + kBtRows - 1) / kBtRows * kBtRows;
+    c128 *tn_partial = vec(ctx, kSlotBlockPartial, static_cast<int64_t>(tn_blocks) * kSq);
+    raise_lds(reinterpret_cast<const void *>(&k_block_solve), kBlockLds);
+    raise_lds(reinterpret_cast<const void *>(&k_block_orth), kBlockLds);
+    auto transposed = [&](const c128 *U, const c128 *V, c128 *C, bool conj) {   // C = U^T V (conj: U^H V)
+        if (conj)
+            hipLaunchKernelGGL(k_block_tn<true>, dim3(tn_blocks), dim3(kThreads), 0, ws.s, U, V, n, R, tn_chunk,
+                               tn_partial);
+        else
+            hipLaunchKernelGGL(k_block_tn<false>, dim3(tn_blocks), dim3(kThreads), 0, ws.s, U, V, n, R, tn_chunk,
+                               tn_partial);
+        RT_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_block_sum, dim3(kSq / kThreads), dim3(kThreads), 0, ws.s, tn_partial, tn_blocks, C);
+        RT_LAUNCH_CHECK();
+    };
+    auto solve = [&](const c128 *M, const c128 *B, c128 *X, int at) {   // X = M^-1 B on the kept directions
+        hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(kThreads), static_cast<uint32_t>(kBlockLds), ws.s, M, B, X,
+                           keep, R, st, at, ws.history, ws.cap);
+        RT_LAUNCH_CHECK();
+    };
+    auto combine = [&](c128 *Y, const c128 *W, const c128 *C, double sign, int mode, int accumulate) {
+        hipLaunchKernelGGL(k_block_nn, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           ws.s, Y, W, C, sign, n, R, st, mode, accumulate);
+        RT_LAUNCH_CHECK();
+    };
+    auto orthonormalize = [&]() {   // P = orth(Z): Gram matrix, its rank-revealing factor, P = Z C
+        transposed(z, z, G, true);
+        hipLaunchKernelGGL(k_block_orth, dim3(1), dim3(kThreads), static_cast<uint32_t>(kBlockLds), ws.s, G, Cm,
+                           keep, R, kBlockDrop, st, ws.history, ws.cap);
+        RT_LAUNCH_CHECK();
+        combine(pd, z, Cm, 1.0, kBlockAll, 0);
+    };
+    auto precondition = [&](const c128 *in, c128 *out) {
+        if (P) (*P)(in, out, ws.skip());
+        else ws.copy(in, out, st);
+    };
+    ws.reset_state(skip_sys);
+    ws.put_re(kTol, tol);
+    ws.put_re(kBest, std::vector<double>(R, HUGE_VAL));
+    const int max_iter = put_limits(ws, limits, skip_sys);
+    // The systems left out (and those converged at the start) are zero columns of every block: the vector slots are
+    // shared with other loops, so they are cleared first.
+    for (c128 *v : {r, z, pd, q}) RT_HIP(hipMemsetAsync(v, 0, sizeof(c128) * R * n, ws.s));
+    if (has_x0) {
+        A(x, q, ws.skip());
+        ws.sub(b, q, r, st);
+        RT_HIP(hipMemsetAsync(q, 0, sizeof(c128) * R * n, ws.s));
+    } else {
+        ws.zero(x);
+        ws.copy(b, r, st);
+    }
+    ws.nrm2(r, kRnormCg, st);
+    ws.nrm2(b, kBnorm, st);
+    hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnormCg, kBnorm, R);
+    RT_LAUNCH_CHECK();
+    const std::vector<double> r0 = ws.fetch_re(kRnormCg), b0 = ws.fetch_re(kBnorm);
+    std::vector<LoopState> state = ws.read_state();
+    // A system converged at the start leaves the block entirely (done): its residual column is cleared.
+    for (int j = 0; j < R; ++j)
+        if (state[j].done && !skip_sys[j]) RT_HIP(hipMemsetAsync(r + j * n, 0, sizeof(c128) * n, ws.s));
+    precondition(r, z);
+    orthonormalize();
+    int it = 0;
+    state = ws.read_state();
+    while (it < max_iter && !all_finished(state)) {
+        const int window = std::min(kPoll, max_iter - it);
+        for (int w = 0; w < window; ++w) {
+            ++it;
+            A(pd, q, ws.skip());
+            transposed(pd, q, D, false);
+            transposed(pd, r, T, false);
+            solve(D, T, alpha, it);
+            combine(x, pd, alpha, 1.0, kBlockSearching, 1);
+            combine(r, q, alpha, -1.0, kBlockRunning, 1);
+            hipLaunchKernelGGL(k_nrm2, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, r, n, ws.chunk, ws.partial, st);
+            RT_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_cocg_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
+                               it, stall, ws.state, ws.history, ws.cap, 1);
+            RT_LAUNCH_CHECK();
+            precondition(r, z);
+            transposed(q, z, E, false);
+            solve(D, E, beta, -1);
+            combine(z, pd, beta, -1.0, kBlockRunning, 1);   // Z + P beta, beta = -D^-1 Q^T Z
+            orthonormalize();
         }
         state = ws.read_state();
         ws.set_scale(state);
@@ -1665,6 +2068,8 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
     auto run = [&](const c128 *rhs, c128 *sol, bool has_x0, const std::vector<double> &tol,
                    const std::vector<char> &skip_sys, const std::vector<int> &limits, int stall) {
         if (o.method == RT_METHOD_CG) return cocg(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, stall);
+        if (o.method == RT_METHOD_BLOCK_CG)
+            return block_cocg(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, stall);
         // A complex64 inner GMRES keeps its basis in complex64: half the basis traffic of the Gram-Schmidt passes.
         if (inner_prec == RT_C64 && !o.fp64_basis)
             return gmres<c64>(ws, A, P, rhs, sol, has_x0, tol, skip_sys, limits, o.restart);
@@ -1705,10 +2110,11 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
         // 1e-4 at most).
         double inner_tol = o.tolerance;
         if (o.inner_tolerance > 0) inner_tol = o.inner_tolerance;
-        const bool fallback = kCocgFallback && o.method == RT_METHOD_CG && o.precision == RT_C64 &&
+        const bool cg_like = o.method == RT_METHOD_CG || o.method == RT_METHOD_BLOCK_CG;
+        const bool fallback = kCocgFallback && cg_like && o.precision == RT_C64 &&
                               op.has(RT_C128) && (!pc || pc->has(RT_C128));
         if (o.inner_tolerance <= 0 && o.precision == RT_C64)
-            inner_tol = std::max(o.tolerance, o.method == RT_METHOD_CG ? (fallback ? kCocgInner : 1e-4) : 1e-6);
+            inner_tol = std::max(o.tolerance, cg_like ? (fallback ? kCocgInner : 1e-4) : 1e-6);
         std::vector<char> done(R);
         std::vector<int> total(R, 0);
         for (int r = 0; r < R; ++r) {
@@ -1716,7 +2122,7 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
             out[r].norms.push_back(rel[r]);
         }
         // An inner complex64 COCG that stagnates (no new best norm for kStall iterations) hands back to this loop.
-        int stall = o.method == RT_METHOD_CG && o.precision == RT_C64 ? kStall : 0;
+        int stall = cg_like && o.precision == RT_C64 ? kStall : 0;
         int outer = 0;
         while (outer < o.max_outer) {
             bool active = false;

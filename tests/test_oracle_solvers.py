@@ -165,3 +165,68 @@ def test_near_field_by_class_equals_per_rooftop_fit():
         a = NF.sai_matrix(g, grid, *args).toarray()
         b = NF.sai_matrix_by_class(g, grid, *args).toarray()
         assert np.abs(a - b).max() < 1e-11 * np.abs(a).max()
+
+
+def test_block_cocg_of_one_column_is_cocg():
+    m, b = symmetric_system(40)
+    X, conv, its, norms = SV.block_cocg(lambda v: m @ v, b[:, None], 1e-10, 100)
+    ref = SV.solve('cg', m, b, tol=1e-10, max_iter=100)
+    assert conv[0] and its[0] == ref['iterationCount']
+    np.testing.assert_allclose(norms[0], ref['residualNorms'], rtol=1e-8)
+    assert rel(X[:, 0], ref['solution']) < 1e-10
+
+
+def test_block_cocg_terminates_within_n_over_s_iterations():
+    """A block Krylov space of s columns gains s dimensions per iteration: on the 40 x 40 well-conditioned complex
+    symmetric system, 8 right-hand sides converge to 1e-12 within 40 / 8 = 5 iterations (plus rounding), against
+    backslash; COCG alone needs far more."""
+    m, _ = symmetric_system(40)
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((40, 8)) + 1j * rng.standard_normal((40, 8))
+    X, conv, its, norms = SV.block_cocg(lambda v: m @ v, B, 1e-12, 50)
+    assert conv.all() and its.max() <= 6
+    assert rel(X, np.linalg.solve(m, B)) < 1e-11
+    single = SV.solve('cg', m, B[:, 0], tol=1e-12, max_iter=100)
+    assert single['iterationCount'] > 2 * its.max()
+
+
+def test_block_cocg_with_preconditioner_matches_backslash():
+    m, _ = symmetric_system(60)
+    s = np.logspace(0, 1, 60)
+    m = s[:, None] * m * s[None, :]
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((60, 5)) + 1j * rng.standard_normal((60, 5))
+    d = np.diag(m).copy()
+    X, conv, its, norms = SV.block_cocg(lambda v: m @ v, B, 1e-10, 200, precondition=lambda v: v / d)
+    assert conv.all() and rel(X, np.linalg.solve(m, B)) < 1e-8
+
+
+def test_block_orth_basis_and_drop():
+    """block_orth: orthonormal kept columns spanning W; a zero column and a column that is a combination of earlier
+    kept ones (the last, in the order columns are taken) are dropped."""
+    rng = np.random.default_rng(6)
+    W = rng.standard_normal((50, 6)) + 1j * rng.standard_normal((50, 6))
+    P, C, kept = SV.block_orth(W, 1e-6)
+    assert kept.all()
+    np.testing.assert_allclose(P.conj().T @ P, np.eye(6), atol=1e-13)
+    assert rel(P @ (P.conj().T @ W), W) < 1e-13
+    W[:, 3] = 2 * W[:, 0] - 1j * W[:, 5]
+    W[:, 4] = 0
+    P, C, kept = SV.block_orth(W, 1e-6)
+    assert kept.tolist() == [True, True, True, True, False, False]
+    k = np.flatnonzero(kept)
+    np.testing.assert_allclose(P[:, k].conj().T @ P[:, k], np.eye(4), atol=1e-12)
+    assert np.all(P[:, ~kept] == 0) and rel(P @ (P.conj().T @ W), W) < 1e-12
+
+
+def test_block_cocg_deflates_a_dependent_block():
+    """Right-hand sides with a repeated column and a zero column: the block drops the dependent directions and every
+    column still matches backslash."""
+    m, _ = symmetric_system(40)
+    rng = np.random.default_rng(8)
+    B = rng.standard_normal((40, 5)) + 1j * rng.standard_normal((40, 5))
+    B[:, 3] = B[:, 1] * (1 - 2j)
+    B[:, 4] = 0
+    X, conv, its, norms = SV.block_cocg(lambda v: m @ v, B, 1e-12, 100)
+    assert conv.all() and its[4] == 0 and np.all(X[:, 4] == 0)
+    assert rel(X, np.linalg.solve(m, B)) < 1e-11
