@@ -60,8 +60,10 @@ def cocg(apply, b, x0, tol, max_iter, precondition=None):
 def block_orth(W, tau):
     """A basis of the columns of W (n x s): the Cholesky factor L of the Gram matrix W^H W, taken column by column,
     keeps column j when what is left of it after the earlier kept columns exceeds tau^2 of its own squared norm (the
-    rank-revealing drop), and P = W C with C = L^-H on the kept columns, zero elsewhere. Returns (P, C, kept).
-    Pinned: P^H P = I on the kept columns and W in the span of P, on full-rank and exactly rank-deficient blocks."""
+    rank-revealing drop). Returns (Q, C, T, kept): Q = W C with C = L^-H on the kept columns (zero elsewhere), and the
+    upper triangular T, T[i, j] = conj(L[j, i]) for kept i and j >= i (zero in the rows of dropped i), so that W = Q T
+    up to the dropped parts. Pinned: Q^H Q = I on the kept columns, W = Q T and W in the span of Q, on full-rank and
+    exactly rank-deficient blocks."""
     s = W.shape[1]
     G = W.conj().T @ W
     L = np.zeros((s, s), dtype=np.complex128)
@@ -77,76 +79,83 @@ def block_orth(W, tau):
     k = np.flatnonzero(kept)
     if k.size:
         C[np.ix_(k, k)] = np.linalg.inv(L[np.ix_(k, k)].conj().T)
-    return W @ C, C, kept
+    T = np.triu(L.conj().T)
+    T[~kept, :] = 0
+    return W @ C, C, T, kept
 
 
-BLOCK_DROP = 1e-6   # the rank-revealing threshold of the direction block
+BLOCK_DROP = 1e-6   # the rank-revealing threshold of the residual and direction blocks
 
 
 def block_cocg(apply, B, tol, max_iter, precondition=None, tau=BLOCK_DROP):
-    """Block COCG on the columns of B (n x s), the complex symmetric form of block conjugate gradients with the
-    unconjugated products of COCG (ConjugateGradient.m:32-59 per column), in the breakdown-free arrangement of Ji and
-    Li (2017): the direction block P is an orthonormal basis (block_orth) of Z + P beta, and
-        Q = A P, D = P^T Q, alpha = D^-1 P^T R, X += P alpha, R -= Q alpha,
-        Z = M R, beta = -D^-1 Q^T Z, P = orth(Z + P beta),
-    so a right-hand side that converges, or a block that loses rank, drops directions rather than making a
-    coefficient matrix singular. Column j stops (its solution and record frozen) once its monitored norm(r_j) /
-    norm(b_j) is below tol, or at max_iter; its residual stays in the block. A singular D (a zero pivot on the kept
-    directions) ends every running column with a NaN norm, as COCG's breakdown (ConjugateGradient.m:56-59).
+    """Block COCG on the columns of B (n x s): block conjugate gradients with the unconjugated products of COCG
+    (ConjugateGradient.m:32-59 per column). The residual block is carried as an orthonormal basis and coordinates,
+    R = Q S (the residual orthonormalization of Dubrulle's retooled block CG), and the direction block P is an
+    orthonormal basis of Z + P beta (the breakdown-free block CG of Ji and Li, 2017):
+        A P, D = P^T A P, a = D^-1 P^T Q, X += P a S, W = Q - A P a = Q' T (block_orth), S' = T S,
+        Z = M Q', beta = -D^-1 (A P)^T Z, P' = orth(Z + P beta).
+    These are the iterates of block COCG (a block Krylov method depends on its blocks only through their spans) with
+    every coefficient matrix formed from orthonormal blocks, so residuals that converge at different rates, or nearly
+    dependent right-hand sides, drop directions instead of making a coefficient matrix singular. The monitored norm of
+    column j is norm(W S_j) / norm(b_j); column j stops (solution and record frozen) below tol or at max_iter. A
+    singular D (on the kept directions) ends every running column with a NaN norm, as COCG's breakdown
+    (ConjugateGradient.m:56-59); an empty direction block ends them unconverged.
     No reference counterpart (the reference solves one right-hand side at a time); pinned by: one column is COCG,
     finite termination within ceil(n / s) iterations on a well-conditioned system, backslash, and block_orth.
     Returns (X, converged per column, iterations per column, monitored norms per column).
     """
     B = np.asarray(B, dtype=np.complex128)
     n, s = B.shape
-    apply_block = lambda P: np.stack([apply(P[:, j]) for j in range(P.shape[1])], axis=1)
-    prec_block = (lambda R: R.copy()) if precondition is None else (
-        lambda R: np.stack([precondition(R[:, j]) for j in range(R.shape[1])], axis=1))
+    apply_block = lambda V: np.stack([apply(V[:, j]) for j in range(V.shape[1])], axis=1)
+    prec_block = (lambda V: V.copy()) if precondition is None else (
+        lambda V: np.stack([precondition(V[:, j]) for j in range(V.shape[1])], axis=1))
     X = np.zeros_like(B)
     R = B.copy()
     bnorm = np.linalg.norm(B, axis=0)
-    live = bnorm > 0
-    R[:, ~live] = 0
-    norms = [[np.linalg.norm(R[:, j]) / bnorm[j] if live[j] else 0.0] for j in range(s)]
-    conv = np.array([not live[j] or norms[j][0] < tol for j in range(s)])
+    norms = [[np.linalg.norm(R[:, j]) / bnorm[j] if bnorm[j] > 0 else 0.0] for j in range(s)]
+    conv = np.array([bnorm[j] == 0 or norms[j][0] < tol for j in range(s)])
+    R[:, conv] = 0
     running = ~conv
     iters = np.zeros(s, dtype=int)
-    P, _, kept = block_orth(prec_block(R), tau)
+    Q, _, S, _ = block_orth(R, tau)
+    P, _, _, kept = block_orth(prec_block(Q), tau)
     it = 0
-    while running.any() and it < max_iter and kept.any():
+    while running.any() and it < max_iter:
+        if not kept.any():
+            running[:] = False
+            break
         it += 1
-        Q = apply_block(P)
-        D = P.T @ Q
+        AP = apply_block(P)
+        D = P.T @ AP
         k = np.flatnonzero(kept)
         Dk = D[np.ix_(k, k)]
-        try:
-            singular = not np.all(np.isfinite(Dk)) or np.linalg.cond(Dk) > 1 / np.finfo(float).eps
-        except np.linalg.LinAlgError:
-            singular = True
-        if singular:
+        if not np.all(np.isfinite(Dk)) or np.linalg.matrix_rank(Dk, tol=0) < k.size or \
+                np.linalg.cond(Dk) > 1 / np.finfo(float).eps:
             for j in np.flatnonzero(running):
                 norms[j].append(np.nan)
                 iters[j] = it
             running[:] = False
             break
-        alpha = np.zeros((s, s), dtype=np.complex128)
-        alpha[k] = np.linalg.solve(Dk, (P.T @ R)[k])
-        X[:, running] += (P @ alpha)[:, running]
-        R -= Q @ alpha
+        a = np.zeros((s, s), dtype=np.complex128)
+        a[k] = np.linalg.solve(Dk, (P.T @ Q)[k])
+        X[:, running] += (P @ (a @ S))[:, running]
+        W = Q - AP @ a
         for j in np.flatnonzero(running):
-            norms[j].append(np.linalg.norm(R[:, j]) / bnorm[j])
+            norms[j].append(np.linalg.norm(W @ S[:, j]) / bnorm[j])
             iters[j] = it
             if norms[j][-1] < tol:
                 conv[j] = True
                 running[j] = False
             elif not np.isfinite(norms[j][-1]) or it >= max_iter:
                 running[j] = False
+        Q, _, T, _ = block_orth(W, tau)
+        S = T @ S
         if not running.any():
             break
-        Z = prec_block(R)
+        Z = prec_block(Q)
         beta = np.zeros((s, s), dtype=np.complex128)
-        beta[k] = -np.linalg.solve(Dk, (Q.T @ Z)[k])
-        P, _, kept = block_orth(Z + P @ beta, tau)
+        beta[k] = -np.linalg.solve(Dk, (AP.T @ Z)[k])
+        P, _, _, kept = block_orth(Z + P @ beta, tau)
     return X, conv, iters, [np.array(v) for v in norms]
 
 

@@ -34,7 +34,11 @@ constexpr int kThreads = 256;
 constexpr int kLdp = 64;        // partial columns per block
 constexpr int kPoll = 8;        // steps enqueued between two polls of the loop states
 constexpr int kPollMax = 32;    // GMRES: the most steps enqueued between two polls
-constexpr int kStall = 40;      // COCG steps without a new best norm that end a complex64 inner solve
+constexpr int kStall = 40;      // COCG steps without a new best norm that end an inner solve of the refinement
+// Block COCG inner solve: a norm this far above the best ends the system. The block recurrence's first steps swing
+// far above the starting norm (a Galerkin step on an indefinite projection), a lost recurrence grows without bound.
+constexpr double kBlockGrowth = 1e4;
+constexpr double kBlockInner = 1e-2;   // the reduction one inner block COCG cycle of the refinement aims at
 // Mixed COCG: the first inner solve in complex64, to kCocgInner or a stall, and the later ones in FP64. Complex64
 // COCG loses the complex symmetry of its products on systems whose residual swings high (on the 512 x 512 plate
 // under grazing plane waves it stalls between 1e-2 and 1e-4); the FP64 inner solve restarted from the refined
@@ -57,7 +61,7 @@ enum ScalSlot {
 enum PoolSlot {
     kSlotPartial, kSlotScal, kSlotState, kSlotHistory, kSlotV, kSlotW, kSlotT, kSlotR, kSlotZ, kSlotD, kSlotAd,
     kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags, kSlotP, kSlotHraw,
-    kSlotPartialL, kSlotHc, kSlotY, kSlotBlockCoef, kSlotBlockPartial
+    kSlotPartialL, kSlotHc, kSlotY, kSlotBlockCoef, kSlotBlockPartial, kSlotBlockBest
 };
 
 // Restarts up to this length run the delayed-reorthogonalization kernels, whose Gram-Schmidt coefficients, rotations
@@ -270,7 +274,8 @@ __global__ void __launch_bounds__(kThreads) k_cocg_update(c128 *__restrict__ x, 
 // not improved on its best for `stall` iterations: a complex64 short recurrence that has lost its complex symmetry to
 // rounding, which the outer FP64 loop then restarts from the true residual. Grid (1, R). In a block loop (block = 1)
 // a system that stops is marked finished only: its residual stays in the block's search space, and it is skipped here
-// from then on.
+// from then on; with stall > 0 a block system also stops once its norm exceeds kBlockGrowth times its best (a block
+// recurrence that has lost its conjugacy diverges, and its solution is kept before it strays far).
 __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restrict__ partial, int blocks,
                                                             c128 *__restrict__ scal, int it, int stall,
                                                             LoopState *st, double *history, int cap, int block) {
@@ -310,7 +315,8 @@ __global__ void __launch_bounds__(kThreads) k_cocg_finalize(const c128 *__restri
     } else if (!isfinite(res)) {
         st[r].finished = 1;
         st[r].done = stop;
-    } else if (it >= static_cast<int>(maxit) || (stall > 0 && it - st[r].best >= stall)) {
+    } else if (it >= static_cast<int>(maxit) || (stall > 0 && it - st[r].best >= stall) ||
+               (block && stall > 0 && res > kBlockGrowth * best)) {
         if (block) st[r].finished = 1;
         else st[r].done = 1;
     }
@@ -479,12 +485,14 @@ __global__ void __launch_bounds__(kThreads) k_block_solve(const c128 *__restrict
 
 // The rank-revealing orthonormalization of a block W from its Gram matrix G = W^H W: the Cholesky factor L (G = L
 // L^H), column by column, keeps column j when what is left of it after the earlier kept columns, d = G_jj - sum
-// |L_jk|^2, exceeds tau^2 G_jj; C = L^-H on the kept columns (upper triangular, zero elsewhere), so that W C is
-// orthonormal there and zero in the dropped columns. No kept column ends every running system (nothing is left to
-// search with). One block; L and C in LDS (kBlockLds dynamic).
+// |L_jk|^2, exceeds tau^2 G_jj. Out: C = L^-H on the kept columns (upper triangular, zero elsewhere), so that Q = W C
+// is orthonormal there and zero in the dropped columns; T (upper triangular) with W = Q T up to the dropped parts:
+// T(i, j) = conj(L(j, i)) for kept i and j >= i, zero in the rows of dropped i; keep. With no column kept and
+// stop_empty, every running system ends (nothing is left to search with). One block; L and C in LDS (kBlockLds).
 __global__ void __launch_bounds__(kThreads) k_block_orth(const c128 *__restrict__ G, c128 *__restrict__ C,
-                                                         c128 *__restrict__ keep, int s, double tau, LoopState *st,
-                                                         double *history, int cap) {
+                                                         c128 *__restrict__ T, c128 *__restrict__ keep, int s,
+                                                         double tau, int stop_empty, LoopState *st, double *history,
+                                                         int cap) {
     extern __shared__ unsigned char orth_lds[];
     constexpr int ld = kBlockLd;
     c128 *L = reinterpret_cast<c128 *>(orth_lds), *Cs = L + kBlockMax * ld;   // row-major [i][j]
@@ -521,7 +529,7 @@ __global__ void __launch_bounds__(kThreads) k_block_orth(const c128 *__restrict_
         }
         __syncthreads();
     }
-    if (count == 0) {
+    if (count == 0 && stop_empty) {
         block_stop_all(st, s, -1, history, cap);
         return;
     }
@@ -544,8 +552,46 @@ __global__ void __launch_bounds__(kThreads) k_block_orth(const c128 *__restrict_
     for (int e = t; e < s * s; e += kThreads) {
         const int i = e % s, j = e / s;
         C[j * kBlockMax + i] = Cs[i * ld + j];
+        c128 v = mk(0.0, 0.0);
+        if (kept[i] && j >= i) v = j == i ? mk(diag[i], 0.0) : conjg(L[j * ld + i]);
+        T[j * kBlockMax + i] = v;
     }
     for (int j = t; j < s; j += kThreads) keep[j] = mk(kept[j] ? 1.0 : 0.0, 0.0);
+}
+
+// Out = A B for s x s matrices (column-major, leading dimension kBlockMax), one block.
+__global__ void __launch_bounds__(kThreads) k_block_mm(const c128 *__restrict__ A, const c128 *__restrict__ B,
+                                                       c128 *__restrict__ Out, int s) {
+    for (int e = threadIdx.x; e < s * s; e += kThreads) {
+        const int i = e % s, j = e / s;
+        c128 v = mk(0.0, 0.0);
+        for (int k = 0; k < s; ++k) v = v + A[k * kBlockMax + i] * B[j * kBlockMax + k];
+        Out[j * kBlockMax + i] = v;
+    }
+}
+
+// The squared residual norms of the block W S (column j: S_j^H G S_j with G = W^H W), written where the COCG
+// finalize reads one block's partial sum: partial[j * kLdp]. One block, a thread per column.
+__global__ void __launch_bounds__(kThreads) k_block_norms(const c128 *__restrict__ S, const c128 *__restrict__ G,
+                                                          c128 *__restrict__ partial, int s) {
+    const int j = threadIdx.x;
+    if (j >= s) return;
+    double v = 0.0;
+    for (int i = 0; i < s; ++i) {
+        c128 gs = mk(0.0, 0.0);
+        for (int k = 0; k < s; ++k) gs = gs + G[k * kBlockMax + i] * S[j * kBlockMax + k];
+        v += cmulc(S[j * kBlockMax + i], gs).x;
+    }
+    partial[static_cast<int64_t>(j) * kLdp] = mk(isnan(v) ? v : fmax(v, 0.0), 0.0);   // a NaN stays visible
+}
+
+// xbest_r = x_r for the systems whose monitored norm reached a new best at iteration it (grid (blocks, R)).
+__global__ void __launch_bounds__(kThreads) k_block_keep_best(const c128 *__restrict__ x, c128 *__restrict__ xbest,
+                                                              int64_t n, int it, const LoopState *__restrict__ st) {
+    SYSTEM(r);
+    if (st[r].best != it) return;
+    const int64_t o = r * n;
+    ROWS(i, n) xbest[o + i] = x[o + i];
 }
 
 // Y_j = [Y_j +] sign sum_i W_i C(i, j) for the columns j the mode selects (kBlockAll: every column; kBlockRunning:
@@ -1667,15 +1713,22 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
 
 // ------------------------------------------------------------------------------------------------------------------
 // Block COCG: the R right-hand sides of one complex symmetric system (A^T = A, a symmetric preconditioner M) share
-// one block Krylov space: block conjugate gradients with the unconjugated products of ConjugateGradient.m:32-59, in
-// the breakdown-free arrangement of Ji and Li (2017), whose direction block P is an orthonormal basis of Z + P beta:
-//   Q = A P, D = P^T Q, alpha = D^-1 P^T R, X += P alpha, R -= Q alpha,
-//   Z = M R, beta = -D^-1 Q^T Z, P = orth(Z + P beta)   (rank-revealing: dependent directions are dropped).
-// A system whose monitored residual norm(r_j) / norm(b_j) falls below its tolerance, or that reaches its iteration
-// limit, stops there (its solution and record frozen) while its residual stays in the block; a right-hand side that
-// converges, or a block that loses rank, then drops directions instead of making a coefficient matrix singular.
+// one block Krylov space: block conjugate gradients with the unconjugated products of ConjugateGradient.m:32-59. The
+// residual block is carried as an orthonormal basis and its coordinates, R = Q S (the residual orthonormalization of
+// Dubrulle's retooled block CG), and the direction block P is an orthonormal basis of Z + P beta (the breakdown-free
+// block CG of Ji and Li, 2017):
+//   A P, D = P^T A P, a = D^-1 P^T Q, X += P a S, W = Q - A P a, W = Q' T (rank-revealing), S' = T S,
+//   Z = M Q', beta = -D^-1 (A P)^T Z, P' = orth(Z + P beta).
+// A block Krylov iteration depends on its blocks only through their spans, so these are the iterates of block COCG,
+// with every coefficient matrix formed from orthonormal blocks: residuals that converge at different rates, or right-
+// hand sides that are (nearly) dependent, drop directions instead of making a coefficient matrix singular. The
+// monitored norm of system j is norm(R_j) / norm(b_j), norm(R_j)^2 = S_j^H W^H W S_j. A system that falls below its
+// tolerance, or reaches its iteration limit, stops there (its solution and record frozen). As an inner solve of the
+// refinement (stall > 0) a system also stops when its norm has not improved for `stall` iterations or exceeds
+// kBlockGrowth times its best, and hands back the iterate of its best norm: the outer loop then restarts a fresh block
+// recurrence from the true residual.
 // ------------------------------------------------------------------------------------------------------------------
-constexpr double kBlockDrop = 1e-6;   // rank-revealing threshold of the direction block
+constexpr double kBlockDrop = 1e-6;   // rank-revealing threshold of the residual and direction blocks
 
 static bool all_finished(const std::vector<LoopState> &st) {
     for (const auto &x : st)
@@ -1693,18 +1746,21 @@ static InnerResult block_cocg(Workspace &ws, const LinOp &A, const LinOp *P, con
         fail(RT_EINVAL, "rooftopMoM:solver:invalidRightHandSide",
              "Block COCG solves at most %d right-hand sides together.", kBlockMax);
     LoopState *st = ws.state;
-    c128 *r = vec(ctx, kSlotR, R * n), *z = vec(ctx, kSlotZ, R * n), *pd = vec(ctx, kSlotD, R * n);
-    c128 *q = vec(ctx, kSlotAd, R * n);
+    // The columns of Q, Z, P and A P are basis vectors, not systems: every product runs on all of them.
+    c128 *qb = vec(ctx, kSlotR, R * n), *zb = vec(ctx, kSlotZ, R * n), *pd = vec(ctx, kSlotD, R * n);
+    c128 *ap = vec(ctx, kSlotAd, R * n);
     constexpr int kSq = kBlockMax * kBlockMax;
-    c128 *coef = vec(ctx, kSlotBlockCoef, 8 * kSq);
-    c128 *D = coef, *T = coef + kSq, *alpha = coef + 2 * kSq, *E = coef + 3 * kSq, *beta = coef + 4 * kSq;
-    c128 *G = coef + 5 * kSq, *Cm = coef + 6 * kSq, *keep = coef + 7 * kSq;
+    c128 *coef = vec(ctx, kSlotBlockCoef, 14 * kSq);
+    c128 *D = coef, *F = coef + kSq, *a = coef + 2 * kSq, *ax = coef + 3 * kSq, *E = coef + 4 * kSq;
+    c128 *beta = coef + 5 * kSq, *G = coef + 6 * kSq, *Cm = coef + 7 * kSq, *Tm = coef + 8 * kSq;
+    c128 *S = coef + 9 * kSq, *S2 = coef + 10 * kSq, *keep_p = coef + 11 * kSq, *keep_r = coef + 12 * kSq;
+    c128 *Tp = coef + 13 * kSq;
     const int tn_blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(512, (n + kBtRows - 1) / kBtRows)));
     const int64_t tn_chunk = ((n + tn_blocks - 1) / tn_blocks + kBtRows - 1) / kBtRows * kBtRows;
     c128 *tn_partial = vec(ctx, kSlotBlockPartial, static_cast<int64_t>(tn_blocks) * kSq);
     raise_lds(reinterpret_cast<const void *>(&k_block_solve), kBlockLds);
     raise_lds(reinterpret_cast<const void *>(&k_block_orth), kBlockLds);
-    auto transposed = [&](const c128 *U, const c128 *V, c128 *C, bool conj) {   // C = U^T V (conj: U^H V)
+    auto transposed = [&](const c128 *U, const c128 *V, c128 *Out, bool conj) {   // Out = U^T V (conj: U^H V)
         if (conj)
             hipLaunchKernelGGL(k_block_tn<true>, dim3(tn_blocks), dim3(kThreads), 0, ws.s, U, V, n, R, tn_chunk,
                                tn_partial);
@@ -1712,82 +1768,110 @@ static InnerResult block_cocg(Workspace &ws, const LinOp &A, const LinOp *P, con
             hipLaunchKernelGGL(k_block_tn<false>, dim3(tn_blocks), dim3(kThreads), 0, ws.s, U, V, n, R, tn_chunk,
                                tn_partial);
         RT_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_block_sum, dim3(kSq / kThreads), dim3(kThreads), 0, ws.s, tn_partial, tn_blocks, C);
+        hipLaunchKernelGGL(k_block_sum, dim3(kSq / kThreads), dim3(kThreads), 0, ws.s, tn_partial, tn_blocks, Out);
         RT_LAUNCH_CHECK();
     };
     auto solve = [&](const c128 *M, const c128 *B, c128 *X, int at) {   // X = M^-1 B on the kept directions
         hipLaunchKernelGGL(k_block_solve, dim3(1), dim3(kThreads), static_cast<uint32_t>(kBlockLds), ws.s, M, B, X,
-                           keep, R, st, at, ws.history, ws.cap);
+                           keep_p, R, st, at, ws.history, ws.cap);
         RT_LAUNCH_CHECK();
     };
-    auto combine = [&](c128 *Y, const c128 *W, const c128 *C, double sign, int mode, int accumulate) {
+    auto combine = [&](c128 *Y, const c128 *W, const c128 *Cc, double sign, int mode, int accumulate) {
         hipLaunchKernelGGL(k_block_nn, dim3(static_cast<unsigned>((n + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                           ws.s, Y, W, C, sign, n, R, st, mode, accumulate);
+                           ws.s, Y, W, Cc, sign, n, R, st, mode, accumulate);
         RT_LAUNCH_CHECK();
     };
-    auto orthonormalize = [&]() {   // P = orth(Z): Gram matrix, its rank-revealing factor, P = Z C
-        transposed(z, z, G, true);
-        hipLaunchKernelGGL(k_block_orth, dim3(1), dim3(kThreads), static_cast<uint32_t>(kBlockLds), ws.s, G, Cm,
-                           keep, R, kBlockDrop, st, ws.history, ws.cap);
+    auto mm = [&](const c128 *L, const c128 *Rm, c128 *Out) {
+        hipLaunchKernelGGL(k_block_mm, dim3(1), dim3(kThreads), 0, ws.s, L, Rm, Out, R);
         RT_LAUNCH_CHECK();
-        combine(pd, z, Cm, 1.0, kBlockAll, 0);
+    };
+    // Gram matrix of the block in `from`, its rank-revealing factors (C, T, keep), and `to` = from C.
+    auto orthonormalize = [&](c128 *from, c128 *to, c128 *T_out, c128 *keep, int stop_empty) {
+        transposed(from, from, G, true);
+        hipLaunchKernelGGL(k_block_orth, dim3(1), dim3(kThreads), static_cast<uint32_t>(kBlockLds), ws.s, G, Cm,
+                           T_out, keep, R, kBlockDrop, stop_empty, st, ws.history, ws.cap);
+        RT_LAUNCH_CHECK();
+        combine(to, from, Cm, 1.0, kBlockAll, 0);
     };
     auto precondition = [&](const c128 *in, c128 *out) {
-        if (P) (*P)(in, out, ws.skip());
-        else ws.copy(in, out, st);
+        if (P) (*P)(in, out, nullptr);
+        else ws.copy(in, out, nullptr);
     };
     ws.reset_state(skip_sys);
     ws.put_re(kTol, tol);
     ws.put_re(kBest, std::vector<double>(R, HUGE_VAL));
     const int max_iter = put_limits(ws, limits, skip_sys);
-    // The systems left out (and those converged at the start) are zero columns of every block: the vector slots are
-    // shared with other loops, so they are cleared first.
-    for (c128 *v : {r, z, pd, q}) RT_HIP(hipMemsetAsync(v, 0, sizeof(c128) * R * n, ws.s));
+    // Systems left out (and those converged at the start) are zero columns of the first residual block; the vector
+    // slots are shared with other loops, so every block starts cleared.
+    for (c128 *v : {qb, zb, pd, ap}) RT_HIP(hipMemsetAsync(v, 0, sizeof(c128) * R * n, ws.s));
     if (has_x0) {
-        A(x, q, ws.skip());
-        ws.sub(b, q, r, st);
-        RT_HIP(hipMemsetAsync(q, 0, sizeof(c128) * R * n, ws.s));
+        A(x, ap, nullptr);
+        ws.sub(b, ap, qb, st);
     } else {
         ws.zero(x);
-        ws.copy(b, r, st);
+        ws.copy(b, qb, st);
     }
-    ws.nrm2(r, kRnormCg, st);
+    ws.nrm2(qb, kRnormCg, st);
     ws.nrm2(b, kBnorm, st);
     hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnormCg, kBnorm, R);
     RT_LAUNCH_CHECK();
     const std::vector<double> r0 = ws.fetch_re(kRnormCg), b0 = ws.fetch_re(kBnorm);
     std::vector<LoopState> state = ws.read_state();
-    // A system converged at the start leaves the block entirely (done): its residual column is cleared.
     for (int j = 0; j < R; ++j)
-        if (state[j].done && !skip_sys[j]) RT_HIP(hipMemsetAsync(r + j * n, 0, sizeof(c128) * n, ws.s));
-    precondition(r, z);
-    orthonormalize();
+        if (state[j].done) RT_HIP(hipMemsetAsync(qb + j * n, 0, sizeof(c128) * n, ws.s));
+    // The best iterate so far (an inner solve of the refinement): the starting one, at its relative norm.
+    c128 *xbest = stall > 0 ? vec(ctx, kSlotBlockBest, R * n) : nullptr;
+    if (xbest) {
+        ws.copy(x, xbest, nullptr);
+        std::vector<double> best(R, HUGE_VAL);
+        for (int j = 0; j < R; ++j)
+            if (b0[j] > 0) best[j] = r0[j] / b0[j];
+        ws.put_re(kBest, best);
+    }
+    orthonormalize(qb, zb, S, keep_r, 0);   // R0 = Q S
+    std::swap(qb, zb);
+    precondition(qb, zb);
+    orthonormalize(zb, pd, Tp, keep_p, 1);
     int it = 0;
     state = ws.read_state();
     while (it < max_iter && !all_finished(state)) {
         const int window = std::min(kPoll, max_iter - it);
         for (int w = 0; w < window; ++w) {
             ++it;
-            A(pd, q, ws.skip());
-            transposed(pd, q, D, false);
-            transposed(pd, r, T, false);
-            solve(D, T, alpha, it);
-            combine(x, pd, alpha, 1.0, kBlockSearching, 1);
-            combine(r, q, alpha, -1.0, kBlockRunning, 1);
-            hipLaunchKernelGGL(k_nrm2, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, r, n, ws.chunk, ws.partial, st);
+            A(pd, ap, nullptr);
+            transposed(pd, ap, D, false);
+            transposed(pd, qb, F, false);
+            solve(D, F, a, it);
+            mm(a, S, ax);
+            combine(x, pd, ax, 1.0, kBlockSearching, 1);
+            combine(qb, ap, a, -1.0, kBlockAll, 1);   // W = Q - A P a
+            transposed(qb, qb, G, true);
+            hipLaunchKernelGGL(k_block_norms, dim3(1), dim3(kThreads), 0, ws.s, S, G, ws.partial, R);
             RT_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_cocg_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, ws.blocks, ws.scal,
-                               it, stall, ws.state, ws.history, ws.cap, 1);
+            hipLaunchKernelGGL(k_cocg_finalize, dim3(1, R), dim3(kThreads), 0, ws.s, ws.partial, 1, ws.scal, it,
+                               stall, ws.state, ws.history, ws.cap, 1);
             RT_LAUNCH_CHECK();
-            precondition(r, z);
-            transposed(q, z, E, false);
+            if (xbest) {
+                hipLaunchKernelGGL(k_block_keep_best, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, x, xbest, n, it, st);
+                RT_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(k_block_orth, dim3(1), dim3(kThreads), static_cast<uint32_t>(kBlockLds), ws.s, G, Cm,
+                               Tm, keep_r, R, kBlockDrop, 0, st, ws.history, ws.cap);
+            RT_LAUNCH_CHECK();
+            combine(zb, qb, Cm, 1.0, kBlockAll, 0);   // Q' = W C
+            std::swap(qb, zb);
+            mm(Tm, S, S2);   // S' = T S
+            std::swap(S, S2);
+            precondition(qb, zb);
+            transposed(ap, zb, E, false);
             solve(D, E, beta, -1);
-            combine(z, pd, beta, -1.0, kBlockRunning, 1);   // Z + P beta, beta = -D^-1 Q^T Z
-            orthonormalize();
+            combine(zb, pd, beta, -1.0, kBlockAll, 1);   // Z + P beta, beta = -D^-1 (A P)^T Z
+            orthonormalize(zb, pd, Tp, keep_p, 1);
         }
         state = ws.read_state();
         ws.set_scale(state);
     }
+    if (xbest) ws.copy(xbest, x, nullptr);
     ctx->timed_scale = 1.0;
     return collect(ws, state, skip_sys, r0, b0);
 }
@@ -2115,14 +2199,20 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
                               op.has(RT_C128) && (!pc || pc->has(RT_C128));
         if (o.inner_tolerance <= 0 && o.precision == RT_C64)
             inner_tol = std::max(o.tolerance, cg_like ? (fallback ? kCocgInner : 1e-4) : 1e-6);
+        // A block COCG cycle reduces its residual by kBlockInner at most, in either precision: a long block recurrence
+        // loses its conjugacy and then its recursive residual parts from the true one.
+        const bool block = o.method == RT_METHOD_BLOCK_CG && o.inner_tolerance <= 0;
+        if (block) inner_tol = std::max(o.tolerance, kBlockInner);
         std::vector<char> done(R);
         std::vector<int> total(R, 0);
         for (int r = 0; r < R; ++r) {
             done[r] = zero_rhs[r] || rel[r] < o.tolerance;
             out[r].norms.push_back(rel[r]);
         }
-        // An inner complex64 COCG that stagnates (no new best norm for kStall iterations) hands back to this loop.
-        int stall = cg_like && o.precision == RT_C64 ? kStall : 0;
+        // An inner complex64 COCG that stagnates (no new best norm for kStall iterations) hands back to this loop; so
+        // does every inner block COCG (whose recurrence can lose its conjugacy in any precision) and every FP64 inner
+        // COCG: each restart begins a fresh recurrence from the true residual.
+        int stall = cg_like ? kStall : 0;
         int outer = 0;
         while (outer < o.max_outer) {
             bool active = false;
@@ -2136,8 +2226,7 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
             ++outer;
             if (fallback && outer == 2) {   // the rest of a mixed COCG in FP64
                 inner_prec = RT_C128;
-                inner_tol = o.tolerance;
-                stall = 0;
+                if (!block) inner_tol = o.tolerance;
             }
             ws.copy(t, rv, nullptr);   // t holds b - A x from true_residual
             // Each system asks the inner solve for what is left to reach the tolerance, no less than the floor.

@@ -115,6 +115,19 @@ def test_plate_plane_waves_against_oracle_and_cocg(rt):
     assert max(r.iterationCount() for r in res) < max(r.iterationCount() for r in lock)
 
 
+def test_refined_block_cycles_reach_the_tolerance(rt):
+    """Block COCG as the inner solve of FP64 iterative refinement: each cycle reduces its residual by 1e-2 at most (or
+    stops at a stall, handing back its best iterate) and the outer loop restarts it from the true residual; every
+    column reaches 1e-8 against the dense matrix."""
+    p, grid, gen, B = _plate(rt, 6)
+    Z = S.dense_matrix(gen, grid)
+    op, pc = rt.Operator(p), rt.Preconditioner(p, 'nearField')
+    res = rt.Solver('blockcg', tolerance=1e-8, maxIterationCount=500, refine=True).solve(op, B, preconditioner=pc)
+    for j, got in enumerate(res):
+        assert got.converged() and got.outerCount >= 2
+        assert rel(Z @ got.solution(), B[:, j]) < 1e-8
+
+
 def test_mixed_block_refines_to_fp64_accuracy(rt):
     """Block COCG as the complex64 inner solve of FP64 iterative refinement: every column to 1e-10."""
     p, grid, gen, B = _plate(rt, 5, seed=9)

@@ -206,14 +206,19 @@ def test_block_orth_basis_and_drop():
     kept ones (the last, in the order columns are taken) are dropped."""
     rng = np.random.default_rng(6)
     W = rng.standard_normal((50, 6)) + 1j * rng.standard_normal((50, 6))
-    P, C, kept = SV.block_orth(W, 1e-6)
+    P, C, T, kept = SV.block_orth(W, 1e-6)
     assert kept.all()
     np.testing.assert_allclose(P.conj().T @ P, np.eye(6), atol=1e-13)
     assert rel(P @ (P.conj().T @ W), W) < 1e-13
+    assert rel(P @ T, W) < 1e-13 and np.all(np.tril(T, -1) == 0)
+    r = np.linalg.qr(W)[1]   # Householder QR, rows scaled to a positive real diagonal: the same factor
+    d = np.diag(r)
+    np.testing.assert_allclose(T, (d.conj() / abs(d))[:, None] * r, atol=1e-12)
     W[:, 3] = 2 * W[:, 0] - 1j * W[:, 5]
     W[:, 4] = 0
-    P, C, kept = SV.block_orth(W, 1e-6)
+    P, C, T, kept = SV.block_orth(W, 1e-6)
     assert kept.tolist() == [True, True, True, True, False, False]
+    assert rel(P @ T, W) < 1e-12
     k = np.flatnonzero(kept)
     np.testing.assert_allclose(P[:, k].conj().T @ P[:, k], np.eye(4), atol=1e-12)
     assert np.all(P[:, ~kept] == 0) and rel(P @ (P.conj().T @ W), W) < 1e-12
