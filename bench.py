@@ -220,10 +220,23 @@ def serve_stream(rt, requests, solver, slots, arrivals, batch=1):
     return out, makespan, stats
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n, budget_s=10.0):
     """The oracle's FP64 path (scipy FFTs on the host cores of this GPU's share, split-FFT product, sparse near-field
-    apply, GMRES(50) with two-pass Gram-Schmidt and the same tolerance) on the same plate: whole solves repeated until about budget_s
-    seconds of CPU work (at least one), setup excluded."""
+    apply, GMRES(50) with two-pass Gram-Schmidt and the same tolerance) on the same plate: whole solves repeated until
+    about budget_s seconds of CPU work (at least one), setup excluded. The threads are capped at 16, the host CPU share
+    of one GPU on an 8-GPU node; the host's own count and CPU model are reported beside them."""
     from oracle import assembly as A
     from oracle import excitation as X
     from oracle import nearfield as NF
@@ -248,7 +261,8 @@ def cpu_baseline(n, budget_s=10.0):
     dt = time.perf_counter() - t0
     S.set_workers(1)
     cpu_baseline.solution = out['solution']   # the oracle's FP64 solution, for the GPU forms' error against it
-    return {'value': count / dt, 'unit': 'solves/s', 'cores': cores, 'kind': 'port',
+    return {'value': count / dt, 'unit': 'solves/s', 'cores': cores, 'kind': 'port', 'cpu_model': cpu_model(),
+            'host_cpus': os.cpu_count(),
             'sample': '%d whole solves of the same %dx%d plate in %.1f s: oracle FP64 split-FFT GMRES(50) + '
                       'near-field, %d iterations each, true residual %.2e (scipy FFT on %d threads)'
                       % (count, n, n, dt, iters, res, cores)}
@@ -271,6 +285,10 @@ def main():
     ap.add_argument('--matvec-only', nargs=3, metavar=('SIDE', 'RHS', 'PRECISION'),
                     help='only the matvec of the SIDE x SIDE plate (lambda/10, orders [16, 8]) on RHS right-hand '
                          'sides in c64 or c128, tuned, then applied --steps times (the counter profiles)')
+    ap.add_argument('--shape', type=str, default=None,
+                    help='with --matvec-only: this launch shape (JSON {"bits", "group", "tile", "teams", "chunks"}, '
+                         'as a plain run reports it) instead of tuning, so that a profiled run launches the same '
+                         'kernels as the plain one')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: this build runs the path on a single MI355X and does not shard a solve '
@@ -294,7 +312,11 @@ def main():
         print('bench: matvec of the %dx%d plate, %d right-hand sides, %s' % (side, side, R, prec), flush=True)
         mp = rt.Problem(rt.Mesh(side, side, 0.1, 0.1), 2 * np.pi, quadratureOrder=(16, 8))
         mop = rt.Operator(mp, precision='mixed')
-        shape, _ = mop.tune(nrhs=R, precision=prec)
+        if args.shape:
+            shape = json.loads(args.shape)
+            mop.setShape(shape, nrhs=R, precision=prec)
+        else:
+            shape, _ = mop.tune(nrhs=R, precision=prec)
         pms, pby, wall = mop.profile(nrhs=R, reps=max(1, args.steps), precision=prec)
         print(json.dumps({'workload': 'plate %dx%d, %d right-hand sides, %s' % (side, side, R, prec),
                           'shape': shape, 'pass_ms': pms, 'pass_bytes': pby, 'wall_ms': wall,
