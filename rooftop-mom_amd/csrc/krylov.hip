@@ -1476,7 +1476,10 @@ Workspace::Workspace(Context *c, int64_t n_, int R_, int max_iter) : ctx(c), s(c
     state = static_cast<LoopState *>(c->scratch(kSlotState, sizeof(LoopState) * R, "the solver's state"));
     cap = max_iter + 2;
     history = static_cast<double *>(c->scratch(kSlotHistory, sizeof(double) * cap * R, "the solver's history"));
-    host_bytes = std::max<size_t>(4096, static_cast<size_t>(R) * 64);
+    // Reads: up to three slots and the loop states of every system. Staging: room for many small writes between
+    // two synchronizations, so that setting a slot or resetting the states does not wait for the device.
+    read_bytes = std::max<size_t>(4096, static_cast<size_t>(R) * 128);
+    host_bytes = read_bytes + 16 * std::max<size_t>(4096, static_cast<size_t>(R) * sizeof(c128));
     if (!c->host_pinned || c->host_pinned_bytes < host_bytes) {
         if (c->host_pinned) RT_HIP(hipHostFree(c->host_pinned));
         c->host_pinned = nullptr;
@@ -1504,31 +1507,55 @@ void Workspace::nrm2(const c128 *x, int slot, const LoopState *st) {
 }
 
 std::vector<double> Workspace::fetch_re(int slot) {
-    RT_HIP(hipMemcpy2DAsync(host, sizeof(c128), scal + slot, sizeof(c128) * kScal, sizeof(c128), R,
-                            hipMemcpyDeviceToHost, s));
+    return fetch_slots({slot}, nullptr)[0];
+}
+
+std::vector<std::vector<double>> Workspace::fetch_slots(const std::vector<int> &slots, std::vector<LoopState> *states) {
+    const size_t per = sizeof(c128) * R;
+    for (size_t k = 0; k < slots.size(); ++k)
+        RT_HIP(hipMemcpy2DAsync(host + k * per, sizeof(c128), scal + slots[k], sizeof(c128) * kScal, sizeof(c128), R,
+                                hipMemcpyDeviceToHost, s));
+    const size_t at = slots.size() * per;
+    if (states) RT_HIP(hipMemcpyAsync(host + at, state, sizeof(LoopState) * R, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    std::vector<double> out(R);
-    const c128 *h = reinterpret_cast<const c128 *>(host);
-    for (int r = 0; r < R; ++r) out[r] = h[r].x;
+    stage_off = 0;   // every staged write before the synchronization has been copied
+    std::vector<std::vector<double>> out(slots.size(), std::vector<double>(R));
+    for (size_t k = 0; k < slots.size(); ++k) {
+        const c128 *h = reinterpret_cast<const c128 *>(host + k * per);
+        for (int r = 0; r < R; ++r) out[k][r] = h[r].x;
+    }
+    if (states) {
+        const LoopState *h = reinterpret_cast<const LoopState *>(host + at);
+        states->assign(h, h + R);
+    }
+    return out;
+}
+
+unsigned char *Workspace::stage(size_t bytes) {
+    bytes = (bytes + 255) / 256 * 256;
+    if (stage_off + bytes > host_bytes - read_bytes) {   // full: wait for the copies in flight to finish
+        RT_HIP(hipStreamSynchronize(s));
+        stage_off = 0;
+    }
+    unsigned char *out = host + read_bytes + stage_off;
+    stage_off += bytes;
     return out;
 }
 
 void Workspace::put_re(int slot, const std::vector<double> &v) {
-    c128 *h = reinterpret_cast<c128 *>(host);
+    c128 *h = reinterpret_cast<c128 *>(stage(sizeof(c128) * R));
     for (int r = 0; r < R; ++r) h[r] = mk(v[r], 0.0);
-    RT_HIP(hipMemcpy2DAsync(scal + slot, sizeof(c128) * kScal, host, sizeof(c128), sizeof(c128), R,
+    RT_HIP(hipMemcpy2DAsync(scal + slot, sizeof(c128) * kScal, h, sizeof(c128), sizeof(c128), R,
                             hipMemcpyHostToDevice, s));
-    RT_HIP(hipStreamSynchronize(s));   // the staging buffer is reused by the next call
 }
 
 void Workspace::reset_state(const std::vector<char> &finished) {
     int *flags = static_cast<int *>(ctx->scratch(kSlotFlags, sizeof(int) * R, "the solver's state"));
-    int *h = reinterpret_cast<int *>(host);
+    int *h = reinterpret_cast<int *>(stage(sizeof(int) * R));
     for (int r = 0; r < R; ++r) h[r] = finished[r] ? 1 : 0;
-    RT_HIP(hipMemcpyAsync(flags, host, sizeof(int) * R, hipMemcpyHostToDevice, s));
+    RT_HIP(hipMemcpyAsync(flags, h, sizeof(int) * R, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_reset_state, dim3((R + 63) / 64), dim3(64), 0, s, state, flags, R);
     RT_LAUNCH_CHECK();
-    RT_HIP(hipStreamSynchronize(s));
 }
 
 void Workspace::set_scale(const std::vector<LoopState> &st) {
@@ -1540,6 +1567,7 @@ void Workspace::set_scale(const std::vector<LoopState> &st) {
 std::vector<LoopState> Workspace::read_state() {
     RT_HIP(hipMemcpyAsync(host, state, sizeof(LoopState) * R, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
+    stage_off = 0;
     const LoopState *h = reinterpret_cast<const LoopState *>(host);
     return std::vector<LoopState>(h, h + R);
 }
@@ -1674,8 +1702,9 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
     ws.nrm2(b, kBnorm, st);
     hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnormCg, kBnorm, R);
     RT_LAUNCH_CHECK();
-    const std::vector<double> r0 = ws.fetch_re(kRnormCg), b0 = ws.fetch_re(kBnorm);
-    std::vector<LoopState> state = ws.read_state();
+    std::vector<LoopState> state;
+    const auto first = ws.fetch_slots({kRnormCg, kBnorm}, &state);
+    const std::vector<double> &r0 = first[0], &b0 = first[1];
     int it = 0;
     while (it < max_iter && !all_done(state)) {
         const int window = std::min(kPoll, max_iter - it);
@@ -1815,8 +1844,9 @@ static InnerResult block_cocg(Workspace &ws, const LinOp &A, const LinOp *P, con
     ws.nrm2(b, kBnorm, st);
     hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnormCg, kBnorm, R);
     RT_LAUNCH_CHECK();
-    const std::vector<double> r0 = ws.fetch_re(kRnormCg), b0 = ws.fetch_re(kBnorm);
-    std::vector<LoopState> state = ws.read_state();
+    std::vector<LoopState> state;
+    const auto first = ws.fetch_slots({kRnormCg, kBnorm}, &state);
+    const std::vector<double> &r0 = first[0], &b0 = first[1];
     for (int j = 0; j < R; ++j)
         if (state[j].done) RT_HIP(hipMemsetAsync(qb + j * n, 0, sizeof(c128) * n, ws.s));
     // The best iterate so far (an inner solve of the refinement): the starting one, at its relative norm.
@@ -1924,8 +1954,9 @@ static InnerResult gmres_cgs(Workspace &ws, const LinOp &A, const LinOp *P, cons
     hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnorm, kReference,
                        R);
     RT_LAUNCH_CHECK();
-    const std::vector<double> r0 = ws.fetch_re(kRnorm), ref = ws.fetch_re(kReference);
-    std::vector<LoopState> state = ws.read_state();
+    std::vector<LoopState> state;
+    const auto first = ws.fetch_slots({kRnorm, kReference}, &state);
+    const std::vector<double> &r0 = first[0], &ref = first[1];
     auto any_active = [&]() {
         for (const auto &q : state)
             if (!q.finished) return true;
@@ -2039,8 +2070,9 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
     hipLaunchKernelGGL(k_initial_check, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, ws.scal, kRnorm, kReference,
                        R);
     RT_LAUNCH_CHECK();
-    const std::vector<double> r0 = ws.fetch_re(kRnorm), ref = ws.fetch_re(kReference);
-    std::vector<LoopState> state = ws.read_state();
+    std::vector<LoopState> state;
+    const auto first = ws.fetch_slots({kRnorm, kReference}, &state);
+    const std::vector<double> &r0 = first[0], &ref = first[1];
     auto any_active = [&]() {
         for (const auto &q : state)
             if (!q.finished) return true;
@@ -2183,10 +2215,17 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
         }
     } else {
         // x_{k+1} = x_k + d_k, A d_k = b - A x_k solved by the inner method in its precision.
-        if (x0) ws.copy(x0, x, nullptr);
-        else ws.zero(x);
         c128 *rv = vec(ctx, kSlotOuterR, R * n), *d = vec(ctx, kSlotOuterD, R * n);
-        std::vector<double> rel = true_residual(x);
+        std::vector<double> rel(R, 1.0);
+        if (x0) {
+            ws.copy(x0, x, nullptr);
+            rel = true_residual(x);
+        } else {   // x = 0: the residual is b itself, at relative norm 1
+            ws.zero(x);
+            ws.copy(b, t, nullptr);
+            for (int r = 0; r < R; ++r)
+                if (zero_rhs[r]) rel[r] = 0.0;
+        }
         // Inner tolerance: complex64 GMRES reaches about 1e-6 before the rounding of its products shows; COCG's
         // short recurrence loses the complex symmetry of complex64 products near 1e-4 (sooner on systems whose
         // residual norm swings high on the way), so a mixed COCG hands what is left after its first inner solve to

@@ -67,13 +67,18 @@ struct Workspace {
     LoopState *state;    // [R]
     double *history;     // [R][cap]
     int cap;
-    unsigned char *host; // pinned staging
+    unsigned char *host; // pinned: [0, read_bytes) takes device-to-host reads, the rest stages host-to-device writes
     size_t host_bytes;
+    size_t read_bytes;
+    size_t stage_off = 0;   // next free byte of the staging part (every copy from it before is still in flight)
     Workspace(Context *c, int64_t n_, int R_, int max_iter);
     const int *skip() const { return &state->done; }
     void nrm2(const c128 *x, int slot, const LoopState *st);
     void dot(const c128 *x, const c128 *y, bool conj, int slot, const LoopState *st);
     std::vector<double> fetch_re(int slot);                  // real part of one slot of every system
+    // Real parts of several slots of every system and the loop states, read with one synchronization.
+    std::vector<std::vector<double>> fetch_slots(const std::vector<int> &slots, std::vector<LoopState> *states);
+    unsigned char *stage(size_t bytes);   // host bytes for one asynchronous host-to-device copy
     void put_re(int slot, const std::vector<double> &v);     // sets one slot of every system (real)
     void copy(const c128 *x, c128 *y, const LoopState *st);
     void sub(const c128 *b, const c128 *ax, c128 *r, const LoopState *st);

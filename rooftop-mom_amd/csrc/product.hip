@@ -950,7 +950,9 @@ void operator_profile(Product &op, int nrhs, int reps, int prec, double *ms, dou
 // The autotuner: for each plan (radices up to 8 or up to 16) it times the pass A + C column groups with the default
 // tile, then the pass B tiles with the best group, and keeps the fastest plan; the shape is recorded for this batch
 // size. Pass A/C time depends on the group only and pass B time on the tile only. The inputs are zeros: the passes
-// do the same work on any data.
+// do the same work on any data. Each candidate runs enough applications for about 2 ms of passes (at least `reps`),
+// twice, and keeps the faster measurement of each pass: a single right-hand side's passes last a few microseconds,
+// and a choice made on one short sample is decided by noise.
 template <typename TC>
 static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipStream_t s) {
     Context *ctx = op.problem->ctx;
@@ -959,6 +961,19 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
     out.allocate(ctx, static_cast<size_t>(op.dims.n) * nrhs, "tuning output");
     in.zero(s);
     PassShape best = default_shape<TC>(op, nrhs);
+    {
+        double ms[3], by[3], wall = 0;
+        time_shape<TC>(op, nrhs, 3, best, in.ptr, out.ptr, ms, by, s, &wall);
+        reps = std::max(reps, std::min(200, static_cast<int>(std::ceil(2.0 / std::max(wall, 1e-3)))));
+    }
+    // Per-pass milliseconds of shape c (and the wall time of one application): the faster of two measurements.
+    auto measure = [&](const PassShape &c, double *ms, double *wall = nullptr) {
+        double by[3], ms2[3], w2 = 0;
+        time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s, wall);
+        time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms2, by, s, wall ? &w2 : nullptr);
+        for (int q = 0; q < 3; ++q) ms[q] = std::min(ms[q], ms2[q]);
+        if (wall) *wall = std::min(*wall, w2);
+    };
     double best_total = 1e300;
     for (int bits = 3; bits <= 4; ++bits) {
         PassShape sh = default_shape<TC>(op, nrhs, bits);
@@ -968,8 +983,8 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
             PassShape c = sh;
             c.group = g;
             if (!shape_fits<TC>(op, c)) continue;
-            double ms[3], by[3];
-            time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s);
+            double ms[3];
+            measure(c, ms);
             if (ms[0] + ms[2] < ac) {
                 ac = ms[0] + ms[2];
                 g_best = g;
@@ -983,8 +998,8 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
                 c.tile = tile;
                 c.teams = teams;
                 if (!shape_fits<TC>(op, c)) continue;
-                double ms[3], by[3];
-                time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s);
+                double ms[3];
+                measure(c, ms);
                 if (ms[1] < b) {
                     b = ms[1];
                     t_best = tile;
@@ -999,15 +1014,15 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
     }
     // Then the chunk count, by the wall time of whole applications (the chunks overlap).
     {
-        double ms[3], by[3], wall = 0;
-        time_shape<TC>(op, nrhs, reps, best, in.ptr, out.ptr, ms, by, s, &wall);
+        double ms[3], wall = 0;
+        measure(best, ms, &wall);
         best_total = wall;
         const PassShape one = best;
         for (int chunks = 2; chunks <= 8 && chunks <= nrhs; chunks *= 2) {
             PassShape c = one;
             c.chunks = chunks;
             if (!b_fits<TC>(op, c)) break;
-            time_shape<TC>(op, nrhs, reps, c, in.ptr, out.ptr, ms, by, s, &wall);
+            measure(c, ms, &wall);
             if (wall < best_total) {
                 best_total = wall;
                 best = c;
