@@ -294,8 +294,9 @@ int rt_server_submit(rt_server *server, const rt_request *request, int64_t *tick
 /* Blocks until the request has run; x (may be NULL) receives the solution (n complex128), out the solver output and
  * times (may be NULL) {queue wait, service, total latency} in seconds. A ticket is waited for once. */
 int rt_server_wait(rt_server *server, int64_t ticket, void *x, int64_t n, rt_output *out, double *times);
-/* Batching by embedding size: queued requests whose FFT periods (next power of two >= 2 count - 1 per direction),
- * face aspect, quadrature orders, preconditioner (near field or none) and solver options are equal join the request
+/* Batching by embedding size: queued requests whose FFT periods (next power of two >= 2 count - 1 per direction,
+ * at most 256: larger grids keep the device busy alone and are solved alone), face aspect, quadrature orders,
+ * preconditioner (near field or none) and solver options are equal join the request
  * a slot takes, up to max_requests (1..64; 1, the default, solves every request alone) and while the group's bytes
  * fit the budget. A group runs on the largest face counts among its requests (the others padded with empty faces),
  * each request with its own generators, spectra, near field and Krylov loop, all of them in one launch per pass of

@@ -52,8 +52,11 @@ void Context::require(uint64_t bytes, const char *what) {
 
 void *Context::alloc(uint64_t bytes, const char *what) {
     require(bytes, what);
+    // Stream-ordered allocation on the context's stream: freeing does not wait for the whole device (hipFree does),
+    // so slots that build and free their requests' buffers do not stall each other, and the device's memory pool
+    // hands freed memory to the next request without a trip to the driver.
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = stream ? hipMallocAsync(&p, bytes, stream) : hipMalloc(&p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         fail(RT_ENOMEM, "rooftopMoM:resources:insufficientMemory", "%s needs %s, but the device refused it (%s).",
@@ -77,7 +80,8 @@ void Context::release(void *ptr) {
         sizes.erase(it);
         held -= bytes;
     }
-    (void)hipFree(ptr);
+    if (stream) (void)hipFreeAsync(ptr, stream);   // after the work queued on the stream (every use of ptr) is done
+    else (void)hipFree(ptr);
 }
 
 void *Context::scratch(int slot, uint64_t bytes, const char *what) {
@@ -106,6 +110,13 @@ void Context::open(int dev, uint64_t budget) {
     RT_HIP(hipMemGetInfo(&free_b, &total_b));
     ceiling = budget ? budget : static_cast<uint64_t>(free_b);
     RT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // The device's pool keeps what the contexts free (up to their ceiling) for their next allocations instead of
+    // returning it to the driver at every synchronization.
+    hipMemPool_t mp = nullptr;
+    if (hipDeviceGetDefaultMemPool(&mp, dev) == hipSuccess && mp) {
+        uint64_t keep = ceiling;
+        (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
+    }
 }
 
 hipStream_t Context::side_stream() {
@@ -135,7 +146,7 @@ void Context::close() {
             (void)hipEventDestroy(e);
             e = nullptr;
         }
-    for (auto &kv : sizes) (void)hipFree(kv.first);
+    for (auto &kv : sizes) (void)hipFree(kv.first);   // the stream is gone: synchronous frees
     sizes.clear();
     pool.clear();
     held = 0;

@@ -99,12 +99,18 @@ struct Server {
     void solve_group(Context &ctx, const std::vector<std::shared_ptr<ServeJob>> &jobs);
 };
 
-// Whether two requests can share one batched solve: the same FFT periods (embedding size), the same face aspect and
-// quadrature orders, the near-field preconditioner or none on both, and equal solver options. Their grids, masks,
-// wavenumbers, face sizes and right-hand sides may differ.
+// Whether two requests can share one batched solve: the same FFT periods (embedding size) of at most kBatchPeriod
+// points, the same face aspect and quadrature orders, the near-field preconditioner or none on both, and equal solver
+// options. Their grids, masks, wavenumbers, face sizes and right-hand sides may differ. Larger grids are solved alone:
+// one of them already keeps the device's memory busy (its kernels are bandwidth-bound), so a group of them would
+// only run one after another in one slot, padded to the largest, instead of side by side in several (measured on the
+// served stream: four grids of 542 to 772 cells a side took 666 ms as a group and at most 389 ms alone).
+constexpr long kBatchPeriod = 256;
+
 static bool batchable(const ServeJob &a, const ServeJob &b) {
     auto period = [](int c) { return c > 1 ? next_pow2(2L * c - 1) : 1L; };
     const SolveOptions &p = a.opts, &q = b.opts;
+    if (period(a.cx) > kBatchPeriod || period(a.cy) > kBatchPeriod) return false;
     return period(a.cx) == period(b.cx) && period(a.cy) == period(b.cy) && a.cx > 1 && a.cy > 1 && b.cx > 1 &&
            b.cy > 1 && a.b * b.a == b.b * a.a && a.near_order == b.near_order && a.far_order == b.far_order &&
            a.preconditioner == b.preconditioner && a.preconditioner != RT_PRECOND_JACOBI &&
