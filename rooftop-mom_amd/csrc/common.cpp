@@ -1,4 +1,5 @@
 // Error state and device-memory accounting of librooftop.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 
@@ -110,11 +111,12 @@ void Context::open(int dev, uint64_t budget) {
     RT_HIP(hipMemGetInfo(&free_b, &total_b));
     ceiling = budget ? budget : static_cast<uint64_t>(free_b);
     RT_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    // The device's pool keeps what the contexts free (up to their ceiling) for their next allocations instead of
-    // returning it to the driver at every synchronization.
+    // The device's pool keeps up to 8 GiB of what the contexts free for their next allocations instead of returning
+    // it to the driver at every synchronization (the buffers of served requests come and go); larger amounts go back,
+    // so that the free memory later contexts measure their ceiling against is not held in the pool.
     hipMemPool_t mp = nullptr;
     if (hipDeviceGetDefaultMemPool(&mp, dev) == hipSuccess && mp) {
-        uint64_t keep = ceiling;
+        uint64_t keep = std::min<uint64_t>(ceiling, 8ull << 30);
         (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
     }
 }
