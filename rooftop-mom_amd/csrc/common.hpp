@@ -95,15 +95,21 @@ struct Context {
     // Kernel timing (rt_ctx_set_timing): while on, the timed kernels (RT_TIMED_* slots of rooftop.h) record a pair of
     // HIP events around their launch on the stream they run on, with their algorithmic bytes; drain_timing() reads
     // the events and accumulates per slot.
+    // bytes: the algorithmic bytes of the launch for the share of a batch's systems it served; full: for every system;
+    // step: the Krylov step the launch belongs to (-1: none), by which the loop sets `bytes` exactly once the
+    // systems' iteration counts are known (each system runs the kernels of the steps up to its last).
     struct TimedRecord {
         hipEvent_t a, b;
         int slot;
         double bytes;
+        double full;
+        int step;
     };
     bool timing = false;
     std::vector<TimedRecord> pending;
     std::vector<hipEvent_t> event_pool;
     double timed_scale = 1.0;   // share of a batch's systems still running (set by a Krylov loop at each poll)
+    int timed_step = -1;        // the Krylov step whose kernels are being enqueued (-1: none)
     int64_t timed_launches[RT_TIMED_COUNT] = {};
     double timed_ms[RT_TIMED_COUNT] = {};
     double timed_bytes[RT_TIMED_COUNT] = {};
@@ -146,7 +152,7 @@ struct KernelTimer {
     }
     void done() {
         if (!a) return;
-        ctx->pending.push_back(Context::TimedRecord{a, b, slot, bytes * ctx->timed_scale});
+        ctx->pending.push_back(Context::TimedRecord{a, b, slot, bytes * ctx->timed_scale, bytes, ctx->timed_step});
         a = b = nullptr;
         if (ctx->pending.size() > 8192) ctx->drain_timing();
     }

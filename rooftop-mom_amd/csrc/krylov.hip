@@ -1564,6 +1564,23 @@ void Workspace::set_scale(const std::vector<LoopState> &st) {
     ctx->timed_scale = static_cast<double>(active) / R;
 }
 
+// The timed launches enqueued since `mark` (pending records of this loop) that belong to a step s get their exact
+// algorithmic bytes: the kernels of step s served the systems whose iteration count reached s, which the loop states
+// read after the step tell. Returns the new mark.
+static size_t settle_timed(Context *ctx, const std::vector<LoopState> &state, size_t mark) {
+    if (!ctx->timing) return mark;
+    if (ctx->pending.size() < mark) mark = 0;   // drained meanwhile: what is pending now is this loop's
+    const int R = static_cast<int>(state.size());
+    for (size_t k = mark; k < ctx->pending.size(); ++k) {
+        Context::TimedRecord &rec = ctx->pending[k];
+        if (rec.step < 0) continue;
+        int active = 0;
+        for (const auto &q : state) active += q.iterations >= rec.step ? 1 : 0;
+        rec.bytes = rec.full * active / R;
+    }
+    return ctx->pending.size();
+}
+
 std::vector<LoopState> Workspace::read_state() {
     RT_HIP(hipMemcpyAsync(host, state, sizeof(LoopState) * R, hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
@@ -1705,11 +1722,13 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
     std::vector<LoopState> state;
     const auto first = ws.fetch_slots({kRnormCg, kBnorm}, &state);
     const std::vector<double> &r0 = first[0], &b0 = first[1];
+    size_t timed_mark = ctx->pending.size();
     int it = 0;
     while (it < max_iter && !all_done(state)) {
         const int window = std::min(kPoll, max_iter - it);
         for (int w = 0; w < window; ++w) {
             ++it;
+            ctx->timed_step = it;
             const c128 *zz = r;
             if (P) {
                 (*P)(r, z, ws.skip());
@@ -1733,7 +1752,9 @@ static InnerResult cocg(Workspace &ws, const LinOp &A, const LinOp *P, const c12
                                it, stall, ws.state, ws.history, ws.cap, 0);
             RT_LAUNCH_CHECK();
         }
+        ctx->timed_step = -1;
         state = ws.read_state();
+        timed_mark = settle_timed(ctx, state, timed_mark);
         ws.set_scale(state);
     }
     ctx->timed_scale = 1.0;
@@ -2079,6 +2100,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
         return false;
     };
     PollPlan plan;
+    size_t timed_mark = ctx->pending.size();
     int it = 0;
     while (it < max_iter && any_active()) {
         hipLaunchKernelGGL(k_cycle_reset, dim3((R + 63) / 64), dim3(64), 0, ws.s, ws.state, R);
@@ -2092,6 +2114,7 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                 const int j = step;
                 ++step;
                 ++it;
+                ctx->timed_step = it;
                 A(pv, t, ws.skip());
                 precondition(t, w);
                 // Algorithmic bytes of the two passes: the live basis columns once, p and w read, q_j and p written.
@@ -2119,7 +2142,9 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                                  n, ws.chunk, ws.scal, it, st);
                 t2.done();
             }
+            ctx->timed_step = -1;
             state = ws.read_state();
+            timed_mark = settle_timed(ctx, state, timed_mark);
             ws.set_scale(state);
             if (all_done(state)) break;
         }

@@ -298,7 +298,8 @@ static void run_chunk(Product &op, const c128 *in, c128 *out, cplx<TC> *work, in
         if (e0) {
             e1 = ctx->take_event();
             RT_HIP(hipEventRecord(e1, s));
-            ctx->pending.push_back(Context::TimedRecord{e0, e1, slot0 + 1, by[1] * ctx->timed_scale});
+            ctx->pending.push_back(
+                Context::TimedRecord{e0, e1, slot0 + 1, by[1] * ctx->timed_scale, by[1], ctx->timed_step});
         }
     }
     KernelTimer tc(ctx, slot0 + 2, by[2]);
