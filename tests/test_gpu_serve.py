@@ -105,3 +105,23 @@ def test_requests_of_one_embedding_size_are_solved_together(rt):
         ax = ex * S.split_fft_product(S.subband_spectra(p.generators(), grid), grid, x)
         assert rel(ax, q['rhs']) < 1.01e-6
         assert np.all(x[ex == 0] == 0)
+
+
+def test_large_grids_are_solved_alone(rt):
+    """Grids of FFT periods above 256 (sides above 128) keep the device busy alone: queued requests of one such period
+    are not grouped (the batch statistics stay at zero), and each result converges."""
+    import bench
+    qs = [bench.served_request(rt, 40 + i, side=s) for i, s in enumerate((140, 150, 160, 40))]
+    solver = rt.Solver('gmres', tolerance=1e-6, restart=50, maxIterationCount=3000, precision='c64')
+    server = rt.Server(slots=1, batch=8)
+    first = server.submit(qs[-1]['L'], qs[-1]['L'], qs[-1]['a'], qs[-1]['a'], qs[-1]['k'], qs[-1]['rhs'],
+                          mask=qs[-1]['mask'], solver=solver, preconditioner='nearField')
+    tickets = [server.submit(q['L'], q['L'], q['a'], q['a'], q['k'], q['rhs'], mask=q['mask'], solver=solver,
+                             preconditioner='nearField') for q in qs[:-1]]
+    out = [server.wait(t, q['n']) for t, q in zip(tickets, qs[:-1])]
+    server.wait(first, qs[-1]['n'])
+    stats = server.stats()
+    server.close()
+    assert stats['batched_solves'] == 0 and stats['batched_requests'] == 0
+    for x, res, times in out:
+        assert res.converged() and res.residualNorm() < 1e-6
