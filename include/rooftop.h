@@ -220,8 +220,9 @@ int rt_product_profile(rt_product *product, int64_t nrhs, int32_t reps, int32_t 
  * of the three passes for nrhs right-hand sides in the given precision -- plan radices up to 8 or 16, pass A/C
  * rooftop columns per block, pass B x positions per block and teams per position, right-hand-side chunks -- and
  * keeps the fastest for every later application with that many right-hand sides. shape (may be NULL, else 5
- * entries) receives {largest radix bits, columns per block, positions per block, pass B teams per position (1: one
- * team transforms both columns of a position, 2: a team per column), chunks (the right-hand sides split into chunks
+ * entries) receives {largest radix bits, columns per block, positions per block, pass B form (1: one team
+ * transforms both columns of a position in LDS, 2: a team per column, 3: one team, the two columns in turn through
+ * one column of LDS), chunks (the right-hand sides split into chunks
  * that alternate between two streams, so that one chunk's passes overlap another's)}; ms (may be NULL) the wall
  * time of one application with it. */
 int rt_product_tune(rt_product *product, int64_t nrhs, int32_t precision, int32_t *shape, double *ms);

@@ -274,6 +274,123 @@ __global__ void __launch_bounds__(MAXT) pass_b2_k(cplx<TC> *__restrict__ work, c
         pass_b_team<TC, P, 1>(t, col1, col0, work + work_index(d, r, 1, px, 0), d.cols_y, tw, row, dxv, dy);
 }
 
+// ------------------------------------------------------------------------------------------------------------------
+// Pass B through one column of LDS: the team transforms the X column of its x position, keeps the outputs of the
+// last forward round in registers (E per thread), transforms the Y column through the same LDS, multiplies in
+// registers and runs the inverse transforms, Y first, back through the LDS. Half the LDS per position of pass_b_k,
+// for twice the barriers and a column held in registers: long columns fit twice the blocks per CU, and 8192-point
+// complex128 columns fit one launch.
+// ------------------------------------------------------------------------------------------------------------------
+template <typename TC, class P>
+struct HeldColumn {
+    static constexpr int R = 1 << P::lr(P::K - 1), U = P::E / R;   // the last round: radix R, U butterflies
+    cplx<TC> v[U][R];                                               // position (t + u T) R + q at v[u][q]
+};
+
+// Forward y transform of the work column w (n values, zero beyond) through buf; the last round stays in h.
+template <typename TC, class P>
+__device__ __forceinline__ void b1_forward(int t, cplx<TC> *buf, const cplx<TC> *__restrict__ w, int n,
+                                           const cplx<TC> *__restrict__ tw, HeldColumn<TC, P> &h) {
+    constexpr int R = HeldColumn<TC, P>::R, U = HeldColumn<TC, P>::U;
+    cround<TC, P, 0, false, true, true>(
+        t, tw, [&](int pos, int) { return clamped_load(w, pos, n); }, [&](int, int pp, cplx<TC> v) { buf[pp] = v; });
+    __syncthreads();
+    static_for<1, P::K - 1>([&](auto S) {
+        cround<TC, P, decltype(S)::value, false, false, true>(
+            t, tw, [&](int, int pp) { return buf[pp]; }, [&](int, int pp, cplx<TC> v) { buf[pp] = v; });
+        __syncthreads();
+    });
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int pb = P::pad((t + u * P::T) * R);
+#pragma unroll
+        for (int q = 0; q < R; ++q) h.v[u][q] = buf[pb + P::pad(q)];
+        dft<TC, R, false, false>(h.v[u]);
+    }
+    __syncthreads();   // buf is free for the next column
+}
+
+// Inverse y transform of the held column h (its first inverse round already applied) through buf into the work
+// column w, cut to n values.
+template <typename TC, class P>
+__device__ __forceinline__ void b1_inverse(int t, cplx<TC> *buf, cplx<TC> *__restrict__ w, int n,
+                                           const cplx<TC> *__restrict__ tw, const HeldColumn<TC, P> &h) {
+    constexpr int R = HeldColumn<TC, P>::R, U = HeldColumn<TC, P>::U;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int pb = P::pad((t + u * P::T) * R);
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[pb + P::pad(q)] = h.v[u][q];
+    }
+    __syncthreads();
+    static_for_down<1, P::K - 1>([&](auto S) {
+        cround<TC, P, decltype(S)::value, true, false, true>(
+            t, tw, [&](int, int pp) { return buf[pp]; }, [&](int, int pp, cplx<TC> v) { buf[pp] = v; });
+        __syncthreads();
+    });
+    cround<TC, P, 0, true, true, true>(
+        t, tw, [&](int, int pp) { return buf[pp]; },
+        [&](int pos, int, cplx<TC> v) {
+            if (pos < n) w[pos] = v;
+        });
+    __syncthreads();   // buf is free for the next column
+}
+
+template <typename TC, int LOG2M, int BITS, int MAXT>
+__global__ void __launch_bounds__(MAXT) pass_b1_k(cplx<TC> *__restrict__ work, const cplx<TC> *__restrict__ bspec,
+                                                  const int *__restrict__ fold_x, const int *__restrict__ border,
+                                                  const cplx<TC> *__restrict__ dx,
+                                                  const cplx<TC> *__restrict__ dy, ProdDims d,
+                                                  const cplx<TC> *__restrict__ tw, int tile, int R, int nblocks,
+                                                  const int *__restrict__ skip, int skip_stride) {
+    using P = BPlan<LOG2M, BITS, sizeof(cplx<TC>)>;
+    static_assert(P::K >= 2, "one column of LDS needs at least two rounds");
+    constexpr int RL = HeldColumn<TC, P>::R, U = HeldColumn<TC, P>::U;
+    constexpr int CH = sizeof(TC) == sizeof(float) ? 4 : 2;   // both columns are live: fewer spectra in flight
+    const int lb = xcd_block(blockIdx.x, nblocks);
+    if (lb < 0) return;
+    const int r = lb % R, pt = lb / R;
+    if (skip && skip[r * skip_stride]) return;
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int Mp = P::pad(P::M - 1) + 1;
+    const int tt = threadIdx.x / P::T, t = threadIdx.x - tt * P::T;
+    const int px = border[pt * tile + tt];   // partners k and M - k run together
+    cplx<TC> *buf = reinterpret_cast<cplx<TC> *>(smem_raw) + tt * Mp;
+    cplx<TC> *w0 = work + work_index(d, r, 0, px, 0);
+    cplx<TC> *w1 = work + work_index(d, r, 1, px, 0);
+    const cplx<TC> *row = bspec + r * d.spec_stride + static_cast<int64_t>(fold_x[px]) * d.My * 3;
+    dx += r * d.dx_stride;
+    dy += r * d.dy_stride;
+    const cplx<TC> dxv = dx[px];
+    // Each transform reads the twiddles through its own opaque pointer: the loads of one column's rounds are not
+    // merged with another's, which would hold every round's twiddles in registers across a whole transform.
+    HeldColumn<TC, P> X, Y;
+    b1_forward<TC, P>(t, buf, w0, d.cols_x, opaque(tw), X);
+    b1_forward<TC, P>(t, buf, w1, d.cols_y, opaque(tw), Y);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int base = (t + u * P::T) * RL;
+        const cplx<TC> *sp = row + 3 * base;
+        const cplx<TC> *dyp = dy + base;
+#pragma unroll
+        for (int q0 = 0; q0 < RL; q0 += CH) {
+#pragma unroll
+            for (int q = q0; q < q0 + CH && q < RL; ++q) {
+                const cplx<TC> a = sp[3 * q], b = sp[3 * q + 1], el = sp[3 * q + 2];
+                const cplx<TC> dyv = dyp[q];
+                const cplx<TC> phi = el * (cmulc(dxv, X.v[u][q]) + cmulc(dyv, Y.v[u][q]));
+                X.v[u][q] = a * X.v[u][q] - dxv * phi;
+                Y.v[u][q] = b * Y.v[u][q] - dyv * phi;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        dft<TC, RL, true, false>(X.v[u]);
+        dft<TC, RL, true, false>(Y.v[u]);
+    }
+    b1_inverse<TC, P>(t, buf, w1, d.cols_y, opaque(tw), Y);
+    b1_inverse<TC, P>(t, buf, w0, d.cols_x, opaque(tw), X);
+}
+
 template <typename TC>
 void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty, cplx<TC> *work,
                    const cplx<TC> *bspec, int nrhs, int tile, int teams, hipStream_t s, const int *skip,
@@ -281,7 +398,7 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
     const FftPlan &p = ty.plan;
     constexpr int es = sizeof(cplx<TC>);
     const int64_t nb = static_cast<int64_t>(op.dims.Mx / tile) * nrhs;
-    const size_t lds = lds_b_bytes(p, tile, es);
+    const size_t lds = lds_b_bytes(p, tile, es, teams);
     const int threads = team_threads(p) * tile;
     const bool dbl = sizeof(TC) == sizeof(double);
     const cplx<TC> *tw = dbl ? reinterpret_cast<const cplx<TC> *>(ty.tw.ptr)
@@ -318,6 +435,14 @@ void launch_pass_b(const Product &op, const PlanTables &tx, const PlanTables &ty
             launch_timed(timer, pass_b_k<TC, L, 3, MT, false>, dim3(xcd_grid(nb)), dim3(threads),                  \
                          static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, tile, nrhs, \
                          static_cast<int>(nb), skip, skip_stride);                                                 \
+        } else if (teams == 3) {                                                                                   \
+            if constexpr (PL::K >= 2) {                                                                            \
+                constexpr int MT = PL::T > 256 ? PL::T : 256;                                                      \
+                set_lds(reinterpret_cast<const void *>(&pass_b1_k<TC, L, B, MT>), lds);                            \
+                launch_timed(timer, pass_b1_k<TC, L, B, MT>, dim3(xcd_grid(nb)), dim3(threads),                    \
+                             static_cast<uint32_t>(lds), s, work, bspec, tx.fold.ptr, tx.border.ptr, dx, dy, op.dims, tw, \
+                             tile, nrhs, static_cast<int>(nb), skip, skip_stride);                                 \
+            }                                                                                                      \
         } else if (teams == 1) {                                                                                   \
             constexpr int MT = PL::T > 256 ? PL::T : 256;                                                          \
             set_lds(reinterpret_cast<const void *>(&pass_b_k<TC, L, B, MT>), lds);                                 \

@@ -70,8 +70,11 @@ __host__ __device__ constexpr int b_pad(const BPad &r, int p) { return p + r.c1 
 
 template <int LOG2M, int BITS, int ES>
 struct BPlan : CPlan<LOG2M, BITS> {
-    static constexpr BPad rule = b_pad_rule(LOG2M, BITS, ES);
-    static constexpr int pad(int p) { return b_pad(rule, p); }
+    // The rule's fields as immediates: b_pad(rule, p) takes the rule by reference, which makes device code read it
+    // from memory and multiply at run time, so every LDS offset would need an address register of its own.
+    static constexpr int A1 = b_pad_rule(LOG2M, BITS, ES).a1, C1 = b_pad_rule(LOG2M, BITS, ES).c1;
+    static constexpr int A2 = b_pad_rule(LOG2M, BITS, ES).a2, C2 = b_pad_rule(LOG2M, BITS, ES).c2;
+    static constexpr int pad(int p) { return p + C1 * (p >> A1) + C2 * (p >> A2); }
 };
 
 template <int I>
@@ -189,11 +192,13 @@ inline int64_t blocks_ac(const ProdDims &d, int nrhs, int g) {
 // (512 for complex128 with radix 16, whose butterflies need more registers), pass B one team of at least 256.
 inline int max_threads_ac(int es, const FftPlan &p) { return es == 16 && p.log2E >= 4 ? 512 : 1024; }
 inline int max_threads_b(const FftPlan &p) { return team_threads(p) > 256 ? team_threads(p) : 256; }
-// Whether pass B can run `tile` x positions per block with `teams` teams per position (2: a team per column, which
-// needs two rounds and whole waves per team, and at most 1024 threads).
+// Whether pass B can run `tile` x positions per block in form `teams`: 1, one team per position with both columns in
+// LDS; 2, a team per column (two rounds and whole waves per team, at most 1024 threads); 3, one team per position
+// whose columns take turns in one column of LDS (at least two rounds).
 inline bool b_threads_fit(const FftPlan &p, int tile, int teams) {
     const int T = team_threads(p);
     if (teams == 1) return T * tile <= max_threads_b(p);
+    if (teams == 3) return p.K >= 2 && T * tile <= max_threads_b(p);
     const int per = 2 * T;
     return teams == 2 && p.K >= 2 && T % 64 == 0 && per <= 1024 && per * tile <= (per > 256 ? per : 256);
 }
@@ -212,8 +217,8 @@ inline size_t lds_ac_bytes(const FftPlan &p, int group, int es) {
 }
 // Padded length of a pass-B column (the BPlan layout).
 inline int b_padded_length(const FftPlan &p, int es) { return b_pad(b_pad_rule(p.log2M, p.bits, es), p.M - 1) + 1; }
-inline size_t lds_b_bytes(const FftPlan &p, int tile, int es) {
-    return static_cast<size_t>(2) * tile * b_padded_length(p, es) * es;
+inline size_t lds_b_bytes(const FftPlan &p, int tile, int es, int teams = 1) {
+    return static_cast<size_t>(teams == 3 ? 1 : 2) * tile * b_padded_length(p, es) * es;
 }
 // Pass B in three launches (pass_b_split.inl) holds one column per block.
 inline size_t lds_b_split_bytes(const FftPlan &p, int es) { return static_cast<size_t>(padded_length(p)) * es; }

@@ -145,7 +145,7 @@ void set_lds(const void *kernel, size_t bytes) {
 template <typename TC>
 static bool b_fits(const Product &op, const PassShape &sh) {
     const FftPlan &py = op.ty.of(sh.bits).plan;
-    return b_threads_fit(py, sh.tile, sh.teams) && lds_b_bytes(py, sh.tile, sizeof(cplx<TC>)) <= lds_bytes_max();
+    return b_threads_fit(py, sh.tile, sh.teams) && lds_b_bytes(py, sh.tile, sizeof(cplx<TC>), sh.teams) <= lds_bytes_max();
 }
 // Whether pass B can run in three launches instead (one column per block).
 template <typename TC>
@@ -993,7 +993,7 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
         }
         if (!g_best) continue;
         sh.group = g_best;
-        for (int teams = 1; teams <= 2; ++teams)
+        for (int teams = 1; teams <= 3; ++teams)
             for (int tile = 1; tile <= 16; tile *= 2) {
                 PassShape c = sh;
                 c.tile = tile;
@@ -1063,9 +1063,9 @@ void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
     const PassShape sh{shape[0], shape[1], shape[2], shape[3], shape[4]};
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
     if ((sh.bits != 3 && sh.bits != 4) || !pow2(sh.group) || !pow2(sh.tile) || sh.group > 16 || sh.tile > 16 ||
-        (sh.teams != 1 && sh.teams != 2) || sh.chunks < 1 || sh.chunks > 64)
+        sh.teams < 1 || sh.teams > 3 || sh.chunks < 1 || sh.chunks > 64)
         fail(RT_EINVAL, "rooftop:product",
-             "a shape is {3 or 4, a power of two <= 16, a power of two <= 16, 1 or 2, 1 .. 64}");
+             "a shape is {3 or 4, a power of two <= 16, a power of two <= 16, 1 .. 3, 1 .. 64}");
     const bool fits = prec == RT_C128 ? shape_fits<double>(op, sh) : shape_fits<float>(op, sh);
     if (!fits) fail(RT_EINVAL, "rooftop:product", "the shape does not fit the device for this product");
     auto &tuned = prec == RT_C128 ? op.tuned64 : op.tuned32;

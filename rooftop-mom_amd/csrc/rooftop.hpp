@@ -77,7 +77,8 @@ struct PassShape {
     int bits = 4;    // largest radix of the plans: 2^bits elements per thread
     int group = 1;   // pass A / C: rooftop columns per block
     int tile = 1;    // pass B: x positions per block
-    int teams = 1;   // pass B: teams per x position (1: one team runs both columns; 2: a team per column)
+    int teams = 1;   // pass B form: 1, one team runs both columns of an x position; 2, a team per column; 3, one
+                     // team whose columns take turns in one column of LDS
     int chunks = 1;  // right-hand-side chunks, alternating between the main and the side stream
 };
 

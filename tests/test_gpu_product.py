@@ -154,9 +154,10 @@ def _oracle_product(p, cx, cy, a, b, x):
 
 @pytest.mark.parametrize('size', [(17, 33), (64, 3), (3, 100), (130, 40), (1, 40), (40, 1), (7, 200), (20, 300)])
 def test_product_shapes_across_transform_lengths(rt, size):
-    """Every launch shape the autotuner may pick (both plans, all column groups, tiles and pass B teams that fit)
-    gives the same product, on grids whose embeddings cover transform lengths 1 .. 1024 (a team per column needs
-    y transforms of at least 512 points)."""
+    """Every launch shape the autotuner may pick (both plans, all column groups, tiles and pass B forms that fit:
+    both columns in LDS, a team per column, the columns in turn through one column of LDS) gives the same product, on
+    grids whose embeddings cover transform lengths 1 .. 1024 (a team per column needs y transforms of at least 512
+    points, one column of LDS at least two rounds)."""
     cx, cy = size
     print('shapes', size, flush=True)
     p = _problem(rt, cx, cy, 0.1, 0.1, (3, 3))
@@ -167,7 +168,7 @@ def test_product_shapes_across_transform_lengths(rt, size):
         for bits in (3, 4):
             for group in (1, 2, 4, 8, 16):
                 for tile, teams, chunks in ((1, 1, 1), (4, 1, 1), (16, 1, 1), (1, 2, 1), (2, 2, 1), (1, 1, 2),
-                                            (1, 2, 3)):
+                                            (1, 2, 3), (1, 3, 1), (4, 3, 1), (1, 3, 2)):
                     try:
                         op.setShape({'bits': bits, 'group': group, 'tile': tile, 'teams': teams, 'chunks': chunks},
                                     nrhs=3, precision=prec)
@@ -206,3 +207,8 @@ def test_long_columns_three_launch_pass_b(rt):
     op = rt.Operator(p, precision='mixed')
     assert rel(op.apply(x), ref) < 1e-12
     assert rel(op.apply(x, precision='c64'), ref) < 5e-6
+    # The same columns through one column of LDS per position (one launch), in both plans and precisions.
+    for prec, tol in (('c128', 1e-12), ('c64', 5e-6)):
+        for bits in (3, 4):
+            op.setShape({'bits': bits, 'group': 1, 'tile': 1, 'teams': 3, 'chunks': 1}, nrhs=2, precision=prec)
+            assert rel(op.apply(x, precision=prec), ref) < tol, (prec, bits)
