@@ -164,24 +164,32 @@ static bool shape_fits(const Product &op, const PassShape &sh) {
     return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx && ac && b;
 }
 
-// Default launch shape (the autotuner may replace it): radices up to 16 in complex64 and up to 8 in complex128;
-// pass A/C groups of 64-byte column segments while the blocks stay numerous; pass B teams of up to 256 threads.
+// Default launch shape (the autotuner may replace it), from the tuner's choices measured over plates of 32 to 2048
+// cells at one and eight right-hand sides: radices up to 8 (complex64 transforms up to 2048 points, complex128 up to
+// 512), else up to 16; pass A/C groups of four columns where they fit a block; pass B through one column of LDS for
+// transforms of at least 128 points in complex64 and 1024 in complex128; pass B tiles of up to 256 threads while the
+// blocks stay a few dozen. The plan and the pass B form depend on the transform lengths alone, so a product runs the
+// same kernels (and the same arithmetic per right-hand side) whatever the batch size.
+int default_bits(const ProdDims &d, bool c64) { return std::max(d.Mx, d.My) <= (c64 ? 2048 : 512) ? 3 : 4; }
+
 template <typename TC>
 static PassShape default_shape(const Product &op, int nrhs, int bits = 0) {
     const ProdDims &d = op.dims;
+    const bool c64 = sizeof(TC) == sizeof(float);
     PassShape s;
-    s.bits = bits ? bits : sizeof(TC) == sizeof(float) ? 4 : 3;
-    if (!op.embedding || op.four_plane) s.bits = 3;   // face-grid and four-plane passes: the radix-8 plans
+    s.bits = bits ? bits : default_bits(d, c64);
+    const bool plain = op.embedding && !op.four_plane;
+    if (!plain) s.bits = 3;   // face-grid and four-plane passes: the radix-8 plans
     const FftPlan &px = op.tx.of(s.bits).plan, &py = op.ty.of(s.bits).plan;
     const int es = sizeof(cplx<TC>);
-    int g = std::max(1, 64 / es);
-    while (g > 1 && (team_threads(px) * g > std::min(1024, max_threads_ac(es, px)) ||
-                     lds_ac_bytes(px, g, es) > 64 * 1024 || blocks_ac(d, nrhs, g) < 1024))
+    s.teams = plain && py.K >= 2 && py.M >= (c64 ? 128 : 1024) ? 3 : 1;
+    int g = 4;
+    while (g > 1 && (team_threads(px) * g > max_threads_ac(es, px) || lds_ac_bytes(px, g, es) > lds_bytes_max()))
         g >>= 1;
     s.group = g;
     int tile = 1;
-    while (tile * 2 <= d.Mx && team_threads(py) * tile * 2 <= 256 && lds_b_bytes(py, tile * 2, es) <= 48 * 1024 &&
-           static_cast<int64_t>(d.Mx / (tile * 2)) * nrhs >= 2048)
+    while (tile * 2 <= d.Mx && team_threads(py) * tile * 2 <= 256 &&
+           lds_b_bytes(py, tile * 2, es, s.teams) <= 48 * 1024 && static_cast<int64_t>(d.Mx / (tile * 2)) * nrhs >= 16)
         tile *= 2;
     s.tile = tile;
     return s;
@@ -509,8 +517,8 @@ std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier
         RT_HIP(hipStreamSynchronize(s));
     }
     // The pass-B spectra of the default plans of the precisions held.
-    if (op->has(RT_C128)) bspec_of<double>(*op, 3, s);
-    if (op->has(RT_C64)) bspec_of<float>(*op, 4, s);
+    if (op->has(RT_C128)) bspec_of<double>(*op, default_bits(op->dims, false), s);
+    if (op->has(RT_C64)) bspec_of<float>(*op, default_bits(op->dims, true), s);
     RT_HIP(hipStreamSynchronize(s));
     return op;
 }

@@ -154,6 +154,9 @@ std::shared_ptr<Problem> make_problem(Context *ctx, int cx, int cy, double a, do
 // The dimensions of the Green's matrix product of a grid: rooftop rows and columns, the power-of-two embedding
 // periods M >= 2 count - 1 and the folded spectrum extents M / 2 + 1.
 ProdDims operator_dims(const GridInfo &g);
+// Radix bits (3: radices up to 8, 4: up to 16) of the default plan of a product's transforms in complex64 or
+// complex128; its pass-B spectra are built with the product.
+int default_bits(const ProdDims &d, bool c64);
 std::unique_ptr<Product> create_operator(Problem *p, int precision, int modifier);
 // The circulant preconditioner of the problem on its face grid (RT_CIRCULANT_* kernel, RT_BUILDER_* builder).
 std::unique_ptr<Product> create_circulant(Problem *p, int kernel, int builder, int bandwidth_x, int bandwidth_y,

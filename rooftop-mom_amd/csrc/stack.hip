@@ -61,10 +61,11 @@ StackedProducts create_stacked(Context *ctx, const std::vector<std::shared_ptr<P
         op->ty.p[v].d.allocate(ctx, static_cast<size_t>(R) * d.My, "divergence tables of the batch");
         op->ty.p[v].d32.allocate(ctx, static_cast<size_t>(R) * d.My, "divergence tables of the batch");
     }
-    // The pass-B spectra of the default plans (radices up to 8 in complex128, up to 16 in complex64).
+    // The pass-B spectra of the default plans, which every member built with its product.
     const bool c128 = held != RT_C64, c64 = held != RT_C128;
-    if (c128) op->bspec64[0].allocate(ctx, static_cast<size_t>(R) * d.spec_stride, "The spectra of the batch");
-    if (c64) op->bspec32[1].allocate(ctx, static_cast<size_t>(R) * d.spec_stride, "The spectra of the batch");
+    const int v64 = default_bits(d, false) >= 4 ? 1 : 0, v32 = default_bits(d, true) >= 4 ? 1 : 0;
+    if (c128) op->bspec64[v64].allocate(ctx, static_cast<size_t>(R) * d.spec_stride, "The spectra of the batch");
+    if (c64) op->bspec32[v32].allocate(ctx, static_cast<size_t>(R) * d.spec_stride, "The spectra of the batch");
     std::vector<std::unique_ptr<Product>> pcs;
     for (int i = 0; i < R; ++i) {
         std::unique_ptr<Product> m = create_operator(members[i].get(), held, RT_MODIFIER_NONE);
@@ -77,8 +78,8 @@ StackedProducts create_stacked(Context *ctx, const std::vector<std::shared_ptr<P
             copy_device(op->ty.p[v].d.ptr + static_cast<int64_t>(i) * d.My, m->ty.p[v].d.ptr, d.My, s);
             copy_device(op->ty.p[v].d32.ptr + static_cast<int64_t>(i) * d.My, m->ty.p[v].d32.ptr, d.My, s);
         }
-        if (c128) copy_device(op->bspec64[0].ptr + i * d.spec_stride, m->bspec64[0].ptr, d.spec_stride, s);
-        if (c64) copy_device(op->bspec32[1].ptr + i * d.spec_stride, m->bspec32[1].ptr, d.spec_stride, s);
+        if (c128) copy_device(op->bspec64[v64].ptr + i * d.spec_stride, m->bspec64[v64].ptr, d.spec_stride, s);
+        if (c64) copy_device(op->bspec32[v32].ptr + i * d.spec_stride, m->bspec32[v32].ptr, d.spec_stride, s);
         if (preconditioner) pcs.push_back(create_preconditioner(members[i].get(), preconditioner, held));
         RT_HIP(hipStreamSynchronize(s));   // the copies have read the member's spectra before they are freed
     }

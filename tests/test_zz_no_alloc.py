@@ -64,9 +64,9 @@ def test_noalloc_apply_refuses_what_needs_memory(rt):
     with pytest.raises(rt.RooftopError) as e:
         _apply(rt, 'rt_product_apply_device_noalloc', op, x, y, n, 8, 'c128')
     assert e.value.identifier == 'rooftop:product:noAlloc'
-    # A launch shape whose pass-B spectra are not built yet (radix-16 plans in complex128) is refused until the
-    # product has been applied once with it.
-    op.setShape({'bits': 4, 'group': 2, 'tile': 1}, nrhs=2, precision='c128')
+    # A launch shape whose pass-B spectra are not built yet (complex128 1024-point transforms default to radices up
+    # to 16, so the radix-8 plan's) is refused until the product has been applied once with it.
+    op.setShape({'bits': 3, 'group': 2, 'tile': 1}, nrhs=2, precision='c128')
     with pytest.raises(rt.RooftopError) as e:
         _apply(rt, 'rt_product_apply_device_noalloc', op, x, y, n, 2, 'c128')
     assert e.value.identifier == 'rooftop:product:noAlloc'
