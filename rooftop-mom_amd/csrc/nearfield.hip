@@ -10,9 +10,9 @@
 // neighbourhood has the same existing rooftops shares its row: rows are fitted once per class, and P is applied as
 // an 18-point stencil over the two rooftop grids with the class coefficients.
 #include <algorithm>
-#include <unordered_map>
 #include <vector>
 
+#include "nearfield_classes.hpp"
 #include "rooftop.hpp"
 
 namespace rt {
@@ -336,20 +336,6 @@ __global__ void to_c64_pc(const c128 *__restrict__ a, c64 *__restrict__ b, int64
 // ------------------------------------------------------------------------------------------------------------------
 // Host side.
 // ------------------------------------------------------------------------------------------------------------------
-// Existence of every rooftop of the grid (a rooftop exists if both its faces are metal; all of them on a full plate).
-static std::vector<uint8_t> rooftop_existence(const Problem &p) {
-    const GridInfo &g = p.g;
-    std::vector<uint8_t> ex(static_cast<size_t>(g.n), 1);
-    if (p.face_mask.empty()) return ex;
-    auto face = [&](int x, int y) { return p.face_mask[x + static_cast<size_t>(g.cx) * y] != 0; };
-    for (int j = 0; j < g.cols_x; ++j)
-        for (int i = 0; i < g.rows_x; ++i) ex[i + static_cast<size_t>(g.rows_x) * j] = face(i, j) && face(i + 1, j);
-    for (int j = 0; j < g.cols_y; ++j)
-        for (int i = 0; i < g.rows_y; ++i)
-            ex[g.nx + i + static_cast<size_t>(g.rows_y) * j] = face(i, j) && face(i, j + 1);
-    return ex;
-}
-
 static GenView view_of(Problem *p) {
     GenView v;
     v.mxx = p->g.has_x ? p->mxx.ptr : nullptr;
@@ -401,54 +387,16 @@ std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precisi
         }
     } else if (kind == RT_PRECOND_NEARFIELD) {
         pc->type = kNearField;
-        const std::vector<uint8_t> ex = rooftop_existence(*p);
-        auto exists_at = [&](int d, int x, int y) -> bool {
-            const int rows = d ? g.rows_y : g.rows_x, cols = d ? g.cols_y : g.cols_x;
-            if (x < 0 || y < 0 || x >= rows || y >= cols) return false;
-            return ex[(d ? g.nx : 0) + x + static_cast<size_t>(rows) * y] != 0;
-        };
-        std::vector<int> cls(static_cast<size_t>(g.n), -1);
-        std::vector<int> rep_dir;
-        std::vector<unsigned long long> rep_bits;
-        std::unordered_map<unsigned long long, int> index;
-        for (int d = 0; d < 2; ++d) {
-            const int rows = d ? g.rows_y : g.rows_x, cols = d ? g.cols_y : g.cols_x;
-            if (!(d ? g.has_y : g.has_x)) continue;
-            for (int y = 0; y < cols; ++y)
-                for (int x = 0; x < rows; ++x) {
-                    const size_t i = (d ? g.nx : 0) + x + static_cast<size_t>(rows) * y;
-                    if (!ex[i]) continue;
-                    unsigned long long bits = 0;
-                    for (int dd = 0; dd < 2; ++dd)
-                        for (int dy = -2; dy <= 2; ++dy)
-                            for (int dx = -2; dx <= 2; ++dx)
-                                if (exists_at(dd, x + dx, y + dy))
-                                    bits |= 1ull << (dd * 25 + (dx + 2) + 5 * (dy + 2));
-                    const unsigned long long key = bits | (static_cast<unsigned long long>(d) << 60);
-                    auto it = index.find(key);
-                    if (it == index.end()) {
-                        it = index.emplace(key, static_cast<int>(rep_dir.size())).first;
-                        rep_dir.push_back(d);
-                        rep_bits.push_back(bits);
-                    }
-                    cls[i] = it->second;
-                }
-        }
-        const int classes = static_cast<int>(rep_dir.size());
+        // Classes (5 x 5 windows: one least-squares fit each) and stencil classes (a class, a direction and the
+        // neighbours' classes: one symmetrized stencil each), formed on the device.
+        NearFieldClasses nc = classify_near_field(*p, s);
+        const int classes = nc.classes, stencils = nc.stencils;
         pc->class_count = classes;
-        pc->cls.allocate(ctx, cls.size(), "The near-field classes");
-        RT_HIP(hipMemcpyAsync(pc->cls.ptr, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        pc->cls = std::move(nc.cls);
         pc->coef64.allocate(ctx, static_cast<size_t>(classes) * kSlots, "The near-field coefficients");
         if (classes > 0) {
-            DeviceArray<int> d_dir;
-            DeviceArray<unsigned long long> d_bits;
-            d_dir.allocate(ctx, classes, "near-field fit");
-            d_bits.allocate(ctx, classes, "near-field fit");
-            RT_HIP(hipMemcpyAsync(d_dir.ptr, rep_dir.data(), classes * sizeof(int), hipMemcpyHostToDevice, s));
-            RT_HIP(hipMemcpyAsync(d_bits.ptr, rep_bits.data(), classes * sizeof(unsigned long long),
-                                  hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(sai_fit, dim3((classes + kFitWaves - 1) / kFitWaves), dim3(64 * kFitWaves), 0, s, v,
-                               d_dir.ptr, d_bits.ptr, classes, pc->coef64.ptr, bad.ptr);
+                               nc.rep_dir.ptr, nc.rep_bits.ptr, classes, pc->coef64.ptr, bad.ptr);
             RT_LAUNCH_CHECK();
             int h_bad = 0;
             RT_HIP(hipMemcpyAsync(&h_bad, bad.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -464,71 +412,23 @@ std::unique_ptr<Product> create_preconditioner(Problem *p, int kind, int precisi
                                0, s, pc->coef64.ptr, pc->coef32.ptr, static_cast<int64_t>(classes) * kSlots);
             RT_LAUNCH_CHECK();
         }
-        // Stencil classes: (own class, own direction, the 18 neighbours' classes), deduplicated.
-        {
-            std::vector<int> scls(static_cast<size_t>(g.n), -1), tuples;
-            std::unordered_map<uint64_t, std::vector<int>> buckets;   // hash -> stencil classes with that hash
-            int key[kSlots + 2];
-            for (int64_t i = 0; i < g.n; ++i) {
-                if (cls[i] < 0) continue;
-                const int di = i < g.nx ? 0 : 1;
-                const int64_t k = di ? i - g.nx : i;
-                const int rows_i = di ? g.rows_y : g.rows_x;
-                const int fx = static_cast<int>(k % rows_i), fy = static_cast<int>(k / rows_i);
-                key[0] = cls[i];
-                key[1] = di;
-                for (int d = 0; d < 2; ++d) {
-                    const int rows = d ? g.rows_y : g.rows_x, cols = d ? g.cols_y : g.cols_x;
-                    const bool present = d ? g.has_y : g.has_x;
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            const int gx = fx + dx, gy = fy + dy;
-                            int cj = -1;
-                            if (present && gx >= 0 && gy >= 0 && gx < rows && gy < cols)
-                                cj = cls[(d ? g.nx : 0) + gx + static_cast<int64_t>(rows) * gy];
-                            key[2 + d * 9 + (dx + 1) + 3 * (dy + 1)] = cj;
-                        }
-                }
-                uint64_t h = 1469598103934665603ull;
-                for (int q = 0; q < kSlots + 2; ++q) h = (h ^ static_cast<uint32_t>(key[q])) * 1099511628211ull;
-                auto &bucket = buckets[h];
-                int id = -1;
-                for (int t : bucket)
-                    if (std::equal(key, key + kSlots + 2, tuples.begin() + static_cast<int64_t>(t) * (kSlots + 2))) {
-                        id = t;
-                        break;
-                    }
-                if (id < 0) {
-                    id = static_cast<int>(tuples.size() / (kSlots + 2));
-                    tuples.insert(tuples.end(), key, key + kSlots + 2);
-                    bucket.push_back(id);
-                }
-                scls[i] = id;
+        pc->stencil_count = stencils;
+        pc->scls = std::move(nc.scls);
+        if (stencils > 0) {
+            const unsigned nb = blocks_for(static_cast<size_t>(stencils) * kSlots, 256, 65535);
+            if (precision != RT_C64) {
+                pc->scoef64.allocate(ctx, static_cast<size_t>(stencils) * kSlots, "The near-field stencils");
+                hipLaunchKernelGGL(stencil_table<double>, dim3(nb), dim3(256), 0, s, nc.tuples.ptr, stencils,
+                                   pc->coef64.ptr, pc->scoef64.ptr);
+                RT_LAUNCH_CHECK();
             }
-            const int stencils = static_cast<int>(tuples.size() / (kSlots + 2));
-            pc->stencil_count = stencils;
-            pc->scls.allocate(ctx, scls.size(), "The near-field stencil classes");
-            RT_HIP(hipMemcpyAsync(pc->scls.ptr, scls.data(), scls.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            if (stencils > 0) {
-                DeviceArray<int> d_tuples;
-                d_tuples.allocate(ctx, tuples.size(), "near-field stencil tuples");
-                RT_HIP(hipMemcpyAsync(d_tuples.ptr, tuples.data(), tuples.size() * sizeof(int), hipMemcpyHostToDevice,
-                                      s));
-                const unsigned nb = blocks_for(static_cast<size_t>(stencils) * kSlots, 256, 65535);
-                if (precision != RT_C64) {
-                    pc->scoef64.allocate(ctx, static_cast<size_t>(stencils) * kSlots, "The near-field stencils");
-                    hipLaunchKernelGGL(stencil_table<double>, dim3(nb), dim3(256), 0, s, d_tuples.ptr, stencils,
-                                       pc->coef64.ptr, pc->scoef64.ptr);
-                    RT_LAUNCH_CHECK();
-                }
-                if (precision != RT_C128) {
-                    pc->scoef32.allocate(ctx, static_cast<size_t>(stencils) * kSlots, "The near-field stencils");
-                    hipLaunchKernelGGL(stencil_table<float>, dim3(nb), dim3(256), 0, s, d_tuples.ptr, stencils,
-                                       pc->coef32.ptr, pc->scoef32.ptr);
-                    RT_LAUNCH_CHECK();
-                }
-                RT_HIP(hipStreamSynchronize(s));
+            if (precision != RT_C128) {
+                pc->scoef32.allocate(ctx, static_cast<size_t>(stencils) * kSlots, "The near-field stencils");
+                hipLaunchKernelGGL(stencil_table<float>, dim3(nb), dim3(256), 0, s, nc.tuples.ptr, stencils,
+                                   pc->coef32.ptr, pc->scoef32.ptr);
+                RT_LAUNCH_CHECK();
             }
+            RT_HIP(hipStreamSynchronize(s));
         }
     } else {
         fail(RT_EINVAL, "rooftop:preconditioner", "unknown preconditioner kind %d", kind);

@@ -61,7 +61,7 @@ uint64_t estimate_request_bytes(int cx, int cy, const SolveOptions &o, int preco
     b += 3 * Hx * My * (16 + (mixed ? 8 : 0));                    // pass-B spectra
     b += 2 * ldj * Mx * (16 + (mixed ? 8 : 0));                   // working arrays
     b += (Mx + My) * 4 * 64;                                      // FFT tables
-    if (preconditioner == RT_PRECOND_NEARFIELD) b += 4 * n + 4096 * 18 * 24;
+    if (preconditioner == RT_PRECOND_NEARFIELD) b += 8 * n + 4096 * 18 * 24;   // classes, stencil classes, tables
     if (preconditioner == RT_PRECOND_JACOBI) b += 24 * n;
     // GMRES: a basis of m + 1 columns (m = min(restart or the iteration limit, n, the iteration limit)), the pending
     // and work vectors, the Hessenberg matrices and the reduction partials; COCG: its vectors.
@@ -72,7 +72,9 @@ uint64_t estimate_request_bytes(int cx, int cy, const SolveOptions &o, int preco
     b += (vectors + 2) * n * 16;                                  // Krylov vectors, right-hand side, solution
     if (o.method == RT_METHOD_GMRES) b += (2 * m * (m + 1) + 6 * (m + 1)) * 16 + (m + 1) * 2048 * 16;
     const uint64_t spectra_transform = Mx * My * 16;              // freed before the solve starts
-    return b + spectra_transform + (b >> 3) + (16u << 20);
+    // The near-field classification's keys, sorts and scans (and the face mask), freed before the solve starts.
+    const uint64_t classify = preconditioner == RT_PRECOND_NEARFIELD ? 48 * n + cells : 0;
+    return b + std::max(spectra_transform, classify) + (b >> 3) + (16u << 20);
 }
 
 struct Server {
