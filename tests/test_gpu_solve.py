@@ -55,6 +55,23 @@ def test_near_field_masked(rt):
     assert rel(rt.Preconditioner(p, 'nearField').apply(x), P.apply(x)) < 1e-11
 
 
+@pytest.mark.parametrize('cx,cy,seed,fill', [(40, 36, 5, 0.3), (33, 47, 6, 0.6), (64, 20, 7, 0.9), (1, 30, 8, 0.8)])
+def test_near_field_on_blob_masks(rt, cx, cy, seed, fill):
+    """Irregular masks (random 4-connected blobs, as the served requests carry): many stencil classes at the blob's
+    boundary, formed on the device, against the oracle's per-rooftop fits."""
+    print('near field blob', cx, cy, seed, fill, flush=True)
+    mask = rt.blob_mask(cx, cy, seed, fill)
+    p, grid, gen = _setup(rt, cx, cy, 0.1, 0.1, (4, 4), mask=mask)
+    mx = mask[:-1] & mask[1:]
+    my = mask[:, :-1] & mask[:, 1:]
+    P = NF.NearField(gen, grid, mx, my)
+    x = np.random.default_rng(seed).standard_normal(grid.count) + 1j * np.random.default_rng(seed + 1).standard_normal(
+        grid.count)
+    pc = rt.Preconditioner(p, 'nearField', precision='mixed')
+    assert rel(pc.apply(x), P.apply(x)) < 1e-11
+    assert rel(pc.apply(x, precision='c64'), P.apply(x)) < 1e-5
+
+
 @pytest.mark.parametrize('method', ['cg', 'gmres'])
 def test_plate_solve_matches_oracle(rt, method):
     """SolverTest.m:9-21 (8x8 plate, plane wave, 1e-10) plus the iteration record against the oracle's loop."""
