@@ -109,6 +109,49 @@ __device__ __forceinline__ void static_for_down(F &&f) {
     }
 }
 
+// The twiddles w^q, q = 1 .. R - 1, of one butterfly from its table row (row[q] = w^q): complex128 reads w^1 and w^4
+// and forms the others by products (each the product of at most four table entries, a few units in the last place);
+// complex64 reads w^1, w^2, w^4 and w^8 and forms the others by one or two products. The twiddle reads were the larger
+// share of the passes' vector-memory requests (one per element and round, against one data read per element and
+// pass); a product costs four VALU instructions.
+template <typename T, int R>
+__device__ __forceinline__ void twiddle_row(const cplx<T> *__restrict__ row, cplx<T> *t) {
+    if constexpr (sizeof(T) == sizeof(double) && R >= 4) {
+        t[1] = row[1];
+        t[2] = t[1] * t[1];
+        t[3] = t[2] * t[1];
+        if constexpr (R >= 8) {
+            t[4] = row[4];
+            t[5] = t[4] * t[1];
+            t[6] = t[4] * t[2];
+            t[7] = t[4] * t[3];
+        }
+        if constexpr (R >= 16) {
+            t[8] = t[4] * t[4];
+#pragma unroll
+            for (int q = 9; q < 16; ++q) t[q] = t[8] * t[q - 8];
+        }
+    } else if constexpr (R >= 4) {   // complex64: w^1, w^2, w^4 and w^8 read, the others one product each
+        t[1] = row[1];
+        t[2] = row[2];
+        t[3] = t[2] * t[1];
+        if constexpr (R >= 8) {
+            t[4] = row[4];
+            t[5] = t[4] * t[1];
+            t[6] = t[4] * t[2];
+            t[7] = t[4] * t[3];
+        }
+        if constexpr (R >= 16) {
+            t[8] = row[8];
+#pragma unroll
+            for (int q = 9; q < 16; ++q) t[q] = t[8] * t[q - 8];
+        }
+    } else {
+#pragma unroll
+        for (int q = 1; q < R; ++q) t[q] = row[q];
+    }
+}
+
 // One round S of a team with a compile-time plan. Thread t holds butterflies beta = t + u T (u < E / r); butterfly
 // beta covers positions base + q Str (Str = L / r). load(pos, ppos) / store(pos, ppos, v) receive the position and
 // its padded LDS offset, both a per-butterfly value plus an immediate.
@@ -134,13 +177,15 @@ __device__ __forceinline__ void cround(int t, const cplx<T> *__restrict__ tw, Lo
         if constexpr (!INV) {
             dft<T, R, false, HALF>(v);
             if constexpr (TWID) {
-                const cplx<T> *w = tws + n * R;
+                cplx<T> w[R];
+                twiddle_row<T, R>(tws + n * R, w);
 #pragma unroll
                 for (int q = 1; q < R; ++q) v[q] = v[q] * w[q];
             }
         } else {
             if constexpr (TWID) {
-                const cplx<T> *w = tws + n * R;
+                cplx<T> w[R];
+                twiddle_row<T, R>(tws + n * R, w);
 #pragma unroll
                 for (int q = 1; q < R; ++q) v[q] = cmulc(w[q], v[q]);
             }
