@@ -220,7 +220,8 @@ int rt_product_profile(rt_product *product, int64_t nrhs, int32_t reps, int32_t 
  * of the three passes for nrhs right-hand sides in the given precision -- plan radices up to 8 or 16, pass A/C
  * rooftop columns per block, pass B x positions per block and teams per position, right-hand-side chunks -- and
  * keeps the fastest for every later application with that many right-hand sides. shape (may be NULL, else 5
- * entries) receives {largest radix bits, columns per block, positions per block, pass B form (1: one team
+ * entries) receives {largest radix bits, pass A/C columns per block (side by side in LDS; -2 or -4: that many in turn
+ * through one column of LDS), pass B positions per block, pass B form (1: one team
  * transforms both columns of a position in LDS, 2: a team per column, 3: one team, the two columns in turn through
  * one column of LDS), chunks (the right-hand sides split into chunks
  * that alternate between two streams, so that one chunk's passes overlap another's)}; ms (may be NULL) the wall

@@ -273,7 +273,7 @@ class Operator(Product):
 
     def setShape(self, shape, nrhs=1, precision=None):
         """Uses shape {'bits', 'group', 'tile', 'teams', 'chunks'} for nrhs right-hand sides (a saved tune() result;
-        teams and chunks default to 1)."""
+        teams and chunks default to 1; a group of -2 or -4 runs that many pass A/C columns in turn)."""
         prec = _PRECISIONS[precision] if precision else (_abi.RT_C128 if self.precision == _abi.RT_MIXED
                                                          else self.precision)
         arr = (C.c_int32 * 5)(shape['bits'], shape['group'], shape['tile'], shape.get('teams', 1),

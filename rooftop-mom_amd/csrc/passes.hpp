@@ -230,6 +230,7 @@ __device__ inline void group_of(const ProdDims &d, int group, int blk, int &r, i
 }
 
 inline int64_t blocks_ac(const ProdDims &d, int nrhs, int g) {
+    if (g < 0) g = -g;   // columns in turn: -g columns per block
     return static_cast<int64_t>(nrhs) * ((d.cols_x + g - 1) / g + (d.cols_y + g - 1) / g);
 }
 
@@ -257,8 +258,16 @@ inline int column_stride(const FftPlan &p, int group, int es) {
     const int skew = bank_elems / group > 0 ? bank_elems / group : 1;
     return up + skew;
 }
+// group < 0: -group columns in turn through one column of LDS (pass_a1_k, pass_c1_k).
 inline size_t lds_ac_bytes(const FftPlan &p, int group, int es) {
+    if (group < 0) return static_cast<size_t>(padded_length(p)) * es;
     return static_cast<size_t>(column_stride(p, group, es)) * group * es;
+}
+// Whether pass A/C can run `group` columns per block: side by side (group > 0, a team per column) or in turn (-2, -4:
+// one team, at least two rounds).
+inline bool ac_threads_fit(const FftPlan &p, int group, int es) {
+    if (group < 0) return (group == -2 || group == -4) && p.K >= 2 && team_threads(p) <= max_threads_ac(es, p);
+    return group >= 1 && team_threads(p) * group <= max_threads_ac(es, p);
 }
 // Padded length of a pass-B column (the BPlan layout).
 inline int b_padded_length(const FftPlan &p, int es) { return b_pad(b_pad_rule(p.log2M, p.bits, es), p.M - 1) + 1; }

@@ -159,14 +159,16 @@ template <typename TC>
 static bool shape_fits(const Product &op, const PassShape &sh) {
     const FftPlan &px = op.tx.of(sh.bits).plan;
     const int es = sizeof(cplx<TC>);
-    const bool ac = team_threads(px) * sh.group <= max_threads_ac(es, px) && lds_ac_bytes(px, sh.group, es) <= lds_bytes_max();
+    const bool ac = ac_threads_fit(px, sh.group, es) && lds_ac_bytes(px, sh.group, es) <= lds_bytes_max() &&
+                    (sh.group > 0 || op.embedding);
     const bool b = b_fits<TC>(op, sh) || (sh.tile == 1 && sh.teams == 1 && b_split_fits<TC>(op, sh.bits));
-    return sh.group >= 1 && sh.tile >= 1 && sh.tile <= op.dims.Mx && ac && b;
+    return sh.tile >= 1 && sh.tile <= op.dims.Mx && ac && b;
 }
 
-// Default launch shape (the autotuner may replace it), from the tuner's choices measured over plates of 32 to 2048
+// Default launch shape (the autotuner may replace it), from the tuner's choices measured over plates of 32 to 4096
 // cells at one and eight right-hand sides: radices up to 8 (complex64 transforms up to 2048 points, complex128 up to
-// 512), else up to 16; pass A/C groups of four columns where they fit a block; pass B through one column of LDS for
+// 512), else up to 16; pass A/C groups of four columns where they fit a block, else four (complex64) or two
+// (complex128) columns in turn through one column of LDS; pass B through one column of LDS for
 // transforms of at least 128 points in complex64 and 1024 in complex128; pass B tiles of up to 256 threads while the
 // blocks stay a few dozen. The plan and the pass B form depend on the transform lengths alone, so a product runs the
 // same kernels (and the same arithmetic per right-hand side) whatever the batch size.
@@ -186,6 +188,9 @@ static PassShape default_shape(const Product &op, int nrhs, int bits = 0) {
     int g = 4;
     while (g > 1 && (team_threads(px) * g > max_threads_ac(es, px) || lds_ac_bytes(px, g, es) > lds_bytes_max()))
         g >>= 1;
+    // Long rows whose side-by-side groups stay below four columns: four (complex64) or two (complex128) in turn.
+    const int turn = c64 ? -4 : -2;
+    if (g < 4 && plain && ac_threads_fit(px, turn, es) && lds_ac_bytes(px, turn, es) <= lds_bytes_max()) g = turn;
     s.group = g;
     int tile = 1;
     while (tile * 2 <= d.Mx && team_threads(py) * tile * 2 <= 256 &&
@@ -988,7 +993,7 @@ static PassShape tune_as(Product &op, int nrhs, int reps, double *best_ms, hipSt
         PassShape sh = default_shape<TC>(op, nrhs, bits);
         double ac = 1e300, b = 1e300;
         int g_best = 0, t_best = 0, k_best = 1;
-        for (int g = 1; g <= 16; g *= 2) {
+        for (int g : {1, 2, 4, 8, 16, -2, -4}) {   // side by side, then in turn
             PassShape c = sh;
             c.group = g;
             if (!shape_fits<TC>(op, c)) continue;
@@ -1070,10 +1075,10 @@ void operator_set_shape(Product &op, int nrhs, int prec, const int *shape) {
     if (!op.has(prec)) fail(RT_EINVAL, "rooftop:product", "the product does not hold the requested precision");
     const PassShape sh{shape[0], shape[1], shape[2], shape[3], shape[4]};
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
-    if ((sh.bits != 3 && sh.bits != 4) || !pow2(sh.group) || !pow2(sh.tile) || sh.group > 16 || sh.tile > 16 ||
-        sh.teams < 1 || sh.teams > 3 || sh.chunks < 1 || sh.chunks > 64)
+    if ((sh.bits != 3 && sh.bits != 4) || !(pow2(sh.group) || sh.group == -2 || sh.group == -4) || !pow2(sh.tile) ||
+        sh.group > 16 || sh.tile > 16 || sh.teams < 1 || sh.teams > 3 || sh.chunks < 1 || sh.chunks > 64)
         fail(RT_EINVAL, "rooftop:product",
-             "a shape is {3 or 4, a power of two <= 16, a power of two <= 16, 1 .. 3, 1 .. 64}");
+             "a shape is {3 or 4, a power of two <= 16 or -2 or -4, a power of two <= 16, 1 .. 3, 1 .. 64}");
     const bool fits = prec == RT_C128 ? shape_fits<double>(op, sh) : shape_fits<float>(op, sh);
     if (!fits) fail(RT_EINVAL, "rooftop:product", "the shape does not fit the device for this product");
     auto &tuned = prec == RT_C128 ? op.tuned64 : op.tuned32;

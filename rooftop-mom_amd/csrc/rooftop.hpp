@@ -75,7 +75,7 @@ struct DimTables {
 // Launch shape of the three matvec passes (picked by the autotuner or the default heuristic).
 struct PassShape {
     int bits = 4;    // largest radix of the plans: 2^bits elements per thread
-    int group = 1;   // pass A / C: rooftop columns per block
+    int group = 1;   // pass A / C: rooftop columns per block, side by side in LDS; -2, -4: in turn through one column
     int tile = 1;    // pass B: x positions per block
     int teams = 1;   // pass B form: 1, one team runs both columns of an x position; 2, a team per column; 3, one
                      // team whose columns take turns in one column of LDS

@@ -166,7 +166,7 @@ def test_product_shapes_across_transform_lengths(rt, size):
     op = rt.Operator(p, precision='mixed')
     for prec, tol in (('c128', 1e-12), ('c64', 5e-6)):
         for bits in (3, 4):
-            for group in (1, 2, 4, 8, 16):
+            for group in (1, 2, 4, 8, 16, -2, -4):
                 for tile, teams, chunks in ((1, 1, 1), (4, 1, 1), (16, 1, 1), (1, 2, 1), (2, 2, 1), (1, 1, 2),
                                             (1, 2, 3), (1, 3, 1), (4, 3, 1), (1, 3, 2)):
                     try:
@@ -212,3 +212,32 @@ def test_long_columns_three_launch_pass_b(rt):
         for bits in (3, 4):
             op.setShape({'bits': bits, 'group': 1, 'tile': 1, 'teams': 3, 'chunks': 1}, nrhs=2, precision=prec)
             assert rel(op.apply(x, precision=prec), ref) < tol, (prec, bits)
+
+
+def test_long_rows_columns_in_turn(rt):
+    """8192-point x transforms (cx = 4096): passes A and C with two or four columns per block in turn through one
+    column of LDS, against the side-by-side form and the oracle, in both plans and precisions, masked and not."""
+    cx, cy = 4096, 3
+    print('long rows', flush=True)
+    for masked in (False, True):
+        mask = None
+        if masked:
+            mask = np.ones((cx, cy), bool)
+            mask[1000:1400, 1] = False
+        p = _problem(rt, cx, cy, 0.1, 0.1, (3, 3), mask=mask)
+        x = _random(p.rooftopCount(), 14, nrhs=3)
+        ref = _oracle_product(p, cx, cy, 0.1, 0.1, x) if not masked else None
+        op = rt.Operator(p, precision='mixed')
+        for prec, tol in (('c128', 1e-12), ('c64', 5e-6)):
+            base = op.apply(x, precision=prec)
+            if ref is not None:
+                assert rel(base, ref) < tol, prec
+            for bits in (3, 4):
+                for group in (2, -2, -4):
+                    try:
+                        op.setShape({'bits': bits, 'group': group, 'tile': 1, 'teams': 1, 'chunks': 1}, nrhs=3,
+                                    precision=prec)
+                    except rt.RooftopError:
+                        assert group == 2   # two teams of 512 or 1024 threads may not fit a block side by side
+                        continue
+                    assert rel(op.apply(x, precision=prec), base) < tol, (masked, prec, bits, group)
