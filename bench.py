@@ -40,9 +40,10 @@ PEAK_HBM_GBS = 8000.0
 # Timed kernel slots of the library (RT_TIMED_* of include/rooftop.h) and the kernel symbols they time.
 TIMED = ['pass_a_k<double> (x FFT)', 'pass_b_k<double> (y FFT + multiply)', 'pass_c_k<double> (inverse x FFT)',
          'pass_a_k<float> (x FFT)', 'pass_b_k<float> (y FFT + multiply)', 'pass_c_k<float> (inverse x FFT)',
-         'k_dcgs_dots ([s z] = V^H [p w])', 'k_dcgs_update (q = (p - V s)/beta; p = (w - V z)/(beta nu))',
+         'k_dcgs_dots ([s z] = V^H [p w])',
+         'k_dcgs_update (q = (p - V s)/beta; p = (w - V z)/(beta nu); Hessenberg columns, rotations, stopping test)',
          'sai_apply<double> (near field)', 'sai_apply<float> (near field)',
-         'k_dgmres_finalize (block sums, Hessenberg columns, rotations, stopping test)']
+         'k_dgmres_sums (block sums; beta, z_j, nu)']
 
 
 class KernelClock:

@@ -245,10 +245,11 @@ enum {
     RT_TIMED_PASS_B_C64 = 4,
     RT_TIMED_PASS_C_C64 = 5,
     RT_TIMED_GS_DOTS = 6,         /* GMRES Gram-Schmidt dot pass: [s z] = V^H [p w] and the norms of p and w */
-    RT_TIMED_GS_UPDATE = 7,       /* update pass: q = (p - V s) / beta, p = (w - V z - z_j q) / (beta nu) */
+    RT_TIMED_GS_UPDATE = 7,       /* update pass: q = (p - V s) / beta, p = (w - V z - z_j q) / (beta nu), and beside
+                                     it the step's Hessenberg columns, rotations and stopping test */
     RT_TIMED_PRECOND_C128 = 8,    /* preconditioner application */
     RT_TIMED_PRECOND_C64 = 9,
-    RT_TIMED_GS_FINALIZE = 10,    /* the dot pass's block sums, the Hessenberg columns, rotations and stopping test */
+    RT_TIMED_GS_SUMS = 10,        /* the dot pass's block sums and the step's coefficients (beta, z_j, nu) */
     RT_TIMED_COUNT = 11
 };
 int rt_ctx_set_timing(rt_ctx *ctx, int32_t on);

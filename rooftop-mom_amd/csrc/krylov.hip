@@ -52,7 +52,9 @@ enum ScalSlot {
     kH2 = 0, kH1 = 64,                              // GMRES: second- and first-pass coefficients
     kRnorm = 128, kInvSub = 130, kReference = 132, kTol = 134, kBnorm = 136,
     kMaxIt = 138, kBest = 140,                      // per-system iteration limit; COCG best monitored norm
-    kAlpha = 142, kT = 143, kOmega = 144, kInvBeta = 145,   // GMRES: p^H p, p^H w, w^H w, 1 / beta
+    kAlpha = 142, kT = 143, kOmega = 144,           // GMRES: p^H p, p^H w, w^H w
+    kB2 = 145, kBeta = 146, kNu = 147, kZj = 148,   // GMRES: the step's beta^2, beta, nu, z_j
+    kInvBeta = 149, kInvBetaNu = 150,               // GMRES: the update pass's coefficients
     kY = 256,                                       // GMRES: solution coefficients of a cycle
     kOuterB = 400, kOuterRes = 401
 };
@@ -219,7 +221,7 @@ __global__ void k_reset_state(LoopState *st, const int *__restrict__ finished, i
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int f = finished[r];
-    st[r] = LoopState{f, 0, f, 0, 0, 0, -1, 0, 0.f};
+    st[r] = LoopState{f, 0, f, 0, 0, 0, -1, 0, -1, 0.f};
 }
 // Marks the systems whose starting residual is already below their tolerance as converged at iteration 0.
 __global__ void k_initial_check(LoopState *st, const c128 *__restrict__ scal, int num_slot, int den_slot, int R) {
@@ -845,125 +847,141 @@ __device__ __forceinline__ c128 givens(c128 a, double sub, c128 &c, c128 &sv) {
     return cmulc(c, a) + cmulc(sv, mk(sub, 0.0));
 }
 
-// Step j (0-based) of the cycle, iteration `it` overall, in two phases. First every block sums slots of the dot pass
-// (a wave per slot: s, z, alpha, t, omega into scal); the last block of a system to finish them (counted on its
-// LoopState, which it returns to zero) then finalizes the step: beta, z_j and nu; column j - 1 of H in its final
-// form, its rotation formed again and the monitored residual of iteration it - 1 refined; column j with the
-// subdiagonal nu (final one step later), its rotation, the rotated right-hand side g, the monitored residual
-// |g(j+1)| / reference and the stopping test (iterate_.m:51-66); 1 / beta and 1 / (beta nu) for the update pass.
-// H holds the columns as formed (Hraw, for H s) and rotated (Hall, for the triangular solve); gall holds g, the
-// cosines, the sines and g before each rotation. Everything the step reads is loaded at once; the norms are one
-// wave's sums; the earlier rotations reach both columns by one wave's scan. Grid (ceil((2 j + 3) / 4), R).
-__global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__restrict__ partial, int blocks,
-                                                              c128 *__restrict__ scal, c128 *__restrict__ Hraw,
-                                                              c128 *__restrict__ Hall, c128 *__restrict__ gall, int m,
-                                                              int j, int it, LoopState *st, double *history, int cap) {
+// beta, z_j and nu of step j by one wave, lane k holding s_k and z_k (zero past j), from the dot pass's p^H p, p^H w
+// and w^H w: beta = sqrt(b2), b2 = alpha - |s|^2, z_j = (t - s^H z) / beta, nu = sqrt(omega - |z|^2 - |z_j|^2) / beta.
+// beta = 0 where b2 is not positive (breakdown, or p not finite); nu = 0 where w lies in the span.
+__device__ __forceinline__ void step_coefficients(c128 sk, c128 zk, double alpha, double omega, c128 t, double &b2,
+                                                  double &beta, c128 &zj, double &nu) {
+    const c128 v = cmulc(sk, zk);
+    const double ss = wave_sum(abs2(sk)), szr = wave_sum(v.x), szi = wave_sum(v.y), zz = wave_sum(abs2(zk));
+    b2 = alpha - ss;
+    beta = b2 > 0 && isfinite(b2) ? sqrt(b2) : 0.0;
+    zj = beta > 0 ? (1.0 / beta) * (t - mk(szr, szi)) : mk(0.0, 0.0);
+    const double n2 = omega - (zz + abs2(zj));
+    nu = beta > 0 && n2 > 0 ? sqrt(n2) / beta : 0.0;
+}
+
+// Step j (0-based) of the cycle, iteration `it` overall, in two launches. First the block sums of the dot pass
+// (k_dgmres_sums: a wave per slot, s, z, alpha, t, omega into scal), whose block that finishes last forms beta, z_j and
+// nu and the update pass's coefficients 1 / beta and 1 / (beta nu). Then the update pass (k_dcgs_update), whose first
+// block per system finalizes the step meanwhile (finalize_step): column j - 1 of H in its final form, its rotation
+// formed again and the monitored residual of iteration it - 1 refined; column j with the subdiagonal nu (final one step
+// later), its rotation, the rotated right-hand side g, the monitored residual |g(j+1)| / reference and the stopping
+// test (iterate_.m:51-66). The serial finalize thus runs beside the update instead of before it. H holds the columns
+// as formed (Hraw, for H s) and rotated (Hall, for the triangular solve); gall holds g, the cosines, the sines and g
+// before each rotation. Grid (ceil((2 j + 3) / 4), R); LoopState.live = it marks the systems whose step runs.
+__global__ void __launch_bounds__(kThreads) k_dgmres_sums(const c128 *__restrict__ partial, int blocks,
+                                                          c128 *__restrict__ scal, int j, int it, LoopState *st) {
     SYSTEM(r);
-    {
-        __shared__ int last;
-        const int x = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-        if (x < 2 * j + 3) dcgs_slot_sum(partial, r, blocks, j, x, threadIdx.x & 63, scal);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // the sums are visible to every XCD ...
-        __syncthreads();
-        if (threadIdx.x == 0) last = atomicAdd(&st[r].arrivals, 1) == static_cast<int>(gridDim.x) - 1;
-        __syncthreads();
-        if (!last) return;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // ... before the last block reads them
-        if (threadIdx.x == 0) st[r].arrivals = 0;
-    }
+    __shared__ int last;
+    const int x = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (x < 2 * j + 3) dcgs_slot_sum(partial, r, blocks, j, x, threadIdx.x & 63, scal);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // the sums are visible to every XCD ...
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&st[r].arrivals, 1) == static_cast<int>(gridDim.x) - 1;
+    __syncthreads();
+    if (!last || threadIdx.x >= 64) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // ... before the last block reads them
+    c128 *sc = scal + r * kScal;
+    const int lane = threadIdx.x, kk = min(lane, max(j - 1, 0));
+    const c128 sk = sc[kH2 + kk], zk = sc[kH1 + kk];
+    double b2, beta, nu;
+    c128 zj;
+    step_coefficients(pick(lane < j, sk, mk(0.0, 0.0)), pick(lane < j, zk, mk(0.0, 0.0)), sc[kAlpha].x,
+                      sc[kOmega].x, sc[kT], b2, beta, zj, nu);
+    if (lane != 0) return;
+    sc[kB2] = mk(b2, 0.0);
+    sc[kBeta] = mk(beta, 0.0);
+    sc[kNu] = mk(nu, 0.0);
+    sc[kZj] = zj;
+    sc[kInvBeta] = mk(beta > 0 ? 1.0 / beta : 0.0, 0.0);   // 0: no update (no new column)
+    sc[kInvBetaNu] = mk(beta > 0 && nu > 0 ? 1.0 / (beta * nu) : 0.0, 0.0);
+    st[r].arrivals = 0;
+    st[r].live = it;
+}
+
+// The finalize of step j for system r by one block (the update pass's last block of the system).
+__device__ __forceinline__ void finalize_step(int r, c128 *__restrict__ scal, c128 *__restrict__ Hraw, c128 *__restrict__ Hall,
+                              c128 *__restrict__ gall, int m, int j, int it, LoopState *st, double *history, int cap) {
     __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp], colf[kLdp + 1], coln[kLdp + 1];
     __shared__ c128 csv[kLdp], snv[kLdp];
-    __shared__ double shb[4];   // nu_{j-1}, beta, nu, breakdown (p in the span of the basis)
     c128 *sc = scal + r * kScal;
     const int ldh = m + 1;
     c128 *Hr = Hraw + static_cast<int64_t>(r) * ldh * m, *Hc = Hall + static_cast<int64_t>(r) * ldh * m;
     c128 *g = gall + static_cast<int64_t>(r) * 4 * ldh, *cs = g + ldh, *sn = g + 2 * ldh, *gs = g + 3 * ldh;
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     // Everything the step reads is loaded before any of it is stored, from addresses clamped into range (a load under
-    // a branch would be waited for inside it), so that the step costs one memory latency: entry k = thread of s, z,
-    // the rotations and column j - 1 as formed (j <= 62 entries), H(k, l) of the columns l < j - 1, l = q mod 4, for
-    // (H s)_k with k = lane, and the scalars.
-    const int kk = min(static_cast<int>(threadIdx.x), max(j - 1, 0)), jm = max(j - 1, 0);
-    const c128 sv0 = sc[kH2 + kk], zv0 = sc[kH1 + kk], cv0 = cs[kk], nv0 = sn[kk];
-    const c128 pv0 = Hr[static_cast<int64_t>(jm) * ldh + kk];
+    // a branch would be waited for inside it): entry k = thread of s, z, the rotations and column j - 1 as formed
+    // (j <= 62 entries), the scalars, and H(k, l) of the columns l < j - 1, l = q mod 4, for (H s)_k with k = lane, in
+    // two halves of eight columns per lane (the second one, where j needs it, once the first is summed), so that the
+    // block stays within the update pass's registers; beta, nu and z_j come from the dot pass's block sums.
+    // s, z, the cosines, the sines and column j - 1 as formed: vector q by wave q and vector 4 by wave 0 too.
+    const int kk = min(lane, max(j - 1, 0)), jm = max(j - 1, 0);
+    auto vec_src = [&](int v) -> const c128 * {
+        return v == 0 ? sc + kH2 : v == 1 ? sc + kH1 : v == 2 ? cs : v == 3 ? sn : Hr + static_cast<int64_t>(jm) * ldh;
+    };
+    auto vec_dst = [&](int v) { return v == 0 ? s : v == 1 ? z : v == 2 ? csv : v == 3 ? snv : colp; };
+    const int v0 = __builtin_amdgcn_readfirstlane(q), v1 = q == 0 ? 4 : v0;   // v1 = v0: the same load again
+    const c128 a0 = vec_src(v0)[kk], a1 = vec_src(v1)[kk];
     const int k0 = max(0, lane - 1), l0 = k0 + ((q - k0) & 3);
-    c128 hk[kLdp / 4];
+    constexpr int kHalf = kLdp / 8;   // columns per half: l = l0 + 4 u, u in [u0, u0 + kHalf)
+    c128 hk[kHalf];
+    auto load_h = [&](int u0) {
 #pragma unroll
-    for (int u = 0; u < kLdp / 4; ++u) {
-        const int l = l0 + 4 * u;
-        const c128 v = Hr[static_cast<int64_t>(min(l, max(j - 2, 0))) * ldh + min(lane, j)];
-        hk[u] = pick(lane <= j && l < j - 1, v, mk(0.0, 0.0));
-    }
+        for (int u = 0; u < kHalf; ++u) {
+            const int l = l0 + 4 * (u0 + u);
+            const c128 v = Hr[static_cast<int64_t>(min(l, max(j - 2, 0))) * ldh + min(lane, j)];
+            hk[u] = pick(lane <= j && l < j - 1, v, mk(0.0, 0.0));
+        }
+    };
+    load_h(0);
     const double hsub = Hr[static_cast<int64_t>(jm) * ldh + j].x;
-    const double alpha = sc[kAlpha].x, omega = sc[kOmega].x, ref = sc[kReference].x, tol = sc[kTol].x;
-    const double maxit = sc[kMaxIt].x;
-    const c128 t = sc[kT], gprev = (j >= 1 ? gs + j - 1 : g)[0];
-    if (static_cast<int>(threadIdx.x) < j) {
-        s[threadIdx.x] = sv0;
-        z[threadIdx.x] = zv0;
-        csv[threadIdx.x] = cv0;
-        snv[threadIdx.x] = nv0;
-        colp[threadIdx.x] = pv0;
+    const double ref = sc[kReference].x, tol = sc[kTol].x, maxit = sc[kMaxIt].x;
+    const double b2 = sc[kB2].x, beta = sc[kBeta].x, nu = sc[kNu].x;
+    const c128 zj = sc[kZj], gprev = (j >= 1 ? gs + j - 1 : g)[0];
+    if (lane < j) {
+        vec_dst(v0)[lane] = a0;
+        vec_dst(v1)[lane] = a1;
     }
-    if (threadIdx.x == 0) shb[0] = j >= 1 ? hsub : 0.0;
+    if (threadIdx.x == 0) z[j] = zj;
     __syncthreads();
-    if (q == 0) {   // |s|^2, s^H z, |z|^2
-        double ss = 0, szr = 0, szi = 0, zz = 0;
-        for (int k = lane; k < j; k += 64) {
-            ss += abs2(s[k]);
-            const c128 v = cmulc(s[k], z[k]);
-            szr += v.x;
-            szi += v.y;
-            zz += abs2(z[k]);
+    // (H s)_k over the columns l < j - 1 as formed, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4 in
+    // increasing order, lane k; column j - 1, final only below, is added last.
+    c128 acc = mk(0.0, 0.0);
+    auto add_h = [&](int u0) {
+#pragma unroll
+        for (int u = 0; u < kHalf; ++u) {
+            const int l = l0 + 4 * (u0 + u);
+            if (l < j - 1) acc = acc + hk[u] * s[l];
         }
-        ss = wave_sum(ss);
-        szr = wave_sum(szr);
-        szi = wave_sum(szi);
-        zz = wave_sum(zz);
-        if (lane == 0) {
-            const double b2 = alpha - ss;
-            const double beta = b2 > 0 && isfinite(b2) ? sqrt(b2) : 0.0;
-            shb[1] = beta;
-            shb[3] = j >= 1 && isfinite(b2) && !(b2 > 0) ? 1.0 : 0.0;
-            const c128 zj = beta > 0 ? (1.0 / beta) * (t - mk(szr, szi)) : mk(0.0, 0.0);
-            z[j] = zj;
-            const double n2 = omega - (zz + abs2(zj));
-            shb[2] = beta > 0 && n2 > 0 ? sqrt(n2) / beta : 0.0;   // 0: w lies in the span (breakdown)
-        }
-    }
-    __syncthreads();
-    const double nup = shb[0], beta = shb[1];
+    };
+    if (lane <= j) add_h(0);
+    const double nup = j >= 1 ? hsub : 0.0;
     // A finite p that vanishes after its reorthogonalization lies in the span of the basis: the subdiagonal of column
     // j - 1 is zero, so the residual of iteration it - 1 is zero and the system has converged there (a zero
     // subdiagonal in iterate_.m:62-66). Column j - 1 is finalized and rotated below; column j is never formed.
-    const bool brk = shb[3] != 0;
+    const bool brk = j >= 1 && isfinite(b2) && !(b2 > 0);
     if (!(beta > 0) && !brk) {   // p is not finite: the system stops without a new column
         if (threadIdx.x == 0) st[r].finished = st[r].done = 1;
         return;
     }
-    // Column j - 1 final: + nu_{j-1} s above the subdiagonal, subdiagonal nu_{j-1} beta.
-    if (j >= 1) {
-        for (int k = threadIdx.x; k <= j; k += kThreads) {
-            colf[k] = k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0);
-            Hr[static_cast<int64_t>(j - 1) * ldh + k] = colf[k];
-        }
+    // Column j - 1 final: + nu_{j-1} s above the subdiagonal, subdiagonal nu_{j-1} beta (stored to H below, after the
+    // last loads).
+    if (j >= 1)
+        for (int k = threadIdx.x; k <= j; k += kThreads) colf[k] = k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0);
+    const bool hs_on = lane <= j && !brk;
+    if (j > 33) {   // uniform over the block: the second half's columns are l >= 32
+        load_h(kHalf);
+        if (hs_on) add_h(kHalf);
     }
-    // (H s)_k over the final columns l < j, rows k <= j (H(k, l) = 0 below l + 1); wave q sums l = q mod 4, lane k.
-    if (lane <= j && !brk) {
+    if (hs_on && l0 <= j - 1 && ((j - 1 - l0) & 3) == 0) {
         const int k = lane;
-        c128 acc = mk(0.0, 0.0);
-#pragma unroll
-        for (int u = 0; u < kLdp / 4; ++u) {
-            const int l = l0 + 4 * u;
-            if (l < j) {
-                const c128 h = l == j - 1 ? (k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0)) : hk[u];
-                acc = acc + h * s[l];
-            }
-        }
-        part[q][k] = acc;
+        acc = acc + (k < j ? colp[k] + nup * s[k] : mk(nup * beta, 0.0)) * s[j - 1];
     }
+    if (hs_on) part[q][lane] = acc;
     __syncthreads();
     for (int k = threadIdx.x; k <= j; k += kThreads) {
+        if (j >= 1) Hr[static_cast<int64_t>(j - 1) * ldh + k] = colf[k];
         if (brk) {
             coln[k] = mk(0.0, 0.0);
             continue;
@@ -974,7 +992,6 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
     }
     __syncthreads();
     if (q != 0) return;
-    const double nu = shb[2];
     // The rotations 0 .. j-2 on both columns. With run_0 = a_0, rotation k gives the final entry
     // conj(c_k) run_k + s_k a_{k+1} and the running entry run_{k+1} = -s_k run_k + c_k a_{k+1} (s_k real): an affine
     // recurrence, whose prefix compositions (M, B) (run_{k+1} = M a_0 + B) one wave scans, lane k holding step k.
@@ -1058,23 +1075,33 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_finalize(const c128 *__rest
     } else if (it >= static_cast<int>(maxit)) {
         st[r].done = 1;
     }
-    sc[kInvBeta] = mk(1.0 / beta, 0.0);
-    sc[kInvSub] = (!conv && nu > 0) ? mk(1.0 / (beta * nu), 0.0) : mk(0.0, 0.0);
 }
 
 // The update pass: q_j = (p - V s) / beta into column j of the basis, p = (w - V z - z_j q_j) / (beta nu). Runs for
-// the systems whose finalize ran in iteration `it`, the ones that stopped in it included (q_j enters their solution).
+// the systems whose step j runs (LoopState.live = it), the ones that stop in it included (q_j enters their solution),
+// unless beta is not positive (no new column). The
+// first block of each system (blockIdx.x = 0) finalizes the step instead (finalize_step), beside the update; the rows
+// start at block kLeadBlocks, so that block b of the rows runs on XCD b mod 8 as the dot pass's tiles expect.
 // S lanes share a row (S = 1 for batches, whose many rows fill the chip; S = 4 for a few systems): lane group s sums
 // the columns k = s mod S, so that each lane's chain of column loads is S times shorter, and the S partial sums meet
 // by lane shuffles in a fixed order.
+constexpr int kLeadBlocks = 8;   // the update pass's blocks before its rows: the finalize and seven idle ones
+
 template <typename VT, int S>
-__global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, int cnt, c128 *__restrict__ p,
+__global__ void __launch_bounds__(kThreads)
+k_dcgs_update(VT *__restrict__ V, int cnt, c128 *__restrict__ p,
                                                           const c128 *__restrict__ w, int64_t n, int64_t chunk,
-                                                          const c128 *__restrict__ scal, int it,
-                                                          const LoopState *__restrict__ st) {
+                                                          c128 *__restrict__ scal, int it, LoopState *st,
+                                                          c128 *__restrict__ Hraw, c128 *__restrict__ Hall,
+                                                          c128 *__restrict__ gall, int m, double *history, int cap) {
     static_assert(S == 1 || S == 2 || S == 4, "one, two or four lanes per row");
     const int r = blockIdx.y;
-    if (st[r].tag != it) return;
+    if (st[r].live != it) return;
+    if (blockIdx.x < kLeadBlocks) {   // dispatched first, so that the finalize starts with the update
+        if (blockIdx.x == 0) finalize_step(r, scal, Hraw, Hall, gall, m, cnt, it, st, history, cap);
+        return;
+    }
+    const int64_t blk = blockIdx.x - kLeadBlocks;
     const int R = gridDim.y;
     __shared__ c128 hs[kLdp], hz[kLdp];
     const c128 *sc = scal + r * kScal;
@@ -1088,11 +1115,12 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, in
         }
         __syncthreads();
     };
-    const double ib = sc[kInvBeta].x, isub = sc[kInvSub].x;
-    const c128 zj = sc[kH1 + cnt];
+    const double ib = sc[kInvBeta].x, isub = sc[kInvBetaNu].x;
+    const c128 zj = sc[kZj];
+    if (!(ib > 0)) return;   // no new column: p is not updated
     if constexpr (S == 1) {
         stage();
-        const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+        const int64_t base = blk * chunk, end = min(n, base + chunk);
         for (int64_t i = base + threadIdx.x; i < end; i += kThreads) {
             c128 as = mk(0.0, 0.0), az = mk(0.0, 0.0);
             for (int k0 = 0; k0 < cnt; k0 += 8) {   // eight columns' loads in flight at once
@@ -1119,7 +1147,7 @@ __global__ void __launch_bounds__(kThreads) k_dcgs_update(VT *__restrict__ V, in
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int sub = lane / kRowsPerWave;                                  // this lane's column group
     const int row = wave * kRowsPerWave + (lane & (kRowsPerWave - 1));   // row within the block's step
-    const int64_t base = blockIdx.x * chunk, end = min(n, base + chunk);
+    const int64_t base = blk * chunk, end = min(n, base + chunk);
     const int last = max(cnt - 1, 0);   // column `cnt` is allocated: a valid address when cnt = 0
     for (int64_t i0 = base; i0 < end; i0 += kThreads / S) {
         const int64_t i = i0 + row;
@@ -2127,19 +2155,20 @@ static InnerResult gmres(Workspace &ws, const LinOp &A, const LinOp *P, const c1
                              ws.partial, st);
                 t1.done();
                 // Algorithmic bytes: the dot pass's partial sums once (the Hessenberg entries and scalars are few).
-                KernelTimer tf(ctx, RT_TIMED_GS_FINALIZE, 16.0 * (2 * j + 3) * ws.dot_blocks * R);
-                launch_timed(&tf, k_dgmres_finalize, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s,
-                             static_cast<const c128 *>(ws.partial), ws.dot_blocks, ws.scal, Hr, H, g, m, j, it,
-                             ws.state, ws.history, ws.cap);
+                KernelTimer tf(ctx, RT_TIMED_GS_SUMS, 16.0 * (2 * j + 3) * ws.dot_blocks * R);
+                launch_timed(&tf, k_dgmres_sums, dim3((2 * j + 3 + 3) / 4, R), dim3(kThreads), 0, ws.s,
+                             static_cast<const c128 *>(ws.partial), ws.dot_blocks, ws.scal, j, it, ws.state);
                 tf.done();
                 KernelTimer t2(ctx, RT_TIMED_GS_UPDATE, j * col + 3 * wb + col);
-                // A few systems leave the chip short of rows: four lanes per row then.
+                // A few systems leave the chip short of rows: four lanes per row then. The first block per system
+                // finalizes the step.
                 if (ws.gs_tile_update)
-                    launch_timed(&t2, k_dcgs_update<VT, 4>, dim3(ws.gs_blocks, R), dim3(kThreads), 0, ws.s, V, j, pv,
-                                 w, n, ws.gs_chunk, ws.scal, it, st);
+                    launch_timed(&t2, k_dcgs_update<VT, 4>, dim3(ws.gs_blocks + kLeadBlocks, R), dim3(kThreads), 0,
+                                 ws.s, V, j, pv, w, n, ws.gs_chunk, ws.scal, it, ws.state, Hr, H, g, m, ws.history,
+                                 ws.cap);
                 else
-                    launch_timed(&t2, k_dcgs_update<VT, 1>, dim3(ws.blocks, R), dim3(kThreads), 0, ws.s, V, j, pv, w,
-                                 n, ws.chunk, ws.scal, it, st);
+                    launch_timed(&t2, k_dcgs_update<VT, 1>, dim3(ws.blocks + kLeadBlocks, R), dim3(kThreads), 0, ws.s,
+                                 V, j, pv, w, n, ws.chunk, ws.scal, it, ws.state, Hr, H, g, m, ws.history, ws.cap);
                 t2.done();
             }
             ctx->timed_step = -1;
