@@ -108,6 +108,25 @@ def test_preconditioned_solve_matches_oracle(rt, method):
     assert res.iterationCount() < plain.iterationCount()
 
 
+def test_gmres_long_cycle_matches_oracle(rt):
+    """GMRES(60) with the near field to 1e-10 on the 16 x 14 plate converges in one cycle of 49 steps: the steps past
+    33, whose finalize reads the second half of the Hessenberg entries for H s, and every rotation of a long cycle,
+    against the oracle's CGS2 loop (iterate_.m:38-71): the same iteration count, the monitored norms and the solution."""
+    p, grid, gen = _setup(rt, 16, 14, 0.1, 0.1, (6, 4))
+    Z = S.dense_matrix(gen, grid)
+    P = NF.NearField(gen, grid)
+    b = X.plane_wave(grid, 2 * np.pi, [0.3, 0.4, -np.sqrt(0.75)], [0.8, -0.6, 0], 1.0, order=4)
+    res = rt.Solver('gmres', tolerance=1e-10, restart=60).solve(rt.Operator(p), b,
+                                                              preconditioner=rt.Preconditioner(p, 'nearField'))
+    ref = SV.solve('gmres', Z, b, tol=1e-10, max_iter=1000, restart=60, preconditioner=P.apply)
+    assert ref['iterationCount'] > 40   # one cycle well past step 33
+    assert res.converged() and res.iterationCount() == ref['iterationCount']
+    k = ref['residualNorms'].size
+    assert res.residualNorms().size == k
+    np.testing.assert_allclose(res.residualNorms(), ref['residualNorms'], rtol=1e-6)
+    assert rel(res.solution(), ref['solution']) < 1e-8
+
+
 @pytest.mark.parametrize('method', ['cg', 'gmres'])
 def test_mixed_precision_refinement_reaches_fp64_accuracy(rt, method):
     p, grid, gen = _setup(rt, 24, 20, 0.1, 0.1, (8, 4))
