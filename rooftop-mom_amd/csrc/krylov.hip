@@ -221,7 +221,7 @@ __global__ void k_reset_state(LoopState *st, const int *__restrict__ finished, i
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int f = finished[r];
-    st[r] = LoopState{f, 0, f, 0, 0, 0, -1, 0, -1, 0.f};
+    st[r] = LoopState{f, 0, f, 0, 0, 0, 0, -1, 0.f};
 }
 // Marks the systems whose starting residual is already below their tolerance as converged at iteration 0.
 __global__ void k_initial_check(LoopState *st, const c128 *__restrict__ scal, int num_slot, int den_slot, int R) {
@@ -1066,7 +1066,6 @@ __device__ __forceinline__ void finalize_step(int r, c128 *__restrict__ scal, c1
     st[r].monitored = static_cast<float>(res);
     st[r].steps = j + 1;
     st[r].iterations = it;
-    st[r].tag = it;
     const bool conv = res < tol;
     if (conv) {
         st[r].converged = st[r].finished = st[r].done = 1;
@@ -1396,7 +1395,6 @@ __global__ void k_cgs_finalize(const c128 *__restrict__ Hraw, c128 *__restrict__
     st[r].monitored = static_cast<float>(res);
     st[r].steps = j + 1;
     st[r].iterations = it;
-    st[r].tag = it;
     const bool conv = res < sc[kTol].x;
     if (conv) {
         st[r].converged = st[r].finished = st[r].done = 1;

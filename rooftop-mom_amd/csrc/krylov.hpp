@@ -40,7 +40,6 @@ struct LoopState {
     int iterations;   // iterations completed
     int steps;        // GMRES: steps of the current cycle completed
     int best;         // COCG: iteration of the best monitored norm
-    int tag;          // GMRES: the last iteration whose finalize ran for this system
     int arrivals;     // GMRES: blocks of the running step's block sums done so far (zero between launches)
     int live;         // GMRES: the iteration whose step runs for this system (set by its block sums)
     float monitored;  // GMRES: monitored residual of the last finalized iteration (the host's poll schedule)
