@@ -63,7 +63,7 @@ enum ScalSlot {
 enum PoolSlot {
     kSlotPartial, kSlotScal, kSlotState, kSlotHistory, kSlotV, kSlotW, kSlotT, kSlotR, kSlotZ, kSlotD, kSlotAd,
     kSlotH, kSlotG, kSlotOuterT, kSlotOuterR, kSlotOuterD, kSlotFlags, kSlotP, kSlotHraw,
-    kSlotPartialL, kSlotHc, kSlotY, kSlotBlockCoef, kSlotBlockPartial, kSlotBlockBest
+    kSlotPartialL, kSlotHc, kSlotY, kSlotBlockCoef, kSlotBlockPartial, kSlotBlockBest, kSlotOuterSolo
 };
 
 // Restarts up to this length run the delayed-reorthogonalization kernels, whose Gram-Schmidt coefficients, rotations
@@ -2304,6 +2304,11 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
         // does every inner block COCG (whose recurrence can lose its conjugacy in any precision) and every FP64 inner
         // COCG: each restart begins a fresh recurrence from the true residual.
         int stall = cg_like ? kStall : 0;
+        // Block COCG: a system whose residual a block cycle does not halve goes on alone (COCG, the block recurrence
+        // of one column) from its iterate in the later cycles. On an indefinite system the recurrence of many
+        // right-hand sides can lose its conjugacy, and a system caught in that would not converge in the block.
+        std::vector<char> solo(R, 0);
+        c128 *d_solo = nullptr;   // the lone systems' corrections (every inner solve zeroes its whole output)
         int outer = 0;
         while (outer < o.max_outer) {
             bool active = false;
@@ -2324,7 +2329,34 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
             std::vector<double> want(R, 0.5);
             for (int r = 0; r < R; ++r)
                 if (!done[r]) want[r] = std::min(0.5, std::max(inner_tol, 0.5 * o.tolerance / rel[r]));
-            InnerResult ir = run(rv, d, false, want, done, limits, stall);
+            bool any_solo = false, any_block = false;
+            std::vector<char> skip_block(R), skip_solo(R);
+            for (int r = 0; r < R; ++r) {
+                skip_block[r] = done[r] || solo[r];
+                skip_solo[r] = done[r] || !solo[r];
+                any_block = any_block || !skip_block[r];
+                any_solo = any_solo || !skip_solo[r];
+            }
+            InnerResult ir;
+            if (!any_solo) {
+                ir = run(rv, d, false, want, done, limits, stall);
+            } else {
+                if (!d_solo) d_solo = vec(ctx, kSlotOuterSolo, R * n);
+                const InnerResult lone = cocg(ws, A, P, rv, d_solo, false, want, skip_solo, limits, stall);
+                if (any_block) {
+                    ir = block_cocg(ws, A, P, rv, d, false, want, skip_block, limits, stall);
+                } else {
+                    ws.zero(d);
+                    ir = lone;
+                }
+                for (int r = 0; r < R; ++r)
+                    if (solo[r]) {
+                        ir.converged[r] = lone.converged[r];
+                        ir.iterations[r] = lone.iterations[r];
+                        ir.norms[r] = lone.norms[r];
+                    }
+                ws.add(d_solo, x, nullptr);   // zero for the systems in the block
+            }
             for (int r = 0; r < R; ++r) {
                 if (done[r]) continue;
                 total[r] += ir.iterations[r];
@@ -2332,9 +2364,12 @@ std::vector<SolveResult> solve_batch(Context *ctx, Product &op, Product *pc, con
                 for (size_t k = 1; k < ir.norms[r].size(); ++k) out[r].norms.push_back(ir.norms[r][k] * rel[r]);
             }
             ws.add(d, x, nullptr);   // d is zero for the systems the inner solve left out
+            const std::vector<double> before = rel;
             rel = true_residual(x);
-            for (int r = 0; r < R; ++r)
+            for (int r = 0; r < R; ++r) {
                 if (!done[r] && (rel[r] < o.tolerance || !std::isfinite(rel[r]))) done[r] = 1;
+                if (block && !done[r] && !solo[r] && !(rel[r] < 0.5 * before[r])) solo[r] = 1;
+            }
         }
         for (int r = 0; r < R; ++r) {
             out[r].iterations = total[r];

@@ -151,3 +151,35 @@ def test_block_limits(rt):
     ref = np.linalg.solve(m, B)
     assert all(r.converged() for r in res)
     assert max(rel(r.solution(), ref[:, j]) for j, r in enumerate(res)) < 1e-8
+
+
+def test_block_cycles_hand_stalled_systems_to_cocg(rt):
+    """The 256 x 256 plate under 64 plane waves (seed 1), where the block recurrence of 64 right-hand sides loses its
+    conjugacy (it diverges after about 60 iterations): the refined block COCG goes on with COCG, from their iterates,
+    for the systems a block cycle does not halve, and every system reaches 1e-6 by the library's FP64 residual, in
+    FP64 and mixed; two columns also by the oracle's product."""
+    import bench
+    n = 256
+    p = rt.Problem(rt.Mesh(n, n, 0.1, 0.1), 2 * np.pi, quadratureOrder=(16, 8))
+    grid = A.Grid(n, n, 0.1, 0.1)
+    op = rt.Operator(p, precision='mixed')
+    pc = rt.Preconditioner(p, 'nearField', precision='mixed')
+    dirs, pols = bench.plane_wave_angles(64, 1)
+    B = rt.plane_waves(p, dirs, pols)
+    gen = p.generators()
+    spectra = S.subband_spectra(gen, grid)
+    for solver in (rt.Solver('blockcg', tolerance=1e-6, maxIterationCount=2000, refine=True),
+                   rt.Solver('blockcg', tolerance=1e-6, maxIterationCount=2000, precision='c64')):
+        print('plate 256 x 64 blockcg', solver.precision, flush=True)
+        xb, results = solver.solveDevice(op, B, preconditioner=pc)
+        its = [r.iterationCount() for r in results]
+        print('converged %d of 64, iterations median %d max %d, outer max %d' % (
+            sum(r.converged() for r in results), int(np.median(its)), max(its), max(r.outerCount for r in results)),
+            flush=True)
+        assert all(r.converged() and r.residualNorm() < 1e-6 for r in results)
+        xh = xb.to_host()
+        for q in (0, 63):
+            b = X.plane_wave(grid, 2 * np.pi, dirs[q], pols[q], 1.0, order=8)
+            assert rel(S.split_fft_product(spectra, grid, xh[:, q]), b) < 1.01e-6
+        xb.free()
+    B.free()
