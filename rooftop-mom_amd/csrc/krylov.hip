@@ -9,10 +9,11 @@
 // vectors in one launch per pass, so the spectra and the near-field coefficients are read once per step for all R.
 //
 // Everything a step needs stays on the device: the scalars (rho, d^T A d, the Hessenberg column), the Givens rotations,
-// the residual histories and the stopping tests, written by one-block-per-system "finalize" kernels. The host enqueues
-// a window of steps and polls the R loop states once per window. A system that stops (convergence, breakdown, the
-// iteration limit) raises its flag, and every later kernel of that system -- products, preconditioner, vector kernels
-// -- returns at once, so each system's result is exactly that of stopping at its own step. Reductions are
+// the residual histories and the stopping tests, written by one block per system (a "finalize" kernel, or in GMRES
+// the update pass's first block). The host enqueues a window of steps and polls the R loop states once per window. A
+// system that stops (convergence, breakdown, the iteration limit) raises its flag, and every later kernel of that
+// system -- products, preconditioner, vector kernels -- returns at once, so each system's result is exactly that of
+// stopping at its own step. Reductions are
 // deterministic: fixed row blocks, wave64 shuffles and a fixed-order second stage.
 //
 // The Arnoldi step orthogonalizes twice, as CGS2 does (orthogonalize_.m:15-23), but with the second pass delayed by
@@ -900,7 +901,7 @@ __global__ void __launch_bounds__(kThreads) k_dgmres_sums(const c128 *__restrict
     st[r].live = it;
 }
 
-// The finalize of step j for system r by one block (the update pass's last block of the system).
+// The finalize of step j for system r by one block (the update pass's first block of the system).
 __device__ __forceinline__ void finalize_step(int r, c128 *__restrict__ scal, c128 *__restrict__ Hraw, c128 *__restrict__ Hall,
                               c128 *__restrict__ gall, int m, int j, int it, LoopState *st, double *history, int cap) {
     __shared__ c128 s[kLdp], z[kLdp + 1], part[4][kLdp + 1], colp[kLdp], colf[kLdp + 1], coln[kLdp + 1];
